@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Benchmark: closed-box porous convection physical steps on one MI355X.
+
+A step is one physical time step of the 256x128x256 (2x1x2), Ra=500,
+lognormal-permeability workload (thermal DYREL solve, Nu, Darcy DYREL solve,
+velocity, WENO5 advection, CFL time step), inputs resident in HBM.
+
+Prints one JSON line:
+  value         T_eff (GB/s) of the fused DYREL pseudo-iterations over the timed
+                steps: algorithmic bytes (8 words per cell per iteration) / summed
+                kernel time from HIP events on the library stream;
+  ms_per_step   wall time per physical step (all kernels, reductions and host
+                convergence checks);
+  roofline      the iteration kernel against the 8 TB/s HBM peak;
+  cpu_baseline  the CPU oracle (unfused reference kernel sequence, FP64, OpenMP)
+                on a bounded sample of the same workload, as T_eff.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "T_eff GB/s (% HBM peak) + s/physical step, FP32+DC vs FP64 at 1e-6 rel err"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _ensure_built() -> None:
+    from hydrothermal3d_amd import _lib
+    if not _lib.LIBPATH.exists():
+        import __graft_entry__
+        __graft_entry__.build()
+
+
+def timing_stats():
+    import ctypes as C
+    from hydrothermal3d_amd import _lib
+
+    class Stats(C.Structure):
+        _fields_ = [("ms", C.c_double), ("launches", C.c_longlong), ("bytes", C.c_double),
+                    ("ms_fp32", C.c_double), ("launches_fp32", C.c_longlong), ("bytes_fp32", C.c_double)]
+
+    s = Stats()
+    _lib.call("ht3d_timing_get", C.byref(s))
+    return {n: getattr(s, n) for n, _ in Stats._fields_}
+
+
+def run_gpu(w, steps: int, warmup: int, precision: int, prm):
+    import hydrothermal3d_amd as ht
+    from hydrothermal3d_amd import _lib, inputs
+
+    g = ht.HydroThermal3D(w.ncx, w.ncy, w.ncz, w.Lx, w.Ly, w.Lz, w.Ra, w.cfl)
+    g["Tc_ex"].set(inputs.initial_temperature(w))
+    inputs.permeability_device(w, g["kfv"])
+    g.set_params(prm, prm, 1, precision)
+    _lib.sync()
+    for s in range(warmup):
+        r = g.step()
+        log(f"warmup step {s}: iters T {r['thermal']['iters']} P {r['darcy']['iters']} Nu {r['Nu']:.5f}")
+    _lib.call("ht3d_timing_reset")
+    _lib.call("ht3d_timing_enable", 1)
+    _lib.sync()
+    t0 = time.perf_counter()
+    reports = []
+    for s in range(steps):
+        r = g.step()
+        reports.append(r)
+        log(f"step {s}: iters T {r['thermal']['iters']} P {r['darcy']['iters']} Nu {r['Nu']:.6f} "
+            f"dt {r['dt_used']:.3e} ({time.perf_counter() - t0:.2f} s)")
+    _lib.sync()
+    wall = time.perf_counter() - t0
+    _lib.call("ht3d_timing_enable", 0)
+    st = timing_stats()
+    kfv = g["kfv"].get()
+    g.close()
+    return wall, st, reports, kfv
+
+
+def cpu_baseline(w, iters: int, kfv=None):
+    """The oracle's unfused DYREL Darcy solve at full size for a fixed number of
+    pseudo-iterations (tol 0), timed on the host; reported as T_eff."""
+    import numpy as np
+    from oracle import oracle as orc
+    from hydrothermal3d_amd import inputs
+
+    nx, ny, nz = w.ncx, w.ncy, w.ncz
+    if kfv is None:   # the same field the GPU run generated, else the oracle's generator
+        kfv = np.ones((nx + 1, ny + 1, nz + 1), order="F")
+        if w.sigma > 0 or w.fault:
+            orc.lognormal_field(kfv, *inputs.field_args(w))
+    T = inputs.initial_temperature(w)
+    Ty = np.zeros((nx, ny + 1, nz), order="F")
+    kx = np.zeros((nx + 1, ny, nz), order="F")
+    ky = np.zeros((nx, ny + 1, nz), order="F")
+    kz = np.zeros((nx, ny, nz + 1), order="F")
+    Rp = np.zeros((nx, ny, nz), order="F")
+    dx, dy, dz = w.h
+    orc.InitDarcy(Ty, kx, ky, kz, T, kfv, 1.0)
+    orc.KillNorthSouthFluxes(Ty)
+    orc.Compute_Rp(Rp, ky, Ty, w.Ra, 1.0 / dy)
+    P = np.zeros((nx + 2, ny + 2, nz + 2), order="F")
+    p = orc.params(nitmax=iters, nitout=10**9, ncheck=0, norm=0, tol=0.0)
+    t0 = time.perf_counter()
+    orc.dyrel_solve(P, Rp, kx, ky, kz, dx=dx, dy=dy, dz=dz, dt=1.0, transient=0, rhoC=0.0, p=p)
+    dt = time.perf_counter() - t0
+    nbytes = 8.0 * w.ncell * 8 * iters
+    return nbytes / dt / 1e9, dt
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="hrl256")
+    ap.add_argument("--precision", choices=["fp64", "dc"], default="fp64")
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--cpu-iters", type=int, default=100, help="pseudo-iterations of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU halo exchange is a design "
+              "note, not built.", file=sys.stderr)
+        return 2
+
+    _ensure_built()
+    import hydrothermal3d_amd as ht
+    from hydrothermal3d_amd import _lib, inputs
+
+    w = inputs.WORKLOADS[args.workload]
+    prm = ht.production_params(tol=args.tol)
+    precision = _lib.FP64 if args.precision == "fp64" else _lib.FP32_DC
+    log(f"device: {_lib.device_name()}; workload {w.name}; precision {args.precision}")
+    wall, st, reports, kfv = run_gpu(w, args.steps, args.warmup, precision, prm)
+    ms_step = wall / args.steps * 1e3
+    teff = st["bytes"] / (st["ms"] / 1e3) / 1e9 if st["ms"] > 0 else 0.0
+    teff32 = st["bytes_fp32"] / (st["ms_fp32"] / 1e3) / 1e9 if st["ms_fp32"] > 0 else None
+    avg_us = st["ms"] / max(st["launches"], 1) * 1e3
+    iters = [(r["thermal"]["iters"], r["darcy"]["iters"]) for r in reports]
+
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        log(f"CPU baseline: oracle Darcy DYREL, {args.cpu_iters} iterations at full size, {threads} threads")
+        cpu_teff, cpu_s = cpu_baseline(w, args.cpu_iters, kfv)
+        cpu = {"value": round(cpu_teff, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+               "sample": f"{w.name}: CPU oracle (unfused reference kernel sequence, FP64) Darcy DYREL solve, "
+                         f"{args.cpu_iters} pseudo-iterations at full size incl. one Gershgorin set-up pass, "
+                         f"{cpu_s:.2f} s"}
+
+    out = {
+        "metric": METRIC,
+        "value": round(teff, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if precision == _lib.FP64 else "f32+f64dc",
+        "data": "synthetic",
+        "config": {"workload": w.name, "ncx": w.ncx, "ncy": w.ncy, "ncz": w.ncz, "Ra": w.Ra,
+                   "precision": args.precision, "tol": args.tol, "norm": "max|F*D|",
+                   "check_interval": prm.ncheck, "lambda_interval": prm.nitout},
+        "s_per_step": round(ms_step / 1e3, 4),
+        "hbm_frac": round(teff / HBM_PEAK_GBS, 4),
+        "iters_per_step": iters,
+        "iteration_kernel_avg_us": round(avg_us, 2),
+        "roofline": {"bound": "hbm", "achieved": round(teff, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(teff / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    if teff32 is not None:
+        out["t_eff_fp32_inner"] = round(teff32, 2)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,377 @@
+// ht3d_dyrel.hip -- fused DYREL pseudo-transient solve (the hot loop).
+//
+// One kernel launch = one pseudo-iteration of HydroThermal3D_v18.jl:448-478
+// (:506-537) on the dimensionless operators of Dimensionless/kernels_HT3D.jl:
+//
+//   SetTemperatureBCs!/SetPressureBCs!  -> ghost values derived on the fly
+//   ComputeFlux!                         -> q on the six faces of the cell
+//   ResidualDiffusion(Transient)! / PC   -> F = (R [+ X/dt] + div q) / D
+//   DYRELUpdate!                         -> F0 = h1 F0 + h2 F ; X -= dtau F0
+//   Multiply + mean/max, Rayleigh sums   -> per-block partials (check iterations)
+//
+// D (GershgorinPoisson!, kernels_HT3D_dimensional.jl:207-221) is recomputed from
+// the six face coefficients the flux already loaded, so an iteration moves the
+// algorithmic minimum 8 N words: read X, F0, kx, ky, kz, R; write X', F0.
+// X is double-buffered (read Xin, write Xout) because neighbours must see the
+// pre-update values, exactly like the reference's separate flux kernel.
+//
+// Layout: E arrays (ncx+2)(ncy+2)(ncz+2), x fastest.  A block is 64 x TY threads
+// over an (x,y) tile; each thread marches KZ cells in z with a 3-deep register
+// queue for X and a 2-deep one for kz.  Tiles are numbered so that the blocks
+// the hardware sends to one XCD (block id mod 8) own a contiguous slab, keeping
+// halo rows inside one L2.  Arithmetic follows the reference's evaluation order
+// with no FMA contraction, so FP64 iterations equal the CPU ones bit for bit.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ht3d_internal.hpp"
+
+namespace ht3d {
+
+enum : unsigned { FL_CHECK = 1u, FL_LAMBDA = 2u, FL_STORE_F = 4u };
+
+template <typename T>
+struct IterArgs {
+    const T *__restrict__ Xin;
+    T *__restrict__ Xout;
+    const T *__restrict__ R;
+    const T *__restrict__ kx;
+    const T *__restrict__ ky;
+    const T *__restrict__ kz;
+    T *__restrict__ F0;
+    T *__restrict__ Fst;         // F of the previous lambda iteration (read) / this one (write)
+    double *__restrict__ partials;
+    int nx, ny, nz;
+    int nbx, nby, nbz;           // tile grid
+    T _dx, _dy, _dz, _dt;
+    T dx, dy, dz, rc_tr_dt;
+    T h1, h2, dtau;
+    int bct[6];
+    T bcv[6];
+    unsigned flags;
+};
+
+// block id -> logical tile id, making the blocks of one XCD (id mod 8) contiguous
+__device__ __forceinline__ unsigned xcd_tile(unsigned bid, unsigned nblocks) {
+    const unsigned q = nblocks / 8u, r = nblocks % 8u;
+    const unsigned x = bid % 8u, l = bid / 8u;
+    return x * q + (x < r ? x : r) + l;
+}
+
+template <typename T>
+__device__ __forceinline__ T ghost(int type, T val, T inner) {
+    return type ? (T)2 * val - inner : inner;
+}
+
+template <typename T, bool TRANSIENT, int TY, int KZ>
+__global__ __launch_bounds__(64 * TY) void dyrel_iter_kernel(IterArgs<T> a) {
+    const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
+    const unsigned t = xcd_tile(blockIdx.x, nblocks);
+    const int bx = t % a.nbx, by = (t / a.nbx) % a.nby, bz = t / (a.nbx * a.nby);
+    const int i = bx * 64 + (int)(threadIdx.x & 63u);
+    const int j = by * TY + (int)(threadIdx.x >> 6);
+    const int k0 = bz * KZ;
+    const int k1 = min(k0 + KZ, a.nz);
+    const int nx = a.nx, ny = a.ny;
+    const size_t sE1 = (size_t)nx + 2, sE2 = sE1 * ((size_t)ny + 2);
+    const size_t sC2 = (size_t)nx * ny;
+    const size_t sX2 = (size_t)(nx + 1) * ny, sY1 = (size_t)nx, sY2 = (size_t)nx * (ny + 1);
+
+    double acc_sum = 0.0, acc_max = 0.0, acc_num = 0.0, acc_den = 0.0;
+    const bool check = a.flags & FL_CHECK, lam = a.flags & FL_LAMBDA, store = a.flags & FL_STORE_F;
+
+    if (i < nx && j < ny) {
+        const T *Xc = a.Xin + (size_t)(i + 1) + (size_t)(j + 1) * sE1;  // column base at E z-index 0
+        T xc = Xc[(size_t)(k0 + 1) * sE2];
+        T xm = k0 == 0 ? ghost(a.bct[4], a.bcv[4], xc) : Xc[(size_t)k0 * sE2];
+        T kzB = a.kz[(size_t)k0 * sC2 + (size_t)j * nx + i];
+        const bool wlo = i == 0, whi = i == nx - 1, slo = j == 0, shi = j == ny - 1;
+#pragma unroll 2
+        for (int k = k0; k < k1; ++k) {
+            const size_t e = (size_t)(k + 1) * sE2;
+            const T xp = (k + 1 == a.nz) ? ghost(a.bct[5], a.bcv[5], xc) : Xc[e + sE2];
+            const T xW = wlo ? ghost(a.bct[0], a.bcv[0], xc) : Xc[e - 1];
+            const T xE = whi ? ghost(a.bct[1], a.bcv[1], xc) : Xc[e + 1];
+            const T xS = slo ? ghost(a.bct[2], a.bcv[2], xc) : Xc[e - sE1];
+            const T xN = shi ? ghost(a.bct[3], a.bcv[3], xc) : Xc[e + sE1];
+            const size_t c = (size_t)k * sC2 + (size_t)j * nx + i;
+            const size_t fxi = (size_t)k * sX2 + (size_t)j * (nx + 1) + i;
+            const size_t fyi = (size_t)k * sY2 + (size_t)j * sY1 + i;
+            const T kW = a.kx[fxi], kE = a.kx[fxi + 1];
+            const T kS = a.ky[fyi], kN = a.ky[fyi + sY1];
+            const T kF = a.kz[c + sC2];
+            const T kB = kzB;
+            // ComputeFlux! (kernels_HT3D.jl:112-114)
+            const T qW = -kW * (a._dx * (xc - xW)), qE = -kE * (a._dx * (xE - xc));
+            const T qS = -kS * (a._dy * (xc - xS)), qN = -kN * (a._dy * (xN - xc));
+            const T qB = -kB * (a._dz * (xc - xm)), qF = -kF * (a._dz * (xp - xc));
+            // ResidualDiffusion(Transient)! (kernels_HT3D.jl:131, Alban_v6:87)
+            T f;
+            if (TRANSIENT)
+                f = a.R[c] + a._dt * xc + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
+            else
+                f = a.R[c] + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
+            // GershgorinPoisson! diagonal D and the "F /= PC" step
+            const T D = (kB / a.dz + kF / a.dz) / a.dz + (kN / a.dy + kS / a.dy) / a.dy +
+                        (kE / a.dx + kW / a.dx) / a.dx + a.rc_tr_dt;
+            f = f / D;
+            // DYRELUpdate! (kernels_HT3D_dimensional.jl:323-324)
+            const T f0 = a.h1 * a.F0[c] + a.h2 * f;
+            a.F0[c] = f0;
+            a.Xout[(size_t)(i + 1) + (size_t)(j + 1) * sE1 + e] = xc - a.dtau * f0;
+            if (check) {
+                const double fd = fabs((double)(f * D));
+                acc_sum += fd;
+                acc_max = fd > acc_max ? fd : acc_max;
+                if (fd != fd) acc_max = fd;
+                if (lam) {
+                    const double d = (double)(a.dtau * f0);
+                    acc_num += (-d) * ((double)f - (double)a.Fst[c]);
+                    acc_den += d * d;
+                }
+            }
+            if (store) a.Fst[c] = f;
+            xm = xc;
+            xc = xp;
+            kzB = kF;
+        }
+    }
+    if (check) block_reduce4(acc_sum, acc_max, acc_num, acc_den, 0x2u, a.partials, blockIdx.x);
+}
+
+// Gershgorin bound (GershgorinPoisson!, kernels_HT3D_dimensional.jl:207-221): max of G
+__global__ __launch_bounds__(256) void gershgorin_max_kernel(const double *__restrict__ kx,
+                                                             const double *__restrict__ ky,
+                                                             const double *__restrict__ kz, int nx, int ny, int nz,
+                                                             double dx, double dy, double dz, double rc_tr_dt,
+                                                             double *__restrict__ partials) {
+    const size_t N = (size_t)nx * ny * nz;
+    double m = -INFINITY;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < N; c += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
+        const double kW = kx[((size_t)k * ny + j) * (nx + 1) + i], kE = kx[((size_t)k * ny + j) * (nx + 1) + i + 1];
+        const double kS = ky[((size_t)k * (ny + 1) + j) * nx + i], kN = ky[((size_t)k * (ny + 1) + j + 1) * nx + i];
+        const double kB = kz[c], kF = kz[c + (size_t)nx * ny];
+        const double diag = (kB / dz + kF / dz) / dz + (kN / dy + kS / dy) / dy + (kE / dx + kW / dx) / dx + rc_tr_dt;
+        const double g = fabs(kE / (dx * dx)) + fabs(kW / (dx * dx)) + fabs(kN / (dy * dy)) + fabs(kS / (dy * dy)) +
+                         fabs(kB / (dz * dz)) + fabs(kF / (dz * dz)) + fabs(diag);
+        const double G = g / diag;
+        m = (G > m || G != G) ? G : m;
+    }
+    block_reduce4(m, 0.0, 0.0, 0.0, 1u, partials, blockIdx.x);
+}
+
+// copy the ghost layer of src into dst (E arrays)
+__global__ __launch_bounds__(256) void copy_ghosts_kernel(double *__restrict__ dst, const double *__restrict__ src,
+                                                          int nx, int ny, int nz) {
+    const int s1 = nx + 2, s2 = ny + 2, s3 = nz + 2;
+    const size_t nf = (size_t)2 * s2 * s3 + (size_t)2 * s1 * s3 + (size_t)2 * s1 * s2;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < nf; t += (size_t)gridDim.x * blockDim.x) {
+        int i, j, k;
+        size_t r = t;
+        if (r < (size_t)2 * s2 * s3) {
+            i = (r & 1) ? s1 - 1 : 0; r >>= 1; j = (int)(r % s2); k = (int)(r / s2);
+        } else if ((r -= (size_t)2 * s2 * s3) < (size_t)2 * s1 * s3) {
+            j = (r & 1) ? s2 - 1 : 0; r >>= 1; i = (int)(r % s1); k = (int)(r / s1);
+        } else {
+            r -= (size_t)2 * s1 * s3;
+            k = (r & 1) ? s3 - 1 : 0; r >>= 1; i = (int)(r % s1); j = (int)(r / s1);
+        }
+        const size_t e = ((size_t)k * s2 + j) * s1 + i;
+        dst[e] = src[e];
+    }
+}
+
+// Nusselt sum: top-face thermal flux qy(:, ncy+1, :) of HydroThermal3D_Alban_v6.jl:345-346,
+// evaluated on the pre-update temperature whose ghosts have been filled.
+__global__ __launch_bounds__(256) void nu_sum_kernel(const double *__restrict__ T, const double *__restrict__ ky,
+                                                     int nx, int ny, int nz, double _dy, double *__restrict__ partials) {
+    double s = 0.0;
+    const size_t n = (size_t)nx * nz;
+    const size_t sE1 = (size_t)nx + 2, sE2 = sE1 * (ny + 2);
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % nx), k = (int)(t / nx);
+        const double kN = ky[((size_t)k * (ny + 1) + ny) * nx + i];
+        const double a = T[(size_t)(k + 1) * sE2 + (size_t)(ny + 1) * sE1 + i + 1];
+        const double b = T[(size_t)(k + 1) * sE2 + (size_t)ny * sE1 + i + 1];
+        s += -kN * (_dy * (a - b));
+    }
+    block_reduce4(s, 0.0, 0.0, 0.0, 0u, partials, blockIdx.x);
+}
+
+// ------------------------------------------------------------------ host driver
+static constexpr int TY = 4;
+static constexpr int KZ = 8;
+
+template <typename T, bool TR>
+static int launch_iter(const IterArgs<T> &a) {
+    const unsigned nb = (unsigned)a.nbx * a.nby * a.nbz;
+    hipLaunchKernelGGL((dyrel_iter_kernel<T, TR, TY, KZ>), dim3(nb), dim3(64 * TY), 0, ctx().stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "dyrel_iter_kernel");
+    return HT3D_OK;
+}
+
+static inline bool lambda_iter(int iter, int nitout) { return iter <= 2 || (nitout > 0 && iter % nitout == 0); }
+
+// face-by-face ghost fill with an explicit BC table (same order as the per-kernel entry)
+int bc_fill_table(double *A, int nx, int ny, int nz, const int type[6], const double val[6]);
+
+int gershgorin_lmax(const double *kx, const double *ky, const double *kz, int nx, int ny, int nz, double dx,
+                    double dy, double dz, double rc_tr_dt, double *lmax) {
+    void *part = nullptr, *res = nullptr;
+    const size_t N = (size_t)nx * ny * nz;
+    unsigned nb = div_up(N, 256);
+    if (nb > 4096) nb = 4096;
+    int rc = workspace(WS_MISC, (size_t)nb * 4 * sizeof(double), &part);
+    if (rc) return rc;
+    rc = workspace(WS_RESULT, 64 * sizeof(double), &res);
+    if (rc) return rc;
+    hipLaunchKernelGGL(gershgorin_max_kernel, dim3(nb), dim3(256), 0, ctx().stream, kx, ky, kz, nx, ny, nz, dx, dy,
+                       dz, rc_tr_dt, (double *)part);
+    HIP_TRY(hipGetLastError());
+    rc = finish_partials((double *)part, nb, 1u, (double *)res);
+    if (rc) return rc;
+    double r4[4];
+    rc = read_result4((double *)res, r4);
+    if (rc) return rc;
+    *lmax = r4[0];
+    return HT3D_OK;
+}
+
+// The DYREL solve.  If nu_out is given (heat solve), the Nusselt number of the
+// last pseudo-iteration is returned through it (sum / nu_div).
+int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep,
+                     double *nu_out, double nu_div) {
+    HT3D_CHECK_INIT();
+    if (!f || !p) return arg_fail("ht3d_dyrel_solve: null argument");
+    const int nx = f->ncx, ny = f->ncy, nz = f->ncz;
+    if (!grid_ok(nx, ny, nz)) return arg_fail("ht3d_dyrel_solve: extents must be >= 3");
+    if (!f->X || !f->R || !f->kx || !f->ky || !f->kz) return arg_fail("ht3d_dyrel_solve: null field pointer");
+    if (p->nitmax < 1 || p->nitout < 1 || !(p->tol >= 0.0)) return arg_fail("ht3d_dyrel_solve: bad parameters");
+    Context &cx = ctx();
+    const size_t N = (size_t)nx * ny * nz, NE = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+
+    void *xb = nullptr, *f0 = nullptr, *fst = nullptr, *part = nullptr, *res = nullptr;
+    int rc;
+    if ((rc = workspace(WS_XB, NE * sizeof(double), &xb))) return rc;
+    if ((rc = workspace(WS_F0, N * sizeof(double), &f0))) return rc;
+    if ((rc = workspace(WS_FCIT, N * sizeof(double), &fst))) return rc;
+
+    IterArgs<double> a;
+    std::memset(&a, 0, sizeof a);
+    a.nx = nx; a.ny = ny; a.nz = nz;
+    a.nbx = (int)div_up(nx, 64); a.nby = (int)div_up(ny, TY); a.nbz = (int)div_up(nz, KZ);
+    const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
+    if ((rc = workspace(WS_PARTIALS, (size_t)nblocks * 4 * sizeof(double), &part))) return rc;
+    if ((rc = workspace(WS_RESULT, 64 * sizeof(double), &res))) return rc;
+
+    // set-up (v18:432-439)
+    const double rc_tr_dt = f->rhoC * (double)f->transient / f->dt;
+    double lmax = 0.0;
+    if ((rc = gershgorin_lmax(f->kx, f->ky, f->kz, nx, ny, nz, f->dx, f->dy, f->dz, rc_tr_dt, &lmax))) return rc;
+    double lmin = lmax / p->lmin_div;
+    const double dtau = 2.0 / sqrt(lmax) * p->CFL;
+    double c = 2.0 * sqrt(lmin) * p->cfact;
+    double h1 = (2 - c * dtau) / (2 + c * dtau), h2 = 2 * dtau / (2 + c * dtau);
+
+    HIP_TRY(hipMemsetAsync(f0, 0, N * sizeof(double), cx.stream));   // ResetA!(Fc, Fc0)
+    HIP_TRY(hipMemsetAsync(fst, 0, N * sizeof(double), cx.stream));
+
+    a.R = f->R; a.kx = f->kx; a.ky = f->ky; a.kz = f->kz;
+    a.F0 = (double *)f0; a.Fst = (double *)fst; a.partials = (double *)part;
+    a._dx = 1.0 / f->dx; a._dy = 1.0 / f->dy; a._dz = 1.0 / f->dz; a._dt = 1.0 / f->dt;
+    a.dx = f->dx; a.dy = f->dy; a.dz = f->dz; a.rc_tr_dt = rc_tr_dt;
+    a.dtau = dtau;
+    for (int q = 0; q < 6; ++q) { a.bct[q] = f->bc_type[q]; a.bcv[q] = f->bc_val[q]; }
+
+    double *buf[2] = {f->X, (double *)xb};
+    int cur = 0;
+    SegmentTimer timer;
+    const double bytes_per_iter = 8.0 * (double)N * sizeof(double);
+    double nF_abs = 0.0, nF_rel = 0.0, nF_ini = 0.0;
+    int status = HT3D_NOT_CONVERGED, iter, n_checks = 0, n_lambda = 0, done_iters = 0;
+    for (iter = 1; iter <= p->nitmax; ++iter) {
+        const bool lam = lambda_iter(iter, p->nitout);
+        const bool chk = lam || (p->ncheck > 0 && iter % p->ncheck == 0);
+        const bool store = iter < p->nitmax && lambda_iter(iter + 1, p->nitout);
+        a.Xin = buf[cur];
+        a.Xout = buf[cur ^ 1];
+        a.h1 = h1; a.h2 = h2;
+        a.flags = (chk ? FL_CHECK : 0u) | (lam ? FL_LAMBDA : 0u) | (store ? FL_STORE_F : 0u);
+        timer.before_launch();
+        rc = f->transient ? launch_iter<double, true>(a) : launch_iter<double, false>(a);
+        if (rc) return rc;
+        ++timer.count;
+        cur ^= 1;
+        done_iters = iter;
+        if (chk || iter == p->nitmax) timer.close();
+        if (chk) {
+            ++n_checks;
+            if ((rc = finish_partials((double *)part, nblocks, 0x2u, (double *)res))) return rc;
+            double r4[4];
+            if ((rc = read_result4((double *)res, r4))) return rc;
+            timer.collect(bytes_per_iter, false);
+            nF_abs = p->norm == 0 ? r4[0] / (double)N / sqrt((double)N) : r4[1];
+            if (r4[0] != r4[0]) nF_abs = r4[0];
+            if (iter == 1) nF_ini = nF_abs;
+            nF_rel = nF_abs / nF_ini;
+            if (std::isnan(nF_abs)) { status = HT3D_NAN; break; }
+            if (nF_rel > 100) { status = HT3D_DIVERGED; break; }
+            if (lam && iter > 1) {
+                lmin = fabs(r4[2]);
+                lmin /= fabs(r4[3]);
+                c = 2.0 * sqrt(lmin) * p->cfact;
+                h1 = (2 - c * dtau) / (2 + c * dtau);
+                h2 = 2 * dtau / (2 + c * dtau);
+                ++n_lambda;
+            }
+            if (nF_abs < p->tol || nF_rel < p->tol) { status = HT3D_OK; break; }
+        }
+    }
+    timer.close();
+    timer.collect(bytes_per_iter, false);
+    // Finalise: ghosts of the result = BC applied to the pre-update state, as the
+    // reference leaves them (its BC kernel runs before the last update).
+    double *res_buf = buf[cur], *prev = buf[cur ^ 1];
+    if ((rc = bc_fill_table(prev, nx, ny, nz, f->bc_type, f->bc_val))) return rc;
+    {
+        const size_t nf = (size_t)2 * ((ny + 2) * (size_t)(nz + 2) + (nx + 2) * (size_t)(nz + 2) + (nx + 2) * (size_t)(ny + 2));
+        unsigned nb = div_up(nf, 256);
+        if (nb > 2048) nb = 2048;
+        hipLaunchKernelGGL(copy_ghosts_kernel, dim3(nb), dim3(256), 0, cx.stream, res_buf, prev, nx, ny, nz);
+        HIP_TRY(hipGetLastError());
+    }
+    if (nu_out) {
+        unsigned nb = div_up((size_t)nx * nz, 256);
+        if (nb > 1024) nb = 1024;
+        void *npart = nullptr;
+        if ((rc = workspace(WS_MISC, (size_t)nb * 4 * sizeof(double), &npart))) return rc;
+        hipLaunchKernelGGL(nu_sum_kernel, dim3(nb), dim3(256), 0, cx.stream, prev, f->ky, nx, ny, nz, 1.0 / f->dy,
+                           (double *)npart);
+        HIP_TRY(hipGetLastError());
+        if ((rc = finish_partials((double *)npart, nb, 0u, (double *)res))) return rc;
+        double r4[4];
+        if ((rc = read_result4((double *)res, r4))) return rc;
+        *nu_out = r4[0] / nu_div;
+    }
+    if (res_buf != f->X) HIP_TRY(hipMemcpyAsync(f->X, res_buf, NE * sizeof(double), hipMemcpyDeviceToDevice, cx.stream));
+    if (rep) {
+        rep->iters = done_iters;
+        rep->status = status;
+        rep->n_checks = n_checks;
+        rep->n_lambda = n_lambda;
+        rep->lmax = lmax; rep->lmin = lmin; rep->dtau = dtau; rep->h1 = h1; rep->h2 = h2;
+        rep->nF_abs = nF_abs; rep->nF_rel = nF_rel; rep->nF_ini = nF_ini;
+    }
+    rc = after_launch("ht3d_dyrel_solve", true);
+    if (rc) return rc;
+    return status;
+}
+
+}  // namespace ht3d
+
+extern "C" int ht3d_dyrel_solve(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep) {
+    return ht3d::dyrel_solve_impl(f, p, rep, nullptr, 1.0);
+}

@@ -1,0 +1,142 @@
+// ht3d_internal.hpp -- shared runtime state and helpers of libht3d.so (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/ht3d.h"
+
+namespace ht3d {
+
+// ------------------------------------------------------------------ context
+struct Context {
+    bool initialised = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool sync = true;               // per-kernel entries synchronise (ParallelStencil semantics)
+    double *pinned = nullptr;       // small pinned host buffer for scalar read-back
+    // live timing of the iteration kernels
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    ht3d_timing_stats stats{};
+    // grow-only device workspaces
+    void *ws[8] = {nullptr};
+    size_t ws_bytes[8] = {0};
+};
+
+Context &ctx();
+void set_error(const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+int arg_fail(const char *what);
+
+// Brackets runs of back-to-back iteration launches with events (when timing is on).
+struct SegmentTimer {
+    bool open = false;
+    long long count = 0;
+    void before_launch();
+    // call after the last launch of a run; `sync_pending` false means the caller
+    // synchronises the stream before calling collect()
+    void close();
+    void collect(double bytes_per_launch, bool fp32);
+};
+
+// returns a device workspace of at least `bytes` for slot `slot` (grow-only)
+int workspace(int slot, size_t bytes, void **out);
+
+enum WsSlot { WS_PARTIALS = 0, WS_RESULT = 1, WS_XB = 2, WS_F0 = 3, WS_FCIT = 4, WS_MISC = 5, WS_F32 = 6, WS_F32B = 7 };
+
+#define HT3D_CHECK_INIT()                                                     \
+    do {                                                                      \
+        if (!::ht3d::ctx().initialised) return ::ht3d::arg_fail("ht3d_init not called"), HT3D_ERR_STATE; \
+    } while (0)
+
+#define HIP_TRY(expr)                                                         \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess) return ::ht3d::hip_fail(e_, #expr);             \
+    } while (0)
+
+// after a launch: check the launch, then synchronise if requested
+int after_launch(const char *name, bool force_sync = false);
+
+// ------------------------------------------------------------------ grids
+struct Grid {
+    int nx, ny, nz;
+    __host__ __device__ size_t c(int i, int j, int k) const { return ((size_t)k * ny + j) * nx + i; }
+    __host__ __device__ size_t e(int i, int j, int k) const {  // E array, (i,j,k) in E coordinates
+        return ((size_t)k * (ny + 2) + j) * (nx + 2) + i;
+    }
+    __host__ __device__ size_t fx(int i, int j, int k) const { return ((size_t)k * ny + j) * (nx + 1) + i; }
+    __host__ __device__ size_t fy(int i, int j, int k) const { return ((size_t)k * (ny + 1) + j) * nx + i; }
+    __host__ __device__ size_t fz(int i, int j, int k) const { return ((size_t)k * ny + j) * nx + i; }
+    __host__ __device__ size_t v(int i, int j, int k) const { return ((size_t)k * (ny + 1) + j) * (nx + 1) + i; }
+    __host__ __device__ size_t x6(int i, int j, int k) const { return ((size_t)k * (ny + 6) + j) * (nx + 6) + i; }
+    size_t ncell() const { return (size_t)nx * ny * nz; }
+    size_t next() const { return (size_t)(nx + 2) * (ny + 2) * (nz + 2); }
+};
+
+inline bool grid_ok(int nx, int ny, int nz) { return nx >= 3 && ny >= 3 && nz >= 3; }
+
+inline unsigned div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// ------------------------------------------------------------------ reductions
+// 4 independent quantities per block: op per slot given at launch
+enum RedOp { RED_SUM = 0, RED_MAX = 1 };
+
+template <typename T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <typename T>
+__device__ inline T wave_max(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        T o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// Block reduction of up to 4 values (slot ops given by `ops` bitmask: bit q set = max).
+// blockDim.x*blockDim.y*blockDim.z must be a multiple of 64 and <= 1024.
+// Writes partials[blockLinear*4 + q] from thread 0.
+__device__ inline void block_reduce4(double a0, double a1, double a2, double a3, unsigned ops, double *partials,
+                                     unsigned block_linear) {
+    __shared__ double red[16][4];
+    const unsigned tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const unsigned nthreads = blockDim.x * blockDim.y * blockDim.z;
+    const unsigned lane = tid & 63u, wid = tid >> 6;
+    double v[4] = {a0, a1, a2, a3};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = (ops >> q & 1u) ? wave_max(v[q]) : wave_sum(v[q]);
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) red[wid][q] = v[q];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const unsigned nw = nthreads >> 6;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double acc = red[0][q];
+            for (unsigned w = 1; w < nw; ++w) {
+                double o = red[w][q];
+                acc = (ops >> q & 1u) ? (o > acc ? o : acc) : acc + o;
+            }
+            partials[(size_t)block_linear * 4 + q] = acc;
+        }
+    }
+}
+
+// Reduce `nblocks` x 4 partials into result[4] (single block launch, fixed order).
+int finish_partials(const double *partials, unsigned nblocks, unsigned ops, double *result_dev);
+// Read back 4 doubles from the device result buffer (synchronises the stream).
+int read_result4(const double *result_dev, double out[4]);
+
+}  // namespace ht3d

@@ -1,0 +1,386 @@
+// ht3d_model.hip -- fused WENO5 advection, the closed-box physical step, and
+// the synthetic permeability generator.
+//
+// WENO5: AdvectWithWeno5 (Dimensionless/kernels_HT3D.jl:156-233, stencils of
+// tools/Weno5_Routines.jl) and AdvectWithWeno5_v2 (tools/Weno5_Routines_v2.jl:1-50)
+// become two passes per direction.  Pass 1 reads T and writes T1 = T - dt L(T);
+// pass 2 reads T1 and writes T = 1/2 (T1 - dt L(T1)) + 1/2 T in place (each
+// thread reads the old T only at its own cell).  The three-layer halo copy of
+// Boundaries_?_Weno5! is replaced by index mirroring inside the gather, so the
+// (ncx+6)^3 scratch array and the ten cell-sized temporaries of v1 are gone.
+//
+// Physical step: HydroThermal3D_Alban_v6.jl:322-466 order (thermal solve, Nu,
+// Darcy solve, velocity, advection, dt) with the DYREL solves of
+// HydroThermal3D_v18.jl:424-538.
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "ht3d_internal.hpp"
+
+namespace ht3d {
+
+int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep,
+                     double *nu_out, double nu_div);
+int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep,
+                  double *nu_out, int *outer, int *inner);
+
+__device__ __forceinline__ double weno5_js_m(double v1, double v2, double v3, double v4, double v5) {
+    double p1 = v1 / 3.0 - 7.0 / 6.0 * v2 + 11.0 / 6.0 * v3;
+    double p2 = -v2 / 6.0 + 5.0 / 6.0 * v3 + v4 / 3.0;
+    double p3 = v3 / 3.0 + 5.0 / 6.0 * v4 - v5 / 6.0;
+    double maxV = fmax(fmax(fmax(fmax(v1 * v1, v2 * v2), v3 * v3), v4 * v4), v5 * v5);
+    double e = 1e-99 + 1e-6 * maxV;
+    double a1 = v1 - 2.0 * v2 + v3, b1 = v1 - 4.0 * v2 + 3.0 * v3;
+    double a2 = v2 - 2.0 * v3 + v4, b2 = v2 - v4;
+    double a3 = v3 - 2.0 * v4 + v5, b3 = 3.0 * v3 - 4.0 * v4 + v5;
+    double w1 = 13.0 / 12.0 * (a1 * a1) + 1.0 / 4.0 * (b1 * b1);
+    double w2 = 13.0 / 12.0 * (a2 * a2) + 1.0 / 4.0 * (b2 * b2);
+    double w3 = 13.0 / 12.0 * (a3 * a3) + 1.0 / 4.0 * (b3 * b3);
+    w1 = 0.1 / ((w1 + e) * (w1 + e));
+    w2 = 0.6 / ((w2 + e) * (w2 + e));
+    w3 = 0.3 / ((w3 + e) * (w3 + e));
+    double w = (w1 + w2 + w3);
+    w1 = w1 / w;
+    w2 = w2 / w;
+    w3 = w3 / w;
+    return w1 * p1 + w2 * p2 + w3 * p3;
+}
+
+struct WenoArgs {
+    const double *__restrict__ Tin;   // E
+    double *Tout;                     // E (may alias Told in pass 2)
+    const double *Told;               // E, pass 2 only
+    const double *__restrict__ V;     // face array normal to the axis
+    int nx, ny, nz;
+    double h, dt;
+    int tlo, thi;
+    double vlo, vhi;
+};
+
+// One WENO5 pass along AXIS (0,1,2).  V2 selects the v2 minus-side stencil quirk
+// and its >= 0 upwind test; STAGE2 fuses TimeAveraging!(T, Told, 2).
+template <int AXIS, bool V2, bool STAGE2>
+__global__ __launch_bounds__(256) void weno_pass_kernel(WenoArgs a) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k = blockIdx.z;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    if (i >= nx || j >= ny || k >= nz) return;
+    const size_t sE1 = (size_t)nx + 2, sE2 = sE1 * (ny + 2);
+    const size_t ec = (size_t)(k + 1) * sE2 + (size_t)(j + 1) * sE1 + (size_t)(i + 1);
+    const int n = AXIS == 0 ? nx : (AXIS == 1 ? ny : nz);
+    const int p0 = AXIS == 0 ? i : (AXIS == 1 ? j : k);
+    const size_t st = AXIS == 0 ? 1 : (AXIS == 1 ? sE1 : sE2);
+    double u[7];
+#pragma unroll
+    for (int o = -3; o <= 3; ++o) {
+        const int p = p0 + o;
+        if (p < 0) {            // low ghost layer -p: mirror of interior -p-1 (Boundaries_?_Weno5!)
+            const double m = a.Tin[ec + (ptrdiff_t)(-p - 1 - p0) * (ptrdiff_t)st];
+            u[o + 3] = a.tlo == 1 ? 2 * a.vlo - m : (a.tlo == 2 ? a.Tin[ec + (ptrdiff_t)(p + n - p0) * (ptrdiff_t)st] : m);
+        } else if (p >= n) {    // high ghost: mirror of 2n-1-p
+            const double m = a.Tin[ec + (ptrdiff_t)(2 * n - 1 - p - p0) * (ptrdiff_t)st];
+            u[o + 3] = a.thi == 1 ? 2 * a.vhi - m : (a.thi == 2 ? a.Tin[ec + (ptrdiff_t)(p - n - p0) * (ptrdiff_t)st] : m);
+        } else {
+            u[o + 3] = a.Tin[ec + (ptrdiff_t)o * (ptrdiff_t)st];
+        }
+    }
+    // face velocities: left face = cell index, right face = +1 along the axis
+    size_t fl;
+    if (AXIS == 0) fl = ((size_t)k * ny + j) * (nx + 1) + i;
+    else if (AXIS == 1) fl = ((size_t)k * (ny + 1) + j) * nx + i;
+    else fl = ((size_t)k * ny + j) * nx + i;
+    const size_t fs = AXIS == 0 ? 1 : (AXIS == 1 ? (size_t)nx : (size_t)nx * ny);
+    const double vl = a.V[fl], vr = a.V[fl + fs];
+    double Vm, Vp;
+    if (V2) {
+        Vm = vl < 0.0 ? vl : 0.0;
+        Vp = vr >= 0.0 ? vr : 0.0;
+    } else {  // ResetA! then V?PlusMinus! (tools/Weno5_Routines.jl:5-42)
+        Vm = vl < 0.0 ? vl : 0.0;
+        Vp = vr > 0.0 ? vr : 0.0;
+    }
+    const double h = a.h;
+    const double dm = weno5_js_m(h * (u[1] - u[0]), h * (u[2] - u[1]), h * (u[3] - u[2]), h * (u[4] - u[3]),
+                                 V2 ? h * (u[5] - u[2]) : h * (u[5] - u[4]));
+    const double dp = weno5_js_m(h * (u[6] - u[5]), h * (u[5] - u[4]), h * (u[4] - u[3]), h * (u[3] - u[2]),
+                                 h * (u[2] - u[1]));
+    double tn = u[3] - a.dt * (Vp * dm + Vm * dp);
+    if (STAGE2) tn = (1.0 / 2.0) * tn + (1.0 - 1.0 / 2.0) * a.Told[ec];
+    a.Tout[ec] = tn;
+}
+
+template <int AXIS, bool V2, bool STAGE2>
+static int launch_weno(const WenoArgs &a) {
+    dim3 blk(256), grd(div_up(a.nx, 64), div_up(a.ny, 4), a.nz);
+    hipLaunchKernelGGL((weno_pass_kernel<AXIS, V2, STAGE2>), grd, blk, 0, ctx().stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "weno_pass_kernel");
+    return HT3D_OK;
+}
+
+template <int AXIS, bool V2>
+static int weno_axis(double *T, double *W, const double *V, int nx, int ny, int nz, double h, double dt, int tlo,
+                     double vlo, int thi, double vhi) {
+    WenoArgs a{T, W, nullptr, V, nx, ny, nz, h, dt, tlo, thi, vlo, vhi};
+    int rc = launch_weno<AXIS, V2, false>(a);
+    if (rc) return rc;
+    WenoArgs b{W, T, T, V, nx, ny, nz, h, dt, tlo, thi, vlo, vhi};
+    return launch_weno<AXIS, V2, true>(b);
+}
+
+// Fused AdvectWithWeno5 / _v2 with the closed-box boundary types
+// (x Neumann, y Dirichlet Tbot/Ttop, z Neumann).  W: E-sized workspace.
+int weno_advect(double *Tc_ex, double *W, const double *Vx, const double *Vy, const double *Vz, double dt, double _dx,
+                double _dy, double _dz, double Ttop, double Tbot, int version, int nx, int ny, int nz) {
+    int rc;
+    if (version == 2) {
+        if ((rc = weno_axis<0, true>(Tc_ex, W, Vx, nx, ny, nz, _dx, dt, 0, 0.0, 0, 0.0))) return rc;
+        if ((rc = weno_axis<1, true>(Tc_ex, W, Vy, nx, ny, nz, _dy, dt, 1, Tbot, 1, Ttop))) return rc;
+        return weno_axis<2, true>(Tc_ex, W, Vz, nx, ny, nz, _dz, dt, 0, 0.0, 0, 0.0);
+    }
+    if ((rc = weno_axis<0, false>(Tc_ex, W, Vx, nx, ny, nz, _dx, dt, 0, 0.0, 0, 0.0))) return rc;
+    if ((rc = weno_axis<1, false>(Tc_ex, W, Vy, nx, ny, nz, _dy, dt, 1, Tbot, 1, Ttop))) return rc;
+    return weno_axis<2, false>(Tc_ex, W, Vz, nx, ny, nz, _dz, dt, 0, 0.0, 0, 0.0);
+}
+
+// ------------------------------------------------------------------ random field
+__global__ __launch_bounds__(256) void lognormal_kernel(double *__restrict__ kfv, int nx, int ny, int nz, double xmin,
+                                                        double ymin, double zmin, double dx, double dy, double dz,
+                                                        const double *__restrict__ K, const double *__restrict__ phi,
+                                                        int M, double sigma, double fault_mult, double fault_half,
+                                                        double n0, double n1, double n2, double x0, double y0,
+                                                        double z0) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k = blockIdx.z;
+    if (i > nx || j > ny || k > nz) return;
+    const double x = xmin + i * dx, y = ymin + j * dy, z = zmin + k * dz;
+    const double amp = sqrt(2.0 / (double)M);
+    double s = 0.0;
+    for (int m = 0; m < M; ++m) s += cos(K[3 * m] * x + K[3 * m + 1] * y + K[3 * m + 2] * z + phi[m]);
+    double v = pow(10.0, sigma * (amp * s));
+    if (fault_mult != 1.0) {
+        const double d = n0 * (x - x0) + n1 * (y - y0) + n2 * (z - z0);
+        if (fabs(d) <= fault_half) v *= fault_mult;
+    }
+    kfv[((size_t)k * (ny + 1) + j) * (nx + 1) + i] = v;
+}
+
+}  // namespace ht3d
+
+using namespace ht3d;
+
+// ------------------------------------------------------------------ model
+struct ht3d_hrl {
+    int nx, ny, nz;
+    double Lx, Ly, Lz, dx, dy, dz;
+    double Ra, Tbot, Ttop, cfl, dt, time, Nu;
+    int weno_version, precision;
+    ht3d_dyrel_params pT, pP;
+    double *Tc_ex, *Pc_ex, *ktv, *kfv, *Ty, *kx, *ky, *kz, *ktx, *kty, *ktz, *Vx, *Vy, *Vz, *Rt, *Rp, *W;
+};
+
+namespace {
+struct SyncGuard {  // the model's internal launches do not synchronise one by one
+    bool saved;
+    SyncGuard() : saved(ctx().sync) { ctx().sync = false; }
+    ~SyncGuard() { ctx().sync = saved; }
+};
+}  // namespace
+
+extern "C" {
+
+int ht3d_weno5_advect(double *Tc_ex, const double *Vx, const double *Vy, const double *Vz, double dt, double _dx,
+                      double _dy, double _dz, double Ttop, double Tbot, int version, int nx, int ny, int nz) {
+    HT3D_CHECK_INIT();
+    if (!grid_ok(nx, ny, nz)) return arg_fail("ht3d_weno5_advect: extents must be >= 3");
+    if (!Tc_ex || !Vx || !Vy || !Vz) return arg_fail("ht3d_weno5_advect: null pointer");
+    void *w = nullptr;
+    const size_t NE = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+    int rc = workspace(WS_XB, NE * sizeof(double), &w);
+    if (rc) return rc;
+    rc = weno_advect(Tc_ex, (double *)w, Vx, Vy, Vz, dt, _dx, _dy, _dz, Ttop, Tbot, version, nx, ny, nz);
+    if (rc) return rc;
+    return after_launch("ht3d_weno5_advect");
+}
+
+int ht3d_lognormal_field(double *kfv, int nx, int ny, int nz, double xmin, double ymin, double zmin, double dx,
+                         double dy, double dz, const double *K, const double *phi, int M, double sigma,
+                         double fault_mult, double fault_half, const double *fault_n, const double *fault_x0) {
+    HT3D_CHECK_INIT();
+    if (!kfv || !K || !phi || M < 1 || nx < 1 || ny < 1 || nz < 1) return arg_fail("ht3d_lognormal_field: bad argument");
+    double *dK = nullptr, *dphi = nullptr;
+    HIP_TRY(hipMalloc(&dK, (size_t)3 * M * sizeof(double)));
+    HIP_TRY(hipMalloc(&dphi, (size_t)M * sizeof(double)));
+    HIP_TRY(hipMemcpyAsync(dK, K, (size_t)3 * M * sizeof(double), hipMemcpyHostToDevice, ctx().stream));
+    HIP_TRY(hipMemcpyAsync(dphi, phi, (size_t)M * sizeof(double), hipMemcpyHostToDevice, ctx().stream));
+    const double n0 = fault_n ? fault_n[0] : 0.0, n1 = fault_n ? fault_n[1] : 0.0, n2 = fault_n ? fault_n[2] : 0.0;
+    const double x0 = fault_x0 ? fault_x0[0] : 0.0, y0 = fault_x0 ? fault_x0[1] : 0.0, z0 = fault_x0 ? fault_x0[2] : 0.0;
+    dim3 grd(div_up(nx + 1, 64), div_up(ny + 1, 4), nz + 1);
+    hipLaunchKernelGGL(lognormal_kernel, grd, dim3(256), 0, ctx().stream, kfv, nx, ny, nz, xmin, ymin, zmin, dx, dy,
+                       dz, dK, dphi, M, sigma, fault_mult, fault_half, n0, n1, n2, x0, y0, z0);
+    int rc = after_launch("ht3d_lognormal_field", true);
+    (void)hipFree(dK);
+    (void)hipFree(dphi);
+    return rc;
+}
+
+int ht3d_hrl_create(int nx, int ny, int nz, double Lx, double Ly, double Lz, double Ra, double cfl, ht3d_hrl **out) {
+    HT3D_CHECK_INIT();
+    if (!out || !grid_ok(nx, ny, nz) || !(Lx > 0) || !(Ly > 0) || !(Lz > 0) || !(cfl > 0))
+        return arg_fail("ht3d_hrl_create: bad argument");
+    ht3d_hrl *m = new (std::nothrow) ht3d_hrl();
+    if (!m) return HT3D_ERR_NOMEM;
+    std::memset(m, 0, sizeof *m);
+    m->nx = nx; m->ny = ny; m->nz = nz;
+    m->Lx = Lx; m->Ly = Ly; m->Lz = Lz;
+    m->dx = Lx / nx; m->dy = Ly / ny; m->dz = Lz / nz;
+    m->Ra = Ra; m->Tbot = 1.0; m->Ttop = 0.0; m->cfl = cfl;
+    const double h = fmin(fmin(m->dx, m->dy), m->dz);
+    m->dt = h * h / 6.1;  // HydroThermal3D_v7.jl:68
+    m->weno_version = 1;
+    m->precision = HT3D_FP64;
+    ht3d_dyrel_params d = {10000, 1000, 0, 0, 1e-8, 0.99, 0.5, 500.0};
+    m->pT = d; m->pP = d;
+    const size_t c = (size_t)nx * ny * nz, E = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+    const size_t V = (size_t)(nx + 1) * (ny + 1) * (nz + 1);
+    const size_t fx = (size_t)(nx + 1) * ny * nz, fy = (size_t)nx * (ny + 1) * nz, fz = (size_t)nx * ny * (nz + 1);
+    struct { double **p; size_t n; } arr[] = {
+        {&m->Tc_ex, E}, {&m->Pc_ex, E}, {&m->ktv, V}, {&m->kfv, V}, {&m->Ty, fy}, {&m->kx, fx}, {&m->ky, fy},
+        {&m->kz, fz}, {&m->ktx, fx}, {&m->kty, fy}, {&m->ktz, fz}, {&m->Vx, fx}, {&m->Vy, fy}, {&m->Vz, fz},
+        {&m->Rt, c}, {&m->Rp, c}, {&m->W, E}};
+    for (auto &a : arr) {
+        hipError_t e = hipMalloc(a.p, a.n * sizeof(double));
+        if (e != hipSuccess) { ht3d_hrl_destroy(m); return hip_fail(e, "hipMalloc(hrl)"); }
+        e = hipMemsetAsync(*a.p, 0, a.n * sizeof(double), ctx().stream);
+        if (e != hipSuccess) { ht3d_hrl_destroy(m); return hip_fail(e, "hipMemset(hrl)"); }
+    }
+    *out = m;
+    return after_launch("ht3d_hrl_create", true);
+}
+
+int ht3d_hrl_destroy(ht3d_hrl *m) {
+    if (!m) return HT3D_OK;
+    if (ctx().initialised) (void)hipStreamSynchronize(ctx().stream);
+    double *ptrs[] = {m->Tc_ex, m->Pc_ex, m->ktv, m->kfv, m->Ty, m->kx, m->ky, m->kz, m->ktx, m->kty,
+                      m->ktz, m->Vx, m->Vy, m->Vz, m->Rt, m->Rp, m->W};
+    for (double *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete m;
+    return HT3D_OK;
+}
+
+int ht3d_hrl_field(ht3d_hrl *m, const char *name, double **dptr, size_t *count) {
+    if (!m || !name || !dptr) return arg_fail("ht3d_hrl_field: bad argument");
+    const int nx = m->nx, ny = m->ny, nz = m->nz;
+    const size_t c = (size_t)nx * ny * nz, E = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+    const size_t V = (size_t)(nx + 1) * (ny + 1) * (nz + 1);
+    const size_t fx = (size_t)(nx + 1) * ny * nz, fy = (size_t)nx * (ny + 1) * nz, fz = (size_t)nx * ny * (nz + 1);
+    struct { const char *n; double *p; size_t c; } tab[] = {
+        {"Tc_ex", m->Tc_ex, E}, {"Pc_ex", m->Pc_ex, E}, {"ktv", m->ktv, V}, {"kfv", m->kfv, V}, {"Ty", m->Ty, fy},
+        {"kx", m->kx, fx}, {"ky", m->ky, fy}, {"kz", m->kz, fz}, {"ktx", m->ktx, fx}, {"kty", m->kty, fy},
+        {"ktz", m->ktz, fz}, {"Vx", m->Vx, fx}, {"Vy", m->Vy, fy}, {"Vz", m->Vz, fz}, {"Rt", m->Rt, c},
+        {"Rp", m->Rp, c}};
+    for (auto &t : tab)
+        if (!std::strcmp(t.n, name)) {
+            *dptr = t.p;
+            if (count) *count = t.c;
+            return HT3D_OK;
+        }
+    return arg_fail("ht3d_hrl_field: unknown field name");
+}
+
+int ht3d_hrl_set_params(ht3d_hrl *m, const ht3d_dyrel_params *pT, const ht3d_dyrel_params *pP, int weno_version,
+                        int precision) {
+    if (!m) return arg_fail("ht3d_hrl_set_params: null model");
+    if (weno_version != 1 && weno_version != 2) return arg_fail("ht3d_hrl_set_params: weno_version must be 1 or 2");
+    if (precision != HT3D_FP64 && precision != HT3D_FP32_DC) return arg_fail("ht3d_hrl_set_params: bad precision");
+    if (pT) m->pT = *pT;
+    if (pP) m->pP = *pP;
+    m->weno_version = weno_version;
+    m->precision = precision;
+    return HT3D_OK;
+}
+
+int ht3d_hrl_set_dt(ht3d_hrl *m, double dt) {
+    if (!m || !(dt > 0)) return arg_fail("ht3d_hrl_set_dt: bad argument");
+    m->dt = dt;
+    return HT3D_OK;
+}
+
+int ht3d_hrl_step(ht3d_hrl *m, ht3d_step_report *rep) {
+    HT3D_CHECK_INIT();
+    if (!m) return arg_fail("ht3d_hrl_step: null model");
+    SyncGuard guard;
+    const int nx = m->nx, ny = m->ny, nz = m->nz;
+    const double _dx = 1.0 / m->dx, _dy = 1.0 / m->dy, _dz = 1.0 / m->dz;
+    const double dt = m->dt, _dt = 1.0 / dt;
+    ht3d_step_report r;
+    std::memset(&r, 0, sizeof r);
+    r.dt_used = dt;
+    int rc;
+
+    // thermal solve
+    if ((rc = ht3d_InitThermal(m->Rt, m->ktx, m->kty, m->ktz, m->Tc_ex, m->ktv, _dt, nx, ny, nz))) return rc;
+    ht3d_dyrel_field f;
+    std::memset(&f, 0, sizeof f);
+    f.ncx = nx; f.ncy = ny; f.ncz = nz; f.dx = m->dx; f.dy = m->dy; f.dz = m->dz; f.dt = dt;
+    f.X = m->Tc_ex; f.R = m->Rt; f.kx = m->ktx; f.ky = m->kty; f.kz = m->ktz;
+    f.transient = 1; f.rhoC = 1.0;
+    const int tt[6] = {0, 0, 1, 1, 0, 0};
+    const double tv[6] = {0.0, 0.0, m->Tbot, m->Ttop, 0.0, 0.0};
+    std::memcpy(f.bc_type, tt, sizeof tt);
+    std::memcpy(f.bc_val, tv, sizeof tv);
+    double nusum = 0.0;
+    int st;
+    if (m->precision == HT3D_FP32_DC)
+        st = dc_solve_impl(&f, &m->pT, &r.thermal, &nusum, &r.dc_outer_thermal, &r.inner_thermal);
+    else
+        st = dyrel_solve_impl(&f, &m->pT, &r.thermal, &nusum, 1.0);
+    if (st != HT3D_OK && st != HT3D_NOT_CONVERGED) { r.status = st; if (rep) *rep = r; return st; }
+    m->Nu = nusum / (m->Lx * m->Lz) / (double)(nx * nz);
+
+    // Darcy solve
+    if ((rc = ht3d_InitDarcy(m->Ty, m->kx, m->ky, m->kz, m->Tc_ex, m->kfv, _dt, nx, ny, nz))) return rc;
+    if ((rc = ht3d_KillNorthSouthFluxes(m->Ty, nx, ny, nz))) return rc;
+    if ((rc = ht3d_Compute_Rp(m->Rp, m->ky, m->Ty, m->Ra, _dy, nx, ny, nz))) return rc;
+    f.X = m->Pc_ex; f.R = m->Rp; f.kx = m->kx; f.ky = m->ky; f.kz = m->kz;
+    f.transient = 0; f.rhoC = 0.0;
+    std::memset(f.bc_type, 0, sizeof f.bc_type);
+    std::memset(f.bc_val, 0, sizeof f.bc_val);
+    int sp;
+    if (m->precision == HT3D_FP32_DC)
+        sp = dc_solve_impl(&f, &m->pP, &r.darcy, nullptr, &r.dc_outer_darcy, &r.inner_darcy);
+    else
+        sp = dyrel_solve_impl(&f, &m->pP, &r.darcy, nullptr, 1.0);
+    if (sp != HT3D_OK && sp != HT3D_NOT_CONVERGED) { r.status = sp; if (rep) *rep = r; return sp; }
+    m->time += dt;
+
+    // velocity and advection
+    if ((rc = ht3d_Init_vel(m->Vx, m->Vy, m->Vz, m->kx, m->ky, m->kz, m->Pc_ex, m->Ty, m->Ra, _dx, _dy, _dz, nx, ny, nz)))
+        return rc;
+    if ((rc = weno_advect(m->Tc_ex, m->W, m->Vx, m->Vy, m->Vz, dt, _dx, _dy, _dz, m->Ttop, m->Tbot, m->weno_version,
+                          nx, ny, nz)))
+        return rc;
+    double vx = 0, vy = 0, vz = 0;
+    if ((rc = ht3d_reduce(&vx, m->Vx, (size_t)(nx + 1) * ny * nz, HT3D_MAX_ABS))) return rc;
+    if ((rc = ht3d_reduce(&vy, m->Vy, (size_t)nx * (ny + 1) * nz, HT3D_MAX_ABS))) return rc;
+    if ((rc = ht3d_reduce(&vz, m->Vz, (size_t)nx * ny * (nz + 1), HT3D_MAX_ABS))) return rc;
+    const double vmax = fmax(fmax(vx, vy), vz);
+    const double h = fmin(fmin(m->dx, m->dy), m->dz);
+    m->dt = m->cfl * h / vmax;
+
+    r.dt_next = m->dt;
+    r.time = m->time;
+    r.Nu = m->Nu;
+    r.vmax = vmax;
+    r.status = (st == HT3D_NOT_CONVERGED || sp == HT3D_NOT_CONVERGED) ? HT3D_NOT_CONVERGED : HT3D_OK;
+    if (rep) *rep = r;
+    rc = after_launch("ht3d_hrl_step", true);
+    if (rc) return rc;
+    return r.status;
+}
+
+}  // extern "C"

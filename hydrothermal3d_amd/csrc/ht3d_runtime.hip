@@ -1,0 +1,347 @@
+// ht3d_runtime.hip -- context, device memory, copies and reductions.
+//
+// Replaces what ParallelStencil/CUDA.jl provide around the kernels of
+// tduretz/HydroThermal3D: device selection (HydroThermal3D_v7.jl:9), @zeros and
+// Data.Array/Array transfers (v7:87-121,153-157) and the host reductions
+// maximum_g/minimum_g/mean_g/sum_g (tools/Macros.jl:20-25).
+#include <cstdio>
+#include <cstring>
+#include <cmath>
+
+#include "ht3d_internal.hpp"
+
+namespace ht3d {
+
+static Context g_ctx;
+static thread_local std::string g_err;
+
+Context &ctx() { return g_ctx; }
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+int hip_fail(hipError_t e, const char *what) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return HT3D_ERR_HIP;
+}
+
+int arg_fail(const char *what) {
+    g_err = what;
+    return HT3D_ERR_ARG;
+}
+
+int workspace(int slot, size_t bytes, void **out) {
+    Context &c = g_ctx;
+    if (slot < 0 || slot >= 8) return arg_fail("workspace slot");
+    if (bytes == 0) bytes = 256;
+    if (c.ws_bytes[slot] < bytes) {
+        if (c.ws[slot]) {
+            hipError_t e = hipStreamSynchronize(c.stream);
+            if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize(workspace)");
+            e = hipFree(c.ws[slot]);
+            if (e != hipSuccess) return hip_fail(e, "hipFree(workspace)");
+            c.ws[slot] = nullptr;
+            c.ws_bytes[slot] = 0;
+        }
+        hipError_t e = hipMalloc(&c.ws[slot], bytes);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(workspace)");
+        c.ws_bytes[slot] = bytes;
+    }
+    *out = c.ws[slot];
+    return HT3D_OK;
+}
+
+int after_launch(const char *name, bool force_sync) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, name);
+    if (g_ctx.sync || force_sync) {
+        e = hipStreamSynchronize(g_ctx.stream);
+        if (e != hipSuccess) return hip_fail(e, name);
+    }
+    return HT3D_OK;
+}
+
+// ------------------------------------------------------------------ timing
+void SegmentTimer::before_launch() {
+    Context &c = g_ctx;
+    if (!c.timing || open) return;
+    (void)hipEventRecord(c.ev0, c.stream);
+    open = true;
+    count = 0;
+}
+
+void SegmentTimer::close() {
+    Context &c = g_ctx;
+    if (!c.timing || !open) return;
+    (void)hipEventRecord(c.ev1, c.stream);
+}
+
+void SegmentTimer::collect(double bytes_per_launch, bool fp32) {
+    Context &c = g_ctx;
+    if (!c.timing || !open) return;
+    float ms = 0.f;
+    if (hipEventSynchronize(c.ev1) == hipSuccess && hipEventElapsedTime(&ms, c.ev0, c.ev1) == hipSuccess) {
+        if (fp32) {
+            c.stats.ms_fp32 += ms;
+            c.stats.launches_fp32 += count;
+            c.stats.bytes_fp32 += bytes_per_launch * (double)count;
+        } else {
+            c.stats.ms += ms;
+            c.stats.launches += count;
+            c.stats.bytes += bytes_per_launch * (double)count;
+        }
+    }
+    open = false;
+    count = 0;
+}
+
+// ------------------------------------------------------------------ reductions
+__global__ void finish_kernel(const double *__restrict__ partials, unsigned nblocks, unsigned ops,
+                              double *__restrict__ result) {
+    // one block of 1024 threads: thread t folds blocks t, t+1024, ... in order,
+    // then a fixed-shape tree; deterministic for a given nblocks.
+    double a[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a[q] = (ops >> q & 1u) ? -INFINITY : 0.0;
+    for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double v = partials[(size_t)b * 4 + q];
+            a[q] = (ops >> q & 1u) ? (v > a[q] || v != v ? v : a[q]) : a[q] + v;
+        }
+    }
+    __shared__ double red[16][4];
+    const unsigned lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a[q] = (ops >> q & 1u) ? wave_max(a[q]) : wave_sum(a[q]);
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) red[wid][q] = a[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < 4; ++q) {
+            double acc = red[0][q];
+            for (unsigned w = 1; w < blockDim.x / 64; ++w) {
+                double o = red[w][q];
+                acc = (ops >> q & 1u) ? (o > acc || o != o ? o : acc) : acc + o;
+            }
+            result[q] = acc;
+        }
+    }
+}
+
+int finish_partials(const double *partials, unsigned nblocks, unsigned ops, double *result_dev) {
+    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(1024), 0, g_ctx.stream, partials, nblocks, ops, result_dev);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "finish_kernel");
+    return HT3D_OK;
+}
+
+int read_result4(const double *result_dev, double out[4]) {
+    HIP_TRY(hipMemcpyAsync(g_ctx.pinned, result_dev, 4 * sizeof(double), hipMemcpyDeviceToHost, g_ctx.stream));
+    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    for (int q = 0; q < 4; ++q) out[q] = g_ctx.pinned[q];
+    return HT3D_OK;
+}
+
+// generic 1D reduction over n doubles into partials (one value in slot 0)
+__global__ __launch_bounds__(256) void reduce_kernel(const double *__restrict__ a, size_t n, int op,
+                                                     double *__restrict__ partials) {
+    const bool is_max = (op == HT3D_MAX || op == HT3D_MAX_ABS);
+    const bool is_min = (op == HT3D_MIN);
+    double acc = is_max ? -INFINITY : (is_min ? INFINITY : 0.0);
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        double v = a[i];
+        if (op == HT3D_SUM_ABS || op == HT3D_MAX_ABS) v = fabs(v);
+        if (is_max) acc = (v > acc || v != v) ? v : acc;
+        else if (is_min) acc = (v < acc || v != v) ? v : acc;
+        else acc += v;
+    }
+    // min via max of negatives keeps a single reduction helper
+    double x = is_min ? -acc : acc;
+    block_reduce4(x, 0.0, 0.0, 0.0, (is_max || is_min) ? 1u : 0u, partials, blockIdx.x);
+}
+
+}  // namespace ht3d
+
+using namespace ht3d;
+
+extern "C" {
+
+int ht3d_init(int device) {
+    Context &c = ctx();
+    if (c.initialised) {
+        if (c.device == device) return HT3D_OK;
+        ht3d_finalize();
+    }
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return arg_fail("ht3d_init: no such device"), HT3D_ERR_ARG;
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    HIP_TRY(hipHostMalloc((void **)&c.pinned, 64 * sizeof(double), hipHostMallocDefault));
+    HIP_TRY(hipEventCreate(&c.ev0));
+    HIP_TRY(hipEventCreate(&c.ev1));
+    c.device = device;
+    c.initialised = true;
+    c.sync = true;
+    return HT3D_OK;
+}
+
+int ht3d_timing_enable(int on) {
+    HT3D_CHECK_INIT();
+    ctx().timing = on != 0;
+    return HT3D_OK;
+}
+
+int ht3d_timing_get(ht3d_timing_stats *out) {
+    HT3D_CHECK_INIT();
+    if (!out) return arg_fail("ht3d_timing_get: null pointer");
+    *out = ctx().stats;
+    return HT3D_OK;
+}
+
+int ht3d_timing_reset(void) {
+    HT3D_CHECK_INIT();
+    ctx().stats = ht3d_timing_stats{};
+    return HT3D_OK;
+}
+
+int ht3d_finalize(void) {
+    Context &c = ctx();
+    if (!c.initialised) return HT3D_OK;
+    (void)hipStreamSynchronize(c.stream);
+    for (int s = 0; s < 8; ++s) {
+        if (c.ws[s]) (void)hipFree(c.ws[s]);
+        c.ws[s] = nullptr;
+        c.ws_bytes[s] = 0;
+    }
+    if (c.ev0) (void)hipEventDestroy(c.ev0);
+    if (c.ev1) (void)hipEventDestroy(c.ev1);
+    c.ev0 = c.ev1 = nullptr;
+    if (c.pinned) (void)hipHostFree(c.pinned);
+    c.pinned = nullptr;
+    (void)hipStreamDestroy(c.stream);
+    c.stream = nullptr;
+    c.initialised = false;
+    return HT3D_OK;
+}
+
+const char *ht3d_strerror(int code) {
+    switch (code) {
+    case HT3D_OK: return "ok";
+    case HT3D_ERR_ARG: return "invalid argument";
+    case HT3D_ERR_HIP: return "HIP runtime error";
+    case HT3D_ERR_NOMEM: return "out of memory";
+    case HT3D_ERR_STATE: return "library not initialised";
+    case HT3D_NOT_CONVERGED: return "pseudo-transient solve reached nitmax";
+    case HT3D_NAN: return "NaN in residual norm";
+    case HT3D_DIVERGED: return "pseudo-transient solve diverged";
+    default: return "unknown status";
+    }
+}
+
+const char *ht3d_last_error(void) { return g_err.c_str(); }
+
+int ht3d_set_sync(int synchronous) {
+    HT3D_CHECK_INIT();
+    ctx().sync = synchronous != 0;
+    return HT3D_OK;
+}
+
+int ht3d_sync(void) {
+    HT3D_CHECK_INIT();
+    HIP_TRY(hipStreamSynchronize(ctx().stream));
+    return HT3D_OK;
+}
+
+int ht3d_device_name(char *buf, size_t len) {
+    HT3D_CHECK_INIT();
+    hipDeviceProp_t p;
+    HIP_TRY(hipGetDeviceProperties(&p, ctx().device));
+    if (buf && len) {
+        std::snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    }
+    return HT3D_OK;
+}
+
+int ht3d_alloc(size_t bytes, void **dptr) {
+    HT3D_CHECK_INIT();
+    if (!dptr) return arg_fail("ht3d_alloc: null out pointer");
+    *dptr = nullptr;
+    hipError_t e = hipMalloc(dptr, bytes ? bytes : 8);
+    if (e == hipErrorOutOfMemory) return hip_fail(e, "hipMalloc"), HT3D_ERR_NOMEM;
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    return HT3D_OK;
+}
+
+int ht3d_free(void *dptr) {
+    HT3D_CHECK_INIT();
+    if (!dptr) return HT3D_OK;
+    HIP_TRY(hipStreamSynchronize(ctx().stream));
+    HIP_TRY(hipFree(dptr));
+    return HT3D_OK;
+}
+
+int ht3d_h2d(void *dst, const void *src, size_t bytes) {
+    HT3D_CHECK_INIT();
+    if (!bytes) return HT3D_OK;
+    if (!dst || !src) return arg_fail("ht3d_h2d: null pointer");
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx().stream));
+    HIP_TRY(hipStreamSynchronize(ctx().stream));
+    return HT3D_OK;
+}
+
+int ht3d_d2h(void *dst, const void *src, size_t bytes) {
+    HT3D_CHECK_INIT();
+    if (!bytes) return HT3D_OK;
+    if (!dst || !src) return arg_fail("ht3d_d2h: null pointer");
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx().stream));
+    HIP_TRY(hipStreamSynchronize(ctx().stream));
+    return HT3D_OK;
+}
+
+int ht3d_d2d(void *dst, const void *src, size_t bytes) {
+    HT3D_CHECK_INIT();
+    if (!bytes) return HT3D_OK;
+    if (!dst || !src) return arg_fail("ht3d_d2d: null pointer");
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx().stream));
+    return after_launch("ht3d_d2d");
+}
+
+int ht3d_memset(void *dptr, int value, size_t bytes) {
+    HT3D_CHECK_INIT();
+    if (!bytes) return HT3D_OK;
+    if (!dptr) return arg_fail("ht3d_memset: null pointer");
+    HIP_TRY(hipMemsetAsync(dptr, value, bytes, ctx().stream));
+    return after_launch("ht3d_memset");
+}
+
+int ht3d_reduce(double *out_host, const double *a, size_t n, int op) {
+    HT3D_CHECK_INIT();
+    if (!out_host || !a || op < HT3D_SUM || op > HT3D_MAX_ABS) return arg_fail("ht3d_reduce: bad argument");
+    if (n == 0) {
+        *out_host = (op == HT3D_MAX || op == HT3D_MAX_ABS) ? -INFINITY : (op == HT3D_MIN ? INFINITY : 0.0);
+        return HT3D_OK;
+    }
+    unsigned nb = div_up(n, 256);
+    if (nb > 2048) nb = 2048;
+    void *part = nullptr, *res = nullptr;
+    int rc = workspace(WS_MISC, (size_t)nb * 4 * sizeof(double), &part);
+    if (rc) return rc;
+    rc = workspace(WS_RESULT, 64 * sizeof(double), &res);
+    if (rc) return rc;
+    hipLaunchKernelGGL(reduce_kernel, dim3(nb), dim3(256), 0, ctx().stream, a, n, op, (double *)part);
+    HIP_TRY(hipGetLastError());
+    const bool mx = (op == HT3D_MAX || op == HT3D_MAX_ABS || op == HT3D_MIN);
+    rc = finish_partials((const double *)part, nb, mx ? 1u : 0u, (double *)res);
+    if (rc) return rc;
+    double r4[4];
+    rc = read_result4((const double *)res, r4);
+    if (rc) return rc;
+    *out_host = (op == HT3D_MIN) ? -r4[0] : r4[0];
+    return HT3D_OK;
+}
+
+}  // extern "C"

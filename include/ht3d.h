@@ -1,0 +1,288 @@
+/*
+ * ht3d.h -- C ABI of libht3d.so, the MI355X (gfx950) implementation of the
+ * HydroThermal3D solver hot path.
+ *
+ * The library replaces ParallelStencil's `@parallel Kernel!(args...)` launches of
+ * tduretz/HydroThermal3D for the pseudo-transient (PT) Darcy / heat solves and
+ * the WENO5 advection.  Three layers of entry points:
+ *
+ *   1. runtime:   context, device memory, copies, reductions;
+ *   2. kernels:   one entry per reference kernel, same name (without "!") and
+ *                 argument order, arrays as device pointers, the cell-grid
+ *                 extents (ncx,ncy,ncz) appended; every staggered extent is
+ *                 derived from them exactly as in the reference:
+ *                   c  = (ncx,ncy,ncz)            E  = (ncx+2,ncy+2,ncz+2)
+ *                   X6 = (ncx+6,ncy+6,ncz+6)      V  = (ncx+1,ncy+1,ncz+1)
+ *                   Fx = (ncx+1,ncy,ncz)  Fy = (ncx,ncy+1,ncz)  Fz = (ncx,ncy,ncz+1)
+ *                 all column-major, x fastest, FP64;
+ *   3. fused:     a whole DYREL solve, a whole WENO5 advection step, a whole
+ *                 closed-box physical step, and the mixed-precision (FP32 inner
+ *                 sweeps + FP64 defect correction) solve.
+ *
+ * Conventions: every function returns 0 (HT3D_OK) or an error code; nothing
+ * throws across the ABI.  Buffers are owned by the caller (ht3d_alloc/ht3d_free);
+ * the library never frees caller memory.  One host thread per context.  Kernels
+ * run on the library stream; the per-kernel entries synchronise before returning
+ * (ParallelStencil launches are synchronous) unless ht3d_set_sync(0) was called.
+ */
+#ifndef HT3D_H
+#define HT3D_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+enum {
+    HT3D_OK = 0,
+    HT3D_ERR_ARG = 1,        /* bad argument (null pointer, extent < 3, ...)   */
+    HT3D_ERR_HIP = 2,        /* HIP runtime error (see ht3d_last_error)         */
+    HT3D_ERR_NOMEM = 3,
+    HT3D_ERR_STATE = 4,      /* library not initialised                         */
+    HT3D_NOT_CONVERGED = 10, /* solve hit nitmax (silent in the reference)      */
+    HT3D_NAN = 11,           /* v18:464 error("Nan ...")                        */
+    HT3D_DIVERGED = 12       /* v18:465 error("Diverged ...")                   */
+};
+
+/* ---------------------------------------------------------------- runtime */
+/* CUDA.device!(GPU_ID), HydroThermal3D_v7.jl:9 */
+int ht3d_init(int device);
+int ht3d_finalize(void);
+const char *ht3d_strerror(int code);
+const char *ht3d_last_error(void);
+int ht3d_set_sync(int synchronous);
+int ht3d_sync(void);
+int ht3d_device_name(char *buf, size_t len);
+
+/* @zeros / Data.Array(host) / Array(device) (HydroThermal3D_v7.jl:87-121,153-157) */
+int ht3d_alloc(size_t bytes, void **dptr);
+int ht3d_free(void *dptr);
+int ht3d_h2d(void *dst, const void *src, size_t bytes);
+int ht3d_d2h(void *dst, const void *src, size_t bytes);
+int ht3d_d2d(void *dst, const void *src, size_t bytes);
+int ht3d_memset(void *dptr, int value, size_t bytes);
+
+/* Live timing of the fused pseudo-iteration kernel: when enabled, the DYREL
+ * drivers bracket each run of back-to-back iteration launches with HIP events
+ * on the library stream and accumulate the kernel time, the launch count and
+ * the algorithmic bytes (8 words per cell per iteration: read X, F0, kx, ky,
+ * kz, R; write X, F0).  No reference counterpart (the reference prints a T_eff
+ * template it never fills, HydroThermal3D_v7.jl:257-258).                   */
+typedef struct {
+    double ms;             /* summed event time of the iteration kernels    */
+    long long launches;    /* iteration-kernel launches timed               */
+    double bytes;          /* algorithmic bytes of those launches           */
+    double ms_fp32;        /* the same for the FP32 inner sweeps            */
+    long long launches_fp32;
+    double bytes_fp32;
+} ht3d_timing_stats;
+int ht3d_timing_enable(int on);
+int ht3d_timing_get(ht3d_timing_stats *out);
+int ht3d_timing_reset(void);
+
+/* maximum_g / minimum_g / sum_g / mean_g of abs.(A) (tools/Macros.jl:20-25);
+ * the result lands in host memory.  Deterministic order.                    */
+enum { HT3D_SUM = 0, HT3D_SUM_ABS = 1, HT3D_MAX = 2, HT3D_MIN = 3, HT3D_MAX_ABS = 4 };
+int ht3d_reduce(double *out_host, const double *a, size_t n, int op);
+
+/* ---------------------------------------------------------------- kernels */
+/* tools/Macros.jl:171-199, tools/Weno5_Routines.jl:271-276 */
+int ht3d_ResetA(double *A, double *B, int n1, int n2, int n3);
+int ht3d_Cpy_inn_to_all(double *A, const double *B, int ncx, int ncy, int ncz);
+int ht3d_Cpy_all_to_inn(double *A, const double *B, int ncx, int ncy, int ncz);
+int ht3d_ArrayEqualArray(double *F1, const double *F2, int n1, int n2, int n3);
+/* kernels_HT3D_dimensional.jl:106-109 */
+int ht3d_Multiply(double *a, const double *b, const double *c, int ncx, int ncy, int ncz);
+
+/* Dimensionless/kernels_HT3D.jl:99-107 */
+int ht3d_InitThermal(double *Rt, double *kx, double *ky, double *kz, const double *Tc_ex, const double *ktv,
+                     double _dt, int ncx, int ncy, int ncz);
+/* kernels_HT3D_dimensional.jl:133-138 */
+int ht3d_InitConductivity(double *kx, double *ky, double *kz, const double *ktv, int ncx, int ncy, int ncz);
+/* Dimensionless/kernels_HT3D.jl:82-90 */
+int ht3d_InitDarcy(double *Ty, double *kx, double *ky, double *kz, const double *Tc_ex, const double *kfv,
+                   double _dt, int ncx, int ncy, int ncz);
+/* Dimensionless/HydroThermal3D_Alban_v6.jl:153-159 */
+int ht3d_KillNorthSouthFluxes(double *Ty, int ncx, int ncy, int ncz);
+/* Dimensionless/kernels_HT3D.jl:92-97 (+ @dmul_ya, tools/Macros.jl:111) */
+int ht3d_Compute_Rp(double *Rp, const double *ky, const double *Ty, double Ra, double _dy, int ncx, int ncy,
+                    int ncz);
+
+/* Ghost fills on E arrays.  Face ghosts as in the reference.  Edge and corner
+ * ghosts race in the reference and are never read by the hot path; here they
+ * are deterministic (x faces over the interior, then y faces, then z faces).
+ * Alban_v6:108-118 (= kernels_HT3D.jl:144-154): Neumann x,z; Dirichlet y.    */
+int ht3d_SetTemperatureBCs(double *Tc_ex, double Tbot, double Ttop, int ncx, int ncy, int ncz);
+/* Alban_v6:120-130: Neumann on all faces (closed box). */
+int ht3d_SetPressureBCs(double *Pc_ex, double Pbot, double Ptop, int ncx, int ncy, int ncz);
+/* kernels_HT3D.jl:144-154: the v7 pressure BC (Dirichlet in y). */
+int ht3d_SetPressureBCs_v7(double *Pc_ex, double Pbot, double Ptop, int ncx, int ncy, int ncz);
+
+/* kernels_HT3D.jl:109-117 (= kernels_HT3D_dimensional.jl:140-146) */
+int ht3d_ComputeFlux(double *qx, double *qy, double *qz, const double *kx, const double *ky, const double *kz,
+                     const double *A, double _dx, double _dy, double _dz, int ncx, int ncy, int ncz);
+/* kernels_HT3D.jl:119-126 */
+int ht3d_UpdateT(double *F, double *T, const double *R, const double *qx, const double *qy, const double *qz,
+                 double _dt, double dtau, double _dx, double _dy, double _dz, int ncx, int ncy, int ncz);
+/* kernels_HT3D.jl:128-134 */
+int ht3d_ResidualDiffusion(double *F, const double *R, const double *qx, const double *qy, const double *qz,
+                           double _dx, double _dy, double _dz, int ncx, int ncy, int ncz);
+/* Alban_v6:84-90 */
+int ht3d_ResidualDiffusionTransient(double *F, const double *T, const double *R, const double *qx,
+                                    const double *qy, const double *qz, double _dt, double _dx, double _dy,
+                                    double _dz, int ncx, int ncy, int ncz);
+/* The two residuals above followed by the "F /= PC" (Jacobi) step that ends every
+ * DYREL residual of kernels_HT3D_dimensional.jl:175,283.                      */
+int ht3d_ResidualDiffusionPC(double *F, const double *R, const double *qx, const double *qy, const double *qz,
+                             const double *PC, double _dx, double _dy, double _dz, int ncx, int ncy, int ncz);
+int ht3d_ResidualDiffusionTransientPC(double *F, const double *T, const double *R, const double *qx,
+                                      const double *qy, const double *qz, const double *PC, double _dt,
+                                      double _dx, double _dy, double _dz, int ncx, int ncy, int ncz);
+/* kernels_HT3D.jl:136-142 (= DampedUpdate!, kernels_HT3D_dimensional.jl:316-320) */
+int ht3d_UpdateP(double *F0, double *P, const double *F, double dampx, double dtau, int ncx, int ncy, int ncz);
+int ht3d_DampedUpdate(double *F0, double *X, const double *F, double dampx, double dtau, int ncx, int ncy, int ncz);
+/* kernels_HT3D_dimensional.jl:322-326 */
+int ht3d_DYRELUpdate(double *F0, double *X, const double *F, double h1, double h2, double dtau, int ncx, int ncy,
+                     int ncz);
+/* kernels_HT3D_dimensional.jl:122-126 (only the residual copy, as in the reference) */
+int ht3d_SwapDYREL(double *Xc, double *Yc, const double *X0c_ex, const double *Y0c, int ncx, int ncy, int ncz);
+/* kernels_HT3D_dimensional.jl:207-221 */
+int ht3d_GershgorinPoisson(double *G, double *D, const double *rhoC, const double *kx, const double *ky,
+                           const double *kz, double transient, double dt, double dx, double dy, double dz, int ncx,
+                           int ncy, int ncz);
+/* kernels_HT3D_dimensional.jl:335-347 */
+int ht3d_GershgorinHydro(double *G, const double *phidrhodP, const double *kx, const double *ky, const double *kz,
+                         double transient, double dt, double dx, double dy, double dz, int ncx, int ncy, int ncz);
+/* kernels_HT3D_dimensional.jl:349-363 */
+int ht3d_RayleighQuotientNumerator(double *G, const double *Fc0, const double *Fcit, const double *Fc, double dtau,
+                                   int ncx, int ncy, int ncz);
+int ht3d_RayleighQuotientDenominator(double *G, const double *Fc0, double dtau, int ncx, int ncy, int ncz);
+/* kernels_HT3D_dimensional.jl:162-181 */
+int ht3d_ResidualTemperatureLinearised(double *F, const double *Tc_ex, const double *Tc_old, const double *rhoC_eff,
+                                       const double *phc_ex, const double *PC, const double *qx, const double *qy,
+                                       const double *qz, double Qt, double transient, double _dt, double _dx,
+                                       double _dy, double _dz, int ncx, int ncy, int ncz);
+/* kernels_HT3D_dimensional.jl:274-289 */
+int ht3d_ResidualFluidPressure(double *F, const double *phc_ex, const double *phirhof, const double *phirhof0,
+                               const double *PC, const double *qx, const double *qy, const double *qz,
+                               double transient, double _dt, double _dx, double _dy, double _dz, int ncx, int ncy,
+                               int ncz);
+/* kernels_HT3D_dimensional.jl:266-272 */
+int ht3d_ComputeDarcyFlux(double *qx, double *qy, double *qz, const double *rhofv, const double *kx,
+                          const double *ky, const double *kz, const double *Pf, double g, double _dx, double _dy,
+                          double _dz, int ncx, int ncy, int ncz);
+/* Dimensionless/kernels_HT3D.jl:73-80 */
+int ht3d_Init_vel(double *Vx, double *Vy, double *Vz, const double *kx, const double *ky, const double *kz,
+                  const double *Pc_ex, const double *Ty, double Ra, double _dx, double _dy, double _dz, int ncx,
+                  int ncy, int ncz);
+
+/* tools/Weno5_Routines.jl:5-42 */
+int ht3d_VxPlusMinus(double *Vxm, double *Vxp, const double *Vx, int ncx, int ncy, int ncz);
+int ht3d_VyPlusMinus(double *Vym, double *Vyp, const double *Vy, int ncx, int ncy, int ncz);
+int ht3d_VzPlusMinus(double *Vzm, double *Vzp, const double *Vz, int ncx, int ncy, int ncz);
+/* tools/Weno5_Routines.jl:44-68 */
+int ht3d_dFdx_Weno5(double *dFdxi, const double *V1, const double *V2, const double *V3, const double *V4,
+                    const double *V5, int ncx, int ncy, int ncz);
+/* tools/Weno5_Routines.jl:70-128; axis 1..3, plus 0 (minus) / 1 (plus) */
+int ht3d_Gradients_Weno5(double *v1, double *v2, double *v3, double *v4, double *v5, const double *Fc_exxx,
+                         int axis, int plus, double _dx, double _dy, double _dz, int ncx, int ncy, int ncz);
+/* tools/Weno5_Routines.jl:130-255 (src_ext=0, Fc is c) and
+ * tools/Weno5_Routines_v2.jl:221-347 (src_ext=1, Fc is E); type 0/1/2 =
+ * Neumann / Dirichlet / periodic                                             */
+int ht3d_Boundaries_Weno5(double *Fc_exxx, const double *Fc, int axis, int src_ext, int type_lo, double val_lo,
+                          int type_hi, double val_hi, int ncx, int ncy, int ncz);
+/* tools/Weno5_Routines.jl:257-269 */
+int ht3d_Advect(double *Fc, const double *Vxp, const double *dTdxm, const double *Vxm, const double *dTdxp,
+                double dt, int ncx, int ncy, int ncz);
+int ht3d_TimeAveraging(double *Fc, const double *Fold, double order, int ncx, int ncy, int ncz);
+/* tools/Weno5_Routines_v2.jl:55-219 */
+int ht3d_Advect_v2(double *Fc, const double *X, const double *V, int axis, double dt, double _dx, double _dy,
+                   double _dz, int ncx, int ncy, int ncz);
+int ht3d_TimeAveraging_v2(double *Fc, const double *Fold, double order, int ncx, int ncy, int ncz);
+
+/* ---------------------------------------------------------------- fused */
+
+/* One DYREL solve (HydroThermal3D_v18.jl:424-478 / :481-538) on the
+ * dimensionless operators of kernels_HT3D.jl: Gershgorin set-up, then up to
+ * nitmax fused pseudo-iterations (ghost fill + flux + residual/D + DYREL update
+ * in one kernel), Rayleigh-quotient lambda_min updates on the reference schedule
+ * and convergence checks.                                                     */
+typedef struct {
+    int nitmax;        /* v18:309 (1e4)                                              */
+    int nitout;        /* lambda_min + reference check schedule: iter<=2 || iter%nitout==0 */
+    int ncheck;        /* >0: also test convergence every ncheck iterations             */
+    int norm;          /* 0: mean|F*D|/sqrt(N) (v18:459-460); 1: max|F*D|               */
+    double tol;        /* converged if norm < tol or norm/norm(iter 1) < tol (v18:476)   */
+    double CFL;        /* v18:435 (0.99)                                                 */
+    double cfact;      /* v18:436 (0.5)                                                  */
+    double lmin_div;   /* v18:434 (500)                                                  */
+} ht3d_dyrel_params;
+
+typedef struct {
+    int iters, status, n_checks, n_lambda;
+    double lmax, lmin, dtau, h1, h2;
+    double nF_abs, nF_rel, nF_ini;
+} ht3d_dyrel_report;
+
+typedef struct {
+    int ncx, ncy, ncz;
+    double dx, dy, dz;
+    double *X;                          /* E, in/out                                  */
+    const double *R;                    /* c: Rt (heat) or Rp (Darcy)                 */
+    const double *kx, *ky, *kz;         /* Fx, Fy, Fz                                 */
+    int transient;                      /* 1: F = R + X/dt + div q; 0: F = R + div q  */
+    double rhoC;                        /* uniform rhoC of the Gershgorin diagonal    */
+    double dt;
+    int bc_type[6];                     /* x-,x+,y-,y+,z-,z+: 0 Neumann, 1 Dirichlet  */
+    double bc_val[6];
+} ht3d_dyrel_field;
+
+int ht3d_dyrel_solve(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep);
+
+/* The same solve with FP32 DYREL sweeps on the correction equation A e = r inside
+ * an FP64 defect-correction loop (r = b - A x and x += e in FP64).  No reference
+ * counterpart: the reference is FP64 only (HydroThermal3D_v7.jl:13).         */
+int ht3d_dyrel_solve_dc(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep,
+                        int *outer_iters, int *inner_iters);
+
+/* One whole WENO5 advection step on Tc_ex (AdvectWithWeno5, kernels_HT3D.jl:156-233,
+ * version 1; AdvectWithWeno5_v2, tools/Weno5_Routines_v2.jl:1-50, version 2) with
+ * the closed-box boundary types, fused into two passes per direction.       */
+int ht3d_weno5_advect(double *Tc_ex, const double *Vx, const double *Vy, const double *Vz, double dt, double _dx,
+                      double _dy, double _dz, double Ttop, double Tbot, int version, int ncx, int ncy, int ncz);
+
+/* Closed-box Horton-Rogers-Lapwood porous convection (HydroThermal3D_Alban_v6.jl)
+ * stepped with DYREL solves and WENO5 advection, all on the device.          */
+typedef struct ht3d_hrl ht3d_hrl;
+typedef struct {
+    ht3d_dyrel_report thermal, darcy;
+    double dt_used, dt_next, time, Nu, vmax;
+    int status;
+    int dc_outer_thermal, dc_outer_darcy;   /* defect-correction outer iterations */
+    int inner_thermal, inner_darcy;         /* FP32 inner sweeps in total          */
+} ht3d_step_report;
+
+enum { HT3D_FP64 = 0, HT3D_FP32_DC = 1 };
+
+int ht3d_hrl_create(int ncx, int ncy, int ncz, double Lx, double Ly, double Lz, double Ra, double cfl,
+                    ht3d_hrl **out);
+int ht3d_hrl_destroy(ht3d_hrl *m);
+/* names: Tc_ex, Pc_ex, ktv, kfv, Ty, kx, ky, kz, Vx, Vy, Vz, Rt, Rp */
+int ht3d_hrl_field(ht3d_hrl *m, const char *name, double **dptr, size_t *count);
+int ht3d_hrl_set_params(ht3d_hrl *m, const ht3d_dyrel_params *pT, const ht3d_dyrel_params *pP, int weno_version,
+                        int precision);
+int ht3d_hrl_set_dt(ht3d_hrl *m, double dt);
+int ht3d_hrl_step(ht3d_hrl *m, ht3d_step_report *rep);
+
+/* Lognormal permeability on vertices (spectral randomisation) plus an optional
+ * planar fault; K (3*M) and phi (M) in host memory.  Set-up only.            */
+int ht3d_lognormal_field(double *kfv, int ncx, int ncy, int ncz, double xmin, double ymin, double zmin, double dx,
+                         double dy, double dz, const double *K, const double *phi, int M, double sigma,
+                         double fault_mult, double fault_half, const double *fault_n, const double *fault_x0);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HT3D_H */

@@ -1,0 +1,154 @@
+"""Parity of the fused DYREL solve and of whole closed-box physical steps
+(HIP path vs the CPU oracle on identical seeded inputs).
+
+Bar: the fused pseudo-iteration is bit-exact (checked on the first iterations,
+before any reduction-order difference can reach h1/h2); whole solves and steps
+match to 1e-10 relative in T, P, V and Nu with identical iteration counts (the
+reference's tolerance for the plain FP64 GPU path is 1e-10 and +-1 iteration).
+"""
+import numpy as np
+import pytest
+
+from helpers import shapes, rand, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL_FIELD = 1e-10
+
+
+def _lognormal_faces(grid, rng, sigma=0.5):
+    sh = shapes(*grid)
+    return [np.asfortranarray(10.0 ** (sigma * rng.standard_normal(sh[k]))) for k in ("Fx", "Fy", "Fz")]
+
+
+def _problem(grid, transient, seed):
+    rng = np.random.default_rng(seed)
+    sh = shapes(*grid)
+    kx, ky, kz = _lognormal_faces(grid, rng)
+    X = rand(sh["E"], rng)
+    R = rand(sh["c"], rng, -50.0, 50.0)
+    if not transient:
+        R -= R.mean()          # compatible right-hand side of the all-Neumann problem
+    return X, R, kx, ky, kz
+
+
+def _bcs(transient):
+    if transient:
+        return (0, 0, 1, 1, 0, 0), (0.0, 0.0, 1.0, 0.0, 0.0, 0.0)
+    return (0,) * 6, (0.0,) * 6
+
+
+def _solve_both(ht, orc, grid, transient, prm, seed=3, dt=2e-3):
+    X, R, kx, ky, kz = _problem(grid, transient, seed)
+    h = (1.0 / grid[0], 0.5 / grid[1], 2.0 / grid[2])
+    bct, bcv = _bcs(transient)
+    D = {k: ht.DeviceArray.from_host(v) for k, v in dict(X=X, R=R, kx=kx, ky=ky, kz=kz).items()}
+    rep_g = ht.dyrel_solve(D["X"], D["R"], D["kx"], D["ky"], D["kz"], dx=h[0], dy=h[1], dz=h[2], dt=dt,
+                           transient=transient, rhoC=1.0 if transient else 0.0, bc_type=bct, bc_val=bcv, params=prm)
+    Xh = X.copy(order="F")
+    p = orc.params(prm.nitmax, prm.nitout, prm.ncheck, prm.norm, prm.tol, prm.CFL, prm.cfact, prm.lmin_div)
+    rep_c, _ = orc.dyrel_solve(Xh, R, kx, ky, kz, dx=h[0], dy=h[1], dz=h[2], dt=dt, transient=transient,
+                               rhoC=1.0 if transient else 0.0, bc_type=bct, bc_val=bcv, p=p)
+    return rep_g, rep_c, D["X"].get(), Xh
+
+
+@pytest.mark.parametrize("transient", [0, 1], ids=["darcy", "heat"])
+@pytest.mark.parametrize("grid", [(3, 3, 3), (13, 9, 11), (70, 5, 19)], ids=lambda g: "x".join(map(str, g)))
+@pytest.mark.parametrize("nit", [1, 2])
+def test_fused_iterations_bitwise(ht, orc, grid, transient, nit):
+    """The first two pseudo-iterations (both checks, lambda update after the 2nd)
+    reproduce the unfused kernel sequence bit for bit, ghosts included."""
+    prm = ht.DyrelParams(nitmax=nit, nitout=1000, tol=0.0)
+    rg, rc, Xg, Xc = _solve_both(ht, orc, grid, transient, prm)
+    assert rg["iters"] == rc["iters"] == nit
+    assert rg["lmax"] == rc["lmax"]
+    assert np.array_equal(Xg, Xc), f"max diff {np.max(np.abs(Xg - Xc))}"
+
+
+@pytest.mark.parametrize("transient", [0, 1], ids=["darcy", "heat"])
+@pytest.mark.parametrize("schedule", ["reference", "production"])
+@pytest.mark.parametrize("grid", [(12, 10, 14), (33, 17, 9)], ids=lambda g: "x".join(map(str, g)))
+def test_dyrel_solve_parity(ht, orc, grid, transient, schedule):
+    if schedule == "reference":
+        prm = ht.DyrelParams(nitmax=6000, nitout=1000, tol=1e-9)
+    else:
+        prm = ht.DyrelParams(nitmax=20000, nitout=100, ncheck=25, norm=1, tol=1e-9)
+    rg, rc, Xg, Xc = _solve_both(ht, orc, grid, transient, prm)
+    assert rg["status"] == rc["status"]
+    assert abs(rg["iters"] - rc["iters"]) <= 1
+    assert rg["n_lambda"] == rc["n_lambda"]
+    assert rel_err(Xg, Xc) <= TOL_FIELD
+    assert abs(rg["lmin"] - rc["lmin"]) <= 1e-8 * abs(rc["lmin"])
+
+
+def test_status_nitmax_and_nan(ht, orc):
+    """nitmax is reported (the reference is silent), NaN in the data raises like
+    the reference's error("Nan ...") (HydroThermal3D_v18.jl:464)."""
+    grid = (8, 8, 8)
+    prm = ht.DyrelParams(nitmax=3, nitout=1000, tol=0.0)
+    rg, rc, _, _ = _solve_both(ht, orc, grid, 0, prm)
+    assert rg["status"] == rc["status"] == 10
+    X, R, kx, ky, kz = _problem(grid, 0, 5)
+    R[3, 3, 3] = np.nan
+    D = [ht.DeviceArray.from_host(a) for a in (X, R, kx, ky, kz)]
+    with pytest.raises(ht.SolveError, match="Nan"):
+        ht.dyrel_solve(*D, dx=0.1, dy=0.1, dz=0.1, dt=1.0, transient=0, rhoC=0.0, params=ht.DyrelParams(nitmax=5))
+
+
+def _gauge(P):
+    inn = P[1:-1, 1:-1, 1:-1]
+    return inn - inn.mean()
+
+
+def _run_steps(ht, orc, w, nsteps, pT, pP, weno=1):
+    from hydrothermal3d_amd import inputs
+    T0 = inputs.initial_temperature(w)
+    g = ht.HydroThermal3D(w.ncx, w.ncy, w.ncz, w.Lx, w.Ly, w.Lz, w.Ra, w.cfl)
+    g["Tc_ex"].set(T0)
+    inputs.permeability_device(w, g["kfv"])
+    kfv = g["kfv"].get()
+    g.set_params(pT, pP, weno)
+    o = orc.HRLModel(w.ncx, w.ncy, w.ncz, w.Lx, w.Ly, w.Lz, w.Ra, w.cfl)
+    o.field("Tc_ex")[...] = T0
+    o.field("kfv")[...] = kfv
+    o.set_params(orc.params(pT.nitmax, pT.nitout, pT.ncheck, pT.norm, pT.tol),
+                 orc.params(pP.nitmax, pP.nitout, pP.ncheck, pP.norm, pP.tol), weno)
+    out = []
+    for s in range(nsteps):
+        rg = g.step()
+        rc = o.step()
+        out.append((rg, rc))
+        assert rg["thermal"]["iters"] == rc["thermal"]["iters"], f"step {s}"
+        assert abs(rg["darcy"]["iters"] - rc["darcy"]["iters"]) <= 1, f"step {s}"
+        assert abs(rg["Nu"] - rc["Nu"]) <= TOL_FIELD * abs(rc["Nu"]), f"step {s}"
+        assert abs(rg["dt_next"] - rc["dt_next"]) <= TOL_FIELD * rc["dt_next"], f"step {s}"
+    errs = {
+        "T": rel_err(g["Tc_ex"].get()[1:-1, 1:-1, 1:-1], o.field("Tc_ex")[1:-1, 1:-1, 1:-1]),
+        "P": rel_err(_gauge(g["Pc_ex"].get()), _gauge(o.field("Pc_ex"))),
+        "Vx": rel_err(g["Vx"].get(), o.field("Vx")),
+        "Vy": rel_err(g["Vy"].get(), o.field("Vy")),
+        "Vz": rel_err(g["Vz"].get(), o.field("Vz")),
+    }
+    g.close()
+    o.close()
+    return out, errs
+
+
+@pytest.mark.parametrize("weno", [1, 2])
+def test_hrl_steps_parity_small(ht, orc, weno):
+    from hydrothermal3d_amd import inputs
+    w = inputs.WORKLOADS["hrl256"].scaled(20, 12, 16, steps=4)
+    pT = ht.DyrelParams(nitmax=20000, nitout=100, ncheck=25, norm=1, tol=1e-9)
+    out, errs = _run_steps(ht, orc, w, 4, pT, pT, weno)
+    for k, e in errs.items():
+        assert e <= TOL_FIELD, f"{k}: {e}"
+
+
+def test_hrl32_reference_schedule(ht, orc):
+    """The 32^3, Ra=200 workload with the reference's DYREL schedule and norm."""
+    from hydrothermal3d_amd import inputs
+    w = inputs.WORKLOADS["hrl32"]
+    p = ht.reference_params(1e-8)
+    out, errs = _run_steps(ht, orc, w, 5, p, p, 1)
+    for k, e in errs.items():
+        assert e <= TOL_FIELD, f"{k}: {e}"
