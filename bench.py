@@ -125,7 +125,7 @@ def main(argv=None) -> int:
     ap.add_argument("--workload", default="hrl256")
     ap.add_argument("--precision", choices=["fp64", "dc"], default="fp64")
     ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--cpu-iters", type=int, default=100, help="pseudo-iterations of the CPU-baseline sample")
+    ap.add_argument("--cpu-iters", type=int, default=600, help="pseudo-iterations of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args(argv)
     if args.gpus != 1:

@@ -168,6 +168,10 @@ __global__ __launch_bounds__(256) void lognormal_kernel(double *__restrict__ kfv
     kfv[((size_t)k * (ny + 1) + j) * (nx + 1) + i] = v;
 }
 
+__global__ __launch_bounds__(256) void fill_kernel(double *__restrict__ a, size_t n, double v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = v;
+}
+
 }  // namespace ht3d
 
 using namespace ht3d;
@@ -256,6 +260,12 @@ int ht3d_hrl_create(int nx, int ny, int nz, double Lx, double Ly, double Lz, dou
         if (e != hipSuccess) { ht3d_hrl_destroy(m); return hip_fail(e, "hipMalloc(hrl)"); }
         e = hipMemsetAsync(*a.p, 0, a.n * sizeof(double), ctx().stream);
         if (e != hipSuccess) { ht3d_hrl_destroy(m); return hip_fail(e, "hipMemset(hrl)"); }
+    }
+    // ktv and kfv start as @ones (HydroThermal3D_Alban_v6.jl:246-247)
+    for (double *p : {m->ktv, m->kfv}) {
+        unsigned nb = div_up(V, 256);
+        if (nb > 4096) nb = 4096;
+        hipLaunchKernelGGL(fill_kernel, dim3(nb), dim3(256), 0, ctx().stream, p, V, 1.0);
     }
     *out = m;
     return after_launch("ht3d_hrl_create", true);

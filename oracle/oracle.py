@@ -183,6 +183,43 @@ def Advect_v2(Fc, X, V, axis, dt, _dx, _dy, _dz):
                       C.c_double(_dz), *grid)
 
 
+def AdvectWithWeno5_v2(Tc_ex, Tc_exxx, Told, Vx, Vy, Vz, dt, _dx, _dy, _dz, Ttop, Tbot):
+    """tools/Weno5_Routines_v2.jl:1-50 (C orchestration of the v2 kernels)."""
+    grid = _grid(Tc_ex, "E")
+    lib().o_AdvectWithWeno5_v2(_ptr(Tc_ex), _ptr(Tc_exxx), _ptr(Told), _ptr(Vx), _ptr(Vy), _ptr(Vz), C.c_double(dt),
+                               C.c_double(_dx), C.c_double(_dy), C.c_double(_dz), C.c_double(Ttop),
+                               C.c_double(Tbot), *grid)
+
+
+def AdvectWithWeno5(Tc_ex, Vx, Vy, Vz, dt, _dx, _dy, _dz, Ttop, Tbot):
+    """Dimensionless/kernels_HT3D.jl:156-233 after Init_vel!, orchestrated here from
+    the oracle's v1 kernels in the reference's launch order."""
+    nx, ny, nz = _grid(Tc_ex, "E")
+    c = (nx, ny, nz)
+    z = lambda: np.zeros(c, order="F")
+    Tc, Told, v = z(), z(), [z() for _ in range(5)]
+    dTdxm, dTdxp = z(), z()
+    Vm = {a: z() for a in "xyz"}
+    Vp = {a: z() for a in "xyz"}
+    X6 = np.zeros((nx + 6, ny + 6, nz + 6), order="F")
+    ResetA(Vm["x"], Vp["x"]); VxPlusMinus(Vm["x"], Vp["x"], Vx)
+    ResetA(Vm["y"], Vp["y"]); VyPlusMinus(Vm["y"], Vp["y"], Vy)
+    ResetA(Vm["z"], Vp["z"]); VzPlusMinus(Vm["z"], Vp["z"], Vz)
+    Cpy_inn_to_all(Tc, Tc_ex)
+    bcs = {1: (0, 0.0, 0, 0.0), 2: (1, Tbot, 1, Ttop), 3: (0, 0.0, 0, 0.0)}
+    for axis, a in ((1, "x"), (2, "y"), (3, "z")):
+        ArrayEqualArray(Told, Tc)
+        for _ in range(2):
+            Boundaries_Weno5(X6, Tc, axis, *bcs[axis])
+            Gradients_Weno5(*v, X6, axis, 0, _dx, _dy, _dz)
+            dFdx_Weno5(dTdxm, *v)
+            Gradients_Weno5(*v, X6, axis, 1, _dx, _dy, _dz)
+            dFdx_Weno5(dTdxp, *v)
+            Advect(Tc, Vp[a], dTdxm, Vm[a], dTdxp, dt)
+        TimeAveraging(Tc, Told, 2.0)
+    Cpy_all_to_inn(Tc_ex, Tc)
+
+
 def weno5_js(v1, v2, v3, v4, v5) -> float:
     return lib().o_weno5_js(v1, v2, v3, v4, v5)
 
