@@ -78,7 +78,7 @@ __global__ __launch_bounds__(64 * TY) void dyrel_iter_kernel(IterArgs<T> a) {
     const size_t sC2 = (size_t)nx * ny;
     const size_t sX2 = (size_t)(nx + 1) * ny, sY1 = (size_t)nx, sY2 = (size_t)nx * (ny + 1);
 
-    double acc_sum = 0.0, acc_max = 0.0, acc_num = 0.0, acc_den = 0.0;
+    double s_hi = 0.0, s_lo = 0.0, acc_max = 0.0, n_hi = 0.0, n_lo = 0.0, d_hi = 0.0, d_lo = 0.0;
     const bool check = a.flags & FL_CHECK, lam = a.flags & FL_LAMBDA, store = a.flags & FL_STORE_F;
 
     if (i < nx && j < ny) {
@@ -122,13 +122,12 @@ __global__ __launch_bounds__(64 * TY) void dyrel_iter_kernel(IterArgs<T> a) {
             a.Xout[(size_t)(i + 1) + (size_t)(j + 1) * sE1 + e] = xc - a.dtau * f0;
             if (check) {
                 const double fd = fabs((double)(f * D));
-                acc_sum += fd;
-                acc_max = fd > acc_max ? fd : acc_max;
-                if (fd != fd) acc_max = fd;
-                if (lam) {
+                dd_add(s_hi, s_lo, fd);
+                acc_max = (fd > acc_max || fd != fd) ? fd : acc_max;
+                if (lam) {  // RayleighQuotientNumerator! / Denominator! terms
                     const double d = (double)(a.dtau * f0);
-                    acc_num += (-d) * ((double)f - (double)a.Fst[c]);
-                    acc_den += d * d;
+                    dd_add(n_hi, n_lo, (-d) * ((double)f - (double)a.Fst[c]));
+                    dd_add(d_hi, d_lo, d * d);
                 }
             }
             if (store) a.Fst[c] = f;
@@ -137,7 +136,10 @@ __global__ __launch_bounds__(64 * TY) void dyrel_iter_kernel(IterArgs<T> a) {
             kzB = kF;
         }
     }
-    if (check) block_reduce4(acc_sum, acc_max, acc_num, acc_den, 0x2u, a.partials, blockIdx.x);
+    if (check) {
+        const double h4[4] = {s_hi, acc_max, n_hi, d_hi}, l4[4] = {s_lo, 0.0, n_lo, d_lo};
+        block_reduce4(h4, l4, 0x2u, a.partials, blockIdx.x);
+    }
 }
 
 // Gershgorin bound (GershgorinPoisson!, kernels_HT3D_dimensional.jl:207-221): max of G
@@ -159,7 +161,8 @@ __global__ __launch_bounds__(256) void gershgorin_max_kernel(const double *__res
         const double G = g / diag;
         m = (G > m || G != G) ? G : m;
     }
-    block_reduce4(m, 0.0, 0.0, 0.0, 1u, partials, blockIdx.x);
+    const double h4[4] = {m, 0.0, 0.0, 0.0}, l4[4] = {0.0, 0.0, 0.0, 0.0};
+    block_reduce4(h4, l4, 1u, partials, blockIdx.x);
 }
 
 // copy the ghost layer of src into dst (E arrays)
@@ -187,7 +190,7 @@ __global__ __launch_bounds__(256) void copy_ghosts_kernel(double *__restrict__ d
 // evaluated on the pre-update temperature whose ghosts have been filled.
 __global__ __launch_bounds__(256) void nu_sum_kernel(const double *__restrict__ T, const double *__restrict__ ky,
                                                      int nx, int ny, int nz, double _dy, double *__restrict__ partials) {
-    double s = 0.0;
+    double s = 0.0, sl = 0.0;
     const size_t n = (size_t)nx * nz;
     const size_t sE1 = (size_t)nx + 2, sE2 = sE1 * (ny + 2);
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
@@ -195,9 +198,10 @@ __global__ __launch_bounds__(256) void nu_sum_kernel(const double *__restrict__ 
         const double kN = ky[((size_t)k * (ny + 1) + ny) * nx + i];
         const double a = T[(size_t)(k + 1) * sE2 + (size_t)(ny + 1) * sE1 + i + 1];
         const double b = T[(size_t)(k + 1) * sE2 + (size_t)ny * sE1 + i + 1];
-        s += -kN * (_dy * (a - b));
+        dd_add(s, sl, -kN * (_dy * (a - b)));
     }
-    block_reduce4(s, 0.0, 0.0, 0.0, 0u, partials, blockIdx.x);
+    const double h4[4] = {s, 0.0, 0.0, 0.0}, l4[4] = {sl, 0.0, 0.0, 0.0};
+    block_reduce4(h4, l4, 0u, partials, blockIdx.x);
 }
 
 // ------------------------------------------------------------------ host driver
@@ -224,7 +228,7 @@ int gershgorin_lmax(const double *kx, const double *ky, const double *kz, int nx
     const size_t N = (size_t)nx * ny * nz;
     unsigned nb = div_up(N, 256);
     if (nb > 4096) nb = 4096;
-    int rc = workspace(WS_MISC, (size_t)nb * 4 * sizeof(double), &part);
+    int rc = workspace(WS_MISC, (size_t)nb * 8 * sizeof(double), &part);
     if (rc) return rc;
     rc = workspace(WS_RESULT, 64 * sizeof(double), &res);
     if (rc) return rc;
@@ -264,7 +268,7 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
     a.nx = nx; a.ny = ny; a.nz = nz;
     a.nbx = (int)div_up(nx, 64); a.nby = (int)div_up(ny, TY); a.nbz = (int)div_up(nz, KZ);
     const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
-    if ((rc = workspace(WS_PARTIALS, (size_t)nblocks * 4 * sizeof(double), &part))) return rc;
+    if ((rc = workspace(WS_PARTIALS, (size_t)nblocks * 8 * sizeof(double), &part))) return rc;
     if ((rc = workspace(WS_RESULT, 64 * sizeof(double), &res))) return rc;
 
     // set-up (v18:432-439)
@@ -347,7 +351,7 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
         unsigned nb = div_up((size_t)nx * nz, 256);
         if (nb > 1024) nb = 1024;
         void *npart = nullptr;
-        if ((rc = workspace(WS_MISC, (size_t)nb * 4 * sizeof(double), &npart))) return rc;
+        if ((rc = workspace(WS_MISC, (size_t)nb * 8 * sizeof(double), &npart))) return rc;
         hipLaunchKernelGGL(nu_sum_kernel, dim3(nb), dim3(256), 0, cx.stream, prev, f->ky, nx, ny, nz, 1.0 / f->dy,
                            (double *)npart);
         HIP_TRY(hipGetLastError());

@@ -83,14 +83,27 @@ inline bool grid_ok(int nx, int ny, int nz) { return nx >= 3 && ny >= 3 && nz >=
 inline unsigned div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
 // ------------------------------------------------------------------ reductions
-// 4 independent quantities per block: op per slot given at launch
-enum RedOp { RED_SUM = 0, RED_MAX = 1 };
+// Sums are accumulated in double-double (hi + lo, TwoSum with no FMA
+// contraction), so a sum of N terms is accurate to about N * 2^-106 of the sum
+// of magnitudes and its final rounding to double does not depend on the
+// order of accumulation except in vanishingly rare ties.  The CPU oracle
+// accumulates the same way, which makes GPU and CPU solves take identical
+// convergence decisions and lambda_min updates.  Maxima are exact.
+__host__ __device__ inline void dd_add(double &hi, double &lo, double x) {
+    const double s = hi + x;
+    const double bp = s - hi;
+    const double e = (hi - (s - bp)) + (x - bp);
+    hi = s;
+    lo += e;
+}
 
-template <typename T>
-__device__ inline T wave_sum(T v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+__host__ __device__ inline void dd_combine(double &hi, double &lo, double h2, double l2) {
+    const double s = hi + h2;
+    const double bp = s - hi;
+    double e = (hi - (s - bp)) + (h2 - bp);
+    e += lo + l2;
+    hi = s + e;
+    lo = e - (hi - s);
 }
 
 template <typename T>
@@ -98,43 +111,63 @@ __device__ inline T wave_max(T v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         T o = __shfl_xor(v, off, 64);
-        v = o > v ? o : v;
+        v = (o > v || o != o) ? o : v;
     }
     return v;
 }
 
-// Block reduction of up to 4 values (slot ops given by `ops` bitmask: bit q set = max).
-// blockDim.x*blockDim.y*blockDim.z must be a multiple of 64 and <= 1024.
-// Writes partials[blockLinear*4 + q] from thread 0.
-__device__ inline void block_reduce4(double a0, double a1, double a2, double a3, unsigned ops, double *partials,
+__device__ inline void wave_dd(double &hi, double &lo) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double h2 = __shfl_xor(hi, off, 64), l2 = __shfl_xor(lo, off, 64);
+        dd_combine(hi, lo, h2, l2);
+    }
+}
+
+// Block reduction of 4 slots; bit q of `ops` set = max slot (value in hi),
+// clear = double-double sum slot (hi, lo).  Writes 8 doubles per block:
+// partials[8*block + 2*q + {0,1}].  Thread count: multiple of 64, <= 1024.
+__device__ inline void block_reduce4(const double hi_in[4], const double lo_in[4], unsigned ops, double *partials,
                                      unsigned block_linear) {
-    __shared__ double red[16][4];
+    __shared__ double red[16][8];
     const unsigned tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
     const unsigned nthreads = blockDim.x * blockDim.y * blockDim.z;
     const unsigned lane = tid & 63u, wid = tid >> 6;
-    double v[4] = {a0, a1, a2, a3};
+    double h[4], l[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = (ops >> q & 1u) ? wave_max(v[q]) : wave_sum(v[q]);
+    for (int q = 0; q < 4; ++q) {
+        h[q] = hi_in[q];
+        l[q] = lo_in[q];
+        if (ops >> q & 1u) h[q] = wave_max(h[q]);
+        else wave_dd(h[q], l[q]);
+    }
     if (lane == 0) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) red[wid][q] = v[q];
+        for (int q = 0; q < 4; ++q) {
+            red[wid][2 * q] = h[q];
+            red[wid][2 * q + 1] = l[q];
+        }
     }
     __syncthreads();
     if (tid == 0) {
         const unsigned nw = nthreads >> 6;
-#pragma unroll
         for (int q = 0; q < 4; ++q) {
-            double acc = red[0][q];
+            double a = red[0][2 * q], b = red[0][2 * q + 1];
             for (unsigned w = 1; w < nw; ++w) {
-                double o = red[w][q];
-                acc = (ops >> q & 1u) ? (o > acc ? o : acc) : acc + o;
+                if (ops >> q & 1u) {
+                    const double o = red[w][2 * q];
+                    a = (o > a || o != o) ? o : a;
+                } else {
+                    dd_combine(a, b, red[w][2 * q], red[w][2 * q + 1]);
+                }
             }
-            partials[(size_t)block_linear * 4 + q] = acc;
+            partials[(size_t)block_linear * 8 + 2 * q] = a;
+            partials[(size_t)block_linear * 8 + 2 * q + 1] = (ops >> q & 1u) ? 0.0 : b;
         }
     }
 }
 
-// Reduce `nblocks` x 4 partials into result[4] (single block launch, fixed order).
+// Reduce `nblocks` x 8 partials into result[4] (single block launch).
 int finish_partials(const double *partials, unsigned nblocks, unsigned ops, double *result_dev);
 // Read back 4 doubles from the device result buffer (synchronises the stream).
 int read_result4(const double *result_dev, double out[4]);

@@ -95,38 +95,27 @@ void SegmentTimer::collect(double bytes_per_launch, bool fp32) {
 }
 
 // ------------------------------------------------------------------ reductions
-__global__ void finish_kernel(const double *__restrict__ partials, unsigned nblocks, unsigned ops,
-                              double *__restrict__ result) {
-    // one block of 1024 threads: thread t folds blocks t, t+1024, ... in order,
-    // then a fixed-shape tree; deterministic for a given nblocks.
-    double a[4];
+__global__ __launch_bounds__(1024) void finish_kernel(const double *__restrict__ partials, unsigned nblocks,
+                                                       unsigned ops, double *__restrict__ result) {
+    double h[4], l[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) a[q] = (ops >> q & 1u) ? -INFINITY : 0.0;
+    for (int q = 0; q < 4; ++q) {
+        h[q] = (ops >> q & 1u) ? -INFINITY : 0.0;
+        l[q] = 0.0;
+    }
     for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            double v = partials[(size_t)b * 4 + q];
-            a[q] = (ops >> q & 1u) ? (v > a[q] || v != v ? v : a[q]) : a[q] + v;
+            const double v = partials[(size_t)b * 8 + 2 * q];
+            if (ops >> q & 1u) h[q] = (v > h[q] || v != v) ? v : h[q];
+            else dd_combine(h[q], l[q], v, partials[(size_t)b * 8 + 2 * q + 1]);
         }
     }
-    __shared__ double red[16][4];
-    const unsigned lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) a[q] = (ops >> q & 1u) ? wave_max(a[q]) : wave_sum(a[q]);
-    if (lane == 0) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) red[wid][q] = a[q];
-    }
+    __shared__ double part[8];
+    block_reduce4(h, l, ops, part, 0);
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int q = 0; q < 4; ++q) {
-            double acc = red[0][q];
-            for (unsigned w = 1; w < blockDim.x / 64; ++w) {
-                double o = red[w][q];
-                acc = (ops >> q & 1u) ? (o > acc || o != o ? o : acc) : acc + o;
-            }
-            result[q] = acc;
-        }
+        for (int q = 0; q < 4; ++q) result[q] = (ops >> q & 1u) ? part[2 * q] : part[2 * q] + part[2 * q + 1];
     }
 }
 
@@ -144,23 +133,22 @@ int read_result4(const double *result_dev, double out[4]) {
     return HT3D_OK;
 }
 
-// generic 1D reduction over n doubles into partials (one value in slot 0)
+// generic 1D reduction over n doubles into partials (slot 0)
 __global__ __launch_bounds__(256) void reduce_kernel(const double *__restrict__ a, size_t n, int op,
                                                      double *__restrict__ partials) {
     const bool is_max = (op == HT3D_MAX || op == HT3D_MAX_ABS);
     const bool is_min = (op == HT3D_MIN);
-    double acc = is_max ? -INFINITY : (is_min ? INFINITY : 0.0);
+    double hi = (is_max || is_min) ? -INFINITY : 0.0, lo = 0.0;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         double v = a[i];
         if (op == HT3D_SUM_ABS || op == HT3D_MAX_ABS) v = fabs(v);
-        if (is_max) acc = (v > acc || v != v) ? v : acc;
-        else if (is_min) acc = (v < acc || v != v) ? v : acc;
-        else acc += v;
+        if (is_min) v = -v;  // min as the max of negatives
+        if (is_max || is_min) hi = (v > hi || v != v) ? v : hi;
+        else dd_add(hi, lo, v);
     }
-    // min via max of negatives keeps a single reduction helper
-    double x = is_min ? -acc : acc;
-    block_reduce4(x, 0.0, 0.0, 0.0, (is_max || is_min) ? 1u : 0u, partials, blockIdx.x);
+    const double h4[4] = {hi, 0.0, 0.0, 0.0}, l4[4] = {lo, 0.0, 0.0, 0.0};
+    block_reduce4(h4, l4, (is_max || is_min) ? 1u : 0u, partials, blockIdx.x);
 }
 
 }  // namespace ht3d
@@ -328,7 +316,7 @@ int ht3d_reduce(double *out_host, const double *a, size_t n, int op) {
     unsigned nb = div_up(n, 256);
     if (nb > 2048) nb = 2048;
     void *part = nullptr, *res = nullptr;
-    int rc = workspace(WS_MISC, (size_t)nb * 4 * sizeof(double), &part);
+    int rc = workspace(WS_MISC, (size_t)nb * 8 * sizeof(double), &part);
     if (rc) return rc;
     rc = workspace(WS_RESULT, 64 * sizeof(double), &res);
     if (rc) return rc;

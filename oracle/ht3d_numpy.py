@@ -253,7 +253,7 @@ def dyrel_solve(X, R, kx, ky, kz, *, dx, dy, dz, dt, transient, rhoC, bc_type, b
         F0, X = DYRELUpdate(F0, X, F, h1, h2, dtau)
         if chk:
             FD = F * D
-            nF_abs = float(np.abs(FD).sum()) / N / math.sqrt(N) if norm == 0 else float(np.abs(FD).max())
+            nF_abs = math.fsum(np.abs(FD).ravel()) / N / math.sqrt(N) if norm == 0 else float(np.abs(FD).max())
             if it == 1:
                 nF_ini = nF_abs
             nF_rel = nF_abs / nF_ini
@@ -265,8 +265,8 @@ def dyrel_solve(X, R, kx, ky, kz, *, dx, dy, dz, dt, transient, rhoC, bc_type, b
                 break
             if lam and it > 1:
                 d = dtau * F0
-                lmin = abs(float(((-d) * (F - Fcit)).sum()))
-                lmin /= abs(float((d * d).sum()))
+                lmin = abs(math.fsum(((-d) * (F - Fcit)).ravel()))
+                lmin /= abs(math.fsum((d * d).ravel()))
                 c = 2.0 * math.sqrt(lmin) * cfact
                 h1, h2 = (2 - c * dtau) / (2 + c * dtau), 2 * dtau / (2 + c * dtau)
                 n_lambda += 1
@@ -307,7 +307,7 @@ class HRL:
         Rt, kx, ky, kz = InitThermal(self.T, self.ktv, _dt)
         self.T, repT, qy = dyrel_solve(self.T, Rt, kx, ky, kz, dx=dx, dy=dy, dz=dz, dt=dt, transient=1, rhoC=1.0,
                                        bc_type=(0, 0, 1, 1, 0, 0), bc_val=(0.0, 0.0, 1.0, 0.0, 0.0, 0.0), **self.prm)
-        Nu = float(qy[:, -1, :].sum()) / (self.L[0] * self.L[2]) / (nx * nz)
+        Nu = math.fsum(qy[:, -1, :].ravel()) / (self.L[0] * self.L[2]) / (nx * nz)
         Ty, kx, ky, kz = InitDarcy(self.T, self.kfv)
         Ty = KillNorthSouthFluxes(Ty)
         Rp = Compute_Rp(ky, Ty, self.Ra, _dy)

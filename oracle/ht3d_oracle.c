@@ -569,34 +569,54 @@ EXPORT void o_Advect_v2(double *Fc, const double *X, const double *V, int axis, 
 
 enum { R_SUM = 0, R_SUM_ABS = 1, R_MAX = 2, R_MIN = 3, R_MAX_ABS = 4 };
 
+/* Sums are accumulated in double-double (TwoSum, no FMA), so the rounded result
+ * is the correctly rounded sum except in vanishingly rare ties, whatever the
+ * order; maxima are exact.  (Julia's pairwise `sum` is a different rounding of
+ * the same sum; the difference is at the last bit.)                          */
+static inline void dd_add(double *hi, double *lo, double x) {
+    double s = *hi + x, bp = s - *hi;
+    double e = (*hi - (s - bp)) + (x - bp);
+    *hi = s;
+    *lo += e;
+}
+
+static inline void dd_combine(double *hi, double *lo, double h2, double l2) {
+    double s = *hi + h2, bp = s - *hi;
+    double e = (*hi - (s - bp)) + (h2 - bp);
+    e += *lo + l2;
+    *hi = s + e;
+    *lo = e - (*hi - s);
+}
+
 EXPORT double o_reduce(const double *a, size_t n1, size_t n23, int op) {
-    double *part = (double *)malloc(n23 * sizeof(double));
+    double *part = (double *)malloc(2 * n23 * sizeof(double));
+    const int is_sum = (op == R_SUM || op == R_SUM_ABS);
     #pragma omp parallel for schedule(static)
     for (long p = 0; p < (long)n23; ++p) {
         const double *r = a + (size_t)p * n1;
-        double acc = (op == R_MAX || op == R_MAX_ABS) ? -INFINITY : (op == R_MIN ? INFINITY : 0.0);
+        double hi = is_sum ? 0.0 : (op == R_MIN ? INFINITY : -INFINITY), lo = 0.0;
         for (size_t i = 0; i < n1; ++i) {
             double v = r[i];
             switch (op) {
-            case R_SUM: acc += v; break;
-            case R_SUM_ABS: acc += fabs(v); break;
-            case R_MAX: acc = v > acc ? v : acc; break;
-            case R_MIN: acc = v < acc ? v : acc; break;
-            default: acc = fabs(v) > acc ? fabs(v) : acc; break;
+            case R_SUM: dd_add(&hi, &lo, v); break;
+            case R_SUM_ABS: dd_add(&hi, &lo, fabs(v)); break;
+            case R_MAX: hi = (v > hi || v != v) ? v : hi; break;
+            case R_MIN: hi = (v < hi || v != v) ? v : hi; break;
+            default: hi = (fabs(v) > hi || v != v) ? fabs(v) : hi; break;
             }
         }
-        part[p] = acc;
+        part[2 * p] = hi;
+        part[2 * p + 1] = lo;
     }
-    double acc = (op == R_MAX || op == R_MAX_ABS) ? -INFINITY : (op == R_MIN ? INFINITY : 0.0);
+    double hi = is_sum ? 0.0 : (op == R_MIN ? INFINITY : -INFINITY), lo = 0.0;
     for (size_t p = 0; p < n23; ++p) {
-        double v = part[p];
-        if (op == R_SUM || op == R_SUM_ABS) acc += v;
-        else if (op == R_MIN) acc = v < acc ? v : acc;
-        else acc = v > acc ? v : acc;
-        if (v != v) acc = v; /* NaN propagates */
+        double v = part[2 * p];
+        if (is_sum) dd_combine(&hi, &lo, v, part[2 * p + 1]);
+        else if (op == R_MIN) hi = (v < hi || v != v) ? v : hi;
+        else hi = (v > hi || v != v) ? v : hi;
     }
     free(part);
-    return acc;
+    return is_sum ? hi + lo : hi;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -878,9 +898,9 @@ EXPORT int o_hrl_step(o_hrl_model *m) {
 
     /* Nusselt number from the last thermal flux (Alban_v6:345-348) */
     {
-        double s = 0.0;
-        for (int k = 1; k <= nz; ++k) for (int i = 1; i <= nx; ++i) s += FY(m->qy, i, ny + 1, k);
-        m->Nu = s / (m->Lx * m->Lz) / (double)(nx * nz);
+        double s = 0.0, sl = 0.0;
+        for (int k = 1; k <= nz; ++k) for (int i = 1; i <= nx; ++i) dd_add(&s, &sl, FY(m->qy, i, ny + 1, k));
+        m->Nu = (s + sl) / (m->Lx * m->Lz) / (double)(nx * nz);
     }
 
     /* Darcy solve (Alban_v6:353-370 with the DYREL loop of v18:489-537) */

@@ -1,0 +1,62 @@
+"""The C ABI: libht3d.so loads and exports every entry point include/ht3d.h
+declares (no GPU call is made), and the product path fails loudly without it."""
+import ctypes as C
+import os
+import re
+import subprocess
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _declared():
+    text = re.sub(r"/\*.*?\*/", "", (ROOT / "include" / "ht3d.h").read_text(), flags=re.S)
+    return re.findall(r"^(?:int|const char \*)\s*(ht3d_\w+)\(", text, flags=re.M)
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    assert len(names) == len(set(names)) and len(names) > 60
+    for must in ("ht3d_init", "ht3d_ComputeFlux", "ht3d_DYRELUpdate", "ht3d_GershgorinPoisson", "ht3d_dyrel_solve",
+                 "ht3d_weno5_advect", "ht3d_hrl_step", "ht3d_reduce"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from hydrothermal3d_amd import _lib
+    if not _lib.LIBPATH.exists():
+        pytest.fail(f"{_lib.LIBPATH} is not built (run __graft_entry__.build())")
+    L = C.CDLL(str(_lib.LIBPATH))
+    missing = [n for n in _declared() if not hasattr(L, n)]
+    assert not missing, f"declared but not exported: {missing}"
+    sigs = _lib.parse_header()
+    assert set(sigs) == set(_declared())
+
+
+def test_uninitialised_calls_return_state_error():
+    from hydrothermal3d_amd import _lib
+    if _lib._initialised:
+        pytest.skip("library already initialised in this process")
+    L = _lib.lib()
+    # no ht3d_init in this process: entry points refuse instead of touching a device
+    assert L.ht3d_ComputeFlux(*([None] * 7), 1.0, 1.0, 1.0, 4, 4, 4) == _lib.ERR_STATE
+    assert L.ht3d_strerror(_lib.NAN).decode().startswith("NaN")
+
+
+def test_product_fails_loudly_without_library(tmp_path):
+    code = ("import hydrothermal3d_amd as ht\n"
+            "try:\n    ht.DeviceArray((2, 2, 2))\nexcept RuntimeError as e:\n    print('RAISED', 'not found' in str(e))\n")
+    env = dict(os.environ, HT3D_LIB=str(tmp_path / "missing.so"), PYTHONPATH=str(ROOT))
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, cwd=ROOT)
+    assert "RAISED True" in out.stdout, out.stderr
+
+
+def test_product_does_not_import_oracle():
+    code = "import sys, hydrothermal3d_amd, hydrothermal3d_amd.solver, hydrothermal3d_amd.inputs; " \
+           "print(any(m == 'oracle' or m.startswith('oracle.') for m in sys.modules))"
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=ROOT,
+                         env=dict(os.environ, PYTHONPATH=str(ROOT)))
+    assert out.stdout.strip() == "False", out.stderr

@@ -1,0 +1,103 @@
+"""Golden fixtures (tests/golden/*.npz, written by tests/golden/make_golden.py from
+the independent NumPy restatement) reproduced by the C oracle (CPU) and by the
+HIP library (GPU)."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from helpers import rel_err
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+def _load(name):
+    return dict(np.load(GOLD / name, allow_pickle=False))
+
+
+def _field_args(g):
+    from hydrothermal3d_amd import inputs
+    nx, ny, nz = (int(v) for v in g["dims"])
+    w = inputs.Workload("golden", nx, ny, nz, *map(float, g["L"]), Ra=float(g["Ra"]), cfl=float(g["cfl"]),
+                        seed=int(g["seed"]), sigma=0.5, modes=int(g["modes"]))
+    return w
+
+
+def test_golden_lognormal_field_cpu(orc):
+    from hydrothermal3d_amd import inputs
+    g = _load("hrl_8x6x8.npz")
+    w = _field_args(g)
+    kfv = np.zeros(g["kfv"].shape, order="F")
+    orc.lognormal_field(kfv, *inputs.field_args(w))
+    assert np.allclose(kfv, g["kfv"], rtol=1e-12, atol=0)
+
+
+def test_golden_hrl_cpu(orc):
+    g = _load("hrl_8x6x8.npz")
+    nx, ny, nz = (int(v) for v in g["dims"])
+    Lx, Ly, Lz = map(float, g["L"])
+    m = orc.HRLModel(nx, ny, nz, Lx, Ly, Lz, float(g["Ra"]), float(g["cfl"]))
+    m.field("Tc_ex")[...] = g["T0"]
+    m.field("kfv")[...] = g["kfv"]
+    p = orc.params(nitmax=20000, nitout=100, ncheck=25, norm=1, tol=1e-10)
+    m.set_params(p, p, 1)
+    for s in range(3):
+        r = m.step()
+        assert r["thermal"]["iters"] == g["iters_T"][s] and r["darcy"]["iters"] == g["iters_P"][s]
+        assert r["Nu"] == g["Nu"][s]
+    I = (slice(1, -1),) * 3
+    assert np.array_equal(m.field("Tc_ex")[I], g["T"][I])
+    assert np.array_equal(m.field("Pc_ex")[I], g["P"][I])
+    for k in ("Vx", "Vy", "Vz"):
+        assert np.array_equal(m.field(k), g[k])
+
+
+@pytest.mark.parametrize("version", [1, 2])
+def test_golden_weno_cpu(orc, version):
+    g = _load("weno_7x5x6.npz")
+    T = np.asfortranarray(g["T"].copy())
+    dt, _dx, _dy, _dz, Ttop, Tbot = map(float, g["args"])
+    if version == 1:
+        orc.AdvectWithWeno5(T, g["Vx"], g["Vy"], g["Vz"], dt, _dx, _dy, _dz, Ttop, Tbot)
+    else:
+        nx, ny, nz = (s - 2 for s in T.shape)
+        orc.AdvectWithWeno5_v2(T, np.zeros((nx + 6, ny + 6, nz + 6), order="F"), np.zeros((nx, ny, nz), order="F"),
+                               g["Vx"], g["Vy"], g["Vz"], dt, _dx, _dy, _dz, Ttop, Tbot)
+    assert np.array_equal(T[1:-1, 1:-1, 1:-1], g[f"T_v{version}"][1:-1, 1:-1, 1:-1])
+
+
+@pytest.mark.gpu
+def test_golden_hrl_gpu(ht):
+    g = _load("hrl_8x6x8.npz")
+    nx, ny, nz = (int(v) for v in g["dims"])
+    Lx, Ly, Lz = map(float, g["L"])
+    m = ht.HydroThermal3D(nx, ny, nz, Lx, Ly, Lz, float(g["Ra"]), float(g["cfl"]))
+    m["Tc_ex"].set(g["T0"])
+    m["kfv"].set(g["kfv"])
+    p = ht.DyrelParams(nitmax=20000, nitout=100, ncheck=25, norm=1, tol=1e-10)
+    m.set_params(p, p, 1)
+    for s in range(3):
+        r = m.step()
+        assert r["thermal"]["iters"] == g["iters_T"][s] and r["darcy"]["iters"] == g["iters_P"][s]
+        assert abs(r["Nu"] - g["Nu"][s]) <= 1e-10 * abs(g["Nu"][s])
+    I = (slice(1, -1),) * 3
+    assert rel_err(m["Tc_ex"].get()[I], g["T"][I]) <= 1e-10
+    P = m["Pc_ex"].get()[I]
+    assert rel_err(P - P.mean(), g["P"][I] - g["P"][I].mean()) <= 1e-10
+    for k in ("Vx", "Vy", "Vz"):
+        assert rel_err(m[k].get(), g[k]) <= 1e-10
+    m.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("version", [1, 2])
+def test_golden_weno_gpu(ht, version):
+    g = _load("weno_7x5x6.npz")
+    dt, _dx, _dy, _dz, Ttop, Tbot = map(float, g["args"])
+    T = ht.DeviceArray.from_host(np.asfortranarray(g["T"]))
+    V = [ht.DeviceArray.from_host(np.asfortranarray(g[k])) for k in ("Vx", "Vy", "Vz")]
+    if version == 1:
+        ht.kernels.AdvectWithWeno5(T, *V, dt, _dx, _dy, _dz, Ttop, Tbot)
+    else:
+        ht.kernels.AdvectWithWeno5_v2(T, None, None, *V, dt, _dx, _dy, _dz, Ttop, Tbot)
+    assert np.array_equal(T.get()[1:-1, 1:-1, 1:-1], g[f"T_v{version}"][1:-1, 1:-1, 1:-1])
