@@ -39,6 +39,7 @@ struct IterArgs {
     const T *__restrict__ kx;
     const T *__restrict__ ky;
     const T *__restrict__ kz;
+    const T *__restrict__ Dst;   // stored Gershgorin diagonal (DSTORED variants)
     T *__restrict__ F0;
     T *__restrict__ Fst;         // F of the previous lambda iteration (read) / this one (write)
     double *__restrict__ partials;
@@ -64,7 +65,11 @@ __device__ __forceinline__ T ghost(int type, T val, T inner) {
     return type ? (T)2 * val - inner : inner;
 }
 
-template <typename T, bool TRANSIENT, int TY, int KZ>
+// One fused pseudo-iteration.  Every neighbour address lies inside the E array
+// (the ghost layer is part of its allocation), so all loads are unconditional
+// and the boundary condition is applied by selecting the ghost value computed
+// from the centre cell; offsets are 32-bit (E arrays stay below 2^32 words).
+template <typename T, bool TRANSIENT, bool DSTORED, bool NTX, int TY, int KZ>
 __global__ __launch_bounds__(64 * TY) void dyrel_iter_kernel(IterArgs<T> a) {
     const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
     const unsigned t = xcd_tile(blockIdx.x, nblocks);
@@ -73,35 +78,41 @@ __global__ __launch_bounds__(64 * TY) void dyrel_iter_kernel(IterArgs<T> a) {
     const int j = by * TY + (int)(threadIdx.x >> 6);
     const int k0 = bz * KZ;
     const int k1 = min(k0 + KZ, a.nz);
-    const int nx = a.nx, ny = a.ny;
-    const size_t sE1 = (size_t)nx + 2, sE2 = sE1 * ((size_t)ny + 2);
-    const size_t sC2 = (size_t)nx * ny;
-    const size_t sX2 = (size_t)(nx + 1) * ny, sY1 = (size_t)nx, sY2 = (size_t)nx * (ny + 1);
+    const uint32_t nx = a.nx, ny = a.ny;
+    const uint32_t sE1 = nx + 2, sE2 = sE1 * (ny + 2);
+    const uint32_t sC2 = nx * ny, sX1 = nx + 1, sX2 = sX1 * ny, sY2 = nx * (ny + 1);
 
     double s_hi = 0.0, s_lo = 0.0, acc_max = 0.0, n_hi = 0.0, n_lo = 0.0, d_hi = 0.0, d_lo = 0.0;
     const bool check = a.flags & FL_CHECK, lam = a.flags & FL_LAMBDA, store = a.flags & FL_STORE_F;
 
-    if (i < nx && j < ny) {
-        const T *Xc = a.Xin + (size_t)(i + 1) + (size_t)(j + 1) * sE1;  // column base at E z-index 0
-        T xc = Xc[(size_t)(k0 + 1) * sE2];
-        T xm = k0 == 0 ? ghost(a.bct[4], a.bcv[4], xc) : Xc[(size_t)k0 * sE2];
-        T kzB = a.kz[(size_t)k0 * sC2 + (size_t)j * nx + i];
-        const bool wlo = i == 0, whi = i == nx - 1, slo = j == 0, shi = j == ny - 1;
+    if ((uint32_t)i < nx && (uint32_t)j < ny) {
+        const bool wlo = i == 0, whi = (uint32_t)i == nx - 1, slo = j == 0, shi = (uint32_t)j == ny - 1;
+        uint32_t e = (uint32_t)(i + 1) + (uint32_t)(j + 1) * sE1 + (uint32_t)(k0 + 1) * sE2;
+        uint32_t c = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k0 * sC2;
+        uint32_t fx = (uint32_t)i + (uint32_t)j * sX1 + (uint32_t)k0 * sX2;
+        uint32_t fy = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k0 * sY2;
+        T xc = a.Xin[e];
+        T xm = a.Xin[e - sE2];
+        if (k0 == 0) xm = ghost(a.bct[4], a.bcv[4], xc);
+        T kzB = a.kz[c];
 #pragma unroll 2
         for (int k = k0; k < k1; ++k) {
-            const size_t e = (size_t)(k + 1) * sE2;
-            const T xp = (k + 1 == a.nz) ? ghost(a.bct[5], a.bcv[5], xc) : Xc[e + sE2];
-            const T xW = wlo ? ghost(a.bct[0], a.bcv[0], xc) : Xc[e - 1];
-            const T xE = whi ? ghost(a.bct[1], a.bcv[1], xc) : Xc[e + 1];
-            const T xS = slo ? ghost(a.bct[2], a.bcv[2], xc) : Xc[e - sE1];
-            const T xN = shi ? ghost(a.bct[3], a.bcv[3], xc) : Xc[e + sE1];
-            const size_t c = (size_t)k * sC2 + (size_t)j * nx + i;
-            const size_t fxi = (size_t)k * sX2 + (size_t)j * (nx + 1) + i;
-            const size_t fyi = (size_t)k * sY2 + (size_t)j * sY1 + i;
-            const T kW = a.kx[fxi], kE = a.kx[fxi + 1];
-            const T kS = a.ky[fyi], kN = a.ky[fyi + sY1];
-            const T kF = a.kz[c + sC2];
+            T xp = a.Xin[e + sE2];
+            T xW = a.Xin[e - 1], xE = a.Xin[e + 1];
+            T xS = a.Xin[e - sE1], xN = a.Xin[e + sE1];
+            // streams read once per iteration: non-temporal, so the L2 keeps the
+            // X planes whose neighbours the other waves of the tile still need
+            const T kW = __builtin_nontemporal_load(a.kx + fx), kE = __builtin_nontemporal_load(a.kx + fx + 1);
+            const T kS = __builtin_nontemporal_load(a.ky + fy), kN = __builtin_nontemporal_load(a.ky + fy + nx);
+            const T kF = __builtin_nontemporal_load(a.kz + c + sC2);
             const T kB = kzB;
+            const T r = __builtin_nontemporal_load(a.R + c);
+            const T f0o = __builtin_nontemporal_load(a.F0 + c);
+            if (k + 1 == a.nz) xp = ghost(a.bct[5], a.bcv[5], xc);
+            if (wlo) xW = ghost(a.bct[0], a.bcv[0], xc);
+            if (whi) xE = ghost(a.bct[1], a.bcv[1], xc);
+            if (slo) xS = ghost(a.bct[2], a.bcv[2], xc);
+            if (shi) xN = ghost(a.bct[3], a.bcv[3], xc);
             // ComputeFlux! (kernels_HT3D.jl:112-114)
             const T qW = -kW * (a._dx * (xc - xW)), qE = -kE * (a._dx * (xE - xc));
             const T qS = -kS * (a._dy * (xc - xS)), qN = -kN * (a._dy * (xN - xc));
@@ -109,17 +120,23 @@ __global__ __launch_bounds__(64 * TY) void dyrel_iter_kernel(IterArgs<T> a) {
             // ResidualDiffusion(Transient)! (kernels_HT3D.jl:131, Alban_v6:87)
             T f;
             if (TRANSIENT)
-                f = a.R[c] + a._dt * xc + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
+                f = r + a._dt * xc + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
             else
-                f = a.R[c] + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
-            // GershgorinPoisson! diagonal D and the "F /= PC" step
-            const T D = (kB / a.dz + kF / a.dz) / a.dz + (kN / a.dy + kS / a.dy) / a.dy +
-                        (kE / a.dx + kW / a.dx) / a.dx + a.rc_tr_dt;
+                f = r + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
+            // GershgorinPoisson! diagonal D (stored once per solve, or recomputed
+            // from the face coefficients already in registers) and "F /= PC"
+            T D;
+            if (DSTORED)
+                D = a.Dst[c];
+            else
+                D = (kB / a.dz + kF / a.dz) / a.dz + (kN / a.dy + kS / a.dy) / a.dy + (kE / a.dx + kW / a.dx) / a.dx +
+                    a.rc_tr_dt;
             f = f / D;
             // DYRELUpdate! (kernels_HT3D_dimensional.jl:323-324)
-            const T f0 = a.h1 * a.F0[c] + a.h2 * f;
-            a.F0[c] = f0;
-            a.Xout[(size_t)(i + 1) + (size_t)(j + 1) * sE1 + e] = xc - a.dtau * f0;
+            const T f0 = a.h1 * f0o + a.h2 * f;
+            __builtin_nontemporal_store(f0, a.F0 + c);
+            if (NTX) __builtin_nontemporal_store(xc - a.dtau * f0, a.Xout + e);
+            else a.Xout[e] = xc - a.dtau * f0;
             if (check) {
                 const double fd = fabs((double)(f * D));
                 dd_add(s_hi, s_lo, fd);
@@ -134,6 +151,10 @@ __global__ __launch_bounds__(64 * TY) void dyrel_iter_kernel(IterArgs<T> a) {
             xm = xc;
             xc = xp;
             kzB = kF;
+            e += sE2;
+            c += sC2;
+            fx += sX2;
+            fy += sY2;
         }
     }
     if (check) {
@@ -143,11 +164,12 @@ __global__ __launch_bounds__(64 * TY) void dyrel_iter_kernel(IterArgs<T> a) {
 }
 
 // Gershgorin bound (GershgorinPoisson!, kernels_HT3D_dimensional.jl:207-221): max of G
+// (optionally storing the diagonal D for the DSTORED iteration variants)
 __global__ __launch_bounds__(256) void gershgorin_max_kernel(const double *__restrict__ kx,
                                                              const double *__restrict__ ky,
                                                              const double *__restrict__ kz, int nx, int ny, int nz,
                                                              double dx, double dy, double dz, double rc_tr_dt,
-                                                             double *__restrict__ partials) {
+                                                             double *__restrict__ partials, double *__restrict__ Dout) {
     const size_t N = (size_t)nx * ny * nz;
     double m = -INFINITY;
     for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < N; c += (size_t)gridDim.x * blockDim.x) {
@@ -159,6 +181,7 @@ __global__ __launch_bounds__(256) void gershgorin_max_kernel(const double *__res
         const double g = fabs(kE / (dx * dx)) + fabs(kW / (dx * dx)) + fabs(kN / (dy * dy)) + fabs(kS / (dy * dy)) +
                          fabs(kB / (dz * dz)) + fabs(kF / (dz * dz)) + fabs(diag);
         const double G = g / diag;
+        if (Dout) Dout[c] = diag;
         m = (G > m || G != G) ? G : m;
     }
     const double h4[4] = {m, 0.0, 0.0, 0.0}, l4[4] = {0.0, 0.0, 0.0, 0.0};
@@ -205,15 +228,161 @@ __global__ __launch_bounds__(256) void nu_sum_kernel(const double *__restrict__ 
 }
 
 // ------------------------------------------------------------------ host driver
-static constexpr int TY = 4;
-static constexpr int KZ = 8;
+// Tile configurations (rows of 64 cells per block, z cells per thread); the
+// autotuner picks one per (grid, operator, precision, D mode).
+struct TileCfg {
+    int ty, kz;
+};
+static constexpr TileCfg kTiles[] = {{2, 8}, {2, 16}, {4, 4}, {4, 8}, {4, 16}, {4, 32}, {8, 4}, {8, 8},
+                                      {8, 16}, {4, 2}, {8, 2}, {16, 2}, {16, 4}, {1, 8}};
+static constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
+static constexpr int kDefaultTile = 2;
+
+template <typename T, bool TR, bool DS, bool NX, int TY, int KZ>
+static void launch_t(const IterArgs<T> &a, unsigned nb, hipStream_t st) {
+    hipLaunchKernelGGL((dyrel_iter_kernel<T, TR, DS, NX, TY, KZ>), dim3(nb), dim3(64 * TY), 0, st, a);
+}
+
+template <typename T, bool TR, bool DS, bool NX>
+static void launch_tile(int tile, const IterArgs<T> &a, unsigned nb, hipStream_t st) {
+    switch (tile) {
+    case 0: launch_t<T, TR, DS, NX, 2, 8>(a, nb, st); break;
+    case 1: launch_t<T, TR, DS, NX, 2, 16>(a, nb, st); break;
+    case 2: launch_t<T, TR, DS, NX, 4, 4>(a, nb, st); break;
+    case 3: launch_t<T, TR, DS, NX, 4, 8>(a, nb, st); break;
+    case 4: launch_t<T, TR, DS, NX, 4, 16>(a, nb, st); break;
+    case 5: launch_t<T, TR, DS, NX, 4, 32>(a, nb, st); break;
+    case 6: launch_t<T, TR, DS, NX, 8, 4>(a, nb, st); break;
+    case 7: launch_t<T, TR, DS, NX, 8, 8>(a, nb, st); break;
+    case 8: launch_t<T, TR, DS, NX, 8, 16>(a, nb, st); break;
+    case 9: launch_t<T, TR, DS, NX, 4, 2>(a, nb, st); break;
+    case 10: launch_t<T, TR, DS, NX, 8, 2>(a, nb, st); break;
+    case 11: launch_t<T, TR, DS, NX, 16, 2>(a, nb, st); break;
+    case 12: launch_t<T, TR, DS, NX, 16, 4>(a, nb, st); break;
+    default: launch_t<T, TR, DS, NX, 1, 8>(a, nb, st); break;
+    }
+}
+
+// variant bits: 1 = read the stored diagonal instead of recomputing it,
+//               2 = non-temporal store of the updated unknown
+struct IterCfg {
+    int tile = kDefaultTile;
+    int variant = 0;
+};
+
+template <typename T>
+static void set_tiling(IterArgs<T> &a, int tile) {
+    a.nbx = (int)div_up(a.nx, 64);
+    a.nby = (int)div_up(a.ny, kTiles[tile].ty);
+    a.nbz = (int)div_up(a.nz, kTiles[tile].kz);
+}
 
 template <typename T, bool TR>
-static int launch_iter(const IterArgs<T> &a) {
+static void launch_variant(const IterArgs<T> &a, const IterCfg &cfg, unsigned nb, hipStream_t st) {
+    switch (cfg.variant & 3) {
+    case 0: launch_tile<T, TR, false, false>(cfg.tile, a, nb, st); break;
+    case 1: launch_tile<T, TR, true, false>(cfg.tile, a, nb, st); break;
+    case 2: launch_tile<T, TR, false, true>(cfg.tile, a, nb, st); break;
+    default: launch_tile<T, TR, true, true>(cfg.tile, a, nb, st); break;
+    }
+}
+
+template <typename T>
+static int launch_iter(const IterArgs<T> &a, bool transient, const IterCfg &cfg) {
     const unsigned nb = (unsigned)a.nbx * a.nby * a.nbz;
-    hipLaunchKernelGGL((dyrel_iter_kernel<T, TR, TY, KZ>), dim3(nb), dim3(64 * TY), 0, ctx().stream, a);
+    hipStream_t st = ctx().stream;
+    if (transient) launch_variant<T, true>(a, cfg, nb, st);
+    else launch_variant<T, false>(a, cfg, nb, st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "dyrel_iter_kernel");
+    return HT3D_OK;
+}
+
+// ---------------------------------------------------------------- autotuner
+struct TuneKey {
+    int nx, ny, nz, transient, prec;
+    bool operator==(const TuneKey &o) const {
+        return nx == o.nx && ny == o.ny && nz == o.nz && transient == o.transient && prec == o.prec;
+    }
+};
+struct TuneEntry {
+    TuneKey key;
+    IterCfg cfg;
+    double us;
+};
+static std::vector<TuneEntry> g_tune;
+static bool g_autotune = true;
+static int g_force_tile = -1, g_force_variant = 0;
+
+static const TuneEntry *tune_find(const TuneKey &k) {
+    for (const auto &t : g_tune)
+        if (t.key == k) return &t;
+    return nullptr;
+}
+
+// Times every tile configuration (both D modes) on the solve's own read-only
+// inputs, writing only scratch outputs (Xout, F0), and keeps the fastest.
+template <typename T>
+static int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, IterCfg *out) {
+    const TuneKey key{a.nx, a.ny, a.nz, transient ? 1 : 0, prec};
+    if (const TuneEntry *t = tune_find(key)) {
+        *out = t->cfg;
+        if (!can_store_d) out->variant &= ~1;
+        return HT3D_OK;
+    }
+    Context &cx = ctx();
+    a.flags = 0;
+    IterCfg best;
+    double best_us = 1e300;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const int reps = 8;
+    for (int var = 0; var < 4; ++var) {
+        if ((var & 1) && !can_store_d) continue;
+        for (int tile = 0; tile < kNumTiles; ++tile) {
+            IterCfg cfg{tile, var};
+            set_tiling(a, tile);
+            int rc = launch_iter(a, transient, cfg);
+            if (rc) return rc;
+            HIP_TRY(hipEventRecord(e0, cx.stream));
+            for (int r = 0; r < reps; ++r)
+                if ((rc = launch_iter(a, transient, cfg))) return rc;
+            HIP_TRY(hipEventRecord(e1, cx.stream));
+            HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            const double us = 1e3 * ms / reps;
+            if (us < best_us) {
+                best_us = us;
+                best = cfg;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g_tune.push_back(TuneEntry{key, best, best_us});
+    *out = best;
+    return HT3D_OK;
+}
+
+static IterCfg &last_cfg() {
+    static IterCfg c;
+    return c;
+}
+
+template <typename T>
+static int choose_cfg(IterArgs<T> &a, bool transient, bool can_store_d, int prec, IterCfg *cfg) {
+    IterCfg c;
+    if (g_force_tile >= 0) {
+        c.tile = g_force_tile;
+        c.variant = g_force_variant & (can_store_d ? 3 : 2);
+    } else if (g_autotune) {
+        int rc = autotune(a, transient, can_store_d, prec, &c);
+        if (rc) return rc;
+    }
+    set_tiling(a, c.tile);
+    *cfg = c;
     return HT3D_OK;
 }
 
@@ -223,7 +392,7 @@ static inline bool lambda_iter(int iter, int nitout) { return iter <= 2 || (nito
 int bc_fill_table(double *A, int nx, int ny, int nz, const int type[6], const double val[6]);
 
 int gershgorin_lmax(const double *kx, const double *ky, const double *kz, int nx, int ny, int nz, double dx,
-                    double dy, double dz, double rc_tr_dt, double *lmax) {
+                    double dy, double dz, double rc_tr_dt, double *lmax, double *Dout) {
     void *part = nullptr, *res = nullptr;
     const size_t N = (size_t)nx * ny * nz;
     unsigned nb = div_up(N, 256);
@@ -233,7 +402,7 @@ int gershgorin_lmax(const double *kx, const double *ky, const double *kz, int nx
     rc = workspace(WS_RESULT, 64 * sizeof(double), &res);
     if (rc) return rc;
     hipLaunchKernelGGL(gershgorin_max_kernel, dim3(nb), dim3(256), 0, ctx().stream, kx, ky, kz, nx, ny, nz, dx, dy,
-                       dz, rc_tr_dt, (double *)part);
+                       dz, rc_tr_dt, (double *)part, Dout);
     HIP_TRY(hipGetLastError());
     rc = finish_partials((double *)part, nb, 1u, (double *)res);
     if (rc) return rc;
@@ -257,42 +426,50 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
     Context &cx = ctx();
     const size_t N = (size_t)nx * ny * nz, NE = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
 
-    void *xb = nullptr, *f0 = nullptr, *fst = nullptr, *part = nullptr, *res = nullptr;
+    void *xb = nullptr, *f0 = nullptr, *fst = nullptr, *part = nullptr, *res = nullptr, *dst = nullptr;
     int rc;
     if ((rc = workspace(WS_XB, NE * sizeof(double), &xb))) return rc;
     if ((rc = workspace(WS_F0, N * sizeof(double), &f0))) return rc;
     if ((rc = workspace(WS_FCIT, N * sizeof(double), &fst))) return rc;
+    if ((rc = workspace(WS_DIAG, N * sizeof(double), &dst))) return rc;
 
     IterArgs<double> a;
     std::memset(&a, 0, sizeof a);
     a.nx = nx; a.ny = ny; a.nz = nz;
-    a.nbx = (int)div_up(nx, 64); a.nby = (int)div_up(ny, TY); a.nbz = (int)div_up(nz, KZ);
-    const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
-    if ((rc = workspace(WS_PARTIALS, (size_t)nblocks * 8 * sizeof(double), &part))) return rc;
+    // most blocks of any tile configuration (1 row, or 2 planes, per block at the least)
+    const unsigned max_blocks = div_up(nx, 64) * div_up(ny, 1) * div_up(nz, 2);
+    if ((rc = workspace(WS_PARTIALS, (size_t)max_blocks * 8 * sizeof(double), &part))) return rc;
     if ((rc = workspace(WS_RESULT, 64 * sizeof(double), &res))) return rc;
 
-    // set-up (v18:432-439)
+    // set-up (v18:432-439); the diagonal D is kept for the DSTORED variants
     const double rc_tr_dt = f->rhoC * (double)f->transient / f->dt;
     double lmax = 0.0;
-    if ((rc = gershgorin_lmax(f->kx, f->ky, f->kz, nx, ny, nz, f->dx, f->dy, f->dz, rc_tr_dt, &lmax))) return rc;
+    if ((rc = gershgorin_lmax(f->kx, f->ky, f->kz, nx, ny, nz, f->dx, f->dy, f->dz, rc_tr_dt, &lmax, (double *)dst)))
+        return rc;
     double lmin = lmax / p->lmin_div;
     const double dtau = 2.0 / sqrt(lmax) * p->CFL;
     double c = 2.0 * sqrt(lmin) * p->cfact;
     double h1 = (2 - c * dtau) / (2 + c * dtau), h2 = 2 * dtau / (2 + c * dtau);
 
-    HIP_TRY(hipMemsetAsync(f0, 0, N * sizeof(double), cx.stream));   // ResetA!(Fc, Fc0)
-    HIP_TRY(hipMemsetAsync(fst, 0, N * sizeof(double), cx.stream));
-
-    a.R = f->R; a.kx = f->kx; a.ky = f->ky; a.kz = f->kz;
+    a.R = f->R; a.kx = f->kx; a.ky = f->ky; a.kz = f->kz; a.Dst = (const double *)dst;
     a.F0 = (double *)f0; a.Fst = (double *)fst; a.partials = (double *)part;
     a._dx = 1.0 / f->dx; a._dy = 1.0 / f->dy; a._dz = 1.0 / f->dz; a._dt = 1.0 / f->dt;
     a.dx = f->dx; a.dy = f->dy; a.dz = f->dz; a.rc_tr_dt = rc_tr_dt;
-    a.dtau = dtau;
+    a.dtau = dtau; a.h1 = h1; a.h2 = h2;
     for (int q = 0; q < 6; ++q) { a.bct[q] = f->bc_type[q]; a.bcv[q] = f->bc_val[q]; }
+    a.Xin = f->X;
+    a.Xout = (double *)xb;
+    IterCfg cfg;
+    if ((rc = choose_cfg(a, f->transient != 0, true, 0, &cfg))) return rc;
+    const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
+
+    HIP_TRY(hipMemsetAsync(f0, 0, N * sizeof(double), cx.stream));   // ResetA!(Fc, Fc0)
+    HIP_TRY(hipMemsetAsync(fst, 0, N * sizeof(double), cx.stream));
 
     double *buf[2] = {f->X, (double *)xb};
     int cur = 0;
     SegmentTimer timer;
+    last_cfg() = cfg;
     const double bytes_per_iter = 8.0 * (double)N * sizeof(double);
     double nF_abs = 0.0, nF_rel = 0.0, nF_ini = 0.0;
     int status = HT3D_NOT_CONVERGED, iter, n_checks = 0, n_lambda = 0, done_iters = 0;
@@ -305,7 +482,7 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
         a.h1 = h1; a.h2 = h2;
         a.flags = (chk ? FL_CHECK : 0u) | (lam ? FL_LAMBDA : 0u) | (store ? FL_STORE_F : 0u);
         timer.before_launch();
-        rc = f->transient ? launch_iter<double, true>(a) : launch_iter<double, false>(a);
+        rc = launch_iter(a, f->transient != 0, cfg);
         if (rc) return rc;
         ++timer.count;
         cur ^= 1;
@@ -376,6 +553,34 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
 
 }  // namespace ht3d
 
-extern "C" int ht3d_dyrel_solve(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep) {
+extern "C" {
+
+int ht3d_dyrel_solve(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep) {
     return ht3d::dyrel_solve_impl(f, p, rep, nullptr, 1.0);
 }
+
+int ht3d_tune_set(int autotune, int force_tile, int force_variant) {
+    if (force_tile >= ht3d::kNumTiles || force_variant < 0 || force_variant > 3)
+        return ht3d::arg_fail("ht3d_tune_set: no such tile configuration or variant");
+    ht3d::g_autotune = autotune != 0;
+    ht3d::g_force_tile = force_tile;
+    ht3d::g_force_variant = force_variant;
+    return HT3D_OK;
+}
+
+int ht3d_tune_clear(void) {
+    ht3d::g_tune.clear();
+    return HT3D_OK;
+}
+
+int ht3d_tune_last(int *rows_per_block, int *z_per_thread, int *variant) {
+    const ht3d::IterCfg &c = ht3d::last_cfg();
+    if (rows_per_block) *rows_per_block = ht3d::kTiles[c.tile].ty;
+    if (z_per_thread) *z_per_thread = ht3d::kTiles[c.tile].kz;
+    if (variant) *variant = c.variant;
+    return HT3D_OK;
+}
+
+int ht3d_tile_count(void) { return ht3d::kNumTiles; }
+
+}  // extern "C"

@@ -31,7 +31,7 @@ int arg_fail(const char *what) {
 
 int workspace(int slot, size_t bytes, void **out) {
     Context &c = g_ctx;
-    if (slot < 0 || slot >= 8) return arg_fail("workspace slot");
+    if (slot < 0 || slot >= WS_COUNT) return arg_fail("workspace slot");
     if (bytes == 0) bytes = 256;
     if (c.ws_bytes[slot] < bytes) {
         if (c.ws[slot]) {
@@ -200,7 +200,7 @@ int ht3d_finalize(void) {
     Context &c = ctx();
     if (!c.initialised) return HT3D_OK;
     (void)hipStreamSynchronize(c.stream);
-    for (int s = 0; s < 8; ++s) {
+    for (int s = 0; s < WS_COUNT; ++s) {
         if (c.ws[s]) (void)hipFree(c.ws[s]);
         c.ws[s] = nullptr;
         c.ws_bytes[s] = 0;

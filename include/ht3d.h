@@ -82,6 +82,20 @@ int ht3d_timing_enable(int on);
 int ht3d_timing_get(ht3d_timing_stats *out);
 int ht3d_timing_reset(void);
 
+/* Tile autotuner of the fused iteration kernel.  A configuration is (rows of 64
+ * cells per block, z cells marched per thread) from a fixed table plus a
+ * variant: bit 0 reads the stored Gershgorin diagonal instead of recomputing it
+ * from the face coefficients (bit-identical either way), bit 1 writes the
+ * updated unknown with a non-temporal store.  With autotuning on (default) the
+ * first solve of each (grid, operator, precision) times every configuration on
+ * its own inputs and caches the fastest; force_tile >= 0 pins a table entry and
+ * variant.  No reference counterpart (ParallelStencil takes explicit
+ * cublocks/cuthreads, HydroThermal3D_v6.jl:264-265).                        */
+int ht3d_tune_set(int autotune, int force_tile, int force_variant);
+int ht3d_tune_clear(void);
+int ht3d_tune_last(int *rows_per_block, int *z_per_thread, int *variant);
+int ht3d_tile_count(void);
+
 /* maximum_g / minimum_g / sum_g / mean_g of abs.(A) (tools/Macros.jl:20-25);
  * the result lands in host memory.  Deterministic order.                    */
 enum { HT3D_SUM = 0, HT3D_SUM_ABS = 1, HT3D_MAX = 2, HT3D_MIN = 3, HT3D_MAX_ABS = 4 };
