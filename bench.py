@@ -55,6 +55,11 @@ def timing_stats():
     return {n: getattr(s, n) for n, _ in Stats._fields_}
 
 
+def rel(a, b):
+    import numpy as np
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
 def run_gpu(w, steps: int, warmup: int, precision: int, prm):
     import hydrothermal3d_amd as ht
     from hydrothermal3d_amd import _lib, inputs
@@ -82,8 +87,10 @@ def run_gpu(w, steps: int, warmup: int, precision: int, prm):
     _lib.call("ht3d_timing_enable", 0)
     st = timing_stats()
     kfv = g["kfv"].get()
+    I = (slice(1, -1),) * 3
+    fields = {"T": g["Tc_ex"].get()[I], "P": g["Pc_ex"].get()[I]}
     g.close()
-    return wall, st, reports, kfv
+    return wall, st, reports, kfv, fields
 
 
 def cpu_baseline(w, iters: int, kfv=None):
@@ -123,7 +130,9 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="hrl256")
-    ap.add_argument("--precision", choices=["fp64", "dc"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "dc", "both"], default="both",
+                    help="fp64: plain FP64 path; dc: FP32 inner sweeps + FP64 defect correction; both: run the two "
+                         "on the same inputs and report the DC speed-up and its error against FP64")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--cpu-iters", type=int, default=600, help="pseudo-iterations of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -139,10 +148,26 @@ def main(argv=None) -> int:
 
     w = inputs.WORKLOADS[args.workload]
     prm = ht.production_params(tol=args.tol)
-    precision = _lib.FP64 if args.precision == "fp64" else _lib.FP32_DC
+    precision = _lib.FP32_DC if args.precision == "dc" else _lib.FP64
     log(f"device: {_lib.device_name()}; workload {w.name}; precision {args.precision}")
-    wall, st, reports, kfv = run_gpu(w, args.steps, args.warmup, precision, prm)
+    wall, st, reports, kfv, fields = run_gpu(w, args.steps, args.warmup, precision, prm)
     ms_step = wall / args.steps * 1e3
+    dc = None
+    if args.precision == "both":
+        log("FP32 inner sweeps + FP64 defect correction on the same inputs")
+        wall_dc, st_dc, rep_dc, _, f_dc = run_gpu(w, args.steps, args.warmup, _lib.FP32_DC, prm)
+        ms_dc = wall_dc / args.steps * 1e3
+        import numpy as np
+        Pg, Pd = fields["P"] - fields["P"].mean(), f_dc["P"] - f_dc["P"].mean()
+        nu = np.array([r["Nu"] for r in reports])
+        nu_dc = np.array([r["Nu"] for r in rep_dc])
+        dc = {"ms_per_step": round(ms_dc, 3), "speedup_vs_fp64": round(ms_step / ms_dc, 3),
+              "rel_err_T": rel(f_dc["T"], fields["T"]), "rel_err_P": rel(Pd, Pg),
+              "rel_err_Nu": float(np.max(np.abs(nu_dc - nu) / np.abs(nu))),
+              "t_eff_fp32_inner": round(st_dc["bytes_fp32"] / (st_dc["ms_fp32"] / 1e3) / 1e9, 2)
+              if st_dc["ms_fp32"] > 0 else None,
+              "inner_iters_per_step": [(r["inner_thermal"], r["inner_darcy"]) for r in rep_dc],
+              "outer_iters_per_step": [(r["dc_outer_thermal"], r["dc_outer_darcy"]) for r in rep_dc]}
     teff = st["bytes"] / (st["ms"] / 1e3) / 1e9 if st["ms"] > 0 else 0.0
     teff32 = st["bytes_fp32"] / (st["ms_fp32"] / 1e3) / 1e9 if st["ms_fp32"] > 0 else None
     avg_us = st["ms"] / max(st["launches"], 1) * 1e3
@@ -169,7 +194,7 @@ def main(argv=None) -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if precision == _lib.FP64 else "f32+f64dc",
+        "dtype": "f64" if args.precision != "dc" else "f32+f64dc",
         "data": "synthetic",
         "config": {"workload": w.name, "ncx": w.ncx, "ncy": w.ncy, "ncz": w.ncz, "Ra": w.Ra,
                    "precision": args.precision, "tol": args.tol, "norm": "max|F*D|",
@@ -184,6 +209,8 @@ def main(argv=None) -> int:
     }
     if teff32 is not None:
         out["t_eff_fp32_inner"] = round(teff32, 2)
+    if dc is not None:
+        out["fp32_dc"] = dc
     print(json.dumps(out), flush=True)
     return 0
 

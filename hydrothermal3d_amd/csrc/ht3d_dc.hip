@@ -1,13 +1,312 @@
-// ht3d_dc.hip -- mixed-precision solve: FP32 DYREL inner sweeps inside an FP64
+// ht3d_dc.hip -- mixed-precision DYREL solve: FP32 inner sweeps inside an FP64
 // defect-correction loop.
-#include "ht3d_internal.hpp"
+//
+// The residual of the dimensionless operators is affine in the unknown:
+// r(x) = R [+ x/dt] + div q(x).  Each outer step evaluates r(x) in FP64 (fused:
+// ghost values, fluxes and the residual in one kernel), runs DYREL
+// pseudo-iterations in FP32 on the correction equation L e = -r (same operator
+// with homogeneous boundary values, FP32 face coefficients), and accumulates
+// x += e in FP64.  Norms and the convergence test are FP64 and use the same
+// norm and tolerance as the FP64 solve; lambda_max is the FP64 Gershgorin bound
+// and lambda_min follows the same Rayleigh-quotient schedule inside each inner
+// solve.  The reference has no mixed-precision path (it is FP64 only,
+// HydroThermal3D_v7.jl:13); the FP32 sweeps move half the bytes per iteration.
+#include <cmath>
+#include <cstring>
+
+#include "ht3d_iter.hpp"
 
 namespace ht3d {
-int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep, double *nu_out,
-                  int *outer, int *inner) {
-    (void)f; (void)p; (void)rep; (void)nu_out; (void)outer; (void)inner;
-    return arg_fail("FP32 defect-correction solve not built yet");
+
+int gershgorin_lmax(const double *kx, const double *ky, const double *kz, int nx, int ny, int nz, double dx,
+                    double dy, double dz, double rc_tr_dt, double *lmax, double *Dout);
+int bc_fill_table(double *A, int nx, int ny, int nz, const int type[6], const double val[6]);
+int nu_sum(const double *T, const double *ky, int nx, int ny, int nz, double _dy, double *sum);
+
+struct DcArgs {
+    const double *__restrict__ X;   // E
+    const double *__restrict__ R;   // c
+    const double *__restrict__ kx;
+    const double *__restrict__ ky;
+    const double *__restrict__ kz;
+    float *__restrict__ r32;        // c: FP32 right-hand side of the correction equation
+    double *__restrict__ partials;
+    int nx, ny, nz;
+    double _dx, _dy, _dz, _dt;
+    int bct[6];
+    double bcv[6];
+};
+
+// FP64 residual r(x) with on-the-fly ghosts; slots: 0 sum|r| (dd), 1 max|r|
+template <bool TR>
+__global__ __launch_bounds__(256) void dc_residual_kernel(DcArgs a) {
+    const int i = blockIdx.x * 64 + (int)(threadIdx.x & 63u);
+    const int j = blockIdx.y * 4 + (int)(threadIdx.x >> 6);
+    const int k = blockIdx.z;
+    double s_hi = 0.0, s_lo = 0.0, mx = 0.0;
+    const uint32_t nx = a.nx, ny = a.ny;
+    if ((uint32_t)i < nx && (uint32_t)j < ny) {
+        const uint32_t sE1 = nx + 2, sE2 = sE1 * (ny + 2);
+        const uint32_t e = (uint32_t)(i + 1) + (uint32_t)(j + 1) * sE1 + (uint32_t)(k + 1) * sE2;
+        const uint32_t c = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k * nx * ny;
+        const uint32_t fx = (uint32_t)i + (uint32_t)j * (nx + 1) + (uint32_t)k * (nx + 1) * ny;
+        const uint32_t fy = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k * nx * (ny + 1);
+        const double xc = a.X[e];
+        double xW = a.X[e - 1], xE = a.X[e + 1], xS = a.X[e - sE1], xN = a.X[e + sE1], xB = a.X[e - sE2],
+               xF = a.X[e + sE2];
+        if (i == 0) xW = ghost(a.bct[0], a.bcv[0], xc);
+        if ((uint32_t)i == nx - 1) xE = ghost(a.bct[1], a.bcv[1], xc);
+        if (j == 0) xS = ghost(a.bct[2], a.bcv[2], xc);
+        if ((uint32_t)j == ny - 1) xN = ghost(a.bct[3], a.bcv[3], xc);
+        if (k == 0) xB = ghost(a.bct[4], a.bcv[4], xc);
+        if (k == a.nz - 1) xF = ghost(a.bct[5], a.bcv[5], xc);
+        const double kW = a.kx[fx], kE = a.kx[fx + 1], kS = a.ky[fy], kN = a.ky[fy + nx], kB = a.kz[c],
+                     kF = a.kz[c + nx * ny];
+        const double qW = -kW * (a._dx * (xc - xW)), qE = -kE * (a._dx * (xE - xc));
+        const double qS = -kS * (a._dy * (xc - xS)), qN = -kN * (a._dy * (xN - xc));
+        const double qB = -kB * (a._dz * (xc - xB)), qF = -kF * (a._dz * (xF - xc));
+        double f;
+        if (TR) f = a.R[c] + a._dt * xc + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
+        else f = a.R[c] + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
+        a.r32[c] = (float)f;
+        const double af = fabs(f);
+        dd_add(s_hi, s_lo, af);
+        mx = (af > mx || af != af) ? af : mx;
+    }
+    const double h4[4] = {s_hi, mx, 0.0, 0.0}, l4[4] = {s_lo, 0.0, 0.0, 0.0};
+    const unsigned bl = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    block_reduce4(h4, l4, 0x2u, a.partials, bl);
 }
+
+// x += e on the interior (E arrays, FP64 += FP32)
+__global__ __launch_bounds__(256) void dc_update_kernel(double *__restrict__ X, const float *__restrict__ e, int nx,
+                                                        int ny, int nz) {
+    const int i = blockIdx.x * 64 + (int)(threadIdx.x & 63u);
+    const int j = blockIdx.y * 4 + (int)(threadIdx.x >> 6);
+    const int k = blockIdx.z;
+    if (i >= nx || j >= ny) return;
+    const size_t idx = (size_t)(i + 1) + (size_t)(j + 1) * (nx + 2) + (size_t)(k + 1) * (nx + 2) * (ny + 2);
+    X[idx] += (double)e[idx];
+}
+
+__global__ __launch_bounds__(256) void to_float_kernel(float *__restrict__ out, const double *__restrict__ in, size_t n) {
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        out[t] = (float)in[t];
+}
+
+static int to_float(float *out, const double *in, size_t n) {
+    unsigned nb = div_up(n, 256);
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(to_float_kernel, dim3(nb), dim3(256), 0, ctx().stream, out, in, n);
+    HIP_TRY(hipGetLastError());
+    return HT3D_OK;
+}
+
+// FP32 inner sweeps use the recomputed-diagonal variant only
+static int launch_iter32(const IterArgs<float> &a, bool transient, int tile) {
+    const unsigned nb = (unsigned)a.nbx * a.nby * a.nbz;
+    if (transient) launch_tile<float, true, false, false>(tile, a, nb, ctx().stream);
+    else launch_tile<float, false, false, false>(tile, a, nb, ctx().stream);
+    HIP_TRY(hipGetLastError());
+    return HT3D_OK;
+}
+
+static int tune32(IterArgs<float> a, bool transient, int *tile_out) {
+    const TuneKey key{a.nx, a.ny, a.nz, transient ? 1 : 0, 1};
+    if (const TuneEntry *t = tune_find(key)) {
+        *tile_out = t->cfg.tile;
+        return HT3D_OK;
+    }
+    if (g_force_tile >= 0 || !g_autotune) {
+        *tile_out = g_force_tile >= 0 ? g_force_tile : kDefaultTile;
+        return HT3D_OK;
+    }
+    a.flags = 0;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int best = kDefaultTile;
+    double best_us = 1e300;
+    for (int tile = 0; tile < kNumTiles; ++tile) {
+        set_tiling(a, tile);
+        int rc = launch_iter32(a, transient, tile);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(e0, ctx().stream));
+        for (int r = 0; r < 8; ++r)
+            if ((rc = launch_iter32(a, transient, tile))) return rc;
+        HIP_TRY(hipEventRecord(e1, ctx().stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_us) {
+            best_us = ms;
+            best = tile;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g_tune.push_back(TuneEntry{key, IterCfg{best, 0}, 1e3 * best_us / 8});
+    *tile_out = best;
+    return HT3D_OK;
+}
+
+static inline bool lam_iter(int it, int nitout) { return it <= 2 || (nitout > 0 && it % nitout == 0); }
+
+int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep, double *nu_out,
+                  int *outer_out, int *inner_out) {
+    HT3D_CHECK_INIT();
+    if (!f || !p) return arg_fail("ht3d_dyrel_solve_dc: null argument");
+    const int nx = f->ncx, ny = f->ncy, nz = f->ncz;
+    if (!grid_ok(nx, ny, nz)) return arg_fail("ht3d_dyrel_solve_dc: extents must be >= 3");
+    if (!f->X || !f->R || !f->kx || !f->ky || !f->kz) return arg_fail("ht3d_dyrel_solve_dc: null field pointer");
+    if (p->nitmax < 1 || p->nitout < 1 || !(p->tol >= 0.0)) return arg_fail("ht3d_dyrel_solve_dc: bad parameters");
+    Context &cx = ctx();
+    const size_t N = (size_t)nx * ny * nz, NE = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+    const size_t nfx = (size_t)(nx + 1) * ny * nz, nfy = (size_t)nx * (ny + 1) * nz, nfz = (size_t)nx * ny * (nz + 1);
+    int rc;
+    void *e0 = nullptr, *e1 = nullptr, *f0 = nullptr, *r32 = nullptr, *k32 = nullptr, *fst = nullptr, *part = nullptr,
+         *res = nullptr;
+    if ((rc = workspace(WS_F32_X, NE * sizeof(float), &e0))) return rc;
+    if ((rc = workspace(WS_F32_XB, NE * sizeof(float), &e1))) return rc;
+    if ((rc = workspace(WS_F32_F0, N * sizeof(float), &f0))) return rc;
+    if ((rc = workspace(WS_F32_R, N * sizeof(float), &r32))) return rc;
+    if ((rc = workspace(WS_F32_K, (nfx + nfy + nfz) * sizeof(float), &k32))) return rc;
+    if ((rc = workspace(WS_DC_RES, N * sizeof(float), &fst))) return rc;
+    const unsigned max_blocks = div_up(nx, 64) * div_up(ny, 1) * div_up(nz, 2);
+    const unsigned res_blocks = div_up(nx, 64) * div_up(ny, 4) * nz;
+    if ((rc = workspace(WS_PARTIALS, (size_t)(max_blocks > res_blocks ? max_blocks : res_blocks) * 8 * sizeof(double),
+                        &part)))
+        return rc;
+    if ((rc = workspace(WS_RESULT, 64 * sizeof(double), &res))) return rc;
+    float *kx32 = (float *)k32, *ky32 = kx32 + nfx, *kz32 = ky32 + nfy;
+    if ((rc = to_float(kx32, f->kx, nfx)) || (rc = to_float(ky32, f->ky, nfy)) || (rc = to_float(kz32, f->kz, nfz)))
+        return rc;
+
+    // lambda_max and the DYREL scalars (HydroThermal3D_v18.jl:432-439)
+    const double rc_tr_dt = f->rhoC * (double)f->transient / f->dt;
+    double lmax = 0.0;
+    if ((rc = gershgorin_lmax(f->kx, f->ky, f->kz, nx, ny, nz, f->dx, f->dy, f->dz, rc_tr_dt, &lmax, nullptr)))
+        return rc;
+    double lmin = lmax / p->lmin_div;
+    const double dtau = 2.0 / sqrt(lmax) * p->CFL;
+
+    DcArgs d;
+    std::memset(&d, 0, sizeof d);
+    d.X = f->X; d.R = f->R; d.kx = f->kx; d.ky = f->ky; d.kz = f->kz; d.r32 = (float *)r32;
+    d.partials = (double *)part; d.nx = nx; d.ny = ny; d.nz = nz;
+    d._dx = 1.0 / f->dx; d._dy = 1.0 / f->dy; d._dz = 1.0 / f->dz; d._dt = 1.0 / f->dt;
+    for (int q = 0; q < 6; ++q) { d.bct[q] = f->bc_type[q]; d.bcv[q] = f->bc_val[q]; }
+    const dim3 rgrid(div_up(nx, 64), div_up(ny, 4), nz);
+
+    IterArgs<float> a;
+    std::memset(&a, 0, sizeof a);
+    a.nx = nx; a.ny = ny; a.nz = nz;
+    a.R = (const float *)r32; a.kx = kx32; a.ky = ky32; a.kz = kz32; a.Dst = nullptr;
+    a.F0 = (float *)f0; a.Fst = (float *)fst; a.partials = (double *)part;
+    a._dx = (float)d._dx; a._dy = (float)d._dy; a._dz = (float)d._dz; a._dt = (float)d._dt;
+    a.dx = (float)f->dx; a.dy = (float)f->dy; a.dz = (float)f->dz; a.rc_tr_dt = (float)rc_tr_dt;
+    a.dtau = (float)dtau;
+    for (int q = 0; q < 6; ++q) { a.bct[q] = f->bc_type[q]; a.bcv[q] = 0.0f; }   // homogeneous for the correction
+    a.Xin = (const float *)e0; a.Xout = (float *)e1;
+    int tile = kDefaultTile;
+    if ((rc = tune32(a, f->transient != 0, &tile))) return rc;
+    set_tiling(a, tile);
+    last_cfg() = IterCfg{tile, 0};
+
+    const double inner_tol = 1e-5;
+    const int max_outer = 60;
+    double nF_abs = 0.0, nF_rel = 0.0, nF_ini = 0.0;
+    int status = HT3D_NOT_CONVERGED, outer = 0, inner_total = 0, n_checks = 0, n_lambda = 0;
+    SegmentTimer timer;
+    const double bytes32 = 8.0 * (double)N * sizeof(float);
+    for (outer = 0; outer < max_outer; ++outer) {
+        // outer FP64 residual and norm
+        if (f->transient) hipLaunchKernelGGL(dc_residual_kernel<true>, rgrid, dim3(256), 0, cx.stream, d);
+        else hipLaunchKernelGGL(dc_residual_kernel<false>, rgrid, dim3(256), 0, cx.stream, d);
+        HIP_TRY(hipGetLastError());
+        if ((rc = finish_partials((double *)part, rgrid.x * rgrid.y * rgrid.z, 0x2u, (double *)res))) return rc;
+        double r4[4];
+        if ((rc = read_result4((double *)res, r4))) return rc;
+        ++n_checks;
+        nF_abs = p->norm == 0 ? r4[0] / (double)N / sqrt((double)N) : r4[1];
+        if (r4[0] != r4[0]) nF_abs = r4[0];
+        if (outer == 0) nF_ini = nF_abs;
+        nF_rel = nF_abs / nF_ini;
+        if (std::isnan(nF_abs)) { status = HT3D_NAN; break; }
+        if (nF_rel > 100) { status = HT3D_DIVERGED; break; }
+        if (nF_abs < p->tol || nF_rel < p->tol) { status = HT3D_OK; break; }
+        if (inner_total >= p->nitmax) break;
+        // inner FP32 DYREL on L e = -r, from e = 0
+        HIP_TRY(hipMemsetAsync(e0, 0, NE * sizeof(float), cx.stream));
+        HIP_TRY(hipMemsetAsync(e1, 0, NE * sizeof(float), cx.stream));
+        HIP_TRY(hipMemsetAsync(f0, 0, N * sizeof(float), cx.stream));
+        HIP_TRY(hipMemsetAsync(fst, 0, N * sizeof(float), cx.stream));
+        double c = 2.0 * sqrt(lmin) * p->cfact;
+        double h1 = (2 - c * dtau) / (2 + c * dtau), h2 = 2 * dtau / (2 + c * dtau);
+        float *buf[2] = {(float *)e0, (float *)e1};
+        int cur = 0;
+        double in_ini = 0.0;
+        const int budget = p->nitmax - inner_total;
+        int it;
+        for (it = 1; it <= budget; ++it) {
+            const bool lam = lam_iter(it, p->nitout);
+            const bool chk = lam || (p->ncheck > 0 && it % p->ncheck == 0);
+            const bool store = it < budget && lam_iter(it + 1, p->nitout);
+            a.Xin = buf[cur];
+            a.Xout = buf[cur ^ 1];
+            a.h1 = (float)h1; a.h2 = (float)h2;
+            a.flags = (chk ? FL_CHECK : 0u) | (lam ? FL_LAMBDA : 0u) | (store ? FL_STORE_F : 0u);
+            timer.before_launch();
+            if ((rc = launch_iter32(a, f->transient != 0, tile))) return rc;
+            ++timer.count;
+            cur ^= 1;
+            if (chk || it == budget) timer.close();
+            if (chk) {
+                if ((rc = finish_partials((double *)part, (unsigned)a.nbx * a.nby * a.nbz, 0x2u, (double *)res))) return rc;
+                if ((rc = read_result4((double *)res, r4))) return rc;
+                timer.collect(bytes32, true);
+                const double nrm = p->norm == 0 ? r4[0] / (double)N / sqrt((double)N) : r4[1];
+                if (it == 1) in_ini = nrm;
+                if (!(nrm == nrm)) { status = HT3D_NAN; break; }
+                if (lam && it > 1 && r4[3] > 0.0) {
+                    lmin = fabs(r4[2]) / fabs(r4[3]);
+                    c = 2.0 * sqrt(lmin) * p->cfact;
+                    h1 = (2 - c * dtau) / (2 + c * dtau);
+                    h2 = 2 * dtau / (2 + c * dtau);
+                    ++n_lambda;
+                }
+                if (nrm <= inner_tol * in_ini) { ++it; break; }
+            }
+        }
+        timer.close();
+        timer.collect(bytes32, true);
+        inner_total += it - 1;
+        if (status == HT3D_NAN) break;
+        // x += e (the correction is in buf[cur])
+        hipLaunchKernelGGL(dc_update_kernel, rgrid, dim3(256), 0, cx.stream, f->X, buf[cur], nx, ny, nz);
+        HIP_TRY(hipGetLastError());
+    }
+    // ghosts of the result from its own interior; Nu from the converged state
+    if ((rc = bc_fill_table(f->X, nx, ny, nz, f->bc_type, f->bc_val))) return rc;
+    if (nu_out) {
+        if ((rc = nu_sum(f->X, f->ky, nx, ny, nz, 1.0 / f->dy, nu_out))) return rc;
+    }
+    if (rep) {
+        rep->iters = inner_total;
+        rep->status = status;
+        rep->n_checks = n_checks;
+        rep->n_lambda = n_lambda;
+        rep->lmax = lmax; rep->lmin = lmin; rep->dtau = dtau;
+        rep->h1 = 0.0; rep->h2 = 0.0;
+        rep->nF_abs = nF_abs; rep->nF_rel = nF_rel; rep->nF_ini = nF_ini;
+    }
+    if (outer_out) *outer_out = outer;
+    if (inner_out) *inner_out = inner_total;
+    rc = after_launch("ht3d_dyrel_solve_dc", true);
+    if (rc) return rc;
+    return status;
+}
+
 }  // namespace ht3d
 
 extern "C" int ht3d_dyrel_solve_dc(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep,
