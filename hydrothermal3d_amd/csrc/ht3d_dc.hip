@@ -213,14 +213,28 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     set_tiling(a, tile);
     last_cfg() = IterCfg{tile, 0};
 
-    const double inner_tol = 1e-5;
-    const int max_outer = 60;
+    // One DYREL trajectory in FP32 with FP64 re-anchoring.  The unknown is
+    // x_anchor (FP64) + e (FP32); every `anchor` sweeps x_anchor += e, e = 0 and
+    // the FP32 right-hand side is refreshed with the FP64 residual r(x_anchor),
+    // whose norm is the convergence test.  F0 (the DYREL momentum), lambda_min
+    // and the lambda schedule run on uninterrupted, so the iteration follows the
+    // FP64 solve up to FP32 rounding, and re-anchoring keeps e small enough that
+    // FP32 never has to resolve more than the residual drop between anchors.
+    const int anchor = p->ncheck > 0 ? p->ncheck : (p->nitout > 0 ? p->nitout : 50);
     double nF_abs = 0.0, nF_rel = 0.0, nF_ini = 0.0;
     int status = HT3D_NOT_CONVERGED, outer = 0, inner_total = 0, n_checks = 0, n_lambda = 0;
     SegmentTimer timer;
     const double bytes32 = 8.0 * (double)N * sizeof(float);
-    for (outer = 0; outer < max_outer; ++outer) {
-        // outer FP64 residual and norm
+    HIP_TRY(hipMemsetAsync(e0, 0, NE * sizeof(float), cx.stream));
+    HIP_TRY(hipMemsetAsync(e1, 0, NE * sizeof(float), cx.stream));
+    HIP_TRY(hipMemsetAsync(f0, 0, N * sizeof(float), cx.stream));
+    HIP_TRY(hipMemsetAsync(fst, 0, N * sizeof(float), cx.stream));
+    double c = 2.0 * sqrt(lmin) * p->cfact;
+    double h1 = (2 - c * dtau) / (2 + c * dtau), h2 = 2 * dtau / (2 + c * dtau);
+    float *buf[2] = {(float *)e0, (float *)e1};
+    int cur = 0;
+    for (outer = 0;; ++outer) {
+        // anchor: FP64 residual, FP32 right-hand side, FP64 norm
         if (f->transient) hipLaunchKernelGGL(dc_residual_kernel<true>, rgrid, dim3(256), 0, cx.stream, d);
         else hipLaunchKernelGGL(dc_residual_kernel<false>, rgrid, dim3(256), 0, cx.stream, d);
         HIP_TRY(hipGetLastError());
@@ -236,55 +250,40 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
         if (nF_rel > 100) { status = HT3D_DIVERGED; break; }
         if (nF_abs < p->tol || nF_rel < p->tol) { status = HT3D_OK; break; }
         if (inner_total >= p->nitmax) break;
-        // inner FP32 DYREL on L e = -r, from e = 0
-        HIP_TRY(hipMemsetAsync(e0, 0, NE * sizeof(float), cx.stream));
-        HIP_TRY(hipMemsetAsync(e1, 0, NE * sizeof(float), cx.stream));
-        HIP_TRY(hipMemsetAsync(f0, 0, N * sizeof(float), cx.stream));
-        HIP_TRY(hipMemsetAsync(fst, 0, N * sizeof(float), cx.stream));
-        double c = 2.0 * sqrt(lmin) * p->cfact;
-        double h1 = (2 - c * dtau) / (2 + c * dtau), h2 = 2 * dtau / (2 + c * dtau);
-        float *buf[2] = {(float *)e0, (float *)e1};
-        int cur = 0;
-        double in_ini = 0.0;
-        const int budget = p->nitmax - inner_total;
-        int it;
-        for (it = 1; it <= budget; ++it) {
+        // `anchor` FP32 sweeps of the same DYREL iteration
+        const int run = (p->nitmax - inner_total) < anchor ? (p->nitmax - inner_total) : anchor;
+        for (int s = 0; s < run; ++s) {
+            const int it = inner_total + 1;
             const bool lam = lam_iter(it, p->nitout);
-            const bool chk = lam || (p->ncheck > 0 && it % p->ncheck == 0);
-            const bool store = it < budget && lam_iter(it + 1, p->nitout);
+            const bool store = it < p->nitmax && lam_iter(it + 1, p->nitout);
             a.Xin = buf[cur];
             a.Xout = buf[cur ^ 1];
             a.h1 = (float)h1; a.h2 = (float)h2;
-            a.flags = (chk ? FL_CHECK : 0u) | (lam ? FL_LAMBDA : 0u) | (store ? FL_STORE_F : 0u);
+            a.flags = (lam ? (FL_CHECK | FL_LAMBDA) : 0u) | (store ? FL_STORE_F : 0u);
             timer.before_launch();
             if ((rc = launch_iter32(a, f->transient != 0, tile))) return rc;
             ++timer.count;
             cur ^= 1;
-            if (chk || it == budget) timer.close();
-            if (chk) {
+            inner_total = it;
+            if (lam || s == run - 1) timer.close();
+            if (lam) {
                 if ((rc = finish_partials((double *)part, (unsigned)a.nbx * a.nby * a.nbz, 0x2u, (double *)res))) return rc;
                 if ((rc = read_result4((double *)res, r4))) return rc;
                 timer.collect(bytes32, true);
-                const double nrm = p->norm == 0 ? r4[0] / (double)N / sqrt((double)N) : r4[1];
-                if (it == 1) in_ini = nrm;
-                if (!(nrm == nrm)) { status = HT3D_NAN; break; }
-                if (lam && it > 1 && r4[3] > 0.0) {
+                if (it > 1 && r4[3] > 0.0) {  // Rayleigh quotient (HydroThermal3D_v18.jl:467-473)
                     lmin = fabs(r4[2]) / fabs(r4[3]);
                     c = 2.0 * sqrt(lmin) * p->cfact;
                     h1 = (2 - c * dtau) / (2 + c * dtau);
                     h2 = 2 * dtau / (2 + c * dtau);
                     ++n_lambda;
                 }
-                if (nrm <= inner_tol * in_ini) { ++it; break; }
             }
         }
-        timer.close();
         timer.collect(bytes32, true);
-        inner_total += it - 1;
-        if (status == HT3D_NAN) break;
-        // x += e (the correction is in buf[cur])
+        // x_anchor += e; the next sweeps start from e = 0
         hipLaunchKernelGGL(dc_update_kernel, rgrid, dim3(256), 0, cx.stream, f->X, buf[cur], nx, ny, nz);
         HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemsetAsync(buf[cur], 0, NE * sizeof(float), cx.stream));
     }
     // ghosts of the result from its own interior; Nu from the converged state
     if ((rc = bc_fill_table(f->X, nx, ny, nz, f->bc_type, f->bc_val))) return rc;
@@ -297,7 +296,7 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
         rep->n_checks = n_checks;
         rep->n_lambda = n_lambda;
         rep->lmax = lmax; rep->lmin = lmin; rep->dtau = dtau;
-        rep->h1 = 0.0; rep->h2 = 0.0;
+        rep->h1 = h1; rep->h2 = h2;
         rep->nF_abs = nF_abs; rep->nF_rel = nF_rel; rep->nF_ini = nF_ini;
     }
     if (outer_out) *outer_out = outer;

@@ -28,7 +28,7 @@ def test_dc_solve_matches_fp64(ht, grid, transient):
     X, R, k = _problem(grid, transient, 17)
     h = dict(dx=1.0 / grid[0], dy=0.5 / grid[1], dz=1.0 / grid[2], dt=1e-3)
     bc = dict(bc_type=(0, 0, 1, 1, 0, 0), bc_val=(0.0, 0.0, 1.0, 0.0, 0.0, 0.0)) if transient else {}
-    prm = ht.production_params(tol=1e-10, nitout=100, ncheck=25)
+    prm = ht.production_params(tol=1e-11, nitout=100, ncheck=25)
     out = {}
     for prec in (0, 1):
         D = [ht.DeviceArray.from_host(a) for a in (X, R, *k)]
@@ -49,7 +49,7 @@ def test_hrl_steps_dc_matches_fp64(ht):
         m = ht.HydroThermal3D(w.ncx, w.ncy, w.ncz, w.Lx, w.Ly, w.Lz, w.Ra, w.cfl)
         m["Tc_ex"].set(inputs.initial_temperature(w))
         inputs.permeability_device(w, m["kfv"])
-        p = ht.production_params(tol=1e-10)
+        p = ht.production_params(tol=1e-11)
         m.set_params(p, p, 1, prec)
         nus = [m.step()["Nu"] for _ in range(3)]
         res[prec] = (m["Tc_ex"].get()[1:-1, 1:-1, 1:-1], m["Pc_ex"].get()[1:-1, 1:-1, 1:-1], np.array(nus))
