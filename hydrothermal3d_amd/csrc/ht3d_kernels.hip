@@ -524,6 +524,30 @@ int ht3d_Init_vel(double *Vx, double *Vy, double *Vz, const double *kx, const do
     });
 }
 
+// Temperature-dependent viscosity mu = exp(-a T): face coefficients become the
+// mobility k/mu(T_face) = k exp(a T_face) with T_face = (T_left + T_right) * 0.5
+// (ghost values at the walls, i.e. the boundary condition).  The dimensionless
+// kernels of the reference have constant viscosity; this is the dimensionless
+// analogue of the T-dependent hydraulic conductivity of
+// Dimensional/kernels_HT3D_dimensional.jl (ComputeHydroConductivity), applied
+// once per Darcy solve after InitDarcy!.
+int ht3d_ApplyViscosity(double *kx, double *ky, double *kz, const double *Tc_ex, double a, int nx, int ny, int nz) {
+    CHECK_GRID(nx, ny, nz);
+    CHECK_PTR(kx); CHECK_PTR(ky); CHECK_PTR(kz); CHECK_PTR(Tc_ex);
+    Grid g{nx, ny, nz};
+    int rc = launch3d("ApplyViscosity", nx + 1, ny, nz, [=] __device__(int i, int j, int k) {
+        kx[g.fx(i, j, k)] = kx[g.fx(i, j, k)] * exp(a * ((Tc_ex[g.e(i, j + 1, k + 1)] + Tc_ex[g.e(i + 1, j + 1, k + 1)]) * 0.5));
+    });
+    if (rc) return rc;
+    rc = launch3d("ApplyViscosity", nx, ny + 1, nz, [=] __device__(int i, int j, int k) {
+        ky[g.fy(i, j, k)] = ky[g.fy(i, j, k)] * exp(a * ((Tc_ex[g.e(i + 1, j, k + 1)] + Tc_ex[g.e(i + 1, j + 1, k + 1)]) * 0.5));
+    });
+    if (rc) return rc;
+    return launch3d("ApplyViscosity", nx, ny, nz + 1, [=] __device__(int i, int j, int k) {
+        kz[g.fz(i, j, k)] = kz[g.fz(i, j, k)] * exp(a * ((Tc_ex[g.e(i + 1, j + 1, k)] + Tc_ex[g.e(i + 1, j + 1, k + 1)]) * 0.5));
+    });
+}
+
 // ------------------------------------------------------------------ WENO5 (v1 kernels)
 static int plus_minus(const char *name, double *Vm, double *Vp, const double *V, int axis, int nx, int ny, int nz) {
     Grid g{nx, ny, nz};

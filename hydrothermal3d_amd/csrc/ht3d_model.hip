@@ -180,7 +180,7 @@ using namespace ht3d;
 struct ht3d_hrl {
     int nx, ny, nz;
     double Lx, Ly, Lz, dx, dy, dz;
-    double Ra, Tbot, Ttop, cfl, dt, time, Nu;
+    double Ra, Tbot, Ttop, cfl, dt, time, Nu, mu_a;
     int weno_version, precision;
     ht3d_dyrel_params pT, pP;
     double *Tc_ex, *Pc_ex, *ktv, *kfv, *Ty, *kx, *ky, *kz, *ktx, *kty, *ktz, *Vx, *Vy, *Vz, *Rt, *Rp, *W;
@@ -314,6 +314,12 @@ int ht3d_hrl_set_params(ht3d_hrl *m, const ht3d_dyrel_params *pT, const ht3d_dyr
     return HT3D_OK;
 }
 
+int ht3d_hrl_set_viscosity(ht3d_hrl *m, double a) {
+    if (!m || !(a == a)) return arg_fail("ht3d_hrl_set_viscosity: bad argument");
+    m->mu_a = a;
+    return HT3D_OK;
+}
+
 int ht3d_hrl_set_dt(ht3d_hrl *m, double dt) {
     if (!m || !(dt > 0)) return arg_fail("ht3d_hrl_set_dt: bad argument");
     m->dt = dt;
@@ -354,6 +360,7 @@ int ht3d_hrl_step(ht3d_hrl *m, ht3d_step_report *rep) {
 
     // Darcy solve
     if ((rc = ht3d_InitDarcy(m->Ty, m->kx, m->ky, m->kz, m->Tc_ex, m->kfv, _dt, nx, ny, nz))) return rc;
+    if (m->mu_a != 0.0 && (rc = ht3d_ApplyViscosity(m->kx, m->ky, m->kz, m->Tc_ex, m->mu_a, nx, ny, nz))) return rc;
     if ((rc = ht3d_KillNorthSouthFluxes(m->Ty, nx, ny, nz))) return rc;
     if ((rc = ht3d_Compute_Rp(m->Rp, m->ky, m->Ty, m->Ra, _dy, nx, ny, nz))) return rc;
     f.X = m->Pc_ex; f.R = m->Rp; f.kx = m->kx; f.ky = m->ky; f.kz = m->kz;

@@ -94,6 +94,9 @@ InitDarcy = _make("InitDarcy", "ht3d_InitDarcy",
                   "Dimensionless/kernels_HT3D.jl:82-90")
 KillNorthSouthFluxes = _make("KillNorthSouthFluxes", "ht3d_KillNorthSouthFluxes", [("Ty", "Fy")],
                              "Dimensionless/HydroThermal3D_Alban_v6.jl:153-159")
+ApplyViscosity = _make("ApplyViscosity", "ht3d_ApplyViscosity",
+                       [("kx", "Fx"), ("ky", "Fy"), ("kz", "Fz"), ("Tc_ex", "E"), ("a", "d")],
+                       "mu = exp(-a T) face mobility (no dimensionless counterpart; cf. kernels_HT3D_dimensional.jl)")
 Compute_Rp = _make("Compute_Rp", "ht3d_Compute_Rp", [("Rp", "c"), ("ky", "Fy"), ("Ty", "Fy"), ("Ra", "d"), ("_dy", "d")],
                    "Dimensionless/kernels_HT3D.jl:92-97")
 

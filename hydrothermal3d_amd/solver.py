@@ -126,6 +126,10 @@ class HydroThermal3D:
     def set_dt(self, dt: float) -> None:
         _lib.call("ht3d_hrl_set_dt", self._h, C.c_double(dt))
 
+    def set_viscosity(self, a: float) -> None:
+        """mu = exp(-a T) in the Darcy solve (a = ln 10 for the 768^3-class workload)."""
+        _lib.call("ht3d_hrl_set_viscosity", self._h, C.c_double(a))
+
     def step(self) -> dict:
         rep = _lib.StepReport()
         st = _lib.lib().ht3d_hrl_step(self._h, C.byref(rep))

@@ -192,6 +192,12 @@ int ht3d_Init_vel(double *Vx, double *Vy, double *Vz, const double *kx, const do
                   const double *Pc_ex, const double *Ty, double Ra, double _dx, double _dy, double _dz, int ncx,
                   int ncy, int ncz);
 
+/* Temperature-dependent viscosity mu = exp(-a T): face coefficients *= exp(a T_face),
+ * T_face the mean of the two adjacent cells (ghosts at walls).  No counterpart in
+ * the dimensionless kernels (constant viscosity there); the dimensionless analogue
+ * of the T-dependent conductivities of kernels_HT3D_dimensional.jl.             */
+int ht3d_ApplyViscosity(double *kx, double *ky, double *kz, const double *Tc_ex, double a, int ncx, int ncy, int ncz);
+
 /* tools/Weno5_Routines.jl:5-42 */
 int ht3d_VxPlusMinus(double *Vxm, double *Vxp, const double *Vx, int ncx, int ncy, int ncz);
 int ht3d_VyPlusMinus(double *Vym, double *Vyp, const double *Vy, int ncx, int ncy, int ncz);
@@ -288,6 +294,8 @@ int ht3d_hrl_field(ht3d_hrl *m, const char *name, double **dptr, size_t *count);
 int ht3d_hrl_set_params(ht3d_hrl *m, const ht3d_dyrel_params *pT, const ht3d_dyrel_params *pP, int weno_version,
                         int precision);
 int ht3d_hrl_set_dt(ht3d_hrl *m, double dt);
+/* mu = exp(-a T) in the Darcy solve (a = 0: constant viscosity, the default) */
+int ht3d_hrl_set_viscosity(ht3d_hrl *m, double a);
 int ht3d_hrl_step(ht3d_hrl *m, ht3d_step_report *rep);
 
 /* Lognormal permeability on vertices (spectral randomisation) plus an optional

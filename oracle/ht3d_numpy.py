@@ -47,6 +47,13 @@ def InitDarcy(T, kfv):
     return Ty, av_yza(kfv), av_xza(kfv), av_xya(kfv)
 
 
+def ApplyViscosity(kx, ky, kz, T, a):
+    """mu = exp(-a T): face coefficients times exp(a (T_left + T_right) * 0.5)."""
+    return (kx * np.exp(a * ((T[:-1, I, I] + T[1:, I, I]) * 0.5)),
+            ky * np.exp(a * ((T[I, :-1, I] + T[I, 1:, I]) * 0.5)),
+            kz * np.exp(a * ((T[I, I, :-1] + T[I, I, 1:]) * 0.5)))
+
+
 def KillNorthSouthFluxes(Ty):
     """HydroThermal3D_Alban_v6.jl:153-159"""
     Ty = Ty.copy()

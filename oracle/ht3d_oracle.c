@@ -119,6 +119,18 @@ EXPORT void o_InitDarcy(double *Ty, double *kx, double *ky, double *kz, const do
     PFOR_K(nz + 1) FOR_JI(nx, ny) FZ(kz, i, j, k) = AV_XYA(kfv, nx + 1, ny + 1);
 }
 
+/* Temperature-dependent viscosity mu = exp(-a T): face coefficients times
+ * exp(a T_face), T_face = (T_left + T_right) * 0.5 with the wall ghosts.  (No
+ * counterpart in the dimensionless kernels, which have constant viscosity.)  */
+EXPORT void o_ApplyViscosity(double *kx, double *ky, double *kz, const double *Tc_ex, double a, int nx, int ny, int nz) {
+    PFOR_K(nz) FOR_JI(nx + 1, ny)
+        FX(kx, i, j, k) = FX(kx, i, j, k) * exp(a * ((Ee(Tc_ex, i, j + 1, k + 1) + Ee(Tc_ex, i + 1, j + 1, k + 1)) * 0.5));
+    PFOR_K(nz) FOR_JI(nx, ny + 1)
+        FY(ky, i, j, k) = FY(ky, i, j, k) * exp(a * ((Ee(Tc_ex, i + 1, j, k + 1) + Ee(Tc_ex, i + 1, j + 1, k + 1)) * 0.5));
+    PFOR_K(nz + 1) FOR_JI(nx, ny)
+        FZ(kz, i, j, k) = FZ(kz, i, j, k) * exp(a * ((Ee(Tc_ex, i + 1, j + 1, k) + Ee(Tc_ex, i + 1, j + 1, k + 1)) * 0.5));
+}
+
 /* KillNorthSouthFluxes!  Dimensionless/HydroThermal3D_Alban_v6.jl:153-159 */
 EXPORT void o_KillNorthSouthFluxes(double *Ty, int nx, int ny, int nz) {
     PFOR_K(nz) for (int i = 1; i <= nx; ++i) {
@@ -792,7 +804,7 @@ EXPORT void o_AdvectWithWeno5_v2(double *Tc_ex, double *Tc_exxx, double *Told, c
 typedef struct {
     int nx, ny, nz;
     double Lx, Ly, Lz, dx, dy, dz;
-    double Ra, Tbot, Ttop, cfl, dt, time, Nu;
+    double Ra, Tbot, Ttop, cfl, dt, time, Nu, mu_a;
     int weno_version;
     /* fields */
     double *Tc_ex, *Pc_ex, *ktv, *kfv, *Ty, *kx, *ky, *kz, *qx, *qy, *qz, *Vx, *Vy, *Vz;
@@ -874,6 +886,8 @@ EXPORT double o_hrl_scalar(const o_hrl_model *m, int which) {
 
 EXPORT void o_hrl_set_dt(o_hrl_model *m, double dt) { m->dt = dt; }
 
+EXPORT void o_hrl_set_viscosity(o_hrl_model *m, double a) { m->mu_a = a; }
+
 /* One physical step.  Returns 0, or the first non-zero solver status. */
 EXPORT int o_hrl_step(o_hrl_model *m) {
     const int nx = m->nx, ny = m->ny, nz = m->nz;
@@ -905,6 +919,7 @@ EXPORT int o_hrl_step(o_hrl_model *m) {
 
     /* Darcy solve (Alban_v6:353-370 with the DYREL loop of v18:489-537) */
     o_InitDarcy(m->Ty, m->kx, m->ky, m->kz, m->Tc_ex, m->kfv, _dt, nx, ny, nz);
+    if (m->mu_a != 0.0) o_ApplyViscosity(m->kx, m->ky, m->kz, m->Tc_ex, m->mu_a, nx, ny, nz);
     o_KillNorthSouthFluxes(m->Ty, nx, ny, nz);
     o_Compute_Rp(m->Rp, m->ky, m->Ty, m->Ra, _dy, nx, ny, nz);
     f.X = m->Pc_ex; f.R = m->Rp; f.transient = 0; f.rhoC = 0.0;

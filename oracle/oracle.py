@@ -52,6 +52,7 @@ def lib() -> C.CDLL:
         _L.o_hrl_set_params.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
         _L.o_hrl_get_reports.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _L.o_hrl_set_dt.argtypes = [C.c_void_p, C.c_double]
+        _L.o_hrl_set_viscosity.argtypes = [C.c_void_p, C.c_double]
         _L.o_dyrel_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     return _L
 
@@ -106,6 +107,8 @@ InitConductivity = _make("InitConductivity", "o_InitConductivity", [("kx", "Fx")
 InitDarcy = _make("InitDarcy", "o_InitDarcy",
                   [("Ty", "Fy"), ("kx", "Fx"), ("ky", "Fy"), ("kz", "Fz"), ("Tc_ex", "E"), ("kfv", "V"), ("_dt", "d")])
 KillNorthSouthFluxes = _make("KillNorthSouthFluxes", "o_KillNorthSouthFluxes", [("Ty", "Fy")])
+ApplyViscosity = _make("ApplyViscosity", "o_ApplyViscosity",
+                       [("kx", "Fx"), ("ky", "Fy"), ("kz", "Fz"), ("Tc_ex", "E"), ("a", "d")])
 Compute_Rp = _make("Compute_Rp", "o_Compute_Rp", [("Rp", "c"), ("ky", "Fy"), ("Ty", "Fy"), ("Ra", "d"), ("_dy", "d")])
 SetTemperatureBCs = _make("SetTemperatureBCs", "o_SetTemperatureBCs", [("Tc_ex", "E"), ("Tbot", "d"), ("Ttop", "d")])
 SetPressureBCs = _make("SetPressureBCs", "o_SetPressureBCs", [("Pc_ex", "E"), ("Pbot", "d"), ("Ptop", "d")])
@@ -312,6 +315,9 @@ class HRLModel:
 
     def set_dt(self, dt: float):
         lib().o_hrl_set_dt(self._m, C.c_double(dt))
+
+    def set_viscosity(self, a: float):
+        lib().o_hrl_set_viscosity(self._m, C.c_double(a))
 
     def step(self) -> dict:
         st = lib().o_hrl_step(self._m)

@@ -190,6 +190,21 @@ def test_dimensional_residuals_bitwise(ht, orc, grid):
     assert np.array_equal(Fd.get(), Fh)
 
 
+@pytest.mark.parametrize("grid", GRIDS[:3], ids=lambda g: "x".join(map(str, g)))
+def test_viscosity_mobility(ht, orc, grid):
+    """Face mobility k exp(a T_face): device exp against libm exp, so ulp-level."""
+    rng = np.random.default_rng(31)
+    sh = shapes(*grid)
+    T = rand(sh["E"], rng, 0.0, 1.0)
+    k = [positive(sh[s], rng) for s in ("Fx", "Fy", "Fz")]
+    d = [ht.DeviceArray.from_host(a) for a in k]
+    ht.kernels.ApplyViscosity(*d, ht.DeviceArray.from_host(T), 2.302585092994046)
+    h = [a.copy(order="F") for a in k]
+    orc.ApplyViscosity(*h, T, 2.302585092994046)
+    for a, b in zip(d, h):
+        assert np.allclose(a.get(), b, rtol=6e-16, atol=0)
+
+
 @pytest.mark.parametrize("op", ["max", "min", "sum", "sum_abs", "max_abs"])
 @pytest.mark.parametrize("n", [1, 63, 64, 1000, 300_001])
 def test_reductions(ht, orc, op, n):

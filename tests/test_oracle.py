@@ -52,6 +52,25 @@ def test_coefficients_numpy_vs_c(orc, grid):
 
 
 @pytest.mark.parametrize("grid", GRIDS, ids=lambda g: "x".join(map(str, g)))
+def test_viscosity_numpy_vs_c(orc, grid):
+    """Face mobility k exp(a T_face); libm exp on both sides of the comparison."""
+    rng = np.random.default_rng(12)
+    sh = shapes(*grid)
+    T = rand(sh["E"], rng, 0.0, 1.0)
+    k = [positive(sh[s], rng) for s in ("Fx", "Fy", "Fz")]
+    kc = [a.copy(order="F") for a in k]
+    orc.ApplyViscosity(*kc, T, math.log(10.0))
+    for a, b in zip(kc, npo.ApplyViscosity(*k, T, math.log(10.0))):
+        assert np.allclose(a, b, rtol=6e-16, atol=0)   # numpy and libm exp differ by an ulp
+    # uniform T: a pure factor 10^T
+    T[...] = 0.25
+    kc = [a.copy(order="F") for a in k]
+    orc.ApplyViscosity(*kc, T, math.log(10.0))
+    for a, b in zip(kc, k):
+        assert np.allclose(a, b * 10 ** 0.25, rtol=1e-15)
+
+
+@pytest.mark.parametrize("grid", GRIDS, ids=lambda g: "x".join(map(str, g)))
 def test_bcs_numpy_vs_c(orc, grid):
     rng = np.random.default_rng(2)
     sh = shapes(*grid)
