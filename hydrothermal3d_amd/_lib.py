@@ -49,7 +49,7 @@ class StepReport(C.Structure):
     _fields_ = [("thermal", DyrelReport), ("darcy", DyrelReport), ("dt_used", C.c_double),
                 ("dt_next", C.c_double), ("time", C.c_double), ("Nu", C.c_double), ("vmax", C.c_double),
                 ("status", C.c_int), ("dc_outer_thermal", C.c_int), ("dc_outer_darcy", C.c_int),
-                ("inner_thermal", C.c_int), ("inner_darcy", C.c_int)]
+                ("inner_thermal", C.c_int), ("inner_darcy", C.c_int), ("n_adv", C.c_int)]
 
 
 _CTYPE = {

@@ -181,7 +181,7 @@ struct ht3d_hrl {
     int nx, ny, nz;
     double Lx, Ly, Lz, dx, dy, dz;
     double Ra, Tbot, Ttop, cfl, dt, time, Nu, mu_a;
-    int weno_version, precision;
+    int weno_version, precision, dt_constant;
     ht3d_dyrel_params pT, pP;
     double *Tc_ex, *Pc_ex, *ktv, *kfv, *Ty, *kx, *ky, *kz, *ktx, *kty, *ktz, *Vx, *Vy, *Vz, *Rt, *Rp, *W;
 };
@@ -320,6 +320,12 @@ int ht3d_hrl_set_viscosity(ht3d_hrl *m, double a) {
     return HT3D_OK;
 }
 
+int ht3d_hrl_set_dt_constant(ht3d_hrl *m, int dt_constant) {
+    if (!m || (dt_constant != 0 && dt_constant != 1)) return arg_fail("ht3d_hrl_set_dt_constant: bad argument");
+    m->dt_constant = dt_constant;
+    return HT3D_OK;
+}
+
 int ht3d_hrl_set_dt(ht3d_hrl *m, double dt) {
     if (!m || !(dt > 0)) return arg_fail("ht3d_hrl_set_dt: bad argument");
     m->dt = dt;
@@ -375,11 +381,9 @@ int ht3d_hrl_step(ht3d_hrl *m, ht3d_step_report *rep) {
     if (sp != HT3D_OK && sp != HT3D_NOT_CONVERGED) { r.status = sp; if (rep) *rep = r; return sp; }
     m->time += dt;
 
-    // velocity and advection
+    // velocity, advective time step from this velocity, advection
+    // (HydroThermal3D_v18.jl:551-570)
     if ((rc = ht3d_Init_vel(m->Vx, m->Vy, m->Vz, m->kx, m->ky, m->kz, m->Pc_ex, m->Ty, m->Ra, _dx, _dy, _dz, nx, ny, nz)))
-        return rc;
-    if ((rc = weno_advect(m->Tc_ex, m->W, m->Vx, m->Vy, m->Vz, dt, _dx, _dy, _dz, m->Ttop, m->Tbot, m->weno_version,
-                          nx, ny, nz)))
         return rc;
     double vx = 0, vy = 0, vz = 0;
     if ((rc = ht3d_reduce(&vx, m->Vx, (size_t)(nx + 1) * ny * nz, HT3D_MAX_ABS))) return rc;
@@ -387,7 +391,23 @@ int ht3d_hrl_step(ht3d_hrl *m, ht3d_step_report *rep) {
     if ((rc = ht3d_reduce(&vz, m->Vz, (size_t)nx * ny * (nz + 1), HT3D_MAX_ABS))) return rc;
     const double vmax = fmax(fmax(vx, vy), vz);
     const double h = fmin(fmin(m->dx, m->dy), m->dz);
-    m->dt = m->cfl * h / vmax;
+    const double dt_adv = m->cfl * h / vmax;
+    int n_adv = 1;
+    double dt_a = dt_adv;
+    if (m->dt_constant) {
+        const double q = std::fmax(std::ceil(dt / dt_adv), 1.0);  // 1 when V = 0
+        if (!(q <= 1e6)) return arg_fail("ht3d_hrl_step: advection sub-step count out of range");
+        n_adv = (int)q;
+        dt_a = dt / n_adv;
+    } else {
+        if (!(dt_adv > 0.0) || !std::isfinite(dt_adv)) return arg_fail("ht3d_hrl_step: zero velocity, no CFL time step");
+        m->dt = dt_adv;
+    }
+    for (int s = 0; s < n_adv; ++s)
+        if ((rc = weno_advect(m->Tc_ex, m->W, m->Vx, m->Vy, m->Vz, dt_a, _dx, _dy, _dz, m->Ttop, m->Tbot,
+                              m->weno_version, nx, ny, nz)))
+            return rc;
+    r.n_adv = n_adv;
 
     r.dt_next = m->dt;
     r.time = m->time;

@@ -90,7 +90,8 @@ class HydroThermal3D:
 
     Fields are device arrays shaped like the reference's (Tc_ex, Pc_ex: E;
     ktv, kfv: V; Vx/Vy/Vz: faces).  `step()` runs thermal solve, Nu, Darcy solve,
-    velocity, WENO5 advection and the CFL time step, all on the device.
+    velocity, the advective CFL time step from that velocity and WENO5
+    advection (HydroThermal3D_v18.jl:551-570), all on the device.
     """
 
     FIELDS = {"Tc_ex": "E", "Pc_ex": "E", "ktv": "V", "kfv": "V", "Ty": "Fy", "kx": "Fx", "ky": "Fy", "kz": "Fz",
@@ -125,6 +126,11 @@ class HydroThermal3D:
 
     def set_dt(self, dt: float) -> None:
         _lib.call("ht3d_hrl_set_dt", self._h, C.c_double(dt))
+
+    def set_dt_constant(self, flag: bool) -> None:
+        """dt_constant of HydroThermal3D_v18.jl:557: keep dt and sub-step the
+        advection in ceil(dt/dt_adv) parts instead of taking dt = dt_adv."""
+        _lib.call("ht3d_hrl_set_dt_constant", self._h, int(bool(flag)))
 
     def set_viscosity(self, a: float) -> None:
         """mu = exp(-a T) in the Darcy solve (a = ln 10 for the 768^3-class workload)."""

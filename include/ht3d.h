@@ -274,7 +274,12 @@ int ht3d_weno5_advect(double *Tc_ex, const double *Vx, const double *Vy, const d
                       double _dy, double _dz, double Ttop, double Tbot, int version, int ncx, int ncy, int ncz);
 
 /* Closed-box Horton-Rogers-Lapwood porous convection (HydroThermal3D_Alban_v6.jl)
- * stepped with DYREL solves and WENO5 advection, all on the device.          */
+ * stepped with DYREL solves and WENO5 advection, all on the device.  The
+ * advective time step follows HydroThermal3D_v18.jl:553-570: dt_adv =
+ * cfl*min(dx,dy,dz)/max|V| from this step's velocity; with dt_constant = 0 the
+ * advection uses dt_adv and the next step's solves use it too, with
+ * dt_constant = 1 the step's dt is kept and the advection is sub-stepped in
+ * ceil(dt/dt_adv) equal parts.                                               */
 typedef struct ht3d_hrl ht3d_hrl;
 typedef struct {
     ht3d_dyrel_report thermal, darcy;
@@ -282,6 +287,7 @@ typedef struct {
     int status;
     int dc_outer_thermal, dc_outer_darcy;   /* defect-correction outer iterations */
     int inner_thermal, inner_darcy;         /* FP32 inner sweeps in total          */
+    int n_adv;                              /* advection sub-steps                 */
 } ht3d_step_report;
 
 enum { HT3D_FP64 = 0, HT3D_FP32_DC = 1 };
@@ -294,6 +300,8 @@ int ht3d_hrl_field(ht3d_hrl *m, const char *name, double **dptr, size_t *count);
 int ht3d_hrl_set_params(ht3d_hrl *m, const ht3d_dyrel_params *pT, const ht3d_dyrel_params *pP, int weno_version,
                         int precision);
 int ht3d_hrl_set_dt(ht3d_hrl *m, double dt);
+/* dt_constant of HydroThermal3D_v18.jl:557 (default 0) */
+int ht3d_hrl_set_dt_constant(ht3d_hrl *m, int dt_constant);
 /* mu = exp(-a T) in the Darcy solve (a = 0: constant viscosity, the default) */
 int ht3d_hrl_set_viscosity(ht3d_hrl *m, double a);
 int ht3d_hrl_step(ht3d_hrl *m, ht3d_step_report *rep);

@@ -288,7 +288,8 @@ def dyrel_solve(X, R, kx, ky, kz, *, dx, dy, dz, dt, transient, rhoC, bc_type, b
 # ------------------------------------------------------------------ physical step
 class HRL:
     """Closed-box HRL convection stepped like HydroThermal3D_Alban_v6.jl:322-466
-    with DYREL solves; same step order as the C oracle's o_hrl_step."""
+    with DYREL solves and the advective time step of HydroThermal3D_v18.jl:551-570;
+    same step order as the C oracle's o_hrl_step."""
 
     def __init__(self, nx, ny, nz, Lx, Ly, Lz, Ra, cfl=0.5):
         self.n = (nx, ny, nz)
@@ -304,6 +305,8 @@ class HRL:
         self.prm = dict(nitmax=10000, nitout=1000, ncheck=0, norm=0, tol=1e-8)
         self.weno = 1
         self.time = 0.0
+        self.mu_a = 0.0
+        self.dt_constant = False
 
     def step(self):
         nx, ny, nz = self.n
@@ -316,13 +319,22 @@ class HRL:
                                        bc_type=(0, 0, 1, 1, 0, 0), bc_val=(0.0, 0.0, 1.0, 0.0, 0.0, 0.0), **self.prm)
         Nu = math.fsum(qy[:, -1, :].ravel()) / (self.L[0] * self.L[2]) / (nx * nz)
         Ty, kx, ky, kz = InitDarcy(self.T, self.kfv)
+        if self.mu_a != 0.0:
+            kx, ky, kz = ApplyViscosity(kx, ky, kz, self.T, self.mu_a)
         Ty = KillNorthSouthFluxes(Ty)
         Rp = Compute_Rp(ky, Ty, self.Ra, _dy)
         self.P, repP, _ = dyrel_solve(self.P, Rp, kx, ky, kz, dx=dx, dy=dy, dz=dz, dt=dt, transient=0, rhoC=0.0,
                                       bc_type=(0,) * 6, bc_val=(0.0,) * 6, **self.prm)
         self.time += dt
         self.V = Init_vel(kx, ky, kz, self.P, Ty, self.Ra, _dx, _dy, _dz)
-        self.T = advect_weno5(self.T, *self.V, dt, _dx, _dy, _dz, 0.0, 1.0, self.weno)
         vmax = max(float(np.abs(v).max()) for v in self.V)
-        self.dt = self.cfl * min(self.h) / vmax
-        return dict(thermal=repT, darcy=repP, Nu=Nu, dt_next=self.dt)
+        dt_adv = self.cfl * min(self.h) / vmax
+        if self.dt_constant:
+            n_adv = max(math.ceil(dt / dt_adv), 1)
+            dt_a = dt / n_adv
+        else:
+            n_adv, dt_a = 1, dt_adv
+            self.dt = dt_adv
+        for _ in range(n_adv):
+            self.T = advect_weno5(self.T, *self.V, dt_a, _dx, _dy, _dz, 0.0, 1.0, self.weno)
+        return dict(thermal=repT, darcy=repP, Nu=Nu, dt_next=self.dt, n_adv=n_adv)

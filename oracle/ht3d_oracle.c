@@ -798,6 +798,7 @@ EXPORT void o_AdvectWithWeno5_v2(double *Tc_ex, double *Tc_exxx, double *Told, c
 /* ------------------------------------------------------------------------ */
 /* Closed-box Horton-Rogers-Lapwood physical step with DYREL solves            */
 /*   step order: HydroThermal3D_Alban_v6.jl:322-466 / HydroThermal3D_v7.jl:167-222 */
+/*   time step:  HydroThermal3D_v18.jl:551-570 (advective dt from this step's V) */
 /*   solves:     HydroThermal3D_v18.jl:424-538                                */
 /* ------------------------------------------------------------------------ */
 
@@ -805,7 +806,7 @@ typedef struct {
     int nx, ny, nz;
     double Lx, Ly, Lz, dx, dy, dz;
     double Ra, Tbot, Ttop, cfl, dt, time, Nu, mu_a;
-    int weno_version;
+    int weno_version, dt_constant, n_adv;
     /* fields */
     double *Tc_ex, *Pc_ex, *ktv, *kfv, *Ty, *kx, *ky, *kz, *qx, *qy, *qz, *Vx, *Vy, *Vz;
     double *Rt, *Rp, *F, *F0, *Fcit, *PC, *G, *dumc;
@@ -888,6 +889,10 @@ EXPORT void o_hrl_set_dt(o_hrl_model *m, double dt) { m->dt = dt; }
 
 EXPORT void o_hrl_set_viscosity(o_hrl_model *m, double a) { m->mu_a = a; }
 
+EXPORT void o_hrl_set_dt_constant(o_hrl_model *m, int dt_constant) { m->dt_constant = dt_constant; }
+
+EXPORT int o_hrl_n_adv(const o_hrl_model *m) { return m->n_adv; }
+
 /* One physical step.  Returns 0, or the first non-zero solver status. */
 EXPORT int o_hrl_step(o_hrl_model *m) {
     const int nx = m->nx, ny = m->ny, nz = m->nz;
@@ -930,20 +935,32 @@ EXPORT int o_hrl_step(o_hrl_model *m) {
     if (sp == 11 || sp == 12) return sp;
     m->time += dt;
 
-    /* advection (Alban_v6:380-455) */
+    /* velocity, then the advective time step from it (v18:553-561): the
+       advection uses dt_adv and so does the next step, or, with dt_constant,
+       the step's dt is split into ceil(dt/dt_adv) advection sub-steps
+       (v18:563-570).  Advection itself: Alban_v6:380-455.                   */
     o_Init_vel(m->Vx, m->Vy, m->Vz, m->kx, m->ky, m->kz, m->Pc_ex, m->Ty, m->Ra, _dx, _dy, _dz, nx, ny, nz);
-    if (m->weno_version == 2)
-        o_AdvectWithWeno5_v2(m->Tc_ex, m->w.Tc_exxx, m->w.Told, m->Vx, m->Vy, m->Vz, dt, _dx, _dy, _dz, m->Ttop,
-                             m->Tbot, nx, ny, nz);
-    else
-        o_AdvectWithWeno5_v1(m->Tc_ex, m->Vx, m->Vy, m->Vz, &m->w, dt, _dx, _dy, _dz, m->Ttop, m->Tbot, nx, ny, nz);
-
-    /* next time step from the advective CFL (HydroThermal3D_v7.jl:220, TestAdvection.jl:96) */
     double vmax = fmax(fmax(o_reduce(m->Vx, (size_t)(nx + 1), (size_t)ny * nz, R_MAX_ABS),
                             o_reduce(m->Vy, (size_t)nx, (size_t)(ny + 1) * nz, R_MAX_ABS)),
                        o_reduce(m->Vz, (size_t)nx, (size_t)ny * (nz + 1), R_MAX_ABS));
     double h = fmin(fmin(m->dx, m->dy), m->dz);
-    m->dt = m->cfl * h / vmax;
+    double dt_adv = m->cfl * h / vmax, dt_a = dt_adv;
+    int n_adv = 1;
+    if (m->dt_constant) {
+        n_adv = (int)fmax(ceil(dt / dt_adv), 1.0);
+        dt_a = dt / n_adv;
+    } else {
+        m->dt = dt_adv;
+    }
+    for (int s = 0; s < n_adv; ++s) {
+        if (m->weno_version == 2)
+            o_AdvectWithWeno5_v2(m->Tc_ex, m->w.Tc_exxx, m->w.Told, m->Vx, m->Vy, m->Vz, dt_a, _dx, _dy, _dz, m->Ttop,
+                                 m->Tbot, nx, ny, nz);
+        else
+            o_AdvectWithWeno5_v1(m->Tc_ex, m->Vx, m->Vy, m->Vz, &m->w, dt_a, _dx, _dy, _dz, m->Ttop, m->Tbot, nx, ny,
+                                 nz);
+    }
+    m->n_adv = n_adv;
     return (st == 10 || sp == 10) ? 10 : 0;
 }
 

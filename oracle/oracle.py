@@ -53,6 +53,8 @@ def lib() -> C.CDLL:
         _L.o_hrl_get_reports.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _L.o_hrl_set_dt.argtypes = [C.c_void_p, C.c_double]
         _L.o_hrl_set_viscosity.argtypes = [C.c_void_p, C.c_double]
+        _L.o_hrl_set_dt_constant.argtypes = [C.c_void_p, C.c_int]
+        _L.o_hrl_n_adv.argtypes = [C.c_void_p]
         _L.o_dyrel_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     return _L
 
@@ -319,12 +321,15 @@ class HRLModel:
     def set_viscosity(self, a: float):
         lib().o_hrl_set_viscosity(self._m, C.c_double(a))
 
+    def set_dt_constant(self, flag: bool):
+        lib().o_hrl_set_dt_constant(self._m, int(bool(flag)))
+
     def step(self) -> dict:
         st = lib().o_hrl_step(self._m)
         rT, rP = Report(), Report()
         lib().o_hrl_get_reports(self._m, C.byref(rT), C.byref(rP))
         return {"status": st, "thermal": rT.as_dict(), "darcy": rP.as_dict(), "dt_next": self.scalar(0),
-                "time": self.scalar(1), "Nu": self.scalar(2)}
+                "time": self.scalar(1), "Nu": self.scalar(2), "n_adv": lib().o_hrl_n_adv(self._m)}
 
     def scalar(self, which: int) -> float:
         return lib().o_hrl_scalar(self._m, which)

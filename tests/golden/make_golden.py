@@ -5,6 +5,9 @@ library.  Run from the repository root:  python tests/golden/make_golden.py
 hrl_8x6x8.npz  closed-box HRL convection, 8x6x8 cells of the 2x1x2 box, Ra=500,
                lognormal k (sigma 0.5, 64 modes), 3 physical steps with DYREL
                (max-norm check every 25 iterations, lambda every 100, tol 1e-10)
+hrl_visc_dtconst_8x6x8.npz
+               the same with viscosity mu = exp(-ln(10) T) and a constant
+               dt = 0.004 split into advection sub-steps
 weno_7x5x6.npz one WENO5 advection step (v1 and v2) of a random field with random
                face velocities and the closed-box boundary types
 """
@@ -37,25 +40,31 @@ def lognormal_numpy(w):
     return np.asfortranarray(10.0 ** (w.sigma * (math.sqrt(2.0 / w.modes) * s)))
 
 
-def hrl():
+def hrl(name, mu_a=0.0, dt_constant=None):
     w = inputs.replace(inputs.WORKLOADS["hrl256"].scaled(8, 6, 8), modes=64)
     T0 = inputs.initial_temperature(w)
     kfv = lognormal_numpy(w)
     m = npo.HRL(w.ncx, w.ncy, w.ncz, w.Lx, w.Ly, w.Lz, w.Ra, w.cfl)
     m.T = T0.copy()
     m.kfv = kfv.copy()
+    m.mu_a = mu_a
+    if dt_constant is not None:
+        m.dt_constant, m.dt = True, dt_constant
     m.prm = dict(nitmax=20000, nitout=100, ncheck=25, norm=1, tol=1e-10)
-    Nu, dt, itT, itP = [], [], [], []
+    Nu, dt, itT, itP, nadv = [], [], [], [], []
     for _ in range(3):
         dt.append(m.dt)
         r = m.step()
         Nu.append(r["Nu"])
         itT.append(r["thermal"]["iters"])
         itP.append(r["darcy"]["iters"])
-    np.savez_compressed(HERE / "hrl_8x6x8.npz", T0=T0, kfv=kfv, T=m.T, P=m.P, Vx=m.V[0], Vy=m.V[1], Vz=m.V[2],
+        nadv.append(r["n_adv"])
+    np.savez_compressed(HERE / name, T0=T0, kfv=kfv, T=m.T, P=m.P, Vx=m.V[0], Vy=m.V[1], Vz=m.V[2],
                         Nu=np.array(Nu), dt=np.array(dt), iters_T=np.array(itT), iters_P=np.array(itP),
-                        dims=np.array([w.ncx, w.ncy, w.ncz]), L=np.array([w.Lx, w.Ly, w.Lz]), Ra=w.Ra, cfl=w.cfl,
-                        modes=w.modes, seed=w.seed)
+                        n_adv=np.array(nadv), dims=np.array([w.ncx, w.ncy, w.ncz]), L=np.array([w.Lx, w.Ly, w.Lz]),
+                        Ra=w.Ra, cfl=w.cfl, modes=w.modes, seed=w.seed, mu_a=mu_a,
+                        dt_constant=-1.0 if dt_constant is None else dt_constant)
+    return nadv
 
 
 def weno():
@@ -73,6 +82,7 @@ def weno():
 
 
 if __name__ == "__main__":
-    hrl()
+    hrl("hrl_8x6x8.npz")
+    print("sub-steps", hrl("hrl_visc_dtconst_8x6x8.npz", mu_a=math.log(10.0), dt_constant=0.004))
     weno()
     print("golden fixtures written to", HERE)

@@ -32,8 +32,20 @@ def test_golden_lognormal_field_cpu(orc):
     assert np.allclose(kfv, g["kfv"], rtol=1e-12, atol=0)
 
 
-def test_golden_hrl_cpu(orc):
-    g = _load("hrl_8x6x8.npz")
+HRL_GOLD = ["hrl_8x6x8.npz", "hrl_visc_dtconst_8x6x8.npz"]
+
+
+def _setup_model(m, g):
+    """Viscosity and the constant-dt mode as the fixture recorded them."""
+    m.set_viscosity(float(g["mu_a"]))
+    if float(g["dt_constant"]) > 0:
+        m.set_dt_constant(True)
+        m.set_dt(float(g["dt_constant"]))
+
+
+@pytest.mark.parametrize("name", HRL_GOLD)
+def test_golden_hrl_cpu(orc, name):
+    g = _load(name)
     nx, ny, nz = (int(v) for v in g["dims"])
     Lx, Ly, Lz = map(float, g["L"])
     m = orc.HRLModel(nx, ny, nz, Lx, Ly, Lz, float(g["Ra"]), float(g["cfl"]))
@@ -41,15 +53,22 @@ def test_golden_hrl_cpu(orc):
     m.field("kfv")[...] = g["kfv"]
     p = orc.params(nitmax=20000, nitout=100, ncheck=25, norm=1, tol=1e-10)
     m.set_params(p, p, 1)
+    _setup_model(m, g)
+    # Bit-exact without viscosity.  exp() of NumPy and of the C library differ
+    # in the last bit for some arguments, so the mobility exp(a T) carries an
+    # ulp into the viscous case, which the solves (tolerance 1e-10) amplify to
+    # ~1e-12: the FP64 parity bar of 1e-10 relative there.
+    tol = 0.0 if float(g["mu_a"]) == 0.0 else 1e-10
     for s in range(3):
         r = m.step()
         assert r["thermal"]["iters"] == g["iters_T"][s] and r["darcy"]["iters"] == g["iters_P"][s]
-        assert r["Nu"] == g["Nu"][s]
+        assert r["n_adv"] == g["n_adv"][s]
+        assert abs(r["Nu"] - g["Nu"][s]) <= tol * abs(g["Nu"][s])
     I = (slice(1, -1),) * 3
-    assert np.array_equal(m.field("Tc_ex")[I], g["T"][I])
-    assert np.array_equal(m.field("Pc_ex")[I], g["P"][I])
+    assert rel_err(m.field("Tc_ex")[I], g["T"][I]) <= tol
+    assert rel_err(m.field("Pc_ex")[I], g["P"][I]) <= tol
     for k in ("Vx", "Vy", "Vz"):
-        assert np.array_equal(m.field(k), g[k])
+        assert rel_err(m.field(k), g[k]) <= tol
 
 
 @pytest.mark.parametrize("version", [1, 2])
@@ -67,8 +86,9 @@ def test_golden_weno_cpu(orc, version):
 
 
 @pytest.mark.gpu
-def test_golden_hrl_gpu(ht):
-    g = _load("hrl_8x6x8.npz")
+@pytest.mark.parametrize("name", HRL_GOLD)
+def test_golden_hrl_gpu(ht, name):
+    g = _load(name)
     nx, ny, nz = (int(v) for v in g["dims"])
     Lx, Ly, Lz = map(float, g["L"])
     m = ht.HydroThermal3D(nx, ny, nz, Lx, Ly, Lz, float(g["Ra"]), float(g["cfl"]))
@@ -76,9 +96,11 @@ def test_golden_hrl_gpu(ht):
     m["kfv"].set(g["kfv"])
     p = ht.DyrelParams(nitmax=20000, nitout=100, ncheck=25, norm=1, tol=1e-10)
     m.set_params(p, p, 1)
+    _setup_model(m, g)
     for s in range(3):
         r = m.step()
         assert r["thermal"]["iters"] == g["iters_T"][s] and r["darcy"]["iters"] == g["iters_P"][s]
+        assert r["n_adv"] == g["n_adv"][s]
         assert abs(r["Nu"] - g["Nu"][s]) <= 1e-10 * abs(g["Nu"][s])
     I = (slice(1, -1),) * 3
     assert rel_err(m["Tc_ex"].get()[I], g["T"][I]) <= 1e-10
