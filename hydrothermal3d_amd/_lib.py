@@ -21,6 +21,7 @@ OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_STATE = 0, 1, 2, 3, 4
 NOT_CONVERGED, NAN, DIVERGED = 10, 11, 12
 SUM, SUM_ABS, MAX, MIN, MAX_ABS = 0, 1, 2, 3, 4
 FP64, FP32_DC = 0, 1
+DT_V18, DT_V18_CONSTANT, DT_V7 = 0, 1, 2
 
 
 class DyrelParams(C.Structure):

@@ -181,7 +181,7 @@ struct ht3d_hrl {
     int nx, ny, nz;
     double Lx, Ly, Lz, dx, dy, dz;
     double Ra, Tbot, Ttop, cfl, dt, time, Nu, mu_a;
-    int weno_version, precision, dt_constant;
+    int weno_version, precision, dt_rule;
     ht3d_dyrel_params pT, pP;
     double *Tc_ex, *Pc_ex, *ktv, *kfv, *Ty, *kx, *ky, *kz, *ktx, *kty, *ktz, *Vx, *Vy, *Vz, *Rt, *Rp, *W;
 };
@@ -320,9 +320,9 @@ int ht3d_hrl_set_viscosity(ht3d_hrl *m, double a) {
     return HT3D_OK;
 }
 
-int ht3d_hrl_set_dt_constant(ht3d_hrl *m, int dt_constant) {
-    if (!m || (dt_constant != 0 && dt_constant != 1)) return arg_fail("ht3d_hrl_set_dt_constant: bad argument");
-    m->dt_constant = dt_constant;
+int ht3d_hrl_set_dt_rule(ht3d_hrl *m, int rule) {
+    if (!m || rule < HT3D_DT_V18 || rule > HT3D_DT_V7) return arg_fail("ht3d_hrl_set_dt_rule: bad argument");
+    m->dt_rule = rule;
     return HT3D_OK;
 }
 
@@ -382,7 +382,7 @@ int ht3d_hrl_step(ht3d_hrl *m, ht3d_step_report *rep) {
     m->time += dt;
 
     // velocity, advective time step from this velocity, advection
-    // (HydroThermal3D_v18.jl:551-570)
+    // (HydroThermal3D_v18.jl:551-570, HydroThermal3D_v7.jl:215-221)
     if ((rc = ht3d_Init_vel(m->Vx, m->Vy, m->Vz, m->kx, m->ky, m->kz, m->Pc_ex, m->Ty, m->Ra, _dx, _dy, _dz, nx, ny, nz)))
         return rc;
     double vx = 0, vy = 0, vz = 0;
@@ -394,13 +394,14 @@ int ht3d_hrl_step(ht3d_hrl *m, ht3d_step_report *rep) {
     const double dt_adv = m->cfl * h / vmax;
     int n_adv = 1;
     double dt_a = dt_adv;
-    if (m->dt_constant) {
+    if (m->dt_rule == HT3D_DT_V18_CONSTANT) {
         const double q = std::fmax(std::ceil(dt / dt_adv), 1.0);  // 1 when V = 0
         if (!(q <= 1e6)) return arg_fail("ht3d_hrl_step: advection sub-step count out of range");
         n_adv = (int)q;
         dt_a = dt / n_adv;
     } else {
         if (!(dt_adv > 0.0) || !std::isfinite(dt_adv)) return arg_fail("ht3d_hrl_step: zero velocity, no CFL time step");
+        if (m->dt_rule == HT3D_DT_V7) dt_a = dt;
         m->dt = dt_adv;
     }
     for (int s = 0; s < n_adv; ++s)

@@ -127,10 +127,12 @@ class HydroThermal3D:
     def set_dt(self, dt: float) -> None:
         _lib.call("ht3d_hrl_set_dt", self._h, C.c_double(dt))
 
-    def set_dt_constant(self, flag: bool) -> None:
-        """dt_constant of HydroThermal3D_v18.jl:557: keep dt and sub-step the
-        advection in ceil(dt/dt_adv) parts instead of taking dt = dt_adv."""
-        _lib.call("ht3d_hrl_set_dt_constant", self._h, int(bool(flag)))
+    def set_dt_rule(self, rule: int) -> None:
+        """Time step rule: DT_V18 (advect with dt_adv = cfl h/max|V| and keep it,
+        HydroThermal3D_v18.jl:553-562), DT_V18_CONSTANT (keep dt, advect in
+        ceil(dt/dt_adv) sub-steps, v18:563-570) or DT_V7 (advect with dt, then
+        dt = dt_adv, HydroThermal3D_v7.jl:215-221)."""
+        _lib.call("ht3d_hrl_set_dt_rule", self._h, int(rule))
 
     def set_viscosity(self, a: float) -> None:
         """mu = exp(-a T) in the Darcy solve (a = ln 10 for the 768^3-class workload)."""

@@ -274,12 +274,14 @@ int ht3d_weno5_advect(double *Tc_ex, const double *Vx, const double *Vy, const d
                       double _dy, double _dz, double Ttop, double Tbot, int version, int ncx, int ncy, int ncz);
 
 /* Closed-box Horton-Rogers-Lapwood porous convection (HydroThermal3D_Alban_v6.jl)
- * stepped with DYREL solves and WENO5 advection, all on the device.  The
- * advective time step follows HydroThermal3D_v18.jl:553-570: dt_adv =
- * cfl*min(dx,dy,dz)/max|V| from this step's velocity; with dt_constant = 0 the
- * advection uses dt_adv and the next step's solves use it too, with
- * dt_constant = 1 the step's dt is kept and the advection is sub-stepped in
- * ceil(dt/dt_adv) equal parts.                                               */
+ * stepped with DYREL solves and WENO5 advection, all on the device.  Time step
+ * rules, with dt_adv = cfl*min(dx,dy,dz)/max|V| from this step's velocity:
+ *   HT3D_DT_V18           advect with dt_adv; the next step's solves use it too
+ *                         (HydroThermal3D_v18.jl:553-562, dt_constant = false);
+ *   HT3D_DT_V18_CONSTANT  keep dt; advect in ceil(dt/dt_adv) equal sub-steps
+ *                         (HydroThermal3D_v18.jl:563-570, dt_constant = true);
+ *   HT3D_DT_V7            advect with this step's dt, then dt = dt_adv for the
+ *                         next step (HydroThermal3D_v7.jl:215-221).          */
 typedef struct ht3d_hrl ht3d_hrl;
 typedef struct {
     ht3d_dyrel_report thermal, darcy;
@@ -300,8 +302,9 @@ int ht3d_hrl_field(ht3d_hrl *m, const char *name, double **dptr, size_t *count);
 int ht3d_hrl_set_params(ht3d_hrl *m, const ht3d_dyrel_params *pT, const ht3d_dyrel_params *pP, int weno_version,
                         int precision);
 int ht3d_hrl_set_dt(ht3d_hrl *m, double dt);
-/* dt_constant of HydroThermal3D_v18.jl:557 (default 0) */
-int ht3d_hrl_set_dt_constant(ht3d_hrl *m, int dt_constant);
+enum { HT3D_DT_V18 = 0, HT3D_DT_V18_CONSTANT = 1, HT3D_DT_V7 = 2 };
+/* time step rule (default HT3D_DT_V18) */
+int ht3d_hrl_set_dt_rule(ht3d_hrl *m, int rule);
 /* mu = exp(-a T) in the Darcy solve (a = 0: constant viscosity, the default) */
 int ht3d_hrl_set_viscosity(ht3d_hrl *m, double a);
 int ht3d_hrl_step(ht3d_hrl *m, ht3d_step_report *rep);

@@ -288,7 +288,8 @@ def dyrel_solve(X, R, kx, ky, kz, *, dx, dy, dz, dt, transient, rhoC, bc_type, b
 # ------------------------------------------------------------------ physical step
 class HRL:
     """Closed-box HRL convection stepped like HydroThermal3D_Alban_v6.jl:322-466
-    with DYREL solves and the advective time step of HydroThermal3D_v18.jl:551-570;
+    with DYREL solves and the advective time step rules of HydroThermal3D_v18.jl:551-570
+    (0, 1) and HydroThermal3D_v7.jl:215-221 (2);
     same step order as the C oracle's o_hrl_step."""
 
     def __init__(self, nx, ny, nz, Lx, Ly, Lz, Ra, cfl=0.5):
@@ -306,7 +307,7 @@ class HRL:
         self.weno = 1
         self.time = 0.0
         self.mu_a = 0.0
-        self.dt_constant = False
+        self.dt_rule = 0          # 0: v18, 1: v18 constant dt + sub-steps, 2: v7
 
     def step(self):
         nx, ny, nz = self.n
@@ -329,11 +330,11 @@ class HRL:
         self.V = Init_vel(kx, ky, kz, self.P, Ty, self.Ra, _dx, _dy, _dz)
         vmax = max(float(np.abs(v).max()) for v in self.V)
         dt_adv = self.cfl * min(self.h) / vmax
-        if self.dt_constant:
+        if self.dt_rule == 1:
             n_adv = max(math.ceil(dt / dt_adv), 1)
             dt_a = dt / n_adv
         else:
-            n_adv, dt_a = 1, dt_adv
+            n_adv, dt_a = 1, (dt if self.dt_rule == 2 else dt_adv)
             self.dt = dt_adv
         for _ in range(n_adv):
             self.T = advect_weno5(self.T, *self.V, dt_a, _dx, _dy, _dz, 0.0, 1.0, self.weno)

@@ -798,7 +798,9 @@ EXPORT void o_AdvectWithWeno5_v2(double *Tc_ex, double *Tc_exxx, double *Told, c
 /* ------------------------------------------------------------------------ */
 /* Closed-box Horton-Rogers-Lapwood physical step with DYREL solves            */
 /*   step order: HydroThermal3D_Alban_v6.jl:322-466 / HydroThermal3D_v7.jl:167-222 */
-/*   time step:  HydroThermal3D_v18.jl:551-570 (advective dt from this step's V) */
+/*   time step:  rule 0 HydroThermal3D_v18.jl:553-562 (advect with dt_adv from  */
+/*               this step's V), 1 v18:563-570 (constant dt, sub-stepped),     */
+/*               2 HydroThermal3D_v7.jl:215-221 (advect with dt, then dt_adv)  */
 /*   solves:     HydroThermal3D_v18.jl:424-538                                */
 /* ------------------------------------------------------------------------ */
 
@@ -806,7 +808,7 @@ typedef struct {
     int nx, ny, nz;
     double Lx, Ly, Lz, dx, dy, dz;
     double Ra, Tbot, Ttop, cfl, dt, time, Nu, mu_a;
-    int weno_version, dt_constant, n_adv;
+    int weno_version, dt_rule, n_adv;
     /* fields */
     double *Tc_ex, *Pc_ex, *ktv, *kfv, *Ty, *kx, *ky, *kz, *qx, *qy, *qz, *Vx, *Vy, *Vz;
     double *Rt, *Rp, *F, *F0, *Fcit, *PC, *G, *dumc;
@@ -889,7 +891,7 @@ EXPORT void o_hrl_set_dt(o_hrl_model *m, double dt) { m->dt = dt; }
 
 EXPORT void o_hrl_set_viscosity(o_hrl_model *m, double a) { m->mu_a = a; }
 
-EXPORT void o_hrl_set_dt_constant(o_hrl_model *m, int dt_constant) { m->dt_constant = dt_constant; }
+EXPORT void o_hrl_set_dt_rule(o_hrl_model *m, int rule) { m->dt_rule = rule; }
 
 EXPORT int o_hrl_n_adv(const o_hrl_model *m) { return m->n_adv; }
 
@@ -936,9 +938,10 @@ EXPORT int o_hrl_step(o_hrl_model *m) {
     m->time += dt;
 
     /* velocity, then the advective time step from it (v18:553-561): the
-       advection uses dt_adv and so does the next step, or, with dt_constant,
-       the step's dt is split into ceil(dt/dt_adv) advection sub-steps
-       (v18:563-570).  Advection itself: Alban_v6:380-455.                   */
+       advection uses dt_adv and so does the next step (rule 0); the step's dt
+       is split into ceil(dt/dt_adv) advection sub-steps (rule 1, v18:563-570);
+       the advection uses dt and the next step dt_adv (rule 2, v7:215-221).
+       Advection itself: Alban_v6:380-455.                                   */
     o_Init_vel(m->Vx, m->Vy, m->Vz, m->kx, m->ky, m->kz, m->Pc_ex, m->Ty, m->Ra, _dx, _dy, _dz, nx, ny, nz);
     double vmax = fmax(fmax(o_reduce(m->Vx, (size_t)(nx + 1), (size_t)ny * nz, R_MAX_ABS),
                             o_reduce(m->Vy, (size_t)nx, (size_t)(ny + 1) * nz, R_MAX_ABS)),
@@ -946,10 +949,11 @@ EXPORT int o_hrl_step(o_hrl_model *m) {
     double h = fmin(fmin(m->dx, m->dy), m->dz);
     double dt_adv = m->cfl * h / vmax, dt_a = dt_adv;
     int n_adv = 1;
-    if (m->dt_constant) {
+    if (m->dt_rule == 1) {
         n_adv = (int)fmax(ceil(dt / dt_adv), 1.0);
         dt_a = dt / n_adv;
     } else {
+        if (m->dt_rule == 2) dt_a = dt;
         m->dt = dt_adv;
     }
     for (int s = 0; s < n_adv; ++s) {

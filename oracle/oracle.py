@@ -53,7 +53,7 @@ def lib() -> C.CDLL:
         _L.o_hrl_get_reports.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _L.o_hrl_set_dt.argtypes = [C.c_void_p, C.c_double]
         _L.o_hrl_set_viscosity.argtypes = [C.c_void_p, C.c_double]
-        _L.o_hrl_set_dt_constant.argtypes = [C.c_void_p, C.c_int]
+        _L.o_hrl_set_dt_rule.argtypes = [C.c_void_p, C.c_int]
         _L.o_hrl_n_adv.argtypes = [C.c_void_p]
         _L.o_dyrel_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     return _L
@@ -321,8 +321,9 @@ class HRLModel:
     def set_viscosity(self, a: float):
         lib().o_hrl_set_viscosity(self._m, C.c_double(a))
 
-    def set_dt_constant(self, flag: bool):
-        lib().o_hrl_set_dt_constant(self._m, int(bool(flag)))
+    def set_dt_rule(self, rule: int):
+        """0: v18 (advect with dt_adv), 1: v18 constant dt with sub-steps, 2: v7."""
+        lib().o_hrl_set_dt_rule(self._m, int(rule))
 
     def step(self) -> dict:
         st = lib().o_hrl_step(self._m)

@@ -49,7 +49,7 @@ def hrl(name, mu_a=0.0, dt_constant=None):
     m.kfv = kfv.copy()
     m.mu_a = mu_a
     if dt_constant is not None:
-        m.dt_constant, m.dt = True, dt_constant
+        m.dt_rule, m.dt = 1, dt_constant
     m.prm = dict(nitmax=20000, nitout=100, ncheck=25, norm=1, tol=1e-10)
     Nu, dt, itT, itP, nadv = [], [], [], [], []
     for _ in range(3):

@@ -39,7 +39,7 @@ def _setup_model(m, g):
     """Viscosity and the constant-dt mode as the fixture recorded them."""
     m.set_viscosity(float(g["mu_a"]))
     if float(g["dt_constant"]) > 0:
-        m.set_dt_constant(True)
+        m.set_dt_rule(1)
         m.set_dt(float(g["dt_constant"]))
 
 

@@ -101,7 +101,7 @@ def _gauge(P):
     return inn - inn.mean()
 
 
-def _run_steps(ht, orc, w, nsteps, pT, pP, weno=1, mu_a=0.0):
+def _run_steps(ht, orc, w, nsteps, pT, pP, weno=1, mu_a=0.0, dt_rule=0):
     from hydrothermal3d_amd import inputs
     T0 = inputs.initial_temperature(w)
     g = ht.HydroThermal3D(w.ncx, w.ncy, w.ncz, w.Lx, w.Ly, w.Lz, w.Ra, w.cfl)
@@ -110,8 +110,10 @@ def _run_steps(ht, orc, w, nsteps, pT, pP, weno=1, mu_a=0.0):
     kfv = g["kfv"].get()
     g.set_params(pT, pP, weno)
     g.set_viscosity(mu_a)
+    g.set_dt_rule(dt_rule)
     o = orc.HRLModel(w.ncx, w.ncy, w.ncz, w.Lx, w.Ly, w.Lz, w.Ra, w.cfl)
     o.set_viscosity(mu_a)
+    o.set_dt_rule(dt_rule)
     o.field("Tc_ex")[...] = T0
     o.field("kfv")[...] = kfv
     o.set_params(orc.params(pT.nitmax, pT.nitout, pT.ncheck, pT.norm, pT.tol),
@@ -121,6 +123,7 @@ def _run_steps(ht, orc, w, nsteps, pT, pP, weno=1, mu_a=0.0):
         rg = g.step()
         rc = o.step()
         out.append((rg, rc))
+        assert rg["n_adv"] == rc["n_adv"], f"step {s}"
         assert abs(rg["thermal"]["iters"] - rc["thermal"]["iters"]) <= 1, f"step {s}"
         assert abs(rg["darcy"]["iters"] - rc["darcy"]["iters"]) <= 1, f"step {s}"
         assert abs(rg["Nu"] - rc["Nu"]) <= TOL_FIELD * abs(rc["Nu"]), f"step {s}"
@@ -147,12 +150,14 @@ def test_hrl_steps_parity_small(ht, orc, weno):
         assert e <= TOL_FIELD, f"{k}: {e}"
 
 
-def test_hrl32_reference_schedule(ht, orc):
-    """The 32^3, Ra=200 workload with the reference's DYREL schedule and norm."""
+@pytest.mark.parametrize("dt_rule", [0, 2], ids=["v18", "v7"])
+def test_hrl32_reference_schedule(ht, orc, dt_rule):
+    """The 32^3, Ra=200 workload with the reference's DYREL schedule and norm,
+    under the v18 and the v7 time step rules."""
     from hydrothermal3d_amd import inputs
     w = inputs.WORKLOADS["hrl32"]
     p = ht.reference_params(1e-8)
-    out, errs = _run_steps(ht, orc, w, 5, p, p, 1)
+    out, errs = _run_steps(ht, orc, w, 5, p, p, 1, dt_rule=dt_rule)
     for k, e in errs.items():
         assert e <= TOL_FIELD, f"{k}: {e}"
 

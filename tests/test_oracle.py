@@ -165,7 +165,10 @@ def test_dyrel_solve_numpy_vs_c(orc, transient, sched):
     assert np.array_equal(Xc, Xn)
 
 
-def test_hrl_steps_numpy_vs_c(orc):
+@pytest.mark.parametrize("dt_rule", [0, 1, 2], ids=["v18", "v18_constant", "v7"])
+def test_hrl_steps_numpy_vs_c(orc, dt_rule):
+    """Three physical steps under each time step rule (v18 variable, v18
+    constant with advection sub-steps, v7), C oracle vs NumPy bit for bit."""
     from hydrothermal3d_amd import inputs
     w = inputs.WORKLOADS["hrl256"].scaled(8, 6, 8)
     T0 = inputs.initial_temperature(w)
@@ -180,8 +183,14 @@ def test_hrl_steps_numpy_vs_c(orc):
     n.T = T0.copy()
     n.kfv = kfv.copy()
     n.prm = dict(nitmax=20000, nitout=100, ncheck=25, norm=1, tol=1e-10)
+    m.set_dt_rule(dt_rule)
+    n.dt_rule = dt_rule
+    if dt_rule == 1:
+        m.set_dt(2e-3)
+        n.dt = 2e-3
     for _ in range(3):
         rc, rn = m.step(), n.step()
+        assert rc["n_adv"] == rn["n_adv"]
         assert rc["thermal"]["iters"] == rn["thermal"]["iters"]
         assert rc["darcy"]["iters"] == rn["darcy"]["iters"]
         assert rc["Nu"] == rn["Nu"]
