@@ -12,8 +12,10 @@ The HIP library is hydrothermal3d_amd/libht3d.so (C ABI: include/ht3d.h).
 from . import _lib
 from .device import DeviceArray, maximum_g, minimum_g, sum_g, mean_abs_g, max_abs_g
 from . import kernels
-from .solver import DyrelParams, HydroThermal3D, dyrel_solve, reference_params, production_params, SolveError
+from .solver import (DyrelParams, HydroThermal3D, HydroThermal3DBatch, dyrel_solve, reference_params,
+                     production_params, SolveError)
 from . import inputs
 
-__all__ = ["DeviceArray", "kernels", "inputs", "DyrelParams", "HydroThermal3D", "dyrel_solve", "reference_params",
+__all__ = ["DeviceArray", "kernels", "inputs", "DyrelParams", "HydroThermal3D", "HydroThermal3DBatch", "dyrel_solve",
+           "reference_params",
            "production_params", "SolveError", "maximum_g", "minimum_g", "sum_g", "mean_abs_g", "max_abs_g"]

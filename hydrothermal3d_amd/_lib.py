@@ -53,6 +53,20 @@ class StepReport(C.Structure):
                 ("inner_thermal", C.c_int), ("inner_darcy", C.c_int), ("n_adv", C.c_int)]
 
 
+class BatchStats(C.Structure):
+    _fields_ = [("ms_solve", C.c_double), ("bytes_solve", C.c_double), ("ms_step", C.c_double),
+                ("steps", C.c_longlong), ("blocks", C.c_longlong), ("graph_launches", C.c_longlong),
+                ("graphs", C.c_longlong), ("repacks", C.c_longlong), ("iterations", C.c_longlong),
+                ("tile_rows", C.c_int * 4), ("tile_z", C.c_int * 4)]
+
+    def as_dict(self) -> dict:
+        d = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            d[name] = list(v) if not isinstance(v, (int, float)) else v
+        return d
+
+
 _CTYPE = {
     "int": C.c_int,
     "double": C.c_double,

@@ -23,20 +23,6 @@ int gershgorin_lmax(const double *kx, const double *ky, const double *kz, int nx
 int bc_fill_table(double *A, int nx, int ny, int nz, const int type[6], const double val[6]);
 int nu_sum(const double *T, const double *ky, int nx, int ny, int nz, double _dy, double *sum);
 
-struct DcArgs {
-    const double *__restrict__ X;   // E
-    const double *__restrict__ R;   // c
-    const double *__restrict__ kx;
-    const double *__restrict__ ky;
-    const double *__restrict__ kz;
-    float *__restrict__ r32;        // c: FP32 right-hand side of the correction equation
-    double *__restrict__ partials;
-    int nx, ny, nz;
-    double _dx, _dy, _dz, _dt;
-    int bct[6];
-    double bcv[6];
-};
-
 // FP64 residual r(x) with on-the-fly ghosts; slots: 0 sum|r| (dd), 1 max|r|
 template <bool TR>
 __global__ __launch_bounds__(256) void dc_residual_kernel(DcArgs a) {
@@ -44,32 +30,8 @@ __global__ __launch_bounds__(256) void dc_residual_kernel(DcArgs a) {
     const int j = blockIdx.y * 4 + (int)(threadIdx.x >> 6);
     const int k = blockIdx.z;
     double s_hi = 0.0, s_lo = 0.0, mx = 0.0;
-    const uint32_t nx = a.nx, ny = a.ny;
-    if ((uint32_t)i < nx && (uint32_t)j < ny) {
-        const uint32_t sE1 = nx + 2, sE2 = sE1 * (ny + 2);
-        const uint32_t e = (uint32_t)(i + 1) + (uint32_t)(j + 1) * sE1 + (uint32_t)(k + 1) * sE2;
-        const uint32_t c = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k * nx * ny;
-        const uint32_t fx = (uint32_t)i + (uint32_t)j * (nx + 1) + (uint32_t)k * (nx + 1) * ny;
-        const uint32_t fy = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k * nx * (ny + 1);
-        const double xc = a.X[e];
-        double xW = a.X[e - 1], xE = a.X[e + 1], xS = a.X[e - sE1], xN = a.X[e + sE1], xB = a.X[e - sE2],
-               xF = a.X[e + sE2];
-        if (i == 0) xW = ghost(a.bct[0], a.bcv[0], xc);
-        if ((uint32_t)i == nx - 1) xE = ghost(a.bct[1], a.bcv[1], xc);
-        if (j == 0) xS = ghost(a.bct[2], a.bcv[2], xc);
-        if ((uint32_t)j == ny - 1) xN = ghost(a.bct[3], a.bcv[3], xc);
-        if (k == 0) xB = ghost(a.bct[4], a.bcv[4], xc);
-        if (k == a.nz - 1) xF = ghost(a.bct[5], a.bcv[5], xc);
-        const double kW = a.kx[fx], kE = a.kx[fx + 1], kS = a.ky[fy], kN = a.ky[fy + nx], kB = a.kz[c],
-                     kF = a.kz[c + nx * ny];
-        const double qW = -kW * (a._dx * (xc - xW)), qE = -kE * (a._dx * (xE - xc));
-        const double qS = -kS * (a._dy * (xc - xS)), qN = -kN * (a._dy * (xN - xc));
-        const double qB = -kB * (a._dz * (xc - xB)), qF = -kF * (a._dz * (xF - xc));
-        double f;
-        if (TR) f = a.R[c] + a._dt * xc + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
-        else f = a.R[c] + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
-        a.r32[c] = (float)f;
-        const double af = fabs(f);
+    if (i < a.nx && j < a.ny) {
+        const double af = fabs(dc_residual_cell<TR>(a, i, j, k));
         dd_add(s_hi, s_lo, af);
         mx = (af > mx || af != af) ? af : mx;
     }
@@ -94,7 +56,7 @@ __global__ __launch_bounds__(256) void to_float_kernel(float *__restrict__ out, 
         out[t] = (float)in[t];
 }
 
-static int to_float(float *out, const double *in, size_t n) {
+int to_float(float *out, const double *in, size_t n) {
     unsigned nb = div_up(n, 256);
     if (nb > 8192) nb = 8192;
     hipLaunchKernelGGL(to_float_kernel, dim3(nb), dim3(256), 0, ctx().stream, out, in, n);

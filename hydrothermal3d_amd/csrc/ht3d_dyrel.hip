@@ -108,19 +108,28 @@ static inline bool lambda_iter(int iter, int nitout) { return iter <= 2 || (nito
 // face-by-face ghost fill with an explicit BC table (same order as the per-kernel entry)
 int bc_fill_table(double *A, int nx, int ny, int nz, const int type[6], const double val[6]);
 
+unsigned gershgorin_blocks(size_t N) {
+    const unsigned nb = div_up(N, 256);
+    return nb > 4096 ? 4096 : nb;
+}
+
+int launch_gershgorin(const double *kx, const double *ky, const double *kz, int nx, int ny, int nz, double dx,
+                      double dy, double dz, double rc_tr_dt, double *partials, unsigned nb, double *Dout) {
+    hipLaunchKernelGGL(gershgorin_max_kernel, dim3(nb), dim3(256), 0, ctx().stream, kx, ky, kz, nx, ny, nz, dx, dy,
+                       dz, rc_tr_dt, partials, Dout);
+    HIP_TRY(hipGetLastError());
+    return HT3D_OK;
+}
+
 int gershgorin_lmax(const double *kx, const double *ky, const double *kz, int nx, int ny, int nz, double dx,
                     double dy, double dz, double rc_tr_dt, double *lmax, double *Dout) {
     void *part = nullptr, *res = nullptr;
-    const size_t N = (size_t)nx * ny * nz;
-    unsigned nb = div_up(N, 256);
-    if (nb > 4096) nb = 4096;
+    const unsigned nb = gershgorin_blocks((size_t)nx * ny * nz);
     int rc = workspace(WS_MISC, (size_t)nb * 8 * sizeof(double), &part);
     if (rc) return rc;
     rc = workspace(WS_RESULT, 64 * sizeof(double), &res);
     if (rc) return rc;
-    hipLaunchKernelGGL(gershgorin_max_kernel, dim3(nb), dim3(256), 0, ctx().stream, kx, ky, kz, nx, ny, nz, dx, dy,
-                       dz, rc_tr_dt, (double *)part, Dout);
-    HIP_TRY(hipGetLastError());
+    if ((rc = launch_gershgorin(kx, ky, kz, nx, ny, nz, dx, dy, dz, rc_tr_dt, (double *)part, nb, Dout))) return rc;
     rc = finish_partials((double *)part, nb, 1u, (double *)res);
     if (rc) return rc;
     double r4[4];
@@ -130,20 +139,40 @@ int gershgorin_lmax(const double *kx, const double *ky, const double *kz, int nx
     return HT3D_OK;
 }
 
+unsigned nu_blocks(int nx, int nz) {
+    const unsigned nb = div_up((size_t)nx * nz, 256);
+    return nb > 1024 ? 1024 : nb;
+}
+
+int launch_nu_partials(const double *T, const double *ky, int nx, int ny, int nz, double _dy, double *partials,
+                       unsigned nb) {
+    hipLaunchKernelGGL(nu_sum_kernel, dim3(nb), dim3(256), 0, ctx().stream, T, ky, nx, ny, nz, _dy, partials);
+    HIP_TRY(hipGetLastError());
+    return HT3D_OK;
+}
+
 // Sum of the top-face flux qy(:, ncy+1, :) of an E array whose ghosts are filled.
 int nu_sum(const double *T, const double *ky, int nx, int ny, int nz, double _dy, double *sum) {
-    unsigned nb = div_up((size_t)nx * nz, 256);
-    if (nb > 1024) nb = 1024;
+    const unsigned nb = nu_blocks(nx, nz);
     void *npart = nullptr, *res = nullptr;
     int rc;
     if ((rc = workspace(WS_MISC, (size_t)nb * 8 * sizeof(double), &npart))) return rc;
     if ((rc = workspace(WS_RESULT, 64 * sizeof(double), &res))) return rc;
-    hipLaunchKernelGGL(nu_sum_kernel, dim3(nb), dim3(256), 0, ctx().stream, T, ky, nx, ny, nz, _dy, (double *)npart);
-    HIP_TRY(hipGetLastError());
+    if ((rc = launch_nu_partials(T, ky, nx, ny, nz, _dy, (double *)npart, nb))) return rc;
     if ((rc = finish_partials((double *)npart, nb, 0u, (double *)res))) return rc;
     double r4[4];
     if ((rc = read_result4((double *)res, r4))) return rc;
     *sum = r4[0];
+    return HT3D_OK;
+}
+
+// copy the ghost layer of src into dst (E arrays)
+int copy_ghosts(double *dst, const double *src, int nx, int ny, int nz) {
+    const size_t nf = (size_t)2 * ((ny + 2) * (size_t)(nz + 2) + (nx + 2) * (size_t)(nz + 2) + (nx + 2) * (size_t)(ny + 2));
+    unsigned nb = div_up(nf, 256);
+    if (nb > 2048) nb = 2048;
+    hipLaunchKernelGGL(copy_ghosts_kernel, dim3(nb), dim3(256), 0, ctx().stream, dst, src, nx, ny, nz);
+    HIP_TRY(hipGetLastError());
     return HT3D_OK;
 }
 
@@ -251,13 +280,7 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
     // reference leaves them (its BC kernel runs before the last update).
     double *res_buf = buf[cur], *prev = buf[cur ^ 1];
     if ((rc = bc_fill_table(prev, nx, ny, nz, f->bc_type, f->bc_val))) return rc;
-    {
-        const size_t nf = (size_t)2 * ((ny + 2) * (size_t)(nz + 2) + (nx + 2) * (size_t)(nz + 2) + (nx + 2) * (size_t)(ny + 2));
-        unsigned nb = div_up(nf, 256);
-        if (nb > 2048) nb = 2048;
-        hipLaunchKernelGGL(copy_ghosts_kernel, dim3(nb), dim3(256), 0, cx.stream, res_buf, prev, nx, ny, nz);
-        HIP_TRY(hipGetLastError());
-    }
+    if ((rc = copy_ghosts(res_buf, prev, nx, ny, nz))) return rc;
     if (nu_out) {
         double sum = 0.0;
         if ((rc = nu_sum(prev, f->ky, nx, ny, nz, 1.0 / f->dy, &sum))) return rc;

@@ -1,0 +1,37 @@
+// ht3d_hrl.hpp -- the closed-box HRL model object and the phases of its
+// physical step, shared by the single-model step (ht3d_model.hip) and the
+// batched step (ht3d_batch.hip).
+#pragma once
+
+#include "ht3d_internal.hpp"
+
+struct ht3d_hrl {
+    int nx, ny, nz;
+    double Lx, Ly, Lz, dx, dy, dz;
+    double Ra, Tbot, Ttop, cfl, dt, time, Nu, mu_a;
+    int weno_version, precision, dt_rule;
+    ht3d_dyrel_params pT, pP;
+    double *Tc_ex, *Pc_ex, *ktv, *kfv, *Ty, *kx, *ky, *kz, *ktx, *kty, *ktz, *Vx, *Vy, *Vz, *Rt, *Rp, *W;
+};
+
+namespace ht3d {
+
+// the model's internal launches do not synchronise one by one
+struct SyncGuard {
+    bool saved;
+    SyncGuard() : saved(ctx().sync) { ctx().sync = false; }
+    ~SyncGuard() { ctx().sync = saved; }
+};
+
+// InitThermal! (HydroThermal3D_Alban_v6.jl:328) and the thermal solve's field
+int hrl_pre_thermal(ht3d_hrl *m, ht3d_dyrel_field *f);
+// InitDarcy!, the T-dependent mobility, KillNorthSouthFluxes!, Compute_Rp!
+// (Alban_v6:354-356) and the Darcy solve's field
+int hrl_pre_darcy(ht3d_hrl *m, ht3d_dyrel_field *f);
+// Init_vel!, the advective time step and WENO5 advection (HydroThermal3D_v18.jl:548-570);
+// fills dt_next, vmax and n_adv of the report and advances m->time by the step's dt
+int hrl_advect(ht3d_hrl *m, ht3d_step_report *r);
+// Nu from the thermal flux sum (Alban_v6:345-348)
+inline void hrl_set_nu(ht3d_hrl *m, double nusum) { m->Nu = nusum / (m->Lx * m->Lz) / (double)(m->nx * m->nz); }
+
+}  // namespace ht3d

@@ -49,11 +49,15 @@ __device__ __forceinline__ T ghost(int type, T val, T inner) {
 // (the ghost layer is part of its allocation), so all loads are unconditional
 // and the boundary condition is applied by selecting the ghost value computed
 // from the centre cell; offsets are 32-bit (E arrays stay below 2^32 words).
+//
+// iter_tile is one (x,y) tile of 64 x TY cells marched over KZ planes from
+// plane bz*KZ; `slot` is where a check iteration writes its block partials.
+// Xin/Xout and flags come separately so the batched kernel can take them from
+// its launch while the rest of `a` lives in a per-problem table.
 template <typename T, bool TRANSIENT, bool DSTORED, bool NTX, int TY, int KZ>
-__global__ __launch_bounds__(64 * TY) void dyrel_iter_kernel(IterArgs<T> a) {
-    const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
-    const unsigned t = xcd_tile(blockIdx.x, nblocks);
-    const int bx = t % a.nbx, by = (t / a.nbx) % a.nby, bz = t / (a.nbx * a.nby);
+__device__ __forceinline__ void iter_tile(const IterArgs<T> &a, const T *__restrict__ Xin, T *__restrict__ Xout,
+                                          unsigned flags, int bx, int by, int bz, double *__restrict__ partials,
+                                          unsigned slot) {
     const int i = bx * 64 + (int)(threadIdx.x & 63u);
     const int j = by * TY + (int)(threadIdx.x >> 6);
     const int k0 = bz * KZ;
@@ -63,7 +67,7 @@ __global__ __launch_bounds__(64 * TY) void dyrel_iter_kernel(IterArgs<T> a) {
     const uint32_t sC2 = nx * ny, sX1 = nx + 1, sX2 = sX1 * ny, sY2 = nx * (ny + 1);
 
     double s_hi = 0.0, s_lo = 0.0, acc_max = 0.0, n_hi = 0.0, n_lo = 0.0, d_hi = 0.0, d_lo = 0.0;
-    const bool check = a.flags & FL_CHECK, lam = a.flags & FL_LAMBDA, store = a.flags & FL_STORE_F;
+    const bool check = flags & FL_CHECK, lam = flags & FL_LAMBDA, store = flags & FL_STORE_F;
 
     if ((uint32_t)i < nx && (uint32_t)j < ny) {
         const bool wlo = i == 0, whi = (uint32_t)i == nx - 1, slo = j == 0, shi = (uint32_t)j == ny - 1;
@@ -71,15 +75,15 @@ __global__ __launch_bounds__(64 * TY) void dyrel_iter_kernel(IterArgs<T> a) {
         uint32_t c = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k0 * sC2;
         uint32_t fx = (uint32_t)i + (uint32_t)j * sX1 + (uint32_t)k0 * sX2;
         uint32_t fy = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k0 * sY2;
-        T xc = a.Xin[e];
-        T xm = a.Xin[e - sE2];
+        T xc = Xin[e];
+        T xm = Xin[e - sE2];
         if (k0 == 0) xm = ghost(a.bct[4], a.bcv[4], xc);
         T kzB = a.kz[c];
 #pragma unroll 2
         for (int k = k0; k < k1; ++k) {
-            T xp = a.Xin[e + sE2];
-            T xW = a.Xin[e - 1], xE = a.Xin[e + 1];
-            T xS = a.Xin[e - sE1], xN = a.Xin[e + sE1];
+            T xp = Xin[e + sE2];
+            T xW = Xin[e - 1], xE = Xin[e + 1];
+            T xS = Xin[e - sE1], xN = Xin[e + sE1];
             // streams read once per iteration: non-temporal, so the L2 keeps the
             // X planes whose neighbours the other waves of the tile still need
             const T kW = __builtin_nontemporal_load(a.kx + fx), kE = __builtin_nontemporal_load(a.kx + fx + 1);
@@ -115,8 +119,8 @@ __global__ __launch_bounds__(64 * TY) void dyrel_iter_kernel(IterArgs<T> a) {
             // DYRELUpdate! (kernels_HT3D_dimensional.jl:323-324)
             const T f0 = a.h1 * f0o + a.h2 * f;
             __builtin_nontemporal_store(f0, a.F0 + c);
-            if (NTX) __builtin_nontemporal_store(xc - a.dtau * f0, a.Xout + e);
-            else a.Xout[e] = xc - a.dtau * f0;
+            if (NTX) __builtin_nontemporal_store(xc - a.dtau * f0, Xout + e);
+            else Xout[e] = xc - a.dtau * f0;
             if (check) {
                 const double fd = fabs((double)(f * D));
                 dd_add(s_hi, s_lo, fd);
@@ -139,8 +143,16 @@ __global__ __launch_bounds__(64 * TY) void dyrel_iter_kernel(IterArgs<T> a) {
     }
     if (check) {
         const double h4[4] = {s_hi, acc_max, n_hi, d_hi}, l4[4] = {s_lo, 0.0, n_lo, d_lo};
-        block_reduce4(h4, l4, 0x2u, a.partials, blockIdx.x);
+        block_reduce4(h4, l4, 0x2u, partials, slot);
     }
+}
+
+template <typename T, bool TRANSIENT, bool DSTORED, bool NTX, int TY, int KZ>
+__global__ __launch_bounds__(64 * TY) void dyrel_iter_kernel(IterArgs<T> a) {
+    const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
+    const unsigned t = xcd_tile(blockIdx.x, nblocks);
+    const int bx = t % a.nbx, by = (t / a.nbx) % a.nby, bz = t / (a.nbx * a.nby);
+    iter_tile<T, TRANSIENT, DSTORED, NTX, TY, KZ>(a, a.Xin, a.Xout, a.flags, bx, by, bz, a.partials, blockIdx.x);
 }
 
 // ------------------------------------------------------------------ host driver
@@ -299,5 +311,53 @@ inline int choose_cfg(IterArgs<T> &a, bool transient, bool can_store_d, int prec
     return HT3D_OK;
 }
 
+
+// ------------------------------------------------------------ defect correction
+// FP64 residual of the dimensionless operators at one cell, with on-the-fly
+// ghost values: r = R [+ x/dt] + div q(x) (ResidualDiffusion(Transient)!,
+// kernels_HT3D.jl:131, HydroThermal3D_Alban_v6.jl:87); r is also stored in FP32
+// as the right-hand side of the correction equation.
+struct DcArgs {
+    const double *__restrict__ X;   // E
+    const double *__restrict__ R;   // c
+    const double *__restrict__ kx;
+    const double *__restrict__ ky;
+    const double *__restrict__ kz;
+    float *__restrict__ r32;        // c: FP32 right-hand side of the correction equation
+    double *__restrict__ partials;
+    int nx, ny, nz;
+    double _dx, _dy, _dz, _dt;
+    int bct[6];
+    double bcv[6];
+};
+
+template <bool TR>
+__device__ __forceinline__ double dc_residual_cell(const DcArgs &a, int i, int j, int k) {
+    const uint32_t nx = a.nx, ny = a.ny;
+    const uint32_t sE1 = nx + 2, sE2 = sE1 * (ny + 2);
+    const uint32_t e = (uint32_t)(i + 1) + (uint32_t)(j + 1) * sE1 + (uint32_t)(k + 1) * sE2;
+    const uint32_t c = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k * nx * ny;
+    const uint32_t fx = (uint32_t)i + (uint32_t)j * (nx + 1) + (uint32_t)k * (nx + 1) * ny;
+    const uint32_t fy = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k * nx * (ny + 1);
+    const double xc = a.X[e];
+    double xW = a.X[e - 1], xE = a.X[e + 1], xS = a.X[e - sE1], xN = a.X[e + sE1], xB = a.X[e - sE2],
+           xF = a.X[e + sE2];
+    if (i == 0) xW = ghost(a.bct[0], a.bcv[0], xc);
+    if ((uint32_t)i == nx - 1) xE = ghost(a.bct[1], a.bcv[1], xc);
+    if (j == 0) xS = ghost(a.bct[2], a.bcv[2], xc);
+    if ((uint32_t)j == ny - 1) xN = ghost(a.bct[3], a.bcv[3], xc);
+    if (k == 0) xB = ghost(a.bct[4], a.bcv[4], xc);
+    if (k == a.nz - 1) xF = ghost(a.bct[5], a.bcv[5], xc);
+    const double kW = a.kx[fx], kE = a.kx[fx + 1], kS = a.ky[fy], kN = a.ky[fy + nx], kB = a.kz[c],
+                 kF = a.kz[c + nx * ny];
+    const double qW = -kW * (a._dx * (xc - xW)), qE = -kE * (a._dx * (xE - xc));
+    const double qS = -kS * (a._dy * (xc - xS)), qN = -kN * (a._dy * (xN - xc));
+    const double qB = -kB * (a._dz * (xc - xB)), qF = -kF * (a._dz * (xF - xc));
+    double f;
+    if (TR) f = a.R[c] + a._dt * xc + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
+    else f = a.R[c] + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
+    a.r32[c] = (float)f;
+    return f;
+}
 
 }  // namespace ht3d

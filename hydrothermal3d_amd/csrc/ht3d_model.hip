@@ -16,7 +16,7 @@
 #include <cstring>
 #include <new>
 
-#include "ht3d_internal.hpp"
+#include "ht3d_hrl.hpp"
 
 namespace ht3d {
 
@@ -176,23 +176,78 @@ __global__ __launch_bounds__(256) void fill_kernel(double *__restrict__ a, size_
 
 using namespace ht3d;
 
-// ------------------------------------------------------------------ model
-struct ht3d_hrl {
-    int nx, ny, nz;
-    double Lx, Ly, Lz, dx, dy, dz;
-    double Ra, Tbot, Ttop, cfl, dt, time, Nu, mu_a;
-    int weno_version, precision, dt_rule;
-    ht3d_dyrel_params pT, pP;
-    double *Tc_ex, *Pc_ex, *ktv, *kfv, *Ty, *kx, *ky, *kz, *ktx, *kty, *ktz, *Vx, *Vy, *Vz, *Rt, *Rp, *W;
-};
+namespace ht3d {
 
-namespace {
-struct SyncGuard {  // the model's internal launches do not synchronise one by one
-    bool saved;
-    SyncGuard() : saved(ctx().sync) { ctx().sync = false; }
-    ~SyncGuard() { ctx().sync = saved; }
-};
-}  // namespace
+int hrl_pre_thermal(ht3d_hrl *m, ht3d_dyrel_field *f) {
+    const int nx = m->nx, ny = m->ny, nz = m->nz;
+    int rc;
+    if ((rc = ht3d_InitThermal(m->Rt, m->ktx, m->kty, m->ktz, m->Tc_ex, m->ktv, 1.0 / m->dt, nx, ny, nz))) return rc;
+    std::memset(f, 0, sizeof *f);
+    f->ncx = nx; f->ncy = ny; f->ncz = nz; f->dx = m->dx; f->dy = m->dy; f->dz = m->dz; f->dt = m->dt;
+    f->X = m->Tc_ex; f->R = m->Rt; f->kx = m->ktx; f->ky = m->kty; f->kz = m->ktz;
+    f->transient = 1; f->rhoC = 1.0;
+    const int tt[6] = {0, 0, 1, 1, 0, 0};
+    const double tv[6] = {0.0, 0.0, m->Tbot, m->Ttop, 0.0, 0.0};
+    std::memcpy(f->bc_type, tt, sizeof tt);
+    std::memcpy(f->bc_val, tv, sizeof tv);
+    return HT3D_OK;
+}
+
+int hrl_pre_darcy(ht3d_hrl *m, ht3d_dyrel_field *f) {
+    const int nx = m->nx, ny = m->ny, nz = m->nz;
+    int rc;
+    if ((rc = ht3d_InitDarcy(m->Ty, m->kx, m->ky, m->kz, m->Tc_ex, m->kfv, 1.0 / m->dt, nx, ny, nz))) return rc;
+    if (m->mu_a != 0.0 && (rc = ht3d_ApplyViscosity(m->kx, m->ky, m->kz, m->Tc_ex, m->mu_a, nx, ny, nz))) return rc;
+    if ((rc = ht3d_KillNorthSouthFluxes(m->Ty, nx, ny, nz))) return rc;
+    if ((rc = ht3d_Compute_Rp(m->Rp, m->ky, m->Ty, m->Ra, 1.0 / m->dy, nx, ny, nz))) return rc;
+    std::memset(f, 0, sizeof *f);
+    f->ncx = nx; f->ncy = ny; f->ncz = nz; f->dx = m->dx; f->dy = m->dy; f->dz = m->dz; f->dt = m->dt;
+    f->X = m->Pc_ex; f->R = m->Rp; f->kx = m->kx; f->ky = m->ky; f->kz = m->kz;
+    f->transient = 0; f->rhoC = 0.0;
+    return HT3D_OK;
+}
+
+int hrl_advect(ht3d_hrl *m, ht3d_step_report *r) {
+    const int nx = m->nx, ny = m->ny, nz = m->nz;
+    const double _dx = 1.0 / m->dx, _dy = 1.0 / m->dy, _dz = 1.0 / m->dz;
+    const double dt = m->dt;
+    int rc;
+    m->time += dt;
+    // velocity, advective time step from this velocity, advection
+    // (HydroThermal3D_v18.jl:551-570, HydroThermal3D_v7.jl:215-221)
+    if ((rc = ht3d_Init_vel(m->Vx, m->Vy, m->Vz, m->kx, m->ky, m->kz, m->Pc_ex, m->Ty, m->Ra, _dx, _dy, _dz, nx, ny, nz)))
+        return rc;
+    double vx = 0, vy = 0, vz = 0;
+    if ((rc = ht3d_reduce(&vx, m->Vx, (size_t)(nx + 1) * ny * nz, HT3D_MAX_ABS))) return rc;
+    if ((rc = ht3d_reduce(&vy, m->Vy, (size_t)nx * (ny + 1) * nz, HT3D_MAX_ABS))) return rc;
+    if ((rc = ht3d_reduce(&vz, m->Vz, (size_t)nx * ny * (nz + 1), HT3D_MAX_ABS))) return rc;
+    const double vmax = fmax(fmax(vx, vy), vz);
+    const double h = fmin(fmin(m->dx, m->dy), m->dz);
+    const double dt_adv = m->cfl * h / vmax;
+    int n_adv = 1;
+    double dt_a = dt_adv;
+    if (m->dt_rule == HT3D_DT_V18_CONSTANT) {
+        const double q = std::fmax(std::ceil(dt / dt_adv), 1.0);  // 1 when V = 0
+        if (!(q <= 1e6)) return arg_fail("ht3d_hrl_step: advection sub-step count out of range");
+        n_adv = (int)q;
+        dt_a = dt / n_adv;
+    } else {
+        if (!(dt_adv > 0.0) || !std::isfinite(dt_adv)) return arg_fail("ht3d_hrl_step: zero velocity, no CFL time step");
+        if (m->dt_rule == HT3D_DT_V7) dt_a = dt;
+        m->dt = dt_adv;
+    }
+    for (int s = 0; s < n_adv; ++s)
+        if ((rc = weno_advect(m->Tc_ex, m->W, m->Vx, m->Vy, m->Vz, dt_a, _dx, _dy, _dz, m->Ttop, m->Tbot,
+                              m->weno_version, nx, ny, nz)))
+            return rc;
+    r->n_adv = n_adv;
+    r->dt_next = m->dt;
+    r->time = m->time;
+    r->vmax = vmax;
+    return HT3D_OK;
+}
+
+}  // namespace ht3d
 
 extern "C" {
 
@@ -336,25 +391,14 @@ int ht3d_hrl_step(ht3d_hrl *m, ht3d_step_report *rep) {
     HT3D_CHECK_INIT();
     if (!m) return arg_fail("ht3d_hrl_step: null model");
     SyncGuard guard;
-    const int nx = m->nx, ny = m->ny, nz = m->nz;
-    const double _dx = 1.0 / m->dx, _dy = 1.0 / m->dy, _dz = 1.0 / m->dz;
-    const double dt = m->dt, _dt = 1.0 / dt;
     ht3d_step_report r;
     std::memset(&r, 0, sizeof r);
-    r.dt_used = dt;
+    r.dt_used = m->dt;
     int rc;
+    ht3d_dyrel_field f;
 
     // thermal solve
-    if ((rc = ht3d_InitThermal(m->Rt, m->ktx, m->kty, m->ktz, m->Tc_ex, m->ktv, _dt, nx, ny, nz))) return rc;
-    ht3d_dyrel_field f;
-    std::memset(&f, 0, sizeof f);
-    f.ncx = nx; f.ncy = ny; f.ncz = nz; f.dx = m->dx; f.dy = m->dy; f.dz = m->dz; f.dt = dt;
-    f.X = m->Tc_ex; f.R = m->Rt; f.kx = m->ktx; f.ky = m->kty; f.kz = m->ktz;
-    f.transient = 1; f.rhoC = 1.0;
-    const int tt[6] = {0, 0, 1, 1, 0, 0};
-    const double tv[6] = {0.0, 0.0, m->Tbot, m->Ttop, 0.0, 0.0};
-    std::memcpy(f.bc_type, tt, sizeof tt);
-    std::memcpy(f.bc_val, tv, sizeof tv);
+    if ((rc = hrl_pre_thermal(m, &f))) return rc;
     double nusum = 0.0;
     int st;
     if (m->precision == HT3D_FP32_DC)
@@ -362,58 +406,20 @@ int ht3d_hrl_step(ht3d_hrl *m, ht3d_step_report *rep) {
     else
         st = dyrel_solve_impl(&f, &m->pT, &r.thermal, &nusum, 1.0);
     if (st != HT3D_OK && st != HT3D_NOT_CONVERGED) { r.status = st; if (rep) *rep = r; return st; }
-    m->Nu = nusum / (m->Lx * m->Lz) / (double)(nx * nz);
+    hrl_set_nu(m, nusum);
 
     // Darcy solve
-    if ((rc = ht3d_InitDarcy(m->Ty, m->kx, m->ky, m->kz, m->Tc_ex, m->kfv, _dt, nx, ny, nz))) return rc;
-    if (m->mu_a != 0.0 && (rc = ht3d_ApplyViscosity(m->kx, m->ky, m->kz, m->Tc_ex, m->mu_a, nx, ny, nz))) return rc;
-    if ((rc = ht3d_KillNorthSouthFluxes(m->Ty, nx, ny, nz))) return rc;
-    if ((rc = ht3d_Compute_Rp(m->Rp, m->ky, m->Ty, m->Ra, _dy, nx, ny, nz))) return rc;
-    f.X = m->Pc_ex; f.R = m->Rp; f.kx = m->kx; f.ky = m->ky; f.kz = m->kz;
-    f.transient = 0; f.rhoC = 0.0;
-    std::memset(f.bc_type, 0, sizeof f.bc_type);
-    std::memset(f.bc_val, 0, sizeof f.bc_val);
+    if ((rc = hrl_pre_darcy(m, &f))) return rc;
     int sp;
     if (m->precision == HT3D_FP32_DC)
         sp = dc_solve_impl(&f, &m->pP, &r.darcy, nullptr, &r.dc_outer_darcy, &r.inner_darcy);
     else
         sp = dyrel_solve_impl(&f, &m->pP, &r.darcy, nullptr, 1.0);
     if (sp != HT3D_OK && sp != HT3D_NOT_CONVERGED) { r.status = sp; if (rep) *rep = r; return sp; }
-    m->time += dt;
 
-    // velocity, advective time step from this velocity, advection
-    // (HydroThermal3D_v18.jl:551-570, HydroThermal3D_v7.jl:215-221)
-    if ((rc = ht3d_Init_vel(m->Vx, m->Vy, m->Vz, m->kx, m->ky, m->kz, m->Pc_ex, m->Ty, m->Ra, _dx, _dy, _dz, nx, ny, nz)))
-        return rc;
-    double vx = 0, vy = 0, vz = 0;
-    if ((rc = ht3d_reduce(&vx, m->Vx, (size_t)(nx + 1) * ny * nz, HT3D_MAX_ABS))) return rc;
-    if ((rc = ht3d_reduce(&vy, m->Vy, (size_t)nx * (ny + 1) * nz, HT3D_MAX_ABS))) return rc;
-    if ((rc = ht3d_reduce(&vz, m->Vz, (size_t)nx * ny * (nz + 1), HT3D_MAX_ABS))) return rc;
-    const double vmax = fmax(fmax(vx, vy), vz);
-    const double h = fmin(fmin(m->dx, m->dy), m->dz);
-    const double dt_adv = m->cfl * h / vmax;
-    int n_adv = 1;
-    double dt_a = dt_adv;
-    if (m->dt_rule == HT3D_DT_V18_CONSTANT) {
-        const double q = std::fmax(std::ceil(dt / dt_adv), 1.0);  // 1 when V = 0
-        if (!(q <= 1e6)) return arg_fail("ht3d_hrl_step: advection sub-step count out of range");
-        n_adv = (int)q;
-        dt_a = dt / n_adv;
-    } else {
-        if (!(dt_adv > 0.0) || !std::isfinite(dt_adv)) return arg_fail("ht3d_hrl_step: zero velocity, no CFL time step");
-        if (m->dt_rule == HT3D_DT_V7) dt_a = dt;
-        m->dt = dt_adv;
-    }
-    for (int s = 0; s < n_adv; ++s)
-        if ((rc = weno_advect(m->Tc_ex, m->W, m->Vx, m->Vy, m->Vz, dt_a, _dx, _dy, _dz, m->Ttop, m->Tbot,
-                              m->weno_version, nx, ny, nz)))
-            return rc;
-    r.n_adv = n_adv;
-
-    r.dt_next = m->dt;
-    r.time = m->time;
+    // velocity, time step, advection
+    if ((rc = hrl_advect(m, &r))) return rc;
     r.Nu = m->Nu;
-    r.vmax = vmax;
     r.status = (st == HT3D_NOT_CONVERGED || sp == HT3D_NOT_CONVERGED) ? HT3D_NOT_CONVERGED : HT3D_OK;
     if (rep) *rep = r;
     rc = after_launch("ht3d_hrl_step", true);

@@ -51,6 +51,13 @@ class SolveError(RuntimeError):
     pass
 
 
+def _step_dict(rep: _lib.StepReport) -> dict:
+    out = {"thermal": rep.thermal.as_dict(), "darcy": rep.darcy.as_dict()}
+    for name, _ in _lib.StepReport._fields_[2:]:
+        out[name] = getattr(rep, name)
+    return out
+
+
 def _status_check(status: int, what: str) -> None:
     if status == _lib.NAN:
         raise SolveError(f"Nan {what}...")          # v18:464
@@ -145,16 +152,69 @@ class HydroThermal3D:
             _lib.check(st, "ht3d_hrl_step")
         _status_check(rep.thermal.status, "T")
         _status_check(rep.darcy.status, "P")
-        out = {"thermal": rep.thermal.as_dict(), "darcy": rep.darcy.as_dict()}
-        for name, _ in _lib.StepReport._fields_[2:]:
-            out[name] = getattr(rep, name)
-        return out
+        return _step_dict(rep)
 
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:
             for a in self._fields.values():
                 a.ptr = 0
             _lib.lib().ht3d_hrl_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HydroThermal3DBatch:
+    """Many closed-box problems stepped together (ht3d_batch_*).
+
+    `models` are HydroThermal3D instances of any sizes, Rayleigh numbers,
+    permeabilities, viscosities and time step rules; they keep their fields and
+    are stepped in lockstep.  The DYREL parameters and the precision are the
+    batch's; the solves of all problems run as one persistent launch per
+    pseudo-iteration over a packed work list, with the runs between checks
+    replayed as HIP graphs when `graphs` is true.
+    """
+
+    def __init__(self, models, thermal: DyrelParams | None = None, darcy: DyrelParams | None = None,
+                 precision: int = _lib.FP64, graphs: bool = True):
+        _lib.ensure_init()
+        self.models = list(models)
+        arr = (C.c_void_p * len(self.models))(*[m._h.value for m in self.models])
+        h = C.c_void_p()
+        _lib.call("ht3d_batch_create", arr, len(self.models), C.byref(h))
+        self._h = h
+        self.set_params(thermal, darcy, precision, graphs)
+
+    def set_params(self, thermal: DyrelParams | None = None, darcy: DyrelParams | None = None,
+                   precision: int = _lib.FP64, graphs: bool = True) -> None:
+        pT, pP = (thermal or DyrelParams()).c(), (darcy or DyrelParams()).c()
+        _lib.call("ht3d_batch_set_params", self._h, C.byref(pT), C.byref(pP), precision, int(bool(graphs)))
+
+    def step(self) -> list[dict]:
+        reps = (_lib.StepReport * len(self.models))()
+        st = _lib.lib().ht3d_batch_step(self._h, reps)
+        if st not in (_lib.OK, _lib.NOT_CONVERGED, _lib.NAN, _lib.DIVERGED):
+            _lib.check(st, "ht3d_batch_step")
+        for r in reps:
+            _status_check(r.thermal.status, "T")
+            _status_check(r.darcy.status, "P")
+        return [_step_dict(r) for r in reps]
+
+    def stats(self) -> dict:
+        s = _lib.BatchStats()
+        _lib.call("ht3d_batch_stats_get", self._h, C.byref(s))
+        return s.as_dict()
+
+    def reset_stats(self) -> None:
+        _lib.call("ht3d_batch_stats_reset", self._h)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.lib().ht3d_batch_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

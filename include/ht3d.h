@@ -309,6 +309,35 @@ int ht3d_hrl_set_dt_rule(ht3d_hrl *m, int rule);
 int ht3d_hrl_set_viscosity(ht3d_hrl *m, double a);
 int ht3d_hrl_step(ht3d_hrl *m, ht3d_step_report *rep);
 
+/* Many independent problems stepped together (no reference counterpart: the
+ * reference steps one problem per process).  A batch groups ht3d_hrl models of
+ * any sizes, Rayleigh numbers, permeabilities, viscosities and time step rules
+ * (the models stay owned by the caller) and steps them in lockstep.  The DYREL
+ * solves of all problems run as one persistent launch per pseudo-iteration
+ * over a packed (problem, tile) work list; the DYREL control runs on the
+ * device, so each run of pseudo-iterations between two checks is replayed as
+ * one HIP graph (use_graphs = 1) or launched kernel by kernel (0).  The DYREL
+ * parameters and the precision are the batch's (all problems share the check
+ * schedule); results equal those of stepping each model alone.              */
+typedef struct ht3d_batch ht3d_batch;
+typedef struct {
+    double ms_solve;            /* batched solve loops (device time, first to last block) */
+    double bytes_solve;         /* their algorithmic bytes: 8 N s per pseudo-iteration,
+                                   6 N x 8 per defect-correction anchor                   */
+    double ms_step;             /* whole batched steps (host wall clock)                  */
+    long long steps, blocks, graph_launches, graphs, repacks, iterations;
+    int tile_rows[4], tile_z[4];/* chosen tiles: thermal FP64, Darcy FP64, thermal FP32, Darcy FP32 */
+} ht3d_batch_stats;
+
+int ht3d_batch_create(ht3d_hrl *const *models, int nprob, ht3d_batch **out);
+int ht3d_batch_destroy(ht3d_batch *b);
+int ht3d_batch_set_params(ht3d_batch *b, const ht3d_dyrel_params *pT, const ht3d_dyrel_params *pP, int precision,
+                          int use_graphs);
+/* one physical step of every problem; reps: one report per problem */
+int ht3d_batch_step(ht3d_batch *b, ht3d_step_report *reps);
+int ht3d_batch_stats_get(const ht3d_batch *b, ht3d_batch_stats *s);
+int ht3d_batch_stats_reset(ht3d_batch *b);
+
 /* Lognormal permeability on vertices (spectral randomisation) plus an optional
  * planar fault; K (3*M) and phi (M) in host memory.  Set-up only.            */
 int ht3d_lognormal_field(double *kfv, int ncx, int ncy, int ncz, double xmin, double ymin, double zmin, double dx,

@@ -1,0 +1,125 @@
+"""The batched step (ht3d_batch_*) against the same models stepped one by one.
+
+Each problem keeps its own grid, Rayleigh number, permeability, viscosity,
+time step rule and WENO version; the batch shares the DYREL parameters.  The
+batched solves run the same pseudo-iteration body and the same double-double
+reductions as the single-problem solves, so iteration counts must agree and
+fields agree to round-off (bar: 1e-10 relative, the FP64 parity bar).  Graph
+replay and kernel-by-kernel launches must give identical results.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(16, 8, 16), (24, 12, 20), (20, 20, 20), (13, 9, 11), (32, 16, 24), (12, 12, 12)]
+TOL = 1e-10
+
+
+def _workloads():
+    from hydrothermal3d_amd import inputs
+    ws = inputs.batch_workloads(len(SIZES), seed=13000, steps=3)
+    return [inputs.replace(w, ncx=a, ncy=b, ncz=c, Lx=1.0, Ly=b / a, Lz=c / a, modes=64)
+            for w, (a, b, c) in zip(ws, SIZES)]
+
+
+def _models(ht, ws, p, precision):
+    from hydrothermal3d_amd import inputs
+    ms = []
+    for q, w in enumerate(ws):
+        m = ht.HydroThermal3D(w.ncx, w.ncy, w.ncz, w.Lx, w.Ly, w.Lz, w.Ra, w.cfl)
+        m["Tc_ex"].set(inputs.initial_temperature(w))
+        inputs.permeability_device(w, m["kfv"])
+        m.set_params(p, p, 2 if q == 5 else 1, precision)
+        if q == 1:
+            m.set_viscosity(math.log(10.0))
+        if q == 3:
+            m.set_dt_rule(ht._lib.DT_V7)
+        ms.append(m)
+    return ms
+
+
+def _fields(m):
+    I = (slice(1, -1),) * 3
+    P = m["Pc_ex"].get()[I]
+    return {"T": m["Tc_ex"].get()[I], "P": P - P.mean(), "Vx": m["Vx"].get(), "Vy": m["Vy"].get(),
+            "Vz": m["Vz"].get()}
+
+
+def _run(ht, precision, graphs, nsteps=3):
+    ws = _workloads()
+    p = ht.DyrelParams(nitmax=20000, nitout=100, ncheck=25, norm=1, tol=1e-9)
+    ms = _models(ht, ws, p, precision)
+    b = ht.HydroThermal3DBatch(ms, p, p, precision=precision, graphs=graphs)
+    reps = [b.step() for _ in range(nsteps)]
+    stats = b.stats()
+    out = [_fields(m) for m in ms]
+    b.close()
+    for m in ms:
+        m.close()
+    return reps, out, stats
+
+
+def _run_single(ht, precision, nsteps=3):
+    ws = _workloads()
+    p = ht.DyrelParams(nitmax=20000, nitout=100, ncheck=25, norm=1, tol=1e-9)
+    ms = _models(ht, ws, p, precision)
+    reps = [[m.step() for m in ms] for _ in range(nsteps)]
+    out = [_fields(m) for m in ms]
+    for m in ms:
+        m.close()
+    return reps, out
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32dc"])
+def test_batch_matches_single_models(ht, precision):
+    rb, fb, stats = _run(ht, precision, graphs=True)
+    rs, fs = _run_single(ht, precision)
+    for s, (stepb, steps_) in enumerate(zip(rb, rs)):
+        for q, (a, c) in enumerate(zip(stepb, steps_)):
+            for kind in ("thermal", "darcy"):
+                assert a[kind]["iters"] == c[kind]["iters"], f"step {s} problem {q} {kind}"
+                assert a[kind]["status"] == c[kind]["status"]
+                assert a[kind]["n_lambda"] == c[kind]["n_lambda"]
+            assert a["n_adv"] == c["n_adv"]
+            assert abs(a["Nu"] - c["Nu"]) <= TOL * abs(c["Nu"]), f"step {s} problem {q}"
+            assert abs(a["dt_next"] - c["dt_next"]) <= TOL * c["dt_next"]
+            if precision == 1:
+                assert a["dc_outer_darcy"] == c["dc_outer_darcy"]
+    for q, (a, c) in enumerate(zip(fb, fs)):
+        for k in a:
+            assert rel_err(a[k], c[k]) <= TOL, f"problem {q} field {k}: {rel_err(a[k], c[k])}"
+    assert stats["graphs"] > 0 and stats["graph_launches"] >= stats["graphs"]
+    assert stats["repacks"] > 0          # problems of different sizes stop at different checks
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32dc"])
+def test_batch_graphs_equal_eager(ht, precision):
+    rg, fg, sg = _run(ht, precision, graphs=True, nsteps=2)
+    re_, fe, se = _run(ht, precision, graphs=False, nsteps=2)
+    assert se["graph_launches"] == 0
+    for a, c in zip(fg, fe):
+        for k in a:
+            assert np.array_equal(a[k], c[k]), k
+    for stepg, stepe in zip(rg, re_):
+        for a, c in zip(stepg, stepe):
+            assert a["thermal"] == c["thermal"] and a["darcy"] == c["darcy"] and a["Nu"] == c["Nu"]
+
+
+def test_batch_nan_raises(ht):
+    ws = _workloads()[:3]
+    p = ht.DyrelParams(nitmax=2000, nitout=100, ncheck=25, norm=1, tol=1e-9)
+    ms = _models(ht, ws, p, 0)
+    T = ms[1]["Tc_ex"].get()
+    T[3, 3, 3] = np.nan
+    ms[1]["Tc_ex"].set(T)
+    b = ht.HydroThermal3DBatch(ms, p, p)
+    with pytest.raises(ht.SolveError, match="Nan"):
+        b.step()
+    b.close()
+    for m in ms:
+        m.close()
