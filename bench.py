@@ -86,11 +86,140 @@ def run_gpu(w, steps: int, warmup: int, precision: int, prm):
     wall = time.perf_counter() - t0
     _lib.call("ht3d_timing_enable", 0)
     st = timing_stats()
+    import ctypes as C
+    ty, kz, var = C.c_int(), C.c_int(), C.c_int()
+    _lib.call("ht3d_tune_last", C.byref(ty), C.byref(kz), C.byref(var))
+    st["tile"] = (ty.value, kz.value, var.value)
     kfv = g["kfv"].get()
     I = (slice(1, -1),) * 3
     fields = {"T": g["Tc_ex"].get()[I], "P": g["Pc_ex"].get()[I]}
     g.close()
     return wall, st, reports, kfv, fields
+
+
+def pmc_traffic(workload: str, tile) -> float | None:
+    """HBM bytes per launch of the iteration kernel for this tile and variant,
+    from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the workload
+    (calibrated against a copy kernel there); None when no summary covers it."""
+    path = ROOT / "profiles" / f"dyrel_iter_{workload}_pmc.md"
+    if not path.exists():
+        return None
+    want = f"| {tile[0]}x{tile[1]} | {tile[2]} |"
+    for line in path.read_text().splitlines():
+        if line.startswith(want):
+            return float(line.split("|")[7])
+    return None
+
+
+def _batch_models(ws, prm, precision):
+    import hydrothermal3d_amd as ht
+    from hydrothermal3d_amd import inputs
+    ms = []
+    for w in ws:
+        m = ht.HydroThermal3D(w.ncx, w.ncy, w.ncz, w.Lx, w.Ly, w.Lz, w.Ra, w.cfl)
+        m["Tc_ex"].set(inputs.initial_temperature(w))
+        inputs.permeability_device(w, m["kfv"])
+        m.set_params(prm, prm, 1, precision)
+        ms.append(m)
+    return ms
+
+
+def _batch_state(ms, reps):
+    I = (slice(1, -1),) * 3
+    out = []
+    for m, r in zip(ms, reps):
+        P = m["Pc_ex"].get()[I]
+        out.append({"T": m["Tc_ex"].get()[I], "P": P - P.mean(), "Nu": r["Nu"]})
+    return out
+
+
+def run_batch(n: int, steps: int, warmup: int, prm):
+    """Many-problem workload: plain FP64 (each model stepped alone), then the
+    batched FP64 and FP32+DC forms with graph-replayed iteration blocks, all on
+    the same inputs and the same number of steps."""
+    import numpy as np
+    import hydrothermal3d_amd as ht
+    from hydrothermal3d_amd import _lib, inputs
+
+    ws = inputs.batch_workloads(n, seed=13000, steps=steps)
+    cells = sum(w.ncell for w in ws)
+    res = {}
+    states = {}
+    for mode in ("plain_fp64", "batched_fp64", "batched_dc"):
+        precision = _lib.FP32_DC if mode == "batched_dc" else _lib.FP64
+        ms = _batch_models(ws, prm, precision)
+        b = None if mode == "plain_fp64" else ht.HydroThermal3DBatch(ms, prm, prm, precision=precision, graphs=True)
+        reps = None
+        for s in range(warmup + steps):
+            if s == warmup:
+                _lib.sync()
+                if b is not None:
+                    b.reset_stats()
+                t0 = time.perf_counter()
+            ts = time.perf_counter()
+            reps = [m.step() for m in ms] if b is None else b.step()
+            it = [(r["thermal"]["iters"], r["darcy"]["iters"]) for r in reps]
+            log(f"batch {mode} step {s}: max iters T {max(i[0] for i in it)} P {max(i[1] for i in it)} "
+                f"({time.perf_counter() - ts:.2f} s)")
+        _lib.sync()
+        wall = time.perf_counter() - t0
+        res[mode] = {"s_per_step": round(wall / steps, 4)}
+        if b is not None:
+            st = b.stats()
+            res[mode].update({"t_eff_solves_gbs": round(st["bytes_solve"] / (st["ms_solve"] / 1e3) / 1e9, 2)
+                              if st["ms_solve"] > 0 else None,
+                              "solve_share": round(st["ms_solve"] / 1e3 / wall, 3),
+                              "graph_launches": st["graph_launches"], "graphs": st["graphs"],
+                              "repacks": st["repacks"], "tiles": [f"{r}x{z}" for r, z in zip(st["tile_rows"],
+                                                                                             st["tile_z"])]})
+            b.close()
+        states[mode] = _batch_state(ms, reps)
+        for m in ms:
+            m.close()
+
+    def errs(a, c):
+        return {"T": max(rel(x["T"], y["T"]) for x, y in zip(a, c)),
+                "P": max(rel(x["P"], y["P"]) for x, y in zip(a, c)),
+                "Nu": max(abs(x["Nu"] - y["Nu"]) / abs(y["Nu"]) for x, y in zip(a, c))}
+
+    plain = res["plain_fp64"]["s_per_step"]
+    return {"workload": f"{n} problems, edge n in {{32,48,64,96,128}}, n^3 or n x n/2 x n, Ra log-uniform "
+                        f"[50,2000], sigma(log10 k) U[0,1], seeds 13000+p",
+            "problems": n, "cells": cells, "steps": steps, "warmup": warmup, **res,
+            "speedup_batched_fp64_vs_plain": round(plain / res["batched_fp64"]["s_per_step"], 3),
+            "speedup_batched_dc_vs_plain_fp64": round(plain / res["batched_dc"]["s_per_step"], 3),
+            "rel_err_batched_fp64_vs_plain": errs(states["batched_fp64"], states["plain_fp64"]),
+            "rel_err_batched_dc_vs_plain_fp64": errs(states["batched_dc"], states["plain_fp64"])}
+
+
+def probe(w, iters: int, copies: int) -> int:
+    """Counter probe for rocprofv3 --pmc: `iters` fused Darcy pseudo-iterations
+    of the workload with the autotuned tile (no check iterations after the first
+    two), then `copies` plain copies of a cell array (ArrayEqualArray, N words
+    read and N written) whose counter readings calibrate FETCH_SIZE/WRITE_SIZE."""
+    import hydrothermal3d_amd as ht
+    from hydrothermal3d_amd import _lib, inputs
+    g = ht.HydroThermal3D(w.ncx, w.ncy, w.ncz, w.Lx, w.Ly, w.Lz, w.Ra, w.cfl)
+    inputs.permeability_device(w, g["kfv"])
+    g["Tc_ex"].set(inputs.initial_temperature(w))
+    k = ht.kernels
+    k.InitDarcy(g["Ty"], g["kx"], g["ky"], g["kz"], g["Tc_ex"], g["kfv"], 1.0)
+    k.KillNorthSouthFluxes(g["Ty"])
+    k.Compute_Rp(g["Rp"], g["ky"], g["Ty"], w.Ra, 1.0 / w.h[1])
+    ht.dyrel_solve(g["Pc_ex"], g["Rp"], g["kx"], g["ky"], g["kz"], dx=w.h[0], dy=w.h[1], dz=w.h[2], dt=1.0,
+                   transient=0, rhoC=0.0, params=ht.DyrelParams(nitmax=iters, nitout=10**6, tol=0.0))
+    a = ht.DeviceArray.zeros((w.ncx, w.ncy, w.ncz))
+    b = ht.DeviceArray.zeros((w.ncx, w.ncy, w.ncz))
+    for _ in range(copies):
+        k.ArrayEqualArray(a, b)
+    _lib.sync()
+    import ctypes as C
+    ty, kz, var = C.c_int(), C.c_int(), C.c_int()
+    _lib.call("ht3d_tune_last", C.byref(ty), C.byref(kz), C.byref(var))
+    log(f"probe: {iters} pseudo-iterations on {w.name} (tile {ty.value}x{kz.value}, variant {var.value}), "
+        f"{copies} copies of {8 * w.ncell} bytes")
+    g.close()
+    return 0
 
 
 def cpu_baseline(w, iters: int, kfv=None):
@@ -136,6 +265,13 @@ def main(argv=None) -> int:
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--cpu-iters", type=int, default=600, help="pseudo-iterations of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=96, help="problems of the batched leg (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--batch-tol", type=float, default=1e-9,
+                    help="DYREL tolerance of the batched leg (1e-9 keeps the solver error of both forms below 1e-6)")
+    ap.add_argument("--batch-warmup", type=int, default=1)
+    ap.add_argument("--probe", action="store_true",
+                    help="counter probe only (fused iterations + calibration copies), no JSON line")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU halo exchange is a design "
@@ -147,6 +283,8 @@ def main(argv=None) -> int:
     from hydrothermal3d_amd import _lib, inputs
 
     w = inputs.WORKLOADS[args.workload]
+    if args.probe:
+        return probe(w, 20, 5)
     prm = ht.production_params(tol=args.tol)
     precision = _lib.FP32_DC if args.precision == "dc" else _lib.FP64
     log(f"device: {_lib.device_name()}; workload {w.name}; precision {args.precision}")
@@ -204,9 +342,16 @@ def main(argv=None) -> int:
         "iters_per_step": iters,
         "iteration_kernel_avg_us": round(avg_us, 2),
         "roofline": {"bound": "hbm", "achieved": round(teff, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(teff / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(teff / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.workload, st["tile"]),
+                     "algorithmic_bytes": 8 * 8 * w.ncell,
+                     "tile": f"{st['tile'][0]}x{st['tile'][1]} variant {st['tile'][2]}"},
         "cpu_baseline": cpu,
     }
+    if args.batch > 0:
+        log(f"batched leg: {args.batch} problems")
+        out["batch"] = run_batch(args.batch, args.batch_steps, args.batch_warmup,
+                                 ht.production_params(tol=args.batch_tol))
+        out["batch"]["tol"] = args.batch_tol
     if teff32 is not None:
         out["t_eff_fp32_inner"] = round(teff32, 2)
     if dc is not None:
