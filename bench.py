@@ -68,6 +68,7 @@ def run_gpu(w, steps: int, warmup: int, precision: int, prm):
     g["Tc_ex"].set(inputs.initial_temperature(w))
     inputs.permeability_device(w, g["kfv"])
     g.set_params(prm, prm, 1, precision)
+    g.set_viscosity(w.mu_a)
     _lib.sync()
     for s in range(warmup):
         r = g.step()
@@ -334,7 +335,7 @@ def main(argv=None) -> int:
         "vs_baseline": None,
         "dtype": "f64" if args.precision != "dc" else "f32+f64dc",
         "data": "synthetic",
-        "config": {"workload": w.name, "ncx": w.ncx, "ncy": w.ncy, "ncz": w.ncz, "Ra": w.Ra,
+        "config": {"workload": w.name, "ncx": w.ncx, "ncy": w.ncy, "ncz": w.ncz, "Ra": w.Ra, "mu_a": w.mu_a,
                    "precision": args.precision, "tol": args.tol, "norm": "max|F*D|",
                    "check_interval": prm.ncheck, "lambda_interval": prm.nitout},
         "s_per_step": round(ms_step / 1e3, 4),

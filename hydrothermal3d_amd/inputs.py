@@ -52,6 +52,7 @@ class Workload:
     fault_mult: float = 100.0
     fault_half_cells: float = 2.5
     noise: float = 0.01
+    mu_a: float = 0.0            # viscosity mu = exp(-mu_a T) in the Darcy solve (0: constant)
 
     @property
     def ncell(self) -> int:
@@ -72,8 +73,9 @@ WORKLOADS = {
                        Ra=500.0, steps=50, seed=1001, sigma=0.5),
     "hrl512_fault": Workload("hrl512_fault: 512x256x512 (2x1x2), Ra=1000, lognormal k + fault", 512, 256, 512,
                              Lx=2.0, Ly=1.0, Lz=2.0, Ra=1000.0, steps=50, seed=1002, sigma=0.5, fault=True),
-    "hrl768_fault": Workload("hrl768_fault: 768x384x768 (2x1x2), Ra=2000, lognormal k + fault", 768, 384, 768,
-                             Lx=2.0, Ly=1.0, Lz=2.0, Ra=2000.0, steps=10, seed=1004, sigma=0.5, fault=True),
+    "hrl768_fault": Workload("hrl768_fault: 768x384x768 (2x1x2), Ra=2000, mu=exp(-ln10 T), lognormal k + fault", 768,
+                             384, 768, Lx=2.0, Ly=1.0, Lz=2.0, Ra=2000.0, steps=10, seed=1004, sigma=0.5, fault=True,
+                             mu_a=math.log(10.0)),
 }
 
 

@@ -58,3 +58,46 @@ def test_hrl_steps_dc_matches_fp64(ht):
     assert rel_err(T1, T0) <= 1e-6
     assert rel_err(P1 - P1.mean(), P0 - P0.mean()) <= 1e-6
     assert np.max(np.abs(N1 - N0) / np.abs(N0)) <= 1e-6
+
+
+@pytest.mark.parametrize("name", ["hrl512_fault", "hrl768_fault"])
+def test_fault_copy_oracle_and_dc(ht, orc, name):
+    """Scaled copies of the faulted workloads (lognormal k x 100 in a 60-degree
+    slab, and for the larger one mu = exp(-ln10 T)): the FP64 GPU steps against
+    the CPU oracle (1e-10, iterations +-1), the FP32+DC steps against the FP64
+    GPU steps (1e-6 in T, P and Nu)."""
+    from hydrothermal3d_amd import inputs
+    w = inputs.WORKLOADS[name].scaled(64, 32, 64)
+    p = ht.production_params(tol=1e-9)
+    res = {}
+    for prec in (0, 1):
+        m = ht.HydroThermal3D(w.ncx, w.ncy, w.ncz, w.Lx, w.Ly, w.Lz, w.Ra, w.cfl)
+        m["Tc_ex"].set(inputs.initial_temperature(w))
+        inputs.permeability_device(w, m["kfv"])
+        m.set_viscosity(w.mu_a)
+        m.set_params(p, p, 1, prec)
+        kfv = m["kfv"].get()
+        reps = [m.step() for _ in range(2)]
+        res[prec] = (m["Tc_ex"].get()[1:-1, 1:-1, 1:-1], m["Pc_ex"].get()[1:-1, 1:-1, 1:-1],
+                     np.array([r["Nu"] for r in reps]), reps)
+        m.close()
+    o = orc.HRLModel(w.ncx, w.ncy, w.ncz, w.Lx, w.Ly, w.Lz, w.Ra, w.cfl)
+    o.set_viscosity(w.mu_a)
+    o.field("Tc_ex")[...] = inputs.initial_temperature(w)
+    o.field("kfv")[...] = kfv
+    po = orc.params(p.nitmax, p.nitout, p.ncheck, p.norm, p.tol)
+    o.set_params(po, po, 1)
+    ro = [o.step() for _ in range(2)]
+    T0, P0, N0, r0 = res[0]
+    for a, c in zip(r0, ro):
+        assert abs(a["thermal"]["iters"] - c["thermal"]["iters"]) <= 1
+        assert abs(a["darcy"]["iters"] - c["darcy"]["iters"]) <= 1
+    To, Po = o.field("Tc_ex")[1:-1, 1:-1, 1:-1], o.field("Pc_ex")[1:-1, 1:-1, 1:-1]
+    assert rel_err(T0, To) <= 1e-10
+    assert rel_err(P0 - P0.mean(), Po - Po.mean()) <= 1e-10
+    assert np.max(np.abs(N0 - np.array([r["Nu"] for r in ro])) / np.abs(N0)) <= 1e-10
+    o.close()
+    T1, P1, N1, _ = res[1]
+    assert rel_err(T1, T0) <= 1e-6
+    assert rel_err(P1 - P1.mean(), P0 - P0.mean()) <= 1e-6
+    assert np.max(np.abs(N1 - N0) / np.abs(N0)) <= 1e-6
