@@ -82,7 +82,7 @@ __global__ __launch_bounds__(64 * TY) void batch_iter_kernel(const IterArgs<T> *
         int bx, by, bz;
         tile_of(it.y, a.nbx, a.nby, bx, by, bz);
         T *x0 = const_cast<T *>(a.Xin), *x1 = a.Xout;
-        iter_tile<T, TR, false, false, TY, KZ>(a, phase ? x1 : x0, phase ? x0 : x1, flags, bx, by, bz, partials, t);
+        iter_tile<T, TR, false, false, false, TY, KZ>(a, phase ? x1 : x0, phase ? x0 : x1, flags, bx, by, bz, partials, t);
         if (flags & FL_CHECK) __syncthreads();  // the block reduction's shared array is reused by the next item
     }
 }

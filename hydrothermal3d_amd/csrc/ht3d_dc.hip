@@ -64,54 +64,6 @@ int to_float(float *out, const double *in, size_t n) {
     return HT3D_OK;
 }
 
-// FP32 inner sweeps use the recomputed-diagonal variant only
-static int launch_iter32(const IterArgs<float> &a, bool transient, int tile) {
-    const unsigned nb = (unsigned)a.nbx * a.nby * a.nbz;
-    if (transient) launch_tile<float, true, false, false>(tile, a, nb, ctx().stream);
-    else launch_tile<float, false, false, false>(tile, a, nb, ctx().stream);
-    HIP_TRY(hipGetLastError());
-    return HT3D_OK;
-}
-
-static int tune32(IterArgs<float> a, bool transient, int *tile_out) {
-    const TuneKey key{a.nx, a.ny, a.nz, transient ? 1 : 0, 1};
-    if (const TuneEntry *t = tune_find(key)) {
-        *tile_out = t->cfg.tile;
-        return HT3D_OK;
-    }
-    if (g_force_tile >= 0 || !g_autotune) {
-        *tile_out = g_force_tile >= 0 ? g_force_tile : kDefaultTile;
-        return HT3D_OK;
-    }
-    a.flags = 0;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    int best = kDefaultTile;
-    double best_us = 1e300;
-    for (int tile = 0; tile < kNumTiles; ++tile) {
-        set_tiling(a, tile);
-        int rc = launch_iter32(a, transient, tile);
-        if (rc) return rc;
-        HIP_TRY(hipEventRecord(e0, ctx().stream));
-        for (int r = 0; r < 8; ++r)
-            if ((rc = launch_iter32(a, transient, tile))) return rc;
-        HIP_TRY(hipEventRecord(e1, ctx().stream));
-        HIP_TRY(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best_us) {
-            best_us = ms;
-            best = tile;
-        }
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    g_tune.push_back(TuneEntry{key, IterCfg{best, 0}, 1e3 * best_us / 8});
-    *tile_out = best;
-    return HT3D_OK;
-}
-
 static inline bool lam_iter(int it, int nitout) { return it <= 2 || (nitout > 0 && it % nitout == 0); }
 
 int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep, double *nu_out,
@@ -134,6 +86,9 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     if ((rc = workspace(WS_F32_R, N * sizeof(float), &r32))) return rc;
     if ((rc = workspace(WS_F32_K, (nfx + nfy + nfz) * sizeof(float), &k32))) return rc;
     if ((rc = workspace(WS_DC_RES, N * sizeof(float), &fst))) return rc;
+    void *f0b = nullptr, *d32 = nullptr;
+    if ((rc = workspace(WS_F32_F0B, N * sizeof(float), &f0b))) return rc;
+    if ((rc = workspace(WS_F32_DIAG, N * sizeof(float), &d32))) return rc;
     const unsigned max_blocks = div_up(nx, 64) * div_up(ny, 1) * div_up(nz, 2);
     const unsigned res_blocks = div_up(nx, 64) * div_up(ny, 4) * nz;
     if ((rc = workspace(WS_PARTIALS, (size_t)(max_blocks > res_blocks ? max_blocks : res_blocks) * 8 * sizeof(double),
@@ -170,10 +125,20 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     a.dtau = (float)dtau;
     for (int q = 0; q < 6; ++q) { a.bct[q] = f->bc_type[q]; a.bcv[q] = 0.0f; }   // homogeneous for the correction
     a.Xin = (const float *)e0; a.Xout = (float *)e1;
-    int tile = kDefaultTile;
-    if ((rc = tune32(a, f->transient != 0, &tile))) return rc;
-    set_tiling(a, tile);
-    last_cfg() = IterCfg{tile, 0};
+    {   // the FP32 diagonal for the stored-D variants
+        unsigned nb = div_up(N, 256);
+        if (nb > 8192) nb = 8192;
+        hipLaunchKernelGGL(diag_kernel<float>, dim3(nb), dim3(256), 0, cx.stream, a, (float *)d32);
+        HIP_TRY(hipGetLastError());
+    }
+    a.Dst = (const float *)d32;
+    IterCfg cfg;
+    PairCfg pcfg;
+    if ((rc = choose_cfg(a, f->transient != 0, true, 1, (float *)f0b, &cfg, &pcfg))) return rc;
+    IterArgs<float> ap = a;   // pair tiling
+    if (pcfg.tile >= 0) set_pair_tiling(ap, pcfg.tile);
+    last_cfg() = cfg;
+    last_pair() = pcfg;
 
     // One DYREL trajectory in FP32 with FP64 re-anchoring.  The unknown is
     // x_anchor (FP64) + e (FP32); every `anchor` sweeps x_anchor += e, e = 0 and
@@ -194,7 +159,9 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     double c = 2.0 * sqrt(lmin) * p->cfact;
     double h1 = (2 - c * dtau) / (2 + c * dtau), h2 = 2 * dtau / (2 + c * dtau);
     float *buf[2] = {(float *)e0, (float *)e1};
-    int cur = 0;
+    float *f0buf[2] = {(float *)f0, (float *)f0b};
+    int cur = 0, fc = 0;
+    auto plain = [&](int it) { return !lam_iter(it, p->nitout) && !(it < p->nitmax && lam_iter(it + 1, p->nitout)); };
     for (outer = 0;; ++outer) {
         // anchor: FP64 residual, FP32 right-hand side, FP64 norm
         if (f->transient) hipLaunchKernelGGL(dc_residual_kernel<true>, rgrid, dim3(256), 0, cx.stream, d);
@@ -216,14 +183,30 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
         const int run = (p->nitmax - inner_total) < anchor ? (p->nitmax - inner_total) : anchor;
         for (int s = 0; s < run; ++s) {
             const int it = inner_total + 1;
+            if (pcfg.tile >= 0 && s + 1 < run && it + 1 < p->nitmax && plain(it) && plain(it + 1)) {
+                ap.Xin = buf[cur];   // sweeps it and it+1 in one launch
+                ap.Xout = buf[cur ^ 1];
+                ap.F0 = f0buf[fc ^ 1];
+                ap.h1 = (float)h1; ap.h2 = (float)h2;
+                timer.before_launch();
+                if ((rc = launch_pair(ap, (const float *)f0buf[fc], f->transient != 0, pcfg))) return rc;
+                timer.count += 2;
+                cur ^= 1;
+                fc ^= 1;
+                inner_total = it + 1;
+                ++s;
+                if (s == run - 1) timer.close();
+                continue;
+            }
             const bool lam = lam_iter(it, p->nitout);
             const bool store = it < p->nitmax && lam_iter(it + 1, p->nitout);
             a.Xin = buf[cur];
             a.Xout = buf[cur ^ 1];
+            a.F0 = f0buf[fc];
             a.h1 = (float)h1; a.h2 = (float)h2;
             a.flags = (lam ? (FL_CHECK | FL_LAMBDA) : 0u) | (store ? FL_STORE_F : 0u);
             timer.before_launch();
-            if ((rc = launch_iter32(a, f->transient != 0, tile))) return rc;
+            if ((rc = launch_iter(a, f->transient != 0, cfg))) return rc;
             ++timer.count;
             cur ^= 1;
             inner_total = it;

@@ -97,9 +97,15 @@ __global__ __launch_bounds__(256) void nu_sum_kernel(const double *__restrict__ 
 std::vector<TuneEntry> g_tune;
 bool g_autotune = true;
 int g_force_tile = -1, g_force_variant = 0;
+int g_pairs = 1;
 
 IterCfg &last_cfg() {
     static IterCfg c;
+    return c;
+}
+
+PairCfg &last_pair() {
+    static PairCfg c;
     return c;
 }
 
@@ -189,10 +195,11 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
     Context &cx = ctx();
     const size_t N = (size_t)nx * ny * nz, NE = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
 
-    void *xb = nullptr, *f0 = nullptr, *fst = nullptr, *part = nullptr, *res = nullptr, *dst = nullptr;
+    void *xb = nullptr, *f0 = nullptr, *f0b = nullptr, *fst = nullptr, *part = nullptr, *res = nullptr, *dst = nullptr;
     int rc;
     if ((rc = workspace(WS_XB, NE * sizeof(double), &xb))) return rc;
     if ((rc = workspace(WS_F0, N * sizeof(double), &f0))) return rc;
+    if ((rc = workspace(WS_F0B, N * sizeof(double), &f0b))) return rc;
     if ((rc = workspace(WS_FCIT, N * sizeof(double), &fst))) return rc;
     if ((rc = workspace(WS_DIAG, N * sizeof(double), &dst))) return rc;
 
@@ -223,25 +230,50 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
     a.Xin = f->X;
     a.Xout = (double *)xb;
     IterCfg cfg;
-    if ((rc = choose_cfg(a, f->transient != 0, true, 0, &cfg))) return rc;
+    PairCfg pcfg;
+    if ((rc = choose_cfg(a, f->transient != 0, true, 0, (double *)f0b, &cfg, &pcfg))) return rc;
     const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
+    IterArgs<double> ap = a;   // the same arguments with the pair tiling
+    if (pcfg.tile >= 0) set_pair_tiling(ap, pcfg.tile);
 
     HIP_TRY(hipMemsetAsync(f0, 0, N * sizeof(double), cx.stream));   // ResetA!(Fc, Fc0)
     HIP_TRY(hipMemsetAsync(fst, 0, N * sizeof(double), cx.stream));
 
     double *buf[2] = {f->X, (double *)xb};
-    int cur = 0;
+    double *f0buf[2] = {(double *)f0, (double *)f0b};
+    int cur = 0, fc = 0;
     SegmentTimer timer;
     last_cfg() = cfg;
+    last_pair() = pcfg;
     const double bytes_per_iter = 8.0 * (double)N * sizeof(double);
     double nF_abs = 0.0, nF_rel = 0.0, nF_ini = 0.0;
     int status = HT3D_NOT_CONVERGED, iter, n_checks = 0, n_lambda = 0, done_iters = 0;
+    auto plain = [&](int it) {   // no check, no stored F, no lambda update
+        return !lambda_iter(it, p->nitout) && !(p->ncheck > 0 && it % p->ncheck == 0) &&
+               !(it < p->nitmax && lambda_iter(it + 1, p->nitout));
+    };
     for (iter = 1; iter <= p->nitmax; ++iter) {
+        if (pcfg.tile >= 0 && iter + 1 < p->nitmax && plain(iter) && plain(iter + 1)) {
+            // iterations iter and iter+1 in one launch (the last iteration of a
+            // solve is always a single launch, so `prev` below is its input)
+            ap.Xin = buf[cur];
+            ap.Xout = buf[cur ^ 1];
+            ap.F0 = f0buf[fc ^ 1];
+            ap.h1 = h1; ap.h2 = h2;
+            timer.before_launch();
+            if ((rc = launch_pair(ap, (const double *)f0buf[fc], f->transient != 0, pcfg))) return rc;
+            timer.count += 2;
+            cur ^= 1;
+            fc ^= 1;
+            done_iters = ++iter;
+            continue;
+        }
         const bool lam = lambda_iter(iter, p->nitout);
         const bool chk = lam || (p->ncheck > 0 && iter % p->ncheck == 0);
         const bool store = iter < p->nitmax && lambda_iter(iter + 1, p->nitout);
         a.Xin = buf[cur];
         a.Xout = buf[cur ^ 1];
+        a.F0 = f0buf[fc];
         a.h1 = h1; a.h2 = h2;
         a.flags = (chk ? FL_CHECK : 0u) | (lam ? FL_LAMBDA : 0u) | (store ? FL_STORE_F : 0u);
         timer.before_launch();
@@ -309,7 +341,7 @@ int ht3d_dyrel_solve(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
 }
 
 int ht3d_tune_set(int autotune, int force_tile, int force_variant) {
-    if (force_tile >= ht3d::kNumTiles || force_variant < 0 || force_variant > 3)
+    if (force_tile >= ht3d::kNumTiles || force_variant < 0 || force_variant > 7)
         return ht3d::arg_fail("ht3d_tune_set: no such tile configuration or variant");
     ht3d::g_autotune = autotune != 0;
     ht3d::g_force_tile = force_tile;
@@ -331,5 +363,19 @@ int ht3d_tune_last(int *rows_per_block, int *z_per_thread, int *variant) {
 }
 
 int ht3d_tile_count(void) { return ht3d::kNumTiles; }
+
+int ht3d_tune_pairs(int mode) {
+    if (mode < 0 || mode > 2) return ht3d::arg_fail("ht3d_tune_pairs: mode must be 0, 1 or 2");
+    ht3d::g_pairs = mode;
+    return HT3D_OK;
+}
+
+int ht3d_tune_last_pair(int *rows_per_block, int *z_per_block, int *variant) {
+    const ht3d::PairCfg &c = ht3d::last_pair();
+    if (rows_per_block) *rows_per_block = c.tile >= 0 ? ht3d::kPairTiles[c.tile].ty : 0;
+    if (z_per_block) *z_per_block = c.tile >= 0 ? ht3d::kPairTiles[c.tile].kz : 0;
+    if (variant) *variant = c.tile >= 0 ? c.variant : -1;
+    return HT3D_OK;
+}
 
 }  // extern "C"

@@ -12,6 +12,13 @@
 namespace ht3d {
 
 // ------------------------------------------------------------------ context
+enum WsSlot {
+    WS_PARTIALS = 0, WS_RESULT = 1, WS_XB = 2, WS_F0 = 3, WS_FCIT = 4, WS_MISC = 5, WS_DIAG = 6,
+    WS_F32_X = 7, WS_F32_XB = 8, WS_F32_F0 = 9, WS_F32_R = 10, WS_F32_K = 11, WS_F32_D = 12, WS_DC_RES = 13,
+    WS_F0B = 14, WS_F32_F0B = 15, WS_F32_DIAG = 16,
+    WS_COUNT = 20
+};
+
 struct Context {
     bool initialised = false;
     int device = 0;
@@ -23,8 +30,8 @@ struct Context {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ht3d_timing_stats stats{};
     // grow-only device workspaces
-    void *ws[16] = {nullptr};
-    size_t ws_bytes[16] = {0};
+    void *ws[WS_COUNT] = {nullptr};
+    size_t ws_bytes[WS_COUNT] = {0};
 };
 
 Context &ctx();
@@ -46,11 +53,6 @@ struct SegmentTimer {
 // returns a device workspace of at least `bytes` for slot `slot` (grow-only)
 int workspace(int slot, size_t bytes, void **out);
 
-enum WsSlot {
-    WS_PARTIALS = 0, WS_RESULT = 1, WS_XB = 2, WS_F0 = 3, WS_FCIT = 4, WS_MISC = 5, WS_DIAG = 6,
-    WS_F32_X = 7, WS_F32_XB = 8, WS_F32_F0 = 9, WS_F32_R = 10, WS_F32_K = 11, WS_F32_D = 12, WS_DC_RES = 13,
-    WS_COUNT = 16
-};
 
 #define HT3D_CHECK_INIT()                                                     \
     do {                                                                      \

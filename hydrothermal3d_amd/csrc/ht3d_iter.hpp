@@ -1,6 +1,8 @@
-// ht3d_iter.hpp -- the fused DYREL pseudo-iteration kernel, its tile table and
-// the autotuner, shared by the FP64 solve (ht3d_dyrel.hip) and the FP32 inner
-// sweeps of the defect-correction solve (ht3d_dc.hip).
+// ht3d_iter.hpp -- the fused DYREL pseudo-iteration kernels (one iteration per
+// launch, and two per launch for runs of plain iterations), their tile tables
+// and the autotuner, shared by the FP64 solve (ht3d_dyrel.hip), the FP32 inner
+// sweeps of the defect-correction solve (ht3d_dc.hip) and the batched solves
+// (ht3d_batch.hip).
 #pragma once
 
 #include <vector>
@@ -45,6 +47,33 @@ __device__ __forceinline__ T ghost(int type, T val, T inner) {
     return type ? (T)2 * val - inner : inner;
 }
 
+// Residual of one cell from its six neighbours and face coefficients:
+// ComputeFlux! (kernels_HT3D.jl:112-114) then ResidualDiffusion(Transient)!
+// (kernels_HT3D.jl:131, HydroThermal3D_Alban_v6.jl:87), in the reference's order.
+template <typename T, bool TRANSIENT>
+__device__ __forceinline__ T cell_residual(const IterArgs<T> &a, T xc, T xW, T xE, T xS, T xN, T xB, T xF, T kW,
+                                           T kE, T kS, T kN, T kB, T kF, T r) {
+    const T qW = -kW * (a._dx * (xc - xW)), qE = -kE * (a._dx * (xE - xc));
+    const T qS = -kS * (a._dy * (xc - xS)), qN = -kN * (a._dy * (xN - xc));
+    const T qB = -kB * (a._dz * (xc - xB)), qF = -kF * (a._dz * (xF - xc));
+    if (TRANSIENT) return r + a._dt * xc + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
+    return r + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
+}
+
+// GershgorinPoisson! diagonal D of one cell (kernels_HT3D_dimensional.jl:207-221)
+template <typename T>
+__device__ __forceinline__ T cell_diag(const IterArgs<T> &a, T kW, T kE, T kS, T kN, T kB, T kF) {
+    return (kB / a.dz + kF / a.dz) / a.dz + (kN / a.dy + kS / a.dy) / a.dy + (kE / a.dx + kW / a.dx) / a.dx +
+           a.rc_tr_dt;
+}
+
+// face coefficients: cached (KT) or streamed past the caches
+template <bool KT, typename T>
+__device__ __forceinline__ T ld_k(const T *p) {
+    if (KT) return *p;
+    return __builtin_nontemporal_load(p);
+}
+
 // One fused pseudo-iteration.  Every neighbour address lies inside the E array
 // (the ghost layer is part of its allocation), so all loads are unconditional
 // and the boundary condition is applied by selecting the ghost value computed
@@ -54,7 +83,7 @@ __device__ __forceinline__ T ghost(int type, T val, T inner) {
 // plane bz*KZ; `slot` is where a check iteration writes its block partials.
 // Xin/Xout and flags come separately so the batched kernel can take them from
 // its launch while the rest of `a` lives in a per-problem table.
-template <typename T, bool TRANSIENT, bool DSTORED, bool NTX, int TY, int KZ>
+template <typename T, bool TRANSIENT, bool DSTORED, bool NTX, bool KT, int TY, int KZ>
 __device__ __forceinline__ void iter_tile(const IterArgs<T> &a, const T *__restrict__ Xin, T *__restrict__ Xout,
                                           unsigned flags, int bx, int by, int bz, double *__restrict__ partials,
                                           unsigned slot) {
@@ -86,9 +115,10 @@ __device__ __forceinline__ void iter_tile(const IterArgs<T> &a, const T *__restr
             T xS = Xin[e - sE1], xN = Xin[e + sE1];
             // streams read once per iteration: non-temporal, so the L2 keeps the
             // X planes whose neighbours the other waves of the tile still need
-            const T kW = __builtin_nontemporal_load(a.kx + fx), kE = __builtin_nontemporal_load(a.kx + fx + 1);
-            const T kS = __builtin_nontemporal_load(a.ky + fy), kN = __builtin_nontemporal_load(a.ky + fy + nx);
-            const T kF = __builtin_nontemporal_load(a.kz + c + sC2);
+            // (the KT variants let the face coefficients be cached)
+            const T kW = ld_k<KT>(a.kx + fx), kE = ld_k<KT>(a.kx + fx + 1);
+            const T kS = ld_k<KT>(a.ky + fy), kN = ld_k<KT>(a.ky + fy + nx);
+            const T kF = ld_k<KT>(a.kz + c + sC2);
             const T kB = kzB;
             const T r = __builtin_nontemporal_load(a.R + c);
             const T f0o = __builtin_nontemporal_load(a.F0 + c);
@@ -97,24 +127,10 @@ __device__ __forceinline__ void iter_tile(const IterArgs<T> &a, const T *__restr
             if (whi) xE = ghost(a.bct[1], a.bcv[1], xc);
             if (slo) xS = ghost(a.bct[2], a.bcv[2], xc);
             if (shi) xN = ghost(a.bct[3], a.bcv[3], xc);
-            // ComputeFlux! (kernels_HT3D.jl:112-114)
-            const T qW = -kW * (a._dx * (xc - xW)), qE = -kE * (a._dx * (xE - xc));
-            const T qS = -kS * (a._dy * (xc - xS)), qN = -kN * (a._dy * (xN - xc));
-            const T qB = -kB * (a._dz * (xc - xm)), qF = -kF * (a._dz * (xp - xc));
-            // ResidualDiffusion(Transient)! (kernels_HT3D.jl:131, Alban_v6:87)
-            T f;
-            if (TRANSIENT)
-                f = r + a._dt * xc + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
-            else
-                f = r + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
-            // GershgorinPoisson! diagonal D (stored once per solve, or recomputed
-            // from the face coefficients already in registers) and "F /= PC"
-            T D;
-            if (DSTORED)
-                D = a.Dst[c];
-            else
-                D = (kB / a.dz + kF / a.dz) / a.dz + (kN / a.dy + kS / a.dy) / a.dy + (kE / a.dx + kW / a.dx) / a.dx +
-                    a.rc_tr_dt;
+            T f = cell_residual<T, TRANSIENT>(a, xc, xW, xE, xS, xN, xm, xp, kW, kE, kS, kN, kB, kF, r);
+            // diagonal D (stored once per solve, or recomputed from the face
+            // coefficients already in registers) and "F /= PC"
+            const T D = DSTORED ? a.Dst[c] : cell_diag(a, kW, kE, kS, kN, kB, kF);
             f = f / D;
             // DYRELUpdate! (kernels_HT3D_dimensional.jl:323-324)
             const T f0 = a.h1 * f0o + a.h2 * f;
@@ -147,12 +163,126 @@ __device__ __forceinline__ void iter_tile(const IterArgs<T> &a, const T *__restr
     }
 }
 
-template <typename T, bool TRANSIENT, bool DSTORED, bool NTX, int TY, int KZ>
+template <typename T, bool TRANSIENT, bool DSTORED, bool NTX, bool KT, int TY, int KZ>
 __global__ __launch_bounds__(64 * TY) void dyrel_iter_kernel(IterArgs<T> a) {
     const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
     const unsigned t = xcd_tile(blockIdx.x, nblocks);
     const int bx = t % a.nbx, by = (t / a.nbx) % a.nby, bz = t / (a.nbx * a.nby);
-    iter_tile<T, TRANSIENT, DSTORED, NTX, TY, KZ>(a, a.Xin, a.Xout, a.flags, bx, by, bz, a.partials, blockIdx.x);
+    iter_tile<T, TRANSIENT, DSTORED, NTX, KT, TY, KZ>(a, a.Xin, a.Xout, a.flags, bx, by, bz, a.partials, blockIdx.x);
+}
+
+// Two pseudo-iterations per launch, for runs of plain iterations (no check, no
+// stored F, no lambda update between them).  Tiles overlap: a block of 64 x TY
+// threads computes iteration n+1 on its cells [bx*62-1, +64) x [by*(TY-2)-1, +TY)
+// and iteration n+2 on the inner 62 x (TY-2) of them.  The intermediate iterate
+// reaches its x/y neighbours through a triple-buffered LDS plane and its z
+// neighbours through a register queue, iteration n+1 running one plane ahead of
+// n+2 along the z march over the planes [bz*KZ, +KZ) (iteration n+1 also on the
+// plane before and after).  Iteration n+2 reuses the face coefficients, R and D
+// its cell loaded for n+1.  F0 is read from F0in and written to a.F0 (an
+// overlapping tile must still see the old F0).  Every operation is the single
+// iteration's in the same order: two single launches and one pair launch give
+// bit-identical X and F0.
+template <typename T, bool TRANSIENT, bool DSTORED, bool KT, int TY, int KZ>
+__global__ __launch_bounds__(64 * TY) void dyrel_pair_kernel(IterArgs<T> a, const T *__restrict__ F0in) {
+    __shared__ T L[3][TY][64];
+    const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
+    const unsigned t = xcd_tile(blockIdx.x, nblocks);
+    const int bx = t % a.nbx, by = (t / a.nbx) % a.nby, bz = t / (a.nbx * a.nby);
+    const int lx = (int)(threadIdx.x & 63u), ly = (int)(threadIdx.x >> 6);
+    const int i = bx * 62 - 1 + lx, j = by * (TY - 2) - 1 + ly;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int k0 = bz * KZ, k1 = min(k0 + KZ, nz);
+    const bool in_dom = i >= 0 && i < nx && j >= 0 && j < ny;
+    const bool inner = in_dom && lx > 0 && lx < 63 && ly > 0 && ly < TY - 1;   // computes iteration n+2
+    const int ic = min(max(i, 0), nx - 1), jc = min(max(j, 0), ny - 1);      // every load in bounds
+    const uint32_t sE1 = nx + 2, sE2 = sE1 * (ny + 2);
+    const uint32_t sC2 = (uint32_t)nx * ny, sX1 = nx + 1, sX2 = sX1 * ny, sY2 = (uint32_t)nx * (ny + 1);
+    const bool wlo = ic == 0, whi = ic == nx - 1, slo = jc == 0, shi = jc == ny - 1;
+    const int ps = max(k0 - 1, 0), pe = min(k1, nz - 1);   // planes of iteration n+1
+    uint32_t e = (uint32_t)(ic + 1) + (uint32_t)(jc + 1) * sE1 + (uint32_t)(ps + 1) * sE2;
+    uint32_t c = (uint32_t)ic + (uint32_t)jc * nx + (uint32_t)ps * sC2;
+    uint32_t fx = (uint32_t)ic + (uint32_t)jc * sX1 + (uint32_t)ps * sX2;
+    uint32_t fy = (uint32_t)ic + (uint32_t)jc * nx + (uint32_t)ps * sY2;
+    T x0c = a.Xin[e];
+    T x0m = a.Xin[e - sE2];
+    if (ps == 0) x0m = ghost(a.bct[4], a.bcv[4], x0c);
+    T kzB = a.kz[c];
+    // plane p-1 as iteration n+1 left it: the iterate, its neighbours in z, and
+    // the coefficients, R, D and F0 the n+2 update needs
+    T x1m = 0, x1c = 0, pkW = 0, pkE = 0, pkS = 0, pkN = 0, pkB = 0, pkF = 0, pr = 0, pD = 1, pf0 = 0;
+    uint32_t pe_idx = 0, pc = 0;
+    for (int p = ps; p <= pe + 1; ++p) {
+        T x1p;
+        if (p <= pe) {   // iteration n+1 on plane p
+            T xp = a.Xin[e + sE2];
+            T xW = a.Xin[e - 1], xE = a.Xin[e + 1];
+            T xS = a.Xin[e - sE1], xN = a.Xin[e + sE1];
+            const T kW = ld_k<KT>(a.kx + fx), kE = ld_k<KT>(a.kx + fx + 1);
+            const T kS = ld_k<KT>(a.ky + fy), kN = ld_k<KT>(a.ky + fy + nx);
+            const T kF = ld_k<KT>(a.kz + c + sC2);
+            const T kB = kzB;
+            const T r = __builtin_nontemporal_load(a.R + c);
+            const T f0o = __builtin_nontemporal_load(F0in + c);
+            if (p + 1 == nz) xp = ghost(a.bct[5], a.bcv[5], x0c);
+            if (wlo) xW = ghost(a.bct[0], a.bcv[0], x0c);
+            if (whi) xE = ghost(a.bct[1], a.bcv[1], x0c);
+            if (slo) xS = ghost(a.bct[2], a.bcv[2], x0c);
+            if (shi) xN = ghost(a.bct[3], a.bcv[3], x0c);
+            T f = cell_residual<T, TRANSIENT>(a, x0c, xW, xE, xS, xN, x0m, xp, kW, kE, kS, kN, kB, kF, r);
+            const T D = DSTORED ? __builtin_nontemporal_load(a.Dst + c) : cell_diag(a, kW, kE, kS, kN, kB, kF);
+            f = f / D;
+            const T f0 = a.h1 * f0o + a.h2 * f;
+            x1p = x0c - a.dtau * f0;
+            L[p % 3][ly][lx] = x1p;
+            x0m = x0c;
+            x0c = xp;
+            kzB = kF;
+            __syncthreads();
+            if (p - 1 >= k0 && inner) {   // iteration n+2 on plane p-1
+                const int q = (p - 1) % 3;
+                T yW = L[q][ly][lx - 1], yE = L[q][ly][lx + 1], yS = L[q][ly - 1][lx], yN = L[q][ly + 1][lx];
+                T yB = x1m;
+                if (p - 1 == 0) yB = ghost(a.bct[4], a.bcv[4], x1c);
+                if (wlo) yW = ghost(a.bct[0], a.bcv[0], x1c);
+                if (whi) yE = ghost(a.bct[1], a.bcv[1], x1c);
+                if (slo) yS = ghost(a.bct[2], a.bcv[2], x1c);
+                if (shi) yN = ghost(a.bct[3], a.bcv[3], x1c);
+                T g = cell_residual<T, TRANSIENT>(a, x1c, yW, yE, yS, yN, yB, x1p, pkW, pkE, pkS, pkN, pkB, pkF, pr);
+                g = g / pD;
+                const T g0 = a.h1 * pf0 + a.h2 * g;
+                __builtin_nontemporal_store(g0, a.F0 + pc);
+                a.Xout[pe_idx] = x1c - a.dtau * g0;
+            }
+            x1m = x1c;
+            x1c = x1p;
+            pkW = kW; pkE = kE; pkS = kS; pkN = kN; pkB = kB; pkF = kF; pr = r; pD = D; pf0 = f0;
+            pe_idx = e;
+            pc = c;
+            e += sE2;
+            c += sC2;
+            fx += sX2;
+            fy += sY2;
+        } else {         // past the top plane: iteration n+2 on plane nz-1 of the top chunk
+            __syncthreads();
+            if (p - 1 >= k0 && p - 1 < k1 && inner) {
+                const int q = (p - 1) % 3;
+                T yW = L[q][ly][lx - 1], yE = L[q][ly][lx + 1], yS = L[q][ly - 1][lx], yN = L[q][ly + 1][lx];
+                T yB = x1m;
+                const T yF = ghost(a.bct[5], a.bcv[5], x1c);
+                if (p - 1 == 0) yB = ghost(a.bct[4], a.bcv[4], x1c);
+                if (wlo) yW = ghost(a.bct[0], a.bcv[0], x1c);
+                if (whi) yE = ghost(a.bct[1], a.bcv[1], x1c);
+                if (slo) yS = ghost(a.bct[2], a.bcv[2], x1c);
+                if (shi) yN = ghost(a.bct[3], a.bcv[3], x1c);
+                T g = cell_residual<T, TRANSIENT>(a, x1c, yW, yE, yS, yN, yB, yF, pkW, pkE, pkS, pkN, pkB, pkF, pr);
+                g = g / pD;
+                const T g0 = a.h1 * pf0 + a.h2 * g;
+                __builtin_nontemporal_store(g0, a.F0 + pc);
+                a.Xout[pe_idx] = x1c - a.dtau * g0;
+            }
+        }
+    }
 }
 
 // ------------------------------------------------------------------ host driver
@@ -166,33 +296,34 @@ static constexpr TileCfg kTiles[] = {{2, 8}, {2, 16}, {4, 4}, {4, 8}, {4, 16}, {
 static constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 static constexpr int kDefaultTile = 2;
 
-template <typename T, bool TR, bool DS, bool NX, int TY, int KZ>
+template <typename T, bool TR, bool DS, bool NX, bool KT, int TY, int KZ>
 inline void launch_t(const IterArgs<T> &a, unsigned nb, hipStream_t st) {
-    hipLaunchKernelGGL((dyrel_iter_kernel<T, TR, DS, NX, TY, KZ>), dim3(nb), dim3(64 * TY), 0, st, a);
+    hipLaunchKernelGGL((dyrel_iter_kernel<T, TR, DS, NX, KT, TY, KZ>), dim3(nb), dim3(64 * TY), 0, st, a);
 }
 
-template <typename T, bool TR, bool DS, bool NX>
+template <typename T, bool TR, bool DS, bool NX, bool KT>
 inline void launch_tile(int tile, const IterArgs<T> &a, unsigned nb, hipStream_t st) {
     switch (tile) {
-    case 0: launch_t<T, TR, DS, NX, 2, 8>(a, nb, st); break;
-    case 1: launch_t<T, TR, DS, NX, 2, 16>(a, nb, st); break;
-    case 2: launch_t<T, TR, DS, NX, 4, 4>(a, nb, st); break;
-    case 3: launch_t<T, TR, DS, NX, 4, 8>(a, nb, st); break;
-    case 4: launch_t<T, TR, DS, NX, 4, 16>(a, nb, st); break;
-    case 5: launch_t<T, TR, DS, NX, 4, 32>(a, nb, st); break;
-    case 6: launch_t<T, TR, DS, NX, 8, 4>(a, nb, st); break;
-    case 7: launch_t<T, TR, DS, NX, 8, 8>(a, nb, st); break;
-    case 8: launch_t<T, TR, DS, NX, 8, 16>(a, nb, st); break;
-    case 9: launch_t<T, TR, DS, NX, 4, 2>(a, nb, st); break;
-    case 10: launch_t<T, TR, DS, NX, 8, 2>(a, nb, st); break;
-    case 11: launch_t<T, TR, DS, NX, 16, 2>(a, nb, st); break;
-    case 12: launch_t<T, TR, DS, NX, 16, 4>(a, nb, st); break;
-    default: launch_t<T, TR, DS, NX, 1, 8>(a, nb, st); break;
+    case 0: launch_t<T, TR, DS, NX, KT, 2, 8>(a, nb, st); break;
+    case 1: launch_t<T, TR, DS, NX, KT, 2, 16>(a, nb, st); break;
+    case 2: launch_t<T, TR, DS, NX, KT, 4, 4>(a, nb, st); break;
+    case 3: launch_t<T, TR, DS, NX, KT, 4, 8>(a, nb, st); break;
+    case 4: launch_t<T, TR, DS, NX, KT, 4, 16>(a, nb, st); break;
+    case 5: launch_t<T, TR, DS, NX, KT, 4, 32>(a, nb, st); break;
+    case 6: launch_t<T, TR, DS, NX, KT, 8, 4>(a, nb, st); break;
+    case 7: launch_t<T, TR, DS, NX, KT, 8, 8>(a, nb, st); break;
+    case 8: launch_t<T, TR, DS, NX, KT, 8, 16>(a, nb, st); break;
+    case 9: launch_t<T, TR, DS, NX, KT, 4, 2>(a, nb, st); break;
+    case 10: launch_t<T, TR, DS, NX, KT, 8, 2>(a, nb, st); break;
+    case 11: launch_t<T, TR, DS, NX, KT, 16, 2>(a, nb, st); break;
+    case 12: launch_t<T, TR, DS, NX, KT, 16, 4>(a, nb, st); break;
+    default: launch_t<T, TR, DS, NX, KT, 1, 8>(a, nb, st); break;
     }
 }
 
 // variant bits: 1 = read the stored diagonal instead of recomputing it,
-//               2 = non-temporal store of the updated unknown
+//               2 = non-temporal store of the updated unknown,
+//               4 = cached (not streamed) face-coefficient loads
 struct IterCfg {
     int tile = kDefaultTile;
     int variant = 0;
@@ -207,11 +338,15 @@ inline void set_tiling(IterArgs<T> &a, int tile) {
 
 template <typename T, bool TR>
 inline void launch_variant(const IterArgs<T> &a, const IterCfg &cfg, unsigned nb, hipStream_t st) {
-    switch (cfg.variant & 3) {
-    case 0: launch_tile<T, TR, false, false>(cfg.tile, a, nb, st); break;
-    case 1: launch_tile<T, TR, true, false>(cfg.tile, a, nb, st); break;
-    case 2: launch_tile<T, TR, false, true>(cfg.tile, a, nb, st); break;
-    default: launch_tile<T, TR, true, true>(cfg.tile, a, nb, st); break;
+    switch (cfg.variant & 7) {
+    case 0: launch_tile<T, TR, false, false, false>(cfg.tile, a, nb, st); break;
+    case 1: launch_tile<T, TR, true, false, false>(cfg.tile, a, nb, st); break;
+    case 2: launch_tile<T, TR, false, true, false>(cfg.tile, a, nb, st); break;
+    case 3: launch_tile<T, TR, true, true, false>(cfg.tile, a, nb, st); break;
+    case 4: launch_tile<T, TR, false, false, true>(cfg.tile, a, nb, st); break;
+    case 5: launch_tile<T, TR, true, false, true>(cfg.tile, a, nb, st); break;
+    case 6: launch_tile<T, TR, false, true, true>(cfg.tile, a, nb, st); break;
+    default: launch_tile<T, TR, true, true, true>(cfg.tile, a, nb, st); break;
     }
 }
 
@@ -226,6 +361,69 @@ inline int launch_iter(const IterArgs<T> &a, bool transient, const IterCfg &cfg)
     return HT3D_OK;
 }
 
+// --------------------------------------------------------- two-iteration launches
+// Pair tiles (rows of the overlapped block, planes per block); variant bits:
+// 1 = stored diagonal, 2 = cached face-coefficient loads.  tile < 0: no pairs.
+static constexpr TileCfg kPairTiles[] = {{8, 16}, {8, 32}, {16, 16}, {16, 32}, {16, 64}};
+static constexpr int kNumPairTiles = sizeof(kPairTiles) / sizeof(kPairTiles[0]);
+
+struct PairCfg {
+    int tile = -1;
+    int variant = 0;
+};
+
+template <typename T>
+inline void set_pair_tiling(IterArgs<T> &a, int tile) {
+    a.nbx = (int)div_up(a.nx, 62);
+    a.nby = (int)div_up(a.ny, kPairTiles[tile].ty - 2);
+    a.nbz = (int)div_up(a.nz, kPairTiles[tile].kz);
+}
+
+template <typename T, bool TR, bool DS, bool KT>
+inline void launch_pair_tile(int tile, const IterArgs<T> &a, const T *F0in, unsigned nb, hipStream_t st) {
+#define PL_(TY, KZ) hipLaunchKernelGGL((dyrel_pair_kernel<T, TR, DS, KT, TY, KZ>), dim3(nb), dim3(64 * TY), 0, st, a, F0in)
+    switch (tile) {
+    case 0: PL_(8, 16); break;
+    case 1: PL_(8, 32); break;
+    case 2: PL_(16, 16); break;
+    case 3: PL_(16, 32); break;
+    default: PL_(16, 64); break;
+    }
+#undef PL_
+}
+
+// a: pair tiling set, a.F0 = the F0 buffer written, F0in = the one read
+template <typename T>
+inline int launch_pair(const IterArgs<T> &a, const T *F0in, bool transient, const PairCfg &pc) {
+    const unsigned nb = (unsigned)a.nbx * a.nby * a.nbz;
+    hipStream_t st = ctx().stream;
+    const bool ds = pc.variant & 1, kt = pc.variant & 2;
+    if (transient) {
+        if (ds) { if (kt) launch_pair_tile<T, true, true, true>(pc.tile, a, F0in, nb, st); else launch_pair_tile<T, true, true, false>(pc.tile, a, F0in, nb, st); }
+        else { if (kt) launch_pair_tile<T, true, false, true>(pc.tile, a, F0in, nb, st); else launch_pair_tile<T, true, false, false>(pc.tile, a, F0in, nb, st); }
+    } else {
+        if (ds) { if (kt) launch_pair_tile<T, false, true, true>(pc.tile, a, F0in, nb, st); else launch_pair_tile<T, false, true, false>(pc.tile, a, F0in, nb, st); }
+        else { if (kt) launch_pair_tile<T, false, false, true>(pc.tile, a, F0in, nb, st); else launch_pair_tile<T, false, false, false>(pc.tile, a, F0in, nb, st); }
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "dyrel_pair_kernel");
+    return HT3D_OK;
+}
+
+// Gershgorin diagonal of every cell in T, with the formula and order the
+// iteration kernels use (so the stored-D variants stay bit-identical)
+template <typename T>
+__global__ __launch_bounds__(256) void diag_kernel(IterArgs<T> a, T *__restrict__ D) {
+    const uint32_t nx = a.nx, ny = a.ny;
+    const size_t N = (size_t)nx * ny * a.nz;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < N; c += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t i = (uint32_t)(c % nx), j = (uint32_t)((c / nx) % ny), k = (uint32_t)(c / ((size_t)nx * ny));
+        const size_t fx = i + (size_t)j * (nx + 1) + (size_t)k * (nx + 1) * ny;
+        const size_t fy = i + (size_t)j * nx + (size_t)k * nx * (ny + 1);
+        D[c] = cell_diag(a, a.kx[fx], a.kx[fx + 1], a.ky[fy], a.ky[fy + nx], a.kz[c], a.kz[c + (size_t)nx * ny]);
+    }
+}
+
 // ---------------------------------------------------------------- autotuner
 struct TuneKey {
     int nx, ny, nz, transient, prec;
@@ -237,10 +435,13 @@ struct TuneEntry {
     TuneKey key;
     IterCfg cfg;
     double us;
+    PairCfg pair;       // fastest pair configuration (timed per iteration)
+    double pair_us;
 };
 extern std::vector<TuneEntry> g_tune;
 extern bool g_autotune;
 extern int g_force_tile, g_force_variant;
+extern int g_pairs;     // 0: never, 1: when the tuner finds them faster, 2: always (best pair tile)
 
 inline const TuneEntry *tune_find(const TuneKey &k) {
     for (const auto &t : g_tune)
@@ -248,69 +449,102 @@ inline const TuneEntry *tune_find(const TuneKey &k) {
     return nullptr;
 }
 
-// Times every tile configuration (both D modes) on the solve's own read-only
-// inputs, writing only scratch outputs (Xout, F0), and keeps the fastest.
+// Times every single-iteration tile and variant, and every pair tile and
+// variant, on the solve's own read-only inputs, writing only scratch outputs
+// (Xout, F0, F0b), and keeps the fastest of each.  Dst (when can_store_d) must
+// hold the diagonal; F0b (when not null) is a second F0 buffer for the pairs.
 template <typename T>
-inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, IterCfg *out) {
+inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T *F0b, TuneEntry *out) {
     const TuneKey key{a.nx, a.ny, a.nz, transient ? 1 : 0, prec};
     if (const TuneEntry *t = tune_find(key)) {
-        *out = t->cfg;
-        if (!can_store_d) out->variant &= ~1;
+        *out = *t;
         return HT3D_OK;
     }
     Context &cx = ctx();
     a.flags = 0;
-    IterCfg best;
-    double best_us = 1e300;
+    TuneEntry te;
+    te.key = key;
+    te.us = 1e300;
+    te.pair_us = 1e300;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     const int reps = 8;
-    for (int var = 0; var < 4; ++var) {
+    auto timed = [&](auto &&launch, double *us) -> int {
+        int rc = launch();
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(e0, cx.stream));
+        for (int r = 0; r < reps; ++r)
+            if ((rc = launch())) return rc;
+        HIP_TRY(hipEventRecord(e1, cx.stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        *us = 1e3 * ms / reps;
+        return HT3D_OK;
+    };
+    for (int var = 0; var < 8; ++var) {
         if ((var & 1) && !can_store_d) continue;
         for (int tile = 0; tile < kNumTiles; ++tile) {
             IterCfg cfg{tile, var};
             set_tiling(a, tile);
-            int rc = launch_iter(a, transient, cfg);
+            double us = 0;
+            int rc = timed([&] { return launch_iter(a, transient, cfg); }, &us);
             if (rc) return rc;
-            HIP_TRY(hipEventRecord(e0, cx.stream));
-            for (int r = 0; r < reps; ++r)
-                if ((rc = launch_iter(a, transient, cfg))) return rc;
-            HIP_TRY(hipEventRecord(e1, cx.stream));
-            HIP_TRY(hipEventSynchronize(e1));
-            float ms = 0.f;
-            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-            const double us = 1e3 * ms / reps;
-            if (us < best_us) {
-                best_us = us;
-                best = cfg;
+            if (us < te.us) { te.us = us; te.cfg = cfg; }
+        }
+    }
+    if (F0b && a.ny >= 3) {
+        for (int var = 0; var < 4; ++var) {
+            if ((var & 1) && !can_store_d) continue;
+            for (int tile = 0; tile < kNumPairTiles; ++tile) {
+                PairCfg pc{tile, var};
+                IterArgs<T> b = a;
+                b.F0 = F0b;
+                set_pair_tiling(b, tile);
+                double us = 0;
+                int rc = timed([&] { return launch_pair(b, (const T *)a.F0, transient, pc); }, &us);
+                if (rc) return rc;
+                us *= 0.5;   // per iteration
+                if (us < te.pair_us) { te.pair_us = us; te.pair = pc; }
             }
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    g_tune.push_back(TuneEntry{key, best, best_us});
-    *out = best;
+    g_tune.push_back(te);
+    *out = te;
     return HT3D_OK;
 }
 
 IterCfg &last_cfg();
+PairCfg &last_pair();
 
+// The single-iteration configuration (its tiling set on `a`) and the pair
+// configuration to use (tile < 0: none).
 template <typename T>
-inline int choose_cfg(IterArgs<T> &a, bool transient, bool can_store_d, int prec, IterCfg *cfg) {
+inline int choose_cfg(IterArgs<T> &a, bool transient, bool can_store_d, int prec, T *F0b, IterCfg *cfg,
+                      PairCfg *pair) {
     IterCfg c;
+    PairCfg pc;
     if (g_force_tile >= 0) {
         c.tile = g_force_tile;
-        c.variant = g_force_variant & (can_store_d ? 3 : 2);
+        c.variant = g_force_variant & (can_store_d ? 7 : 6);
     } else if (g_autotune) {
-        int rc = autotune(a, transient, can_store_d, prec, &c);
+        TuneEntry te;
+        int rc = autotune(a, transient, can_store_d, prec, F0b, &te);
         if (rc) return rc;
+        c = te.cfg;
+        if (!can_store_d) c.variant &= ~1;
+        if (te.pair.tile >= 0 && (g_pairs == 2 || (g_pairs == 1 && te.pair_us < te.us))) pc = te.pair;
+        if (!can_store_d) pc.variant &= ~1;
     }
+    if (!F0b || g_pairs == 0) pc.tile = -1;
     set_tiling(a, c.tile);
     *cfg = c;
+    *pair = pc;
     return HT3D_OK;
 }
-
 
 // ------------------------------------------------------------ defect correction
 // FP64 residual of the dimensionless operators at one cell, with on-the-fly

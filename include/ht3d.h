@@ -86,7 +86,8 @@ int ht3d_timing_reset(void);
  * cells per block, z cells marched per thread) from a fixed table plus a
  * variant: bit 0 reads the stored Gershgorin diagonal instead of recomputing it
  * from the face coefficients (bit-identical either way), bit 1 writes the
- * updated unknown with a non-temporal store.  With autotuning on (default) the
+ * updated unknown with a non-temporal store, bit 2 lets the face-coefficient
+ * loads be cached.  With autotuning on (default) the
  * first solve of each (grid, operator, precision) times every configuration on
  * its own inputs and caches the fastest; force_tile >= 0 pins a table entry and
  * variant.  No reference counterpart (ParallelStencil takes explicit
@@ -95,6 +96,13 @@ int ht3d_tune_set(int autotune, int force_tile, int force_variant);
 int ht3d_tune_clear(void);
 int ht3d_tune_last(int *rows_per_block, int *z_per_thread, int *variant);
 int ht3d_tile_count(void);
+/* Two pseudo-iterations per launch for runs of plain iterations (overlapped
+ * tiles, the intermediate iterate kept on chip; bit-identical to two single
+ * launches).  mode 0: never, 1: when the tuner times them faster per iteration
+ * (default), 2: always.  ht3d_tune_last_pair reports the last solve's pair
+ * tile (0 rows when none was used).                                          */
+int ht3d_tune_pairs(int mode);
+int ht3d_tune_last_pair(int *rows_per_block, int *z_per_block, int *variant);
 
 /* maximum_g / minimum_g / sum_g / mean_g of abs.(A) (tools/Macros.jl:20-25);
  * the result lands in host memory.  Deterministic order.                    */
