@@ -91,6 +91,8 @@ def run_gpu(w, steps: int, warmup: int, precision: int, prm):
     ty, kz, var = C.c_int(), C.c_int(), C.c_int()
     _lib.call("ht3d_tune_last", C.byref(ty), C.byref(kz), C.byref(var))
     st["tile"] = (ty.value, kz.value, var.value)
+    _lib.call("ht3d_tune_last_pair", C.byref(ty), C.byref(kz), C.byref(var))
+    st["pair"] = (ty.value, kz.value, var.value) if ty.value > 0 else None
     kfv = g["kfv"].get()
     I = (slice(1, -1),) * 3
     fields = {"T": g["Tc_ex"].get()[I], "P": g["Pc_ex"].get()[I]}
@@ -99,16 +101,17 @@ def run_gpu(w, steps: int, warmup: int, precision: int, prm):
 
 
 def pmc_traffic(workload: str, tile) -> float | None:
-    """HBM bytes per launch of the iteration kernel for this tile and variant,
-    from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the workload
-    (calibrated against a copy kernel there); None when no summary covers it."""
+    """HBM bytes per launch of the single-iteration kernel for this tile and
+    variant, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the
+    workload (calibrated against a copy kernel there); None when no summary
+    covers it."""
     path = ROOT / "profiles" / f"dyrel_iter_{workload}_pmc.md"
     if not path.exists():
         return None
-    want = f"| {tile[0]}x{tile[1]} | {tile[2]} |"
+    want = f"| single | {tile[0]}x{tile[1]} | {tile[2]} |"
     for line in path.read_text().splitlines():
         if line.startswith(want):
-            return float(line.split("|")[7])
+            return float(line.split("|")[9])
     return None
 
 
@@ -217,7 +220,10 @@ def probe(w, iters: int, copies: int) -> int:
     import ctypes as C
     ty, kz, var = C.c_int(), C.c_int(), C.c_int()
     _lib.call("ht3d_tune_last", C.byref(ty), C.byref(kz), C.byref(var))
-    log(f"probe: {iters} pseudo-iterations on {w.name} (tile {ty.value}x{kz.value}, variant {var.value}), "
+    single = f"{ty.value}x{kz.value} variant {var.value}"
+    _lib.call("ht3d_tune_last_pair", C.byref(ty), C.byref(kz), C.byref(var))
+    pair = f"{ty.value}x{kz.value} variant {var.value}" if ty.value > 0 else "none"
+    log(f"probe: {iters} pseudo-iterations on {w.name} (single tile {single}, pair tile {pair}), "
         f"{copies} copies of {8 * w.ncell} bytes")
     g.close()
     return 0
@@ -345,7 +351,9 @@ def main(argv=None) -> int:
         "roofline": {"bound": "hbm", "achieved": round(teff, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(teff / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.workload, st["tile"]),
                      "algorithmic_bytes": 8 * 8 * w.ncell,
-                     "tile": f"{st['tile'][0]}x{st['tile'][1]} variant {st['tile'][2]}"},
+                     "tile": f"{st['tile'][0]}x{st['tile'][1]} variant {st['tile'][2]}",
+                     "pair_tile": None if st["pair"] is None else
+                     f"{st['pair'][0]}x{st['pair'][1]} variant {st['pair'][2]}"},
         "cpu_baseline": cpu,
     }
     if args.batch > 0:
