@@ -260,6 +260,48 @@ def cpu_baseline(w, iters: int, kfv=None):
     return nbytes / dt / 1e9, dt
 
 
+def run_config0(steps: int, tol: float):
+    """The reference's CPU-runnable configuration in full: 32^3 cells, Ra=200,
+    k=1, `steps` physical steps, max-norm tolerance `tol`, on the CPU oracle
+    (unfused reference kernel sequence, OpenMP) and on the GPU, with the GPU
+    result checked against the CPU one."""
+    import numpy as np
+    import hydrothermal3d_amd as ht
+    from hydrothermal3d_amd import _lib, inputs
+    from oracle import oracle as orc
+    w = inputs.WORKLOADS["hrl32"]
+    T0 = inputs.initial_temperature(w)
+    prm = ht.production_params(tol=tol)
+    g = ht.HydroThermal3D(w.ncx, w.ncy, w.ncz, w.Lx, w.Ly, w.Lz, w.Ra, w.cfl)
+    g["Tc_ex"].set(T0)
+    g.set_params(prm, prm, 1, _lib.FP64)
+    _lib.sync()
+    t0 = time.perf_counter()
+    rg = [g.step() for _ in range(steps)]
+    _lib.sync()
+    gpu_s = time.perf_counter() - t0
+    o = orc.HRLModel(w.ncx, w.ncy, w.ncz, w.Lx, w.Ly, w.Lz, w.Ra, w.cfl)
+    o.field("Tc_ex")[...] = T0
+    po = orc.params(prm.nitmax, prm.nitout, prm.ncheck, prm.norm, prm.tol)
+    o.set_params(po, po, 1)
+    t0 = time.perf_counter()
+    ro = [o.step() for _ in range(steps)]
+    cpu_s = time.perf_counter() - t0
+    I = (slice(1, -1),) * 3
+    Tg, To = g["Tc_ex"].get()[I], o.field("Tc_ex")[I]
+    out = {"workload": w.name, "steps": steps, "tol": tol,
+           "cpu_s_per_step": round(cpu_s / steps, 4), "gpu_s_per_step": round(gpu_s / steps, 5),
+           "gpu_vs_cpu_speedup": round(cpu_s / gpu_s, 1),
+           "iters_equal": all(a["thermal"]["iters"] == c["thermal"]["iters"] and
+                              a["darcy"]["iters"] == c["darcy"]["iters"] for a, c in zip(rg, ro)),
+           "rel_err_T_gpu_vs_cpu": rel(Tg, To),
+           "rel_err_Nu_gpu_vs_cpu": max(abs(a["Nu"] - c["Nu"]) / abs(c["Nu"]) for a, c in zip(rg, ro)),
+           "Nu_last": rg[-1]["Nu"]}
+    g.close()
+    o.close()
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,6 +314,8 @@ def main(argv=None) -> int:
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--cpu-iters", type=int, default=600, help="pseudo-iterations of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config0-steps", type=int, default=20,
+                    help="physical steps of the 32^3 configuration run in full on CPU and GPU (0: skip)")
     ap.add_argument("--batch", type=int, default=96, help="problems of the batched leg (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--batch-tol", type=float, default=1e-9,
@@ -356,6 +400,11 @@ def main(argv=None) -> int:
                      f"{st['pair'][0]}x{st['pair'][1]} variant {st['pair'][2]}"},
         "cpu_baseline": cpu,
     }
+    if not args.no_cpu and args.config0_steps > 0:
+        log(f"32^3 configuration in full on the CPU oracle and the GPU ({args.config0_steps} steps)")
+        c0 = run_config0(args.config0_steps, args.tol)
+        c0["cores"] = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["config_32cube"] = c0
     if args.batch > 0:
         log(f"batched leg: {args.batch} problems")
         out["batch"] = run_batch(args.batch, args.batch_steps, args.batch_warmup,

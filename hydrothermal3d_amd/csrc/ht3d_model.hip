@@ -168,6 +168,68 @@ __global__ __launch_bounds__(256) void lognormal_kernel(double *__restrict__ kfv
     kfv[((size_t)k * (ny + 1) + j) * (nx + 1) + i] = v;
 }
 
+// Init_vel! (kernels_HT3D.jl:73-80) over the three face arrays in one pass over
+// the vertex-shaped index space, fused with max|V| (for the advective time step)
+// and, when T is given, the top-face Nusselt sum of HydroThermal3D_Alban_v6.jl:345-346
+// (flux -ky (T(ny+1) - T(ny)) / dy of an E array whose ghosts are filled).
+// Slots: 0 max|V|, 1 Nu sum (double-double).
+__global__ __launch_bounds__(256) void velocity_vmax_kernel(
+    double *__restrict__ Vx, double *__restrict__ Vy, double *__restrict__ Vz, const double *__restrict__ kx,
+    const double *__restrict__ ky, const double *__restrict__ kz, const double *__restrict__ P,
+    const double *__restrict__ Ty, double Ra, double _dx, double _dy, double _dz, const double *__restrict__ T,
+    const double *__restrict__ kt_y, int nx, int ny, int nz, double *__restrict__ partials) {
+    const Grid g{nx, ny, nz};
+    const size_t n1 = (size_t)nx + 1, n12 = n1 * (ny + 1), n = n12 * (nz + 1);
+    double vmax = 0.0, s = 0.0, sl = 0.0;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % n1), j = (int)((t / n1) % (ny + 1)), k = (int)(t / n12);
+        if (j < ny && k < nz) {
+            const double v = -kx[g.fx(i, j, k)] * _dx * (P[g.e(i + 1, j + 1, k + 1)] - P[g.e(i, j + 1, k + 1)]);
+            Vx[g.fx(i, j, k)] = v;
+            vmax = (fabs(v) > vmax || v != v) ? fabs(v) : vmax;
+        }
+        if (i < nx && k < nz) {
+            const double v = -ky[g.fy(i, j, k)] * (_dy * (P[g.e(i + 1, j + 1, k + 1)] - P[g.e(i + 1, j, k + 1)]) -
+                                                   Ra * Ty[g.fy(i, j, k)]);
+            Vy[g.fy(i, j, k)] = v;
+            vmax = (fabs(v) > vmax || v != v) ? fabs(v) : vmax;
+            if (T && j == ny) {
+                const double a = T[g.e(i + 1, ny + 1, k + 1)], b = T[g.e(i + 1, ny, k + 1)];
+                dd_add(s, sl, -kt_y[g.fy(i, ny, k)] * (_dy * (a - b)));
+            }
+        }
+        if (i < nx && j < ny) {
+            const double v = -kz[g.fz(i, j, k)] * _dz * (P[g.e(i + 1, j + 1, k + 1)] - P[g.e(i + 1, j + 1, k)]);
+            Vz[g.fz(i, j, k)] = v;
+            vmax = (fabs(v) > vmax || v != v) ? fabs(v) : vmax;
+        }
+    }
+    const double h4[4] = {vmax, s, 0.0, 0.0}, l4[4] = {0.0, sl, 0.0, 0.0};
+    block_reduce4(h4, l4, 0x1u, partials, blockIdx.x);
+}
+
+// V, max|V| and (T given) the Nu sum in one pass and one read-back
+int velocity_vmax(double *Vx, double *Vy, double *Vz, const double *kx, const double *ky, const double *kz,
+                  const double *P, const double *Ty, double Ra, double _dx, double _dy, double _dz, const double *T,
+                  const double *kt_y, int nx, int ny, int nz, double *vmax, double *nusum) {
+    const size_t n = (size_t)(nx + 1) * (ny + 1) * (nz + 1);
+    unsigned nb = div_up(n, 256);
+    if (nb > 4096) nb = 4096;
+    void *part = nullptr, *res = nullptr;
+    int rc;
+    if ((rc = workspace(WS_MISC, (size_t)nb * 8 * sizeof(double), &part))) return rc;
+    if ((rc = workspace(WS_RESULT, 64 * sizeof(double), &res))) return rc;
+    hipLaunchKernelGGL(velocity_vmax_kernel, dim3(nb), dim3(256), 0, ctx().stream, Vx, Vy, Vz, kx, ky, kz, P, Ty, Ra,
+                       _dx, _dy, _dz, T, kt_y, nx, ny, nz, (double *)part);
+    HIP_TRY(hipGetLastError());
+    if ((rc = finish_partials((double *)part, nb, 0x1u, (double *)res))) return rc;
+    double r4[4];
+    if ((rc = read_result4((double *)res, r4))) return rc;
+    *vmax = r4[0];
+    if (nusum) *nusum = r4[1];
+    return HT3D_OK;
+}
+
 __global__ __launch_bounds__(256) void fill_kernel(double *__restrict__ a, size_t n, double v) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = v;
 }
@@ -215,13 +277,10 @@ int hrl_advect(ht3d_hrl *m, ht3d_step_report *r) {
     m->time += dt;
     // velocity, advective time step from this velocity, advection
     // (HydroThermal3D_v18.jl:551-570, HydroThermal3D_v7.jl:215-221)
-    if ((rc = ht3d_Init_vel(m->Vx, m->Vy, m->Vz, m->kx, m->ky, m->kz, m->Pc_ex, m->Ty, m->Ra, _dx, _dy, _dz, nx, ny, nz)))
+    double vmax = 0.0;
+    if ((rc = velocity_vmax(m->Vx, m->Vy, m->Vz, m->kx, m->ky, m->kz, m->Pc_ex, m->Ty, m->Ra, _dx, _dy, _dz, nullptr,
+                            nullptr, nx, ny, nz, &vmax, nullptr)))
         return rc;
-    double vx = 0, vy = 0, vz = 0;
-    if ((rc = ht3d_reduce(&vx, m->Vx, (size_t)(nx + 1) * ny * nz, HT3D_MAX_ABS))) return rc;
-    if ((rc = ht3d_reduce(&vy, m->Vy, (size_t)nx * (ny + 1) * nz, HT3D_MAX_ABS))) return rc;
-    if ((rc = ht3d_reduce(&vz, m->Vz, (size_t)nx * ny * (nz + 1), HT3D_MAX_ABS))) return rc;
-    const double vmax = fmax(fmax(vx, vy), vz);
     const double h = fmin(fmin(m->dx, m->dy), m->dz);
     const double dt_adv = m->cfl * h / vmax;
     int n_adv = 1;
@@ -284,6 +343,25 @@ int ht3d_lognormal_field(double *kfv, int nx, int ny, int nz, double xmin, doubl
     (void)hipFree(dK);
     (void)hipFree(dphi);
     return rc;
+}
+
+int ht3d_velocity_dt_nu(double *Vx, double *Vy, double *Vz, const double *kx, const double *ky, const double *kz,
+                        const double *Pc_ex, const double *Ty, double Ra, double dx, double dy, double dz, double cfl,
+                        const double *Tc_ex, const double *kty, double Lx, double Lz, double *out, int nx, int ny,
+                        int nz) {
+    HT3D_CHECK_INIT();
+    if (!grid_ok(nx, ny, nz)) return arg_fail("ht3d_velocity_dt_nu: extents must be >= 3");
+    if (!Vx || !Vy || !Vz || !kx || !ky || !kz || !Pc_ex || !Ty || !out || (Tc_ex && !kty))
+        return arg_fail("ht3d_velocity_dt_nu: null pointer");
+    if (!(dx > 0) || !(dy > 0) || !(dz > 0) || !(cfl > 0)) return arg_fail("ht3d_velocity_dt_nu: bad spacing or cfl");
+    double vmax = 0.0, nusum = 0.0;
+    int rc = velocity_vmax(Vx, Vy, Vz, kx, ky, kz, Pc_ex, Ty, Ra, 1.0 / dx, 1.0 / dy, 1.0 / dz, Tc_ex, kty, nx, ny, nz,
+                           &vmax, &nusum);
+    if (rc) return rc;
+    out[0] = vmax;
+    out[1] = cfl * fmin(fmin(dx, dy), dz) / vmax;                       // HydroThermal3D_v18.jl:554 rule
+    out[2] = Tc_ex ? nusum / (Lx * Lz) / (double)(nx * nz) : 0.0;      // HydroThermal3D_Alban_v6.jl:346
+    return after_launch("ht3d_velocity_dt_nu");
 }
 
 int ht3d_hrl_create(int nx, int ny, int nz, double Lx, double Ly, double Lz, double Ra, double cfl, ht3d_hrl **out) {

@@ -285,4 +285,27 @@ def AdvectWithWeno5(Tc_ex, Vx, Vy, Vz, dt, _dx, _dy, _dz, Ttop, Tbot):
               1, *grid)
 
 
-__all__ = [n for n in dir() if not n.startswith("_") and n[0].isupper()]
+def velocity_dt_nu(Vx, Vy, Vz, kx, ky, kz, Pc_ex, Ty, Ra, dx, dy, dz, cfl, Tc_ex=None, kty=None, Lx=1.0, Lz=1.0):
+    """Init_vel! (kernels_HT3D.jl:73-80), max|V|, the advective time step
+    cfl*min(h)/max|V| (HydroThermal3D_v18.jl:554) and, when Tc_ex and kty are
+    given, Nu of HydroThermal3D_Alban_v6.jl:345-346, in one pass.  Returns
+    (vmax, dt, Nu)."""
+    grid = _grid_from(Pc_ex, "E")
+    for nm, a, k in (("Vx", Vx, "Fx"), ("Vy", Vy, "Fy"), ("Vz", Vz, "Fz"), ("kx", kx, "Fx"), ("ky", ky, "Fy"),
+                     ("kz", kz, "Fz"), ("Ty", Ty, "Fy")):
+        _check(a, k, grid, "velocity_dt_nu", nm)
+    if (Tc_ex is None) != (kty is None):
+        raise ValueError("velocity_dt_nu: give both Tc_ex and kty, or neither")
+    if Tc_ex is not None:
+        _check(Tc_ex, "E", grid, "velocity_dt_nu", "Tc_ex")
+        _check(kty, "Fy", grid, "velocity_dt_nu", "kty")
+    out = (C.c_double * 3)()
+    _lib.call("ht3d_velocity_dt_nu", C.c_void_p(Vx.ptr), C.c_void_p(Vy.ptr), C.c_void_p(Vz.ptr), C.c_void_p(kx.ptr),
+              C.c_void_p(ky.ptr), C.c_void_p(kz.ptr), C.c_void_p(Pc_ex.ptr), C.c_void_p(Ty.ptr), C.c_double(Ra),
+              C.c_double(dx), C.c_double(dy), C.c_double(dz), C.c_double(cfl),
+              C.c_void_p(Tc_ex.ptr if Tc_ex is not None else None), C.c_void_p(kty.ptr if kty is not None else None),
+              C.c_double(Lx), C.c_double(Lz), out, *grid)
+    return out[0], out[1], out[2]
+
+
+__all__ = [n for n in dir() if not n.startswith("_") and n[0].isupper()] + ["velocity_dt_nu"]

@@ -281,6 +281,17 @@ int ht3d_dyrel_solve_dc(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, h
 int ht3d_weno5_advect(double *Tc_ex, const double *Vx, const double *Vy, const double *Vz, double dt, double _dx,
                       double _dy, double _dz, double Ttop, double Tbot, int version, int ncx, int ncy, int ncz);
 
+/* Init_vel! (kernels_HT3D.jl:73-80), the advective time step and the Nusselt
+ * number in one pass and one read-back: out[0] = max|V| over Vx, Vy, Vz,
+ * out[1] = cfl*min(dx,dy,dz)/max|V| (HydroThermal3D_v18.jl:554, v7:220),
+ * out[2] = Nu = sum_top(-kty dT/dy)/(Lx*Lz)/(ncx*ncz) of Tc_ex with filled
+ * ghosts (HydroThermal3D_Alban_v6.jl:345-346; 0 when Tc_ex is null).
+ * V is bit-identical to ht3d_Init_vel.                                      */
+int ht3d_velocity_dt_nu(double *Vx, double *Vy, double *Vz, const double *kx, const double *ky, const double *kz,
+                        const double *Pc_ex, const double *Ty, double Ra, double dx, double dy, double dz, double cfl,
+                        const double *Tc_ex, const double *kty, double Lx, double Lz, double *out, int ncx, int ncy,
+                        int ncz);
+
 /* Closed-box Horton-Rogers-Lapwood porous convection (HydroThermal3D_Alban_v6.jl)
  * stepped with DYREL solves and WENO5 advection, all on the device.  Time step
  * rules, with dt_adv = cfl*min(dx,dy,dz)/max|V| from this step's velocity:

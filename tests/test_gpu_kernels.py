@@ -227,3 +227,30 @@ def test_shape_mismatch_raises(ht):
     A = ht.DeviceArray.zeros(sh["E"])
     with pytest.raises(ValueError):
         ht.kernels.ComputeFlux(bad, *good[1:], A, 1.0, 1.0, 1.0)
+
+
+@pytest.mark.parametrize("grid", GRIDS, ids=lambda g: "x".join(map(str, g)))
+def test_velocity_dt_nu(ht, orc, grid):
+    """Fused velocity + time step + Nu: V bit-identical to Init_vel (the oracle's
+    kernels_HT3D.jl:73-80 restatement), max|V| exact, dt by the v18 rule, Nu as
+    the oracle's top-face flux sum (HydroThermal3D_Alban_v6.jl:345-346)."""
+    rng = np.random.default_rng(77)
+    sh = shapes(*grid)
+    nx, ny, nz = grid
+    P, T = rand(sh["E"], rng), rand(sh["E"], rng, 0.0, 1.0)
+    k = [positive(sh[s], rng) for s in ("Fx", "Fy", "Fz")]
+    Ty, kty = rand(sh["Fy"], rng), positive(sh["Fy"], rng)
+    Ra, h, cfl, Lx, Lz = 517.25, (0.7 / nx, 0.45 / ny, 1.3 / nz), 0.5, 0.7, 1.3
+    dev = {n: ht.DeviceArray.from_host(a) for n, a in dict(P=P, T=T, kx=k[0], ky=k[1], kz=k[2], Ty=Ty, kty=kty).items()}
+    V = [ht.DeviceArray.zeros(sh[s]) for s in ("Fx", "Fy", "Fz")]
+    vmax, dt, nu = ht.kernels.velocity_dt_nu(*V, dev["kx"], dev["ky"], dev["kz"], dev["P"], dev["Ty"], Ra, *h, cfl,
+                                             dev["T"], dev["kty"], Lx, Lz)
+    Vo = [np.zeros(sh[s], order="F") for s in ("Fx", "Fy", "Fz")]
+    orc.Init_vel(*Vo, *k, P, Ty, Ra, 1.0 / h[0], 1.0 / h[1], 1.0 / h[2])
+    for a, b in zip(V, Vo):
+        assert np.array_equal(a.get(), b)
+    assert vmax == max(np.abs(v).max() for v in Vo)
+    assert dt == cfl * min(h) / vmax
+    flux = -kty[:, ny, :] * ((1.0 / h[1]) * (T[1:-1, ny + 1, 1:-1] - T[1:-1, ny, 1:-1]))
+    import math
+    assert abs(nu - math.fsum(flux.ravel()) / (Lx * Lz) / (nx * nz)) <= 1e-15 * abs(nu)
