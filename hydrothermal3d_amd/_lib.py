@@ -69,6 +69,7 @@ class BatchStats(C.Structure):
 
 _CTYPE = {
     "int": C.c_int,
+    "float": C.c_float,
     "double": C.c_double,
     "size_t": C.c_size_t,
 }

@@ -12,6 +12,8 @@
 
 namespace ht3d {
 
+int to_float(float *out, const double *in, size_t n);   // ht3d_dc.hip
+
 // 3D launch: block (64,4,1); grid covers (n1,n2,n3).
 template <typename Fn>
 __global__ __launch_bounds__(256) void for3d(int n1, int n2, int n3, Fn fn) {
@@ -193,11 +195,13 @@ int ht3d_Compute_Rp(double *Rp, const double *ky, const double *Ty, double Ra, d
 // Every read is of the interior or of a ghost written by an earlier launch, so
 // edges and corners (which race in the reference and are never read by the
 // hot path) are deterministic and independent of earlier ghost values.
-static int bc_fill6(const char *name, double *A, int nx, int ny, int nz, const int type[6], const double val[6]) {
+}  // extern "C"
+template <typename T>
+static int bc_fill6(const char *name, T *A, int nx, int ny, int nz, const int type[6], const double val[6]) {
     Grid g{nx, ny, nz};
     const int s1 = nx + 2, s2 = ny + 2, s3 = nz + 2;
     const int t0 = type[0], t1 = type[1], t2 = type[2], t3 = type[3], t4 = type[4], t5 = type[5];
-    const double v0 = val[0], v1 = val[1], v2 = val[2], v3 = val[3], v4 = val[4], v5 = val[5];
+    const T v0 = (T)val[0], v1 = (T)val[1], v2 = (T)val[2], v3 = (T)val[3], v4 = (T)val[4], v5 = (T)val[5];
     int rc = launch3d(name, 2, ny, nz, [=] __device__(int f, int j0, int k0) {
         const int j = j0 + 1, k = k0 + 1;
         if (f == 0) A[g.e(0, j, k)] = t0 ? 2 * v0 - A[g.e(1, j, k)] : A[g.e(1, j, k)];
@@ -215,8 +219,6 @@ static int bc_fill6(const char *name, double *A, int nx, int ny, int nz, const i
         else A[g.e(i, j, s3 - 1)] = t5 ? 2 * v5 - A[g.e(i, j, s3 - 2)] : A[g.e(i, j, s3 - 2)];
     });
 }
-
-}  // extern "C"
 
 namespace ht3d {
 int bc_fill_table(double *A, int nx, int ny, int nz, const int type[6], const double val[6]) {
@@ -707,4 +709,140 @@ int ht3d_Advect_v2(double *Fc, const double *X, const double *V, int axis, doubl
     });
 }
 
+
+// ---------------------------------------------------------------- FP32 sweeps
+// FP32 versions of the kernels a defect-correction driver runs in its inner
+// sweeps (the correction equation carries homogeneous boundary values), with
+// the FP64 entries' expressions and order evaluated in float, as the fused FP32
+// sweep does.  No reference counterpart: the reference is FP64 only
+// (HydroThermal3D_v7.jl:13).  ht3d_to_f32 and ht3d_add_f32 move data between the
+// FP64 anchor and the FP32 correction.
+int ht3d_SetTemperatureBCs_f32(float *Tc_ex, float Tbot, float Ttop, int nx, int ny, int nz) {
+    CHECK_GRID(nx, ny, nz);
+    CHECK_PTR(Tc_ex);
+    const int t[6] = {0, 0, 1, 1, 0, 0};
+    const double v[6] = {0.0, 0.0, (double)Tbot, (double)Ttop, 0.0, 0.0};
+    return bc_fill6("SetTemperatureBCs_f32", Tc_ex, nx, ny, nz, t, v);
+}
+
+int ht3d_SetPressureBCs_f32(float *Pc_ex, float Pbot, float Ptop, int nx, int ny, int nz) {
+    (void)Pbot; (void)Ptop;
+    CHECK_GRID(nx, ny, nz);
+    CHECK_PTR(Pc_ex);
+    const int t[6] = {0, 0, 0, 0, 0, 0};
+    const double v[6] = {0, 0, 0, 0, 0, 0};
+    return bc_fill6("SetPressureBCs_f32", Pc_ex, nx, ny, nz, t, v);
+}
+
+int ht3d_ComputeFlux_f32(float *qx, float *qy, float *qz, const float *kx, const float *ky, const float *kz,
+                         const float *A, float _dx, float _dy, float _dz, int nx, int ny, int nz) {
+    CHECK_GRID(nx, ny, nz);
+    CHECK_PTR(qx); CHECK_PTR(qy); CHECK_PTR(qz); CHECK_PTR(kx); CHECK_PTR(ky); CHECK_PTR(kz); CHECK_PTR(A);
+    Grid g{nx, ny, nz};
+    int rc = launch3d("ComputeFlux_f32", nx + 1, ny, nz, [=] __device__(int i, int j, int k) {
+        qx[g.fx(i, j, k)] = -kx[g.fx(i, j, k)] * (_dx * (A[g.e(i + 1, j + 1, k + 1)] - A[g.e(i, j + 1, k + 1)]));
+    });
+    if (rc) return rc;
+    rc = launch3d("ComputeFlux_f32", nx, ny + 1, nz, [=] __device__(int i, int j, int k) {
+        qy[g.fy(i, j, k)] = -ky[g.fy(i, j, k)] * (_dy * (A[g.e(i + 1, j + 1, k + 1)] - A[g.e(i + 1, j, k + 1)]));
+    });
+    if (rc) return rc;
+    return launch3d("ComputeFlux_f32", nx, ny, nz + 1, [=] __device__(int i, int j, int k) {
+        qz[g.fz(i, j, k)] = -kz[g.fz(i, j, k)] * (_dz * (A[g.e(i + 1, j + 1, k + 1)] - A[g.e(i + 1, j + 1, k)]));
+    });
+}
+
+int ht3d_ResidualDiffusion_f32(float *F, const float *R, const float *qx, const float *qy, const float *qz, float _dx,
+                               float _dy, float _dz, int nx, int ny, int nz) {
+    CHECK_GRID(nx, ny, nz);
+    CHECK_PTR(F); CHECK_PTR(R); CHECK_PTR(qx); CHECK_PTR(qy); CHECK_PTR(qz);
+    Grid g{nx, ny, nz};
+    return launch3d("ResidualDiffusion_f32", nx, ny, nz, [=] __device__(int i, int j, int k) {
+        F[g.c(i, j, k)] = R[g.c(i, j, k)] + _dx * DQX + _dy * DQY + _dz * DQZ;
+    });
+}
+
+int ht3d_ResidualDiffusionTransient_f32(float *F, const float *T, const float *R, const float *qx, const float *qy,
+                                        const float *qz, float _dt, float _dx, float _dy, float _dz, int nx, int ny,
+                                        int nz) {
+    CHECK_GRID(nx, ny, nz);
+    CHECK_PTR(F); CHECK_PTR(T); CHECK_PTR(R); CHECK_PTR(qx); CHECK_PTR(qy); CHECK_PTR(qz);
+    Grid g{nx, ny, nz};
+    return launch3d("ResidualDiffusionTransient_f32", nx, ny, nz, [=] __device__(int i, int j, int k) {
+        F[g.c(i, j, k)] = R[g.c(i, j, k)] + _dt * T[g.e(i + 1, j + 1, k + 1)] + _dx * DQX + _dy * DQY + _dz * DQZ;
+    });
+}
+
+int ht3d_ResidualDiffusionPC_f32(float *F, const float *R, const float *qx, const float *qy, const float *qz,
+                                 const float *PC, float _dx, float _dy, float _dz, int nx, int ny, int nz) {
+    CHECK_GRID(nx, ny, nz);
+    CHECK_PTR(F); CHECK_PTR(R); CHECK_PTR(qx); CHECK_PTR(qy); CHECK_PTR(qz); CHECK_PTR(PC);
+    Grid g{nx, ny, nz};
+    return launch3d("ResidualDiffusionPC_f32", nx, ny, nz, [=] __device__(int i, int j, int k) {
+        float f = R[g.c(i, j, k)] + _dx * DQX + _dy * DQY + _dz * DQZ;
+        F[g.c(i, j, k)] = f / PC[g.c(i, j, k)];
+    });
+}
+
+int ht3d_ResidualDiffusionTransientPC_f32(float *F, const float *T, const float *R, const float *qx, const float *qy,
+                                          const float *qz, const float *PC, float _dt, float _dx, float _dy,
+                                          float _dz, int nx, int ny, int nz) {
+    CHECK_GRID(nx, ny, nz);
+    CHECK_PTR(F); CHECK_PTR(T); CHECK_PTR(R); CHECK_PTR(qx); CHECK_PTR(qy); CHECK_PTR(qz); CHECK_PTR(PC);
+    Grid g{nx, ny, nz};
+    return launch3d("ResidualDiffusionTransientPC_f32", nx, ny, nz, [=] __device__(int i, int j, int k) {
+        float f = R[g.c(i, j, k)] + _dt * T[g.e(i + 1, j + 1, k + 1)] + _dx * DQX + _dy * DQY + _dz * DQZ;
+        F[g.c(i, j, k)] = f / PC[g.c(i, j, k)];
+    });
+}
+
+int ht3d_DYRELUpdate_f32(float *F0, float *X, const float *F, float h1, float h2, float dtau, int nx, int ny,
+                         int nz) {
+    CHECK_GRID(nx, ny, nz);
+    CHECK_PTR(F0); CHECK_PTR(X); CHECK_PTR(F);
+    Grid g{nx, ny, nz};
+    return launch3d("DYRELUpdate_f32", nx, ny, nz, [=] __device__(int i, int j, int k) {
+        const size_t c = g.c(i, j, k), e = g.e(i + 1, j + 1, k + 1);
+        float f0 = h1 * F0[c] + h2 * F[c];
+        F0[c] = f0;
+        X[e] = X[e] - dtau * f0;
+    });
+}
+
+int ht3d_GershgorinPoisson_f32(float *G, float *D, const float *rhoC, const float *kx, const float *ky,
+                               const float *kz, float transient, float dt, float dx, float dy, float dz, int nx,
+                               int ny, int nz) {
+    CHECK_GRID(nx, ny, nz);
+    CHECK_PTR(G); CHECK_PTR(D); CHECK_PTR(rhoC); CHECK_PTR(kx); CHECK_PTR(ky); CHECK_PTR(kz);
+    Grid g{nx, ny, nz};
+    return launch3d("GershgorinPoisson_f32", nx, ny, nz, [=] __device__(int i, int j, int k) {
+        const float kW = kx[g.fx(i, j, k)], kE = kx[g.fx(i + 1, j, k)];
+        const float kS = ky[g.fy(i, j, k)], kN = ky[g.fy(i, j + 1, k)];
+        const float kF = kz[g.fz(i, j, k + 1)], kB = kz[g.fz(i, j, k)];
+        const float rc = rhoC[g.c(i, j, k)] * transient / dt;
+        const float diag = (kB / dz + kF / dz) / dz + (kN / dy + kS / dy) / dy + (kE / dx + kW / dx) / dx + rc;
+        const float off = fabsf(kE / (dx * dx)) + fabsf(kW / (dx * dx)) + fabsf(kN / (dy * dy)) +
+                          fabsf(kS / (dy * dy)) + fabsf(kB / (dz * dz)) + fabsf(kF / (dz * dz));
+        D[g.c(i, j, k)] = diag;
+        G[g.c(i, j, k)] = (off + fabsf(diag)) / diag;
+    });
+}
+
+int ht3d_to_f32(float *out, const double *in, size_t n) {
+    CHECK_PTR(out); CHECK_PTR(in);
+    if (!n) return HT3D_OK;
+    int rc = to_float(out, in, n);
+    if (rc) return rc;
+    return after_launch("to_f32");
+}
+
+int ht3d_add_f32(double *X, const float *e, int nx, int ny, int nz) {
+    CHECK_GRID(nx, ny, nz);
+    CHECK_PTR(X); CHECK_PTR(e);
+    Grid g{nx, ny, nz};
+    return launch3d("add_f32", nx, ny, nz, [=] __device__(int i, int j, int k) {
+        const size_t c = g.e(i + 1, j + 1, k + 1);
+        X[c] += (double)e[c];
+    });
+}
 }  // extern "C"

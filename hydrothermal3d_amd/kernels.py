@@ -14,6 +14,8 @@ from __future__ import annotations
 
 import ctypes as C
 
+import numpy as np
+
 from . import _lib
 from .device import DeviceArray
 
@@ -28,18 +30,19 @@ def _grid_from(a: DeviceArray, kind: str) -> tuple[int, int, int]:
     return tuple(s - d for s, d in zip(a.shape, o))
 
 
-def _check(a, kind: str, grid, name: str, argname: str) -> None:
+def _check(a, kind: str, grid, name: str, argname: str, itemsize: int = 8) -> None:
     if not isinstance(a, DeviceArray):
         raise TypeError(f"{name}: argument {argname} must be a DeviceArray, got {type(a).__name__}")
     want = tuple(g + d for g, d in zip(grid, _OFF[kind]))
     if a.shape != want:
         raise ValueError(f"{name}: {argname} has shape {a.shape}, expected {want} ({kind} of grid {grid})")
-    if a.dtype.itemsize != 8:
-        raise ValueError(f"{name}: {argname} must be float64")
+    if a.dtype.itemsize != itemsize:
+        raise ValueError(f"{name}: {argname} must be float{8 * itemsize}")
 
 
-def _make(name: str, cname: str, kinds: list[tuple[str, str]], ref: str):
-    """kinds: [(argname, kind)] with kind in _OFF, 'd' (double) or 'i' (int)."""
+def _make(name: str, cname: str, kinds: list[tuple[str, str]], ref: str, fp32: bool = False):
+    """kinds: [(argname, kind)] with kind in _OFF, 'd' (double) or 'i' (int).
+    fp32: float32 arrays and float scalars (the _f32 entries)."""
     first = next(i for i, (_, k) in enumerate(kinds) if k in _OFF)
 
     def fn(*args):
@@ -49,10 +52,10 @@ def _make(name: str, cname: str, kinds: list[tuple[str, str]], ref: str):
         cargs = []
         for (argname, kind), v in zip(kinds, args):
             if kind in _OFF:
-                _check(v, kind, grid, name, argname)
+                _check(v, kind, grid, name, argname, 4 if fp32 else 8)
                 cargs.append(C.c_void_p(v.ptr))
             elif kind == "d":
-                cargs.append(C.c_double(float(v)))
+                cargs.append(C.c_float(float(v)) if fp32 else C.c_double(float(v)))
             else:
                 cargs.append(C.c_int(int(v)))
         _lib.call(cname, *cargs, *grid)
@@ -285,6 +288,44 @@ def AdvectWithWeno5(Tc_ex, Vx, Vy, Vz, dt, _dx, _dy, _dz, Ttop, Tbot):
               1, *grid)
 
 
+# ------------------------------------------------ FP32 inner-sweep kernels
+_F32 = {
+    "SetTemperatureBCs_f32": [("Tc_ex", "E"), ("Tbot", "d"), ("Ttop", "d")],
+    "SetPressureBCs_f32": [("Pc_ex", "E"), ("Pbot", "d"), ("Ptop", "d")],
+    "ComputeFlux_f32": [("qx", "Fx"), ("qy", "Fy"), ("qz", "Fz"), ("kx", "Fx"), ("ky", "Fy"), ("kz", "Fz"),
+                        ("A", "E"), ("_dx", "d"), ("_dy", "d"), ("_dz", "d")],
+    "ResidualDiffusion_f32": [("F", "c"), ("R", "c"), ("qx", "Fx"), ("qy", "Fy"), ("qz", "Fz"), ("_dx", "d"),
+                              ("_dy", "d"), ("_dz", "d")],
+    "ResidualDiffusionTransient_f32": [("F", "c"), ("T", "E"), ("R", "c"), ("qx", "Fx"), ("qy", "Fy"), ("qz", "Fz"),
+                                       ("_dt", "d"), ("_dx", "d"), ("_dy", "d"), ("_dz", "d")],
+    "ResidualDiffusionPC_f32": [("F", "c"), ("R", "c"), ("qx", "Fx"), ("qy", "Fy"), ("qz", "Fz"), ("PC", "c"),
+                                ("_dx", "d"), ("_dy", "d"), ("_dz", "d")],
+    "ResidualDiffusionTransientPC_f32": [("F", "c"), ("T", "E"), ("R", "c"), ("qx", "Fx"), ("qy", "Fy"),
+                                         ("qz", "Fz"), ("PC", "c"), ("_dt", "d"), ("_dx", "d"), ("_dy", "d"),
+                                         ("_dz", "d")],
+    "DYRELUpdate_f32": [("F0", "c"), ("X", "E"), ("F", "c"), ("h1", "d"), ("h2", "d"), ("dtau", "d")],
+    "GershgorinPoisson_f32": [("G", "c"), ("D", "c"), ("rhoC", "c"), ("kx", "Fx"), ("ky", "Fy"), ("kz", "Fz"),
+                              ("transient", "d"), ("dt", "d"), ("dx", "d"), ("dy", "d"), ("dz", "d")],
+}
+for _n, _k in _F32.items():
+    globals()[_n] = _make(_n, "ht3d_" + _n, _k, "FP32 inner sweep of the defect-correction solve", fp32=True)
+
+
+def to_f32(out: DeviceArray, src: DeviceArray) -> None:
+    """FP64 -> FP32 copy of a whole array (same shape)."""
+    if out.shape != src.shape or out.dtype.itemsize != 4 or src.dtype.itemsize != 8:
+        raise ValueError("to_f32: out must be float32 and src float64 of the same shape")
+    _lib.call("ht3d_to_f32", C.c_void_p(out.ptr), C.c_void_p(src.ptr), C.c_size_t(int(np.prod(src.shape))))
+
+
+def add_f32(X: DeviceArray, e: DeviceArray) -> None:
+    """X += e on the interior of an FP64 E array (e: FP32 E array)."""
+    grid = _grid_from(X, "E")
+    _check(X, "E", grid, "add_f32", "X")
+    _check(e, "E", grid, "add_f32", "e", 4)
+    _lib.call("ht3d_add_f32", C.c_void_p(X.ptr), C.c_void_p(e.ptr), *grid)
+
+
 def velocity_dt_nu(Vx, Vy, Vz, kx, ky, kz, Pc_ex, Ty, Ra, dx, dy, dz, cfl, Tc_ex=None, kty=None, Lx=1.0, Lz=1.0):
     """Init_vel! (kernels_HT3D.jl:73-80), max|V|, the advective time step
     cfl*min(h)/max|V| (HydroThermal3D_v18.jl:554) and, when Tc_ex and kty are
@@ -308,4 +349,4 @@ def velocity_dt_nu(Vx, Vy, Vz, kx, ky, kz, Pc_ex, Ty, Ra, dx, dy, dz, cfl, Tc_ex
     return out[0], out[1], out[2]
 
 
-__all__ = [n for n in dir() if not n.startswith("_") and n[0].isupper()] + ["velocity_dt_nu"]
+__all__ = [n for n in dir() if not n.startswith("_") and n[0].isupper()] + ["velocity_dt_nu", "to_f32", "add_f32"]

@@ -269,6 +269,34 @@ typedef struct {
 
 int ht3d_dyrel_solve(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep);
 
+/* FP32 versions of the inner-sweep kernels for a defect-correction driver (the
+ * fused ht3d_dyrel_solve_dc runs the same arithmetic).  Same argument order as
+ * the FP64 entries, float arrays and scalars, expressions evaluated in float in
+ * the reference's order.  No reference counterpart (the reference is FP64 only,
+ * HydroThermal3D_v7.jl:13).  ht3d_to_f32 converts n doubles; ht3d_add_f32 adds
+ * an FP32 correction to the interior of an FP64 E array (X += e).           */
+int ht3d_SetTemperatureBCs_f32(float *Tc_ex, float Tbot, float Ttop, int ncx, int ncy, int ncz);
+int ht3d_SetPressureBCs_f32(float *Pc_ex, float Pbot, float Ptop, int ncx, int ncy, int ncz);
+int ht3d_ComputeFlux_f32(float *qx, float *qy, float *qz, const float *kx, const float *ky, const float *kz,
+                         const float *A, float _dx, float _dy, float _dz, int ncx, int ncy, int ncz);
+int ht3d_ResidualDiffusion_f32(float *F, const float *R, const float *qx, const float *qy, const float *qz, float _dx,
+                               float _dy, float _dz, int ncx, int ncy, int ncz);
+int ht3d_ResidualDiffusionTransient_f32(float *F, const float *T, const float *R, const float *qx, const float *qy,
+                                        const float *qz, float _dt, float _dx, float _dy, float _dz, int ncx, int ncy,
+                                        int ncz);
+int ht3d_ResidualDiffusionPC_f32(float *F, const float *R, const float *qx, const float *qy, const float *qz,
+                                 const float *PC, float _dx, float _dy, float _dz, int ncx, int ncy, int ncz);
+int ht3d_ResidualDiffusionTransientPC_f32(float *F, const float *T, const float *R, const float *qx, const float *qy,
+                                          const float *qz, const float *PC, float _dt, float _dx, float _dy,
+                                          float _dz, int ncx, int ncy, int ncz);
+int ht3d_DYRELUpdate_f32(float *F0, float *X, const float *F, float h1, float h2, float dtau, int ncx, int ncy,
+                         int ncz);
+int ht3d_GershgorinPoisson_f32(float *G, float *D, const float *rhoC, const float *kx, const float *ky,
+                               const float *kz, float transient, float dt, float dx, float dy, float dz, int ncx,
+                               int ncy, int ncz);
+int ht3d_to_f32(float *out, const double *in, size_t n);
+int ht3d_add_f32(double *X, const float *e, int ncx, int ncy, int ncz);
+
 /* The same solve with FP32 DYREL sweeps on the correction equation A e = r inside
  * an FP64 defect-correction loop (r = b - A x and x += e in FP64).  No reference
  * counterpart: the reference is FP64 only (HydroThermal3D_v7.jl:13).         */
