@@ -31,6 +31,9 @@
 #include <stddef.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define EXPORT __attribute__((visibility("default")))
 
@@ -890,6 +893,23 @@ EXPORT double o_hrl_scalar(const o_hrl_model *m, int which) {
 EXPORT void o_hrl_set_dt(o_hrl_model *m, double dt) { m->dt = dt; }
 
 EXPORT void o_hrl_set_viscosity(o_hrl_model *m, double a) { m->mu_a = a; }
+
+/* OpenMP threads of the oracle's loops (small grids run faster on one) */
+EXPORT void o_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
+EXPORT int o_get_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 EXPORT void o_hrl_set_dt_rule(o_hrl_model *m, int rule) { m->dt_rule = rule; }
 

@@ -55,6 +55,8 @@ def lib() -> C.CDLL:
         _L.o_hrl_set_viscosity.argtypes = [C.c_void_p, C.c_double]
         _L.o_hrl_set_dt_rule.argtypes = [C.c_void_p, C.c_int]
         _L.o_hrl_n_adv.argtypes = [C.c_void_p]
+        _L.o_set_threads.argtypes = [C.c_int]
+        _L.o_get_threads.restype = C.c_int
         _L.o_dyrel_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     return _L
 
@@ -235,6 +237,13 @@ SUM, SUM_ABS, MAX, MIN, MAX_ABS = 0, 1, 2, 3, 4
 def reduce(a: np.ndarray, op: int) -> float:
     a = np.asfortranarray(a, dtype=np.float64)
     return lib().o_reduce(_ptr(a), C.c_size_t(a.shape[0]), C.c_size_t(a.size // a.shape[0]), op)
+
+
+def set_threads(n: int) -> int:
+    """Set the oracle's OpenMP thread count; returns the previous one."""
+    old = lib().o_get_threads()
+    lib().o_set_threads(int(n))
+    return old
 
 
 def lognormal_field(kfv, xmin, ymin, zmin, dx, dy, dz, K, phi, M, sigma, fault_mult, fault_half, n, x0):

@@ -473,3 +473,48 @@ def test_status_codes(orc):
     assert rep["status"] == 11                                   # v18:464 "Nan"
     rep, _ = orc.dyrel_solve(X.copy(order="F"), R, *k, **kw, p=orc.params(nitmax=2000, nitout=1, CFL=3.0))
     assert rep["status"] == 12                                   # v18:465 "Diverged"
+
+
+# ---------------------------------------------------------------- textbook physics
+def _onset_growth(orc, Ra, dt, n=16, nz=3, t_end=0.1, eps=1e-3):
+    """Growth rate of the x-y roll (cos(pi x) sin(pi y)) of the conductive state
+    in a unit box, from the closed-box model steps (thermal DYREL solve, Darcy
+    DYREL solve, velocity, WENO5 advection) at a constant dt."""
+    o = orc.HRLModel(n, n, nz, 1.0, 1.0, nz / n, Ra, 0.5)
+    o.set_dt_rule(1)
+    o.set_dt(dt)
+    h = 1.0 / n
+    c = (np.arange(n + 2) - 0.5) * h
+    X, Y = np.meshgrid(c, c, indexing="ij")
+    T = (1 - Y) + eps * np.cos(np.pi * X) * np.sin(np.pi * Y)
+    o.field("Tc_ex")[...] = np.asfortranarray(np.repeat(T[:, :, None], nz + 2, axis=2))
+    p = orc.params(100000, 100, 25, 1, 1e-11)
+    o.set_params(p, p, 1)
+    base = (1 - Y)[1:-1, 1:-1, None]
+    steps = int(round(t_end / dt))
+    amp = []
+    for _ in range(steps):
+        o.step()
+        amp.append(np.abs(o.field("Tc_ex")[1:-1, 1:-1, 1:-1] - base).max())
+    o.close()
+    k0 = steps // 4
+    return math.log(amp[-1] / amp[k0]) / (dt * (steps - 1 - k0))
+
+
+@pytest.mark.parametrize("Ra", [30.0, 60.0])
+def test_convection_onset_growth_rate(orc, Ra):
+    """Horton-Rogers-Lapwood onset (the textbook case for the coupled model):
+    a perturbation k = m = pi of the conductive state grows at
+    sigma = Ra k^2/(k^2+m^2) - (k^2+m^2) = Ra/2 - 2 pi^2, so it decays below
+    Ra_c = 4 pi^2 and grows above.  The steps are first order in time, so the
+    rate is extrapolated from dt and dt/2 (Richardson); it must match linear
+    theory within 5 % (16 cells per unit length)."""
+    threads = orc.set_threads(1)     # a 768-cell grid: threading costs more than it saves
+    try:
+        s1, s2 = _onset_growth(orc, Ra, 0.005), _onset_growth(orc, Ra, 0.0025)
+    finally:
+        orc.set_threads(threads)
+    sigma = 2 * s2 - s1
+    theory = Ra / 2 - 2 * math.pi ** 2
+    assert math.copysign(1, sigma) == math.copysign(1, theory)
+    assert abs(sigma - theory) <= 0.05 * abs(theory), (sigma, theory)
