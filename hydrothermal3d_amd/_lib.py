@@ -32,7 +32,8 @@ class DyrelParams(C.Structure):
 class DyrelReport(C.Structure):
     _fields_ = [("iters", C.c_int), ("status", C.c_int), ("n_checks", C.c_int), ("n_lambda", C.c_int),
                 ("lmax", C.c_double), ("lmin", C.c_double), ("dtau", C.c_double), ("h1", C.c_double),
-                ("h2", C.c_double), ("nF_abs", C.c_double), ("nF_rel", C.c_double), ("nF_ini", C.c_double)]
+                ("h2", C.c_double), ("nF_abs", C.c_double), ("nF_rel", C.c_double), ("nF_ini", C.c_double),
+                ("ncheck", C.c_int)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -413,6 +413,9 @@ struct ht3d_batch {
     std::vector<IterArgs<float>> h_it32[2];
     std::vector<DcArgs> h_dc[2];
     int tile64[2] = {-1, -1}, tile32[2] = {-1, -1};
+    // check-interval tuner inputs per [kind][fp32]: sweep and check costs (us), last iterations
+    double t_it[2][2] = {{-1, -1}, {-1, -1}}, t_chk[2][2] = {{-1, -1}, {-1, -1}};
+    double last_iters[2][2] = {{-1, -1}, {-1, -1}};
     // work list, partials, states
     uint2 *d_items = nullptr, *d_range = nullptr, *d_grange = nullptr;
     unsigned *d_nitems = nullptr;  // [0] item count, [1] iteration base
@@ -685,6 +688,40 @@ static int autotune_batch(ht3d_batch *b, int kind, bool transient, bool fp32, co
         }
     }
     *tile_out = best_tile;
+    // costs for the check-interval tuner: one sweep, and what a check (FP64) or
+    // an anchor with its update (defect correction) adds, on the chosen tile
+    {
+        fill_tables(b, kind, f, best_tile, best_tile);
+        int rc;
+        if ((rc = upload_tables(b, kind)) || (rc = upload_items(b, best_tile, all, nullptr))) return rc;
+        HIP_TRY(hipStreamSynchronize(s));
+        auto time_ops = [&](const std::vector<BOp> &ops, double *us) -> int {
+            const int reps = 4;
+            int r2 = HT3D_OK;
+            for (int r = -1; r < reps && !r2; ++r) {
+                if (r == 0) HIP_TRY(hipEventRecord(b->ev0, s));
+                hipLaunchKernelGGL(batch_activate_kernel, dim3(div_up(b->nprob, 256)), dim3(256), 0, s, b->d_state,
+                                   b->nprob);
+                HIP_TRY(hipGetLastError());
+                r2 = enqueue_ops(b, ops, kind, transient, best_tile, p);
+            }
+            if (r2) return r2;
+            HIP_TRY(hipEventRecord(b->ev1, s));
+            HIP_TRY(hipEventSynchronize(b->ev1));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            *us = 1e3 * ms / reps;
+            return HT3D_OK;
+        };
+        double plain = 0, chk = 0;
+        if ((rc = time_ops({BOp{fp32 ? OP_ITER32 : OP_ITER64, 0u, 0, 0}}, &plain))) return rc;
+        if (fp32) rc = time_ops({BOp{OP_ANCHOR_RES, 0u, 0, 0}, BOp{OP_ANCHOR, 0u, 0, 0}, BOp{OP_UPDATE, 0u, 0, 0}}, &chk);
+        else rc = time_ops({BOp{OP_ITER64, FL_CHECK, 0, 0}, BOp{OP_CHECK64, 0u, 1, 0}}, &chk);
+        if (rc) return rc;
+        const double t_sweep = 1e3 * best / 4;
+        b->t_it[kind][fp32 ? 1 : 0] = t_sweep;
+        b->t_chk[kind][fp32 ? 1 : 0] = fp32 ? fmax(chk - (plain - t_sweep), 1.0) : fmax(chk - plain, 1.0);
+    }
     return HT3D_OK;
 }
 
@@ -734,7 +771,7 @@ static int poll_loop(ht3d_batch *b, int kind, bool transient, int tile, const ht
 }
 
 // The batched solve of one kind; reports and (thermal) Nusselt sums per problem.
-static int batch_solve(ht3d_batch *b, int kind, const ht3d_dyrel_params &p, const std::vector<ht3d_dyrel_field> &f,
+static int batch_solve(ht3d_batch *b, int kind, const ht3d_dyrel_params &p_in, const std::vector<ht3d_dyrel_field> &f,
                        std::vector<ht3d_dyrel_report> &rep, std::vector<int> &outer, std::vector<double> *nusum) {
     hipStream_t s = ctx().stream;
     const bool transient = f[0].transient != 0;
@@ -752,6 +789,10 @@ static int batch_solve(ht3d_batch *b, int kind, const ht3d_dyrel_params &p, cons
         }
     int &tile = fp32 ? b->tile32[kind] : b->tile64[kind];
     if (tile < 0 && (rc = autotune_batch(b, kind, transient, fp32, f, &tile))) return rc;
+    ht3d_dyrel_params p = p_in;
+    const int pk = fp32 ? 1 : 0;
+    if (p.ncheck < 0)   // the check-interval tuner (the anchor interval in the defect-correction form)
+        p.ncheck = tuned_check_interval(b->last_iters[kind][pk], b->t_it[kind][pk], b->t_chk[kind][pk], p.nitout);
     fill_tables(b, kind, f, tile, tile);
     if ((rc = upload_tables(b, kind))) return rc;
     std::vector<char> all(b->nprob, 1);
@@ -834,10 +875,14 @@ static int batch_solve(ht3d_batch *b, int kind, const ht3d_dyrel_params &p, cons
     HIP_TRY(hipStreamSynchronize(s));
 
     // finalise per problem; Nu sums from the state the reference evaluates it on
+    double maxit = 0;
+    for (int q = 0; q < b->nprob; ++q) maxit = fmax(maxit, (double)b->h_state[0][q].r.iters);
+    b->last_iters[kind][pk] = maxit;
     for (int q = 0; q < b->nprob; ++q) {
         const BState &st = b->h_state[0][q];
         const ht3d_dyrel_field &d = f[q];
         rep[q] = st.r;
+        rep[q].ncheck = p.ncheck > 0 ? p.ncheck : p.nitout;
         outer[q] = st.outer;
         const double *nu_src = d.X;
         if (!fp32) {

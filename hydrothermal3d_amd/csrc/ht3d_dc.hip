@@ -11,6 +11,7 @@
 // and lambda_min follows the same Rayleigh-quotient schedule inside each inner
 // solve.  The reference has no mixed-precision path (it is FP64 only,
 // HydroThermal3D_v7.jl:13); the FP32 sweeps move half the bytes per iteration.
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -147,7 +148,12 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     // and the lambda schedule run on uninterrupted, so the iteration follows the
     // FP64 solve up to FP32 rounding, and re-anchoring keeps e small enough that
     // FP32 never has to resolve more than the residual drop between anchors.
-    const int anchor = p->ncheck > 0 ? p->ncheck : (p->nitout > 0 ? p->nitout : 50);
+    TuneEntry *te = tune_find(TuneKey{nx, ny, nz, f->transient != 0 ? 1 : 0, 1});
+    int anchor = p->ncheck > 0 ? p->ncheck : (p->nitout > 0 ? p->nitout : 50);
+    if (p->ncheck < 0)   // the check-interval tuner, with the anchor as the check
+        anchor = tuned_check_interval(te ? te->last_iters : -1.0,
+                                      te ? (pcfg.tile >= 0 && te->pair_us < te->us ? te->pair_us : te->us) : -1.0,
+                                      te ? te->anchor_us : -1.0, p->nitout);
     double nF_abs = 0.0, nF_rel = 0.0, nF_ini = 0.0;
     int status = HT3D_NOT_CONVERGED, outer = 0, inner_total = 0, n_checks = 0, n_lambda = 0;
     SegmentTimer timer;
@@ -164,12 +170,18 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     auto plain = [&](int it) { return !lam_iter(it, p->nitout) && !(it < p->nitmax && lam_iter(it + 1, p->nitout)); };
     for (outer = 0;; ++outer) {
         // anchor: FP64 residual, FP32 right-hand side, FP64 norm
+        if (outer == 1 && te) HIP_TRY(hipStreamSynchronize(cx.stream));   // time the second anchor alone
+        const auto ta = std::chrono::steady_clock::now();
         if (f->transient) hipLaunchKernelGGL(dc_residual_kernel<true>, rgrid, dim3(256), 0, cx.stream, d);
         else hipLaunchKernelGGL(dc_residual_kernel<false>, rgrid, dim3(256), 0, cx.stream, d);
         HIP_TRY(hipGetLastError());
         if ((rc = finish_partials((double *)part, rgrid.x * rgrid.y * rgrid.z, 0x2u, (double *)res))) return rc;
         double r4[4];
         if ((rc = read_result4((double *)res, r4))) return rc;
+        if (outer == 1 && te) {
+            const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - ta).count();
+            te->anchor_us = te->anchor_us > 0 && te->anchor_us < us ? te->anchor_us : us;
+        }
         ++n_checks;
         nF_abs = p->norm == 0 ? r4[0] / (double)N / sqrt((double)N) : r4[1];
         if (r4[0] != r4[0]) nF_abs = r4[0];
@@ -235,7 +247,9 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     if (nu_out) {
         if ((rc = nu_sum(f->X, f->ky, nx, ny, nz, 1.0 / f->dy, nu_out))) return rc;
     }
+    if (te) te->last_iters = inner_total;
     if (rep) {
+        rep->ncheck = anchor;
         rep->iters = inner_total;
         rep->status = status;
         rep->n_checks = n_checks;

@@ -233,6 +233,12 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
     PairCfg pcfg;
     if ((rc = choose_cfg(a, f->transient != 0, true, 0, (double *)f0b, &cfg, &pcfg))) return rc;
     const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
+    TuneEntry *te = tune_find(TuneKey{nx, ny, nz, f->transient != 0 ? 1 : 0, 0});
+    int ncheck = p->ncheck;
+    if (ncheck < 0)   // the check-interval tuner
+        ncheck = tuned_check_interval(te ? te->last_iters : -1.0,
+                                      te ? (pcfg.tile >= 0 && te->pair_us < te->us ? te->pair_us : te->us) : -1.0,
+                                      te ? te->check_us : -1.0, p->nitout);
     IterArgs<double> ap = a;   // the same arguments with the pair tiling
     if (pcfg.tile >= 0) set_pair_tiling(ap, pcfg.tile);
 
@@ -249,7 +255,7 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
     double nF_abs = 0.0, nF_rel = 0.0, nF_ini = 0.0;
     int status = HT3D_NOT_CONVERGED, iter, n_checks = 0, n_lambda = 0, done_iters = 0;
     auto plain = [&](int it) {   // no check, no stored F, no lambda update
-        return !lambda_iter(it, p->nitout) && !(p->ncheck > 0 && it % p->ncheck == 0) &&
+        return !lambda_iter(it, p->nitout) && !(ncheck > 0 && it % ncheck == 0) &&
                !(it < p->nitmax && lambda_iter(it + 1, p->nitout));
     };
     for (iter = 1; iter <= p->nitmax; ++iter) {
@@ -269,7 +275,7 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
             continue;
         }
         const bool lam = lambda_iter(iter, p->nitout);
-        const bool chk = lam || (p->ncheck > 0 && iter % p->ncheck == 0);
+        const bool chk = lam || (ncheck > 0 && iter % ncheck == 0);
         const bool store = iter < p->nitmax && lambda_iter(iter + 1, p->nitout);
         a.Xin = buf[cur];
         a.Xout = buf[cur ^ 1];
@@ -319,7 +325,9 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
         *nu_out = sum / nu_div;
     }
     if (res_buf != f->X) HIP_TRY(hipMemcpyAsync(f->X, res_buf, NE * sizeof(double), hipMemcpyDeviceToDevice, cx.stream));
+    if (te) te->last_iters = done_iters;
     if (rep) {
+        rep->ncheck = ncheck;
         rep->iters = done_iters;
         rep->status = status;
         rep->n_checks = n_checks;
@@ -363,6 +371,11 @@ int ht3d_tune_last(int *rows_per_block, int *z_per_thread, int *variant) {
 }
 
 int ht3d_tile_count(void) { return ht3d::kNumTiles; }
+
+int ht3d_tune_check_interval(double iters, double t_iter_us, double t_check_us, int nitout) {
+    if (nitout < 1) return 0;
+    return ht3d::tuned_check_interval(iters, t_iter_us, t_check_us, nitout);
+}
 
 int ht3d_tune_pairs(int mode) {
     if (mode < 0 || mode > 2) return ht3d::arg_fail("ht3d_tune_pairs: mode must be 0, 1 or 2");

@@ -5,6 +5,8 @@
 // (ht3d_batch.hip).
 #pragma once
 
+#include <chrono>
+#include <cmath>
 #include <vector>
 
 #include "ht3d_internal.hpp"
@@ -437,16 +439,41 @@ struct TuneEntry {
     double us;
     PairCfg pair;       // fastest pair configuration (timed per iteration)
     double pair_us;
+    double check_us = -1.0;   // extra cost of one convergence check (check launch, finish, read-back)
+    double last_iters = -1.0; // iterations of the last solve with this key
+    double anchor_us = -1.0;  // defect-correction anchor: FP64 residual, finish, read-back
 };
 extern std::vector<TuneEntry> g_tune;
 extern bool g_autotune;
 extern int g_force_tile, g_force_variant;
 extern int g_pairs;     // 0: never, 1: when the tuner finds them faster, 2: always (best pair tile)
 
-inline const TuneEntry *tune_find(const TuneKey &k) {
-    for (const auto &t : g_tune)
+inline TuneEntry *tune_find(const TuneKey &k) {
+    for (auto &t : g_tune)
         if (t.key == k) return &t;
     return nullptr;
+}
+
+// Check-interval tuner (a requested interval < 0).  A solve of I iterations
+// checked every L costs I t_it + (I/L) t_chk + (L/2) t_it on average (it stops at
+// the first check after convergence), which is least at L* = sqrt(2 I t_chk / t_it).
+// L is taken among the divisors of the lambda interval (so checks stay aligned
+// with the lambda refreshes), from the previous solve's I; 50 before any history.
+inline int tuned_check_interval(double iters_prev, double t_it, double t_chk, int nitout) {
+    static const int cand[] = {10, 20, 25, 50, 100, 200, 250, 500, 1000};
+    int fallback = nitout;
+    for (int c : cand)
+        if (c <= nitout && nitout % c == 0 && c <= 50) fallback = c;
+    if (!(iters_prev > 0) || !(t_it > 0) || !(t_chk > 0)) return fallback;
+    const double L = sqrt(2.0 * iters_prev * t_chk / t_it);
+    int best = -1;
+    double bd = 1e300;
+    for (int c : cand) {
+        if (c > nitout || nitout % c) continue;
+        const double d = fabs(log((double)c / L));
+        if (d < bd) { bd = d; best = c; }
+    }
+    return best > 0 ? best : nitout;
 }
 
 // Times every single-iteration tile and variant, and every pair tile and
@@ -493,6 +520,26 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
             if (rc) return rc;
             if (us < te.us) { te.us = us; te.cfg = cfg; }
         }
+    }
+    if (a.partials) {   // cost of a check: check launch + finish kernel + read-back, less a plain launch
+        void *res = nullptr;
+        int rc = workspace(WS_RESULT, 64 * sizeof(double), &res);
+        if (rc) return rc;
+        IterArgs<T> c = a;
+        c.flags = FL_CHECK;
+        set_tiling(c, te.cfg.tile);
+        const unsigned nb = (unsigned)c.nbx * c.nby * c.nbz;
+        double r4[4];
+        double best = 1e300;
+        for (int r = 0; r < 4; ++r) {
+            const auto t0 = std::chrono::steady_clock::now();
+            if ((rc = launch_iter(c, transient, te.cfg)) || (rc = finish_partials(c.partials, nb, 0x2u, (double *)res)) ||
+                (rc = read_result4((const double *)res, r4)))
+                return rc;
+            const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+            best = us < best ? us : best;
+        }
+        te.check_us = best - te.us > 1.0 ? best - te.us : 1.0;
     }
     if (F0b && a.ny >= 3) {
         for (int var = 0; var < 4; ++var) {

@@ -102,6 +102,11 @@ int ht3d_tile_count(void);
  * (default), 2: always.  ht3d_tune_last_pair reports the last solve's pair
  * tile (0 rows when none was used).                                          */
 int ht3d_tune_pairs(int mode);
+/* The check-interval tuner's rule (used when ncheck < 0): the divisor L of
+ * nitout closest (in ratio) to sqrt(2 iters t_check / t_iter), which minimises
+ * iters*t_iter + (iters/L)*t_check + (L/2)*t_iter; without history (any
+ * argument <= 0) the largest divisor of nitout up to 50.  Host only.          */
+int ht3d_tune_check_interval(double iters, double t_iter_us, double t_check_us, int nitout);
 int ht3d_tune_last_pair(int *rows_per_block, int *z_per_block, int *variant);
 
 /* maximum_g / minimum_g / sum_g / mean_g of abs.(A) (tools/Macros.jl:20-25);
@@ -240,7 +245,8 @@ int ht3d_TimeAveraging_v2(double *Fc, const double *Fold, double order, int ncx,
 typedef struct {
     int nitmax;        /* v18:309 (1e4)                                              */
     int nitout;        /* lambda_min + reference check schedule: iter<=2 || iter%nitout==0 */
-    int ncheck;        /* >0: also test convergence every ncheck iterations             */
+    int ncheck;        /* >0: also test convergence every ncheck iterations; <0: every L
+                          iterations, L chosen by the check-interval tuner (divisor of nitout) */
     int norm;          /* 0: mean|F*D|/sqrt(N) (v18:459-460); 1: max|F*D|               */
     double tol;        /* converged if norm < tol or norm/norm(iter 1) < tol (v18:476)   */
     double CFL;        /* v18:435 (0.99)                                                 */
@@ -252,6 +258,7 @@ typedef struct {
     int iters, status, n_checks, n_lambda;
     double lmax, lmin, dtau, h1, h2;
     double nF_abs, nF_rel, nF_ini;
+    int ncheck;        /* check interval used (the tuner's choice when ncheck < 0) */
 } ht3d_dyrel_report;
 
 typedef struct {

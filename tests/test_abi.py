@@ -60,3 +60,23 @@ def test_product_does_not_import_oracle():
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=ROOT,
                          env=dict(os.environ, PYTHONPATH=str(ROOT)))
     assert out.stdout.strip() == "False", out.stderr
+
+
+def test_check_interval_rule():
+    """The check-interval tuner's cost model (host function, no GPU): L* =
+    sqrt(2 I t_chk / t_it) snapped to a divisor of nitout, 50 without history."""
+    import math
+    from hydrothermal3d_amd import _lib
+    L = _lib.lib().ht3d_tune_check_interval
+    assert L(-1.0, 100.0, 50.0, 100) == 50
+    assert L(2000.0, -1.0, 50.0, 100) == 50
+    assert L(2000.0, 100.0, 70.0, 100) == 50          # L* = 52.9
+    assert L(400.0, 100.0, 70.0, 100) == 25           # L* = 23.7
+    assert L(20000.0, 100.0, 70.0, 1000) == 200       # L* = 167 -> 200 (log-nearest of 100, 200, 250)
+    assert L(100.0, 1000.0, 1.0, 100) == 10           # cheap checks: the shortest interval
+    for it, ti, tc, nout in ((900.0, 98.0, 60.0, 100), (3500.0, 700.0, 300.0, 500)):
+        got = L(it, ti, tc, nout)
+        assert nout % got == 0
+        best = math.sqrt(2 * it * tc / ti)
+        divs = [c for c in (10, 20, 25, 50, 100, 200, 250, 500, 1000) if c <= nout and nout % c == 0]
+        assert got == min(divs, key=lambda c: abs(math.log(c / best)))

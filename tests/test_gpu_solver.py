@@ -171,3 +171,50 @@ def test_hrl_viscosity_parity(ht, orc):
     out, errs = _run_steps(ht, orc, w, 3, p, p, 1, mu_a=math.log(10.0))
     for k, e in errs.items():
         assert e <= TOL_FIELD, f"{k}: {e}"
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32dc"])
+def test_tuned_check_interval(ht, precision):
+    """ncheck < 0: the first solve (no history) checks every 50, the next one at
+    the tuner's interval (a divisor of nitout); both converge to the tolerance
+    and agree with a fixed-interval solve to the tolerance's effect."""
+    grid = (40, 24, 36)
+    ht._lib.call("ht3d_tune_clear")
+    X, R, kx, ky, kz = _problem(grid, 0, 21)
+    h = dict(dx=1.0 / grid[0], dy=0.5 / grid[1], dz=2.0 / grid[2], dt=1.0)
+
+    def solve(ncheck):
+        D = [ht.DeviceArray.from_host(a) for a in (X, R, kx, ky, kz)]
+        p = ht.DyrelParams(nitmax=20000, nitout=100, ncheck=ncheck, norm=1, tol=1e-10)
+        rep = ht.dyrel_solve(*D, transient=0, rhoC=0.0, params=p, precision=precision, **h)
+        P = D[0].get()[1:-1, 1:-1, 1:-1]
+        return rep, P - P.mean()
+
+    r1, P1 = solve(-1)
+    r2, P2 = solve(-1)
+    rf, Pf = solve(50)
+    assert r1["ncheck"] == 50 and r2["ncheck"] in (10, 20, 25, 50, 100)
+    for r in (r1, r2):
+        assert r["status"] == 0 and r["nF_abs"] < 1e-10
+    assert rel_err(P2, Pf) <= 1e-7 and rel_err(P1, Pf) <= 1e-7
+
+
+def test_batch_tuned_check_interval(ht):
+    from hydrothermal3d_amd import inputs
+    ws = [inputs.replace(w, ncx=a, ncy=b, ncz=c, Lx=1.0, Ly=b / a, Lz=c / a, modes=64)
+          for w, (a, b, c) in zip(inputs.batch_workloads(3, seed=13000), [(16, 8, 16), (24, 12, 20), (20, 20, 20)])]
+    p = ht.DyrelParams(nitmax=20000, nitout=100, ncheck=-1, norm=1, tol=1e-9)
+    ms = []
+    for w in ws:
+        m = ht.HydroThermal3D(w.ncx, w.ncy, w.ncz, w.Lx, w.Ly, w.Lz, w.Ra, w.cfl)
+        m["Tc_ex"].set(inputs.initial_temperature(w))
+        inputs.permeability_device(w, m["kfv"])
+        ms.append(m)
+    b = ht.HydroThermal3DBatch(ms, p, p)
+    for _ in range(3):
+        for r in b.step():
+            assert r["thermal"]["status"] == 0 and r["darcy"]["status"] == 0
+            assert r["darcy"]["ncheck"] in (10, 20, 25, 50, 100)
+    b.close()
+    for m in ms:
+        m.close()
