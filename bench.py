@@ -312,6 +312,7 @@ def main(argv=None) -> int:
                     help="fp64: plain FP64 path; dc: FP32 inner sweeps + FP64 defect correction; both: run the two "
                          "on the same inputs and report the DC speed-up and its error against FP64")
     ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--check", default="50", help="check interval: an integer, or 'auto' for the check-interval tuner")
     ap.add_argument("--cpu-iters", type=int, default=600, help="pseudo-iterations of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config0-steps", type=int, default=20,
@@ -336,7 +337,8 @@ def main(argv=None) -> int:
     w = inputs.WORKLOADS[args.workload]
     if args.probe:
         return probe(w, 20, 5)
-    prm = ht.production_params(tol=args.tol)
+    ncheck = -1 if args.check == "auto" else int(args.check)
+    prm = ht.production_params(tol=args.tol, ncheck=ncheck)
     precision = _lib.FP32_DC if args.precision == "dc" else _lib.FP64
     log(f"device: {_lib.device_name()}; workload {w.name}; precision {args.precision}")
     wall, st, reports, kfv, fields = run_gpu(w, args.steps, args.warmup, precision, prm)
@@ -408,7 +410,7 @@ def main(argv=None) -> int:
     if args.batch > 0:
         log(f"batched leg: {args.batch} problems")
         out["batch"] = run_batch(args.batch, args.batch_steps, args.batch_warmup,
-                                 ht.production_params(tol=args.batch_tol))
+                                 ht.production_params(tol=args.batch_tol, ncheck=ncheck))
         out["batch"]["tol"] = args.batch_tol
     if teff32 is not None:
         out["t_eff_fp32_inner"] = round(teff32, 2)
