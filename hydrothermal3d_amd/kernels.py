@@ -54,6 +54,11 @@ def _make(name: str, cname: str, kinds: list[tuple[str, str]], ref: str, fp32: b
             if kind in _OFF:
                 _check(v, kind, grid, name, argname, 4 if fp32 else 8)
                 cargs.append(C.c_void_p(v.ptr))
+            elif kind == "W":   # (ncx+2, ncz+2) table of surface weights
+                if not isinstance(v, DeviceArray) or v.shape != (grid[0] + 2, grid[2] + 2) or v.dtype.itemsize != 8:
+                    raise ValueError(f"{name}: {argname} must be a float64 DeviceArray of shape "
+                                     f"{(grid[0] + 2, grid[2] + 2)}")
+                cargs.append(C.c_void_p(v.ptr))
             elif kind == "d":
                 cargs.append(C.c_float(float(v)) if fp32 else C.c_double(float(v)))
             else:
@@ -175,6 +180,51 @@ Init_vel = _make(
     "Init_vel", "ht3d_Init_vel",
     [("Vx", "Fx"), ("Vy", "Fy"), ("Vz", "Fz"), ("kx", "Fx"), ("ky", "Fy"), ("kz", "Fz"), ("Pc_ex", "E"), ("Ty", "Fy"),
      ("Ra", "d"), ("_dx", "d"), ("_dy", "d"), ("_dz", "d")], "Dimensionless/kernels_HT3D.jl:73-80")
+
+
+# --------------------------------------------------------------- dimensional (v18) property kernels
+_D = "Dimensional/kernels_HT3D_dimensional.jl"
+ComputeThermalConductivity = _make(
+    "ComputeThermalConductivity", "ht3d_ComputeThermalConductivity",
+    [("ktv", "V"), ("Tce", "E"), ("phv", "V"), ("phi0", "d"), ("scale_kt", "d"), ("scale_T", "d")], _D + ":40-49")
+ComputeHydroConductivity = _make(
+    "ComputeHydroConductivity", "ht3d_ComputeHydroConductivity",
+    [("kfv", "V"), ("Tce", "E"), ("Pce", "E"), ("phv", "V"), ("ymin", "d"), ("dy", "d"), ("k_fact", "d"),
+     ("delta", "d"), ("scale_sigma", "d"), ("scale_T", "d"), ("scale_L", "d"), ("scale_t", "d"), ("scale_eta", "d"),
+     ("mode", "i")], _D + ":51-83")
+ComputeFluidDensity = _make(
+    "ComputeFluidDensity", "ht3d_ComputeFluidDensity",
+    [("rhofv", "V"), ("Tce", "E"), ("Pce", "E"), ("phv", "V"), ("scale_sigma", "d"), ("scale_T", "d"),
+     ("scale_rho", "d")], _D + ":85-99")
+SmoothConductivityV2C = _make("SmoothConductivityV2C", "ht3d_SmoothConductivityV2C", [("ktc", "c"), ("ktv", "V")],
+                              _D + ":110-113")
+SmoothConductivityC2V = _make("SmoothConductivityC2V", "ht3d_SmoothConductivityC2V", [("ktv", "V"), ("ktc", "c")],
+                              _D + ":115-118")
+ComputeRhoCeffective = _make(
+    "ComputeRhoCeffective", "ht3d_ComputeRhoCeffective",
+    [("rhoC_eff", "c"), ("Tc_ex", "E"), ("Pc_ex", "E"), ("phc_ex", "E"), ("phi_i", "d"), ("sca_T", "d"),
+     ("sca_sigma", "d"), ("sca_C", "d"), ("sca_rho", "d")], _D + ":148-160 (ComputeρCeffective!)")
+_PHI = [("phc_ex", "E"), ("phi_i", "d"), ("scale_rho", "d"), ("scale_T", "d"), ("scale_sigma", "d")]
+PhiRhof = _make("PhiRhof", "ht3d_PhiRhof", [("phirhofc", "c"), ("Tc_ex", "E"), ("Pc_ex", "E")] + _PHI,
+                _D + ":225-235 (ϕρf)")
+PhiRhof_old = _make("PhiRhof_old", "ht3d_PhiRhof_old", [("phirhofc", "c"), ("Tc0", "c"), ("Pc0", "c")] + _PHI,
+                    _D + ":237-247 (ϕρf_old)")
+PhiDrhoDP = _make("PhiDrhoDP", "ht3d_PhiDrhoDP", [("phidrhodP", "c"), ("Tc_ex", "E"), ("Pc_ex", "E")] + _PHI,
+                  _D + ":249-259 (ϕdρdP!)")
+dim_Init_vel = _make(
+    "dim_Init_vel", "ht3d_dim_Init_vel",
+    [("Vx", "Fx"), ("Vy", "Fy"), ("Vz", "Fz"), ("qx", "Fx"), ("qy", "Fy"), ("qz", "Fz"), ("rhofv", "V")],
+    _D + ":328-332 (the dimensional Init_vel!)")
+dim_SetTemperatureBCs = _make(
+    "dim_SetTemperatureBCs", "ht3d_dim_SetTemperatureBCs",
+    [("Tc_ex", "E"), ("ky", "Fy"), ("wN", "W"), ("phc_ex", "E")] +
+    [(n, "d") for n in ("qyS", "_dy", "kS", "TS", "TN", "y_plateau", "a2", "b2", "dTdy", "dx", "dy", "sticky_air",
+                        "sca_T", "sca_kT", "phi_i")], _D + ":183-205 (the dimensional SetTemperatureBCs!)")
+dim_SetPressureBCs = _make(
+    "dim_SetPressureBCs", "ht3d_dim_SetPressureBCs",
+    [("Pc_ex", "E"), ("wN", "W"), ("phc_ex", "E")] +
+    [(n, "d") for n in ("PS", "PN", "y_plateau", "a2", "b2", "rhof", "g", "dx", "dy", "sticky_air")],
+    _D + ":291-314 (the dimensional SetPressureBCs!)")
 
 # --------------------------------------------------------------- WENO5 (tools/Weno5_Routines*.jl)
 VxPlusMinus = _make("VxPlusMinus", "ht3d_VxPlusMinus", [("Vxm", "c"), ("Vxp", "c"), ("Vx", "Fx")],

@@ -205,6 +205,61 @@ int ht3d_Init_vel(double *Vx, double *Vy, double *Vz, const double *kx, const do
                   const double *Pc_ex, const double *Ty, double Ra, double _dx, double _dy, double _dz, int ncx,
                   int ncy, int ncz);
 
+/* ---- Dimensional (v18) property kernels and immersed-surface ghost fills.
+ * They evaluate the closures of Dimensional/HydroThermal3D_v18.jl:29-39 (rock and
+ * fluid conductivity, heat capacity, density, the fluid-density polynomial and
+ * its pressure derivative, viscosity, depth-dependent permeability, porosity
+ * per phase) in the reference's order; only 10^x and exp() may differ from a
+ * CPU evaluation, by an ulp.  Phases: 1 air, -1 immersed surface cell, 2 fault,
+ * 3 intrusion.  wN is the (ncx+2, ncz+2) table of surface weights.           */
+/* kernels_HT3D_dimensional.jl:40-49 (vertex array from extended centroids) */
+int ht3d_ComputeThermalConductivity(double *ktv, const double *Tce, const double *phv, double phi0, double scale_kt,
+                                    double scale_T, int ncx, int ncy, int ncz);
+/* kernels_HT3D_dimensional.jl:51-83; mode 1: rho_f k/mu, 2: k/mu, 3: k */
+int ht3d_ComputeHydroConductivity(double *kfv, const double *Tce, const double *Pce, const double *phv, double ymin,
+                                  double dy, double k_fact, double delta, double scale_sigma, double scale_T,
+                                  double scale_L, double scale_t, double scale_eta, int mode, int ncx, int ncy,
+                                  int ncz);
+/* kernels_HT3D_dimensional.jl:85-99 */
+int ht3d_ComputeFluidDensity(double *rhofv, const double *Tce, const double *Pce, const double *phv,
+                             double scale_sigma, double scale_T, double scale_rho, int ncx, int ncy, int ncz);
+/* kernels_HT3D_dimensional.jl:110-113 (cells = @av of vertices) */
+int ht3d_SmoothConductivityV2C(double *ktc, const double *ktv, int ncx, int ncy, int ncz);
+/* kernels_HT3D_dimensional.jl:115-118 (inner vertices = @av of cells) */
+int ht3d_SmoothConductivityC2V(double *ktv, const double *ktc, int ncx, int ncy, int ncz);
+/* ComputeρCeffective!  kernels_HT3D_dimensional.jl:148-160 (air cells untouched) */
+int ht3d_ComputeRhoCeffective(double *rhoC_eff, const double *Tc_ex, const double *Pc_ex, const double *phc_ex,
+                              double phi_i, double sca_T, double sca_sigma, double sca_C, double sca_rho, int ncx,
+                              int ncy, int ncz);
+/* ϕρf  kernels_HT3D_dimensional.jl:225-235 (phase-1 cells untouched) */
+int ht3d_PhiRhof(double *phirhofc, const double *Tc_ex, const double *Pc_ex, const double *phc_ex, double phi_i,
+                 double scale_rho, double scale_T, double scale_sigma, int ncx, int ncy, int ncz);
+/* ϕρf_old  kernels_HT3D_dimensional.jl:237-247 (T, P as cell arrays) */
+int ht3d_PhiRhof_old(double *phirhofc, const double *Tc0, const double *Pc0, const double *phc_ex, double phi_i,
+                     double scale_rho, double scale_T, double scale_sigma, int ncx, int ncy, int ncz);
+/* ϕdρdP!  kernels_HT3D_dimensional.jl:249-259 */
+int ht3d_PhiDrhoDP(double *phidrhodP, const double *Tc_ex, const double *Pc_ex, const double *phc_ex, double phi_i,
+                   double scale_rho, double scale_T, double scale_sigma, int ncx, int ncy, int ncz);
+/* Init_vel!  kernels_HT3D_dimensional.jl:328-332 (V = q / face-averaged rho_f;
+ * named dim_ because Dimensionless/kernels_HT3D.jl has another Init_vel!) */
+int ht3d_dim_Init_vel(double *Vx, double *Vy, double *Vz, const double *qx, const double *qy, const double *qz,
+                      const double *rhofv, int ncx, int ncy, int ncz);
+/* SetTemperatureBCs!  kernels_HT3D_dimensional.jl:183-205: air = TN, Neumann x/z,
+ * basal heat flux qyS (bottom), immersed-surface Dirichlet TN.  Every argument of
+ * the reference is kept; kS, TS, y_plateau, a2, b2, dTdy, dx, dy, sticky_air,
+ * sca_T, sca_kT and phi_i are unused there and here.  A rule reads its source
+ * cell after the source's own rule, so the result does not depend on thread order. */
+int ht3d_dim_SetTemperatureBCs(double *Tc_ex, const double *ky, const double *wN, const double *phc_ex, double qyS,
+                               double _dy, double kS, double TS, double TN, double y_plateau, double a2, double b2,
+                               double dTdy, double dx, double dy, double sticky_air, double sca_T, double sca_kT,
+                               double phi_i, int ncx, int ncy, int ncz);
+/* SetPressureBCs!  kernels_HT3D_dimensional.jl:291-314: air = PN, Neumann x/z and
+ * bottom, immersed-surface Dirichlet PN (PS, y_plateau, a2, b2, rhof, g, dx, dy,
+ * sticky_air unused, as in the reference) */
+int ht3d_dim_SetPressureBCs(double *Pc_ex, const double *wN, const double *phc_ex, double PS, double PN,
+                            double y_plateau, double a2, double b2, double rhof, double g, double dx, double dy,
+                            double sticky_air, int ncx, int ncy, int ncz);
+
 /* Temperature-dependent viscosity mu = exp(-a T): face coefficients *= exp(a T_face),
  * T_face the mean of the two adjacent cells (ghosts at walls).  No counterpart in
  * the dimensionless kernels (constant viscosity there); the dimensionless analogue

@@ -58,6 +58,8 @@ def lib() -> C.CDLL:
         _L.o_set_threads.argtypes = [C.c_int]
         _L.o_get_threads.restype = C.c_int
         _L.o_dyrel_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        _L.o_eos.restype = C.c_double
+        _L.o_eos.argtypes = [C.c_int, C.c_double, C.c_double]
     return _L
 
 
@@ -79,7 +81,12 @@ def _make(name: str, cname: str, kinds):
         grid = _grid(args[first], kinds[first][1])
         cargs = []
         for (argname, kind), v in zip(kinds, args):
-            if kind in _OFF:
+            if kind == "W":
+                want = (grid[0] + 2, grid[2] + 2)
+                if v.shape != want:
+                    raise ValueError(f"{name}: {argname} shape {v.shape} != {want}")
+                cargs.append(_ptr(v))
+            elif kind in _OFF:
                 want = tuple(g + d for g, d in zip(grid, _OFF[kind]))
                 if v.shape != want:
                     raise ValueError(f"{name}: {argname} shape {v.shape} != {want}")
@@ -175,6 +182,51 @@ Gradients_Weno5 = _make("Gradients_Weno5", "o_Gradients_Weno5",
 Advect = _make("Advect", "o_Advect", [("Fc", "c"), ("Vxp", "c"), ("dTdxm", "c"), ("Vxm", "c"), ("dTdxp", "c"), ("dt", "d")])
 TimeAveraging = _make("TimeAveraging", "o_TimeAveraging", [("Fc", "c"), ("Fold", "c"), ("order", "d")])
 TimeAveraging_v2 = _make("TimeAveraging_v2", "o_TimeAveraging_v2", [("Fc", "E"), ("Fold", "c"), ("order", "d")])
+
+
+# Dimensional (v18) property kernels and immersed-surface ghost fills (ht3d_oracle_dim.c)
+ComputeThermalConductivity = _make(
+    "ComputeThermalConductivity", "o_ComputeThermalConductivity",
+    [("ktv", "V"), ("Tce", "E"), ("phv", "V"), ("phi0", "d"), ("scale_kt", "d"), ("scale_T", "d")])
+ComputeHydroConductivity = _make(
+    "ComputeHydroConductivity", "o_ComputeHydroConductivity",
+    [("kfv", "V"), ("Tce", "E"), ("Pce", "E"), ("phv", "V"), ("ymin", "d"), ("dy", "d"), ("k_fact", "d"),
+     ("delta", "d"), ("scale_sigma", "d"), ("scale_T", "d"), ("scale_L", "d"), ("scale_t", "d"),
+     ("scale_eta", "d"), ("mode", "i")])
+ComputeFluidDensity = _make(
+    "ComputeFluidDensity", "o_ComputeFluidDensity",
+    [("rhofv", "V"), ("Tce", "E"), ("Pce", "E"), ("phv", "V"), ("scale_sigma", "d"), ("scale_T", "d"),
+     ("scale_rho", "d")])
+SmoothConductivityV2C = _make("SmoothConductivityV2C", "o_SmoothConductivityV2C", [("ktc", "c"), ("ktv", "V")])
+SmoothConductivityC2V = _make("SmoothConductivityC2V", "o_SmoothConductivityC2V", [("ktv", "V"), ("ktc", "c")])
+ComputeRhoCeffective = _make(
+    "ComputeRhoCeffective", "o_ComputeRhoCeffective",
+    [("rhoC_eff", "c"), ("Tc_ex", "E"), ("Pc_ex", "E"), ("phc_ex", "E"), ("phi_i", "d"), ("sca_T", "d"),
+     ("sca_sigma", "d"), ("sca_C", "d"), ("sca_rho", "d")])
+_PHI = [("phc_ex", "E"), ("phi_i", "d"), ("scale_rho", "d"), ("scale_T", "d"), ("scale_sigma", "d")]
+PhiRhof = _make("PhiRhof", "o_PhiRhof", [("out", "c"), ("Tc_ex", "E"), ("Pc_ex", "E")] + _PHI)
+PhiRhof_old = _make("PhiRhof_old", "o_PhiRhof_old", [("out", "c"), ("Tc0", "c"), ("Pc0", "c")] + _PHI)
+PhiDrhoDP = _make("PhiDrhoDP", "o_PhiDrhoDP", [("out", "c"), ("Tc_ex", "E"), ("Pc_ex", "E")] + _PHI)
+dim_Init_vel = _make("dim_Init_vel", "o_dim_Init_vel",
+                     [("Vx", "Fx"), ("Vy", "Fy"), ("Vz", "Fz"), ("qx", "Fx"), ("qy", "Fy"), ("qz", "Fz"),
+                      ("rhofv", "V")])
+dim_SetTemperatureBCs = _make(
+    "dim_SetTemperatureBCs", "o_dim_SetTemperatureBCs",
+    [("Tc_ex", "E"), ("ky", "Fy"), ("wN", "W"), ("phc_ex", "E")] +
+    [(n, "d") for n in ("qyS", "_dy", "kS", "TS", "TN", "y_plateau", "a2", "b2", "dTdy", "dx", "dy", "sticky_air",
+                        "sca_T", "sca_kT", "phi_i")])
+dim_SetPressureBCs = _make(
+    "dim_SetPressureBCs", "o_dim_SetPressureBCs",
+    [("Pc_ex", "E"), ("wN", "W"), ("phc_ex", "E")] +
+    [(n, "d") for n in ("PS", "PN", "y_plateau", "a2", "b2", "rhof", "g", "dx", "dy", "sticky_air")])
+
+EOS = {"kTs": 0, "kTf": 1, "Cf": 2, "Cs": 3, "rhos": 4, "rhof_C": 5, "drhodP_C": 6, "muf": 7, "kF": 8, "phi": 9,
+       "pow4": 10}
+
+
+def eos(name: str, a: float, b: float = 0.0) -> float:
+    """The oracle's scalar EOS / property closure `name` (HydroThermal3D_v18.jl:29-39)."""
+    return lib().o_eos(EOS[name], C.c_double(a), C.c_double(b))
 
 
 def Boundaries_Weno5(Fc_exxx, Fc, axis, type_lo, val_lo, type_hi, val_hi):

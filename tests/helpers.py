@@ -32,3 +32,33 @@ def rel_err(a, b):
     b = np.asarray(b, dtype=np.float64)
     den = max(np.max(np.abs(b)), 1e-300)
     return float(np.max(np.abs(a - b)) / den)
+
+
+def dim_state(nx, ny, nz, rng, air_from=None, immersed=False, TN=None, PN=None):
+    """Synthetic inputs of the dimensional (v18) kernels in the model's scaled
+    units: T in [293, 1000] K and P in [1e5, 3e8] Pa divided by their scales,
+    vertex and centroid phases in {0, 2, 3} below the row `air_from` and air (1)
+    from it up (ghost rows included), air cells already at the surface values
+    (as the v18 initial conditions leave them), and optionally one immersed
+    surface cell (phase -1) per column, directly below the air, away from the
+    ghost layers.  Returns a dict of Fortran-ordered arrays and scalars."""
+    from hydrothermal3d_amd.dimensional import v18_scaling
+    sc = v18_scaling()
+    sh = shapes(nx, ny, nz)
+    T = np.asfortranarray(rng.uniform(293.0, 1000.0, sh["E"]) / sc.T)
+    P = np.asfortranarray(rng.uniform(1e5, 3e8, sh["E"]) / sc.sigma)
+    phc = np.asfortranarray(rng.choice([0.0, 2.0, 3.0], size=sh["E"]))
+    phv = np.asfortranarray(rng.choice([0.0, 2.0, 3.0], size=sh["V"]))
+    TN = 293.15 / sc.T if TN is None else TN
+    PN = 1e5 / sc.sigma if PN is None else PN
+    wN = np.asfortranarray(rng.uniform(0.3, 1.0, size=(nx + 2, nz + 2)))
+    if air_from is not None:
+        phc[:, air_from:, :] = 1.0
+        phv[:, air_from:, :] = 1.0
+        T[phc == 1.0] = TN
+        P[phc == 1.0] = PN
+        if immersed:
+            cols = [(i, k) for i in range(2, nx) for k in range(2, nz) if rng.uniform() < 0.6]
+            for i, k in cols:
+                phc[i, air_from - 1, k] = -1.0
+    return dict(sc=sc, T=T, P=P, phc=phc, phv=phv, TN=TN, PN=PN, wN=wN, sh=sh)

@@ -1,0 +1,249 @@
+"""Pins of the oracle's dimensional (v18) property kernels and closures
+(oracle/ht3d_oracle_dim.c) -- CPU only.
+
+* the pressure derivative of the fluid-density polynomial against the output
+  stored in the reference notebook (notebooks/Gershgorin_TH3D.ipynb), both as
+  the closed-form derivative closure and as a finite difference of rho_f;
+* the literal fourth power against exact rational arithmetic;
+* the water-viscosity closure against tabulated values;
+* every kernel against the independent NumPy restatement;
+* closed forms (uniform fields, the permeability profile, the fault factor,
+  air cells, the immersed-surface interpolation identity);
+* the ghost fills against a literal one-thread execution of the reference
+  kernel body.
+"""
+import json
+import re
+from fractions import Fraction
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from helpers import dim_state, shapes
+
+GOLD = Path(__file__).resolve().parent / "golden" / "notebook_outputs.json"
+
+
+def _julia_scalar_fn(expr: str):
+    """The notebook's printed Julia expression as a Python function of (T_C, p),
+    integer powers written out as products (Julia's literal powers)."""
+    def powrep(m):
+        v, n = m.group(1), int(m.group(2))
+        return "(" + "*".join([v] * n) + ")"
+    py = re.sub(r"(\w+) \.\^ (\d)", powrep, expr).replace(".*", "*")
+    return eval("lambda T_C, p: " + py)  # noqa: S307 (fixture text from the repository)
+
+
+@pytest.fixture(scope="module")
+def npd():
+    from oracle import ht3d_numpy_dim
+    return ht3d_numpy_dim
+
+
+def _samples(rng, n=400):
+    return rng.uniform(0.0, 700.0, n), rng.uniform(1e5, 3e8, n)
+
+
+def test_drhodp_matches_notebook_output(orc):
+    d_nb = _julia_scalar_fn(json.loads(GOLD.read_text())["drhof_dp"])
+    rng = np.random.default_rng(1)
+    Tc, p = _samples(rng)
+    got = np.array([orc.eos("drhodP_C", a, b) for a, b in zip(Tc, p)])
+    want = np.array([d_nb(a, b) for a, b in zip(Tc, p)])
+    # same terms and order: equal up to the last bit of the largest term
+    assert np.max(np.abs(got - want)) <= 4e-16 * np.max(np.abs(want))
+
+
+def test_rhof_pressure_slope_matches_notebook_derivative(orc):
+    """Central differences of the oracle's rho_f(T, p) in p reproduce the
+    notebook's derivative: pins every pressure-dependent coefficient of rho_f."""
+    d_nb = _julia_scalar_fn(json.loads(GOLD.read_text())["drhof_dp"])
+    rng = np.random.default_rng(2)
+    for Tc, p in zip(*_samples(rng, 200)):
+        h = 2e4
+        fd = (orc.eos("rhof_C", Tc, p + h) - orc.eos("rhof_C", Tc, p - h)) / (2 * h)
+        assert abs(fd - d_nb(Tc, p)) <= 1e-6 * abs(d_nb(Tc, p)) + 1e-14
+
+
+def test_rhof_temperature_terms_match_numpy(orc, npd):
+    """The pressure-independent terms of rho_f (not reached by the derivative
+    pin) are checked against the independent restatement, bit for bit."""
+    rng = np.random.default_rng(3)
+    Tc, p = _samples(rng)
+    p[:50] = 0.0
+    got = np.array([orc.eos("rhof_C", a, b) for a, b in zip(Tc, p)])
+    assert np.array_equal(got, npd.rhof_C(Tc, p))
+    assert np.array_equal(np.array([orc.eos("drhodP_C", a, b) for a, b in zip(Tc, p)]), npd.drhodP_C(Tc, p))
+    for name in ("kTs", "kTf", "Cf", "Cs", "rhos"):
+        T = rng.uniform(250.0, 1200.0, 50)
+        assert np.array_equal(np.array([orc.eos(name, t) for t in T]), getattr(npd, name)(T)), name
+
+
+def test_pow4_is_the_rounded_fourth_power(orc, npd):
+    rng = np.random.default_rng(4)
+    xs = np.concatenate([rng.uniform(1e5, 3e8, 300), rng.uniform(0.0, 700.0, 300), [3.0, 1e8, 0.1]])
+    exact = 0
+    for x in xs:
+        got = orc.eos("pow4", x)
+        want = float(Fraction(x) ** 4)
+        assert abs(got - want) <= np.spacing(want)
+        exact += got == want
+        assert npd.pow4(np.float64(x)) == got
+    assert exact >= len(xs) - 3
+
+
+@pytest.mark.parametrize("T_K,mu", [(293.15, 1.0016e-3), (323.15, 5.47e-4), (373.15, 2.82e-4)])
+def test_water_viscosity(orc, T_K, mu):
+    """mu_f(T) against tabulated liquid-water viscosities (20, 50, 100 C); the
+    closure is a Vogel-type fit, accurate to about 2 % over liquid water."""
+    assert abs(orc.eos("muf", T_K) / mu - 1.0) < 0.02
+
+
+def test_phi_and_permeability_law(orc):
+    assert orc.eos("phi", 2.0, 0.01) == 0.03 and orc.eos("phi", 0.0, 0.01) == 0.01
+    assert orc.eos("phi", 3.0, 0.01) == 0.01
+    assert orc.eos("kF", 0.0, 3000.0) == 5e-16
+    assert abs(orc.eos("kF", -3000.0, 3000.0) - 5e-16 / np.e) < 1e-30
+
+
+GRIDS = [(6, 5, 4), (9, 7, 8)]
+
+
+@pytest.mark.parametrize("grid", GRIDS, ids=lambda g: "x".join(map(str, g)))
+def test_property_kernels_match_numpy(orc, npd, grid):
+    rng = np.random.default_rng(sum(grid))
+    d = dim_state(*grid, rng, air_from=grid[1] - 1)
+    sc, sh = d["sc"], d["sh"]
+    z = lambda s: np.zeros(sh[s], order="F")
+    # thermal conductivity at vertices
+    ktv = z("V")
+    orc.ComputeThermalConductivity(ktv, d["T"], d["phv"], 0.01, sc.kt, sc.T)
+    assert np.array_equal(ktv, npd.ComputeThermalConductivity(d["T"], d["phv"], 0.01, sc.kt, sc.T))
+    # smoothing vertex -> cell -> vertex
+    ktc = z("c")
+    orc.SmoothConductivityV2C(ktc, ktv)
+    assert np.array_equal(ktc, npd.SmoothConductivityV2C(ktv))
+    ktv2 = ktv.copy(order="F")
+    orc.SmoothConductivityC2V(ktv2, ktc)
+    assert np.array_equal(ktv2, npd.SmoothConductivityC2V(ktv, ktc))
+    # hydraulic conductivity, three modes (exp differs by an ulp between libm and NumPy)
+    for mode in (1, 2, 3):
+        kfv = np.asfortranarray(rng.uniform(1.0, 2.0, sh["V"]))
+        want = npd.ComputeHydroConductivity(kfv, d["T"], d["P"], d["phv"], -30e3 / sc.L, 35e3 / sc.L / grid[1],
+                                            50.0, 3000.0 / sc.L, sc.sigma, sc.T, sc.L, sc.t, sc.eta, mode)
+        orc.ComputeHydroConductivity(kfv, d["T"], d["P"], d["phv"], -30e3 / sc.L, 35e3 / sc.L / grid[1], 50.0,
+                                     3000.0 / sc.L, sc.sigma, sc.T, sc.L, sc.t, sc.eta, mode)
+        np.testing.assert_allclose(kfv, want, rtol=1e-15, atol=0)
+    # fluid density at vertices
+    rhofv = z("V")
+    orc.ComputeFluidDensity(rhofv, d["T"], d["P"], d["phv"], sc.sigma, sc.T, sc.rho)
+    assert np.array_equal(rhofv, npd.ComputeFluidDensity(d["T"], d["P"], d["phv"], sc.sigma, sc.T, sc.rho))
+    # effective rho*C (air cells untouched)
+    rc0 = np.asfortranarray(rng.uniform(-1, 1, sh["c"]))
+    rc = rc0.copy(order="F")
+    orc.ComputeRhoCeffective(rc, d["T"], d["P"], d["phc"], 0.01, sc.T, sc.sigma, sc.C, sc.rho)
+    assert np.array_equal(rc, npd.ComputeRhoCeffective(rc0, d["T"], d["P"], d["phc"], 0.01, sc.T, sc.sigma, sc.C,
+                                                       sc.rho))
+    # phi rho_f (extended and cell inputs) and phi drho/dP
+    for fn, T, P, deriv in (("PhiRhof", d["T"], d["P"], False),
+                            ("PhiRhof_old", np.asfortranarray(d["T"][1:-1, 1:-1, 1:-1]),
+                             np.asfortranarray(d["P"][1:-1, 1:-1, 1:-1]), False),
+                            ("PhiDrhoDP", d["T"], d["P"], True)):
+        out0 = np.asfortranarray(rng.uniform(-1, 1, sh["c"]))
+        out = out0.copy(order="F")
+        getattr(orc, fn)(out, T, P, d["phc"], 0.01, sc.rho, sc.T, sc.sigma)
+        assert np.array_equal(out, npd.PhiRhof(out0, T, P, d["phc"], 0.01, sc.rho, sc.T, sc.sigma, deriv)), fn
+    # dimensional Init_vel
+    q = [np.asfortranarray(rng.uniform(-1, 1, sh[s])) for s in ("Fx", "Fy", "Fz")]
+    rho = np.asfortranarray(rng.uniform(0.5, 2.0, sh["V"]))
+    V = [z(s) for s in ("Fx", "Fy", "Fz")]
+    orc.dim_Init_vel(*V, *q, rho)
+    for a, b in zip(V, npd.dim_Init_vel(*q, rho)):
+        assert np.array_equal(a, b)
+
+
+def test_property_closed_forms(orc):
+    nx, ny, nz = 5, 6, 4
+    sh = shapes(nx, ny, nz)
+    from hydrothermal3d_amd.dimensional import v18_scaling
+    sc = v18_scaling()
+    T_K, P_Pa, phi0 = 600.0, 5e7, 0.02
+    T = np.full(sh["E"], T_K / sc.T, order="F")
+    P = np.full(sh["E"], P_Pa / sc.sigma, order="F")
+    phv = np.zeros(sh["V"], order="F")
+    phv[2, :, :] = 2.0
+    phv[:, -1, :] = 1.0
+    # uniform T: conductivity is the phase mixture of the closures at T (up to the
+    # rounding of the eight-point mean)
+    ktv = np.zeros(sh["V"], order="F")
+    orc.ComputeThermalConductivity(ktv, T, phv, phi0, sc.kt, sc.T)
+    Tv = ktv * 0 + T_K
+    f = np.where(phv == 2.0, 3 * phi0, phi0)
+    want = ((1 - f) * (3.1138 - 0.0023 * Tv) + f * (-0.069 + 0.0012 * Tv)) / sc.kt
+    np.testing.assert_allclose(ktv, want, rtol=1e-14)
+    # a constant survives vertex -> cell -> vertex smoothing
+    c = np.zeros(sh["c"], order="F")
+    k1 = np.full(sh["V"], 0.3, order="F")
+    orc.SmoothConductivityV2C(c, k1)
+    orc.SmoothConductivityC2V(k1, c)
+    np.testing.assert_allclose(k1, 0.3, rtol=2e-16)
+    # a field linear in (x, y, z) is averaged to its cell-centre value
+    xv, yv, zv = np.meshgrid(np.arange(nx + 1.0), np.arange(ny + 1.0), np.arange(nz + 1.0), indexing="ij")
+    lin = np.asfortranarray(1.0 + 2.0 * xv - 3.0 * yv + 0.5 * zv)
+    orc.SmoothConductivityV2C(c, lin)
+    xc, yc, zc = np.meshgrid(np.arange(nx) + 0.5, np.arange(ny) + 0.5, np.arange(nz) + 0.5, indexing="ij")
+    np.testing.assert_allclose(c, 1.0 + 2.0 * xc - 3.0 * yc + 0.5 * zc, rtol=1e-14)
+    # permeability law (mode 3): 5e-16 exp(y/delta) / L^2, times k_fact on faults
+    ymin, dy, delta, kfact = -30e3 / sc.L, 35e3 / sc.L / ny, 3000.0 / sc.L, 50.0
+    kfv = np.zeros(sh["V"], order="F")
+    orc.ComputeHydroConductivity(kfv, T, P, phv, ymin, dy, kfact, delta, sc.sigma, sc.T, sc.L, sc.t, sc.eta, 3)
+    y = ymin + np.arange(ny + 1) * dy
+    prof = 5e-16 * np.exp(y * sc.L / (delta * sc.L)) / (sc.L * sc.L)
+    want = np.broadcast_to(prof[None, :, None], sh["V"]) * np.where(phv == 2.0, kfact, 1.0)
+    np.testing.assert_allclose(kfv, want, rtol=1e-15)
+    # mode 1 / mode 2 ratio is rho_f(T, P) * scale_L^2 / (scale_eta * scale_t)
+    k1v, k2v = np.zeros(sh["V"], order="F"), np.zeros(sh["V"], order="F")
+    args = (T, P, phv, ymin, dy, kfact, delta, sc.sigma, sc.T, sc.L, sc.t, sc.eta)
+    orc.ComputeHydroConductivity(k1v, *args, 1)
+    orc.ComputeHydroConductivity(k2v, *args, 2)
+    rho = orc.eos("rhof_C", T_K - 273.15, P_Pa)
+    np.testing.assert_allclose(k1v / k2v, rho * sc.L ** 2 / sc.eta / sc.t, rtol=1e-13)
+    # fluid density: rho_f / scale_rho, zero in air
+    rf = np.zeros(sh["V"], order="F")
+    orc.ComputeFluidDensity(rf, T, P, phv, sc.sigma, sc.T, sc.rho)
+    np.testing.assert_allclose(rf[phv != 1.0], rho / sc.rho, rtol=1e-14)
+    assert np.all(rf[phv == 1.0] == 0.0)
+
+
+@pytest.mark.parametrize("immersed", [False, True], ids=["flat", "immersed"])
+@pytest.mark.parametrize("grid", GRIDS, ids=lambda g: "x".join(map(str, g)))
+def test_ghost_fills_match_serial_reference(orc, npd, grid, immersed):
+    nx, ny, nz = grid
+    rng = np.random.default_rng(7 + nx)
+    d = dim_state(nx, ny, nz, rng, air_from=ny - 1, immersed=immersed)
+    sh = d["sh"]
+    ky = np.asfortranarray(rng.uniform(0.5, 2.0, sh["Fy"]))
+    qyS, _dy = 37e-3 / d["sc"].W * d["sc"].L ** 2, ny / 3.5
+    # temperature
+    T = d["T"].copy(order="F")
+    orc.dim_SetTemperatureBCs(T, ky, d["wN"], d["phc"], qyS, _dy, 1.05, 0.0, d["TN"], *([0.0] * 10))
+    Ts = npd.SetTemperatureBCs_serial(d["T"], ky, d["wN"], d["phc"], qyS, _dy, d["TN"])
+    m = np.ones(sh["E"], dtype=bool)
+    m[[0, -1], :, -1] = False          # x-face/z-face shared lines: thread-order dependent in the reference
+    assert np.array_equal(T[m], Ts[m])
+    # every ghost the hot path reads was written
+    assert np.all(T[1:-1, 0, 1:-1] != d["T"][1:-1, 0, 1:-1])
+    # the immersed cell interpolates the surface value with the cell below
+    if immersed:
+        idx = np.argwhere(d["phc"] == -1.0)
+        assert len(idx) > 0
+        for i, j, k in idx:
+            w = d["wN"][i, k]
+            assert abs(w * T[i, j, k] + (1 - w) * T[i, j - 1, k] - d["TN"]) < 1e-14 * abs(d["TN"]) / w
+    # pressure
+    P = d["P"].copy(order="F")
+    orc.dim_SetPressureBCs(P, d["wN"], d["phc"], 0.0, d["PN"], *([0.0] * 8))
+    Ps = npd.SetPressureBCs_serial(d["P"], d["wN"], d["phc"], d["PN"])
+    assert np.array_equal(P, Ps)
+    assert np.array_equal(P[1:-1, 0, 1:-1], P[1:-1, 1, 1:-1])
