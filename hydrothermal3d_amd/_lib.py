@@ -47,6 +47,18 @@ class DyrelField(C.Structure):
                 ("bc_type", C.c_int * 6), ("bc_val", C.c_double * 6)]
 
 
+class V18Field(C.Structure):
+    """ht3d_v18_field: the dimensional (v18) heat or Darcy DYREL solve."""
+    _fields_ = [("ncx", C.c_int), ("ncy", C.c_int), ("ncz", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
+                ("transient", C.c_double), ("X", C.c_void_p), ("kx", C.c_void_p), ("ky", C.c_void_p),
+                ("kz", C.c_void_p), ("phc_ex", C.c_void_p), ("wN", C.c_void_p), ("rhoC", C.c_void_p),
+                ("PC", C.c_void_p), ("G", C.c_void_p), ("top", C.c_double), ("norm_scale", C.c_double),
+                ("Tc_old", C.c_void_p), ("qyS", C.c_double), ("Qt", C.c_double), ("Tc_ex", C.c_void_p),
+                ("phirhof0", C.c_void_p), ("rhofv", C.c_void_p), ("g", C.c_double), ("phi_i", C.c_double),
+                ("scale_rho", C.c_double), ("scale_T", C.c_double), ("scale_sigma", C.c_double)]
+
+
 class StepReport(C.Structure):
     _fields_ = [("thermal", DyrelReport), ("darcy", DyrelReport), ("dt_used", C.c_double),
                 ("dt_next", C.c_double), ("time", C.c_double), ("Nu", C.c_double), ("vmax", C.c_double),

@@ -17,152 +17,20 @@
 #include "ht3d_internal.hpp"
 #include "ht3d_launch.hpp"
 
+#include "ht3d_dim.hpp"
+
 namespace ht3d {
 namespace dim {
 
-// ---------------------------------------------------------------- closures
-__device__ __forceinline__ double pow4(double x) {
-    const double x2 = x * x;
-    const double e2 = fma(x, x, -x2);
-    const double x4 = x2 * x2;
-    const double e4 = fma(x2, x2, -x4);
-    const double lo = e4 + (x2 * 2.0) * e2;
-    return x4 + lo;
-}
-
-__device__ __forceinline__ double kTs(double T) { return 3.1138 - 0.0023 * T; }
-__device__ __forceinline__ double kTf(double T) { return -0.069 + 0.0012 * T; }
-__device__ __forceinline__ double Cf(double T) { return 1000.0 + 7.5 * T; }
-__device__ __forceinline__ double Cs(double T) { return 0.5915 * T + 636.14; }
-__device__ __forceinline__ double rhos(double T) { return 2800.0 * (1.0 - (0.000024 * (T - 293.0))); }
-
-// fluid density (kg/m^3) of temperature in C and pressure in Pa: fifteen terms
-__device__ __forceinline__ double rhof_C(double Tc, double p) {
-    const double p2 = p * p, p3 = p2 * p, T2 = Tc * Tc, T3 = T2 * Tc;
-    double r = 1006.0;
-    r = r + (7.424e-7 * p);
-    r = r + (-0.3922 * Tc);
-    r = r + (-4.441e-15 * p2);
-    r = r + ((4.547e-9 * p) * Tc);
-    r = r + (-0.003774 * T2);
-    r = r + (1.451e-23 * p3);
-    r = r + ((-1.793e-17 * p2) * Tc);
-    r = r + ((7.485e-12 * p) * T2);
-    r = r + (2.955e-6 * T3);
-    r = r + (-1.463e-32 * pow4(p));
-    r = r + ((1.361e-26 * p3) * Tc);
-    r = r + ((4.018e-21 * p2) * T2);
-    r = r + ((-7.372e-15 * p) * T3);
-    r = r + (5.698e-11 * pow4(Tc));
-    return r;
-}
-
-// its pressure derivative (the closure as printed by the reference's notebook)
-__device__ __forceinline__ double drhodP_C(double Tc, double p) {
-    const double p2 = p * p, p3 = p2 * p, T2 = Tc * Tc, T3 = T2 * Tc;
-    double d = -7.372e-15 * T3;
-    d = d + ((8.036e-21 * T2) * p);
-    d = d + (7.485e-12 * T2);
-    d = d + ((4.083e-26 * Tc) * p2);
-    d = d - ((3.586e-17 * Tc) * p);
-    d = d + (4.547e-9 * Tc);
-    d = d - (5.852e-32 * p3);
-    d = d + (4.353e-23 * p2);
-    d = d - (8.882e-15 * p);
-    d = d + 7.424e-7;
-    return d;
-}
-
-__device__ __forceinline__ double muf(double T) { return 2.414e-5 * pow(10.0, 247.8 / (T - 140.0)); }
-__device__ __forceinline__ double kF(double y, double delta) { return 5e-16 * exp(y / delta); }
-__device__ __forceinline__ double phi_of(double phase, double phi0) { return phase == 2.0 ? 3.0 * phi0 : phi0; }
-
-// Mean of the eight extended centroids around vertex (i,j,k) (0-based vertex
-// index; the centroids are E indices i..i+1, j..j+1, k..k+1), the upper z plane
-// summed first as in kernels_HT3D_dimensional.jl:42-43.
-__device__ __forceinline__ double vertex_mean(const double *A, const Grid &g, int i, int j, int k) {
-    const double up = ((A[g.e(i + 1, j + 1, k + 1)] + A[g.e(i + 1, j, k + 1)]) + A[g.e(i, j + 1, k + 1)]) + A[g.e(i, j, k + 1)];
-    const double lo = ((A[g.e(i + 1, j + 1, k)] + A[g.e(i + 1, j, k)]) + A[g.e(i, j + 1, k)]) + A[g.e(i, j, k)];
-    return 0.125 * up + 0.125 * lo;
-}
-
-// ParallelStencil @av: x, then y, then z neighbours, times 0.125.  idx(i,j,k)
-// gives the linear index in the source array.
-template <typename Idx>
-__device__ __forceinline__ double cube_mean(const double *A, Idx idx, int i, int j, int k) {
-    return (A[idx(i, j, k)] + A[idx(i + 1, j, k)] + A[idx(i, j + 1, k)] + A[idx(i + 1, j + 1, k)] +
-            A[idx(i, j, k + 1)] + A[idx(i + 1, j, k + 1)] + A[idx(i, j + 1, k + 1)] + A[idx(i + 1, j + 1, k + 1)]) *
-           0.125;
-}
-
-// ---------------------------------------------------------------- ghost fills
-// Rules of SetTemperatureBCs! / SetPressureBCs!, in the priority that the
-// order of the kernel body gives them (later statements override earlier).
-enum BcRule { R_KEEP, R_AIR, R_XLO, R_XHI, R_BOT, R_IMM, R_ZLO, R_ZHI };
-
-struct BcArgs {
-    double *A;
-    const double *ky, *wN, *ph;
-    double top, qyS_dy;
-    Grid g;
-    int temperature;
-};
-
-// (i,j,k) are 0-based E indices.
-__device__ __forceinline__ int bc_rule(const BcArgs &a, int i, int j, int k) {
-    const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
-    if (i >= 1 && i <= nx && j >= 1 && j <= ny) {
-        if (k == 0) return R_ZLO;
-        if (k == nz + 1) return R_ZHI;
-    }
-    const double ph = a.ph[a.g.e(i, j, k)];
-    if (ph == -1.0 && j > 0) return R_IMM;
-    const int kx_hi = a.temperature ? nz + 1 : nz;   // the temperature x fill runs to the top z ghost
-    if (j >= 1 && j <= ny && k >= 1 && k <= kx_hi) {
-        if (i == 0) return R_XLO;
-        if (i == nx + 1) return R_XHI;
-    }
-    if (j == 0 && i >= 1 && i <= nx && k >= 1 && k <= nz) return R_BOT;
-    if (ph == 1.0) return R_AIR;
-    return R_KEEP;
-}
-
-// Final value of cell (i,j,k): each rule reads its source cell's final value
-// (the order of dependencies), so no cell is read while another thread of the
-// launch rewrites it.  Chains longer than five rules (stacked immersed cells)
-// read the stored value, as the CPU oracle does.
-template <int D>
-__device__ double bc_value(const BcArgs &a, int i, int j, int k) {
-    if constexpr (D > 4) {
-        return a.A[a.g.e(i, j, k)];
-    } else {
-        switch (bc_rule(a, i, j, k)) {
-            case R_ZLO: return bc_value<D + 1>(a, i, j, 1);
-            case R_ZHI: return bc_value<D + 1>(a, i, j, a.g.nz);
-            case R_IMM: {
-                const double w = a.wN[(size_t)k * (a.g.nx + 2) + i];
-                return (a.top - (1.0 - w) * bc_value<D + 1>(a, i, j - 1, k)) / w;
-            }
-            case R_XLO: return bc_value<D + 1>(a, 1, j, k);
-            case R_XHI: return bc_value<D + 1>(a, a.g.nx, j, k);
-            case R_BOT:
-                if (a.temperature) {
-                    const double kS = a.ky[a.g.fy(i - 1, 0, k - 1)];
-                    return a.qyS_dy / kS + bc_value<D + 1>(a, i, 1, k);
-                }
-                return bc_value<D + 1>(a, i, 1, k);
-            case R_AIR: return a.top;
-            default: return a.A[a.g.e(i, j, k)];
-        }
-    }
-}
-
-static int bc_launch(const char *name, const BcArgs &a) {
+// The fill is done in place: every rule's sources are cells that no rule
+// rewrites (bc_value), so reading a.A while writing `out` (the same array) is
+// free of races.
+static int bc_launch(const char *name, const BcArgs &a, double *out) {
     const Grid g = a.g;
     return launch3d(name, g.nx + 2, g.ny + 2, g.nz + 2, [=] __device__(int i, int j, int k) {
         if (bc_rule(a, i, j, k) == R_KEEP) return;
-        const double v = bc_value<0>(a, i, j, k);
-        a.A[g.e(i, j, k)] = v;
+        const double v = bc_value(a, i, j, k);
+        out[g.e(i, j, k)] = v;
     });
 }
 
@@ -335,7 +203,7 @@ int ht3d_dim_SetTemperatureBCs(double *Tc_ex, const double *ky, const double *wN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Tc_ex); CHECK_PTR(ky); CHECK_PTR(wN); CHECK_PTR(phc_ex);
     BcArgs a{Tc_ex, ky, wN, phc_ex, TN, qyS / _dy, Grid{nx, ny, nz}, 1};
-    return bc_launch("dim_SetTemperatureBCs", a);
+    return bc_launch("dim_SetTemperatureBCs", a, Tc_ex);
 }
 
 int ht3d_dim_SetPressureBCs(double *Pc_ex, const double *wN, const double *phc_ex, double PS, double PN,
@@ -345,7 +213,7 @@ int ht3d_dim_SetPressureBCs(double *Pc_ex, const double *wN, const double *phc_e
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Pc_ex); CHECK_PTR(wN); CHECK_PTR(phc_ex);
     BcArgs a{Pc_ex, nullptr, wN, phc_ex, PN, 0.0, Grid{nx, ny, nz}, 0};
-    return bc_launch("dim_SetPressureBCs", a);
+    return bc_launch("dim_SetPressureBCs", a, Pc_ex);
 }
 
 }  // extern "C"

@@ -1,0 +1,501 @@
+// ht3d_v18.hip -- fused DYREL solves of the dimensional (v18) model: the heat
+// loop of Dimensional/HydroThermal3D_v18.jl:432-478 and the Darcy loop of
+// :489-537, one kernel launch per pseudo-iteration.
+//
+// One pseudo-iteration of the reference is four to five launches:
+//   heat:  SetTemperatureBCs!, ComputeFlux!, ResidualTemperatureLinearised!, DYRELUpdate!
+//   Darcy: ϕρf, SetPressureBCs!, ComputeDarcyFlux!, ResidualFluidPressure!, DYRELUpdate!
+// (kernels_HT3D_dimensional.jl:162-326).  Here one kernel does all of it per
+// cell, in the reference's order of operations, so the result equals the
+// unfused sequence bit for bit:
+//   * the ghost/air/immersed values the BC kernel would write are computed from
+//     the neighbours when needed (the same rule evaluator as the ghost-fill
+//     entry), so the unknown's ghost layer is only written at the end;
+//   * a per-cell class byte, built once per solve from the phases, selects the
+//     path: F = 0 (air and immersed cells), the fast path (plain neighbour
+//     loads; walls by select: Neumann copies, basal heat flux), or the exact
+//     rule evaluation (cells next to air, immersed cells or such ghosts);
+//   * heat: the Gershgorin diagonal is recomputed from the face coefficients
+//     and rho*C already loaded (it is what GershgorinPoisson! stores in PC);
+//     Darcy: phi*rho_f of the cell is evaluated from the EOS each iteration
+//     (the reference's ϕρf launch), g*<rho_f> on the y faces is formed once
+//     per solve, and the stored diagonal is read;
+//   * check iterations accumulate |F*PC|, max|F*PC| and the Rayleigh-quotient
+//     sums in double-double block partials, like the dimensionless solve.
+// Memory left as the reference leaves it: the unknown with ghosts, air and
+// immersed cells filled from the state before the last update; PC and G;
+// for the Darcy solve phi*rho_f of the last iteration in the rhoC buffer
+// (at immersed cells it is evaluated from the final immersed value).
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+#include "ht3d_dim.hpp"
+#include "ht3d_internal.hpp"
+#include "ht3d_iter.hpp"
+
+namespace ht3d {
+
+int copy_ghosts(double *dst, const double *src, int nx, int ny, int nz);   // ht3d_dyrel.hip
+
+namespace v18 {
+
+enum : uint8_t { C_ZERO = 1, C_SRC = 2, C_SLOW = 4, C_FAULT = 8 };
+
+struct Args {
+    const double *__restrict__ Xin;
+    double *__restrict__ Xout;
+    const double *__restrict__ kx;
+    const double *__restrict__ ky;
+    const double *__restrict__ kz;
+    const uint8_t *__restrict__ cls;
+    double *__restrict__ F0;
+    double *__restrict__ Fst;
+    double *__restrict__ partials;
+    const double *__restrict__ aux;    // heat: T of the previous step (c); Darcy: temperature (E)
+    const double *__restrict__ aux2;   // heat: rho*C (c); Darcy: phi*rho_f of the previous step (c)
+    const double *__restrict__ Dst;    // Darcy: Gershgorin diagonal
+    const double *__restrict__ gy;     // Darcy: g * @av_xza(rho_f) on the y faces
+    dim::BcArgs bc;                    // ghost rules; bc.A is the launch's Xin
+    int nx, ny, nz, nbx, nby, nbz;
+    double _dx, _dy, _dz, _dt, dx, dy, dz, dt;
+    double transient, tr_dt, Qt, qyS_dy;
+    double phi_i, s_rho, s_T, s_sigma;
+    double h1, h2, dtau;
+    unsigned flags;
+};
+
+// Class byte of each cell (phases do not change during a solve).
+__global__ __launch_bounds__(256) void class_kernel(const double *__restrict__ ph, int nx, int ny, int nz,
+                                                    uint8_t *__restrict__ cls) {
+    const size_t N = (size_t)nx * ny * nz;
+    const Grid g{nx, ny, nz};
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((size_t)nx * ny));
+        const int ie = i + 1, je = j + 1, ke = k + 1;
+        const double p = ph[g.e(ie, je, ke)];
+        uint8_t c = 0;
+        if (fabs(p) == 1.0) c |= C_ZERO;
+        if (p == 3.0) c |= C_SRC;
+        if (p == 2.0) c |= C_FAULT;
+        auto special = [&](int a, int b, int d) { return fabs(ph[g.e(a, b, d)]) == 1.0; };
+        bool slow = false;
+        // x walls: the ghost copies the cell unless the ghost is itself immersed
+        slow |= i == 0 ? ph[g.e(0, je, ke)] == -1.0 : special(ie - 1, je, ke);
+        slow |= i == nx - 1 ? ph[g.e(nx + 1, je, ke)] == -1.0 : special(ie + 1, je, ke);
+        // bottom ghost: always the basal rule (heat flux or copy)
+        if (j > 0) slow |= special(ie, je - 1, ke);
+        // top: the ghost keeps its value unless it is air or immersed
+        slow |= special(ie, je + 1, ke);
+        // z walls: always the copy of the cell
+        if (k > 0) slow |= special(ie, je, ke - 1);
+        if (k < nz - 1) slow |= special(ie, je, ke + 1);
+        if (slow) c |= C_SLOW;
+        cls[t] = c;
+    }
+}
+
+// g * @av_xza(rho_f) on the y faces: the gravity term of ComputeDarcyFlux!
+// (kernels_HT3D_dimensional.jl:269), formed once per solve.
+__global__ __launch_bounds__(256) void gravity_kernel(const double *__restrict__ rhofv, double g_, int nx, int ny,
+                                                      int nz, double *__restrict__ gy) {
+    const size_t n = (size_t)nx * (ny + 1) * nz;
+    const Grid g{nx, ny, nz};
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % nx), j = (int)((t / nx) % (ny + 1)), k = (int)(t / ((size_t)nx * (ny + 1)));
+        gy[t] = g_ * ((rhofv[g.v(i, j, k)] + rhofv[g.v(i + 1, j, k)] + rhofv[g.v(i, j, k + 1)] +
+                       rhofv[g.v(i + 1, j, k + 1)]) * 0.25);
+    }
+}
+
+// Copies the cells that a ghost-fill rule sets (ghosts, air, immersed) from src to dst.
+__global__ __launch_bounds__(256) void copy_special_kernel(dim::BcArgs a, double *__restrict__ dst) {
+    const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
+    const size_t n = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % (nx + 2)), j = (int)((t / (nx + 2)) % (ny + 2)),
+                  k = (int)(t / ((size_t)(nx + 2) * (ny + 2)));
+        if (dim::bc_rule(a, i, j, k) != dim::R_KEEP) dst[t] = a.A[t];
+    }
+}
+
+template <bool DARCY, int TY, int KZ>
+__global__ __launch_bounds__(64 * TY) void iter_kernel(Args a) {
+    const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
+    const unsigned t = xcd_tile(blockIdx.x, nblocks);
+    const int bx = t % a.nbx, by = (t / a.nbx) % a.nby, bz = t / (a.nbx * a.nby);
+    const int i = bx * 64 + (int)(threadIdx.x & 63u);
+    const int j = by * TY + (int)(threadIdx.x >> 6);
+    const int k0 = bz * KZ;
+    const int k1 = min(k0 + KZ, a.nz);
+    const uint32_t nx = a.nx, ny = a.ny;
+    const uint32_t sE1 = nx + 2, sE2 = sE1 * (ny + 2);
+    const uint32_t sC2 = nx * ny, sX1 = nx + 1, sX2 = sX1 * ny, sY2 = nx * (ny + 1);
+    const bool check = a.flags & FL_CHECK, lam = a.flags & FL_LAMBDA, store = a.flags & FL_STORE_F;
+    double s_hi = 0.0, s_lo = 0.0, acc_max = 0.0, n_hi = 0.0, n_lo = 0.0, d_hi = 0.0, d_lo = 0.0;
+
+    if ((uint32_t)i < nx && (uint32_t)j < ny) {
+        uint32_t e = (uint32_t)(i + 1) + (uint32_t)(j + 1) * sE1 + (uint32_t)(k0 + 1) * sE2;
+        uint32_t c = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k0 * sC2;
+        uint32_t fx = (uint32_t)i + (uint32_t)j * sX1 + (uint32_t)k0 * sX2;
+        uint32_t fy = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k0 * sY2;
+        double xc = a.Xin[e];
+        double xm = a.Xin[e - sE2];
+        double kzB = a.kz[c];
+        for (int k = k0; k < k1; ++k) {
+            const double xp = a.Xin[e + sE2];
+            const double xW = a.Xin[e - 1], xE = a.Xin[e + 1];
+            const double xS = a.Xin[e - sE1], xN = a.Xin[e + sE1];
+            const double kW = a.kx[fx], kE = a.kx[fx + 1];
+            const double kS = a.ky[fy], kN = a.ky[fy + nx];
+            const double kF = a.kz[c + sC2];
+            const double kB = kzB;
+            const uint8_t cl = a.cls[c];
+            const double f0o = __builtin_nontemporal_load(a.F0 + c);
+            double f = 0.0, D = 1.0;
+            if (!(cl & C_ZERO)) {
+                double w = xW, ea = xE, s = xS, n = xN, b = xm, fr = xp;
+                if (cl & C_SLOW) {   // every neighbour from the ghost-fill rules
+                    const int ie = i + 1, je = j + 1, ke = k + 1;
+                    w = dim::bc_value(a.bc, ie - 1, je, ke);
+                    ea = dim::bc_value(a.bc, ie + 1, je, ke);
+                    s = dim::bc_value(a.bc, ie, je - 1, ke);
+                    n = dim::bc_value(a.bc, ie, je + 1, ke);
+                    b = dim::bc_value(a.bc, ie, je, ke - 1);
+                    fr = dim::bc_value(a.bc, ie, je, ke + 1);
+                } else {             // walls: copies of the cell, basal heat flux
+                    if (i == 0) w = xc;
+                    if ((uint32_t)i == nx - 1) ea = xc;
+                    if (k == 0) b = xc;
+                    if (k == a.nz - 1) fr = xc;
+                    if (j == 0) s = DARCY ? xc : a.qyS_dy / kS + xc;
+                }
+                const double qW = -kW * (a._dx * (xc - w)), qE = -kE * (a._dx * (ea - xc));
+                const double qB = -kB * (a._dz * (xc - b)), qF = -kF * (a._dz * (fr - xc));
+                if (DARCY) {
+                    // ϕρf of the cell (kernels_HT3D_dimensional.jl:225-235), then
+                    // ComputeDarcyFlux! and ResidualFluidPressure! (:266-289)
+                    const double phi = (cl & C_FAULT) ? 3.0 * a.phi_i : a.phi_i;
+                    const double Tk = a.aux[e];
+                    const double prf = phi * dim::rhof_C(Tk * a.s_T - 273.15, xc * a.s_sigma) / a.s_rho;
+                    const double qS = -kS * (a._dy * (xc - s) - a.gy[fy]);
+                    const double qN = -kN * (a._dy * (n - xc) - a.gy[fy + nx]);
+                    f = a.tr_dt * (prf - a.aux2[c]);
+                    f += (qE - qW) * a._dx;
+                    f += (qN - qS) * a._dy;
+                    f += (qF - qB) * a._dz;
+                    D = a.Dst[c];
+                } else {
+                    // ComputeFlux! and ResidualTemperatureLinearised! (:140-181)
+                    const double rc = a.aux2[c];
+                    const double qS = -kS * (a._dy * (xc - s)), qN = -kN * (a._dy * (n - xc));
+                    f = a.transient * rc * a._dt * (xc - a.aux[c]);
+                    f += (qE - qW) * a._dx;
+                    f += (qN - qS) * a._dy;
+                    f += (qF - qB) * a._dz;
+                    f -= a.transient * ((cl & C_SRC) ? a.Qt : 0.0);
+                    D = (kB / a.dz + kF / a.dz) / a.dz + (kN / a.dy + kS / a.dy) / a.dy +
+                        (kE / a.dx + kW / a.dx) / a.dx + rc * a.transient / a.dt;
+                }
+                f /= D;
+            }
+            // DYRELUpdate! (kernels_HT3D_dimensional.jl:323-324)
+            const double f0 = a.h1 * f0o + a.h2 * f;
+            __builtin_nontemporal_store(f0, a.F0 + c);
+            a.Xout[e] = xc - a.dtau * f0;
+            if (check) {
+                const double fd = fabs(f * D);
+                dd_add(s_hi, s_lo, fd);
+                acc_max = (fd > acc_max || fd != fd) ? fd : acc_max;
+                if (lam) {
+                    const double d = a.dtau * f0;
+                    dd_add(n_hi, n_lo, (-d) * (f - a.Fst[c]));
+                    dd_add(d_hi, d_lo, d * d);
+                }
+            }
+            if (store) a.Fst[c] = f;
+            xm = xc;
+            xc = xp;
+            kzB = kF;
+            e += sE2;
+            c += sC2;
+            fx += sX2;
+            fy += sY2;
+        }
+    }
+    if (check) {
+        const double h4[4] = {s_hi, acc_max, n_hi, d_hi}, l4[4] = {s_lo, 0.0, n_lo, d_lo};
+        block_reduce4(h4, l4, 0x2u, a.partials, blockIdx.x);
+    }
+}
+
+struct Tile {
+    int ty, kz;
+};
+static constexpr Tile kV18Tiles[] = {{4, 4}, {4, 8}, {8, 4}, {2, 8}, {8, 8}, {4, 16}};
+static constexpr int kNumV18Tiles = sizeof(kV18Tiles) / sizeof(kV18Tiles[0]);
+
+template <bool DARCY>
+static void launch(int tile, Args &a, hipStream_t st) {
+    const Tile tl = kV18Tiles[tile];
+    a.nbx = (int)div_up(a.nx, 64);
+    a.nby = (int)div_up(a.ny, tl.ty);
+    a.nbz = (int)div_up(a.nz, tl.kz);
+    const unsigned nb = (unsigned)a.nbx * a.nby * a.nbz;
+    a.bc.A = a.Xin;
+    switch (tile) {
+    case 0: hipLaunchKernelGGL((iter_kernel<DARCY, 4, 4>), dim3(nb), dim3(256), 0, st, a); break;
+    case 1: hipLaunchKernelGGL((iter_kernel<DARCY, 4, 8>), dim3(nb), dim3(256), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((iter_kernel<DARCY, 8, 4>), dim3(nb), dim3(512), 0, st, a); break;
+    case 3: hipLaunchKernelGGL((iter_kernel<DARCY, 2, 8>), dim3(nb), dim3(128), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((iter_kernel<DARCY, 8, 8>), dim3(nb), dim3(512), 0, st, a); break;
+    default: hipLaunchKernelGGL((iter_kernel<DARCY, 4, 16>), dim3(nb), dim3(256), 0, st, a); break;
+    }
+}
+
+static unsigned max_blocks(int nx, int ny, int nz) {
+    unsigned m = 0;
+    for (const Tile &t : kV18Tiles) {
+        const unsigned nb = div_up(nx, 64) * div_up(ny, t.ty) * div_up(nz, t.kz);
+        m = nb > m ? nb : m;
+    }
+    return m;
+}
+
+// Tile choice per (grid, operator): each candidate is timed on three plain
+// iterations of the solve's own inputs, written to scratch (the unknown and
+// F0 of the solve are untouched).
+static std::map<std::tuple<int, int, int, int>, int> &tile_cache() {
+    static std::map<std::tuple<int, int, int, int>, int> m;
+    return m;
+}
+int g_v18_force_tile = -1;
+
+template <bool DARCY>
+static int choose_tile(const Args &base, double *scratch_x, double *scratch_f0, int *tile_out) {
+    if (g_v18_force_tile >= 0 && g_v18_force_tile < kNumV18Tiles) { *tile_out = g_v18_force_tile; return HT3D_OK; }
+    const auto key = std::make_tuple(base.nx, base.ny, base.nz, DARCY ? 1 : 0);
+    auto it = tile_cache().find(key);
+    if (it != tile_cache().end()) { *tile_out = it->second; return HT3D_OK; }
+    hipStream_t st = ctx().stream;
+    const size_t N = (size_t)base.nx * base.ny * base.nz;
+    HIP_TRY(hipMemsetAsync(scratch_f0, 0, N * sizeof(double), st));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int best = 0;
+    float best_ms = 1e30f;
+    for (int tile = 0; tile < kNumV18Tiles; ++tile) {
+        Args a = base;
+        a.Xout = scratch_x;
+        a.F0 = scratch_f0;
+        a.flags = 0;
+        launch<DARCY>(tile, a, st);   // warm-up
+        HIP_TRY(hipEventRecord(e0, st));
+        for (int r = 0; r < 3; ++r) launch<DARCY>(tile, a, st);
+        HIP_TRY(hipEventRecord(e1, st));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) { best_ms = ms; best = tile; }
+    }
+    HIP_TRY(hipGetLastError());
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    tile_cache()[key] = best;
+    *tile_out = best;
+    return HT3D_OK;
+}
+
+int g_v18_last_tile = -1;
+
+static inline bool lambda_iter(int iter, int nitout) { return iter <= 2 || (nitout > 0 && iter % nitout == 0); }
+
+template <bool DARCY>
+static int solve(const ht3d_v18_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep) {
+    HT3D_CHECK_INIT();
+    const char *name = DARCY ? "ht3d_v18_darcy_solve" : "ht3d_v18_thermal_solve";
+    if (!f || !p) return arg_fail("v18 solve: null argument");
+    const int nx = f->ncx, ny = f->ncy, nz = f->ncz;
+    if (!grid_ok(nx, ny, nz)) return arg_fail("v18 solve: extents must be >= 3");
+    if (!f->X || !f->kx || !f->ky || !f->kz || !f->phc_ex || !f->wN || !f->rhoC)
+        return arg_fail("v18 solve: null field pointer");
+    if (DARCY ? (!f->Tc_ex || !f->phirhof0 || !f->rhofv) : !f->Tc_old)
+        return arg_fail("v18 solve: null field pointer");
+    if (p->nitmax < 1 || p->nitout < 1 || !(p->tol >= 0.0)) return arg_fail("v18 solve: bad parameters");
+    if ((size_t)(nx + 2) * (ny + 2) * (nz + 2) >= ((size_t)1 << 32)) return arg_fail("v18 solve: grid too large");
+    Context &cx = ctx();
+    hipStream_t st = cx.stream;
+    const size_t N = (size_t)nx * ny * nz, NE = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+
+    void *xb, *f0, *f0b, *fst, *part, *res, *cls, *gy = nullptr, *pcw, *gw;
+    int rc;
+    if ((rc = workspace(WS_XB, NE * sizeof(double), &xb))) return rc;
+    if ((rc = workspace(WS_F0, N * sizeof(double), &f0))) return rc;
+    if ((rc = workspace(WS_F0B, N * sizeof(double), &f0b))) return rc;
+    if ((rc = workspace(WS_FCIT, N * sizeof(double), &fst))) return rc;
+    if ((rc = workspace(WS_PARTIALS, (size_t)max_blocks(nx, ny, nz) * 8 * sizeof(double), &part))) return rc;
+    if ((rc = workspace(WS_RESULT, 64 * sizeof(double), &res))) return rc;
+    if ((rc = workspace(WS_V18_CLS, N, &cls))) return rc;
+    if ((rc = workspace(WS_DIAG, N * sizeof(double), &pcw))) return rc;
+    if ((rc = workspace(WS_V18_G, N * sizeof(double), &gw))) return rc;
+    if (DARCY && (rc = workspace(WS_V18_GY, (size_t)nx * (ny + 1) * nz * sizeof(double), &gy))) return rc;
+    double *PC = f->PC ? f->PC : (double *)pcw;
+    double *G = f->G ? f->G : (double *)gw;
+
+    // set-up: GershgorinPoisson! and lambda_max (v18:432-439 / :495-502)
+    if ((rc = ht3d_GershgorinPoisson(G, PC, f->rhoC, f->kx, f->ky, f->kz, f->transient, f->dt, f->dx, f->dy, f->dz,
+                                     nx, ny, nz)))
+        return rc;
+    double lmax = 0.0;
+    if ((rc = ht3d_reduce(&lmax, G, N, HT3D_MAX))) return rc;
+    double lmin = lmax / p->lmin_div;
+    const double dtau = 2.0 / sqrt(lmax) * p->CFL;
+    double c = 2.0 * sqrt(lmin) * p->cfact;
+    double h1 = (2 - c * dtau) / (2 + c * dtau), h2 = 2 * dtau / (2 + c * dtau);
+
+    const unsigned nb_lin = div_up(N, 256) > 4096 ? 4096 : div_up(N, 256);
+    hipLaunchKernelGGL(class_kernel, dim3(nb_lin), dim3(256), 0, st, f->phc_ex, nx, ny, nz, (uint8_t *)cls);
+    if (DARCY)
+        hipLaunchKernelGGL(gravity_kernel, dim3(nb_lin), dim3(256), 0, st, f->rhofv, f->g, nx, ny, nz, (double *)gy);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemsetAsync(f0, 0, N * sizeof(double), st));    // ResetA!(Fc, Fc0)
+    HIP_TRY(hipMemsetAsync(fst, 0, N * sizeof(double), st));
+    // the second buffer needs the ghosts that no rule rewrites
+    if ((rc = copy_ghosts((double *)xb, f->X, nx, ny, nz))) return rc;
+
+    Args a;
+    std::memset(&a, 0, sizeof a);
+    a.nx = nx; a.ny = ny; a.nz = nz;
+    a.kx = f->kx; a.ky = f->ky; a.kz = f->kz;
+    a.cls = (const uint8_t *)cls;
+    a.Fst = (double *)fst;
+    a.partials = (double *)part;
+    a.aux = DARCY ? f->Tc_ex : f->Tc_old;
+    a.aux2 = DARCY ? f->phirhof0 : f->rhoC;
+    a.Dst = PC;
+    a.gy = (const double *)gy;
+    a._dx = 1.0 / f->dx; a._dy = 1.0 / f->dy; a._dz = 1.0 / f->dz;
+    a.dx = f->dx; a.dy = f->dy; a.dz = f->dz; a.dt = f->dt;
+    a._dt = 1.0 / f->dt;
+    a.transient = f->transient;
+    a.tr_dt = f->transient * a._dt;
+    a.Qt = f->Qt;
+    a.qyS_dy = f->qyS / a._dy;
+    a.phi_i = f->phi_i; a.s_rho = f->scale_rho; a.s_T = f->scale_T; a.s_sigma = f->scale_sigma;
+    a.bc = dim::BcArgs{f->X, f->ky, f->wN, f->phc_ex, f->top, a.qyS_dy, Grid{nx, ny, nz}, DARCY ? 0 : 1};
+    a.dtau = dtau; a.h1 = h1; a.h2 = h2;
+    a.Xin = f->X;
+    a.F0 = (double *)f0;
+    int tile = 0;
+    if ((rc = choose_tile<DARCY>(a, (double *)xb, (double *)f0b, &tile))) return rc;   // writes xb's interior only
+    g_v18_last_tile = tile;
+
+    const int ncheck = p->ncheck < 0 ? tuned_check_interval(-1.0, -1.0, -1.0, p->nitout) : p->ncheck;
+    double *buf[2] = {f->X, (double *)xb};
+    int cur = 0;
+    SegmentTimer timer;
+    // algorithmic bytes per cell: unknown and F0 read and written, five (heat)
+    // or seven (Darcy) read-only words, the class byte
+    const double bytes_per_iter = ((DARCY ? 11.0 : 9.0) * 8.0 + 1.0) * (double)N;
+    double nF_abs = 0.0, nF_rel = 0.0, nF_ini = 0.0;
+    int status = HT3D_NOT_CONVERGED, iter, n_checks = 0, n_lambda = 0, done_iters = 0;
+    for (iter = 1; iter <= p->nitmax; ++iter) {
+        const bool lam = lambda_iter(iter, p->nitout);
+        const bool chk = lam || (ncheck > 0 && iter % ncheck == 0);
+        const bool store = iter < p->nitmax && lambda_iter(iter + 1, p->nitout);
+        a.Xin = buf[cur];
+        a.Xout = buf[cur ^ 1];
+        a.h1 = h1; a.h2 = h2;
+        a.flags = (chk ? FL_CHECK : 0u) | (lam ? FL_LAMBDA : 0u) | (store ? FL_STORE_F : 0u);
+        timer.before_launch();
+        launch<DARCY>(tile, a, st);
+        HIP_TRY(hipGetLastError());
+        ++timer.count;
+        cur ^= 1;
+        done_iters = iter;
+        if (chk || iter == p->nitmax) timer.close();
+        if (chk) {
+            ++n_checks;
+            const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
+            if ((rc = finish_partials((double *)part, nblocks, 0x2u, (double *)res))) return rc;
+            double r4[4];
+            if ((rc = read_result4((double *)res, r4))) return rc;
+            timer.collect(bytes_per_iter, false);
+            nF_abs = (p->norm == 0 ? r4[0] / (double)N / sqrt((double)N) : r4[1]) * f->norm_scale;
+            if (r4[0] != r4[0]) nF_abs = r4[0];
+            if (iter == 1) nF_ini = nF_abs;
+            nF_rel = nF_abs / nF_ini;
+            if (std::isnan(nF_abs)) { status = HT3D_NAN; break; }
+            if (nF_rel > 100) { status = HT3D_DIVERGED; break; }
+            if (lam && iter > 1) {
+                lmin = fabs(r4[2]);
+                lmin /= fabs(r4[3]);
+                c = 2.0 * sqrt(lmin) * p->cfact;
+                h1 = (2 - c * dtau) / (2 + c * dtau);
+                h2 = 2 * dtau / (2 + c * dtau);
+                ++n_lambda;
+            }
+            if (nF_abs < p->tol || nF_rel < p->tol) { status = HT3D_OK; break; }
+        }
+    }
+    timer.close();
+    timer.collect(bytes_per_iter, false);
+
+    // Finalise as the reference leaves memory: the BC of the last iteration
+    // applied to its input (ghosts, air and immersed cells), the update on the rest.
+    double *res_buf = buf[cur], *prev = buf[cur ^ 1];
+    if (DARCY) {   // ϕρf of the last iteration (before its BC and update)
+        if ((rc = ht3d_PhiRhof(f->rhoC, f->Tc_ex, prev, f->phc_ex, f->phi_i, f->scale_rho, f->scale_T,
+                               f->scale_sigma, nx, ny, nz)))
+            return rc;
+        if ((rc = ht3d_dim_SetPressureBCs(prev, f->wN, f->phc_ex, 0.0, f->top, 0, 0, 0, 0, 0, 0, 0, 0, nx, ny, nz)))
+            return rc;
+    } else {
+        if ((rc = ht3d_dim_SetTemperatureBCs(prev, f->ky, f->wN, f->phc_ex, f->qyS, a._dy, 0, 0, f->top, 0, 0, 0, 0,
+                                             0, 0, 0, 0, 0, 0, nx, ny, nz)))
+            return rc;
+    }
+    dim::BcArgs bc = a.bc;
+    bc.A = prev;
+    const unsigned nb_e = div_up(NE, 256) > 4096 ? 4096 : div_up(NE, 256);
+    hipLaunchKernelGGL(copy_special_kernel, dim3(nb_e), dim3(256), 0, st, bc, res_buf);
+    HIP_TRY(hipGetLastError());
+    if (res_buf != f->X) HIP_TRY(hipMemcpyAsync(f->X, res_buf, NE * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (rep) {
+        rep->ncheck = ncheck;
+        rep->iters = done_iters;
+        rep->status = status;
+        rep->n_checks = n_checks;
+        rep->n_lambda = n_lambda;
+        rep->lmax = lmax; rep->lmin = lmin; rep->dtau = dtau; rep->h1 = h1; rep->h2 = h2;
+        rep->nF_abs = nF_abs; rep->nF_rel = nF_rel; rep->nF_ini = nF_ini;
+    }
+    rc = after_launch(name, true);
+    if (rc) return rc;
+    return status;
+}
+
+}  // namespace v18
+}  // namespace ht3d
+
+extern "C" {
+
+int ht3d_v18_thermal_solve(const ht3d_v18_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep) {
+    return ht3d::v18::solve<false>(f, p, rep);
+}
+
+int ht3d_v18_darcy_solve(const ht3d_v18_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep) {
+    return ht3d::v18::solve<true>(f, p, rep);
+}
+
+int ht3d_v18_tune(int force_tile, int *last_tile) {
+    HT3D_CHECK_INIT();
+    if (force_tile >= ht3d::v18::kNumV18Tiles) return ht3d::arg_fail("ht3d_v18_tune: tile index"), HT3D_ERR_ARG;
+    ht3d::v18::g_v18_force_tile = force_tile;
+    if (last_tile) *last_tile = ht3d::v18::g_v18_last_tile;
+    return HT3D_OK;
+}
+
+}  // extern "C"

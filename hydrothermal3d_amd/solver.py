@@ -48,7 +48,12 @@ def production_params(tol: float = 1e-8, nitout: int = 100, ncheck: int = 50, ni
 
 
 class SolveError(RuntimeError):
-    pass
+    """A solve stopped on NaN or divergence (the reference's error() calls);
+    `report` holds the solve report when the caller has one."""
+
+    def __init__(self, msg: str, report: dict | None = None):
+        super().__init__(msg)
+        self.report = report
 
 
 def _step_dict(rep: _lib.StepReport) -> dict:
@@ -58,11 +63,11 @@ def _step_dict(rep: _lib.StepReport) -> dict:
     return out
 
 
-def _status_check(status: int, what: str) -> None:
+def _status_check(status: int, what: str, report: dict | None = None) -> None:
     if status == _lib.NAN:
-        raise SolveError(f"Nan {what}...")          # v18:464
+        raise SolveError(f"Nan {what}...", report)          # v18:464
     if status == _lib.DIVERGED:
-        raise SolveError(f"Diverged {what}...")     # v18:465
+        raise SolveError(f"Diverged {what}...", report)     # v18:465
 
 
 def dyrel_solve(X: DeviceArray, R: DeviceArray, kx: DeviceArray, ky: DeviceArray, kz: DeviceArray, *,
@@ -88,7 +93,7 @@ def dyrel_solve(X: DeviceArray, R: DeviceArray, kx: DeviceArray, ky: DeviceArray
         out.update(dc_outer=outer.value, dc_inner=inner.value)
     if st not in (_lib.OK, _lib.NOT_CONVERGED, _lib.NAN, _lib.DIVERGED):
         _lib.check(st, "ht3d_dyrel_solve")
-    _status_check(st, "T" if transient else "P")
+    _status_check(st, "T" if transient else "P", out)
     return out
 
 

@@ -331,6 +331,40 @@ typedef struct {
 
 int ht3d_dyrel_solve(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep);
 
+/* ---- Fused DYREL solves of the dimensional (v18) model: the heat loop of
+ * Dimensional/HydroThermal3D_v18.jl:432-478 (SetTemperatureBCs!, ComputeFlux!,
+ * ResidualTemperatureLinearised!, DYRELUpdate!) and the Darcy loop of :489-537
+ * (ϕρf, SetPressureBCs!, ComputeDarcyFlux!, ResidualFluidPressure!,
+ * DYRELUpdate!), from GershgorinPoisson! to convergence, one launch per
+ * pseudo-iteration.  The caller runs the property kernels before (as the v18
+ * driver does: conductivities, rho*C or phi*drho/dP, ϕρf_old, fluid density).
+ * Same iterations, lambda updates and fields as the unfused kernel sequence;
+ * memory is left as that sequence leaves it (see ht3d_v18.hip).             */
+typedef struct {
+    int ncx, ncy, ncz;
+    double dx, dy, dz, dt, transient;
+    double *X;                    /* Tc_ex (heat) or Pc_ex (Darcy): E, in/out                       */
+    const double *kx, *ky, *kz;   /* face coefficients (InitConductivity!)                          */
+    const double *phc_ex;         /* E: centroid phases (1 air, -1 immersed, 2 fault, 3 intrusion)  */
+    const double *wN;             /* (ncx+2, ncz+2): immersed-surface weights                       */
+    double *rhoC;                 /* c: heat: rho*C (in); Darcy: phi*drho/dP (in), phi*rho_f (out)   */
+    double *PC, *G;               /* c: Gershgorin diagonal and bound (out; NULL: internal)         */
+    double top;                   /* TN or PN, the surface value                                    */
+    double norm_scale;            /* dimensional scale of the norm (v18:460 / :517)                 */
+    const double *Tc_old;         /* heat: c, temperature of the previous step (Tc0)                */
+    double qyS, Qt;               /* heat: basal heat flux; heat source of phase-3 cells            */
+    const double *Tc_ex;          /* Darcy: E, temperature of the fluid-density closure            */
+    const double *phirhof0;       /* Darcy: c, phi*rho_f of the previous step (ϕρf_old, Xc0)         */
+    const double *rhofv;          /* Darcy: V, fluid density (ComputeFluidDensity)                  */
+    double g, phi_i, scale_rho, scale_T, scale_sigma;   /* Darcy: gravity, porosity, scales        */
+} ht3d_v18_field;
+int ht3d_v18_thermal_solve(const ht3d_v18_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep);
+int ht3d_v18_darcy_solve(const ht3d_v18_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep);
+/* tile of the v18 iteration kernel: force_tile >= 0 forces one of the six
+ * candidates, -1 lets the autotuner choose; *last_tile (may be NULL) returns
+ * the tile of the last solve.  Analogue of cublocks/cuthreads.              */
+int ht3d_v18_tune(int force_tile, int *last_tile);
+
 /* FP32 versions of the inner-sweep kernels for a defect-correction driver (the
  * fused ht3d_dyrel_solve_dc runs the same arithmetic).  Same argument order as
  * the FP64 entries, float arrays and scalars, expressions evaluated in float in

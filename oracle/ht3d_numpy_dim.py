@@ -240,3 +240,47 @@ def SetPressureBCs_serial(P, wN, ph, PN):
                 if iz == n3 and 1 < ix <= n1 - 1 and 1 < iy <= n2 - 1:
                     P[a(ix, iy, n3)] = P[a(ix, iy, n3 - 1)]
     return P
+
+
+# ------------------------------------------------------ v18 residuals and fluxes
+def ComputeFlux(kx, ky, kz, A, _dx, _dy, _dz):
+    """kernels_HT3D_dimensional.jl:140-146"""
+    I = slice(1, -1)
+    return (-kx * (_dx * (A[1:, I, I] - A[:-1, I, I])),
+            -ky * (_dy * (A[I, 1:, I] - A[I, :-1, I])),
+            -kz * (_dz * (A[I, I, 1:] - A[I, I, :-1])))
+
+
+def ComputeDarcyFlux(rhofv, kx, ky, kz, P, g, _dx, _dy, _dz):
+    """kernels_HT3D_dimensional.jl:266-272 (rho_f averaged to the y faces)"""
+    from oracle.ht3d_numpy import av_xza
+    I = slice(1, -1)
+    return (-kx * (_dx * (P[1:, I, I] - P[:-1, I, I])),
+            -ky * (_dy * (P[I, 1:, I] - P[I, :-1, I]) - g * av_xza(rhofv)),
+            -kz * (_dz * (P[I, I, 1:] - P[I, I, :-1])))
+
+
+def _flux_div(F, qx, qy, qz, _dx, _dy, _dz):
+    F = F + (qx[1:] - qx[:-1]) * _dx
+    F = F + (qy[:, 1:] - qy[:, :-1]) * _dy
+    return F + (qz[:, :, 1:] - qz[:, :, :-1]) * _dz
+
+
+def ResidualTemperatureLinearised(T, Told, rhoC, ph, PC, qx, qy, qz, Qt, transient, _dt, _dx, _dy, _dz):
+    """kernels_HT3D_dimensional.jl:162-181"""
+    I = slice(1, -1)
+    p = ph[I, I, I]
+    F = transient * rhoC * _dt * (T[I, I, I] - Told)
+    F = _flux_div(F, qx, qy, qz, _dx, _dy, _dz)
+    F = F - transient * np.where(p == 3.0, Qt, 0.0)
+    F = F / PC
+    return np.asfortranarray(np.where(np.abs(p) != 1.0, F, 0.0))
+
+
+def ResidualFluidPressure(ph, phirhof, phirhof0, PC, qx, qy, qz, transient, _dt, _dx, _dy, _dz):
+    """kernels_HT3D_dimensional.jl:274-289"""
+    I = slice(1, -1)
+    p = ph[I, I, I]
+    F = transient * _dt * (phirhof - phirhof0)
+    F = _flux_div(F, qx, qy, qz, _dx, _dy, _dz) / PC
+    return np.asfortranarray(np.where(np.abs(p) != 1.0, F, 0.0))

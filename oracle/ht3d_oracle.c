@@ -558,7 +558,6 @@ EXPORT void o_Advect_v2(double *Fc, const double *X, const double *V, int axis, 
 /* Reductions (maximum_g / minimum_g / mean_g / sum_g, tools/Macros.jl:20-25) */
 /* ------------------------------------------------------------------------ */
 
-enum { R_SUM = 0, R_SUM_ABS = 1, R_MAX = 2, R_MIN = 3, R_MAX_ABS = 4 };
 
 /* Sums are accumulated in double-double (TwoSum, no FMA), so the rounded result
  * is the correctly rounded sum except in vanishingly rare ties, whatever the
@@ -614,25 +613,7 @@ EXPORT double o_reduce(const double *a, size_t n1, size_t n23, int op) {
 /* DYREL solve: Dimensional/HydroThermal3D_v18.jl:424-538                    */
 /* ------------------------------------------------------------------------ */
 
-typedef struct {
-    int nitmax;      /* maximum pseudo-iterations (v18: 1e4)                          */
-    int nitout;      /* lambda_min / reference-norm schedule: iter<=2 || iter%nitout==0  */
-    int ncheck;      /* >0: also test convergence every ncheck iterations               */
-    int norm;        /* 0: mean|F*D|/sqrt(N) (v18:459-460); 1: max|F*D|                 */
-    double tol;      /* converged if norm < tol or norm/norm_ini < tol (v18:476)         */
-    double CFL;      /* 0.99 (v18:435)                                                  */
-    double cfact;    /* 0.5  (v18:436)                                                  */
-    double lmin_div; /* 500  (v18:434)                                                  */
-} o_dyrel_params;
-
-typedef struct {
-    int iters;
-    int status;      /* 0 converged, 10 nitmax, 11 NaN, 12 diverged (norm_rel > 100) */
-    int n_checks;
-    int n_lambda;
-    double lmax, lmin, dtau, h1, h2;
-    double nF_abs, nF_rel, nF_ini;
-} o_dyrel_report;
+/* o_dyrel_params and o_dyrel_report: ht3d_oracle_common.h */
 
 typedef struct {
     int nx, ny, nz;

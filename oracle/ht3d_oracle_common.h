@@ -35,4 +35,53 @@
 #define AV_XYA(A, n1, n2) ((AT(A, n1, n2, i, j, k) + AT(A, n1, n2, i + 1, j, k) + AT(A, n1, n2, i, j + 1, k) + AT(A, n1, n2, i + 1, j + 1, k)) * 0.25)
 #define AV_YI(A, n1, n2) ((AT(A, n1, n2, i + 1, j, k + 1) + AT(A, n1, n2, i + 1, j + 1, k + 1)) * 0.5)
 
+/* DYREL parameters and report (Dimensional/HydroThermal3D_v18.jl:424-538) */
+typedef struct {
+    int nitmax;      /* maximum pseudo-iterations (v18: 1e4)                          */
+    int nitout;      /* lambda_min / reference-norm schedule: iter<=2 || iter%nitout==0  */
+    int ncheck;      /* >0: also test convergence every ncheck iterations               */
+    int norm;        /* 0: mean|F*D|/sqrt(N) (v18:459-460); 1: max|F*D|                 */
+    double tol;      /* converged if norm < tol or norm/norm_ini < tol (v18:476)         */
+    double CFL;      /* 0.99 (v18:435)                                                  */
+    double cfact;    /* 0.5  (v18:436)                                                  */
+    double lmin_div; /* 500  (v18:434)                                                  */
+} o_dyrel_params;
+
+typedef struct {
+    int iters;
+    int status;      /* 0 converged, 10 nitmax, 11 NaN, 12 diverged (norm_rel > 100) */
+    int n_checks;
+    int n_lambda;
+    double lmax, lmin, dtau, h1, h2;
+    double nF_abs, nF_rel, nF_ini;
+} o_dyrel_report;
+
+enum { R_SUM = 0, R_SUM_ABS = 1, R_MAX = 2, R_MIN = 3, R_MAX_ABS = 4 };
+
+/* kernels of ht3d_oracle.c used by the v18 solves of ht3d_oracle_dim.c */
+double o_reduce(const double *a, size_t n1, size_t n23, int op);
+void o_ResetA(double *A, double *B, int n1, int n2, int n3);
+void o_SwapDYREL(double *Xc, double *Yc, const double *X0c_ex, const double *Y0c, int nx, int ny, int nz);
+void o_ComputeFlux(double *qx, double *qy, double *qz, const double *kx, const double *ky, const double *kz,
+                   const double *A, double _dx, double _dy, double _dz, int nx, int ny, int nz);
+void o_ComputeDarcyFlux(double *qx, double *qy, double *qz, const double *rhofv, const double *kx, const double *ky,
+                        const double *kz, const double *Pf, double g, double _dx, double _dy, double _dz, int nx,
+                        int ny, int nz);
+void o_ResidualTemperatureLinearised(double *F, const double *Tc_ex, const double *Tc_old, const double *rhoC_eff,
+                                     const double *phc_ex, const double *PC, const double *qx, const double *qy,
+                                     const double *qz, double Qt, double transient, double _dt, double _dx,
+                                     double _dy, double _dz, int nx, int ny, int nz);
+void o_ResidualFluidPressure(double *F, const double *phc_ex, const double *phirhof, const double *phirhof0,
+                             const double *PC, const double *qx, const double *qy, const double *qz,
+                             double transient, double _dt, double _dx, double _dy, double _dz, int nx, int ny,
+                             int nz);
+void o_DYRELUpdate(double *F0, double *X, const double *F, double h1, double h2, double dtau, int nx, int ny, int nz);
+void o_Multiply(double *a, const double *b, const double *c, int nx, int ny, int nz);
+void o_GershgorinPoisson(double *G, double *D, const double *rhoC, const double *kx, const double *ky,
+                         const double *kz, double transient, double dt, double dx, double dy, double dz, int nx,
+                         int ny, int nz);
+void o_RayleighQuotientNumerator(double *G, const double *Fc0, const double *Fcit, const double *Fc, double dtau,
+                                 int nx, int ny, int nz);
+void o_RayleighQuotientDenominator(double *G, const double *Fc0, double dtau, int nx, int ny, int nz);
+
 #endif

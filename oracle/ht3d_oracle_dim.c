@@ -349,3 +349,124 @@ EXPORT void o_dim_SetPressureBCs(double *Pc_ex, const double *wN, const double *
     bc_ctx c = {nx, ny, nz, 0, NULL, NULL, wN, phc_ex, PN, 0.0};
     bc_apply(&c, Pc_ex);
 }
+
+/* ------------------------------------------------------------------------ */
+/* v18 DYREL solves, unfused: Dimensional/HydroThermal3D_v18.jl:432-478 (heat) */
+/* and :489-537 (Darcy), from the Gershgorin set-up to convergence.           */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int nx, ny, nz;
+    double dx, dy, dz, dt, transient;
+    double *Tc_ex;                 /* E, in/out                                          */
+    const double *Tc0;             /* c: temperature of the previous step (InitThermal!) */
+    const double *rhoC;            /* c: rho*C (ComputeρCeffective!)                       */
+    const double *kx, *ky, *kz;    /* face conductivities (InitConductivity!)            */
+    const double *phc_ex, *wN;
+    double qyS, TN, Qt, norm_scale;
+    double *G, *PC, *F, *F0, *Fcit, *dumc, *qx, *qy, *qz;   /* left as the reference leaves them */
+} o_v18_thermal;
+
+typedef struct {
+    int nx, ny, nz;
+    double dx, dy, dz, dt, transient;
+    double *Pc_ex;                 /* E, in/out                                              */
+    const double *Tc_ex;           /* E: temperature of the fluid-density closure            */
+    double *phirho;                /* c: phi*drho/dP on entry, phi*rho_f of the last iteration on exit */
+    const double *phirhof0;        /* c: phi*rho_f of the previous step (ϕρf_old)            */
+    const double *rhofv;           /* V: fluid density (ComputeFluidDensity)                 */
+    const double *kx, *ky, *kz, *phc_ex, *wN;
+    double PN, g, phi_i, scale_rho, scale_T, scale_sigma, norm_scale;
+    double *G, *PC, *F, *F0, *Fcit, *dumc, *qx, *qy, *qz;
+} o_v18_darcy;
+
+/* The DYREL control flow shared by both solves (v18:432-478): `sweep` runs the
+   reference's per-iteration kernels on the unknown X. */
+typedef void (*sweep_fn)(void *ctx);
+
+static int v18_control(int nx, int ny, int nz, const double *rhoC, const double *kx, const double *ky,
+                       const double *kz, double transient, double dt, double dx, double dy, double dz, double *X,
+                       double *G, double *PC, double *F, double *F0, double *Fcit, double *dumc, double norm_scale,
+                       const o_dyrel_params *p, o_dyrel_report *rep, sweep_fn sweep, void *ctx) {
+    const size_t N = (size_t)nx * ny * nz;
+    o_GershgorinPoisson(G, PC, rhoC, kx, ky, kz, transient, dt, dx, dy, dz, nx, ny, nz);
+    double lmax = o_reduce(G, (size_t)nx, (size_t)ny * nz, R_MAX);
+    double lmin = lmax / p->lmin_div;
+    const double dtau = 2.0 / sqrt(lmax) * p->CFL;
+    double c = 2.0 * sqrt(lmin) * p->cfact;
+    double h1 = (2 - c * dtau) / (2 + c * dtau), h2 = 2 * dtau / (2 + c * dtau);
+    o_ResetA(F, F0, nx, ny, nz);
+    double nF_abs = 0.0, nF_rel = 0.0, nF_ini = 0.0;
+    int status = 10, iter, n_checks = 0, n_lambda = 0;
+    for (iter = 1; iter <= p->nitmax; ++iter) {
+        const int lam = iter <= 2 || iter % p->nitout == 0;
+        const int chk = lam || (p->ncheck > 0 && iter % p->ncheck == 0);
+        if (lam) o_SwapDYREL(NULL, Fcit, X, F, nx, ny, nz);
+        sweep(ctx);
+        o_DYRELUpdate(F0, X, F, h1, h2, dtau, nx, ny, nz);
+        if (chk) {
+            ++n_checks;
+            o_Multiply(dumc, F, PC, nx, ny, nz);
+            if (p->norm == 0)
+                nF_abs = o_reduce(dumc, (size_t)nx, (size_t)ny * nz, R_SUM_ABS) / (double)N / sqrt((double)N) * norm_scale;
+            else
+                nF_abs = o_reduce(dumc, (size_t)nx, (size_t)ny * nz, R_MAX_ABS) * norm_scale;
+            if (iter == 1) nF_ini = nF_abs;
+            nF_rel = nF_abs / nF_ini;
+            if (isnan(nF_abs)) { status = 11; break; }
+            if (nF_rel > 100) { status = 12; break; }
+            if (lam && iter > 1) {
+                o_RayleighQuotientNumerator(dumc, F0, Fcit, F, dtau, nx, ny, nz);
+                lmin = fabs(o_reduce(dumc, (size_t)nx, (size_t)ny * nz, R_SUM));
+                o_RayleighQuotientDenominator(dumc, F0, dtau, nx, ny, nz);
+                lmin /= fabs(o_reduce(dumc, (size_t)nx, (size_t)ny * nz, R_SUM));
+                c = 2.0 * sqrt(lmin) * p->cfact;
+                h1 = (2 - c * dtau) / (2 + c * dtau);
+                h2 = 2 * dtau / (2 + c * dtau);
+                ++n_lambda;
+            }
+            if (nF_abs < p->tol || nF_rel < p->tol) { status = 0; break; }
+        }
+    }
+    if (rep) {
+        rep->iters = iter > p->nitmax ? p->nitmax : iter;
+        rep->status = status;
+        rep->n_checks = n_checks;
+        rep->n_lambda = n_lambda;
+        rep->lmax = lmax; rep->lmin = lmin; rep->dtau = dtau; rep->h1 = h1; rep->h2 = h2;
+        rep->nF_abs = nF_abs; rep->nF_rel = nF_rel; rep->nF_ini = nF_ini;
+    }
+    return status;
+}
+
+static void thermal_sweep(void *ctx) {   /* v18:449-452 */
+    o_v18_thermal *f = (o_v18_thermal *)ctx;
+    const int nx = f->nx, ny = f->ny, nz = f->nz;
+    const double _dx = 1.0 / f->dx, _dy = 1.0 / f->dy, _dz = 1.0 / f->dz, _dt = 1.0 / f->dt;
+    o_dim_SetTemperatureBCs(f->Tc_ex, f->ky, f->wN, f->phc_ex, f->qyS, _dy, 0, 0, f->TN, 0, 0, 0, 0, 0, 0, 0, 0, 0,
+                            0, nx, ny, nz);
+    o_ComputeFlux(f->qx, f->qy, f->qz, f->kx, f->ky, f->kz, f->Tc_ex, _dx, _dy, _dz, nx, ny, nz);
+    o_ResidualTemperatureLinearised(f->F, f->Tc_ex, f->Tc0, f->rhoC, f->phc_ex, f->PC, f->qx, f->qy, f->qz, f->Qt,
+                                    f->transient, _dt, _dx, _dy, _dz, nx, ny, nz);
+}
+
+static void darcy_sweep(void *ctx) {     /* v18:508-512 */
+    o_v18_darcy *f = (o_v18_darcy *)ctx;
+    const int nx = f->nx, ny = f->ny, nz = f->nz;
+    const double _dx = 1.0 / f->dx, _dy = 1.0 / f->dy, _dz = 1.0 / f->dz, _dt = 1.0 / f->dt;
+    o_PhiRhof(f->phirho, f->Tc_ex, f->Pc_ex, f->phc_ex, f->phi_i, f->scale_rho, f->scale_T, f->scale_sigma, nx, ny, nz);
+    o_dim_SetPressureBCs(f->Pc_ex, f->wN, f->phc_ex, 0, f->PN, 0, 0, 0, 0, 0, 0, 0, 0, nx, ny, nz);
+    o_ComputeDarcyFlux(f->qx, f->qy, f->qz, f->rhofv, f->kx, f->ky, f->kz, f->Pc_ex, f->g, _dx, _dy, _dz, nx, ny, nz);
+    o_ResidualFluidPressure(f->F, f->phc_ex, f->phirho, f->phirhof0, f->PC, f->qx, f->qy, f->qz, f->transient, _dt,
+                            _dx, _dy, _dz, nx, ny, nz);
+}
+
+EXPORT int o_v18_thermal_solve(o_v18_thermal *f, const o_dyrel_params *p, o_dyrel_report *rep) {
+    return v18_control(f->nx, f->ny, f->nz, f->rhoC, f->kx, f->ky, f->kz, f->transient, f->dt, f->dx, f->dy, f->dz,
+                       f->Tc_ex, f->G, f->PC, f->F, f->F0, f->Fcit, f->dumc, f->norm_scale, p, rep, thermal_sweep, f);
+}
+
+EXPORT int o_v18_darcy_solve(o_v18_darcy *f, const o_dyrel_params *p, o_dyrel_report *rep) {
+    return v18_control(f->nx, f->ny, f->nz, f->phirho, f->kx, f->ky, f->kz, f->transient, f->dt, f->dx, f->dy, f->dz,
+                       f->Pc_ex, f->G, f->PC, f->F, f->F0, f->Fcit, f->dumc, f->norm_scale, p, rep, darcy_sweep, f);
+}

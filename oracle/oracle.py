@@ -406,3 +406,59 @@ class HRLModel:
             self.close()
         except Exception:
             pass
+
+
+# --------------------------------------------------------------- v18 DYREL solves (ht3d_oracle_dim.c)
+class V18Thermal(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("dx", C.c_double), ("dy", C.c_double),
+                ("dz", C.c_double), ("dt", C.c_double), ("transient", C.c_double), ("Tc_ex", C.c_void_p),
+                ("Tc0", C.c_void_p), ("rhoC", C.c_void_p), ("kx", C.c_void_p), ("ky", C.c_void_p),
+                ("kz", C.c_void_p), ("phc_ex", C.c_void_p), ("wN", C.c_void_p), ("qyS", C.c_double),
+                ("TN", C.c_double), ("Qt", C.c_double), ("norm_scale", C.c_double)] + \
+               [(n, C.c_void_p) for n in ("G", "PC", "F", "F0", "Fcit", "dumc", "qx", "qy", "qz")]
+
+
+class V18Darcy(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("dx", C.c_double), ("dy", C.c_double),
+                ("dz", C.c_double), ("dt", C.c_double), ("transient", C.c_double), ("Pc_ex", C.c_void_p),
+                ("Tc_ex", C.c_void_p), ("phirho", C.c_void_p), ("phirhof0", C.c_void_p), ("rhofv", C.c_void_p),
+                ("kx", C.c_void_p), ("ky", C.c_void_p), ("kz", C.c_void_p), ("phc_ex", C.c_void_p),
+                ("wN", C.c_void_p), ("PN", C.c_double), ("g", C.c_double), ("phi_i", C.c_double),
+                ("scale_rho", C.c_double), ("scale_T", C.c_double), ("scale_sigma", C.c_double),
+                ("norm_scale", C.c_double)] + \
+               [(n, C.c_void_p) for n in ("G", "PC", "F", "F0", "Fcit", "dumc", "qx", "qy", "qz")]
+
+
+def _v18_work(nx, ny, nz):
+    w = {k: np.zeros((nx, ny, nz), order="F") for k in ("G", "PC", "F", "F0", "Fcit", "dumc")}
+    w["qx"] = np.zeros((nx + 1, ny, nz), order="F")
+    w["qy"] = np.zeros((nx, ny + 1, nz), order="F")
+    w["qz"] = np.zeros((nx, ny, nz + 1), order="F")
+    return w
+
+
+def v18_thermal_solve(Tc_ex, Tc0, rhoC, kx, ky, kz, phc_ex, wN, *, dx, dy, dz, dt, transient, qyS, TN, Qt=0.0,
+                      norm_scale=1.0, p: Params | None = None):
+    """HydroThermal3D_v18.jl:432-478, unfused; returns (report, work arrays)."""
+    nx, ny, nz = _grid(Tc_ex, "E")
+    w = _v18_work(nx, ny, nz)
+    f = V18Thermal(nx, ny, nz, dx, dy, dz, dt, float(transient), _ptr(Tc_ex), _ptr(Tc0), _ptr(rhoC), _ptr(kx),
+                   _ptr(ky), _ptr(kz), _ptr(phc_ex), _ptr(wN), qyS, TN, Qt, norm_scale,
+                   *(_ptr(w[k]) for k in ("G", "PC", "F", "F0", "Fcit", "dumc", "qx", "qy", "qz")))
+    rep = Report()
+    lib().o_v18_thermal_solve(C.byref(f), C.byref(p or params()), C.byref(rep))
+    return rep.as_dict(), w
+
+
+def v18_darcy_solve(Pc_ex, Tc_ex, phirho, phirhof0, rhofv, kx, ky, kz, phc_ex, wN, *, dx, dy, dz, dt, transient, PN,
+                    g, phi_i, scale_rho, scale_T, scale_sigma, norm_scale=1.0, p: Params | None = None):
+    """HydroThermal3D_v18.jl:489-537, unfused; returns (report, work arrays)."""
+    nx, ny, nz = _grid(Pc_ex, "E")
+    w = _v18_work(nx, ny, nz)
+    f = V18Darcy(nx, ny, nz, dx, dy, dz, dt, float(transient), _ptr(Pc_ex), _ptr(Tc_ex), _ptr(phirho),
+                 _ptr(phirhof0), _ptr(rhofv), _ptr(kx), _ptr(ky), _ptr(kz), _ptr(phc_ex), _ptr(wN), PN, g, phi_i,
+                 scale_rho, scale_T, scale_sigma, norm_scale,
+                 *(_ptr(w[k]) for k in ("G", "PC", "F", "F0", "Fcit", "dumc", "qx", "qy", "qz")))
+    rep = Report()
+    lib().o_v18_darcy_solve(C.byref(f), C.byref(p or params()), C.byref(rep))
+    return rep.as_dict(), w

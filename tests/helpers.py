@@ -62,3 +62,89 @@ def dim_state(nx, ny, nz, rng, air_from=None, immersed=False, TN=None, PN=None):
             for i, k in cols:
                 phc[i, air_from - 1, k] = -1.0
     return dict(sc=sc, T=T, P=P, phc=phc, phv=phv, TN=TN, PN=PN, wN=wN, sh=sh)
+
+
+def v18_problem(nx, ny, nz, seed=0, immersed=False, Qt=None):
+    """A small dimensional (v18) problem in the reference's scaled units, with
+    the property set-up of Dimensional/HydroThermal3D_v18.jl:426-430 (heat) and
+    :484-503 (Darcy) done by the CPU oracle: a domain 70 x 35 x 50 km with a
+    sloping surface (air above), a fault band (phase 2), an intrusion (phase 3),
+    a geotherm and a hydrostatic-like pressure, optionally an immersed-surface
+    layer (phase -1, the lowest air cell of each column, weight in (0.2, 1)).
+    Returns a dict of Fortran arrays and scalars for both solves."""
+    from oracle import oracle as orc
+    from hydrothermal3d_amd.dimensional import v18_scaling
+    sc = v18_scaling()
+    rng = np.random.default_rng(seed)
+    sh = shapes(nx, ny, nz)
+    xmin, xmax, ymin, ymax, zmin, zmax = 30e3 / sc.L, 100e3 / sc.L, -30e3 / sc.L, 5e3 / sc.L, -25.0 / sc.L, 25e3 / sc.L
+    dx, dy, dz = (xmax - xmin) / nx, (ymax - ymin) / ny, (zmax - zmin) / nz
+    xce = xmin - dx / 2 + dx * np.arange(nx + 2)
+    yce = ymin - dy / 2 + dy * np.arange(ny + 2)
+    zce = zmin - dz / 2 + dz * np.arange(nz + 2)
+    xv, yv, zv = xmin + dx * np.arange(nx + 1), ymin + dy * np.arange(ny + 1), zmin + dz * np.arange(nz + 1)
+
+    def surface(x):
+        return np.clip(-17.8e3 / sc.L + (x - 32e3 / sc.L) * (17.8 / 37.4), -17.8e3 / sc.L, 1.5e3 / sc.L)
+
+    def phases(X, Y, Z):
+        ph = np.zeros(X.shape)
+        ph[np.abs(Y - (0.5 * X - 45e3 / sc.L)) < 2.5 * dy] = 2.0
+        ph[((X - 60e3 / sc.L) / (8e3 / sc.L)) ** 2 + ((Y + 20e3 / sc.L) / (5e3 / sc.L)) ** 2 +
+           ((Z - 12e3 / sc.L) / (15e3 / sc.L)) ** 2 < 1.0] = 3.0
+        ph[Y > surface(X)] = 1.0
+        return ph
+
+    XE, YE, ZE = np.meshgrid(xce, yce, zce, indexing="ij")
+    XV, YV, ZV = np.meshgrid(xv, yv, zv, indexing="ij")
+    phc = np.asfortranarray(phases(XE, YE, ZE))
+    phv = np.asfortranarray(phases(XV, YV, ZV))
+    wN = np.ones((nx + 2, nz + 2), order="F")
+    if immersed:
+        for i in range(nx + 2):
+            for k in range(nz + 2):
+                air = np.nonzero(phc[i, :, k] == 1.0)[0]
+                if len(air) and 2 <= air[0] <= ny:       # interior cell, the cell below is rock
+                    phc[i, air[0], k] = -1.0
+                    wN[i, k] = rng.uniform(0.2, 1.0)
+    Ttop, Tbot = 293.15 / sc.T, (293.15 + 600.0) / sc.T
+    Ptop, Pbot = 1e5 / sc.sigma, (1e5 + 294e6) / sc.sigma
+    ys = surface(XE)
+    T = Ttop + (Ttop - Tbot) / (ys - ymin) * (YE - ys) + 0.01 * rng.standard_normal(XE.shape)
+    P = Ptop + (Ptop - Pbot) / (ys - ymin) * (YE - ys) * (1.0 + 0.01 * rng.standard_normal(XE.shape))
+    T[phc == 1.0] = Ttop
+    P[phc == 1.0] = Ptop
+    T, P = np.asfortranarray(T), np.asfortranarray(P)
+    phi_i, delta, k_fact = 1e-2, 3000.0 / sc.L, 50.0
+    z = lambda s: np.zeros(sh[s], order="F")
+    # heat set-up (v18:426-430)
+    ktv, dumc = z("V"), z("c")
+    orc.ComputeThermalConductivity(ktv, T, phv, phi_i, sc.kt, sc.T)
+    orc.SmoothConductivityV2C(dumc, ktv)
+    orc.SmoothConductivityC2V(ktv, dumc)
+    kT = [z("Fx"), z("Fy"), z("Fz")]
+    orc.InitConductivity(*kT, ktv)
+    rhoC = z("c")
+    orc.ComputeRhoCeffective(rhoC, T, P, phc, phi_i, sc.T, sc.sigma, sc.C, sc.rho)
+    T0 = np.asfortranarray(T[1:-1, 1:-1, 1:-1])
+    # Darcy set-up (v18:484-503)
+    kfv = z("V")
+    orc.ComputeHydroConductivity(kfv, T, P, phv, ymin, dy, k_fact, delta, sc.sigma, sc.T, sc.L, sc.t, sc.eta, 1)
+    orc.SmoothConductivityV2C(dumc, kfv)
+    orc.SmoothConductivityC2V(kfv, dumc)
+    kP = [z("Fx"), z("Fy"), z("Fz")]
+    orc.InitConductivity(*kP, kfv)
+    phirho = z("c")
+    orc.PhiDrhoDP(phirho, T, P, phc, phi_i, sc.rho, sc.T, sc.sigma)
+    P0 = np.asfortranarray(P[1:-1, 1:-1, 1:-1])
+    phirhof0 = z("c")
+    orc.PhiRhof_old(phirhof0, T0, P0, phc, phi_i, sc.rho, sc.T, sc.sigma)
+    rhofv = z("V")
+    orc.ComputeFluidDensity(rhofv, T, P, phv, sc.sigma, sc.T, sc.rho)
+    dt = 1e11 / sc.t
+    heat = dict(dx=dx, dy=dy, dz=dz, dt=dt, transient=1.0, qyS=37e-3 / (sc.W / sc.L ** 2), TN=Ttop,
+                Qt=(3.4e-4 / (sc.W / sc.L ** 3)) if Qt is None else Qt, norm_scale=sc.rho * sc.C * sc.T / sc.t)
+    darcy = dict(dx=dx, dy=dy, dz=dz, dt=dt, transient=1.0, PN=Ptop, g=-9.81 / (sc.L / sc.t ** 2), phi_i=phi_i,
+                 scale_rho=sc.rho, scale_T=sc.T, scale_sigma=sc.sigma, norm_scale=sc.rho / sc.t)
+    return dict(T=T, P=P, T0=T0, rhoC=rhoC, kT=kT, phirho=phirho, phirhof0=phirhof0, rhofv=rhofv, kP=kP, phc=phc,
+                wN=wN, heat=heat, darcy=darcy, sc=sc)

@@ -1,0 +1,121 @@
+"""Fused v18 DYREL solves (ht3d_v18_thermal_solve / ht3d_v18_darcy_solve)
+against the oracle's unfused kernel sequence of HydroThermal3D_v18.jl:432-478
+and :489-537 on identical inputs: the same iterations, statuses, lambda
+updates and norms, and the same fields bit for bit (unknown with ghosts, air
+and immersed cells; PC and G; phi*rho_f of the last Darcy iteration except at
+immersed cells).  Grids leave partial tiles in every direction."""
+import numpy as np
+import pytest
+
+from helpers import v18_problem
+
+pytestmark = pytest.mark.gpu
+
+GRIDS = [(16, 12, 6), (70, 37, 9), (37, 70, 13)]
+KEYS = ("iters", "status", "n_checks", "n_lambda", "lmax", "lmin", "dtau", "h1", "h2", "nF_abs", "nF_rel", "nF_ini")
+
+
+def _problem(grid, immersed, dt_s, seed=3):
+    d = v18_problem(*grid, seed=seed, immersed=immersed)
+    d["heat"].update(dt=dt_s / d["sc"].t, norm_scale=1e30)
+    d["darcy"].update(dt=dt_s / d["sc"].t, norm_scale=1e30)
+    return d
+
+
+def _run(ht, orc, d, kind, prm):
+    from hydrothermal3d_amd import dimensional as dim
+    from hydrothermal3d_amd.solver import SolveError
+    dev = lambda a: ht.DeviceArray.from_host(np.asfortranarray(a))
+
+    def solve(fn, *args, **kw):   # NaN / divergence raise, as the reference does; keep the report
+        try:
+            return fn(*args, **kw)
+        except SolveError as e:
+            assert e.report is not None
+            return e.report
+
+    grid = d["T0"].shape
+    PCd, Gd = ht.DeviceArray.zeros(grid), ht.DeviceArray.zeros(grid)
+    p = orc.params(**prm)
+    gp = ht.DyrelParams(**prm)
+    if kind == "heat":
+        Xh = d["T"].copy(order="F")
+        rep_o, w = orc.v18_thermal_solve(Xh, d["T0"], d["rhoC"], *d["kT"], d["phc"], d["wN"], **d["heat"], p=p)
+        Xd = dev(d["T"])
+        rep_g = solve(dim.v18_thermal_solve, Xd, dev(d["T0"]), dev(d["rhoC"]), *map(dev, d["kT"]), dev(d["phc"]),
+                                      dev(d["wN"]), **d["heat"], params=gp, PC=PCd, G=Gd)
+        extra = None
+    else:
+        Xh, ph = d["P"].copy(order="F"), d["phirho"].copy(order="F")
+        rep_o, w = orc.v18_darcy_solve(Xh, d["T"], ph, d["phirhof0"], d["rhofv"], *d["kP"], d["phc"], d["wN"],
+                                       **d["darcy"], p=p)
+        Xd, phd = dev(d["P"]), dev(d["phirho"])
+        rep_g = solve(dim.v18_darcy_solve, Xd, dev(d["T"]), phd, dev(d["phirhof0"]), dev(d["rhofv"]), *map(dev, d["kP"]),
+                                    dev(d["phc"]), dev(d["wN"]), **d["darcy"], params=gp, PC=PCd, G=Gd)
+        extra = (phd.get(), ph)
+    return rep_o, rep_g, (Xd.get(), Xh), (PCd.get(), w["PC"]), (Gd.get(), w["G"]), extra
+
+
+def _compare(d, out):
+    rep_o, rep_g, X, PC, G, extra = out
+    for k in KEYS:
+        assert rep_g[k] == rep_o[k], (k, rep_g[k], rep_o[k])
+    assert np.array_equal(X[0], X[1])
+    assert np.array_equal(PC[0], PC[1]) and np.array_equal(G[0], G[1])
+    if extra is not None:
+        keep = d["phc"][1:-1, 1:-1, 1:-1] != -1.0
+        assert np.array_equal(extra[0][keep], extra[1][keep])
+
+
+@pytest.mark.parametrize("kind", ["heat", "darcy"])
+@pytest.mark.parametrize("immersed", [False, True], ids=["flat", "immersed"])
+@pytest.mark.parametrize("grid", GRIDS, ids=lambda g: "x".join(map(str, g)))
+def test_fixed_iterations_bitwise(ht, orc, grid, immersed, kind):
+    """250 pseudo-iterations (checks and lambda updates at 1, 2, 100, 200)."""
+    print(f"v18 {kind} {grid} immersed={immersed}", flush=True)
+    d = _problem(grid, immersed, 1e14)
+    if immersed:
+        d["wN"][:] = np.where(d["wN"] < 1.0, 0.9, 1.0)
+    _compare(d, _run(ht, orc, d, kind, dict(nitmax=250, nitout=100, ncheck=0, norm=0, tol=0.0)))
+
+
+@pytest.mark.parametrize("kind", ["heat", "darcy"])
+@pytest.mark.parametrize("grid", GRIDS[:2], ids=lambda g: "x".join(map(str, g)))
+def test_converged_solve(ht, orc, grid, kind):
+    """To relative tolerance 1e-10 with extra checks every 25 iterations."""
+    print(f"v18 converge {kind} {grid}", flush=True)
+    d = _problem(grid, False, 1e15, seed=8)
+    out = _run(ht, orc, d, kind, dict(nitmax=20000, nitout=100, ncheck=25, norm=1, tol=1e-10))
+    assert out[0]["status"] == 0
+    _compare(d, out)
+
+
+def test_divergence_detected_alike(ht, orc):
+    """Small immersed-surface weights make the heat loop diverge (the
+    extrapolation is outside the Gershgorin bound): the same iteration and
+    norms on both paths."""
+    d = _problem((16, 12, 6), True, 1e15)
+    print("v18 divergence (16,12,6)", flush=True)
+    out = _run(ht, orc, d, "heat", dict(nitmax=2000, nitout=100, ncheck=0, norm=0, tol=1e-10))
+    assert out[0]["status"] == ht._lib.DIVERGED
+    _compare(d, out)
+
+
+def test_every_tile_bitwise(ht, orc):
+    import ctypes as C
+    d = _problem((70, 37, 9), True, 1e14)
+    d["wN"][:] = np.where(d["wN"] < 1.0, 0.9, 1.0)
+    ref = None
+    try:
+        for t in range(6):
+            print(f"v18 tile {t}", flush=True)
+            ht._lib.call("ht3d_v18_tune", t, None)
+            out = _run(ht, orc, d, "darcy", dict(nitmax=120, nitout=100, ncheck=0, norm=0, tol=0.0))
+            last = C.c_int()
+            ht._lib.call("ht3d_v18_tune", t, C.byref(last))
+            assert last.value == t
+            _compare(d, out)
+            ref = out[2][0] if ref is None else ref
+            assert np.array_equal(out[2][0], ref)
+    finally:
+        ht._lib.call("ht3d_v18_tune", -1, None)

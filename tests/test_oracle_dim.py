@@ -247,3 +247,103 @@ def test_ghost_fills_match_serial_reference(orc, npd, grid, immersed):
     Ps = npd.SetPressureBCs_serial(d["P"], d["wN"], d["phc"], d["PN"])
     assert np.array_equal(P, Ps)
     assert np.array_equal(P[1:-1, 0, 1:-1], P[1:-1, 1, 1:-1])
+
+
+# ------------------------------------------------------------ v18 DYREL solves
+def _v18(grid, immersed=False, seed=0, dt_s=1e15, Qt=None):
+    from helpers import v18_problem
+    d = v18_problem(*grid, seed=seed, immersed=immersed, Qt=Qt)
+    d["heat"].update(dt=dt_s / d["sc"].t, norm_scale=1e30)    # relative convergence only
+    d["darcy"].update(dt=dt_s / d["sc"].t, norm_scale=1e30)
+    return d
+
+
+def test_v18_residuals_match_numpy(orc, npd):
+    d = _v18((9, 8, 5), immersed=True)
+    h, dd = d["heat"], d["darcy"]
+    sh = shapes(9, 8, 5)
+    rng = np.random.default_rng(5)
+    _d = (1 / h["dx"], 1 / h["dy"], 1 / h["dz"])
+    q = [np.zeros(sh[s], order="F") for s in ("Fx", "Fy", "Fz")]
+    orc.ComputeFlux(*q, *d["kT"], d["T"], *_d)
+    qn = npd.ComputeFlux(*d["kT"], d["T"], *_d)
+    for a, b in zip(q, qn):
+        assert np.array_equal(a, b)
+    PC = np.asfortranarray(rng.uniform(0.5, 2.0, sh["c"]))
+    F = np.zeros(sh["c"], order="F")
+    orc.ResidualTemperatureLinearised(F, d["T"], d["T0"], d["rhoC"], d["phc"], PC, *q, h["Qt"], 1.0, 1 / h["dt"], *_d)
+    assert np.array_equal(F, npd.ResidualTemperatureLinearised(d["T"], d["T0"], d["rhoC"], d["phc"], PC, *qn,
+                                                               h["Qt"], 1.0, 1 / h["dt"], *_d))
+    orc.ComputeDarcyFlux(*q, d["rhofv"], *d["kP"], d["P"], dd["g"], *_d)
+    qn = npd.ComputeDarcyFlux(d["rhofv"], *d["kP"], d["P"], dd["g"], *_d)
+    for a, b in zip(q, qn):
+        assert np.array_equal(a, b)
+    orc.ResidualFluidPressure(F, d["phc"], d["phirho"], d["phirhof0"], PC, *q, 1.0, 1 / dd["dt"], *_d)
+    assert np.array_equal(F, npd.ResidualFluidPressure(d["phc"], d["phirho"], d["phirhof0"], PC, *qn, 1.0,
+                                                       1 / dd["dt"], *_d))
+
+
+def _heat_residual(npd, d, T):
+    h = d["heat"]
+    _d = (1 / h["dx"], 1 / h["dy"], 1 / h["dz"])
+    Tb = npd.SetTemperatureBCs_serial(T, d["kT"][1], d["wN"], d["phc"], h["qyS"], _d[1], h["TN"])
+    q = npd.ComputeFlux(*d["kT"], Tb, *_d)
+    return npd.ResidualTemperatureLinearised(Tb, d["T0"], d["rhoC"], d["phc"], 1.0, *q, h["Qt"], h["transient"],
+                                             1 / h["dt"], *_d)
+
+
+@pytest.mark.parametrize("immersed", [False, True], ids=["flat", "immersed"])
+def test_v18_heat_solve_matches_dense_solve(orc, npd, immersed):
+    """The converged state of the oracle's v18 heat loop equals the dense solve
+    of the affine residual assembled from the NumPy restatement (serial ghost
+    fill, fluxes, linearised residual) on the cells that carry an equation."""
+    grid = (6, 8, 4)
+    d = _v18(grid, immersed=immersed, dt_s=1e14)
+    if immersed:
+        d["wN"][:] = np.where(d["wN"] < 1.0, 0.9, 1.0)    # a stable extrapolation weight
+    free = np.abs(d["phc"][1:-1, 1:-1, 1:-1]) != 1.0
+    idx = np.flatnonzero(free.ravel(order="F"))
+    base = d["T"].copy(order="F")
+
+    def apply(x):
+        T = base.copy(order="F")
+        inner = T[1:-1, 1:-1, 1:-1].ravel(order="F")
+        inner[idx] = x
+        T[1:-1, 1:-1, 1:-1] = inner.reshape(grid, order="F")
+        return _heat_residual(npd, d, T).ravel(order="F")[idx]
+
+    f0 = apply(np.zeros(len(idx)))
+    M = np.column_stack([apply(np.eye(len(idx))[:, c]) - f0 for c in range(len(idx))])
+    x = np.linalg.solve(M, -f0)
+    T = base.copy(order="F")
+    rep, _ = orc.v18_thermal_solve(T, d["T0"], d["rhoC"], *d["kT"], d["phc"], d["wN"], **d["heat"],
+                                   p=orc.params(nitmax=100000, nitout=100, ncheck=10, norm=1, tol=1e-13))
+    assert rep["status"] == 0
+    got = T[1:-1, 1:-1, 1:-1].ravel(order="F")[idx]
+    assert np.max(np.abs(got - x)) <= 1e-10 * np.max(np.abs(x))
+
+
+@pytest.mark.parametrize("immersed", [False, True], ids=["flat", "immersed"])
+def test_v18_darcy_solve_zeroes_the_residual(orc, npd, immersed):
+    """The Darcy loop is nonlinear (the fluid density depends on P): its
+    converged state makes the NumPy restatement's residual (serial ghost fill,
+    EOS, Darcy flux with gravity) vanish relative to the initial one."""
+    grid = (9, 8, 5)
+    d = _v18(grid, immersed=immersed, dt_s=1e14)
+    d["wN"][:] = np.where(d["wN"] < 1.0, 0.9, 1.0)
+    dd = d["darcy"]
+    _d = (1 / dd["dx"], 1 / dd["dy"], 1 / dd["dz"])
+
+    def residual(P):
+        Pb = npd.SetPressureBCs_serial(P, d["wN"], d["phc"], dd["PN"])
+        prf = npd.PhiRhof(np.zeros(grid, order="F"), d["T"], Pb, d["phc"], dd["phi_i"], dd["scale_rho"],
+                          dd["scale_T"], dd["scale_sigma"])
+        q = npd.ComputeDarcyFlux(d["rhofv"], *d["kP"], Pb, dd["g"], *_d)
+        return npd.ResidualFluidPressure(d["phc"], prf, d["phirhof0"], 1.0, *q, dd["transient"], 1 / dd["dt"], *_d)
+
+    r0 = np.max(np.abs(residual(d["P"])))
+    P, phirho = d["P"].copy(order="F"), d["phirho"].copy(order="F")
+    rep, _ = orc.v18_darcy_solve(P, d["T"], phirho, d["phirhof0"], d["rhofv"], *d["kP"], d["phc"], d["wN"],
+                                 **dd, p=orc.params(nitmax=100000, nitout=100, ncheck=10, norm=1, tol=1e-13))
+    assert rep["status"] == 0 and rep["iters"] > 50
+    assert np.max(np.abs(residual(P))) <= 1e-10 * r0
