@@ -148,3 +148,77 @@ def v18_problem(nx, ny, nz, seed=0, immersed=False, Qt=None):
                  scale_rho=sc.rho, scale_T=sc.T, scale_sigma=sc.sigma, norm_scale=sc.rho / sc.t)
     return dict(T=T, P=P, T0=T0, rhoC=rhoC, kT=kT, phirho=phirho, phirhof0=phirhof0, rhofv=rhofv, kP=kP, phc=phc,
                 wN=wN, heat=heat, darcy=darcy, sc=sc)
+
+
+class V18OracleModel:
+    """The v18 time step of HydroThermal3D_v18.jl:398-571 with the CPU oracle's
+    kernels and unfused DYREL loops, on the same host state and in the same
+    order as hydrothermal3d_amd.v18.V18Model (the checker of that driver)."""
+
+    def __init__(self, nx, ny, nz, geo, pT, pP):
+        import math  # noqa: F401
+        from oracle import oracle as orc
+        self.orc, self.geo, self.sc, self.ph = orc, geo, geo["sc"], geo["phys"]
+        self.pT, self.pP = pT, pP
+        sh = shapes(nx, ny, nz)
+        z = lambda s: np.zeros(sh[s], order="F")
+        self.T, self.P = geo["T"].copy(order="F"), geo["P"].copy(order="F")
+        self.phc, self.phv, self.wN = geo["phc"], geo["phv"], geo["wN"]
+        self.Tc0, self.Pc0, self.Xc0, self.rho, self.dumc = z("c"), z("c"), z("c"), z("c"), z("c")
+        self.k = z("V")
+        self.kx, self.ky, self.kz = z("Fx"), z("Fy"), z("Fz")
+        self.Vx, self.Vy, self.Vz = z("Fx"), z("Fy"), z("Fz")
+        self.it, self.time, self.dt = 0, 0.0, self.ph.dt / self.sc.t
+
+    def _faces(self):
+        o = self.orc
+        o.SmoothConductivityV2C(self.dumc, self.k)
+        o.SmoothConductivityC2V(self.k, self.dumc)
+        o.InitConductivity(self.kx, self.ky, self.kz, self.k)
+
+    def step(self):
+        import math
+        o, sc, ph, g = self.orc, self.sc, self.ph, self.geo
+        transient = 0.0 if self.it == 0 else 1.0
+        if self.it == 1:
+            self.T[self.phc == 3.0] = ph.T_pluton / sc.T
+        self.Tc0[...] = self.T[1:-1, 1:-1, 1:-1]
+        self.Pc0[...] = self.P[1:-1, 1:-1, 1:-1]
+        h = dict(dx=g["dx"], dy=g["dy"], dz=g["dz"], dt=self.dt, transient=transient)
+        # heat
+        o.ComputeThermalConductivity(self.k, self.T, self.phv, ph.phi_i, sc.kt, sc.T)
+        self._faces()
+        o.ComputeRhoCeffective(self.rho, self.T, self.P, self.phc, ph.phi_i, sc.T, sc.sigma, sc.C, sc.rho)
+        rT, _ = o.v18_thermal_solve(self.T, self.Tc0, self.rho, self.kx, self.ky, self.kz, self.phc, self.wN, **h,
+                                    qyS=ph.qyS / (sc.W / sc.L ** 2), TN=ph.Ttop / sc.T, Qt=ph.Qt / (sc.W / sc.L ** 3),
+                                    norm_scale=sc.rho * sc.C * sc.T / sc.t, p=self.pT)
+        # Darcy
+        args = (self.T, self.P, self.phv, g["ymin"], g["dy"], ph.k_fact, ph.delta / sc.L, sc.sigma, sc.T, sc.L, sc.t,
+                sc.eta)
+        o.ComputeHydroConductivity(self.k, *args, 1)
+        self._faces()
+        o.PhiDrhoDP(self.rho, self.T, self.P, self.phc, ph.phi_i, sc.rho, sc.T, sc.sigma)
+        o.PhiRhof_old(self.Xc0, self.Tc0, self.Pc0, self.phc, ph.phi_i, sc.rho, sc.T, sc.sigma)
+        o.ComputeFluidDensity(self.k, self.T, self.P, self.phv, sc.sigma, sc.T, sc.rho)
+        grav = ph.g / (sc.L / sc.t ** 2)
+        rP, _ = o.v18_darcy_solve(self.P, self.T, self.rho, self.Xc0, self.k, self.kx, self.ky, self.kz, self.phc,
+                                  self.wN, **h, PN=ph.Ptop / sc.sigma, g=grav, phi_i=ph.phi_i, scale_rho=sc.rho,
+                                  scale_T=sc.T, scale_sigma=sc.sigma, norm_scale=sc.rho / sc.t, p=self.pP)
+        # velocity
+        o.ComputeHydroConductivity(self.k, *args, 2)
+        self._faces()
+        o.ComputeFluidDensity(self.k, self.T, self.P, self.phv, sc.sigma, sc.T, sc.rho)
+        o.ComputeDarcyFlux(self.Vx, self.Vy, self.Vz, self.k, self.kx, self.ky, self.kz, self.P, grav,
+                           1 / g["dx"], 1 / g["dy"], 1 / g["dz"])
+        n_adv = 0
+        if self.it > 0:
+            self.time += self.dt
+            vmax = max(np.max(np.abs(self.Vx)), np.max(np.abs(self.Vy)), np.max(np.abs(self.Vz)))
+            dt_adv = ph.C_adv * 1.0 / 6.1 * min(g["dx"], g["dy"], g["dz"]) / vmax
+            n_adv = int(math.ceil(self.dt / dt_adv))
+            Told, T6 = np.zeros(self.Tc0.shape, order="F"), np.zeros(tuple(s + 4 for s in self.T.shape), order="F")
+            for _ in range(n_adv):
+                o.AdvectWithWeno5_v2(self.T, T6, Told, self.Vx, self.Vy, self.Vz, self.dt / n_adv, 1 / g["dx"],
+                                     1 / g["dy"], 1 / g["dz"], ph.Ttop / sc.T, (ph.Ttop + ph.dT) / sc.T)
+        self.it += 1
+        return dict(thermal=rT, darcy=rP, n_adv=n_adv)
