@@ -302,6 +302,75 @@ def run_config0(steps: int, tol: float):
     return out
 
 
+def run_v18(iters: int, unfused_iters: int):
+    """The dimensional (v18) DYREL loops (HydroThermal3D_v18.jl:432-478, :489-537)
+    on a 256x128x256 synthetic v18 geometry after the v18 property set-up:
+    the fused solves (one launch per pseudo-iteration; kernel time from HIP
+    events, algorithmic bytes 73 B/cell heat, 89 B/cell Darcy) against the
+    reference's unfused launch sequence made of the library's per-kernel
+    entries on the same GPU (4 launches per heat iteration, 5 per Darcy one;
+    launched asynchronously, wall time over `unfused_iters` iterations)."""
+    import hydrothermal3d_amd as ht
+    from hydrothermal3d_amd import _lib, kernels as K
+    from hydrothermal3d_amd.device import DeviceArray
+    from hydrothermal3d_amd.v18 import V18Model, v18_geometry
+    nx, ny, nz = 256, 128, 256
+    N = nx * ny * nz
+    geo = v18_geometry(nx, ny, nz, seed=18)
+    prm = ht.DyrelParams(nitmax=iters, nitout=100, tol=0.0)
+    m = V18Model(nx, ny, nz, geo, prm, prm)
+    a, sc, ph = m.a, m.sc, m.phys
+    K.Cpy_inn_to_all(a["Tc0"], a["Tc_ex"])
+    K.Cpy_inn_to_all(a["Pc0"], a["Pc_ex"])
+    out = {"grid": f"{nx}x{ny}x{nz}", "iters": iters, "unfused_iters": unfused_iters}
+    z = lambda s: DeviceArray.zeros(s)
+    F, F0 = z((nx, ny, nz)), z((nx, ny, nz))
+    qx, qy, qz = z((nx + 1, ny, nz)), z((nx, ny + 1, nz)), z((nx, ny, nz + 1))
+    _d = (1.0 / m.dx, 1.0 / m.dy, 1.0 / m.dz)
+    for kind in ("heat", "darcy"):
+        m.thermal(1.0) if kind == "heat" else None            # untimed: tile choice, state
+        _lib.call("ht3d_timing_enable", 1)
+        _lib.call("ht3d_timing_reset")
+        rep = m.thermal(1.0) if kind == "heat" else m.darcy(1.0)
+        _lib.sync()
+        st = timing_stats()
+        _lib.call("ht3d_timing_enable", 0)
+        us = st["ms"] / max(st["launches"], 1) * 1e3
+        teff = st["bytes"] / (st["ms"] / 1e3) / 1e9 if st["ms"] > 0 else 0.0
+        h1, h2, dtau = rep["h1"], rep["h2"], rep["dtau"]
+        _lib.call("ht3d_set_sync", 0)
+        _lib.sync()
+        t0 = time.perf_counter()
+        for _ in range(unfused_iters):
+            if kind == "heat":
+                K.dim_SetTemperatureBCs(a["Tc_ex"], a["ky"], a["wN"], a["phc_ex"], ph.qyS / (sc.W / sc.L ** 2), _d[1],
+                                        0.0, 0.0, ph.Ttop / sc.T, *([0.0] * 10))
+                K.ComputeFlux(qx, qy, qz, a["kx"], a["ky"], a["kz"], a["Tc_ex"], *_d)
+                K.ResidualTemperatureLinearised(F, a["Tc_ex"], a["Tc0"], a["rhoC_phirho"], a["phc_ex"], a["PC"], qx,
+                                                qy, qz, 0.0, 1.0, 1.0 / m.dt, *_d)
+                K.DYRELUpdate(F0, a["Tc_ex"], F, h1, h2, dtau)
+            else:
+                K.PhiRhof(a["rhoC_phirho"], a["Tc_ex"], a["Pc_ex"], a["phc_ex"], ph.phi_i, sc.rho, sc.T, sc.sigma)
+                K.dim_SetPressureBCs(a["Pc_ex"], a["wN"], a["phc_ex"], 0.0, ph.Ptop / sc.sigma, *([0.0] * 8))
+                K.ComputeDarcyFlux(qx, qy, qz, a["k_rhof"], a["kx"], a["ky"], a["kz"], a["Pc_ex"],
+                                   ph.g / (sc.L / sc.t ** 2), *_d)
+                K.ResidualFluidPressure(F, a["phc_ex"], a["rhoC_phirho"], a["Xc0"], a["PC"], qx, qy, qz, 1.0,
+                                        1.0 / m.dt, *_d)
+                K.DYRELUpdate(F0, a["Pc_ex"], F, h1, h2, dtau)
+        _lib.sync()
+        un_us = (time.perf_counter() - t0) / unfused_iters * 1e6
+        _lib.call("ht3d_set_sync", 1)
+        import ctypes as C
+        tile = C.c_int()
+        _lib.call("ht3d_v18_tune", -1, C.byref(tile))
+        out[kind] = {"fused_us_per_iter": round(us, 2), "t_eff_GBs": round(teff, 1),
+                     "hbm_frac": round(teff / HBM_PEAK_GBS, 4),
+                     "algorithmic_bytes_per_iter": int(st["bytes"] / max(st["launches"], 1)),
+                     "tile": tile.value, "iters": rep["iters"], "status": rep["status"],
+                     "unfused_us_per_iter": round(un_us, 2), "fusion_speedup": round(un_us / us, 2)}
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -322,6 +391,8 @@ def main(argv=None) -> int:
     ap.add_argument("--batch-tol", type=float, default=1e-9,
                     help="DYREL tolerance of the batched leg (1e-9 keeps the solver error of both forms below 1e-6)")
     ap.add_argument("--batch-warmup", type=int, default=1)
+    ap.add_argument("--v18-iters", type=int, default=400,
+                    help="pseudo-iterations per fused v18 solve in the v18 leg (0: skip it)")
     ap.add_argument("--probe", action="store_true",
                     help="counter probe only (fused iterations + calibration copies), no JSON line")
     args = ap.parse_args(argv)
@@ -412,6 +483,9 @@ def main(argv=None) -> int:
         out["batch"] = run_batch(args.batch, args.batch_steps, args.batch_warmup,
                                  ht.production_params(tol=args.batch_tol, ncheck=ncheck))
         out["batch"]["tol"] = args.batch_tol
+    if args.v18_iters > 0:
+        log("v18 leg: fused dimensional DYREL loops vs the unfused launch sequence")
+        out["v18"] = run_v18(args.v18_iters, 100)
     if teff32 is not None:
         out["t_eff_fp32_inner"] = round(teff32, 2)
     if dc is not None:

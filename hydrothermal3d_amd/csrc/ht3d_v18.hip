@@ -11,10 +11,16 @@
 //   * the ghost/air/immersed values the BC kernel would write are computed from
 //     the neighbours when needed (the same rule evaluator as the ghost-fill
 //     entry), so the unknown's ghost layer is only written at the end;
+//   * the ghost rules are applied once to the unknown before the first
+//     iteration (as the reference's first BC launch does), so air cells hold
+//     the surface value; the update leaves them there (F = 0 gives F0 = 0), so
+//     an air neighbour is a plain load;
 //   * a per-cell class byte, built once per solve from the phases, selects the
-//     path: F = 0 (air and immersed cells), the fast path (plain neighbour
-//     loads; walls by select: Neumann copies, basal heat flux), or the exact
-//     rule evaluation (cells next to air, immersed cells or such ghosts);
+//     path: F = 0 (air and immersed cells: only the update, no coefficient or
+//     neighbour loads), the fast path (plain neighbour loads; walls by select:
+//     Neumann copies, basal heat flux), or the exact rule evaluation (cells
+//     next to an immersed-surface cell, whose value follows the cell below it
+//     every iteration);
 //   * heat: the Gershgorin diagonal is recomputed from the face coefficients
 //     and rho*C already loaded (it is what GershgorinPoisson! stores in PC);
 //     Darcy: phi*rho_f of the cell is evaluated from the EOS each iteration
@@ -66,34 +72,38 @@ struct Args {
     unsigned flags;
 };
 
-// Class byte of each cell (phases do not change during a solve).
+// Class byte of each cell (phases do not change during a solve), and the
+// number of F = 0 cells in *nzero.  Air neighbours and ghosts need no rule
+// evaluation once the rules have been applied to the unknown; immersed-surface
+// cells do (their value follows the cell below every iteration).
 __global__ __launch_bounds__(256) void class_kernel(const double *__restrict__ ph, int nx, int ny, int nz,
-                                                    uint8_t *__restrict__ cls) {
+                                                    uint8_t *__restrict__ cls, unsigned long long *__restrict__ nzero) {
+    __shared__ unsigned block_zeros;
+    if (threadIdx.x == 0) block_zeros = 0;
+    __syncthreads();
     const size_t N = (size_t)nx * ny * nz;
     const Grid g{nx, ny, nz};
+    unsigned zeros = 0;
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (size_t)gridDim.x * blockDim.x) {
         const int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((size_t)nx * ny));
         const int ie = i + 1, je = j + 1, ke = k + 1;
         const double p = ph[g.e(ie, je, ke)];
         uint8_t c = 0;
-        if (fabs(p) == 1.0) c |= C_ZERO;
+        if (fabs(p) == 1.0) { c |= C_ZERO; ++zeros; }
         if (p == 3.0) c |= C_SRC;
         if (p == 2.0) c |= C_FAULT;
-        auto special = [&](int a, int b, int d) { return fabs(ph[g.e(a, b, d)]) == 1.0; };
-        bool slow = false;
-        // x walls: the ghost copies the cell unless the ghost is itself immersed
-        slow |= i == 0 ? ph[g.e(0, je, ke)] == -1.0 : special(ie - 1, je, ke);
-        slow |= i == nx - 1 ? ph[g.e(nx + 1, je, ke)] == -1.0 : special(ie + 1, je, ke);
-        // bottom ghost: always the basal rule (heat flux or copy)
-        if (j > 0) slow |= special(ie, je - 1, ke);
-        // top: the ghost keeps its value unless it is air or immersed
-        slow |= special(ie, je + 1, ke);
-        // z walls: always the copy of the cell
-        if (k > 0) slow |= special(ie, je, ke - 1);
-        if (k < nz - 1) slow |= special(ie, je, ke + 1);
+        auto imm = [&](int a, int b, int d) { return ph[g.e(a, b, d)] == -1.0; };
+        // every neighbour but the bottom ghost (always the basal rule) and the
+        // z ghosts (always copies of the cell); an immersed x ghost overrides
+        // the copy, an immersed top ghost its stored value
+        const bool slow = imm(ie - 1, je, ke) || imm(ie + 1, je, ke) || (j > 0 && imm(ie, je - 1, ke)) ||
+                          imm(ie, je + 1, ke) || (k > 0 && imm(ie, je, ke - 1)) || (k < nz - 1 && imm(ie, je, ke + 1));
         if (slow) c |= C_SLOW;
         cls[t] = c;
     }
+    if (zeros) atomicAdd(&block_zeros, zeros);
+    __syncthreads();
+    if (threadIdx.x == 0 && block_zeros) atomicAdd(nzero, (unsigned long long)block_zeros);
 }
 
 // g * @av_xza(rho_f) on the y faces: the gravity term of ComputeDarcyFlux!
@@ -145,18 +155,17 @@ __global__ __launch_bounds__(64 * TY) void iter_kernel(Args a) {
         double kzB = a.kz[c];
         for (int k = k0; k < k1; ++k) {
             const double xp = a.Xin[e + sE2];
-            const double xW = a.Xin[e - 1], xE = a.Xin[e + 1];
-            const double xS = a.Xin[e - sE1], xN = a.Xin[e + sE1];
-            const double kW = a.kx[fx], kE = a.kx[fx + 1];
-            const double kS = a.ky[fy], kN = a.ky[fy + nx];
             const double kF = a.kz[c + sC2];
             const double kB = kzB;
             const uint8_t cl = a.cls[c];
             const double f0o = __builtin_nontemporal_load(a.F0 + c);
             double f = 0.0, D = 1.0;
-            if (!(cl & C_ZERO)) {
-                double w = xW, ea = xE, s = xS, n = xN, b = xm, fr = xp;
-                if (cl & C_SLOW) {   // every neighbour from the ghost-fill rules
+            if (!(cl & C_ZERO)) {   // air and immersed cells: F = 0, nothing else to load
+                const double kW = a.kx[fx], kE = a.kx[fx + 1];
+                const double kS = a.ky[fy], kN = a.ky[fy + nx];
+                double w = a.Xin[e - 1], ea = a.Xin[e + 1], s = a.Xin[e - sE1], n = a.Xin[e + sE1];
+                double b = xm, fr = xp;
+                if (cl & C_SLOW) {   // next to an immersed-surface cell: every neighbour from the rules
                     const int ie = i + 1, je = j + 1, ke = k + 1;
                     w = dim::bc_value(a.bc, ie - 1, je, ke);
                     ea = dim::bc_value(a.bc, ie + 1, je, ke);
@@ -356,13 +365,26 @@ static int solve(const ht3d_v18_field *f, const ht3d_dyrel_params *p, ht3d_dyrel
     double h1 = (2 - c * dtau) / (2 + c * dtau), h2 = 2 * dtau / (2 + c * dtau);
 
     const unsigned nb_lin = div_up(N, 256) > 4096 ? 4096 : div_up(N, 256);
-    hipLaunchKernelGGL(class_kernel, dim3(nb_lin), dim3(256), 0, st, f->phc_ex, nx, ny, nz, (uint8_t *)cls);
+    unsigned long long *nzero_dev = (unsigned long long *)((double *)res + 32);
+    HIP_TRY(hipMemsetAsync(nzero_dev, 0, sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(class_kernel, dim3(nb_lin), dim3(256), 0, st, f->phc_ex, nx, ny, nz, (uint8_t *)cls, nzero_dev);
     if (DARCY)
         hipLaunchKernelGGL(gravity_kernel, dim3(nb_lin), dim3(256), 0, st, f->rhofv, f->g, nx, ny, nz, (double *)gy);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemsetAsync(f0, 0, N * sizeof(double), st));    // ResetA!(Fc, Fc0)
     HIP_TRY(hipMemsetAsync(fst, 0, N * sizeof(double), st));
-    // the second buffer needs the ghosts that no rule rewrites
+    unsigned long long nzero = 0;
+    HIP_TRY(hipMemcpyAsync(&nzero, nzero_dev, sizeof nzero, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    // the ghost rules on the initial unknown (what the first BC launch of the
+    // reference does): air cells and air ghosts then hold the surface value
+    if (DARCY)
+        rc = ht3d_dim_SetPressureBCs(f->X, f->wN, f->phc_ex, 0.0, f->top, 0, 0, 0, 0, 0, 0, 0, 0, nx, ny, nz);
+    else
+        rc = ht3d_dim_SetTemperatureBCs(f->X, f->ky, f->wN, f->phc_ex, f->qyS, 1.0 / f->dy, 0, 0, f->top, 0, 0, 0, 0,
+                                        0, 0, 0, 0, 0, 0, nx, ny, nz);
+    if (rc) return rc;
+    // the second buffer needs the ghosts (no iteration writes them)
     if ((rc = copy_ghosts((double *)xb, f->X, nx, ny, nz))) return rc;
 
     Args a;
@@ -396,9 +418,11 @@ static int solve(const ht3d_v18_field *f, const ht3d_dyrel_params *p, ht3d_dyrel
     double *buf[2] = {f->X, (double *)xb};
     int cur = 0;
     SegmentTimer timer;
-    // algorithmic bytes per cell: unknown and F0 read and written, five (heat)
-    // or seven (Darcy) read-only words, the class byte
-    const double bytes_per_iter = ((DARCY ? 11.0 : 9.0) * 8.0 + 1.0) * (double)N;
+    // algorithmic bytes per iteration: for a cell with an equation the unknown
+    // and F0 read and written, five (heat) or seven (Darcy) read-only words and
+    // the class byte; for an air or immersed cell the unknown, F0, kz and the
+    // class byte (its update is X -= dtau*F0 with F0 = h1*F0)
+    const double bytes_per_iter = ((DARCY ? 11.0 : 9.0) * 8.0 + 1.0) * (double)(N - nzero) + 41.0 * (double)nzero;
     double nF_abs = 0.0, nF_rel = 0.0, nF_ini = 0.0;
     int status = HT3D_NOT_CONVERGED, iter, n_checks = 0, n_lambda = 0, done_iters = 0;
     for (iter = 1; iter <= p->nitmax; ++iter) {

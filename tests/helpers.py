@@ -155,9 +155,12 @@ class V18OracleModel:
     kernels and unfused DYREL loops, on the same host state and in the same
     order as hydrothermal3d_amd.v18.V18Model (the checker of that driver)."""
 
-    def __init__(self, nx, ny, nz, geo, pT, pP):
-        import math  # noqa: F401
+    def __init__(self, nx, ny, nz, geo, pT, pP, ulp_seed=None):
+        """ulp_seed: if given, every permeability from ComputeHydroConductivity
+        (whose exp() and 10^x may round differently on another machine) is moved
+        by one ulp at seeded random cells, to measure the result's sensitivity."""
         from oracle import oracle as orc
+        self.ulp_rng = None if ulp_seed is None else np.random.default_rng(ulp_seed)
         self.orc, self.geo, self.sc, self.ph = orc, geo, geo["sc"], geo["phys"]
         self.pT, self.pP = pT, pP
         sh = shapes(nx, ny, nz)
@@ -169,6 +172,13 @@ class V18OracleModel:
         self.kx, self.ky, self.kz = z("Fx"), z("Fy"), z("Fz")
         self.Vx, self.Vy, self.Vz = z("Fx"), z("Fy"), z("Fz")
         self.it, self.time, self.dt = 0, 0.0, self.ph.dt / self.sc.t
+
+    def _hydro(self, *args):
+        self.orc.ComputeHydroConductivity(self.k, *args)
+        if self.ulp_rng is not None:
+            sign = self.ulp_rng.choice([-1.0, 0.0, 1.0], size=self.k.shape)
+            self.k[...] = np.where(sign > 0, np.nextafter(self.k, np.inf),
+                                   np.where(sign < 0, np.nextafter(self.k, -np.inf), self.k))
 
     def _faces(self):
         o = self.orc
@@ -195,7 +205,7 @@ class V18OracleModel:
         # Darcy
         args = (self.T, self.P, self.phv, g["ymin"], g["dy"], ph.k_fact, ph.delta / sc.L, sc.sigma, sc.T, sc.L, sc.t,
                 sc.eta)
-        o.ComputeHydroConductivity(self.k, *args, 1)
+        self._hydro(*args, 1)
         self._faces()
         o.PhiDrhoDP(self.rho, self.T, self.P, self.phc, ph.phi_i, sc.rho, sc.T, sc.sigma)
         o.PhiRhof_old(self.Xc0, self.Tc0, self.Pc0, self.phc, ph.phi_i, sc.rho, sc.T, sc.sigma)
@@ -205,7 +215,7 @@ class V18OracleModel:
                                   self.wN, **h, PN=ph.Ptop / sc.sigma, g=grav, phi_i=ph.phi_i, scale_rho=sc.rho,
                                   scale_T=sc.T, scale_sigma=sc.sigma, norm_scale=sc.rho / sc.t, p=self.pP)
         # velocity
-        o.ComputeHydroConductivity(self.k, *args, 2)
+        self._hydro(*args, 2)
         self._faces()
         o.ComputeFluidDensity(self.k, self.T, self.P, self.phv, sc.sigma, sc.T, sc.rho)
         o.ComputeDarcyFlux(self.Vx, self.Vy, self.Vz, self.k, self.kx, self.ky, self.kz, self.P, grav,

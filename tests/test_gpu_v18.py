@@ -126,25 +126,39 @@ def test_every_tile_bitwise(ht, orc):
 def test_model_steps_match_oracle(ht, orc, grid, immersed):
     """Three passes of the v18 time loop (the steady initialisation step, the
     intrusion step, one more) by hydrothermal3d_amd.v18.V18Model against the
-    oracle's kernel sequence.  The property kernels call exp() and 10^x, whose
-    device and host results may differ by an ulp, so fields agree to 1e-9
-    relative and iteration counts and statuses exactly."""
+    oracle's kernel sequence.  The permeabilities come from exp() and 10^x,
+    whose device and host results may differ by an ulp; the Rayleigh quotient
+    sum(-dtau F0 (F - Fcit)) cancels nearly equal residuals and amplifies that
+    (the solves themselves are bit-exact on equal inputs, above).  So the
+    tolerance is measured: a second oracle run with every permeability moved by
+    one ulp at random gives the spread of each quantity, and the GPU must stay
+    within 10x that spread (plus 1e-12 relative).  Iteration counts, statuses
+    and advection sub-steps agree exactly."""
     from helpers import V18OracleModel
     from hydrothermal3d_amd.v18 import V18Model, v18_geometry
     geo = v18_geometry(*grid, seed=4, immersed=immersed)
     prm = dict(nitmax=300, nitout=100, tol=1e-18)
     ref = V18OracleModel(*grid, geo, orc.params(**prm), orc.params(**prm))
+    alt = V18OracleModel(*grid, geo, orc.params(**prm), orc.params(**prm), ulp_seed=9)
     gpu = V18Model(*grid, geo, ht.DyrelParams(**prm), ht.DyrelParams(**prm))
+
+    def close(got, want, other, what):
+        tol = 10.0 * abs(other - want) + 1e-12 * abs(want) + 1e-300
+        assert abs(got - want) <= tol, (what, got, want, other)
+
     for s in range(3):
         print(f"v18 model step {s} {grid} immersed={immersed}", flush=True)
-        ro, rg = ref.step(), gpu.step()
+        ro, ra, rg = ref.step(), alt.step(), gpu.step()
         assert rg["n_adv"] == ro["n_adv"]
         for solve in ("thermal", "darcy"):
             for k in ("iters", "status", "n_checks", "n_lambda"):
                 assert rg[solve][k] == ro[solve][k], (s, solve, k)
             for k in ("lmax", "lmin", "h1", "h2", "nF_rel"):
-                assert rg[solve][k] == pytest.approx(ro[solve][k], rel=1e-7, abs=1e-300), (s, solve, k)
-        for name, host in (("Tc_ex", ref.T), ("Pc_ex", ref.P), ("Vx", ref.Vx), ("Vy", ref.Vy), ("Vz", ref.Vz)):
+                close(rg[solve][k], ro[solve][k], ra[solve][k], (s, solve, k))
+        for name, host, host2 in (("Tc_ex", ref.T, alt.T), ("Pc_ex", ref.P, alt.P), ("Vx", ref.Vx, alt.Vx),
+                                  ("Vy", ref.Vy, alt.Vy), ("Vz", ref.Vz, alt.Vz)):
             got = gpu[name].get()
-            err = np.max(np.abs(got - host)) / np.max(np.abs(host))
-            assert err < 1e-9, (s, name, err)
+            scale = np.max(np.abs(host))
+            err, spread = np.max(np.abs(got - host)) / scale, np.max(np.abs(host2 - host)) / scale
+            print(f"  {name}: gpu-oracle {err:.2e}, one-ulp spread {spread:.2e}", flush=True)
+            assert err <= 10.0 * spread + 1e-12, (s, name, err, spread)
