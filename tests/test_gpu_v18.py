@@ -132,8 +132,9 @@ def test_model_steps_match_oracle(ht, orc, grid, immersed):
     (the solves themselves are bit-exact on equal inputs, above).  So the
     tolerance is measured: a second oracle run with every permeability moved by
     one ulp at random gives the spread of each quantity, and the GPU must stay
-    within 10x that spread (plus 1e-12 relative).  Iteration counts, statuses
-    and advection sub-steps agree exactly."""
+    within 10x that spread (plus 1e-12 relative; 1e-11 absolute for the
+    relative norm, which sits at round-off level once converged).  Iteration
+    counts, statuses and advection sub-steps agree exactly."""
     from helpers import V18OracleModel
     from hydrothermal3d_amd.v18 import V18Model, v18_geometry
     geo = v18_geometry(*grid, seed=4, immersed=immersed)
@@ -142,8 +143,8 @@ def test_model_steps_match_oracle(ht, orc, grid, immersed):
     alt = V18OracleModel(*grid, geo, orc.params(**prm), orc.params(**prm), ulp_seed=9)
     gpu = V18Model(*grid, geo, ht.DyrelParams(**prm), ht.DyrelParams(**prm))
 
-    def close(got, want, other, what):
-        tol = 10.0 * abs(other - want) + 1e-12 * abs(want) + 1e-300
+    def close(got, want, other, what, floor=1e-300):
+        tol = 10.0 * abs(other - want) + 1e-12 * abs(want) + floor
         assert abs(got - want) <= tol, (what, got, want, other)
 
     for s in range(3):
@@ -153,8 +154,10 @@ def test_model_steps_match_oracle(ht, orc, grid, immersed):
         for solve in ("thermal", "darcy"):
             for k in ("iters", "status", "n_checks", "n_lambda"):
                 assert rg[solve][k] == ro[solve][k], (s, solve, k)
-            for k in ("lmax", "lmin", "h1", "h2", "nF_rel"):
+            for k in ("lmax", "lmin", "h1", "h2"):
                 close(rg[solve][k], ro[solve][k], ra[solve][k], (s, solve, k))
+            # a relative norm at round-off level (1e-13) carries no digits: absolute floor
+            close(rg[solve]["nF_rel"], ro[solve]["nF_rel"], ra[solve]["nF_rel"], (s, solve), floor=1e-11)
         for name, host, host2 in (("Tc_ex", ref.T, alt.T), ("Pc_ex", ref.P, alt.P), ("Vx", ref.Vx, alt.Vx),
                                   ("Vy", ref.Vy, alt.Vy), ("Vz", ref.Vz, alt.Vz)):
             got = gpu[name].get()
