@@ -137,6 +137,23 @@ for (jl, c, grid) in (
         (:VzPlusMinus!, :ht3d_VzPlusMinus, :(cells_c(args[1]))),
         (:dFdx_Weno5!, :ht3d_dFdx_Weno5, :(cells_c(args[1]))),
         (:TimeAveraging!, :ht3d_TimeAveraging, :(cells_c(args[2]))),
+        # kernels_HT3D_dimensional.jl (the v18 driver)
+        (:InitConductivity!, :ht3d_InitConductivity, :(cells_c(args[1]) .- (1, 0, 0))),
+        (:SwapDYREL!, :ht3d_SwapDYREL, :(cells_c(args[2]))),
+        (:GershgorinHydro!, :ht3d_GershgorinHydro, :(cells_c(args[1]))),
+        (:RayleighQuotientNumerator!, :ht3d_RayleighQuotientNumerator, :(cells_c(args[1]))),
+        (:RayleighQuotientDenominator!, :ht3d_RayleighQuotientDenominator, :(cells_c(args[1]))),
+        (:ResidualTemperatureLinearised!, :ht3d_ResidualTemperatureLinearised, :(cells_c(args[1]))),
+        (:ResidualFluidPressure!, :ht3d_ResidualFluidPressure, :(cells_c(args[1]))),
+        (:ComputeDarcyFlux!, :ht3d_ComputeDarcyFlux, :(cells_E(args[8]))),
+        (:ComputeThermalConductivity, :ht3d_ComputeThermalConductivity, :(cells_E(args[2]))),
+        (:ComputeFluidDensity, :ht3d_ComputeFluidDensity, :(cells_E(args[2]))),
+        (:SmoothConductivityV2C, :ht3d_SmoothConductivityV2C, :(cells_c(args[1]))),
+        (:SmoothConductivityC2V, :ht3d_SmoothConductivityC2V, :(cells_c(args[2]))),
+        (:ComputeρCeffective!, :ht3d_ComputeRhoCeffective, :(cells_c(args[1]))),
+        (:ϕρf, :ht3d_PhiRhof, :(cells_c(args[1]))),
+        (:ϕρf_old, :ht3d_PhiRhof_old, :(cells_c(args[1]))),
+        (:ϕdρdP!, :ht3d_PhiDrhoDP, :(cells_c(args[1]))),
     )
     @eval function $jl(args...)
         g = $grid
@@ -157,6 +174,85 @@ ResetA!(A::DevArray, B::DevArray) = check(ccall((:ht3d_ResetA, libht3d), Cint,
     (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint), A.ptr, B.ptr, A.dims...), "ht3d_ResetA")
 ArrayEqualArray!(A::DevArray, B::DevArray) = check(ccall((:ht3d_ArrayEqualArray, libht3d), Cint,
     (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint), A.ptr, B.ptr, A.dims...), "ht3d_ArrayEqualArray")
+
+# Dimensional kernels whose names the dimensionless files also use: the v18
+# driver's methods differ in arity, so Julia dispatch picks the right entry.
+InitThermal!(Tc0::DevArray, Tc_ex::DevArray) = check(ccall((:ht3d_Cpy_inn_to_all, libht3d), Cint,
+    (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint), Tc0.ptr, Tc_ex.ptr, cells_c(Tc0)...), "InitThermal!")
+InitDarcy!(A::DevArray, B::DevArray) = ArrayEqualArray!(A, B)
+function Init_vel!(Vx::DevArray, Vy::DevArray, Vz::DevArray, qx::DevArray, qy::DevArray, qz::DevArray, ρfv::DevArray)
+    check(ccall((:ht3d_dim_Init_vel, libht3d), Cint, (ntuple(_ -> Ptr{Cvoid}, 7)..., Cint, Cint, Cint),
+                Vx.ptr, Vy.ptr, Vz.ptr, qx.ptr, qy.ptr, qz.ptr, ρfv.ptr, (Vy.dims .- (0, 1, 0))...), "Init_vel!")
+end
+function SetTemperatureBCs!(Tc_ex::DevArray, ky::DevArray, wN::DevArray, phc_ex::DevArray, s...)
+    length(s) == 15 || error("SetTemperatureBCs!: 19 arguments expected")
+    check(ccall((:ht3d_dim_SetTemperatureBCs, libht3d), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, ntuple(_ -> Float64, 15)..., Cint, Cint, Cint),
+                Tc_ex.ptr, ky.ptr, wN.ptr, phc_ex.ptr, Float64.(s)..., cells_E(Tc_ex)...), "SetTemperatureBCs!")
+end
+function SetPressureBCs!(Pc_ex::DevArray, wN::DevArray, phc_ex::DevArray, s...)
+    length(s) == 10 || error("SetPressureBCs!: 13 arguments expected")
+    check(ccall((:ht3d_dim_SetPressureBCs, libht3d), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, ntuple(_ -> Float64, 10)..., Cint, Cint, Cint),
+                Pc_ex.ptr, wN.ptr, phc_ex.ptr, Float64.(s)..., cells_E(Pc_ex)...), "SetPressureBCs!")
+end
+function ComputeHydroConductivity(kfv::DevArray, Tce::DevArray, Pce::DevArray, phv::DevArray, ymin, Δy, k_fact, δ,
+                                  scale_σ, scale_T, scale_L, scale_t, scale_η, mode::Integer)
+    check(ccall((:ht3d_ComputeHydroConductivity, libht3d), Cint,
+                (ntuple(_ -> Ptr{Cvoid}, 4)..., ntuple(_ -> Float64, 9)..., Cint, Cint, Cint, Cint),
+                kfv.ptr, Tce.ptr, Pce.ptr, phv.ptr, ymin, Δy, k_fact, δ, scale_σ, scale_T, scale_L, scale_t, scale_η,
+                mode, cells_E(Tce)...), "ComputeHydroConductivity")
+end
+
+# The two DYREL loops of HydroThermal3D_v18.jl (:443-478 heat, :500-537 Darcy),
+# fused: a driver replaces the `for iter = 1:nitmax ... end` block (and the
+# GershgorinPoisson!/ResetA! set-up before it) by one call.  `field` is the
+# library's ht3d_v18_field layout, `params` (nitmax, nitout, ncheck, norm, tol,
+# CFL, cfact, lmin_div); returns (iters, status, lmin, nF_abs, nF_rel).
+struct DyrelParams
+    nitmax::Cint; nitout::Cint; ncheck::Cint; norm::Cint
+    tol::Float64; CFL::Float64; cfact::Float64; lmin_div::Float64
+end
+mutable struct DyrelReport
+    iters::Cint; status::Cint; n_checks::Cint; n_lambda::Cint
+    lmax::Float64; lmin::Float64; dtau::Float64; h1::Float64; h2::Float64
+    nF_abs::Float64; nF_rel::Float64; nF_ini::Float64; ncheck::Cint
+    DyrelReport() = new(0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0)
+end
+struct V18Field
+    ncx::Cint; ncy::Cint; ncz::Cint
+    dx::Float64; dy::Float64; dz::Float64; dt::Float64; transient::Float64
+    X::Ptr{Cvoid}; kx::Ptr{Cvoid}; ky::Ptr{Cvoid}; kz::Ptr{Cvoid}; phc_ex::Ptr{Cvoid}; wN::Ptr{Cvoid}
+    rhoC::Ptr{Cvoid}; PC::Ptr{Cvoid}; G::Ptr{Cvoid}; top::Float64; norm_scale::Float64
+    Tc_old::Ptr{Cvoid}; qyS::Float64; Qt::Float64
+    Tc_ex::Ptr{Cvoid}; phirhof0::Ptr{Cvoid}; rhofv::Ptr{Cvoid}
+    g::Float64; phi_i::Float64; scale_rho::Float64; scale_T::Float64; scale_sigma::Float64
+end
+function v18_solve!(name::Symbol, f::V18Field, p::DyrelParams)
+    r = DyrelReport()
+    rc = ccall((name, libht3d), Cint, (Ref{V18Field}, Ref{DyrelParams}, Ref{DyrelReport}), f, p, r)
+    rc == 11 && error("Nan ...")          # v18:464
+    rc == 12 && error("Diverged ...")     # v18:465
+    rc in (0, 10) || check(rc, string(name))
+    return r
+end
+# heat loop: Tc_ex updated; ρC_ϕρ = rho*C (ComputeρCeffective!); PC, Xc0 receive D and G
+function v18_thermal_solve!(Tc_ex, Tc0, ρC_ϕρ, kx, ky, kz, phc_ex, wN, PC, Xc0; dx, dy, dz, dt, transient, qyS, Ttop,
+                            Qt, norm_scale, params::DyrelParams)
+    n = cells_E(Tc_ex)
+    f = V18Field(n..., dx, dy, dz, dt, transient, Tc_ex.ptr, kx.ptr, ky.ptr, kz.ptr, phc_ex.ptr, wN.ptr, ρC_ϕρ.ptr,
+                 PC.ptr, Xc0.ptr, Ttop, norm_scale, Tc0.ptr, qyS, Qt, C_NULL, C_NULL, C_NULL, 0, 0, 0, 0, 0)
+    return v18_solve!(:ht3d_v18_thermal_solve, f, params)
+end
+# Darcy loop: Pc_ex updated; ρC_ϕρ = ϕdρdP on entry, ϕρf on exit; Xc0 = ϕρf_old; k_ρf = fluid density
+function v18_darcy_solve!(Pc_ex, Tc_ex, ρC_ϕρ, Xc0, k_ρf, kx, ky, kz, phc_ex, wN, PC; dx, dy, dz, dt, transient, Ptop,
+                          g, ϕi, scale_ρ, scale_T, scale_σ, norm_scale, params::DyrelParams)
+    n = cells_E(Pc_ex)
+    f = V18Field(n..., dx, dy, dz, dt, transient, Pc_ex.ptr, kx.ptr, ky.ptr, kz.ptr, phc_ex.ptr, wN.ptr, ρC_ϕρ.ptr,
+                 PC.ptr, C_NULL, Ptop, norm_scale, C_NULL, 0, 0, Tc_ex.ptr, Xc0.ptr, k_ρf.ptr, g, ϕi, scale_ρ,
+                 scale_T, scale_σ)
+    return v18_solve!(:ht3d_v18_darcy_solve, f, params)
+end
 
 # WENO5: the v1 orchestration (kernels_HT3D.jl:156-233) and v2 (Weno5_Routines_v2.jl:1-50)
 # as one fused call each; the scratch arguments of the reference signatures are accepted.
