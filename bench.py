@@ -391,6 +391,9 @@ def main(argv=None) -> int:
     ap.add_argument("--batch-tol", type=float, default=1e-9,
                     help="DYREL tolerance of the batched leg (1e-9 keeps the solver error of both forms below 1e-6)")
     ap.add_argument("--batch-warmup", type=int, default=1)
+    ap.add_argument("--forward-only", action="store_true",
+                    help="sweep every launch forwards (default: alternate the direction between launches)")
+    ap.add_argument("--v18-only", action="store_true", help="run only the v18 leg and print its JSON")
     ap.add_argument("--v18-iters", type=int, default=400,
                     help="pseudo-iterations per fused v18 solve in the v18 leg (0: skip it)")
     ap.add_argument("--probe", action="store_true",
@@ -406,8 +409,13 @@ def main(argv=None) -> int:
     from hydrothermal3d_amd import _lib, inputs
 
     w = inputs.WORKLOADS[args.workload]
+    if args.forward_only:
+        _lib.call("ht3d_tune_sweep", 0)
     if args.probe:
         return probe(w, 20, 5)
+    if args.v18_only:
+        print(json.dumps({"v18": run_v18(max(args.v18_iters, 1), 100)}), flush=True)
+        return 0
     ncheck = -1 if args.check == "auto" else int(args.check)
     prm = ht.production_params(tol=args.tol, ncheck=ncheck)
     precision = _lib.FP32_DC if args.precision == "dc" else _lib.FP64
@@ -460,7 +468,8 @@ def main(argv=None) -> int:
         "data": "synthetic",
         "config": {"workload": w.name, "ncx": w.ncx, "ncy": w.ncy, "ncz": w.ncz, "Ra": w.Ra, "mu_a": w.mu_a,
                    "precision": args.precision, "tol": args.tol, "norm": "max|F*D|",
-                   "check_interval": prm.ncheck, "lambda_interval": prm.nitout},
+                   "check_interval": prm.ncheck, "lambda_interval": prm.nitout,
+                   "sweep": "forward" if args.forward_only else "alternating"},
         "s_per_step": round(ms_step / 1e3, 4),
         "hbm_frac": round(teff / HBM_PEAK_GBS, 4),
         "iters_per_step": iters,

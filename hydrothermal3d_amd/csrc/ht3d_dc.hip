@@ -167,6 +167,7 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     float *buf[2] = {(float *)e0, (float *)e1};
     float *f0buf[2] = {(float *)f0, (float *)f0b};
     int cur = 0, fc = 0;
+    bool rev = false;   // sweep direction of the next launch
     auto plain = [&](int it) { return !lam_iter(it, p->nitout) && !(it < p->nitmax && lam_iter(it + 1, p->nitout)); };
     for (outer = 0;; ++outer) {
         // anchor: FP64 residual, FP32 right-hand side, FP64 norm
@@ -200,6 +201,8 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
                 ap.Xout = buf[cur ^ 1];
                 ap.F0 = f0buf[fc ^ 1];
                 ap.h1 = (float)h1; ap.h2 = (float)h2;
+                ap.flags = rev ? FL_REVERSE : 0u;
+                rev = g_alternate && !rev;
                 timer.before_launch();
                 if ((rc = launch_pair(ap, (const float *)f0buf[fc], f->transient != 0, pcfg))) return rc;
                 timer.count += 2;
@@ -216,7 +219,8 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
             a.Xout = buf[cur ^ 1];
             a.F0 = f0buf[fc];
             a.h1 = (float)h1; a.h2 = (float)h2;
-            a.flags = (lam ? (FL_CHECK | FL_LAMBDA) : 0u) | (store ? FL_STORE_F : 0u);
+            a.flags = (lam ? (FL_CHECK | FL_LAMBDA) : 0u) | (store ? FL_STORE_F : 0u) | (rev ? FL_REVERSE : 0u);
+            rev = g_alternate && !rev;
             timer.before_launch();
             if ((rc = launch_iter(a, f->transient != 0, cfg))) return rc;
             ++timer.count;

@@ -98,6 +98,7 @@ std::vector<TuneEntry> g_tune;
 bool g_autotune = true;
 int g_force_tile = -1, g_force_variant = 0;
 int g_pairs = 1;
+int g_alternate = 1;
 
 IterCfg &last_cfg() {
     static IterCfg c;
@@ -248,6 +249,7 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
     double *buf[2] = {f->X, (double *)xb};
     double *f0buf[2] = {(double *)f0, (double *)f0b};
     int cur = 0, fc = 0;
+    bool rev = false;   // sweep direction of the next launch
     SegmentTimer timer;
     last_cfg() = cfg;
     last_pair() = pcfg;
@@ -266,6 +268,8 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
             ap.Xout = buf[cur ^ 1];
             ap.F0 = f0buf[fc ^ 1];
             ap.h1 = h1; ap.h2 = h2;
+            ap.flags = rev ? FL_REVERSE : 0u;
+            rev = g_alternate && !rev;
             timer.before_launch();
             if ((rc = launch_pair(ap, (const double *)f0buf[fc], f->transient != 0, pcfg))) return rc;
             timer.count += 2;
@@ -281,7 +285,9 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
         a.Xout = buf[cur ^ 1];
         a.F0 = f0buf[fc];
         a.h1 = h1; a.h2 = h2;
-        a.flags = (chk ? FL_CHECK : 0u) | (lam ? FL_LAMBDA : 0u) | (store ? FL_STORE_F : 0u);
+        a.flags = (chk ? FL_CHECK : 0u) | (lam ? FL_LAMBDA : 0u) | (store ? FL_STORE_F : 0u) |
+                  (rev ? FL_REVERSE : 0u);
+        rev = g_alternate && !rev;
         timer.before_launch();
         rc = launch_iter(a, f->transient != 0, cfg);
         if (rc) return rc;
@@ -375,6 +381,11 @@ int ht3d_tile_count(void) { return ht3d::kNumTiles; }
 int ht3d_tune_check_interval(double iters, double t_iter_us, double t_check_us, int nitout) {
     if (nitout < 1) return 0;
     return ht3d::tuned_check_interval(iters, t_iter_us, t_check_us, nitout);
+}
+
+int ht3d_tune_sweep(int alternate) {
+    ht3d::g_alternate = alternate ? 1 : 0;
+    return HT3D_OK;
 }
 
 int ht3d_tune_pairs(int mode) {

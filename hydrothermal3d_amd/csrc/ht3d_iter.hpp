@@ -13,7 +13,13 @@
 
 namespace ht3d {
 
-enum : unsigned { FL_CHECK = 1u, FL_LAMBDA = 2u, FL_STORE_F = 4u };
+// FL_REVERSE: sweep each XCD's slab of tiles backwards.  The solves alternate
+// it from launch to launch, so a launch starts on the planes the previous one
+// touched last, which are still in the MALL (256 MB, about half of an
+// iteration's working set at 256x128x256); results do not depend on it.
+enum : unsigned { FL_CHECK = 1u, FL_LAMBDA = 2u, FL_STORE_F = 4u, FL_REVERSE = 8u };
+
+extern int g_alternate;   // alternate FL_REVERSE between launches (default 1)
 
 template <typename T>
 struct IterArgs {
@@ -37,11 +43,13 @@ struct IterArgs {
     unsigned flags;
 };
 
-// block id -> logical tile id, making the blocks of one XCD (id mod 8) contiguous
-__device__ __forceinline__ unsigned xcd_tile(unsigned bid, unsigned nblocks) {
+// block id -> logical tile id, making the blocks of one XCD (id mod 8) contiguous;
+// with `rev` each XCD walks its slab from the far end
+__device__ __forceinline__ unsigned xcd_tile(unsigned bid, unsigned nblocks, bool rev = false) {
     const unsigned q = nblocks / 8u, r = nblocks % 8u;
     const unsigned x = bid % 8u, l = bid / 8u;
-    return x * q + (x < r ? x : r) + l;
+    const unsigned len = q + (x < r ? 1u : 0u);
+    return x * q + (x < r ? x : r) + (rev ? len - 1u - l : l);
 }
 
 template <typename T>
@@ -168,7 +176,7 @@ __device__ __forceinline__ void iter_tile(const IterArgs<T> &a, const T *__restr
 template <typename T, bool TRANSIENT, bool DSTORED, bool NTX, bool KT, int TY, int KZ>
 __global__ __launch_bounds__(64 * TY) void dyrel_iter_kernel(IterArgs<T> a) {
     const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
-    const unsigned t = xcd_tile(blockIdx.x, nblocks);
+    const unsigned t = xcd_tile(blockIdx.x, nblocks, a.flags & FL_REVERSE);
     const int bx = t % a.nbx, by = (t / a.nbx) % a.nby, bz = t / (a.nbx * a.nby);
     iter_tile<T, TRANSIENT, DSTORED, NTX, KT, TY, KZ>(a, a.Xin, a.Xout, a.flags, bx, by, bz, a.partials, blockIdx.x);
 }
@@ -189,7 +197,7 @@ template <typename T, bool TRANSIENT, bool DSTORED, bool KT, int TY, int KZ>
 __global__ __launch_bounds__(64 * TY) void dyrel_pair_kernel(IterArgs<T> a, const T *__restrict__ F0in) {
     __shared__ T L[3][TY][64];
     const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
-    const unsigned t = xcd_tile(blockIdx.x, nblocks);
+    const unsigned t = xcd_tile(blockIdx.x, nblocks, a.flags & FL_REVERSE);
     const int bx = t % a.nbx, by = (t / a.nbx) % a.nby, bz = t / (a.nbx * a.nby);
     const int lx = (int)(threadIdx.x & 63u), ly = (int)(threadIdx.x >> 6);
     const int i = bx * 62 - 1 + lx, j = by * (TY - 2) - 1 + ly;
@@ -516,7 +524,11 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
             IterCfg cfg{tile, var};
             set_tiling(a, tile);
             double us = 0;
-            int rc = timed([&] { return launch_iter(a, transient, cfg); }, &us);
+            int rc = timed([&] {
+                if (g_alternate) a.flags ^= FL_REVERSE;
+                return launch_iter(a, transient, cfg);
+            }, &us);
+            a.flags = 0;
             if (rc) return rc;
             if (us < te.us) { te.us = us; te.cfg = cfg; }
         }
@@ -550,7 +562,10 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
                 b.F0 = F0b;
                 set_pair_tiling(b, tile);
                 double us = 0;
-                int rc = timed([&] { return launch_pair(b, (const T *)a.F0, transient, pc); }, &us);
+                int rc = timed([&] {
+                    if (g_alternate) b.flags ^= FL_REVERSE;
+                    return launch_pair(b, (const T *)a.F0, transient, pc);
+                }, &us);
                 if (rc) return rc;
                 us *= 0.5;   // per iteration
                 if (us < te.pair_us) { te.pair_us = us; te.pair = pc; }

@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void copy_special_kernel(dim::BcArgs a, double
 template <bool DARCY, int TY, int KZ>
 __global__ __launch_bounds__(64 * TY) void iter_kernel(Args a) {
     const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
-    const unsigned t = xcd_tile(blockIdx.x, nblocks);
+    const unsigned t = xcd_tile(blockIdx.x, nblocks, a.flags & FL_REVERSE);
     const int bx = t % a.nbx, by = (t / a.nbx) % a.nby, bz = t / (a.nbx * a.nby);
     const int i = bx * 64 + (int)(threadIdx.x & 63u);
     const int j = by * TY + (int)(threadIdx.x >> 6);
@@ -302,7 +302,10 @@ static int choose_tile(const Args &base, double *scratch_x, double *scratch_f0, 
         a.flags = 0;
         launch<DARCY>(tile, a, st);   // warm-up
         HIP_TRY(hipEventRecord(e0, st));
-        for (int r = 0; r < 3; ++r) launch<DARCY>(tile, a, st);
+        for (int r = 0; r < 3; ++r) {
+            if (g_alternate) a.flags ^= FL_REVERSE;
+            launch<DARCY>(tile, a, st);
+        }
         HIP_TRY(hipEventRecord(e1, st));
         HIP_TRY(hipEventSynchronize(e1));
         float ms = 0.0f;
@@ -417,6 +420,7 @@ static int solve(const ht3d_v18_field *f, const ht3d_dyrel_params *p, ht3d_dyrel
     const int ncheck = p->ncheck < 0 ? tuned_check_interval(-1.0, -1.0, -1.0, p->nitout) : p->ncheck;
     double *buf[2] = {f->X, (double *)xb};
     int cur = 0;
+    bool rev = false;   // sweep direction of the next launch (alternated: MALL reuse)
     SegmentTimer timer;
     // algorithmic bytes per iteration: for a cell with an equation the unknown
     // and F0 read and written, five (heat) or seven (Darcy) read-only words and
@@ -432,7 +436,9 @@ static int solve(const ht3d_v18_field *f, const ht3d_dyrel_params *p, ht3d_dyrel
         a.Xin = buf[cur];
         a.Xout = buf[cur ^ 1];
         a.h1 = h1; a.h2 = h2;
-        a.flags = (chk ? FL_CHECK : 0u) | (lam ? FL_LAMBDA : 0u) | (store ? FL_STORE_F : 0u);
+        a.flags = (chk ? FL_CHECK : 0u) | (lam ? FL_LAMBDA : 0u) | (store ? FL_STORE_F : 0u) |
+                  (rev ? FL_REVERSE : 0u);
+        rev = g_alternate && !rev;
         timer.before_launch();
         launch<DARCY>(tile, a, st);
         HIP_TRY(hipGetLastError());

@@ -108,6 +108,11 @@ int ht3d_tune_pairs(int mode);
  * argument <= 0) the largest divisor of nitout up to 50.  Host only.          */
 int ht3d_tune_check_interval(double iters, double t_iter_us, double t_check_us, int nitout);
 int ht3d_tune_last_pair(int *rows_per_block, int *z_per_block, int *variant);
+/* Sweep direction of the fused iteration kernels: with alternate = 1 (default)
+ * every other launch walks each XCD's slab of tiles backwards, starting on the
+ * planes the previous launch touched last (still in the 256 MB MALL).  Results
+ * do not depend on it; 0 sweeps forward always.  No reference counterpart.   */
+int ht3d_tune_sweep(int alternate);
 
 /* maximum_g / minimum_g / sum_g / mean_g of abs.(A) (tools/Macros.jl:20-25);
  * the result lands in host memory.  Deterministic order.                    */

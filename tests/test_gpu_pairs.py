@@ -53,3 +53,20 @@ def test_pairs_bitwise(ht, grid, transient, precision):
     for key in ("iters", "status", "n_checks", "n_lambda", "lmin", "h1", "h2", "nF_abs"):
         assert r0[key] == r2[key], key
     assert np.array_equal(X0, X2)
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32dc"])
+def test_sweep_direction_does_not_change_results(ht, precision):
+    """Alternating the sweep direction between launches (ht3d_tune_sweep) only
+    reorders the tiles of a launch: fields and reports are bit-identical."""
+    out = []
+    try:
+        for alt in (0, 1):
+            ht._lib.call("ht3d_tune_sweep", alt)
+            out.append(_solve(ht, (70, 37, 45), 1, precision, 1))
+    finally:
+        ht._lib.call("ht3d_tune_sweep", 1)
+    (r0, X0, _), (r1, X1, _) = out
+    for key in ("iters", "status", "n_checks", "n_lambda", "lmin", "h1", "h2", "nF_abs"):
+        assert r0[key] == r1[key], key
+    assert np.array_equal(X0, X1)
