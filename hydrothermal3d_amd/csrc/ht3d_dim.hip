@@ -216,4 +216,149 @@ int ht3d_dim_SetPressureBCs(double *Pc_ex, const double *wN, const double *phc_e
     return bc_launch("dim_SetPressureBCs", a, Pc_ex);
 }
 
+// ---------------------------------------------------------------- output fields
+// Dimensional/kernels_HT3D_postprocessing.jl: cell arrays for the VTK output,
+// NaN in air unless noted.
+
+int ht3d_Phase(double *X, const double *phc_ex, int nx, int ny, int nz) {
+    CHECK_GRID(nx, ny, nz);
+    CHECK_PTR(X); CHECK_PTR(phc_ex);
+    Grid g{nx, ny, nz};
+    return launch3d("Phase", nx, ny, nz, [=] __device__(int i, int j, int k) {
+        const double ph = phc_ex[g.e(i + 1, j + 1, k + 1)];
+        X[g.c(i, j, k)] = ph != 1.0 ? ph : NAN;
+    });
+}
+
+static int scaled_field(const char *name, double *X, const double *F, const double *phc_ex, double scale, int nx,
+                        int ny, int nz) {
+    CHECK_GRID(nx, ny, nz);
+    CHECK_PTR(X); CHECK_PTR(F); CHECK_PTR(phc_ex);
+    Grid g{nx, ny, nz};
+    return launch3d(name, nx, ny, nz, [=] __device__(int i, int j, int k) {
+        const size_t e = g.e(i + 1, j + 1, k + 1);
+        X[g.c(i, j, k)] = phc_ex[e] != 1.0 ? F[e] * scale : NAN;
+    });
+}
+
+int ht3d_Pressure(double *X, const double *Pc_ex, const double *phc_ex, double scale_sigma, int nx, int ny, int nz) {
+    return scaled_field("Pressure", X, Pc_ex, phc_ex, scale_sigma, nx, ny, nz);
+}
+
+int ht3d_Temperature(double *X, const double *Tc_ex, const double *phc_ex, double scale_T, int nx, int ny, int nz) {
+    return scaled_field("Temperature", X, Tc_ex, phc_ex, scale_T, nx, ny, nz);
+}
+
+int ht3d_Velocity(double *X, double *Y, double *Z, const double *Vx, const double *Vy, const double *Vz,
+                  const double *phc_ex, double scale_V, int nx, int ny, int nz) {
+    CHECK_GRID(nx, ny, nz);
+    CHECK_PTR(X); CHECK_PTR(Y); CHECK_PTR(Z); CHECK_PTR(Vx); CHECK_PTR(Vy); CHECK_PTR(Vz); CHECK_PTR(phc_ex);
+    Grid g{nx, ny, nz};
+    return launch3d("Velocity", nx, ny, nz, [=] __device__(int i, int j, int k) {
+        const size_t c = g.c(i, j, k);
+        if (phc_ex[g.e(i + 1, j + 1, k + 1)] != 1.0) {
+            X[c] = 0.5 * (Vx[g.fx(i, j, k)] + Vx[g.fx(i + 1, j, k)]) * scale_V;
+            Y[c] = 0.5 * (Vy[g.fy(i, j, k)] + Vy[g.fy(i, j + 1, k)]) * scale_V;
+            Z[c] = 0.5 * (Vz[g.fz(i, j, k)] + Vz[g.fz(i, j, k + 1)]) * scale_V;
+        } else {
+            X[c] = NAN;   // Y and Z are left as they are, as in the reference
+        }
+    });
+}
+
+int ht3d_EffectiveThermalConductivity(double *X, const double *Tc_ex, double phi0, const double *phc_ex,
+                                      double scale_T, int nx, int ny, int nz) {
+    CHECK_GRID(nx, ny, nz);
+    CHECK_PTR(X); CHECK_PTR(Tc_ex); CHECK_PTR(phc_ex);
+    Grid g{nx, ny, nz};
+    return launch3d("EffectiveThermalConductivity", nx, ny, nz, [=] __device__(int i, int j, int k) {
+        const size_t e = g.e(i + 1, j + 1, k + 1);
+        const double ph = phc_ex[e];
+        if (ph != 1.0) {
+            const double T = Tc_ex[e] * scale_T;
+            X[g.c(i, j, k)] = ((1.0 - phi_of(ph, phi0)) * kTs(T) + phi_of(ph, phi0) * kTf(T));
+        } else {
+            X[g.c(i, j, k)] = NAN;
+        }
+    });
+}
+
+int ht3d_EffectiveHeatCapacity(double *Ceff, const double *Tc_ex, const double *Pc_ex, const double *phc_ex,
+                               double phi_i, double sca_T, double sca_sigma, double sca_C, double sca_rho, int nx,
+                               int ny, int nz) {
+    (void)phi_i; (void)sca_sigma; (void)sca_C; (void)sca_rho;
+    CHECK_GRID(nx, ny, nz);
+    CHECK_PTR(Ceff); CHECK_PTR(Tc_ex); CHECK_PTR(Pc_ex); CHECK_PTR(phc_ex);
+    Grid g{nx, ny, nz};
+    return launch3d("EffectiveHeatCapacity", nx, ny, nz, [=] __device__(int i, int j, int k) {
+        const size_t e = g.e(i + 1, j + 1, k + 1);
+        if (phc_ex[e] != 1.0) Ceff[g.c(i, j, k)] = Cs(Tc_ex[e] * sca_T);
+    });
+}
+
+int ht3d_Permeability(double *X, double k_fact, double ymin, double dy, const double *phc_ex, double delta,
+                      double scale_L, int nx, int ny, int nz) {
+    CHECK_GRID(nx, ny, nz);
+    CHECK_PTR(X); CHECK_PTR(phc_ex);
+    Grid g{nx, ny, nz};
+    return launch3d("Permeability", nx, ny, nz, [=] __device__(int i, int j, int k) {
+        const double ph = phc_ex[g.e(i + 1, j + 1, k + 1)];
+        if (ph != 1.0) {
+            const double y = ymin + (double)j * dy + dy / 2.0;
+            double v = kF(y * scale_L, delta * scale_L);
+            if (ph == 2.0) v *= k_fact;
+            X[g.c(i, j, k)] = v;
+        } else {
+            X[g.c(i, j, k)] = NAN;
+        }
+    });
+}
+
+int ht3d_FluidDensity(double *X, const double *Tc_ex, const double *Pc_ex, const double *phc_ex, double scale_T,
+                      double scale_sigma, int nx, int ny, int nz) {
+    CHECK_GRID(nx, ny, nz);
+    CHECK_PTR(X); CHECK_PTR(Tc_ex); CHECK_PTR(Pc_ex); CHECK_PTR(phc_ex);
+    Grid g{nx, ny, nz};
+    return launch3d("FluidDensity", nx, ny, nz, [=] __device__(int i, int j, int k) {
+        const size_t e = g.e(i + 1, j + 1, k + 1);
+        X[g.c(i, j, k)] = phc_ex[e] != 1.0 ? rhof_C(Tc_ex[e] * scale_T - 273.15, Pc_ex[e] * scale_sigma) : NAN;
+    });
+}
+
+int ht3d_Viscosity(double *X, const double *Tc_ex, const double *phc_ex, double scale_T, int nx, int ny, int nz) {
+    CHECK_GRID(nx, ny, nz);
+    CHECK_PTR(X); CHECK_PTR(Tc_ex); CHECK_PTR(phc_ex);
+    Grid g{nx, ny, nz};
+    return launch3d("Viscosity", nx, ny, nz, [=] __device__(int i, int j, int k) {
+        const size_t e = g.e(i + 1, j + 1, k + 1);
+        X[g.c(i, j, k)] = phc_ex[e] != 1.0 ? muf(Tc_ex[e] * scale_T) : NAN;
+    });
+}
+
+int ht3d_Peclet(double *X, const double *Tc_ex, const double *Pc_ex, const double *Vx, const double *Vy,
+                const double *Vz, const double *phc_ex, double phi0, double scale_T, double scale_sigma,
+                double scale_V, double scale_L, int nx, int ny, int nz) {
+    CHECK_GRID(nx, ny, nz);
+    CHECK_PTR(X); CHECK_PTR(Tc_ex); CHECK_PTR(Pc_ex); CHECK_PTR(Vx); CHECK_PTR(Vy); CHECK_PTR(Vz); CHECK_PTR(phc_ex);
+    Grid g{nx, ny, nz};
+    return launch3d("Peclet", nx, ny, nz, [=] __device__(int i, int j, int k) {
+        const size_t e = g.e(i + 1, j + 1, k + 1);
+        const double ph = phc_ex[e];
+        if (ph != 1.0) {
+            const double x = 0.5 * (Vx[g.fx(i, j, k)] + Vx[g.fx(i + 1, j, k)]) * scale_V;
+            const double y = 0.5 * (Vy[g.fy(i, j, k)] + Vy[g.fy(i, j + 1, k)]) * scale_V;
+            const double z = 0.5 * (Vz[g.fz(i, j, k)] + Vz[g.fz(i, j, k + 1)]) * scale_V;
+            const double V = sqrt(x * x + y * y + z * z);
+            const double T = Tc_ex[e] * scale_T;
+            const double P = Pc_ex[e] * scale_sigma;
+            const double rho = rhof_C(T - 273.15, P);
+            const double C = Cf(T);
+            const double keff = ((1.0 - phi_of(ph, phi0)) * kTs(T) + phi_of(ph, phi0) * kTf(T));
+            X[g.c(i, j, k)] = scale_L * rho * C * V / keff;
+        } else {
+            X[g.c(i, j, k)] = NAN;
+        }
+    });
+}
+
 }  // extern "C"

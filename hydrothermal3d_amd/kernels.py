@@ -226,6 +226,37 @@ dim_SetPressureBCs = _make(
     [(n, "d") for n in ("PS", "PN", "y_plateau", "a2", "b2", "rhof", "g", "dx", "dy", "sticky_air")],
     _D + ":291-314 (the dimensional SetPressureBCs!)")
 
+
+# --------------------------------------------------------------- output fields (kernels_HT3D_postprocessing.jl)
+_PP = "Dimensional/kernels_HT3D_postprocessing.jl"
+Phase = _make("Phase", "ht3d_Phase", [("X", "c"), ("phc_ex", "E")], _PP + ":1-10")
+Pressure = _make("Pressure", "ht3d_Pressure", [("X", "c"), ("Pc_ex", "E"), ("phc_ex", "E"), ("scale_sigma", "d")],
+                 _PP + ":12-21")
+Temperature = _make("Temperature", "ht3d_Temperature", [("X", "c"), ("Tc_ex", "E"), ("phc_ex", "E"), ("scale_T", "d")],
+                    _PP + ":23-32")
+Velocity = _make("Velocity", "ht3d_Velocity",
+                 [("X", "c"), ("Y", "c"), ("Z", "c"), ("Vx", "Fx"), ("Vy", "Fy"), ("Vz", "Fz"), ("phc_ex", "E"),
+                  ("scale_V", "d")], _PP + ":34-45")
+EffectiveThermalConductivity = _make(
+    "EffectiveThermalConductivity", "ht3d_EffectiveThermalConductivity",
+    [("X", "c"), ("Tc_ex", "E"), ("phi0", "d"), ("phc_ex", "E"), ("scale_T", "d")], _PP + ":47-57")
+EffectiveHeatCapacity = _make(
+    "EffectiveHeatCapacity", "ht3d_EffectiveHeatCapacity",
+    [("Ceff", "c"), ("Tc_ex", "E"), ("Pc_ex", "E"), ("phc_ex", "E"), ("phi_i", "d"), ("sca_T", "d"),
+     ("sca_sigma", "d"), ("sca_C", "d"), ("sca_rho", "d")], _PP + ":60-70")
+Permeability = _make("Permeability", "ht3d_Permeability",
+                     [("X", "c"), ("k_fact", "d"), ("ymin", "d"), ("dy", "d"), ("phc_ex", "E"), ("delta", "d"),
+                      ("scale_L", "d")], _PP + ":72-85")
+FluidDensity = _make("FluidDensity", "ht3d_FluidDensity",
+                     [("X", "c"), ("Tc_ex", "E"), ("Pc_ex", "E"), ("phc_ex", "E"), ("scale_T", "d"),
+                      ("scale_sigma", "d")], _PP + ":87-98")
+Viscosity = _make("Viscosity", "ht3d_Viscosity", [("X", "c"), ("Tc_ex", "E"), ("phc_ex", "E"), ("scale_T", "d")],
+                  _PP + ":100-109")
+Peclet = _make("Peclet", "ht3d_Peclet",
+               [("X", "c"), ("Tc_ex", "E"), ("Pc_ex", "E"), ("Vx", "Fx"), ("Vy", "Fy"), ("Vz", "Fz"),
+                ("phc_ex", "E"), ("phi0", "d"), ("scale_T", "d"), ("scale_sigma", "d"), ("scale_V", "d"),
+                ("scale_L", "d")], _PP + ":111-128")
+
 # --------------------------------------------------------------- WENO5 (tools/Weno5_Routines*.jl)
 VxPlusMinus = _make("VxPlusMinus", "ht3d_VxPlusMinus", [("Vxm", "c"), ("Vxp", "c"), ("Vx", "Fx")],
                     "tools/Weno5_Routines.jl:5-16")

@@ -128,6 +128,7 @@ class V18Model:
         self.sc: Scaling = geo["sc"]
         self.phys: V18Physics = geo["phys"]
         self.dx, self.dy, self.dz, self.ymin = geo["dx"], geo["dy"], geo["dz"], geo["ymin"]
+        self.xmin, self.zmin = self.phys.xmin / self.sc.L, self.phys.zmin / self.sc.L
         self.pT = thermal or DyrelParams(nitmax=10000, nitout=1000, tol=1e-18)     # v18:308-311
         self.pP = darcy or DyrelParams(nitmax=10000, nitout=1000, tol=1e-18)
         c, E, V = (nx, ny, nz), (nx + 2, ny + 2, nz + 2), (nx + 1, ny + 1, nz + 1)
@@ -238,3 +239,58 @@ class V18Model:
         out = dict(it=self.it, thermal=rT, darcy=rP, dt=dt_used, dt_next=self.dt, n_adv=n_adv, time=self.time)
         self.it += 1
         return out
+
+    # ------------------------------------------------------------------ output and restart
+    def output_fields(self) -> dict:
+        """The VTK fields of HydroThermal3D_v18.jl:671-686, computed on the
+        device by the kernels of kernels_HT3D_postprocessing.jl in the same
+        order through one scratch array (so, as there, HeatCapacity is NaN in
+        air, left by Peclet).  Velocity's second and third components start
+        from zero arrays where the reference reuses the solver's residual
+        arrays (they keep those values in air)."""
+        a, sc, ph = self.a, self.sc, self.phys
+        c = self.grid
+        dumc = a["dumc"]
+        X, Y, Z = DeviceArray.zeros(c), DeviceArray.zeros(c), DeviceArray.zeros(c)
+        out = {}
+        K.Phase(dumc, a["phc_ex"]); out["Phase"] = dumc.get()
+        K.Pressure(dumc, a["Pc_ex"], a["phc_ex"], sc.sigma); out["Pressure"] = dumc.get()
+        K.Temperature(dumc, a["Tc_ex"], a["phc_ex"], sc.T); out["Temperature"] = dumc.get()
+        K.Velocity(X, Y, Z, a["Vx"], a["Vy"], a["Vz"], a["phc_ex"], sc.V)
+        out["Velocity"] = (X.get(), Y.get(), Z.get())
+        K.EffectiveThermalConductivity(dumc, a["Tc_ex"], ph.phi_i, a["phc_ex"], sc.T); out["kThermal"] = dumc.get()
+        K.Permeability(dumc, ph.k_fact, self.ymin, self.dy, a["phc_ex"], ph.delta / sc.L, sc.L)
+        out["Perm"] = dumc.get()
+        K.FluidDensity(dumc, a["Tc_ex"], a["Pc_ex"], a["phc_ex"], sc.T, sc.sigma); out["Density"] = dumc.get()
+        K.Viscosity(dumc, a["Tc_ex"], a["phc_ex"], sc.T); out["Viscosity"] = dumc.get()
+        K.Peclet(dumc, a["Tc_ex"], a["Pc_ex"], a["Vx"], a["Vy"], a["Vz"], a["phc_ex"], ph.phi_i, sc.T, sc.sigma, sc.V,
+                 sc.L)
+        out["Peclet"] = dumc.get()
+        K.EffectiveHeatCapacity(dumc, a["Tc_ex"], a["Pc_ex"], a["phc_ex"], ph.phi_i, sc.T, sc.sigma, sc.C, sc.rho)
+        out["HeatCapacity"] = dumc.get()
+        return out
+
+    def cell_centres(self):
+        nx, ny, nz = self.grid
+        return (self.xmin + self.dx / 2 + self.dx * np.arange(nx), self.ymin + self.dy / 2 + self.dy * np.arange(ny),
+                self.zmin + self.dz / 2 + self.dz * np.arange(nz))
+
+    def save_vtk(self, name) -> "Path":
+        """HT3DOutput%05d.vtr of v18:663-687 (grid points at the cell centres)."""
+        from .vtkio import write_vtr
+        return write_vtr(name, *self.cell_centres(), self.output_fields())
+
+    def save_breakpoint(self, name) -> "Path":
+        """Breakpoint%05d (v18:579-588): Tc_ex, Pc_ex, time and dt."""
+        from .vtkio import save_breakpoint
+        return save_breakpoint(name, self.a["Tc_ex"].get(), self.a["Pc_ex"].get(), self.time, self.dt)
+
+    def restart(self, path, it: int) -> None:
+        """Restart from a breakpoint at time step `it` (v18:384-395)."""
+        from .vtkio import load_breakpoint
+        b = load_breakpoint(path)
+        self.a["Tc_ex"].set(b["Tc_ex"])
+        self.a["Pc_ex"].set(b["Pc_ex"])
+        self.time, self.dt = b["time"], b["dt"]
+        self.it = it + 1
+        self.pluton_introduced = self.it > 1

@@ -265,6 +265,38 @@ int ht3d_dim_SetPressureBCs(double *Pc_ex, const double *wN, const double *phc_e
                             double y_plateau, double a2, double b2, double rhof, double g, double dx, double dy,
                             double sticky_air, int ncx, int ncy, int ncz);
 
+/* ---- Output fields of Dimensional/kernels_HT3D_postprocessing.jl (the VTK
+ * fields of HydroThermal3D_v18.jl:663-687): cell arrays in physical units,
+ * NaN where the phase is 1 (air).                                             */
+/* Phase!  :1-10 */
+int ht3d_Phase(double *X, const double *phc_ex, int ncx, int ncy, int ncz);
+/* Pressure!  :12-21 and Temperature!  :23-32 (field * scale) */
+int ht3d_Pressure(double *X, const double *Pc_ex, const double *phc_ex, double scale_sigma, int ncx, int ncy,
+                  int ncz);
+int ht3d_Temperature(double *X, const double *Tc_ex, const double *phc_ex, double scale_T, int ncx, int ncy, int ncz);
+/* Velocity!  :34-45 (cell averages of the face velocities; in air only X is set) */
+int ht3d_Velocity(double *X, double *Y, double *Z, const double *Vx, const double *Vy, const double *Vz,
+                  const double *phc_ex, double scale_V, int ncx, int ncy, int ncz);
+/* EffectiveThermalConductivity!  :47-57 */
+int ht3d_EffectiveThermalConductivity(double *X, const double *Tc_ex, double phi0, const double *phc_ex,
+                                      double scale_T, int ncx, int ncy, int ncz);
+/* EffectiveHeatCapacity!  :60-70 (the solid's C; air cells untouched) */
+int ht3d_EffectiveHeatCapacity(double *Ceff, const double *Tc_ex, const double *Pc_ex, const double *phc_ex,
+                               double phi_i, double sca_T, double sca_sigma, double sca_C, double sca_rho, int ncx,
+                               int ncy, int ncz);
+/* Permeability!  :72-85 */
+int ht3d_Permeability(double *X, double k_fact, double ymin, double dy, const double *phc_ex, double delta,
+                      double scale_L, int ncx, int ncy, int ncz);
+/* FluidDensity!  :87-98 */
+int ht3d_FluidDensity(double *X, const double *Tc_ex, const double *Pc_ex, const double *phc_ex, double scale_T,
+                      double scale_sigma, int ncx, int ncy, int ncz);
+/* Viscosity!  :100-109 */
+int ht3d_Viscosity(double *X, const double *Tc_ex, const double *phc_ex, double scale_T, int ncx, int ncy, int ncz);
+/* Peclet!  :111-128 */
+int ht3d_Peclet(double *X, const double *Tc_ex, const double *Pc_ex, const double *Vx, const double *Vy,
+                const double *Vz, const double *phc_ex, double phi0, double scale_T, double scale_sigma,
+                double scale_V, double scale_L, int ncx, int ncy, int ncz);
+
 /* Temperature-dependent viscosity mu = exp(-a T): face coefficients *= exp(a T_face),
  * T_face the mean of the two adjacent cells (ghosts at walls).  No counterpart in
  * the dimensionless kernels (constant viscosity there); the dimensionless analogue

@@ -284,3 +284,71 @@ def ResidualFluidPressure(ph, phirhof, phirhof0, PC, qx, qy, qz, transient, _dt,
     F = transient * _dt * (phirhof - phirhof0)
     F = _flux_div(F, qx, qy, qz, _dx, _dy, _dz) / PC
     return np.asfortranarray(np.where(np.abs(p) != 1.0, F, 0.0))
+
+
+# ------------------------------------------------------ output fields (kernels_HT3D_postprocessing.jl)
+def _inner(A):
+    return A[1:-1, 1:-1, 1:-1]
+
+
+def _air(ph):
+    return _inner(ph) == 1.0
+
+
+def Phase(ph):
+    return np.asfortranarray(np.where(_air(ph), np.nan, _inner(ph)))
+
+
+def Scaled(F, ph, scale):
+    """Pressure! / Temperature!"""
+    return np.asfortranarray(np.where(_air(ph), np.nan, _inner(F) * scale))
+
+
+def Velocity(X, Y, Z, Vx, Vy, Vz, ph, sV):
+    X, Y, Z = (np.array(a, order="F", copy=True) for a in (X, Y, Z))
+    air = _air(ph)
+    vx = 0.5 * (Vx[:-1] + Vx[1:]) * sV
+    vy = 0.5 * (Vy[:, :-1] + Vy[:, 1:]) * sV
+    vz = 0.5 * (Vz[:, :, :-1] + Vz[:, :, 1:]) * sV
+    X[...] = np.where(air, np.nan, vx)
+    Y[~air], Z[~air] = vy[~air], vz[~air]
+    return X, Y, Z
+
+
+def EffectiveThermalConductivity(T, phi0, ph, sT):
+    Ts = _inner(T) * sT
+    f = phi(_inner(ph), phi0)
+    return np.asfortranarray(np.where(_air(ph), np.nan, (1.0 - f) * kTs(Ts) + f * kTf(Ts)))
+
+
+def EffectiveHeatCapacity(C, T, ph, sT):
+    out = np.array(C, order="F", copy=True)
+    m = ~_air(ph)
+    out[m] = Cs(_inner(T) * sT)[m]
+    return out
+
+
+def Permeability(k_fact, ymin, dy, ph, delta, sL, shape):
+    y = (ymin + np.arange(shape[1], dtype=np.float64) * dy + dy / 2.0)[None, :, None]
+    v = np.broadcast_to(kF(y * sL, delta * sL), shape)
+    v = np.where(_inner(ph) == 2.0, v * k_fact, v)
+    return np.asfortranarray(np.where(_air(ph), np.nan, v))
+
+
+def FluidDensity(T, P, ph, sT, ss):
+    return np.asfortranarray(np.where(_air(ph), np.nan, rhof_C(_inner(T) * sT - 273.15, _inner(P) * ss)))
+
+
+def Viscosity(T, ph, sT):
+    return np.asfortranarray(np.where(_air(ph), np.nan, muf(_inner(T) * sT)))
+
+
+def Peclet(T, P, Vx, Vy, Vz, ph, phi0, sT, ss, sV, sL):
+    x = 0.5 * (Vx[:-1] + Vx[1:]) * sV
+    y = 0.5 * (Vy[:, :-1] + Vy[:, 1:]) * sV
+    z = 0.5 * (Vz[:, :, :-1] + Vz[:, :, 1:]) * sV
+    V = np.sqrt(x * x + y * y + z * z)
+    Ts, Ps = _inner(T) * sT, _inner(P) * ss
+    f = phi(_inner(ph), phi0)
+    keff = (1.0 - f) * kTs(Ts) + f * kTf(Ts)
+    return np.asfortranarray(np.where(_air(ph), np.nan, sL * rhof_C(Ts - 273.15, Ps) * Cf(Ts) * V / keff))

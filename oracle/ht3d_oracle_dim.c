@@ -470,3 +470,108 @@ EXPORT int o_v18_darcy_solve(o_v18_darcy *f, const o_dyrel_params *p, o_dyrel_re
     return v18_control(f->nx, f->ny, f->nz, f->phirho, f->kx, f->ky, f->kz, f->transient, f->dt, f->dx, f->dy, f->dz,
                        f->Pc_ex, f->G, f->PC, f->F, f->F0, f->Fcit, f->dumc, f->norm_scale, p, rep, darcy_sweep, f);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Output fields: Dimensional/kernels_HT3D_postprocessing.jl (cell arrays,   */
+/* NaN in air unless noted)                                                  */
+/* ------------------------------------------------------------------------ */
+
+#define PH_C (Ee(phc_ex, i + 1, j + 1, k + 1))
+
+/* Phase!  :1-10 */
+EXPORT void o_Phase(double *X, const double *phc_ex, int nx, int ny, int nz) {
+    PFOR_K(nz) FOR_JI(nx, ny) Cc(X, i, j, k) = PH_C != 1.0 ? PH_C : NAN;
+}
+
+/* Pressure! / Temperature!  :12-32 (field * scale) */
+EXPORT void o_Pressure(double *X, const double *Pc_ex, const double *phc_ex, double scale, int nx, int ny, int nz) {
+    PFOR_K(nz) FOR_JI(nx, ny) Cc(X, i, j, k) = PH_C != 1.0 ? Ee(Pc_ex, i + 1, j + 1, k + 1) * scale : NAN;
+}
+
+/* Velocity!  :34-45 (face velocities averaged to the cell; only X is set to NaN in air) */
+EXPORT void o_Velocity(double *X, double *Y, double *Z, const double *Vx, const double *Vy, const double *Vz,
+                       const double *phc_ex, double scale_V, int nx, int ny, int nz) {
+    PFOR_K(nz) FOR_JI(nx, ny) {
+        if (PH_C != 1.0) {
+            Cc(X, i, j, k) = 0.5 * (FX(Vx, i, j, k) + FX(Vx, i + 1, j, k)) * scale_V;
+            Cc(Y, i, j, k) = 0.5 * (FY(Vy, i, j, k) + FY(Vy, i, j + 1, k)) * scale_V;
+            Cc(Z, i, j, k) = 0.5 * (FZ(Vz, i, j, k) + FZ(Vz, i, j, k + 1)) * scale_V;
+        } else {
+            Cc(X, i, j, k) = NAN;
+        }
+    }
+}
+
+/* EffectiveThermalConductivity!  :47-57 */
+EXPORT void o_EffectiveThermalConductivity(double *X, const double *Tc_ex, double phi0, const double *phc_ex,
+                                           double scale_T, int nx, int ny, int nz) {
+    PFOR_K(nz) FOR_JI(nx, ny) {
+        if (PH_C != 1.0) {
+            const double T = Ee(Tc_ex, i + 1, j + 1, k + 1) * scale_T;
+            Cc(X, i, j, k) = ((1.0 - phi_of(PH_C, phi0)) * kTs(T) + phi_of(PH_C, phi0) * kTf(T));
+        } else {
+            Cc(X, i, j, k) = NAN;
+        }
+    }
+}
+
+/* EffectiveHeatCapacity!  :60-70 (the solid's C; air cells untouched) */
+EXPORT void o_EffectiveHeatCapacity(double *Ceff, const double *Tc_ex, const double *Pc_ex, const double *phc_ex,
+                                    double phi_i, double sca_T, double sca_sigma, double sca_C, double sca_rho,
+                                    int nx, int ny, int nz) {
+    PFOR_K(nz) FOR_JI(nx, ny) {
+        if (PH_C != 1.0) Cc(Ceff, i, j, k) = Cs(Ee(Tc_ex, i + 1, j + 1, k + 1) * sca_T);
+    }
+}
+
+/* Permeability!  :72-85 (cell-centre depth) */
+EXPORT void o_Permeability(double *X, double k_fact, double ymin, double dy, const double *phc_ex, double delta,
+                           double scale_L, int nx, int ny, int nz) {
+    PFOR_K(nz) FOR_JI(nx, ny) {
+        if (PH_C != 1.0) {
+            const double y = ymin + (double)(j - 1) * dy + dy / 2.0;
+            double v = kF(y * scale_L, delta * scale_L);
+            if (PH_C == 2.0) v *= k_fact;
+            Cc(X, i, j, k) = v;
+        } else {
+            Cc(X, i, j, k) = NAN;
+        }
+    }
+}
+
+/* FluidDensity!  :87-98 (kg/m^3) */
+EXPORT void o_FluidDensity(double *X, const double *Tc_ex, const double *Pc_ex, const double *phc_ex, double scale_T,
+                           double scale_sigma, int nx, int ny, int nz) {
+    PFOR_K(nz) FOR_JI(nx, ny) {
+        Cc(X, i, j, k) = PH_C != 1.0 ? rhof_C(Ee(Tc_ex, i + 1, j + 1, k + 1) * scale_T - 273.15,
+                                              Ee(Pc_ex, i + 1, j + 1, k + 1) * scale_sigma)
+                                     : NAN;
+    }
+}
+
+/* Viscosity!  :100-109 (Pa s) */
+EXPORT void o_Viscosity(double *X, const double *Tc_ex, const double *phc_ex, double scale_T, int nx, int ny, int nz) {
+    PFOR_K(nz) FOR_JI(nx, ny) Cc(X, i, j, k) = PH_C != 1.0 ? muf(Ee(Tc_ex, i + 1, j + 1, k + 1) * scale_T) : NAN;
+}
+
+/* Peclet!  :111-128: L rho_f C_f |V| / k_eff */
+EXPORT void o_Peclet(double *X, const double *Tc_ex, const double *Pc_ex, const double *Vx, const double *Vy,
+                     const double *Vz, const double *phc_ex, double phi0, double scale_T, double scale_sigma,
+                     double scale_V, double scale_L, int nx, int ny, int nz) {
+    PFOR_K(nz) FOR_JI(nx, ny) {
+        if (PH_C != 1.0) {
+            const double x = 0.5 * (FX(Vx, i, j, k) + FX(Vx, i + 1, j, k)) * scale_V;
+            const double y = 0.5 * (FY(Vy, i, j, k) + FY(Vy, i, j + 1, k)) * scale_V;
+            const double z = 0.5 * (FZ(Vz, i, j, k) + FZ(Vz, i, j, k + 1)) * scale_V;
+            const double V = sqrt(x * x + y * y + z * z);
+            const double T = Ee(Tc_ex, i + 1, j + 1, k + 1) * scale_T;
+            const double P = Ee(Pc_ex, i + 1, j + 1, k + 1) * scale_sigma;
+            const double rho = rhof_C(T - 273.15, P);
+            const double C = Cf(T);
+            const double keff = ((1.0 - phi_of(PH_C, phi0)) * kTs(T) + phi_of(PH_C, phi0) * kTf(T));
+            Cc(X, i, j, k) = scale_L * rho * C * V / keff;
+        } else {
+            Cc(X, i, j, k) = NAN;
+        }
+    }
+}

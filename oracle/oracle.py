@@ -220,6 +220,26 @@ dim_SetPressureBCs = _make(
     [("Pc_ex", "E"), ("wN", "W"), ("phc_ex", "E")] +
     [(n, "d") for n in ("PS", "PN", "y_plateau", "a2", "b2", "rhof", "g", "dx", "dy", "sticky_air")])
 
+
+Phase = _make("Phase", "o_Phase", [("X", "c"), ("phc_ex", "E")])
+Pressure = _make("Pressure", "o_Pressure", [("X", "c"), ("Pc_ex", "E"), ("phc_ex", "E"), ("scale", "d")])
+Temperature = Pressure
+Velocity = _make("Velocity", "o_Velocity", [("X", "c"), ("Y", "c"), ("Z", "c"), ("Vx", "Fx"), ("Vy", "Fy"),
+                                            ("Vz", "Fz"), ("phc_ex", "E"), ("scale_V", "d")])
+EffectiveThermalConductivity = _make("EffectiveThermalConductivity", "o_EffectiveThermalConductivity",
+                                     [("X", "c"), ("Tc_ex", "E"), ("phi0", "d"), ("phc_ex", "E"), ("scale_T", "d")])
+EffectiveHeatCapacity = _make("EffectiveHeatCapacity", "o_EffectiveHeatCapacity",
+                              [("Ceff", "c"), ("Tc_ex", "E"), ("Pc_ex", "E"), ("phc_ex", "E")] +
+                              [(n, "d") for n in ("phi_i", "sca_T", "sca_sigma", "sca_C", "sca_rho")])
+Permeability = _make("Permeability", "o_Permeability", [("X", "c"), ("k_fact", "d"), ("ymin", "d"), ("dy", "d"),
+                                                        ("phc_ex", "E"), ("delta", "d"), ("scale_L", "d")])
+FluidDensity = _make("FluidDensity", "o_FluidDensity", [("X", "c"), ("Tc_ex", "E"), ("Pc_ex", "E"), ("phc_ex", "E"),
+                                                        ("scale_T", "d"), ("scale_sigma", "d")])
+Viscosity = _make("Viscosity", "o_Viscosity", [("X", "c"), ("Tc_ex", "E"), ("phc_ex", "E"), ("scale_T", "d")])
+Peclet = _make("Peclet", "o_Peclet", [("X", "c"), ("Tc_ex", "E"), ("Pc_ex", "E"), ("Vx", "Fx"), ("Vy", "Fy"),
+                                      ("Vz", "Fz"), ("phc_ex", "E")] +
+               [(n, "d") for n in ("phi0", "scale_T", "scale_sigma", "scale_V", "scale_L")])
+
 EOS = {"kTs": 0, "kTf": 1, "Cf": 2, "Cs": 3, "rhos": 4, "rhof_C": 5, "drhodP_C": 6, "muf": 7, "kF": 8, "phi": 9,
        "pow4": 10}
 

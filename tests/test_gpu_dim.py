@@ -117,3 +117,42 @@ def test_ghost_fill_rejects_bad_weight_table(ht):
     with pytest.raises(ValueError):
         ht.kernels.dim_SetPressureBCs(_dev(ht, d["P"]), _dev(ht, d["wN"][:, :-1]), _dev(ht, d["phc"]),
                                       *((0.0,) * 10))
+
+
+@pytest.mark.parametrize("grid", GRIDS[:2], ids=lambda g: "x".join(map(str, g)))
+def test_output_fields(ht, orc, grid):
+    """kernels_HT3D_postprocessing.jl on the GPU against the oracle: bit for bit
+    except Permeability and Viscosity (exp, 10^x: 1e-14 relative)."""
+    rng = np.random.default_rng(300 + grid[0])
+    d = dim_state(*grid, rng, air_from=grid[1] - 1)
+    sc, sh = d["sc"], d["sh"]
+    K = ht.kernels
+    Td, Pd, phc = _dev(ht, d["T"]), _dev(ht, d["P"]), _dev(ht, d["phc"])
+    V = [np.asfortranarray(rng.uniform(-1, 1, sh[s])) for s in ("Fx", "Fy", "Fz")]
+    Vd = [_dev(ht, a) for a in V]
+    X, Xd = np.zeros(sh["c"], order="F"), ht.DeviceArray.zeros(sh["c"])
+    eq = lambda a, b: np.array_equal(a, b, equal_nan=True)
+    print(f"output fields {grid}", flush=True)
+    orc.Phase(X, d["phc"]); K.Phase(Xd, phc); assert eq(Xd.get(), X)
+    orc.Pressure(X, d["P"], d["phc"], sc.sigma); K.Pressure(Xd, Pd, phc, sc.sigma); assert eq(Xd.get(), X)
+    orc.Temperature(X, d["T"], d["phc"], sc.T); K.Temperature(Xd, Td, phc, sc.T); assert eq(Xd.get(), X)
+    Y0 = [np.asfortranarray(rng.uniform(-1, 1, sh["c"])) for _ in range(3)]
+    Yh = [a.copy(order="F") for a in Y0]
+    Yd = [_dev(ht, a) for a in Y0]
+    orc.Velocity(*Yh, *V, d["phc"], sc.V); K.Velocity(*Yd, *Vd, phc, sc.V)
+    for a, b in zip(Yd, Yh):
+        assert eq(a.get(), b)
+    orc.EffectiveThermalConductivity(X, d["T"], 0.01, d["phc"], sc.T)
+    K.EffectiveThermalConductivity(Xd, Td, 0.01, phc, sc.T); assert eq(Xd.get(), X)
+    orc.EffectiveHeatCapacity(X, d["T"], d["P"], d["phc"], 0.01, sc.T, sc.sigma, sc.C, sc.rho)
+    K.EffectiveHeatCapacity(Xd, Td, Pd, phc, 0.01, sc.T, sc.sigma, sc.C, sc.rho); assert eq(Xd.get(), X)
+    orc.FluidDensity(X, d["T"], d["P"], d["phc"], sc.T, sc.sigma)
+    K.FluidDensity(Xd, Td, Pd, phc, sc.T, sc.sigma); assert eq(Xd.get(), X)
+    orc.Peclet(X, d["T"], d["P"], *V, d["phc"], 0.01, sc.T, sc.sigma, sc.V, sc.L)
+    K.Peclet(Xd, Td, Pd, *Vd, phc, 0.01, sc.T, sc.sigma, sc.V, sc.L); assert eq(Xd.get(), X)
+    orc.Permeability(X, 50.0, -30e3 / sc.L, 5.0, d["phc"], 3000.0 / sc.L, sc.L)
+    K.Permeability(Xd, 50.0, -30e3 / sc.L, 5.0, phc, 3000.0 / sc.L, sc.L)
+    np.testing.assert_allclose(Xd.get(), X, rtol=1e-14)
+    orc.Viscosity(X, d["T"], d["phc"], sc.T)
+    K.Viscosity(Xd, Td, phc, sc.T)
+    np.testing.assert_allclose(Xd.get(), X, rtol=1e-14)

@@ -165,3 +165,39 @@ def test_model_steps_match_oracle(ht, orc, grid, immersed):
             err, spread = np.max(np.abs(got - host)) / scale, np.max(np.abs(host2 - host)) / scale
             print(f"  {name}: gpu-oracle {err:.2e}, one-ulp spread {spread:.2e}", flush=True)
             assert err <= 10.0 * spread + 1e-12, (s, name, err, spread)
+
+
+def test_model_output_and_restart(ht, tmp_path):
+    """V18Model writes the v18 VTK fields (read back equal to output_fields) and
+    a breakpoint; a model restarted from it takes the same next step as the one
+    that continued (bit for bit)."""
+    from hydrothermal3d_amd.v18 import V18Model, v18_geometry
+    from hydrothermal3d_amd.vtkio import read_vtr
+    grid = (24, 16, 8)
+    geo = v18_geometry(*grid, seed=6)
+    prm = ht.DyrelParams(nitmax=200, nitout=100, tol=1e-18)
+    m = V18Model(*grid, geo, prm, prm)
+    print("v18 output/restart", flush=True)
+    m.step()
+    m.step()
+    fields = m.output_fields()
+    path = m.save_vtk(tmp_path / "HT3DOutput00002")
+    (x, y, z), f = read_vtr(path)
+    assert len(x) == grid[0] and len(y) == grid[1] and len(z) == grid[2]
+    assert set(f) == {"Phase", "Pressure", "Temperature", "Velocity", "kThermal", "Perm", "Density", "Viscosity",
+                      "Peclet", "HeatCapacity"}
+    for k, v in fields.items():
+        if k == "Velocity":
+            for a, b in zip(f[k], v):
+                assert np.array_equal(a, b, equal_nan=True)
+        else:
+            assert np.array_equal(f[k], v, equal_nan=True), k
+    air = geo["phc"][1:-1, 1:-1, 1:-1] == 1.0
+    assert np.all(np.isnan(f["Temperature"][air])) and np.all(np.isfinite(f["Temperature"][~air]))
+    bp = m.save_breakpoint(tmp_path / "Breakpoint00002")
+    r = V18Model(*grid, geo, prm, prm)
+    r.restart(bp, 2)
+    a, b = m.step(), r.step()
+    assert a["thermal"]["iters"] == b["thermal"]["iters"] and a["n_adv"] == b["n_adv"]
+    assert np.array_equal(m["Tc_ex"].get(), r["Tc_ex"].get())
+    assert np.array_equal(m["Pc_ex"].get(), r["Pc_ex"].get())

@@ -347,3 +347,71 @@ def test_v18_darcy_solve_zeroes_the_residual(orc, npd, immersed):
                                  **dd, p=orc.params(nitmax=100000, nitout=100, ncheck=10, norm=1, tol=1e-13))
     assert rep["status"] == 0 and rep["iters"] > 50
     assert np.max(np.abs(residual(P))) <= 1e-10 * r0
+
+
+# ------------------------------------------------------------ output fields
+def _eq_nan(a, b):
+    return np.array_equal(a, b, equal_nan=True)
+
+
+def test_output_fields_match_numpy(orc, npd):
+    grid = (9, 7, 6)
+    d = dim_state(*grid, np.random.default_rng(31), air_from=5)
+    sc, sh = d["sc"], d["sh"]
+    rng = np.random.default_rng(32)
+    z = lambda s: np.zeros(sh[s], order="F")
+    V = [np.asfortranarray(rng.uniform(-1, 1, sh[s])) for s in ("Fx", "Fy", "Fz")]
+    X = z("c")
+    orc.Phase(X, d["phc"])
+    assert _eq_nan(X, npd.Phase(d["phc"]))
+    orc.Pressure(X, d["P"], d["phc"], sc.sigma)
+    assert _eq_nan(X, npd.Scaled(d["P"], d["phc"], sc.sigma))
+    orc.Temperature(X, d["T"], d["phc"], sc.T)
+    assert _eq_nan(X, npd.Scaled(d["T"], d["phc"], sc.T))
+    XYZ0 = [np.asfortranarray(rng.uniform(-1, 1, sh["c"])) for _ in range(3)]
+    XYZ = [a.copy(order="F") for a in XYZ0]
+    orc.Velocity(*XYZ, *V, d["phc"], sc.V)
+    for a, b in zip(XYZ, npd.Velocity(*XYZ0, *V, d["phc"], sc.V)):
+        assert _eq_nan(a, b)
+    orc.EffectiveThermalConductivity(X, d["T"], 0.01, d["phc"], sc.T)
+    assert _eq_nan(X, npd.EffectiveThermalConductivity(d["T"], 0.01, d["phc"], sc.T))
+    C0 = np.asfortranarray(rng.uniform(-1, 1, sh["c"]))
+    C = C0.copy(order="F")
+    orc.EffectiveHeatCapacity(C, d["T"], d["P"], d["phc"], 0.01, sc.T, sc.sigma, sc.C, sc.rho)
+    assert _eq_nan(C, npd.EffectiveHeatCapacity(C0, d["T"], d["phc"], sc.T))
+    orc.Permeability(X, 50.0, -30e3 / sc.L, 5.0, d["phc"], 3000.0 / sc.L, sc.L)
+    np.testing.assert_allclose(X, npd.Permeability(50.0, -30e3 / sc.L, 5.0, d["phc"], 3000.0 / sc.L, sc.L, sh["c"]),
+                               rtol=1e-15)
+    orc.FluidDensity(X, d["T"], d["P"], d["phc"], sc.T, sc.sigma)
+    assert _eq_nan(X, npd.FluidDensity(d["T"], d["P"], d["phc"], sc.T, sc.sigma))
+    orc.Viscosity(X, d["T"], d["phc"], sc.T)
+    np.testing.assert_allclose(X, npd.Viscosity(d["T"], d["phc"], sc.T), rtol=1e-15)
+    orc.Peclet(X, d["T"], d["P"], *V, d["phc"], 0.01, sc.T, sc.sigma, sc.V, sc.L)
+    np.testing.assert_allclose(X, npd.Peclet(d["T"], d["P"], *V, d["phc"], 0.01, sc.T, sc.sigma, sc.V, sc.L),
+                               rtol=1e-15)
+
+
+def test_output_fields_closed_forms(orc):
+    """Uniform flow and temperature: the Peclet number is L rho_f C_f |V| / k_eff
+    from the closures; the velocity field is the face value; air is NaN."""
+    from hydrothermal3d_amd.dimensional import v18_scaling
+    sc = v18_scaling()
+    grid = (5, 4, 3)
+    sh = shapes(*grid)
+    T = np.full(sh["E"], 500.0 / sc.T, order="F")
+    P = np.full(sh["E"], 2e7 / sc.sigma, order="F")
+    ph = np.zeros(sh["E"], order="F")
+    ph[:, -2:, :] = 1.0
+    v = (3e-9 / sc.V, -4e-9 / sc.V, 12e-9 / sc.V)
+    V = [np.full(sh[s], vv, order="F") for s, vv in zip(("Fx", "Fy", "Fz"), v)]
+    X = np.zeros(sh["c"], order="F")
+    orc.Peclet(X, T, P, *V, ph, 0.01, sc.T, sc.sigma, sc.V, sc.L)
+    rho = orc.eos("rhof_C", 500.0 - 273.15, 2e7)
+    want = sc.L * rho * (1000 + 7.5 * 500.0) * 13e-9 / (0.99 * (3.1138 - 0.0023 * 500) + 0.01 * (-0.069 + 0.0012 * 500))
+    rock = ph[1:-1, 1:-1, 1:-1] != 1.0
+    np.testing.assert_allclose(X[rock], want, rtol=1e-12)
+    assert np.all(np.isnan(X[~rock]))
+    Y = [np.zeros(sh["c"], order="F") for _ in range(3)]
+    orc.Velocity(*Y, *V, ph, sc.V)
+    for a, vv in zip(Y, (3e-9, -4e-9, 12e-9)):
+        np.testing.assert_allclose(a[:, :2, :], vv, rtol=1e-14)
