@@ -29,7 +29,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib, kernels as K
-from .device import DeviceArray, max_abs_g
+from .device import DeviceArray, max_abs_g, maximum_g, minimum_g
 from .dimensional import Scaling, v18_scaling, v18_darcy_solve, v18_thermal_solve
 from .solver import DyrelParams
 
@@ -237,8 +237,18 @@ class V18Model:
             self.time += self.dt
             n_adv = self.advect()
         out = dict(it=self.it, thermal=rT, darcy=rP, dt=dt_used, dt_next=self.dt, n_adv=n_adv, time=self.time)
+        out.update(self.extrema())
         self.it += 1
         return out
+
+    def extrema(self) -> dict:
+        """min/max of Tc_ex (K), Pc_ex (Pa) and Vy (m/s), the diagnostics the v18
+        driver prints after each step (v18:575-577), as device reductions (8 bytes
+        read back each) instead of host copies."""
+        a, sc = self.a, self.sc
+        return {"T_min": minimum_g(a["Tc_ex"]) * sc.T, "T_max": maximum_g(a["Tc_ex"]) * sc.T,
+                "P_min": minimum_g(a["Pc_ex"]) * sc.sigma, "P_max": maximum_g(a["Pc_ex"]) * sc.sigma,
+                "Vy_min": minimum_g(a["Vy"]) * sc.V, "Vy_max": maximum_g(a["Vy"]) * sc.V}
 
     # ------------------------------------------------------------------ output and restart
     def output_fields(self) -> dict:

@@ -151,6 +151,11 @@ def test_model_steps_match_oracle(ht, orc, grid, immersed):
         print(f"v18 model step {s} {grid} immersed={immersed}", flush=True)
         ro, ra, rg = ref.step(), alt.step(), gpu.step()
         assert rg["n_adv"] == ro["n_adv"]
+        # device-side diagnostics (v18:575-577) against the host arrays
+        assert rg["T_max"] == pytest.approx(np.max(ref.T) * ref.sc.T, rel=1e-8)
+        assert rg["P_min"] == pytest.approx(np.min(ref.P) * ref.sc.sigma, rel=1e-8)
+        assert rg["Vy_max"] == pytest.approx(np.max(ref.Vy) * ref.sc.V, rel=1e-8,
+                                             abs=1e-8 * np.max(np.abs(ref.Vy)) * ref.sc.V)
         for solve in ("thermal", "darcy"):
             for k in ("iters", "status", "n_checks", "n_lambda"):
                 assert rg[solve][k] == ro[solve][k], (s, solve, k)
