@@ -11,7 +11,7 @@ from helpers import v18_problem
 
 pytestmark = pytest.mark.gpu
 
-GRIDS = [(16, 12, 6), (70, 37, 9), (37, 70, 13)]
+GRIDS = [(16, 12, 6), (70, 37, 9), (37, 70, 13), (250, 106, 3)]   # the last: the v18 grid (v18:291-294)
 KEYS = ("iters", "status", "n_checks", "n_lambda", "lmax", "lmin", "dtau", "h1", "h2", "nF_abs", "nF_rel", "nF_ini")
 
 
@@ -122,7 +122,7 @@ def test_every_tile_bitwise(ht, orc):
 
 
 @pytest.mark.parametrize("immersed", [False, True], ids=["flat", "immersed"])
-@pytest.mark.parametrize("grid", [(24, 16, 8), (70, 30, 9)], ids=lambda g: "x".join(map(str, g)))
+@pytest.mark.parametrize("grid", [(24, 16, 8), (70, 30, 9), (250, 106, 3)], ids=lambda g: "x".join(map(str, g)))
 def test_model_steps_match_oracle(ht, orc, grid, immersed):
     """Three passes of the v18 time loop (the steady initialisation step, the
     intrusion step, one more) by hydrothermal3d_amd.v18.V18Model against the
