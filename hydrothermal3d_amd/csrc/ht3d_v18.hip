@@ -26,6 +26,9 @@
 //     Darcy: phi*rho_f of the cell is evaluated from the EOS each iteration
 //     (the reference's ϕρf launch), g*<rho_f> on the y faces is formed once
 //     per solve, and the stored diagonal is read;
+//   * the once-read streams (class byte, old T or T, rho*C or phi*rho_f_old, D)
+//     are loaded and the unknown stored non-temporally, which leaves the L2 to
+//     the planes of the unknown that neighbouring waves re-read;
 //   * check iterations accumulate |F*PC|, max|F*PC| and the Rayleigh-quotient
 //     sums in double-double block partials, like the dimensionless solve.
 // Memory left as the reference leaves it: the unknown with ghosts, air and
@@ -157,7 +160,7 @@ __global__ __launch_bounds__(64 * TY) void iter_kernel(Args a) {
             const double xp = a.Xin[e + sE2];
             const double kF = a.kz[c + sC2];
             const double kB = kzB;
-            const uint8_t cl = a.cls[c];
+            const uint8_t cl = __builtin_nontemporal_load(a.cls + c);
             const double f0o = __builtin_nontemporal_load(a.F0 + c);
             double f = 0.0, D = 1.0;
             if (!(cl & C_ZERO)) {   // air and immersed cells: F = 0, nothing else to load
@@ -186,20 +189,20 @@ __global__ __launch_bounds__(64 * TY) void iter_kernel(Args a) {
                     // ϕρf of the cell (kernels_HT3D_dimensional.jl:225-235), then
                     // ComputeDarcyFlux! and ResidualFluidPressure! (:266-289)
                     const double phi = (cl & C_FAULT) ? 3.0 * a.phi_i : a.phi_i;
-                    const double Tk = a.aux[e];
+                    const double Tk = __builtin_nontemporal_load(a.aux + e);
                     const double prf = phi * dim::rhof_C(Tk * a.s_T - 273.15, xc * a.s_sigma) / a.s_rho;
                     const double qS = -kS * (a._dy * (xc - s) - a.gy[fy]);
                     const double qN = -kN * (a._dy * (n - xc) - a.gy[fy + nx]);
-                    f = a.tr_dt * (prf - a.aux2[c]);
+                    f = a.tr_dt * (prf - __builtin_nontemporal_load(a.aux2 + c));
                     f += (qE - qW) * a._dx;
                     f += (qN - qS) * a._dy;
                     f += (qF - qB) * a._dz;
-                    D = a.Dst[c];
+                    D = __builtin_nontemporal_load(a.Dst + c);
                 } else {
                     // ComputeFlux! and ResidualTemperatureLinearised! (:140-181)
-                    const double rc = a.aux2[c];
+                    const double rc = __builtin_nontemporal_load(a.aux2 + c);
                     const double qS = -kS * (a._dy * (xc - s)), qN = -kN * (a._dy * (n - xc));
-                    f = a.transient * rc * a._dt * (xc - a.aux[c]);
+                    f = a.transient * rc * a._dt * (xc - __builtin_nontemporal_load(a.aux + c));
                     f += (qE - qW) * a._dx;
                     f += (qN - qS) * a._dy;
                     f += (qF - qB) * a._dz;
@@ -212,7 +215,7 @@ __global__ __launch_bounds__(64 * TY) void iter_kernel(Args a) {
             // DYRELUpdate! (kernels_HT3D_dimensional.jl:323-324)
             const double f0 = a.h1 * f0o + a.h2 * f;
             __builtin_nontemporal_store(f0, a.F0 + c);
-            a.Xout[e] = xc - a.dtau * f0;
+            __builtin_nontemporal_store(xc - a.dtau * f0, a.Xout + e);
             if (check) {
                 const double fd = fabs(f * D);
                 dd_add(s_hi, s_lo, fd);
