@@ -75,7 +75,7 @@ __global__ __launch_bounds__(64 * TY) void batch_iter_kernel(const IterArgs<T> *
                                                              unsigned flags) {
     const unsigned n = *n_items;
     for (unsigned w = blockIdx.x; w < n; w += gridDim.x) {
-        const unsigned t = xcd_tile(w, n);
+        const unsigned t = xcd_tile(w, n, flags & FL_REVERSE);
         const uint2 it = items[t];
         if (!st[it.x].active) continue;
         const IterArgs<T> &a = tab[it.x];
@@ -835,7 +835,8 @@ static int batch_solve(ht3d_batch *b, int kind, const ht3d_dyrel_params &p_in, c
                 const bool lam = lam_it(it, p.nitout);
                 chk = lam || (p.ncheck > 0 && it % p.ncheck == 0);
                 const bool store = it < p.nitmax && lam_it(it + 1, p.nitout);
-                const unsigned fl = (chk ? FL_CHECK : 0u) | (lam ? FL_LAMBDA : 0u) | (store ? FL_STORE_F : 0u);
+                const unsigned fl = (chk ? FL_CHECK : 0u) | (lam ? FL_LAMBDA : 0u) | (store ? FL_STORE_F : 0u) |
+                                    ((g_alternate && (it & 1) == 0) ? FL_REVERSE : 0u);
                 ops.push_back(BOp{OP_ITER64, fl, (it - 1) & 1, 0});
                 if (chk) ops.push_back(BOp{OP_CHECK64, 0u, L, lam ? 1 : 0});
                 if (chk || it == p.nitmax) break;
@@ -859,7 +860,8 @@ static int batch_solve(ht3d_batch *b, int kind, const ht3d_dyrel_params &p_in, c
                 const int it = i0 + k + 1;
                 const bool lam = lam_it(it, p.nitout);
                 const bool store = it < p.nitmax && lam_it(it + 1, p.nitout);
-                const unsigned fl = (lam ? (FL_CHECK | FL_LAMBDA) : 0u) | (store ? FL_STORE_F : 0u);
+                const unsigned fl = (lam ? (FL_CHECK | FL_LAMBDA) : 0u) | (store ? FL_STORE_F : 0u) |
+                                    ((g_alternate && (it & 1) == 0) ? FL_REVERSE : 0u);
                 ops.push_back(BOp{OP_ITER32, fl, (i0 + k) & 1, 0});
                 if (lam) ops.push_back(BOp{OP_LAM32, 0u, 0, it > 1 ? 1 : 0});
             }
