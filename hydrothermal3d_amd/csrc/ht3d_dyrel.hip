@@ -28,6 +28,9 @@
 #include "ht3d_iter.hpp"
 
 namespace ht3d {
+namespace v18 {
+void clear_tiles();   // ht3d_v18.hip
+}
 
 // Gershgorin bound (GershgorinPoisson!, kernels_HT3D_dimensional.jl:207-221): max of G
 // (optionally storing the diagonal D for the DSTORED iteration variants)
@@ -365,6 +368,7 @@ int ht3d_tune_set(int autotune, int force_tile, int force_variant) {
 
 int ht3d_tune_clear(void) {
     ht3d::g_tune.clear();
+    ht3d::v18::clear_tiles();
     return HT3D_OK;
 }
 

@@ -283,6 +283,7 @@ static std::map<std::tuple<int, int, int, int>, int> &tile_cache() {
     return m;
 }
 int g_v18_force_tile = -1;
+void clear_tiles() { tile_cache().clear(); }
 
 template <bool DARCY>
 static int choose_tile(const Args &base, double *scratch_x, double *scratch_f0, int *tile_out) {
