@@ -5,6 +5,7 @@
 // (ht3d_batch.hip).
 #pragma once
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <vector>
@@ -118,8 +119,7 @@ __device__ __forceinline__ void iter_tile(const IterArgs<T> &a, const T *__restr
         T xm = Xin[e - sE2];
         if (k0 == 0) xm = ghost(a.bct[4], a.bcv[4], xc);
         T kzB = a.kz[c];
-#pragma unroll 2
-        for (int k = k0; k < k1; ++k) {
+        auto plane = [&](int k) {
             T xp = Xin[e + sE2];
             T xW = Xin[e - 1], xE = Xin[e + 1];
             T xS = Xin[e - sE1], xN = Xin[e + sE1];
@@ -165,6 +165,15 @@ __device__ __forceinline__ void iter_tile(const IterArgs<T> &a, const T *__restr
             c += sC2;
             fx += sX2;
             fy += sY2;
+        };
+        // a full chunk of KZ planes is unrolled completely, so the loads of later
+        // planes can be issued ahead of earlier planes' arithmetic (about 1 %
+        // faster); the last, partial chunk of the z range keeps a rolled loop
+        if (k1 - k0 == KZ) {
+#pragma unroll
+            for (int q = 0; q < KZ; ++q) plane(k0 + q);
+        } else {
+            for (int k = k0; k < k1; ++k) plane(k);
         }
     }
     if (check) {
@@ -504,7 +513,25 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    const int reps = 8;
+    // The timed launches ping-pong the iterate between two buffers, as the
+    // solve does (what the MALL holds at the next launch depends on it): a
+    // scratch copy of the input and the solve's second buffer, so the solve's
+    // own X is never written.
+    const size_t ne = (size_t)(a.nx + 2) * (a.ny + 2) * (a.nz + 2);
+    T *xs[2] = {nullptr, a.Xout};
+    {
+        void *p = nullptr;
+        if (int rc = workspace(WS_TUNE_X, ne * sizeof(T), &p)) return rc;
+        xs[0] = (T *)p;
+        HIP_TRY(hipMemcpyAsync(xs[0], a.Xin, ne * sizeof(T), hipMemcpyDeviceToDevice, cx.stream));
+    }
+    int pp = 0;
+    auto flip = [&](IterArgs<T> &b) {
+        b.Xin = xs[pp];
+        b.Xout = xs[pp ^ 1];
+        pp ^= 1;
+    };
+    int reps = 8;
     auto timed = [&](auto &&launch, double *us) -> int {
         int rc = launch();
         if (rc) return rc;
@@ -518,21 +545,46 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
         *us = 1e3 * ms / reps;
         return HT3D_OK;
     };
+    // stage 1: every (tile, variant) once; stage 2: the fastest few again, in
+    // interleaved rounds with more launches each, keeping each one's best time
+    // (a single short timing is noisy at the 1 % level that separates the
+    // leading tiles)
+    struct Cand {
+        IterCfg cfg;
+        double us;
+    };
+    std::vector<Cand> cands;
+    auto time_cfg = [&](const IterCfg &cfg, double *us) -> int {
+        set_tiling(a, cfg.tile);
+        int rc = timed([&] {
+            if (g_alternate) a.flags ^= FL_REVERSE;
+            flip(a);
+            return launch_iter(a, transient, cfg);
+        }, us);
+        a.flags = 0;
+        return rc;
+    };
     for (int var = 0; var < 8; ++var) {
         if ((var & 1) && !can_store_d) continue;
         for (int tile = 0; tile < kNumTiles; ++tile) {
-            IterCfg cfg{tile, var};
-            set_tiling(a, tile);
-            double us = 0;
-            int rc = timed([&] {
-                if (g_alternate) a.flags ^= FL_REVERSE;
-                return launch_iter(a, transient, cfg);
-            }, &us);
-            a.flags = 0;
-            if (rc) return rc;
-            if (us < te.us) { te.us = us; te.cfg = cfg; }
+            Cand cd{IterCfg{tile, var}, 0.0};
+            if (int rc = time_cfg(cd.cfg, &cd.us)) return rc;
+            cands.push_back(cd);
         }
     }
+    std::sort(cands.begin(), cands.end(), [](const Cand &x, const Cand &y) { return x.us < y.us; });
+    const size_t nfin = cands.size() < 4 ? cands.size() : 4;
+    std::vector<double> fin(nfin, 1e300);
+    reps = 16;
+    for (int round = 0; round < 3; ++round)
+        for (size_t q = 0; q < nfin; ++q) {
+            double us = 0;
+            if (int rc = time_cfg(cands[q].cfg, &us)) return rc;
+            fin[q] = us < fin[q] ? us : fin[q];
+        }
+    reps = 8;
+    for (size_t q = 0; q < nfin; ++q)
+        if (fin[q] < te.us) { te.us = fin[q]; te.cfg = cands[q].cfg; }
     if (a.partials) {   // cost of a check: check launch + finish kernel + read-back, less a plain launch
         void *res = nullptr;
         int rc = workspace(WS_RESULT, 64 * sizeof(double), &res);
@@ -564,6 +616,7 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
                 double us = 0;
                 int rc = timed([&] {
                     if (g_alternate) b.flags ^= FL_REVERSE;
+                    flip(b);
                     return launch_pair(b, (const T *)a.F0, transient, pc);
                 }, &us);
                 if (rc) return rc;
