@@ -100,6 +100,18 @@ def run_gpu(w, steps: int, warmup: int, precision: int, prm):
     return wall, st, reports, kfv, fields
 
 
+def _cpu_model() -> str:
+    """The host CPU's model name (from /proc/cpuinfo), for the CPU baseline."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def pmc_traffic(workload: str, tile) -> float | None:
     """HBM bytes per launch of the single-iteration kernel for this tile and
     variant, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the
@@ -382,7 +394,8 @@ def main(argv=None) -> int:
                          "on the same inputs and report the DC speed-up and its error against FP64")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--check", default="50", help="check interval: an integer, or 'auto' for the check-interval tuner")
-    ap.add_argument("--cpu-iters", type=int, default=600, help="pseudo-iterations of the CPU-baseline sample")
+    ap.add_argument("--cpu-iters", type=int, default=3000,
+                    help="pseudo-iterations of the CPU-baseline sample (about 10 s on 16 host threads)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config0-steps", type=int, default=20,
                     help="physical steps of the 32^3 configuration run in full on CPU and GPU (0: skip)")
@@ -448,7 +461,7 @@ def main(argv=None) -> int:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         log(f"CPU baseline: oracle Darcy DYREL, {args.cpu_iters} iterations at full size, {threads} threads")
         cpu_teff, cpu_s = cpu_baseline(w, args.cpu_iters, kfv)
-        cpu = {"value": round(cpu_teff, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+        cpu = {"value": round(cpu_teff, 3), "unit": "GB/s", "cores": threads, "kind": "port", "cpu_model": _cpu_model(),
                "sample": f"{w.name}: CPU oracle (unfused reference kernel sequence, FP64) Darcy DYREL solve, "
                          f"{args.cpu_iters} pseudo-iterations at full size incl. one Gershgorin set-up pass, "
                          f"{cpu_s:.2f} s"}
@@ -481,6 +494,7 @@ def main(argv=None) -> int:
                      "pair_tile": None if st["pair"] is None else
                      f"{st['pair'][0]}x{st['pair'][1]} variant {st['pair'][2]}"},
         "cpu_baseline": cpu,
+        "device": _lib.device_name(),
     }
     if not args.no_cpu and args.config0_steps > 0:
         log(f"32^3 configuration in full on the CPU oracle and the GPU ({args.config0_steps} steps)")

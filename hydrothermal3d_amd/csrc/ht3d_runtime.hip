@@ -249,7 +249,9 @@ int ht3d_device_name(char *buf, size_t len) {
     hipDeviceProp_t p;
     HIP_TRY(hipGetDeviceProperties(&p, ctx().device));
     if (buf && len) {
-        std::snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+        // some runtimes leave the marketing name empty: the architecture names the part then
+        std::snprintf(buf, len, "%s (%s, %d CUs, %.1f GiB)", p.name[0] ? p.name : "AMD GPU", p.gcnArchName,
+                      p.multiProcessorCount, (double)p.totalGlobalMem / (1024.0 * 1024.0 * 1024.0));
     }
     return HT3D_OK;
 }

@@ -220,6 +220,31 @@ def test_reductions(ht, orc, op, n):
         assert abs(got - want) <= 1e-13 * max(1.0, np.abs(a).sum())
 
 
+@pytest.mark.parametrize("op, want", [(0, 0.0), (1, 0.0), (2, -np.inf), (3, np.inf), (4, -np.inf)],
+                         ids=["sum", "sum_abs", "max", "min", "max_abs"])
+def test_empty_reduction_is_the_identity(ht, op, want):
+    """An empty array reduces to the identity of the operation, with no launch."""
+    d = ht.DeviceArray.zeros((0,))
+    assert ht.device.reduce(d, op) == want
+
+
+@pytest.mark.parametrize("grid", [(2, 3, 3), (3, 2, 3), (3, 3, 2), (0, 5, 5)], ids=lambda g: "x".join(map(str, g)))
+def test_grid_below_three_cells_is_refused(ht, grid):
+    """Every kernel entry needs at least 3 cells per direction (the 3-layer WENO
+    mirror and the ghost rules): a smaller grid is refused by the C-ABI with
+    HT3D_ERR_ARG before any launch, and the library stays usable."""
+    sh = shapes(*grid)
+    q = [ht.DeviceArray.zeros(sh[k]) for k in ("Fx", "Fy", "Fz")]
+    k = [ht.DeviceArray.zeros(sh[k]) for k in ("Fx", "Fy", "Fz")]
+    A = ht.DeviceArray.zeros(sh["E"])
+    with pytest.raises(ht._lib.HT3DError) as err:
+        ht.kernels.ComputeFlux(*q, *k, A, 1.0, 1.0, 1.0)
+    assert err.value.code == 1
+    ok = shapes(3, 3, 3)
+    ht.kernels.ComputeFlux(*[ht.DeviceArray.zeros(ok[s]) for s in ("Fx", "Fy", "Fz", "Fx", "Fy", "Fz")],
+                           ht.DeviceArray.zeros(ok["E"]), 1.0, 1.0, 1.0)
+
+
 def test_shape_mismatch_raises(ht):
     sh = shapes(5, 6, 7)
     bad = ht.DeviceArray.zeros((7, 6, 7))
