@@ -88,9 +88,9 @@ def run_gpu(w, steps: int, warmup: int, precision: int, prm):
     _lib.call("ht3d_timing_enable", 0)
     st = timing_stats()
     import ctypes as C
-    ty, kz, var = C.c_int(), C.c_int(), C.c_int()
-    _lib.call("ht3d_tune_last", C.byref(ty), C.byref(kz), C.byref(var))
-    st["tile"] = (ty.value, kz.value, var.value)
+    ty, kz, xc, var = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    _lib.call("ht3d_tune_last_tile", C.byref(ty), C.byref(kz), C.byref(xc), C.byref(var))
+    st["tile"] = (ty.value, kz.value, var.value, xc.value)
     _lib.call("ht3d_tune_last_pair", C.byref(ty), C.byref(kz), C.byref(var))
     st["pair"] = (ty.value, kz.value, var.value) if ty.value > 0 else None
     kfv = g["kfv"].get()
@@ -112,6 +112,12 @@ def _cpu_model() -> str:
     return "unknown"
 
 
+def tile_label(tile) -> str:
+    """rows x z-planes of a single-iteration tile, with " x2" when each lane
+    holds two x-cells."""
+    return f"{tile[0]}x{tile[1]}" + (" x2" if len(tile) > 3 and tile[3] == 2 else "")
+
+
 def pmc_traffic(workload: str, tile) -> float | None:
     """HBM bytes per launch of the single-iteration kernel for this tile and
     variant, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the
@@ -120,7 +126,7 @@ def pmc_traffic(workload: str, tile) -> float | None:
     path = ROOT / "profiles" / f"dyrel_iter_{workload}_pmc.md"
     if not path.exists():
         return None
-    want = f"| single | {tile[0]}x{tile[1]} | {tile[2]} |"
+    want = f"| single | {tile_label(tile)} | {tile[2]} |"
     for line in path.read_text().splitlines():
         if line.startswith(want):
             return float(line.split("|")[9])
@@ -230,9 +236,9 @@ def probe(w, iters: int, copies: int) -> int:
         k.ArrayEqualArray(a, b)
     _lib.sync()
     import ctypes as C
-    ty, kz, var = C.c_int(), C.c_int(), C.c_int()
-    _lib.call("ht3d_tune_last", C.byref(ty), C.byref(kz), C.byref(var))
-    single = f"{ty.value}x{kz.value} variant {var.value}"
+    ty, kz, xc, var = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    _lib.call("ht3d_tune_last_tile", C.byref(ty), C.byref(kz), C.byref(xc), C.byref(var))
+    single = f"{tile_label((ty.value, kz.value, var.value, xc.value))} variant {var.value}"
     _lib.call("ht3d_tune_last_pair", C.byref(ty), C.byref(kz), C.byref(var))
     pair = f"{ty.value}x{kz.value} variant {var.value}" if ty.value > 0 else "none"
     log(f"probe: {iters} pseudo-iterations on {w.name} (single tile {single}, pair tile {pair}), "
@@ -490,7 +496,7 @@ def main(argv=None) -> int:
         "roofline": {"bound": "hbm", "achieved": round(teff, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(teff / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.workload, st["tile"]),
                      "algorithmic_bytes": 8 * 8 * w.ncell,
-                     "tile": f"{st['tile'][0]}x{st['tile'][1]} variant {st['tile'][2]}",
+                     "tile": f"{tile_label(st['tile'])} variant {st['tile'][2]}",
                      "pair_tile": None if st["pair"] is None else
                      f"{st['pair'][0]}x{st['pair'][1]} variant {st['pair'][2]}"},
         "cpu_baseline": cpu,

@@ -380,6 +380,12 @@ int ht3d_tune_last(int *rows_per_block, int *z_per_thread, int *variant) {
     return HT3D_OK;
 }
 
+int ht3d_tune_last_tile(int *rows_per_block, int *z_per_thread, int *x_per_thread, int *variant) {
+    const ht3d::IterCfg &c = ht3d::last_cfg();
+    if (x_per_thread) *x_per_thread = ht3d::kTiles[c.tile].cpl;
+    return ht3d_tune_last(rows_per_block, z_per_thread, variant);
+}
+
 int ht3d_tile_count(void) { return ht3d::kNumTiles; }
 
 int ht3d_tune_check_interval(double iters, double t_iter_us, double t_check_us, int nitout) {

@@ -182,12 +182,162 @@ __device__ __forceinline__ void iter_tile(const IterArgs<T> &a, const T *__restr
     }
 }
 
-template <typename T, bool TRANSIENT, bool DSTORED, bool NTX, bool KT, int TY, int KZ>
+// CPL consecutive x-cells as one vector (element alignment only: the E arrays'
+// rows start at odd offsets, and gfx950 loads and stores vectors at element
+// alignment).
+template <typename T, int CPL>
+struct CellVec {
+    typedef T type __attribute__((ext_vector_type(CPL), aligned(sizeof(T))));
+};
+
+template <typename T, int CPL>
+__device__ __forceinline__ typename CellVec<T, CPL>::type ldv(const T *p) {
+    return *reinterpret_cast<const typename CellVec<T, CPL>::type *>(p);
+}
+template <typename T, int CPL>
+__device__ __forceinline__ typename CellVec<T, CPL>::type ldv_nt(const T *p) {
+    return __builtin_nontemporal_load(reinterpret_cast<const typename CellVec<T, CPL>::type *>(p));
+}
+template <bool KT, typename T, int CPL>
+__device__ __forceinline__ typename CellVec<T, CPL>::type ldv_k(const T *p) {
+    if (KT) return ldv<T, CPL>(p);
+    return ldv_nt<T, CPL>(p);
+}
+template <typename T, int CPL>
+__device__ __forceinline__ void stv(typename CellVec<T, CPL>::type v, T *p) {
+    *reinterpret_cast<typename CellVec<T, CPL>::type *>(p) = v;
+}
+template <typename T, int CPL>
+__device__ __forceinline__ void stv_nt(typename CellVec<T, CPL>::type v, T *p) {
+    __builtin_nontemporal_store(v, reinterpret_cast<typename CellVec<T, CPL>::type *>(p));
+}
+
+// iter_tile with CPL (2 or 4) x-neighbouring cells per lane: a block covers
+// 64*CPL x TY cells.  The centre, lateral and z rows of X, the face
+// coefficients, R, F0 (and D, F) arrive as CPL-element vector loads, which
+// divides the load instructions per cell by about CPL; the west neighbour of
+// the first cell and the east neighbour of the last are single loads, and a
+// face between two cells of a lane is loaded once.  Each cell is computed with
+// iter_tile's operations in the same order, so X and F0 are bit-identical to
+// iter_tile; the check sums are the same double-double sums in another order.
+// Needs nx % CPL == 0 (a lane's cells never cross the end of a row).
+template <typename T, bool TRANSIENT, bool DSTORED, bool NTX, bool KT, int TY, int KZ, int CPL>
+__device__ __forceinline__ void iter_tile_v(const IterArgs<T> &a, const T *__restrict__ Xin, T *__restrict__ Xout,
+                                            unsigned flags, int bx, int by, int bz, double *__restrict__ partials,
+                                            unsigned slot) {
+    typedef typename CellVec<T, CPL>::type V;
+    const int i = bx * (64 * CPL) + CPL * (int)(threadIdx.x & 63u);
+    const int j = by * TY + (int)(threadIdx.x >> 6);
+    const int k0 = bz * KZ;
+    const int k1 = min(k0 + KZ, a.nz);
+    const uint32_t nx = a.nx, ny = a.ny;
+    const uint32_t sE1 = nx + 2, sE2 = sE1 * (ny + 2);
+    const uint32_t sC2 = nx * ny, sX1 = nx + 1, sX2 = sX1 * ny, sY2 = nx * (ny + 1);
+
+    double s_hi = 0.0, s_lo = 0.0, acc_max = 0.0, n_hi = 0.0, n_lo = 0.0, d_hi = 0.0, d_lo = 0.0;
+    const bool check = flags & FL_CHECK, lam = flags & FL_LAMBDA, store = flags & FL_STORE_F;
+
+    if ((uint32_t)i < nx && (uint32_t)j < ny) {
+        const bool wlo = i == 0, whi = (uint32_t)i + CPL == nx, slo = j == 0, shi = (uint32_t)j == ny - 1;
+        uint32_t e = (uint32_t)(i + 1) + (uint32_t)(j + 1) * sE1 + (uint32_t)(k0 + 1) * sE2;
+        uint32_t c = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k0 * sC2;
+        uint32_t fx = (uint32_t)i + (uint32_t)j * sX1 + (uint32_t)k0 * sX2;
+        uint32_t fy = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k0 * sY2;
+        V xc = ldv<T, CPL>(Xin + e);
+        V xm = ldv<T, CPL>(Xin + e - sE2);
+        if (k0 == 0) {
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) xm[q] = ghost(a.bct[4], a.bcv[4], (T)xc[q]);
+        }
+        V kzB = ldv_k<KT, T, CPL>(a.kz + c);
+        auto plane = [&](int k) {
+            V xp = ldv<T, CPL>(Xin + e + sE2);
+            T xW = Xin[e - 1], xE = Xin[e + CPL];
+            V xS = ldv<T, CPL>(Xin + e - sE1), xN = ldv<T, CPL>(Xin + e + sE1);
+            // x-faces: fx (west of the first cell), then fx+1 .. fx+CPL
+            const T kW = ld_k<KT>(a.kx + fx);
+            const V kE = ldv_k<KT, T, CPL>(a.kx + fx + 1);
+            const V kS = ldv_k<KT, T, CPL>(a.ky + fy), kN = ldv_k<KT, T, CPL>(a.ky + fy + nx);
+            const V kF = ldv_k<KT, T, CPL>(a.kz + c + sC2);
+            const V kB = kzB;
+            const V r = ldv_nt<T, CPL>(a.R + c);
+            const V f0o = ldv_nt<T, CPL>(a.F0 + c);
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                if (k + 1 == a.nz) xp[q] = ghost(a.bct[5], a.bcv[5], (T)xc[q]);
+                if (slo) xS[q] = ghost(a.bct[2], a.bcv[2], (T)xc[q]);
+                if (shi) xN[q] = ghost(a.bct[3], a.bcv[3], (T)xc[q]);
+            }
+            if (wlo) xW = ghost(a.bct[0], a.bcv[0], (T)xc[0]);
+            if (whi) xE = ghost(a.bct[1], a.bcv[1], (T)xc[CPL - 1]);
+            V f, D, g, xo;
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                const T w = q == 0 ? xW : (T)xc[q - 1], ea = q == CPL - 1 ? xE : (T)xc[q + 1];
+                const T kw = q == 0 ? kW : (T)kE[q - 1];
+                f[q] = cell_residual<T, TRANSIENT>(a, xc[q], w, ea, xS[q], xN[q], xm[q], xp[q], kw, kE[q], kS[q], kN[q],
+                                                   kB[q], kF[q], r[q]);
+                if (!DSTORED) D[q] = cell_diag(a, kw, (T)kE[q], (T)kS[q], (T)kN[q], (T)kB[q], (T)kF[q]);
+            }
+            if (DSTORED) D = ldv<T, CPL>(a.Dst + c);
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                f[q] = f[q] / D[q];
+                g[q] = a.h1 * f0o[q] + a.h2 * f[q];   // DYRELUpdate! (kernels_HT3D_dimensional.jl:323-324)
+                xo[q] = xc[q] - a.dtau * g[q];
+            }
+            stv_nt<T, CPL>(g, a.F0 + c);
+            if (NTX) stv_nt<T, CPL>(xo, Xout + e);
+            else stv<T, CPL>(xo, Xout + e);
+            if (check) {
+                V fst = xo;   // only read on lambda iterations
+                if (lam) fst = ldv<T, CPL>(a.Fst + c);
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) {
+                    const double fd = fabs((double)(f[q] * D[q]));
+                    dd_add(s_hi, s_lo, fd);
+                    acc_max = (fd > acc_max || fd != fd) ? fd : acc_max;
+                    if (lam) {   // RayleighQuotientNumerator! / Denominator! terms
+                        const double d = (double)(a.dtau * g[q]);
+                        dd_add(n_hi, n_lo, (-d) * ((double)f[q] - (double)fst[q]));
+                        dd_add(d_hi, d_lo, d * d);
+                    }
+                }
+            }
+            if (store) stv<T, CPL>(f, a.Fst + c);
+            xm = xc;
+            xc = xp;
+            kzB = kF;
+            e += sE2;
+            c += sC2;
+            fx += sX2;
+            fy += sY2;
+        };
+        if (k1 - k0 == KZ) {
+#pragma unroll
+            for (int q = 0; q < KZ; ++q) plane(k0 + q);
+        } else {
+            for (int k = k0; k < k1; ++k) plane(k);
+        }
+    }
+    if (check) {
+        const double h4[4] = {s_hi, acc_max, n_hi, d_hi}, l4[4] = {s_lo, 0.0, n_lo, d_lo};
+        block_reduce4(h4, l4, 0x2u, partials, slot);
+    }
+}
+
+// CPL: cells per lane along x (1: iter_tile; 2 or 4: iter_tile_v, nx % CPL == 0)
+template <typename T, bool TRANSIENT, bool DSTORED, bool NTX, bool KT, int TY, int KZ, int CPL>
 __global__ __launch_bounds__(64 * TY) void dyrel_iter_kernel(IterArgs<T> a) {
     const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
     const unsigned t = xcd_tile(blockIdx.x, nblocks, a.flags & FL_REVERSE);
     const int bx = t % a.nbx, by = (t / a.nbx) % a.nby, bz = t / (a.nbx * a.nby);
-    iter_tile<T, TRANSIENT, DSTORED, NTX, KT, TY, KZ>(a, a.Xin, a.Xout, a.flags, bx, by, bz, a.partials, blockIdx.x);
+    if constexpr (CPL > 1)
+        iter_tile_v<T, TRANSIENT, DSTORED, NTX, KT, TY, KZ, CPL>(a, a.Xin, a.Xout, a.flags, bx, by, bz, a.partials,
+                                                                 blockIdx.x);
+    else
+        iter_tile<T, TRANSIENT, DSTORED, NTX, KT, TY, KZ>(a, a.Xin, a.Xout, a.flags, bx, by, bz, a.partials,
+                                                          blockIdx.x);
 }
 
 // Two pseudo-iterations per launch, for runs of plain iterations (no check, no
@@ -309,15 +459,18 @@ __global__ __launch_bounds__(64 * TY) void dyrel_pair_kernel(IterArgs<T> a, cons
 // autotuner picks one per (grid, operator, precision, D mode).
 struct TileCfg {
     int ty, kz;
+    int cpl = 1;   // cells per lane along x
 };
 static constexpr TileCfg kTiles[] = {{2, 8}, {2, 16}, {4, 4}, {4, 8}, {4, 16}, {4, 32}, {8, 4}, {8, 8},
-                                      {8, 16}, {4, 2}, {8, 2}, {16, 2}, {16, 4}, {1, 8}};
+                                      {8, 16}, {4, 2}, {8, 2}, {16, 2}, {16, 4}, {1, 8},
+                                      {4, 4, 2}, {4, 8, 2}, {2, 8, 2}, {8, 4, 2}, {4, 16, 2}, {2, 16, 2},
+                                      {4, 4, 4}, {4, 8, 4}, {2, 8, 4}, {4, 16, 4}, {2, 16, 4}};
 static constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 static constexpr int kDefaultTile = 2;
 
-template <typename T, bool TR, bool DS, bool NX, bool KT, int TY, int KZ>
+template <typename T, bool TR, bool DS, bool NX, bool KT, int TY, int KZ, int CPL = 1>
 inline void launch_t(const IterArgs<T> &a, unsigned nb, hipStream_t st) {
-    hipLaunchKernelGGL((dyrel_iter_kernel<T, TR, DS, NX, KT, TY, KZ>), dim3(nb), dim3(64 * TY), 0, st, a);
+    hipLaunchKernelGGL((dyrel_iter_kernel<T, TR, DS, NX, KT, TY, KZ, CPL>), dim3(nb), dim3(64 * TY), 0, st, a);
 }
 
 template <typename T, bool TR, bool DS, bool NX, bool KT>
@@ -336,7 +489,18 @@ inline void launch_tile(int tile, const IterArgs<T> &a, unsigned nb, hipStream_t
     case 10: launch_t<T, TR, DS, NX, KT, 8, 2>(a, nb, st); break;
     case 11: launch_t<T, TR, DS, NX, KT, 16, 2>(a, nb, st); break;
     case 12: launch_t<T, TR, DS, NX, KT, 16, 4>(a, nb, st); break;
-    default: launch_t<T, TR, DS, NX, KT, 1, 8>(a, nb, st); break;
+    case 13: launch_t<T, TR, DS, NX, KT, 1, 8>(a, nb, st); break;
+    case 14: launch_t<T, TR, DS, NX, KT, 4, 4, 2>(a, nb, st); break;
+    case 15: launch_t<T, TR, DS, NX, KT, 4, 8, 2>(a, nb, st); break;
+    case 16: launch_t<T, TR, DS, NX, KT, 2, 8, 2>(a, nb, st); break;
+    case 17: launch_t<T, TR, DS, NX, KT, 8, 4, 2>(a, nb, st); break;
+    case 18: launch_t<T, TR, DS, NX, KT, 4, 16, 2>(a, nb, st); break;
+    case 19: launch_t<T, TR, DS, NX, KT, 2, 16, 2>(a, nb, st); break;
+    case 20: launch_t<T, TR, DS, NX, KT, 4, 4, 4>(a, nb, st); break;
+    case 21: launch_t<T, TR, DS, NX, KT, 4, 8, 4>(a, nb, st); break;
+    case 22: launch_t<T, TR, DS, NX, KT, 2, 8, 4>(a, nb, st); break;
+    case 23: launch_t<T, TR, DS, NX, KT, 4, 16, 4>(a, nb, st); break;
+    default: launch_t<T, TR, DS, NX, KT, 2, 16, 4>(a, nb, st); break;
     }
 }
 
@@ -348,9 +512,21 @@ struct IterCfg {
     int variant = 0;
 };
 
+// A tile with CPL cells per lane needs nx % CPL == 0; otherwise its twin
+// (same rows and planes) with the most cells per lane that nx allows is used.
+inline int tile_for_grid(int tile, int nx) {
+    if (nx % kTiles[tile].cpl == 0) return tile;
+    for (int cpl = kTiles[tile].cpl / 2; cpl >= 1; cpl /= 2) {
+        if (nx % cpl) continue;
+        for (int t = 0; t < kNumTiles; ++t)
+            if (kTiles[t].cpl == cpl && kTiles[t].ty == kTiles[tile].ty && kTiles[t].kz == kTiles[tile].kz) return t;
+    }
+    return kDefaultTile;
+}
+
 template <typename T>
 inline void set_tiling(IterArgs<T> &a, int tile) {
-    a.nbx = (int)div_up(a.nx, 64);
+    a.nbx = (int)div_up(a.nx, 64 * kTiles[tile].cpl);
     a.nby = (int)div_up(a.ny, kTiles[tile].ty);
     a.nbz = (int)div_up(a.nz, kTiles[tile].kz);
 }
@@ -567,6 +743,7 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
     for (int var = 0; var < 8; ++var) {
         if ((var & 1) && !can_store_d) continue;
         for (int tile = 0; tile < kNumTiles; ++tile) {
+            if (tile_for_grid(tile, a.nx) != tile) continue;
             Cand cd{IterCfg{tile, var}, 0.0};
             if (int rc = time_cfg(cd.cfg, &cd.us)) return rc;
             cands.push_back(cd);
@@ -643,7 +820,7 @@ inline int choose_cfg(IterArgs<T> &a, bool transient, bool can_store_d, int prec
     IterCfg c;
     PairCfg pc;
     if (g_force_tile >= 0) {
-        c.tile = g_force_tile;
+        c.tile = tile_for_grid(g_force_tile, a.nx);
         c.variant = g_force_variant & (can_store_d ? 7 : 6);
     } else if (g_autotune) {
         TuneEntry te;

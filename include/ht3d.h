@@ -83,18 +83,21 @@ int ht3d_timing_get(ht3d_timing_stats *out);
 int ht3d_timing_reset(void);
 
 /* Tile autotuner of the fused iteration kernel.  A configuration is (rows of 64
- * cells per block, z cells marched per thread) from a fixed table plus a
+ * lanes per block, z cells marched per thread, x cells per lane: 1, or 2 as
+ * one vector load, offered only for an even ncx) from a fixed table plus a
  * variant: bit 0 reads the stored Gershgorin diagonal instead of recomputing it
  * from the face coefficients (bit-identical either way), bit 1 writes the
  * updated unknown with a non-temporal store, bit 2 lets the face-coefficient
  * loads be cached.  With autotuning on (default) the
  * first solve of each (grid, operator, precision) times every configuration on
  * its own inputs and caches the fastest; force_tile >= 0 pins a table entry and
- * variant.  No reference counterpart (ParallelStencil takes explicit
+ * variant (a 2-cell entry on an odd ncx runs as its 1-cell twin).
+ * ht3d_tune_last_tile also reports the x cells per lane.  No reference counterpart (ParallelStencil takes explicit
  * cublocks/cuthreads, HydroThermal3D_v6.jl:264-265).                        */
 int ht3d_tune_set(int autotune, int force_tile, int force_variant);
 int ht3d_tune_clear(void);
 int ht3d_tune_last(int *rows_per_block, int *z_per_thread, int *variant);
+int ht3d_tune_last_tile(int *rows_per_block, int *z_per_thread, int *x_per_thread, int *variant);
 int ht3d_tile_count(void);
 /* Two pseudo-iterations per launch for runs of plain iterations (overlapped
  * tiles, the intermediate iterate kept on chip; bit-identical to two single
