@@ -91,8 +91,8 @@ def run_gpu(w, steps: int, warmup: int, precision: int, prm):
     ty, kz, xc, var = C.c_int(), C.c_int(), C.c_int(), C.c_int()
     _lib.call("ht3d_tune_last_tile", C.byref(ty), C.byref(kz), C.byref(xc), C.byref(var))
     st["tile"] = (ty.value, kz.value, var.value, xc.value)
-    _lib.call("ht3d_tune_last_pair", C.byref(ty), C.byref(kz), C.byref(var))
-    st["pair"] = (ty.value, kz.value, var.value) if ty.value > 0 else None
+    _lib.call("ht3d_tune_last_pair_tile", C.byref(ty), C.byref(kz), C.byref(xc), C.byref(var))
+    st["pair"] = (ty.value, kz.value, var.value, xc.value) if ty.value > 0 else None
     kfv = g["kfv"].get()
     I = (slice(1, -1),) * 3
     fields = {"T": g["Tc_ex"].get()[I], "P": g["Pc_ex"].get()[I]}
@@ -113,9 +113,9 @@ def _cpu_model() -> str:
 
 
 def tile_label(tile) -> str:
-    """rows x z-planes of a single-iteration tile, with " x2" when each lane
-    holds two x-cells."""
-    return f"{tile[0]}x{tile[1]}" + (" x2" if len(tile) > 3 and tile[3] == 2 else "")
+    """rows x z-planes of an iteration tile (rows, z, variant[, x cells per
+    lane]), with " x2" / " x4" when each lane holds that many x-cells."""
+    return f"{tile[0]}x{tile[1]}" + (f" x{tile[3]}" if len(tile) > 3 and tile[3] > 1 else "")
 
 
 def pmc_traffic(workload: str, tile) -> float | None:
@@ -239,8 +239,8 @@ def probe(w, iters: int, copies: int) -> int:
     ty, kz, xc, var = C.c_int(), C.c_int(), C.c_int(), C.c_int()
     _lib.call("ht3d_tune_last_tile", C.byref(ty), C.byref(kz), C.byref(xc), C.byref(var))
     single = f"{tile_label((ty.value, kz.value, var.value, xc.value))} variant {var.value}"
-    _lib.call("ht3d_tune_last_pair", C.byref(ty), C.byref(kz), C.byref(var))
-    pair = f"{ty.value}x{kz.value} variant {var.value}" if ty.value > 0 else "none"
+    _lib.call("ht3d_tune_last_pair_tile", C.byref(ty), C.byref(kz), C.byref(xc), C.byref(var))
+    pair = f"{tile_label((ty.value, kz.value, var.value, xc.value))} variant {var.value}" if ty.value > 0 else "none"
     log(f"probe: {iters} pseudo-iterations on {w.name} (single tile {single}, pair tile {pair}), "
         f"{copies} copies of {8 * w.ncell} bytes")
     g.close()
@@ -498,7 +498,7 @@ def main(argv=None) -> int:
                      "algorithmic_bytes": 8 * 8 * w.ncell,
                      "tile": f"{tile_label(st['tile'])} variant {st['tile'][2]}",
                      "pair_tile": None if st["pair"] is None else
-                     f"{st['pair'][0]}x{st['pair'][1]} variant {st['pair'][2]}"},
+                     f"{tile_label(st['pair'])} variant {st['pair'][2]}"},
         "cpu_baseline": cpu,
         "device": _lib.device_name(),
     }

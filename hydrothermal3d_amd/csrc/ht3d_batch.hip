@@ -513,6 +513,7 @@ static void fill_tables(ht3d_batch *b, int kind, const std::vector<ht3d_dyrel_fi
         a.nbx = (int)div_up(nx, 64); a.nby = (int)div_up(ny, kBTiles[tile64].ty); a.nbz = (int)div_up(nz, kBTiles[tile64].kz);
         a._dx = 1.0 / d.dx; a._dy = 1.0 / d.dy; a._dz = 1.0 / d.dz; a._dt = 1.0 / d.dt;
         a.dx = d.dx; a.dy = d.dy; a.dz = d.dz; a.rc_tr_dt = rc_tr_dt;
+        set_inv_h2(a, d.dx, d.dy, d.dz);
         for (int k = 0; k < 6; ++k) { a.bct[k] = d.bc_type[k]; a.bcv[k] = d.bc_val[k]; }
 
         const size_t nfx = (size_t)(nx + 1) * ny * nz, nfy = (size_t)nx * (ny + 1) * nz;
@@ -526,6 +527,7 @@ static void fill_tables(ht3d_batch *b, int kind, const std::vector<ht3d_dyrel_fi
         g.nbx = (int)div_up(nx, 64); g.nby = (int)div_up(ny, kBTiles[tile32].ty); g.nbz = (int)div_up(nz, kBTiles[tile32].kz);
         g._dx = (float)a._dx; g._dy = (float)a._dy; g._dz = (float)a._dz; g._dt = (float)a._dt;
         g.dx = (float)d.dx; g.dy = (float)d.dy; g.dz = (float)d.dz; g.rc_tr_dt = (float)rc_tr_dt;
+        set_inv_h2(g, d.dx, d.dy, d.dz);
         for (int k = 0; k < 6; ++k) { g.bct[k] = d.bc_type[k]; g.bcv[k] = 0.0f; }  // homogeneous for the correction
 
         DcArgs &c = b->h_dc[kind][q];

@@ -24,17 +24,21 @@ int gershgorin_lmax(const double *kx, const double *ky, const double *kz, int nx
 int bc_fill_table(double *A, int nx, int ny, int nz, const int type[6], const double val[6]);
 int nu_sum(const double *T, const double *ky, int nx, int ny, int nz, double _dy, double *sum);
 
-// FP64 residual r(x) with on-the-fly ghosts; slots: 0 sum|r| (dd), 1 max|r|
+// FP64 residual r(x) with on-the-fly ghosts; slots: 0 sum|r| (dd), 1 max|r|.
+// A thread marches kResZ planes, so a launch leaves few block partials.
+constexpr int kResZ = 16;
 template <bool TR>
 __global__ __launch_bounds__(256) void dc_residual_kernel(DcArgs a) {
     const int i = blockIdx.x * 64 + (int)(threadIdx.x & 63u);
     const int j = blockIdx.y * 4 + (int)(threadIdx.x >> 6);
-    const int k = blockIdx.z;
+    const int k0 = blockIdx.z * kResZ, k1 = min(k0 + kResZ, a.nz);
     double s_hi = 0.0, s_lo = 0.0, mx = 0.0;
     if (i < a.nx && j < a.ny) {
-        const double af = fabs(dc_residual_cell<TR>(a, i, j, k));
-        dd_add(s_hi, s_lo, af);
-        mx = (af > mx || af != af) ? af : mx;
+        for (int k = k0; k < k1; ++k) {
+            const double af = fabs(dc_residual_cell<TR>(a, i, j, k));
+            dd_add(s_hi, s_lo, af);
+            mx = (af > mx || af != af) ? af : mx;
+        }
     }
     const double h4[4] = {s_hi, mx, 0.0, 0.0}, l4[4] = {s_lo, 0.0, 0.0, 0.0};
     const unsigned bl = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
@@ -91,7 +95,7 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     if ((rc = workspace(WS_F32_F0B, N * sizeof(float), &f0b))) return rc;
     if ((rc = workspace(WS_F32_DIAG, N * sizeof(float), &d32))) return rc;
     const unsigned max_blocks = div_up(nx, 64) * div_up(ny, 1) * div_up(nz, 2);
-    const unsigned res_blocks = div_up(nx, 64) * div_up(ny, 4) * nz;
+    const unsigned res_blocks = div_up(nx, 64) * div_up(ny, 4) * div_up(nz, kResZ);
     if ((rc = workspace(WS_PARTIALS, (size_t)(max_blocks > res_blocks ? max_blocks : res_blocks) * 8 * sizeof(double),
                         &part)))
         return rc;
@@ -114,7 +118,8 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     d.partials = (double *)part; d.nx = nx; d.ny = ny; d.nz = nz;
     d._dx = 1.0 / f->dx; d._dy = 1.0 / f->dy; d._dz = 1.0 / f->dz; d._dt = 1.0 / f->dt;
     for (int q = 0; q < 6; ++q) { d.bct[q] = f->bc_type[q]; d.bcv[q] = f->bc_val[q]; }
-    const dim3 rgrid(div_up(nx, 64), div_up(ny, 4), nz);
+    const dim3 rgrid(div_up(nx, 64), div_up(ny, 4), div_up(nz, kResZ));   // residual: kResZ planes per block
+    const dim3 ugrid(div_up(nx, 64), div_up(ny, 4), nz);                   // update: one plane per block
 
     IterArgs<float> a;
     std::memset(&a, 0, sizeof a);
@@ -123,6 +128,7 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     a.F0 = (float *)f0; a.Fst = (float *)fst; a.partials = (double *)part;
     a._dx = (float)d._dx; a._dy = (float)d._dy; a._dz = (float)d._dz; a._dt = (float)d._dt;
     a.dx = (float)f->dx; a.dy = (float)f->dy; a.dz = (float)f->dz; a.rc_tr_dt = (float)rc_tr_dt;
+    set_inv_h2(a, f->dx, f->dy, f->dz);
     a.dtau = (float)dtau;
     for (int q = 0; q < 6; ++q) { a.bct[q] = f->bc_type[q]; a.bcv[q] = 0.0f; }   // homogeneous for the correction
     a.Xin = (const float *)e0; a.Xout = (float *)e1;
@@ -242,7 +248,7 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
         }
         timer.collect(bytes32, true);
         // x_anchor += e; the next sweeps start from e = 0
-        hipLaunchKernelGGL(dc_update_kernel, rgrid, dim3(256), 0, cx.stream, f->X, buf[cur], nx, ny, nz);
+        hipLaunchKernelGGL(dc_update_kernel, ugrid, dim3(256), 0, cx.stream, f->X, buf[cur], nx, ny, nz);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemsetAsync(buf[cur], 0, NE * sizeof(float), cx.stream));
     }

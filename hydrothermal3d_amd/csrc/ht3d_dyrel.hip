@@ -101,6 +101,8 @@ std::vector<TuneEntry> g_tune;
 bool g_autotune = true;
 int g_force_tile = -1, g_force_variant = 0;
 int g_pairs = 1;
+int g_force_pair_tile = -1;
+int g_force_pair_variant = 0;
 int g_alternate = 1;
 
 IterCfg &last_cfg() {
@@ -229,6 +231,7 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
     a.F0 = (double *)f0; a.Fst = (double *)fst; a.partials = (double *)part;
     a._dx = 1.0 / f->dx; a._dy = 1.0 / f->dy; a._dz = 1.0 / f->dz; a._dt = 1.0 / f->dt;
     a.dx = f->dx; a.dy = f->dy; a.dz = f->dz; a.rc_tr_dt = rc_tr_dt;
+    set_inv_h2(a, f->dx, f->dy, f->dz);
     a.dtau = dtau; a.h1 = h1; a.h2 = h2;
     for (int q = 0; q < 6; ++q) { a.bct[q] = f->bc_type[q]; a.bcv[q] = f->bc_val[q]; }
     a.Xin = f->X;
@@ -411,5 +414,21 @@ int ht3d_tune_last_pair(int *rows_per_block, int *z_per_block, int *variant) {
     if (variant) *variant = c.tile >= 0 ? c.variant : -1;
     return HT3D_OK;
 }
+
+int ht3d_tune_last_pair_tile(int *rows_per_block, int *z_per_block, int *x_per_thread, int *variant) {
+    const ht3d::PairCfg &c = ht3d::last_pair();
+    if (x_per_thread) *x_per_thread = c.tile >= 0 ? ht3d::kPairTiles[c.tile].cpl : 0;
+    return ht3d_tune_last_pair(rows_per_block, z_per_block, variant);
+}
+
+int ht3d_tune_force_pair(int tile, int variant) {
+    if (tile >= ht3d::kNumPairTiles || variant < 0 || variant > 3)
+        return ht3d::arg_fail("ht3d_tune_force_pair: no such pair tile or variant");
+    ht3d::g_force_pair_tile = tile;
+    ht3d::g_force_pair_variant = variant;
+    return HT3D_OK;
+}
+
+int ht3d_pair_tile_count(void) { return ht3d::kNumPairTiles; }
 
 }  // extern "C"

@@ -38,6 +38,7 @@ struct IterArgs {
     int nbx, nby, nbz;           // tile grid
     T _dx, _dy, _dz, _dt;
     T dx, dy, dz, rc_tr_dt;
+    T idx2, idy2, idz2;          // 1/dx^2, 1/dy^2, 1/dz^2 (the FP32 diagonal)
     T h1, h2, dtau;
     int bct[6];
     T bcv[6];
@@ -71,11 +72,25 @@ __device__ __forceinline__ T cell_residual(const IterArgs<T> &a, T xc, T xW, T x
     return r + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
 }
 
-// GershgorinPoisson! diagonal D of one cell (kernels_HT3D_dimensional.jl:207-221)
+// GershgorinPoisson! diagonal D of one cell (kernels_HT3D_dimensional.jl:207-221).
+// FP64 keeps the reference's divisions (bit parity with the oracle); the FP32
+// sweeps of the defect correction, which have no reference counterpart, use
+// products with 1/h^2, which leaves them bound by memory instead of by nine
+// FP32 divisions per cell.
 template <typename T>
 __device__ __forceinline__ T cell_diag(const IterArgs<T> &a, T kW, T kE, T kS, T kN, T kB, T kF) {
+    if constexpr (sizeof(T) == 4)
+        return (kB + kF) * a.idz2 + (kN + kS) * a.idy2 + (kE + kW) * a.idx2 + a.rc_tr_dt;
     return (kB / a.dz + kF / a.dz) / a.dz + (kN / a.dy + kS / a.dy) / a.dy + (kE / a.dx + kW / a.dx) / a.dx +
            a.rc_tr_dt;
+}
+
+// the 1/h^2 factors of the FP32 diagonal, rounded once from FP64
+template <typename T>
+inline void set_inv_h2(IterArgs<T> &a, double dx, double dy, double dz) {
+    a.idx2 = (T)(1.0 / (dx * dx));
+    a.idy2 = (T)(1.0 / (dy * dy));
+    a.idz2 = (T)(1.0 / (dz * dz));
 }
 
 // face coefficients: cached (KT) or streamed past the caches
@@ -454,6 +469,137 @@ __global__ __launch_bounds__(64 * TY) void dyrel_pair_kernel(IterArgs<T> a, cons
     }
 }
 
+// dyrel_pair_kernel with CPL (2 or 4) x-cells per lane.  The overlap is one
+// whole lane on each side: a block's lanes hold the cells
+// [CPL*(bx*62-1), +64*CPL) of TY rows, lanes 1..62 of rows 1..TY-2 compute
+// iteration n+2, so every lane's first cell is a multiple of CPL and its cells
+// lie all inside or all outside the grid (nx % CPL == 0).  A lane's cells go
+// through the LDS plane as one vector; x neighbours inside a lane stay in
+// registers.  Same operations in the same order as the single iteration: X
+// and F0 are bit-identical to two single launches.
+template <typename T, bool TRANSIENT, bool DSTORED, bool KT, int TY, int KZ, int CPL>
+__global__ __launch_bounds__(64 * TY) void dyrel_pair_kernel_v(IterArgs<T> a, const T *__restrict__ F0in) {
+    typedef typename CellVec<T, CPL>::type V;
+    __shared__ T L[3][TY][64 * CPL];
+    const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
+    const unsigned t = xcd_tile(blockIdx.x, nblocks, a.flags & FL_REVERSE);
+    const int bx = t % a.nbx, by = (t / a.nbx) % a.nby, bz = t / (a.nbx * a.nby);
+    const int lx = (int)(threadIdx.x & 63u), ly = (int)(threadIdx.x >> 6);
+    const int i = CPL * (bx * 62 - 1 + lx), j = by * (TY - 2) - 1 + ly;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int k0 = bz * KZ, k1 = min(k0 + KZ, nz);
+    const bool in_dom = i >= 0 && i < nx && j >= 0 && j < ny;
+    const bool inner = in_dom && lx > 0 && lx < 63 && ly > 0 && ly < TY - 1;   // computes iteration n+2
+    const int ic = min(max(i, 0), nx - CPL), jc = min(max(j, 0), ny - 1);    // every load in bounds
+    const uint32_t sE1 = nx + 2, sE2 = sE1 * (ny + 2);
+    const uint32_t sC2 = (uint32_t)nx * ny, sX1 = nx + 1, sX2 = sX1 * ny, sY2 = (uint32_t)nx * (ny + 1);
+    const bool wlo = ic == 0, whi = ic + CPL == nx, slo = jc == 0, shi = jc == ny - 1;
+    const int ps = max(k0 - 1, 0), pe = min(k1, nz - 1);   // planes of iteration n+1
+    uint32_t e = (uint32_t)(ic + 1) + (uint32_t)(jc + 1) * sE1 + (uint32_t)(ps + 1) * sE2;
+    uint32_t c = (uint32_t)ic + (uint32_t)jc * nx + (uint32_t)ps * sC2;
+    uint32_t fx = (uint32_t)ic + (uint32_t)jc * sX1 + (uint32_t)ps * sX2;
+    uint32_t fy = (uint32_t)ic + (uint32_t)jc * nx + (uint32_t)ps * sY2;
+    V x0c = ldv<T, CPL>(a.Xin + e);
+    V x0m = ldv<T, CPL>(a.Xin + e - sE2);
+    if (ps == 0) {
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) x0m[q] = ghost(a.bct[4], a.bcv[4], (T)x0c[q]);
+    }
+    V kzB = ldv_k<KT, T, CPL>(a.kz + c);
+    // plane p-1 as iteration n+1 left it: the iterate, its neighbours in z, and
+    // the coefficients, R, D and F0 the n+2 update needs
+    V x1m = {}, x1c = {}, pkE = {}, pkS = {}, pkN = {}, pkB = {}, pkF = {}, pr = {}, pD = {}, pf0 = {};
+    T pkW = 0;
+    uint32_t pe_idx = 0, pc = 0;
+    // iteration n+2 on plane p-1 of this lane (yF: the n+1 iterate of plane p)
+    auto second = [&](int p, const V &yF) {
+        const int qb = (p - 1) % 3;
+        T yW = L[qb][ly][CPL * lx - 1], yE = L[qb][ly][CPL * lx + CPL];
+        V yS = *reinterpret_cast<const V *>(&L[qb][ly - 1][CPL * lx]);
+        V yN = *reinterpret_cast<const V *>(&L[qb][ly + 1][CPL * lx]);
+        V yB = x1m;
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) {
+            if (p - 1 == 0) yB[q] = ghost(a.bct[4], a.bcv[4], (T)x1c[q]);
+            if (slo) yS[q] = ghost(a.bct[2], a.bcv[2], (T)x1c[q]);
+            if (shi) yN[q] = ghost(a.bct[3], a.bcv[3], (T)x1c[q]);
+        }
+        if (wlo) yW = ghost(a.bct[0], a.bcv[0], (T)x1c[0]);
+        if (whi) yE = ghost(a.bct[1], a.bcv[1], (T)x1c[CPL - 1]);
+        V g0, xo;
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) {
+            const T w = q == 0 ? yW : (T)x1c[q - 1], ea = q == CPL - 1 ? yE : (T)x1c[q + 1];
+            const T kw = q == 0 ? pkW : (T)pkE[q - 1];
+            T g = cell_residual<T, TRANSIENT>(a, x1c[q], w, ea, yS[q], yN[q], yB[q], yF[q], kw, pkE[q], pkS[q], pkN[q],
+                                              pkB[q], pkF[q], pr[q]);
+            g = g / pD[q];
+            g0[q] = a.h1 * pf0[q] + a.h2 * g;
+            xo[q] = x1c[q] - a.dtau * g0[q];
+        }
+        stv_nt<T, CPL>(g0, a.F0 + pc);
+        stv<T, CPL>(xo, a.Xout + pe_idx);
+    };
+    for (int p = ps; p <= pe + 1; ++p) {
+        if (p <= pe) {   // iteration n+1 on plane p
+            V xp = ldv<T, CPL>(a.Xin + e + sE2);
+            T xW = a.Xin[e - 1], xE = a.Xin[e + CPL];
+            V xS = ldv<T, CPL>(a.Xin + e - sE1), xN = ldv<T, CPL>(a.Xin + e + sE1);
+            const T kW = ld_k<KT>(a.kx + fx);
+            const V kE = ldv_k<KT, T, CPL>(a.kx + fx + 1);
+            const V kS = ldv_k<KT, T, CPL>(a.ky + fy), kN = ldv_k<KT, T, CPL>(a.ky + fy + nx);
+            const V kF = ldv_k<KT, T, CPL>(a.kz + c + sC2);
+            const V kB = kzB;
+            const V r = ldv_nt<T, CPL>(a.R + c);
+            const V f0o = ldv_nt<T, CPL>(F0in + c);
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                if (p + 1 == nz) xp[q] = ghost(a.bct[5], a.bcv[5], (T)x0c[q]);
+                if (slo) xS[q] = ghost(a.bct[2], a.bcv[2], (T)x0c[q]);
+                if (shi) xN[q] = ghost(a.bct[3], a.bcv[3], (T)x0c[q]);
+            }
+            if (wlo) xW = ghost(a.bct[0], a.bcv[0], (T)x0c[0]);
+            if (whi) xE = ghost(a.bct[1], a.bcv[1], (T)x0c[CPL - 1]);
+            V D, f0, x1p;
+            if (DSTORED) D = ldv_nt<T, CPL>(a.Dst + c);
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                const T w = q == 0 ? xW : (T)x0c[q - 1], ea = q == CPL - 1 ? xE : (T)x0c[q + 1];
+                const T kw = q == 0 ? kW : (T)kE[q - 1];
+                T f = cell_residual<T, TRANSIENT>(a, x0c[q], w, ea, xS[q], xN[q], x0m[q], xp[q], kw, kE[q], kS[q], kN[q],
+                                                  kB[q], kF[q], r[q]);
+                if (!DSTORED) D[q] = cell_diag(a, kw, (T)kE[q], (T)kS[q], (T)kN[q], (T)kB[q], (T)kF[q]);
+                f = f / D[q];
+                f0[q] = a.h1 * f0o[q] + a.h2 * f;
+                x1p[q] = x0c[q] - a.dtau * f0[q];
+            }
+            *reinterpret_cast<V *>(&L[p % 3][ly][CPL * lx]) = x1p;
+            x0m = x0c;
+            x0c = xp;
+            kzB = kF;
+            __syncthreads();
+            if (p - 1 >= k0 && inner) second(p, x1p);
+            x1m = x1c;
+            x1c = x1p;
+            pkW = kW; pkE = kE; pkS = kS; pkN = kN; pkB = kB; pkF = kF; pr = r; pD = D; pf0 = f0;
+            pe_idx = e;
+            pc = c;
+            e += sE2;
+            c += sC2;
+            fx += sX2;
+            fy += sY2;
+        } else {         // past the top plane: iteration n+2 on plane nz-1 of the top chunk
+            __syncthreads();
+            if (p - 1 >= k0 && p - 1 < k1 && inner) {
+                V yF;
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) yF[q] = ghost(a.bct[5], a.bcv[5], (T)x1c[q]);
+                second(p, yF);
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------------ host driver
 // Tile configurations (rows of 64 cells per block, z cells per thread); the
 // autotuner picks one per (grid, operator, precision, D mode).
@@ -559,7 +705,9 @@ inline int launch_iter(const IterArgs<T> &a, bool transient, const IterCfg &cfg)
 // --------------------------------------------------------- two-iteration launches
 // Pair tiles (rows of the overlapped block, planes per block); variant bits:
 // 1 = stored diagonal, 2 = cached face-coefficient loads.  tile < 0: no pairs.
-static constexpr TileCfg kPairTiles[] = {{8, 16}, {8, 32}, {16, 16}, {16, 32}, {16, 64}};
+static constexpr TileCfg kPairTiles[] = {{8, 16}, {8, 32}, {16, 16}, {16, 32}, {16, 64},
+                                          {8, 16, 2}, {8, 32, 2}, {16, 32, 2},
+                                          {8, 16, 4}, {8, 32, 4}, {16, 32, 4}};
 static constexpr int kNumPairTiles = sizeof(kPairTiles) / sizeof(kPairTiles[0]);
 
 struct PairCfg {
@@ -567,9 +715,19 @@ struct PairCfg {
     int variant = 0;
 };
 
+// Whether a pair tile can run on a grid of x extent nx: CPL cells per lane
+// need nx % CPL == 0; in FP64 the vector tiles need more registers than a
+// 1024-thread block has (4 cells per lane, or 2 with 16 rows) and are not used.
+template <typename T>
+inline bool pair_tile_ok(int tile, int nx) {
+    const int cpl = kPairTiles[tile].cpl;
+    if (sizeof(T) == 8 && (cpl == 4 || (cpl == 2 && kPairTiles[tile].ty > 8))) return false;
+    return nx % cpl == 0;
+}
+
 template <typename T>
 inline void set_pair_tiling(IterArgs<T> &a, int tile) {
-    a.nbx = (int)div_up(a.nx, 62);
+    a.nbx = (int)div_up(a.nx, 62 * kPairTiles[tile].cpl);
     a.nby = (int)div_up(a.ny, kPairTiles[tile].ty - 2);
     a.nbz = (int)div_up(a.nz, kPairTiles[tile].kz);
 }
@@ -577,14 +735,29 @@ inline void set_pair_tiling(IterArgs<T> &a, int tile) {
 template <typename T, bool TR, bool DS, bool KT>
 inline void launch_pair_tile(int tile, const IterArgs<T> &a, const T *F0in, unsigned nb, hipStream_t st) {
 #define PL_(TY, KZ) hipLaunchKernelGGL((dyrel_pair_kernel<T, TR, DS, KT, TY, KZ>), dim3(nb), dim3(64 * TY), 0, st, a, F0in)
+#define PV_(TY, KZ, CPL) \
+    hipLaunchKernelGGL((dyrel_pair_kernel_v<T, TR, DS, KT, TY, KZ, CPL>), dim3(nb), dim3(64 * TY), 0, st, a, F0in)
     switch (tile) {
     case 0: PL_(8, 16); break;
     case 1: PL_(8, 32); break;
     case 2: PL_(16, 16); break;
     case 3: PL_(16, 32); break;
-    default: PL_(16, 64); break;
+    case 4: PL_(16, 64); break;
+    case 5: PV_(8, 16, 2); break;
+    case 6: PV_(8, 32, 2); break;
+    case 7:
+        if constexpr (sizeof(T) == 4) PV_(16, 32, 2);   // pair_tile_ok keeps FP64 off it
+        break;
+    default:
+        if constexpr (sizeof(T) == 4) {   // pair_tile_ok keeps FP64 off these
+            if (tile == 8) PV_(8, 16, 4);
+            else if (tile == 9) PV_(8, 32, 4);
+            else PV_(16, 32, 4);
+        }
+        break;
     }
 #undef PL_
+#undef PV_
 }
 
 // a: pair tiling set, a.F0 = the F0 buffer written, F0in = the one read
@@ -639,6 +812,8 @@ struct TuneEntry {
 extern std::vector<TuneEntry> g_tune;
 extern bool g_autotune;
 extern int g_force_tile, g_force_variant;
+extern int g_force_pair_tile;      // >= 0: pin this pair tile (no pairs where it cannot run)
+extern int g_force_pair_variant;
 extern int g_pairs;     // 0: never, 1: when the tuner finds them faster, 2: always (best pair tile)
 
 inline TuneEntry *tune_find(const TuneKey &k) {
@@ -786,6 +961,7 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
         for (int var = 0; var < 4; ++var) {
             if ((var & 1) && !can_store_d) continue;
             for (int tile = 0; tile < kNumPairTiles; ++tile) {
+                if (!pair_tile_ok<T>(tile, a.nx)) continue;
                 PairCfg pc{tile, var};
                 IterArgs<T> b = a;
                 b.F0 = F0b;
@@ -830,6 +1006,11 @@ inline int choose_cfg(IterArgs<T> &a, bool transient, bool can_store_d, int prec
         if (!can_store_d) c.variant &= ~1;
         if (te.pair.tile >= 0 && (g_pairs == 2 || (g_pairs == 1 && te.pair_us < te.us))) pc = te.pair;
         if (!can_store_d) pc.variant &= ~1;
+    }
+    if (g_force_pair_tile >= 0) {   // pinned: that tile, or no pairs where it cannot run
+        const bool ok = pair_tile_ok<T>(g_force_pair_tile, a.nx) && a.ny >= 3;
+        pc.tile = ok ? g_force_pair_tile : -1;
+        pc.variant = g_force_pair_variant & (can_store_d ? 3 : 2);
     }
     if (!F0b || g_pairs == 0) pc.tile = -1;
     set_tiling(a, c.tile);

@@ -103,13 +103,25 @@ __global__ __launch_bounds__(1024) void finish_kernel(const double *__restrict__
         h[q] = (ops >> q & 1u) ? -INFINITY : 0.0;
         l[q] = 0.0;
     }
-    for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) {
+    // four blocks' partials per thread and pass are loaded before they are
+    // combined, so their loads overlap instead of waiting one after another
+    const double2 *p2 = reinterpret_cast<const double2 *>(partials);
+    for (unsigned b0 = threadIdx.x; b0 < nblocks; b0 += 4 * blockDim.x) {
+        double2 v[4][4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const double v = partials[(size_t)b * 8 + 2 * q];
-            if (ops >> q & 1u) h[q] = (v > h[q] || v != v) ? v : h[q];
-            else dd_combine(h[q], l[q], v, partials[(size_t)b * 8 + 2 * q + 1]);
+        for (int u = 0; u < 4; ++u) {
+            const unsigned b = b0 + u * blockDim.x;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                v[u][q] = b < nblocks ? p2[(size_t)b * 4 + q] : make_double2((ops >> q & 1u) ? -INFINITY : 0.0, 0.0);
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (ops >> q & 1u) h[q] = (v[u][q].x > h[q] || v[u][q].x != v[u][q].x) ? v[u][q].x : h[q];
+                else dd_combine(h[q], l[q], v[u][q].x, v[u][q].y);
+            }
     }
     __shared__ double part[8];
     block_reduce4(h, l, ops, part, 0);

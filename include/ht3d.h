@@ -111,6 +111,17 @@ int ht3d_tune_pairs(int mode);
  * argument <= 0) the largest divisor of nitout up to 50.  Host only.          */
 int ht3d_tune_check_interval(double iters, double t_iter_us, double t_check_us, int nitout);
 int ht3d_tune_last_pair(int *rows_per_block, int *z_per_block, int *variant);
+/* The pair tile table also has tiles with 2 or 4 x-cells per lane (for an ncx
+ * divisible by that number; FP32 only for 4).  ht3d_tune_last_pair_tile also
+ * reports the cells per lane (0 when no pair tile was used).
+ * ht3d_tune_force_pair pins pair tile `tile` (< 0: none) and variant (bit 0
+ * stored diagonal, bit 1 cached coefficients) for the plain runs of every
+ * solve, whatever the timings; a solve whose grid the tile does not fit runs
+ * without pairs, and pairs mode 0 still turns pairs off.  No reference
+ * counterpart.                                                               */
+int ht3d_tune_last_pair_tile(int *rows_per_block, int *z_per_block, int *x_per_thread, int *variant);
+int ht3d_tune_force_pair(int tile, int variant);
+int ht3d_pair_tile_count(void);
 /* Sweep direction of the fused iteration kernels: with alternate = 1 (default)
  * every other launch walks each XCD's slab of tiles backwards, starting on the
  * planes the previous launch touched last (still in the 256 MB MALL).  Results

@@ -70,3 +70,44 @@ def test_sweep_direction_does_not_change_results(ht, precision):
     for key in ("iters", "status", "n_checks", "n_lambda", "lmin", "h1", "h2", "nF_abs"):
         assert r0[key] == r1[key], key
     assert np.array_equal(X0, X1)
+
+
+PAIR_TILES = {0: 1, 1: 1, 2: 1, 3: 1, 4: 1, 5: 2, 6: 2, 7: 2, 8: 4, 9: 4, 10: 4}   # pair tile: x cells per lane
+
+
+def _pair_tile_runs(tile, nx, precision):
+    """Whether pair tile `tile` runs on this grid: nx divisible by its cells per
+    lane; in FP64 no 4-cell tile and no 2-cell tile with 16 rows."""
+    cpl = PAIR_TILES[tile]
+    if precision == 0 and (cpl == 4 or tile == 7):
+        return False
+    return nx % cpl == 0
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32dc"])
+@pytest.mark.parametrize("transient", [0, 1], ids=["darcy", "heat"])
+@pytest.mark.parametrize("grid", [(70, 37, 45), (16, 9, 70), (260, 7, 9), (133, 20, 19)],
+                         ids=lambda g: "x".join(map(str, g)))
+def test_every_pair_tile_bitwise(ht, grid, transient, precision):
+    """Each pair tile pinned in turn (ht3d_tune_force_pair), cycling through the
+    variants: bit-identical to pairs off, or not used when the grid does not
+    allow it (the report then shows no pair tile)."""
+    import ctypes as C
+    r0, X0, rows0 = _solve(ht, grid, transient, precision, 0)
+    assert rows0 == 0
+    try:
+        for tile, cpl in PAIR_TILES.items():
+            variant = tile % 4
+            ht._lib.call("ht3d_tune_force_pair", tile, variant)
+            r, X, rows = _solve(ht, grid, transient, precision, 1)
+            ty, kz, xc, var = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+            ht._lib.call("ht3d_tune_last_pair_tile", C.byref(ty), C.byref(kz), C.byref(xc), C.byref(var))
+            if _pair_tile_runs(tile, grid[0], precision):
+                assert (xc.value, var.value) == (cpl, variant), (tile, xc.value, var.value)
+            else:
+                assert rows == 0 and xc.value == 0, (tile, rows, xc.value)
+            for key in ("iters", "status", "n_checks", "n_lambda", "lmin", "h1", "h2", "nF_abs"):
+                assert r0[key] == r[key], (tile, key)
+            assert np.array_equal(X0, X), tile
+    finally:
+        ht._lib.call("ht3d_tune_force_pair", -1, 0)
