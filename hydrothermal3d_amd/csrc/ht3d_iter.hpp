@@ -915,10 +915,17 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
         a.flags = 0;
         return rc;
     };
+    // Grids under 2^20 cells are launch-bound and solved in milliseconds: there
+    // the tuner tries a short list (4x4, 8x8, 2x8, their 2- and 4-cell forms,
+    // variants 0 and 6) instead of the whole table.
+    const bool small = (size_t)a.nx * a.ny * a.nz < ((size_t)1 << 20);
+    auto short_tile = [](int tile) { return tile == 2 || tile == 7 || tile == 0 || tile == 14 || tile == 16 || tile == 20; };
     for (int var = 0; var < 8; ++var) {
         if ((var & 1) && !can_store_d) continue;
+        if (small && var != 0 && var != 6) continue;
         for (int tile = 0; tile < kNumTiles; ++tile) {
             if (tile_for_grid(tile, a.nx) != tile) continue;
+            if (small && !short_tile(tile)) continue;
             Cand cd{IterCfg{tile, var}, 0.0};
             if (int rc = time_cfg(cd.cfg, &cd.us)) return rc;
             cands.push_back(cd);
@@ -962,6 +969,7 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
             if ((var & 1) && !can_store_d) continue;
             for (int tile = 0; tile < kNumPairTiles; ++tile) {
                 if (!pair_tile_ok<T>(tile, a.nx)) continue;
+                if (small && tile != 0 && tile != 5) continue;
                 PairCfg pc{tile, var};
                 IterArgs<T> b = a;
                 b.F0 = F0b;
