@@ -218,3 +218,41 @@ def test_batch_tuned_check_interval(ht):
     b.close()
     for m in ms:
         m.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32dc"])
+def test_precision_workload_128_copy(ht, orc, precision):
+    """The 512x256x512 precision workload (Ra=1000, lognormal k + a dipping
+    fault) on a 128x64x128 copy, two physical steps, against the CPU oracle in
+    FP64: the FP64 path within the field tolerance with iteration counts within
+    one; the FP32 sweeps + FP64 defect correction within 1e-6 relative (T, P
+    gauge-fixed, Nu), a DC convergence test falling at most one anchor interval
+    from the FP64 one."""
+    from hydrothermal3d_amd import _lib, inputs
+    w = inputs.WORKLOADS["hrl512_fault"].scaled(128, 64, 128)
+    p = ht.DyrelParams(nitmax=40000, nitout=100, ncheck=50, norm=1, tol=1e-9)
+    T0 = inputs.initial_temperature(w)
+    g = ht.HydroThermal3D(w.ncx, w.ncy, w.ncz, w.Lx, w.Ly, w.Lz, w.Ra, w.cfl)
+    g["Tc_ex"].set(T0)
+    inputs.permeability_device(w, g["kfv"])
+    g.set_params(p, p, 1, _lib.FP64 if precision == 0 else _lib.FP32_DC)
+    o = orc.HRLModel(w.ncx, w.ncy, w.ncz, w.Lx, w.Ly, w.Lz, w.Ra, w.cfl)
+    o.field("Tc_ex")[...] = T0
+    o.field("kfv")[...] = g["kfv"].get()
+    po = orc.params(p.nitmax, p.nitout, p.ncheck, p.norm, p.tol)
+    o.set_params(po, po, 1)
+    slack = 1 if precision == 0 else p.ncheck
+    tol = TOL_FIELD if precision == 0 else 1e-6
+    try:
+        for s in range(2):
+            print(f"precision workload copy: step {s} precision {precision}", flush=True)
+            rg, rc = g.step(), o.step()
+            for kind in ("thermal", "darcy"):
+                assert abs(rg[kind]["iters"] - rc[kind]["iters"]) <= slack, (s, kind, rg[kind], rc[kind])
+            assert abs(rg["Nu"] - rc["Nu"]) <= tol * abs(rc["Nu"]), (s, rg["Nu"], rc["Nu"])
+        I = (slice(1, -1),) * 3
+        assert rel_err(g["Tc_ex"].get()[I], o.field("Tc_ex")[I]) <= tol
+        assert rel_err(_gauge(g["Pc_ex"].get()), _gauge(o.field("Pc_ex"))) <= tol
+    finally:
+        g.close()
+        o.close()
