@@ -133,12 +133,75 @@ __global__ __launch_bounds__(256) void copy_special_kernel(dim::BcArgs a, double
     }
 }
 
-template <bool DARCY, int TY, int KZ>
+// One equation cell of the fused v18 iteration: the rules for its neighbours
+// (immersed-surface neighbours re-evaluated, walls and the basal heat flux
+// applied), its residual F and its diagonal D (returned in D); returns F / D.
+// t1: heat T of the previous step / Darcy temperature of the cell; t2: heat
+// rho*C / Darcy phi*rho_f of the previous step; Dv: Darcy stored diagonal;
+// gyS, gyN: Darcy gravity terms on the cell's y faces.  The one- and two-cell
+// kernels both call it, so they evaluate the same operations in the same order.
+template <bool DARCY>
+__device__ __forceinline__ double v18_cell(const Args &a, uint8_t cl, int i, int j, int k, double xc, double w,
+                                           double ea, double s, double n, double b, double fr, double kW, double kE,
+                                           double kS, double kN, double kB, double kF, double t1, double t2,
+                                           double Dv, double gyS, double gyN, double &D) {
+    if (cl & C_SLOW) {   // next to an immersed-surface cell: every neighbour from the rules
+        const int ie = i + 1, je = j + 1, ke = k + 1;
+        w = dim::bc_value(a.bc, ie - 1, je, ke);
+        ea = dim::bc_value(a.bc, ie + 1, je, ke);
+        s = dim::bc_value(a.bc, ie, je - 1, ke);
+        n = dim::bc_value(a.bc, ie, je + 1, ke);
+        b = dim::bc_value(a.bc, ie, je, ke - 1);
+        fr = dim::bc_value(a.bc, ie, je, ke + 1);
+    } else {             // walls: copies of the cell, basal heat flux
+        if (i == 0) w = xc;
+        if (i == a.nx - 1) ea = xc;
+        if (k == 0) b = xc;
+        if (k == a.nz - 1) fr = xc;
+        if (j == 0) s = DARCY ? xc : a.qyS_dy / kS + xc;
+    }
+    const double qW = -kW * (a._dx * (xc - w)), qE = -kE * (a._dx * (ea - xc));
+    const double qB = -kB * (a._dz * (xc - b)), qF = -kF * (a._dz * (fr - xc));
+    double f;
+    if (DARCY) {
+        // phi*rho_f of the cell (kernels_HT3D_dimensional.jl:225-235), then
+        // ComputeDarcyFlux! and ResidualFluidPressure! (:266-289)
+        const double phi = (cl & C_FAULT) ? 3.0 * a.phi_i : a.phi_i;
+        const double prf = phi * dim::rhof_C(t1 * a.s_T - 273.15, xc * a.s_sigma) / a.s_rho;
+        const double qS = -kS * (a._dy * (xc - s) - gyS);
+        const double qN = -kN * (a._dy * (n - xc) - gyN);
+        f = a.tr_dt * (prf - t2);
+        f += (qE - qW) * a._dx;
+        f += (qN - qS) * a._dy;
+        f += (qF - qB) * a._dz;
+        D = Dv;
+    } else {
+        // ComputeFlux! and ResidualTemperatureLinearised! (:140-181)
+        const double rc = t2;
+        const double qS = -kS * (a._dy * (xc - s)), qN = -kN * (a._dy * (n - xc));
+        f = a.transient * rc * a._dt * (xc - t1);
+        f += (qE - qW) * a._dx;
+        f += (qN - qS) * a._dy;
+        f += (qF - qB) * a._dz;
+        f -= a.transient * ((cl & C_SRC) ? a.Qt : 0.0);
+        D = (kB / a.dz + kF / a.dz) / a.dz + (kN / a.dy + kS / a.dy) / a.dy + (kE / a.dx + kW / a.dx) / a.dx +
+            rc * a.transient / a.dt;
+    }
+    return f / D;
+}
+
+// The fused v18 pseudo-iteration with CPL (1 or 2) x-cells per lane: a block
+// covers 64*CPL x TY cells marched over KZ planes.  With CPL = 2 every stream
+// comes in as 2-element vector loads (nx even); the loads of the neighbours,
+// coefficients and operator streams are skipped when all the lane's cells are
+// air or immersed (F = 0).
+template <bool DARCY, int TY, int KZ, int CPL>
 __global__ __launch_bounds__(64 * TY) void iter_kernel(Args a) {
+    typedef typename CellVec<double, CPL>::type V;
     const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
     const unsigned t = xcd_tile(blockIdx.x, nblocks, a.flags & FL_REVERSE);
     const int bx = t % a.nbx, by = (t / a.nbx) % a.nby, bz = t / (a.nbx * a.nby);
-    const int i = bx * 64 + (int)(threadIdx.x & 63u);
+    const int i = bx * (64 * CPL) + CPL * (int)(threadIdx.x & 63u);
     const int j = by * TY + (int)(threadIdx.x >> 6);
     const int k0 = bz * KZ;
     const int k1 = min(k0 + KZ, a.nz);
@@ -153,80 +216,82 @@ __global__ __launch_bounds__(64 * TY) void iter_kernel(Args a) {
         uint32_t c = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k0 * sC2;
         uint32_t fx = (uint32_t)i + (uint32_t)j * sX1 + (uint32_t)k0 * sX2;
         uint32_t fy = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k0 * sY2;
-        double xc = a.Xin[e];
-        double xm = a.Xin[e - sE2];
-        double kzB = a.kz[c];
+        V xc = ldv<double, CPL>(a.Xin + e);
+        V xm = ldv<double, CPL>(a.Xin + e - sE2);
+        V kzB = ldv<double, CPL>(a.kz + c);
         for (int k = k0; k < k1; ++k) {
-            const double xp = a.Xin[e + sE2];
-            const double kF = a.kz[c + sC2];
-            const double kB = kzB;
-            const uint8_t cl = __builtin_nontemporal_load(a.cls + c);
-            const double f0o = __builtin_nontemporal_load(a.F0 + c);
-            double f = 0.0, D = 1.0;
-            if (!(cl & C_ZERO)) {   // air and immersed cells: F = 0, nothing else to load
-                const double kW = a.kx[fx], kE = a.kx[fx + 1];
-                const double kS = a.ky[fy], kN = a.ky[fy + nx];
-                double w = a.Xin[e - 1], ea = a.Xin[e + 1], s = a.Xin[e - sE1], n = a.Xin[e + sE1];
-                double b = xm, fr = xp;
-                if (cl & C_SLOW) {   // next to an immersed-surface cell: every neighbour from the rules
-                    const int ie = i + 1, je = j + 1, ke = k + 1;
-                    w = dim::bc_value(a.bc, ie - 1, je, ke);
-                    ea = dim::bc_value(a.bc, ie + 1, je, ke);
-                    s = dim::bc_value(a.bc, ie, je - 1, ke);
-                    n = dim::bc_value(a.bc, ie, je + 1, ke);
-                    b = dim::bc_value(a.bc, ie, je, ke - 1);
-                    fr = dim::bc_value(a.bc, ie, je, ke + 1);
-                } else {             // walls: copies of the cell, basal heat flux
-                    if (i == 0) w = xc;
-                    if ((uint32_t)i == nx - 1) ea = xc;
-                    if (k == 0) b = xc;
-                    if (k == a.nz - 1) fr = xc;
-                    if (j == 0) s = DARCY ? xc : a.qyS_dy / kS + xc;
-                }
-                const double qW = -kW * (a._dx * (xc - w)), qE = -kE * (a._dx * (ea - xc));
-                const double qB = -kB * (a._dz * (xc - b)), qF = -kF * (a._dz * (fr - xc));
+            const V xp = ldv<double, CPL>(a.Xin + e + sE2);
+            const V kF = ldv<double, CPL>(a.kz + c + sC2);
+            uint8_t cl[CPL];
+            if constexpr (CPL == 1) {
+                cl[0] = __builtin_nontemporal_load(a.cls + c);
+            } else {   // two class bytes in one load (c is even: nx is even)
+                const uint16_t c2 = __builtin_nontemporal_load(reinterpret_cast<const uint16_t *>(a.cls + c));
+                cl[0] = (uint8_t)(c2 & 0xffu);
+                cl[CPL - 1] = (uint8_t)(c2 >> 8);
+            }
+            const V f0o = ldv_nt<double, CPL>(a.F0 + c);
+            V f, D;
+            uint8_t all = C_ZERO;
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                f[q] = 0.0;
+                D[q] = 1.0;
+                all &= cl[q];
+            }
+            if (!(all & C_ZERO)) {   // air and immersed cells: F = 0, nothing else to load
+                const double kW = a.kx[fx];
+                const V kE = ldv<double, CPL>(a.kx + fx + 1);
+                const V kS = ldv<double, CPL>(a.ky + fy), kN = ldv<double, CPL>(a.ky + fy + nx);
+                const double xW = a.Xin[e - 1], xE = a.Xin[e + CPL];
+                const V xS = ldv<double, CPL>(a.Xin + e - sE1), xN = ldv<double, CPL>(a.Xin + e + sE1);
+                V t1, t2, Dv = {}, gS = {}, gN = {};
                 if (DARCY) {
-                    // ϕρf of the cell (kernels_HT3D_dimensional.jl:225-235), then
-                    // ComputeDarcyFlux! and ResidualFluidPressure! (:266-289)
-                    const double phi = (cl & C_FAULT) ? 3.0 * a.phi_i : a.phi_i;
-                    const double Tk = __builtin_nontemporal_load(a.aux + e);
-                    const double prf = phi * dim::rhof_C(Tk * a.s_T - 273.15, xc * a.s_sigma) / a.s_rho;
-                    const double qS = -kS * (a._dy * (xc - s) - a.gy[fy]);
-                    const double qN = -kN * (a._dy * (n - xc) - a.gy[fy + nx]);
-                    f = a.tr_dt * (prf - __builtin_nontemporal_load(a.aux2 + c));
-                    f += (qE - qW) * a._dx;
-                    f += (qN - qS) * a._dy;
-                    f += (qF - qB) * a._dz;
-                    D = __builtin_nontemporal_load(a.Dst + c);
+                    t1 = ldv_nt<double, CPL>(a.aux + e);
+                    t2 = ldv_nt<double, CPL>(a.aux2 + c);
+                    Dv = ldv_nt<double, CPL>(a.Dst + c);
+                    gS = ldv<double, CPL>(a.gy + fy);
+                    gN = ldv<double, CPL>(a.gy + fy + nx);
                 } else {
-                    // ComputeFlux! and ResidualTemperatureLinearised! (:140-181)
-                    const double rc = __builtin_nontemporal_load(a.aux2 + c);
-                    const double qS = -kS * (a._dy * (xc - s)), qN = -kN * (a._dy * (n - xc));
-                    f = a.transient * rc * a._dt * (xc - __builtin_nontemporal_load(a.aux + c));
-                    f += (qE - qW) * a._dx;
-                    f += (qN - qS) * a._dy;
-                    f += (qF - qB) * a._dz;
-                    f -= a.transient * ((cl & C_SRC) ? a.Qt : 0.0);
-                    D = (kB / a.dz + kF / a.dz) / a.dz + (kN / a.dy + kS / a.dy) / a.dy +
-                        (kE / a.dx + kW / a.dx) / a.dx + rc * a.transient / a.dt;
+                    t1 = ldv_nt<double, CPL>(a.aux + c);
+                    t2 = ldv_nt<double, CPL>(a.aux2 + c);
                 }
-                f /= D;
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) {
+                    if (cl[q] & C_ZERO) continue;
+                    const double w = q == 0 ? xW : (double)xc[q - 1], ea = q == CPL - 1 ? xE : (double)xc[q + 1];
+                    const double kw = q == 0 ? kW : (double)kE[q - 1];
+                    double d;
+                    f[q] = v18_cell<DARCY>(a, cl[q], i + q, j, k, xc[q], w, ea, xS[q], xN[q], xm[q], xp[q], kw, kE[q],
+                                           kS[q], kN[q], kzB[q], kF[q], t1[q], t2[q], Dv[q], gS[q], gN[q], d);
+                    D[q] = d;
+                }
             }
             // DYRELUpdate! (kernels_HT3D_dimensional.jl:323-324)
-            const double f0 = a.h1 * f0o + a.h2 * f;
-            __builtin_nontemporal_store(f0, a.F0 + c);
-            __builtin_nontemporal_store(xc - a.dtau * f0, a.Xout + e);
+            V g, xo;
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                g[q] = a.h1 * f0o[q] + a.h2 * f[q];
+                xo[q] = xc[q] - a.dtau * g[q];
+            }
+            stv_nt<double, CPL>(g, a.F0 + c);
+            stv_nt<double, CPL>(xo, a.Xout + e);
             if (check) {
-                const double fd = fabs(f * D);
-                dd_add(s_hi, s_lo, fd);
-                acc_max = (fd > acc_max || fd != fd) ? fd : acc_max;
-                if (lam) {
-                    const double d = a.dtau * f0;
-                    dd_add(n_hi, n_lo, (-d) * (f - a.Fst[c]));
-                    dd_add(d_hi, d_lo, d * d);
+                V fst = xo;   // only read on lambda iterations
+                if (lam) fst = ldv<double, CPL>(a.Fst + c);
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) {
+                    const double fd = fabs(f[q] * D[q]);
+                    dd_add(s_hi, s_lo, fd);
+                    acc_max = (fd > acc_max || fd != fd) ? fd : acc_max;
+                    if (lam) {
+                        const double d = a.dtau * g[q];
+                        dd_add(n_hi, n_lo, (-d) * (f[q] - fst[q]));
+                        dd_add(d_hi, d_lo, d * d);
+                    }
                 }
             }
-            if (store) a.Fst[c] = f;
+            if (store) stv<double, CPL>(f, a.Fst + c);
             xm = xc;
             xc = xp;
             kzB = kF;
@@ -244,32 +309,47 @@ __global__ __launch_bounds__(64 * TY) void iter_kernel(Args a) {
 
 struct Tile {
     int ty, kz;
+    int cpl = 1;   // x-cells per lane (2: nx even only)
 };
-static constexpr Tile kV18Tiles[] = {{4, 4}, {4, 8}, {8, 4}, {2, 8}, {8, 8}, {4, 16}};
+static constexpr Tile kV18Tiles[] = {{4, 4}, {4, 8}, {8, 4}, {2, 8}, {8, 8}, {4, 16},
+                                     {4, 4, 2}, {4, 8, 2}, {2, 8, 2}, {4, 16, 2}};
 static constexpr int kNumV18Tiles = sizeof(kV18Tiles) / sizeof(kV18Tiles[0]);
 
 template <bool DARCY>
 static void launch(int tile, Args &a, hipStream_t st) {
     const Tile tl = kV18Tiles[tile];
-    a.nbx = (int)div_up(a.nx, 64);
+    a.nbx = (int)div_up(a.nx, 64 * tl.cpl);
     a.nby = (int)div_up(a.ny, tl.ty);
     a.nbz = (int)div_up(a.nz, tl.kz);
     const unsigned nb = (unsigned)a.nbx * a.nby * a.nbz;
     a.bc.A = a.Xin;
     switch (tile) {
-    case 0: hipLaunchKernelGGL((iter_kernel<DARCY, 4, 4>), dim3(nb), dim3(256), 0, st, a); break;
-    case 1: hipLaunchKernelGGL((iter_kernel<DARCY, 4, 8>), dim3(nb), dim3(256), 0, st, a); break;
-    case 2: hipLaunchKernelGGL((iter_kernel<DARCY, 8, 4>), dim3(nb), dim3(512), 0, st, a); break;
-    case 3: hipLaunchKernelGGL((iter_kernel<DARCY, 2, 8>), dim3(nb), dim3(128), 0, st, a); break;
-    case 4: hipLaunchKernelGGL((iter_kernel<DARCY, 8, 8>), dim3(nb), dim3(512), 0, st, a); break;
-    default: hipLaunchKernelGGL((iter_kernel<DARCY, 4, 16>), dim3(nb), dim3(256), 0, st, a); break;
+    case 0: hipLaunchKernelGGL((iter_kernel<DARCY, 4, 4, 1>), dim3(nb), dim3(256), 0, st, a); break;
+    case 1: hipLaunchKernelGGL((iter_kernel<DARCY, 4, 8, 1>), dim3(nb), dim3(256), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((iter_kernel<DARCY, 8, 4, 1>), dim3(nb), dim3(512), 0, st, a); break;
+    case 3: hipLaunchKernelGGL((iter_kernel<DARCY, 2, 8, 1>), dim3(nb), dim3(128), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((iter_kernel<DARCY, 8, 8, 1>), dim3(nb), dim3(512), 0, st, a); break;
+    case 5: hipLaunchKernelGGL((iter_kernel<DARCY, 4, 16, 1>), dim3(nb), dim3(256), 0, st, a); break;
+    case 6: hipLaunchKernelGGL((iter_kernel<DARCY, 4, 4, 2>), dim3(nb), dim3(256), 0, st, a); break;
+    case 7: hipLaunchKernelGGL((iter_kernel<DARCY, 4, 8, 2>), dim3(nb), dim3(256), 0, st, a); break;
+    case 8: hipLaunchKernelGGL((iter_kernel<DARCY, 2, 8, 2>), dim3(nb), dim3(128), 0, st, a); break;
+    default: hipLaunchKernelGGL((iter_kernel<DARCY, 4, 16, 2>), dim3(nb), dim3(256), 0, st, a); break;
     }
+}
+
+// A 2-cell tile needs an even nx; otherwise its 1-cell twin is used.
+static int tile_for_grid(int tile, int nx) {
+    if (kV18Tiles[tile].cpl == 1 || nx % 2 == 0) return tile;
+    for (int t = 0; t < kNumV18Tiles; ++t)
+        if (kV18Tiles[t].cpl == 1 && kV18Tiles[t].ty == kV18Tiles[tile].ty && kV18Tiles[t].kz == kV18Tiles[tile].kz)
+            return t;
+    return 0;
 }
 
 static unsigned max_blocks(int nx, int ny, int nz) {
     unsigned m = 0;
     for (const Tile &t : kV18Tiles) {
-        const unsigned nb = div_up(nx, 64) * div_up(ny, t.ty) * div_up(nz, t.kz);
+        const unsigned nb = div_up(nx, 64 * t.cpl) * div_up(ny, t.ty) * div_up(nz, t.kz);
         m = nb > m ? nb : m;
     }
     return m;
@@ -287,7 +367,10 @@ void clear_tiles() { tile_cache().clear(); }
 
 template <bool DARCY>
 static int choose_tile(const Args &base, double *scratch_x, double *scratch_f0, int *tile_out) {
-    if (g_v18_force_tile >= 0 && g_v18_force_tile < kNumV18Tiles) { *tile_out = g_v18_force_tile; return HT3D_OK; }
+    if (g_v18_force_tile >= 0 && g_v18_force_tile < kNumV18Tiles) {
+        *tile_out = tile_for_grid(g_v18_force_tile, base.nx);
+        return HT3D_OK;
+    }
     const auto key = std::make_tuple(base.nx, base.ny, base.nz, DARCY ? 1 : 0);
     auto it = tile_cache().find(key);
     if (it != tile_cache().end()) { *tile_out = it->second; return HT3D_OK; }
@@ -310,6 +393,7 @@ static int choose_tile(const Args &base, double *scratch_x, double *scratch_f0, 
     for (int tile = 0; tile < kNumV18Tiles; ++tile) tbest[tile] = 1e30f;
     for (int round = 0; round < 3; ++round)
         for (int tile = 0; tile < kNumV18Tiles; ++tile) {
+            if (tile_for_grid(tile, base.nx) != tile) continue;
             Args a = base;
             a.F0 = scratch_f0;
             a.flags = 0;
@@ -539,6 +623,8 @@ int ht3d_v18_thermal_solve(const ht3d_v18_field *f, const ht3d_dyrel_params *p, 
 int ht3d_v18_darcy_solve(const ht3d_v18_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep) {
     return ht3d::v18::solve<true>(f, p, rep);
 }
+
+int ht3d_v18_tile_count(void) { return ht3d::v18::kNumV18Tiles; }
 
 int ht3d_v18_tune(int force_tile, int *last_tile) {
     HT3D_CHECK_INIT();

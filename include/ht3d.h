@@ -411,10 +411,14 @@ typedef struct {
 } ht3d_v18_field;
 int ht3d_v18_thermal_solve(const ht3d_v18_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep);
 int ht3d_v18_darcy_solve(const ht3d_v18_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep);
-/* tile of the v18 iteration kernel: force_tile >= 0 forces one of the six
- * candidates, -1 lets the autotuner choose; *last_tile (may be NULL) returns
- * the tile of the last solve.  Analogue of cublocks/cuthreads.              */
+/* tile of the v18 iteration kernel: force_tile >= 0 forces one of the
+ * ht3d_v18_tile_count() candidates (6 with one x-cell per lane, then 4 with
+ * two, which need an even ncx and otherwise run as their one-cell twin), -1
+ * lets the autotuner choose; *last_tile (may be NULL) returns the tile of the
+ * last solve.  Every tile gives bit-identical results.  Analogue of
+ * cublocks/cuthreads.                                                        */
 int ht3d_v18_tune(int force_tile, int *last_tile);
+int ht3d_v18_tile_count(void);
 
 /* FP32 versions of the inner-sweep kernels for a defect-correction driver (the
  * fused ht3d_dyrel_solve_dc runs the same arithmetic).  Same argument order as
