@@ -1,6 +1,6 @@
 """Second, independent restatement of the hot path in NumPy array slicing.
 
-TEST INFRASTRUCTURE ONLY (the checker of the checker).  Written from the
+TEST INFRASTRUCTURE ONLY (it verifies the C oracle).  Written from the
 reference's Julia kernels independently of ht3d_oracle.c: whole-array slices
 with 0-based offsets instead of 1-based point loops.  Every expression keeps the
 reference's left-to-right evaluation order and NumPy never fuses multiply-adds,

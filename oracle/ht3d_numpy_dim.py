@@ -1,6 +1,6 @@
 """Second restatement of the dimensional (v18) property kernels, in NumPy.
 
-TEST INFRASTRUCTURE ONLY (the checker of the checker), written from the
+TEST INFRASTRUCTURE ONLY (it verifies the C oracle), written from the
 reference independently of ht3d_oracle_dim.c: whole-array slices with 0-based
 offsets, and Dekker's exact product (Veltkamp splitting) where the C oracle
 uses fma.  Same left-to-right evaluation order, so results equal the C oracle

@@ -1,7 +1,7 @@
 /*
  * ht3d_oracle.c -- CPU oracle for the HydroThermal3D solver hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP library
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle that the HIP library
  * (hydrothermal3d_amd/libht3d.so) is compared against.  Only tests/,
  * __graft_entry__.smoke() and bench.py's CPU-baseline leg may load it; the
  * product path never does.

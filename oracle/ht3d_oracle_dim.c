@@ -1,8 +1,8 @@
 /*
  * ht3d_oracle_dim.c -- CPU oracle for the dimensional (v18) property kernels.
  *
- * TEST INFRASTRUCTURE ONLY, like ht3d_oracle.c: the checker for the HIP
- * library's dimensional entries.  The product path never loads it.
+ * TEST INFRASTRUCTURE ONLY, like ht3d_oracle.c: the oracle the HIP library's
+ * dimensional entries are compared with.  The product path never loads it.
  *
  * Restates, one loop nest per reference kernel and in FP64:
  *   Dimensional/HydroThermal3D_v18.jl:29-39           (EOS and property closures)

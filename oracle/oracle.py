@@ -1,7 +1,7 @@
 """Python binding of the CPU oracle (oracle/libht3d_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
-CPU-baseline leg of bench.py as the checker.  The product path never imports
+CPU-baseline leg of bench.py, to compare results with.  The product path never imports
 this module.
 
 Functions carry the reference kernel names (without "!") and argument order and

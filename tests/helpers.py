@@ -153,7 +153,7 @@ def v18_problem(nx, ny, nz, seed=0, immersed=False, Qt=None):
 class V18OracleModel:
     """The v18 time step of HydroThermal3D_v18.jl:398-571 with the CPU oracle's
     kernels and unfused DYREL loops, on the same host state and in the same
-    order as hydrothermal3d_amd.v18.V18Model (the checker of that driver)."""
+    order as hydrothermal3d_amd.v18.V18Model (the oracle for that driver)."""
 
     def __init__(self, nx, ny, nz, geo, pT, pP, ulp_seed=None):
         """ulp_seed: if given, every permeability from ComputeHydroConductivity
