@@ -915,14 +915,16 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
         a.flags = 0;
         return rc;
     };
-    // Grids under 2^20 cells are launch-bound and solved in milliseconds: there
-    // the tuner tries a short list (4x4, 8x8, 2x8, their 2- and 4-cell forms,
-    // variants 0 and 6) instead of the whole table.
-    const bool small = (size_t)a.nx * a.ny * a.nz < ((size_t)1 << 20);
-    auto short_tile = [](int tile) { return tile == 2 || tile == 7 || tile == 0 || tile == 14 || tile == 16 || tile == 20; };
+    // Grids under 2^18 cells (64^3-class and smaller) are launch-bound and
+    // solved in a few milliseconds, about what a full tuning pass takes: there
+    // the tuner tries the shapes such grids pick when the whole table is timed
+    // (4x2 and 8x2 planes, 4x4, 1x8) with the cached-coefficient variants
+    // (4-7), and no pairs.
+    const bool small = (size_t)a.nx * a.ny * a.nz < ((size_t)1 << 18);
+    auto short_tile = [](int tile) { return tile == 9 || tile == 10 || tile == 2 || tile == 13; };
     for (int var = 0; var < 8; ++var) {
         if ((var & 1) && !can_store_d) continue;
-        if (small && var != 0 && var != 6) continue;
+        if (small && var < 4) continue;
         for (int tile = 0; tile < kNumTiles; ++tile) {
             if (tile_for_grid(tile, a.nx) != tile) continue;
             if (small && !short_tile(tile)) continue;
@@ -964,12 +966,11 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
         }
         te.check_us = best - te.us > 1.0 ? best - te.us : 1.0;
     }
-    if (F0b && a.ny >= 3) {
+    if (F0b && a.ny >= 3 && !small) {
         for (int var = 0; var < 4; ++var) {
             if ((var & 1) && !can_store_d) continue;
             for (int tile = 0; tile < kNumPairTiles; ++tile) {
                 if (!pair_tile_ok<T>(tile, a.nx)) continue;
-                if (small && tile != 0 && tile != 5) continue;
                 PairCfg pc{tile, var};
                 IterArgs<T> b = a;
                 b.F0 = F0b;
