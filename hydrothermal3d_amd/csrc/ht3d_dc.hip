@@ -45,15 +45,128 @@ __global__ __launch_bounds__(256) void dc_residual_kernel(DcArgs a) {
     block_reduce4(h4, l4, 0x2u, a.partials, bl);
 }
 
-// x += e on the interior (E arrays, FP64 += FP32)
-__global__ __launch_bounds__(256) void dc_update_kernel(double *__restrict__ X, const float *__restrict__ e, int nx,
-                                                        int ny, int nz) {
+// dc_residual_kernel with two x-cells per thread (nx even): X, the face
+// coefficients, R and r32 as 2-element vector loads and stores, X(k-1), X(k),
+// X(k+1) and kz(k), kz(k+1) kept in registers along the z march; the cell
+// formula and order of dc_residual_cell.
+template <bool TR>
+__global__ __launch_bounds__(256) void dc_residual_kernel2(DcArgs a) {
+    typedef CellVec<double, 2>::type V;
+    typedef CellVec<float, 2>::type F2;
+    const int i = blockIdx.x * 128 + 2 * (int)(threadIdx.x & 63u);
+    const int j = blockIdx.y * 4 + (int)(threadIdx.x >> 6);
+    const int k0 = blockIdx.z * kResZ, k1 = min(k0 + kResZ, a.nz);
+    double s_hi = 0.0, s_lo = 0.0, mx = 0.0;
+    if (i < a.nx && j < a.ny) {
+        const uint32_t nx = a.nx, ny = a.ny;
+        const uint32_t sE1 = nx + 2, sE2 = sE1 * (ny + 2), sC2 = nx * ny, sX2 = (nx + 1) * ny, sY2 = nx * (ny + 1);
+        uint32_t e = (uint32_t)(i + 1) + (uint32_t)(j + 1) * sE1 + (uint32_t)(k0 + 1) * sE2;
+        uint32_t c = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k0 * sC2;
+        uint32_t fx = (uint32_t)i + (uint32_t)j * (nx + 1) + (uint32_t)k0 * sX2;
+        uint32_t fy = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k0 * sY2;
+        V xc = ldv<double, 2>(a.X + e), xB = ldv<double, 2>(a.X + e - sE2), kB = ldv<double, 2>(a.kz + c);
+        if (k0 == 0) {
+            xB.x = ghost(a.bct[4], a.bcv[4], (double)xc.x);
+            xB.y = ghost(a.bct[4], a.bcv[4], (double)xc.y);
+        }
+        for (int k = k0; k < k1; ++k) {
+            V xF = ldv<double, 2>(a.X + e + sE2);
+            if (k == a.nz - 1) {
+                xF.x = ghost(a.bct[5], a.bcv[5], (double)xc.x);
+                xF.y = ghost(a.bct[5], a.bcv[5], (double)xc.y);
+            }
+            double xW = a.X[e - 1], xE = a.X[e + 2];
+            V xS = ldv<double, 2>(a.X + e - sE1), xN = ldv<double, 2>(a.X + e + sE1);
+            if (i == 0) xW = ghost(a.bct[0], a.bcv[0], (double)xc.x);
+            if ((uint32_t)i + 2 == nx) xE = ghost(a.bct[1], a.bcv[1], (double)xc.y);
+            if (j == 0) {
+                xS.x = ghost(a.bct[2], a.bcv[2], (double)xc.x);
+                xS.y = ghost(a.bct[2], a.bcv[2], (double)xc.y);
+            }
+            if ((uint32_t)j == ny - 1) {
+                xN.x = ghost(a.bct[3], a.bcv[3], (double)xc.x);
+                xN.y = ghost(a.bct[3], a.bcv[3], (double)xc.y);
+            }
+            const double kW = a.kx[fx];
+            const V kE = ldv<double, 2>(a.kx + fx + 1);
+            const V kS = ldv<double, 2>(a.ky + fy), kN = ldv<double, 2>(a.ky + fy + nx), kF = ldv<double, 2>(a.kz + c + sC2);
+            const V R = ldv<double, 2>(a.R + c);
+            V f;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const double x0 = xc[q], w = q == 0 ? xW : (double)xc[0], ea = q == 1 ? xE : (double)xc[1];
+                const double kw = q == 0 ? kW : (double)kE[0];
+                const double qW = -kw * (a._dx * (x0 - w)), qE = -kE[q] * (a._dx * (ea - x0));
+                const double qS = -kS[q] * (a._dy * (x0 - xS[q])), qN = -kN[q] * (a._dy * (xN[q] - x0));
+                const double qB = -kB[q] * (a._dz * (x0 - xB[q])), qF = -kF[q] * (a._dz * (xF[q] - x0));
+                if (TR) f[q] = R[q] + a._dt * x0 + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
+                else f[q] = R[q] + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
+            }
+            F2 r;
+            r.x = (float)f.x;
+            r.y = (float)f.y;
+            stv<float, 2>(r, a.r32 + c);
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const double af = fabs((double)f[q]);
+                dd_add(s_hi, s_lo, af);
+                mx = (af > mx || af != af) ? af : mx;
+            }
+            xB = xc;
+            xc = xF;
+            kB = kF;
+            e += sE2;
+            c += sC2;
+            fx += sX2;
+            fy += sY2;
+        }
+    }
+    const double h4[4] = {s_hi, mx, 0.0, 0.0}, l4[4] = {s_lo, 0.0, 0.0, 0.0};
+    const unsigned bl = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    block_reduce4(h4, l4, 0x2u, a.partials, bl);
+}
+
+// DYREL scalars of the FP32 sweeps, kept on the device: st = {lmin, h1, h2,
+// lambda updates so far}, hv = {h1, h2} in FP32 for the sweep kernels.
+__global__ void dc_lambda_set(double *__restrict__ st, float *__restrict__ hv, double lmin, double h1, double h2) {
+    st[0] = lmin;
+    st[1] = h1;
+    st[2] = h2;
+    st[3] = 0.0;
+    hv[0] = (float)h1;
+    hv[1] = (float)h2;
+}
+
+// The Rayleigh-quotient update of lambda_min, c, h1, h2
+// (HydroThermal3D_v18.jl:467-473) from a lambda iteration's finished sums
+// (res[2] numerator, res[3] denominator), on the device: the sweeps go on
+// without a read-back.
+__global__ void dc_lambda_update(const double *__restrict__ res, double *__restrict__ st, float *__restrict__ hv,
+                                 double dtau, double cfact, int it) {
+    if (it > 1 && res[3] > 0.0) {
+        const double lmin = fabs(res[2]) / fabs(res[3]);
+        const double c = 2.0 * sqrt(lmin) * cfact;
+        const double h1 = (2 - c * dtau) / (2 + c * dtau), h2 = 2 * dtau / (2 + c * dtau);
+        st[0] = lmin;
+        st[1] = h1;
+        st[2] = h2;
+        st[3] += 1.0;
+        hv[0] = (float)h1;
+        hv[1] = (float)h2;
+    }
+}
+
+// x += e and e = 0 on the interior (E arrays, FP64 += FP32; e's ghosts are
+// never written and stay 0)
+__global__ __launch_bounds__(256) void dc_update_kernel(double *__restrict__ X, float *__restrict__ e, int nx, int ny,
+                                                        int nz) {
     const int i = blockIdx.x * 64 + (int)(threadIdx.x & 63u);
     const int j = blockIdx.y * 4 + (int)(threadIdx.x >> 6);
     const int k = blockIdx.z;
     if (i >= nx || j >= ny) return;
     const size_t idx = (size_t)(i + 1) + (size_t)(j + 1) * (nx + 2) + (size_t)(k + 1) * (nx + 2) * (ny + 2);
     X[idx] += (double)e[idx];
+    e[idx] = 0.0f;
 }
 
 __global__ __launch_bounds__(256) void to_float_kernel(float *__restrict__ out, const double *__restrict__ in, size_t n) {
@@ -118,7 +231,9 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     d.partials = (double *)part; d.nx = nx; d.ny = ny; d.nz = nz;
     d._dx = 1.0 / f->dx; d._dy = 1.0 / f->dy; d._dz = 1.0 / f->dz; d._dt = 1.0 / f->dt;
     for (int q = 0; q < 6; ++q) { d.bct[q] = f->bc_type[q]; d.bcv[q] = f->bc_val[q]; }
-    const dim3 rgrid(div_up(nx, 64), div_up(ny, 4), div_up(nz, kResZ));   // residual: kResZ planes per block
+    // residual: kResZ planes per block, two x-cells per thread when nx is even
+    const bool res2 = nx % 2 == 0;
+    const dim3 rgrid(div_up(nx, res2 ? 128 : 64), div_up(ny, 4), div_up(nz, kResZ));
     const dim3 ugrid(div_up(nx, 64), div_up(ny, 4), nz);                   // update: one plane per block
 
     IterArgs<float> a;
@@ -170,6 +285,14 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     HIP_TRY(hipMemsetAsync(fst, 0, N * sizeof(float), cx.stream));
     double c = 2.0 * sqrt(lmin) * p->cfact;
     double h1 = (2 - c * dtau) / (2 + c * dtau), h2 = 2 * dtau / (2 + c * dtau);
+    // h1, h2 live on the device (lambda updates need no read-back); the
+    // sweep kernels read them through hv
+    double *st = (double *)res + 16;
+    float *hv = (float *)((double *)res + 24);
+    hipLaunchKernelGGL(dc_lambda_set, dim3(1), dim3(1), 0, cx.stream, st, hv, lmin, h1, h2);
+    HIP_TRY(hipGetLastError());
+    a.hv = hv;
+    ap.hv = hv;
     float *buf[2] = {(float *)e0, (float *)e1};
     float *f0buf[2] = {(float *)f0, (float *)f0b};
     int cur = 0, fc = 0;
@@ -179,8 +302,13 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
         // anchor: FP64 residual, FP32 right-hand side, FP64 norm
         if (outer == 1 && te) HIP_TRY(hipStreamSynchronize(cx.stream));   // time the second anchor alone
         const auto ta = std::chrono::steady_clock::now();
-        if (f->transient) hipLaunchKernelGGL(dc_residual_kernel<true>, rgrid, dim3(256), 0, cx.stream, d);
-        else hipLaunchKernelGGL(dc_residual_kernel<false>, rgrid, dim3(256), 0, cx.stream, d);
+        if (res2) {
+            if (f->transient) hipLaunchKernelGGL(dc_residual_kernel2<true>, rgrid, dim3(256), 0, cx.stream, d);
+            else hipLaunchKernelGGL(dc_residual_kernel2<false>, rgrid, dim3(256), 0, cx.stream, d);
+        } else {
+            if (f->transient) hipLaunchKernelGGL(dc_residual_kernel<true>, rgrid, dim3(256), 0, cx.stream, d);
+            else hipLaunchKernelGGL(dc_residual_kernel<false>, rgrid, dim3(256), 0, cx.stream, d);
+        }
         HIP_TRY(hipGetLastError());
         if ((rc = finish_partials((double *)part, rgrid.x * rgrid.y * rgrid.z, 0x2u, (double *)res))) return rc;
         double r4[4];
@@ -206,7 +334,6 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
                 ap.Xin = buf[cur];   // sweeps it and it+1 in one launch
                 ap.Xout = buf[cur ^ 1];
                 ap.F0 = f0buf[fc ^ 1];
-                ap.h1 = (float)h1; ap.h2 = (float)h2;
                 ap.flags = rev ? FL_REVERSE : 0u;
                 rev = g_alternate && !rev;
                 timer.before_launch();
@@ -224,7 +351,6 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
             a.Xin = buf[cur];
             a.Xout = buf[cur ^ 1];
             a.F0 = f0buf[fc];
-            a.h1 = (float)h1; a.h2 = (float)h2;
             a.flags = (lam ? (FL_CHECK | FL_LAMBDA) : 0u) | (store ? FL_STORE_F : 0u) | (rev ? FL_REVERSE : 0u);
             rev = g_alternate && !rev;
             timer.before_launch();
@@ -232,25 +358,26 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
             ++timer.count;
             cur ^= 1;
             inner_total = it;
-            if (lam || s == run - 1) timer.close();
-            if (lam) {
+            if (s == run - 1) timer.close();
+            if (lam) {   // Rayleigh quotient (HydroThermal3D_v18.jl:467-473), on the device
                 if ((rc = finish_partials((double *)part, (unsigned)a.nbx * a.nby * a.nbz, 0x2u, (double *)res))) return rc;
-                if ((rc = read_result4((double *)res, r4))) return rc;
-                timer.collect(bytes32, true);
-                if (it > 1 && r4[3] > 0.0) {  // Rayleigh quotient (HydroThermal3D_v18.jl:467-473)
-                    lmin = fabs(r4[2]) / fabs(r4[3]);
-                    c = 2.0 * sqrt(lmin) * p->cfact;
-                    h1 = (2 - c * dtau) / (2 + c * dtau);
-                    h2 = 2 * dtau / (2 + c * dtau);
-                    ++n_lambda;
-                }
+                hipLaunchKernelGGL(dc_lambda_update, dim3(1), dim3(1), 0, cx.stream, (const double *)res, st, hv, dtau,
+                                   p->cfact, it);
+                HIP_TRY(hipGetLastError());
             }
         }
         timer.collect(bytes32, true);
         // x_anchor += e; the next sweeps start from e = 0
         hipLaunchKernelGGL(dc_update_kernel, ugrid, dim3(256), 0, cx.stream, f->X, buf[cur], nx, ny, nz);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemsetAsync(buf[cur], 0, NE * sizeof(float), cx.stream));
+    }
+    {   // the final DYREL scalars for the report
+        double stv[4];
+        if ((rc = read_result4(st, stv))) return rc;
+        lmin = stv[0];
+        h1 = stv[1];
+        h2 = stv[2];
+        n_lambda = (int)stv[3];
     }
     // ghosts of the result from its own interior; Nu from the converged state
     if ((rc = bc_fill_table(f->X, nx, ny, nz, f->bc_type, f->bc_val))) return rc;

@@ -40,6 +40,7 @@ struct IterArgs {
     T dx, dy, dz, rc_tr_dt;
     T idx2, idy2, idz2;          // 1/dx^2, 1/dy^2, 1/dz^2 (the FP32 diagonal)
     T h1, h2, dtau;
+    const T *hv;                 // when not null: h1, h2 read from the device (hv[0], hv[1])
     int bct[6];
     T bcv[6];
     unsigned flags;
@@ -123,6 +124,7 @@ __device__ __forceinline__ void iter_tile(const IterArgs<T> &a, const T *__restr
 
     double s_hi = 0.0, s_lo = 0.0, acc_max = 0.0, n_hi = 0.0, n_lo = 0.0, d_hi = 0.0, d_lo = 0.0;
     const bool check = flags & FL_CHECK, lam = flags & FL_LAMBDA, store = flags & FL_STORE_F;
+    const T h1 = a.hv ? a.hv[0] : a.h1, h2 = a.hv ? a.hv[1] : a.h2;
 
     if ((uint32_t)i < nx && (uint32_t)j < ny) {
         const bool wlo = i == 0, whi = (uint32_t)i == nx - 1, slo = j == 0, shi = (uint32_t)j == ny - 1;
@@ -158,7 +160,7 @@ __device__ __forceinline__ void iter_tile(const IterArgs<T> &a, const T *__restr
             const T D = DSTORED ? a.Dst[c] : cell_diag(a, kW, kE, kS, kN, kB, kF);
             f = f / D;
             // DYRELUpdate! (kernels_HT3D_dimensional.jl:323-324)
-            const T f0 = a.h1 * f0o + a.h2 * f;
+            const T f0 = h1 * f0o + h2 * f;
             __builtin_nontemporal_store(f0, a.F0 + c);
             if (NTX) __builtin_nontemporal_store(xc - a.dtau * f0, Xout + e);
             else Xout[e] = xc - a.dtau * f0;
@@ -251,6 +253,7 @@ __device__ __forceinline__ void iter_tile_v(const IterArgs<T> &a, const T *__res
 
     double s_hi = 0.0, s_lo = 0.0, acc_max = 0.0, n_hi = 0.0, n_lo = 0.0, d_hi = 0.0, d_lo = 0.0;
     const bool check = flags & FL_CHECK, lam = flags & FL_LAMBDA, store = flags & FL_STORE_F;
+    const T h1 = a.hv ? a.hv[0] : a.h1, h2 = a.hv ? a.hv[1] : a.h2;
 
     if ((uint32_t)i < nx && (uint32_t)j < ny) {
         const bool wlo = i == 0, whi = (uint32_t)i + CPL == nx, slo = j == 0, shi = (uint32_t)j == ny - 1;
@@ -298,7 +301,7 @@ __device__ __forceinline__ void iter_tile_v(const IterArgs<T> &a, const T *__res
 #pragma unroll
             for (int q = 0; q < CPL; ++q) {
                 f[q] = f[q] / D[q];
-                g[q] = a.h1 * f0o[q] + a.h2 * f[q];   // DYRELUpdate! (kernels_HT3D_dimensional.jl:323-324)
+                g[q] = h1 * f0o[q] + h2 * f[q];   // DYRELUpdate! (kernels_HT3D_dimensional.jl:323-324)
                 xo[q] = xc[q] - a.dtau * g[q];
             }
             stv_nt<T, CPL>(g, a.F0 + c);
@@ -384,6 +387,7 @@ __global__ __launch_bounds__(64 * TY) void dyrel_pair_kernel(IterArgs<T> a, cons
     const uint32_t sC2 = (uint32_t)nx * ny, sX1 = nx + 1, sX2 = sX1 * ny, sY2 = (uint32_t)nx * (ny + 1);
     const bool wlo = ic == 0, whi = ic == nx - 1, slo = jc == 0, shi = jc == ny - 1;
     const int ps = max(k0 - 1, 0), pe = min(k1, nz - 1);   // planes of iteration n+1
+    const T h1 = a.hv ? a.hv[0] : a.h1, h2 = a.hv ? a.hv[1] : a.h2;
     uint32_t e = (uint32_t)(ic + 1) + (uint32_t)(jc + 1) * sE1 + (uint32_t)(ps + 1) * sE2;
     uint32_t c = (uint32_t)ic + (uint32_t)jc * nx + (uint32_t)ps * sC2;
     uint32_t fx = (uint32_t)ic + (uint32_t)jc * sX1 + (uint32_t)ps * sX2;
@@ -416,7 +420,7 @@ __global__ __launch_bounds__(64 * TY) void dyrel_pair_kernel(IterArgs<T> a, cons
             T f = cell_residual<T, TRANSIENT>(a, x0c, xW, xE, xS, xN, x0m, xp, kW, kE, kS, kN, kB, kF, r);
             const T D = DSTORED ? __builtin_nontemporal_load(a.Dst + c) : cell_diag(a, kW, kE, kS, kN, kB, kF);
             f = f / D;
-            const T f0 = a.h1 * f0o + a.h2 * f;
+            const T f0 = h1 * f0o + h2 * f;
             x1p = x0c - a.dtau * f0;
             L[p % 3][ly][lx] = x1p;
             x0m = x0c;
@@ -434,7 +438,7 @@ __global__ __launch_bounds__(64 * TY) void dyrel_pair_kernel(IterArgs<T> a, cons
                 if (shi) yN = ghost(a.bct[3], a.bcv[3], x1c);
                 T g = cell_residual<T, TRANSIENT>(a, x1c, yW, yE, yS, yN, yB, x1p, pkW, pkE, pkS, pkN, pkB, pkF, pr);
                 g = g / pD;
-                const T g0 = a.h1 * pf0 + a.h2 * g;
+                const T g0 = h1 * pf0 + h2 * g;
                 __builtin_nontemporal_store(g0, a.F0 + pc);
                 a.Xout[pe_idx] = x1c - a.dtau * g0;
             }
@@ -461,7 +465,7 @@ __global__ __launch_bounds__(64 * TY) void dyrel_pair_kernel(IterArgs<T> a, cons
                 if (shi) yN = ghost(a.bct[3], a.bcv[3], x1c);
                 T g = cell_residual<T, TRANSIENT>(a, x1c, yW, yE, yS, yN, yB, yF, pkW, pkE, pkS, pkN, pkB, pkF, pr);
                 g = g / pD;
-                const T g0 = a.h1 * pf0 + a.h2 * g;
+                const T g0 = h1 * pf0 + h2 * g;
                 __builtin_nontemporal_store(g0, a.F0 + pc);
                 a.Xout[pe_idx] = x1c - a.dtau * g0;
             }
@@ -495,6 +499,7 @@ __global__ __launch_bounds__(64 * TY) void dyrel_pair_kernel_v(IterArgs<T> a, co
     const uint32_t sC2 = (uint32_t)nx * ny, sX1 = nx + 1, sX2 = sX1 * ny, sY2 = (uint32_t)nx * (ny + 1);
     const bool wlo = ic == 0, whi = ic + CPL == nx, slo = jc == 0, shi = jc == ny - 1;
     const int ps = max(k0 - 1, 0), pe = min(k1, nz - 1);   // planes of iteration n+1
+    const T h1 = a.hv ? a.hv[0] : a.h1, h2 = a.hv ? a.hv[1] : a.h2;
     uint32_t e = (uint32_t)(ic + 1) + (uint32_t)(jc + 1) * sE1 + (uint32_t)(ps + 1) * sE2;
     uint32_t c = (uint32_t)ic + (uint32_t)jc * nx + (uint32_t)ps * sC2;
     uint32_t fx = (uint32_t)ic + (uint32_t)jc * sX1 + (uint32_t)ps * sX2;
@@ -534,7 +539,7 @@ __global__ __launch_bounds__(64 * TY) void dyrel_pair_kernel_v(IterArgs<T> a, co
             T g = cell_residual<T, TRANSIENT>(a, x1c[q], w, ea, yS[q], yN[q], yB[q], yF[q], kw, pkE[q], pkS[q], pkN[q],
                                               pkB[q], pkF[q], pr[q]);
             g = g / pD[q];
-            g0[q] = a.h1 * pf0[q] + a.h2 * g;
+            g0[q] = h1 * pf0[q] + h2 * g;
             xo[q] = x1c[q] - a.dtau * g0[q];
         }
         stv_nt<T, CPL>(g0, a.F0 + pc);
@@ -570,7 +575,7 @@ __global__ __launch_bounds__(64 * TY) void dyrel_pair_kernel_v(IterArgs<T> a, co
                                                   kB[q], kF[q], r[q]);
                 if (!DSTORED) D[q] = cell_diag(a, kw, (T)kE[q], (T)kS[q], (T)kN[q], (T)kB[q], (T)kF[q]);
                 f = f / D[q];
-                f0[q] = a.h1 * f0o[q] + a.h2 * f;
+                f0[q] = h1 * f0o[q] + h2 * f;
                 x1p[q] = x0c[q] - a.dtau * f0[q];
             }
             *reinterpret_cast<V *>(&L[p % 3][ly][CPL * lx]) = x1p;
@@ -919,7 +924,8 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
     // solved in a few milliseconds, about what a full tuning pass takes: there
     // the tuner tries the shapes such grids pick when the whole table is timed
     // (4x2 and 8x2 planes, 4x4, 1x8) with the cached-coefficient variants
-    // (4-7), and no pairs.
+    // (4-7), and two pair tiles (for the forced-pairs mode; timing never
+    // picks them there).
     const bool small = (size_t)a.nx * a.ny * a.nz < ((size_t)1 << 18);
     auto short_tile = [](int tile) { return tile == 9 || tile == 10 || tile == 2 || tile == 13; };
     for (int var = 0; var < 8; ++var) {
@@ -966,11 +972,13 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
         }
         te.check_us = best - te.us > 1.0 ? best - te.us : 1.0;
     }
-    if (F0b && a.ny >= 3 && !small) {
+    if (F0b && a.ny >= 3) {
         for (int var = 0; var < 4; ++var) {
             if ((var & 1) && !can_store_d) continue;
+            if (small && var < 2) continue;
             for (int tile = 0; tile < kNumPairTiles; ++tile) {
                 if (!pair_tile_ok<T>(tile, a.nx)) continue;
+                if (small && tile != 0 && tile != 5) continue;
                 PairCfg pc{tile, var};
                 IterArgs<T> b = a;
                 b.F0 = F0b;
