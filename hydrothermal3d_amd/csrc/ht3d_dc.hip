@@ -25,7 +25,8 @@ int bc_fill_table(double *A, int nx, int ny, int nz, const int type[6], const do
 int nu_sum(const double *T, const double *ky, int nx, int ny, int nz, double _dy, double *sum);
 
 // FP64 residual r(x) with on-the-fly ghosts; slots: 0 sum|r| (dd), 1 max|r|.
-// A thread marches kResZ planes, so a launch leaves few block partials.
+// A thread marches kResZ planes (unrolled by 8: the loads of several planes in
+// flight), so a launch leaves few block partials for the finishing reduction.
 constexpr int kResZ = 16;
 template <bool TR>
 __global__ __launch_bounds__(256) void dc_residual_kernel(DcArgs a) {
@@ -34,6 +35,7 @@ __global__ __launch_bounds__(256) void dc_residual_kernel(DcArgs a) {
     const int k0 = blockIdx.z * kResZ, k1 = min(k0 + kResZ, a.nz);
     double s_hi = 0.0, s_lo = 0.0, mx = 0.0;
     if (i < a.nx && j < a.ny) {
+#pragma unroll 8
         for (int k = k0; k < k1; ++k) {
             const double af = fabs(dc_residual_cell<TR>(a, i, j, k));
             dd_add(s_hi, s_lo, af);
@@ -69,6 +71,7 @@ __global__ __launch_bounds__(256) void dc_residual_kernel2(DcArgs a) {
             xB.x = ghost(a.bct[4], a.bcv[4], (double)xc.x);
             xB.y = ghost(a.bct[4], a.bcv[4], (double)xc.y);
         }
+#pragma unroll 8
         for (int k = k0; k < k1; ++k) {
             V xF = ldv<double, 2>(a.X + e + sE2);
             if (k == a.nz - 1) {
@@ -156,17 +159,23 @@ __global__ void dc_lambda_update(const double *__restrict__ res, double *__restr
     }
 }
 
-// x += e and e = 0 on the interior (E arrays, FP64 += FP32; e's ghosts are
-// never written and stay 0)
-__global__ __launch_bounds__(256) void dc_update_kernel(double *__restrict__ X, float *__restrict__ e, int nx, int ny,
-                                                        int nz) {
-    const int i = blockIdx.x * 64 + (int)(threadIdx.x & 63u);
-    const int j = blockIdx.y * 4 + (int)(threadIdx.x >> 6);
-    const int k = blockIdx.z;
-    if (i >= nx || j >= ny) return;
-    const size_t idx = (size_t)(i + 1) + (size_t)(j + 1) * (nx + 2) + (size_t)(k + 1) * (nx + 2) * (ny + 2);
-    X[idx] += (double)e[idx];
-    e[idx] = 0.0f;
+// x += e and e = 0 over the whole E array, two entries per thread.  e's ghosts
+// are never written and stay 0, so X's ghosts keep their values (they are
+// refilled from the interior after the solve); a flat pass streams both arrays
+// without the row and plane gaps of an interior-only walk.
+__global__ __launch_bounds__(256) void dc_update_kernel(double *__restrict__ X, float *__restrict__ e, size_t n) {
+    const size_t t = ((size_t)blockIdx.x * 256 + threadIdx.x) * 2;
+    if (t + 1 < n) {
+        CellVec<double, 2>::type x = ldv<double, 2>(X + t);
+        const CellVec<float, 2>::type v = ldv<float, 2>(e + t);
+        x.x += (double)v.x;
+        x.y += (double)v.y;
+        stv<double, 2>(x, X + t);
+        stv<float, 2>(CellVec<float, 2>::type{0.0f, 0.0f}, e + t);
+    } else if (t < n) {
+        X[t] += (double)e[t];
+        e[t] = 0.0f;
+    }
 }
 
 __global__ __launch_bounds__(256) void to_float_kernel(float *__restrict__ out, const double *__restrict__ in, size_t n) {
@@ -234,7 +243,7 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     // residual: kResZ planes per block, two x-cells per thread when nx is even
     const bool res2 = nx % 2 == 0;
     const dim3 rgrid(div_up(nx, res2 ? 128 : 64), div_up(ny, 4), div_up(nz, kResZ));
-    const dim3 ugrid(div_up(nx, 64), div_up(ny, 4), nz);                   // update: one plane per block
+    const dim3 ugrid(div_up(NE, 512));                                     // update: two entries per thread
 
     IterArgs<float> a;
     std::memset(&a, 0, sizeof a);
@@ -311,8 +320,42 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
         }
         HIP_TRY(hipGetLastError());
         if ((rc = finish_partials((double *)part, rgrid.x * rgrid.y * rgrid.z, 0x2u, (double *)res))) return rc;
+        if ((rc = post_result4((double *)res))) return rc;
+        // The next run's first sweep is queued before the anchor norm is waited
+        // for, so the GPU works through the read-back.  Sweeps write only e and
+        // F0 (FP32), never X: when the anchor shows convergence the sweep is
+        // dropped.  Only plain sweeps (a pair when the loop below would pair
+        // them) go in this way, and not while the anchor is being timed.
+        const int run = (p->nitmax - inner_total) < anchor ? (p->nitmax - inner_total) : anchor;
+        int spec = 0;   // sweeps already queued for this run
+        if (run > 0 && !(outer == 1 && te) && plain(inner_total + 1)) {
+            const int it = inner_total + 1;
+            timer.before_launch();
+            if (pcfg.tile >= 0 && run > 1 && it + 1 < p->nitmax && plain(it + 1)) {
+                ap.Xin = buf[cur];
+                ap.Xout = buf[cur ^ 1];
+                ap.F0 = f0buf[fc ^ 1];
+                ap.flags = rev ? FL_REVERSE : 0u;
+                rev = g_alternate && !rev;
+                if ((rc = launch_pair(ap, (const float *)f0buf[fc], f->transient != 0, pcfg))) return rc;
+                timer.count += 2;
+                fc ^= 1;
+                spec = 2;
+            } else {
+                a.Xin = buf[cur];
+                a.Xout = buf[cur ^ 1];
+                a.F0 = f0buf[fc];
+                a.flags = rev ? FL_REVERSE : 0u;
+                rev = g_alternate && !rev;
+                if ((rc = launch_iter(a, f->transient != 0, cfg))) return rc;
+                ++timer.count;
+                spec = 1;
+            }
+            cur ^= 1;
+            if (spec == run) timer.close();
+        }
         double r4[4];
-        if ((rc = read_result4((double *)res, r4))) return rc;
+        if ((rc = wait_result4(r4))) return rc;
         if (outer == 1 && te) {
             const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - ta).count();
             te->anchor_us = te->anchor_us > 0 && te->anchor_us < us ? te->anchor_us : us;
@@ -326,9 +369,9 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
         if (nF_rel > 100) { status = HT3D_DIVERGED; break; }
         if (nF_abs < p->tol || nF_rel < p->tol) { status = HT3D_OK; break; }
         if (inner_total >= p->nitmax) break;
-        // `anchor` FP32 sweeps of the same DYREL iteration
-        const int run = (p->nitmax - inner_total) < anchor ? (p->nitmax - inner_total) : anchor;
-        for (int s = 0; s < run; ++s) {
+        // `anchor` FP32 sweeps of the same DYREL iteration (the first may be running)
+        inner_total += spec;
+        for (int s = spec; s < run; ++s) {
             const int it = inner_total + 1;
             if (pcfg.tile >= 0 && s + 1 < run && it + 1 < p->nitmax && plain(it) && plain(it + 1)) {
                 ap.Xin = buf[cur];   // sweeps it and it+1 in one launch
@@ -368,7 +411,7 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
         }
         timer.collect(bytes32, true);
         // x_anchor += e; the next sweeps start from e = 0
-        hipLaunchKernelGGL(dc_update_kernel, ugrid, dim3(256), 0, cx.stream, f->X, buf[cur], nx, ny, nz);
+        hipLaunchKernelGGL(dc_update_kernel, ugrid, dim3(256), 0, cx.stream, f->X, buf[cur], (size_t)NE);
         HIP_TRY(hipGetLastError());
     }
     {   // the final DYREL scalars for the report

@@ -177,5 +177,9 @@ __device__ inline void block_reduce4(const double hi_in[4], const double lo_in[4
 int finish_partials(const double *partials, unsigned nblocks, unsigned ops, double *result_dev);
 // Read back 4 doubles from the device result buffer (synchronises the stream).
 int read_result4(const double *result_dev, double out[4]);
+// read_result4 in two halves: queue the copy, and later wait for it (work
+// queued in between runs before the wait returns)
+int post_result4(const double *result_dev);
+int wait_result4(double out[4]);
 
 }  // namespace ht3d

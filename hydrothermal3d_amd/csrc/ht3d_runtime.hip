@@ -138,6 +138,23 @@ int finish_partials(const double *partials, unsigned nblocks, unsigned ops, doub
     return HT3D_OK;
 }
 
+// The copy is followed by an event, and the wait is on that event only: work
+// queued after post_result4 keeps the GPU busy while the host waits.
+static hipEvent_t g_res_event = nullptr;
+
+int post_result4(const double *result_dev) {
+    if (!g_res_event) HIP_TRY(hipEventCreateWithFlags(&g_res_event, hipEventDisableTiming));
+    HIP_TRY(hipMemcpyAsync(g_ctx.pinned, result_dev, 4 * sizeof(double), hipMemcpyDeviceToHost, g_ctx.stream));
+    HIP_TRY(hipEventRecord(g_res_event, g_ctx.stream));
+    return HT3D_OK;
+}
+
+int wait_result4(double out[4]) {
+    HIP_TRY(hipEventSynchronize(g_res_event));
+    for (int q = 0; q < 4; ++q) out[q] = g_ctx.pinned[q];
+    return HT3D_OK;
+}
+
 int read_result4(const double *result_dev, double out[4]) {
     HIP_TRY(hipMemcpyAsync(g_ctx.pinned, result_dev, 4 * sizeof(double), hipMemcpyDeviceToHost, g_ctx.stream));
     HIP_TRY(hipStreamSynchronize(g_ctx.stream));
