@@ -101,3 +101,39 @@ def test_fault_copy_oracle_and_dc(ht, orc, name):
     assert rel_err(T1, T0) <= 1e-6
     assert rel_err(P1 - P1.mean(), P0 - P0.mean()) <= 1e-6
     assert np.max(np.abs(N1 - N0) / np.abs(N0)) <= 1e-6
+
+
+@pytest.mark.parametrize("ncheck", [1, 2, 3, 7])
+@pytest.mark.parametrize("transient", [0, 1], ids=["darcy", "heat"])
+def test_dc_anchor_intervals(ht, transient, ncheck):
+    """Anchor intervals of 1, 2, 3 and 7 sweeps.  The first sweep of each run
+    is queued before the anchor norm is read back (a two-sweep launch when
+    pairs are on and the run has room for one) and its result is dropped when
+    the norm shows convergence; with 1 or 2 sweeps per anchor it is the whole
+    run.  Pairs forced and pairs off must give the same solve bit for bit, and
+    both must stay within 1e-6 of the FP64 solve."""
+    grid = (40, 21, 30)
+    X, R, k = _problem(grid, transient, 29)
+    h = dict(dx=1.0 / grid[0], dy=0.5 / grid[1], dz=1.0 / grid[2], dt=1e-3)
+    bc = dict(bc_type=(0, 0, 1, 1, 0, 0), bc_val=(0.0, 0.0, 1.0, 0.0, 0.0, 0.0)) if transient else {}
+    prm = ht.production_params(tol=1e-11, nitout=100, ncheck=ncheck)
+    out = {}
+    try:
+        for key, prec, pairs in (("fp64", 0, 1), ("dc_single", 1, 0), ("dc_pairs", 1, 2)):
+            ht._lib.call("ht3d_tune_pairs", pairs)
+            D = [ht.DeviceArray.from_host(a) for a in (X, R, *k)]
+            rep = ht.dyrel_solve(*D, transient=transient, rhoC=float(transient), params=prm, precision=prec,
+                                 **h, **bc)
+            assert rep["status"] == 0, (key, rep)
+            out[key] = (rep, D[0].get()[1:-1, 1:-1, 1:-1])
+    finally:
+        ht._lib.call("ht3d_tune_pairs", 1)
+    (r0, x0), (r1, x1), (r2, x2) = out["fp64"], out["dc_single"], out["dc_pairs"]
+    assert np.array_equal(x1, x2)
+    for key in ("iters", "n_checks", "n_lambda", "lmin", "nF_abs", "nF_rel"):
+        assert r1[key] == r2[key], (key, r1[key], r2[key])
+    assert r1["iters"] % ncheck == 0 or r1["iters"] == prm.nitmax
+    a, b = x1, x0
+    if not transient:
+        a, b = a - a.mean(), b - b.mean()
+    assert rel_err(a, b) <= 1e-6
