@@ -192,8 +192,9 @@ def run_batch(n: int, steps: int, warmup: int, prm):
                               if st["ms_solve"] > 0 else None,
                               "solve_share": round(st["ms_solve"] / 1e3 / wall, 3),
                               "graph_launches": st["graph_launches"], "graphs": st["graphs"],
-                              "repacks": st["repacks"], "tiles": [f"{r}x{z}" for r, z in zip(st["tile_rows"],
-                                                                                             st["tile_z"])]})
+                              "repacks": st["repacks"],
+                              "tiles": [f"{r}x{z}" + (f" x{c}" if c > 1 else "")
+                                        for r, z, c in zip(st["tile_rows"], st["tile_z"], st["tile_cpl"])]})
             b.close()
         states[mode] = _batch_state(ms, reps)
         for m in ms:

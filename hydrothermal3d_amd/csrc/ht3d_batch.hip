@@ -54,10 +54,19 @@ struct BState {
     int active, outer, inner, pad;
 };
 
-// tiles of the batched launches (short tiles suit the small grids of a batch)
-static constexpr TileCfg kBTiles[] = {{2, 8}, {4, 4}, {4, 8}, {8, 4}, {4, 2}, {1, 8}};
+// tiles of the batched launches (short tiles suit the small grids of a batch);
+// the last six hold 2 or 4 x-cells per lane (vector loads, bit-identical
+// cells), offered when every problem's ncx is a multiple of that count, and 4
+// only to the FP32 sweeps
+static constexpr TileCfg kBTiles[] = {{2, 8},    {4, 4},    {4, 8},    {8, 4},    {4, 2},    {1, 8},
+                                      {4, 4, 2}, {4, 8, 2}, {2, 8, 2}, {4, 4, 4}, {4, 8, 4}, {2, 8, 4}};
 static constexpr int kNumBTiles = sizeof(kBTiles) / sizeof(kBTiles[0]);
 static constexpr int kBDefaultTile = 1;
+
+// tiles of one problem under batch tile `tile`
+static inline unsigned btile_count(int nx, int ny, int nz, int tile) {
+    return div_up(nx, 64 * kBTiles[tile].cpl) * div_up(ny, kBTiles[tile].ty) * div_up(nz, kBTiles[tile].kz);
+}
 
 __device__ __forceinline__ void tile_of(unsigned l, int nbx, int nby, int &bx, int &by, int &bz) {
     bx = (int)(l % (unsigned)nbx);
@@ -66,7 +75,7 @@ __device__ __forceinline__ void tile_of(unsigned l, int nbx, int nby, int &bx, i
 }
 
 // ---------------------------------------------------------------- kernels
-template <typename T, bool TR, int TY, int KZ>
+template <typename T, bool TR, int TY, int KZ, int CPL>
 __global__ __launch_bounds__(64 * TY) void batch_iter_kernel(const IterArgs<T> *__restrict__ tab,
                                                              const uint2 *__restrict__ items,
                                                              const unsigned *__restrict__ n_items,
@@ -82,13 +91,18 @@ __global__ __launch_bounds__(64 * TY) void batch_iter_kernel(const IterArgs<T> *
         int bx, by, bz;
         tile_of(it.y, a.nbx, a.nby, bx, by, bz);
         T *x0 = const_cast<T *>(a.Xin), *x1 = a.Xout;
-        iter_tile<T, TR, false, false, false, TY, KZ>(a, phase ? x1 : x0, phase ? x0 : x1, flags, bx, by, bz, partials, t);
+        if constexpr (CPL == 1)
+            iter_tile<T, TR, false, false, false, TY, KZ>(a, phase ? x1 : x0, phase ? x0 : x1, flags, bx, by, bz, partials, t);
+        else
+            iter_tile_v<T, TR, false, false, false, TY, KZ, CPL>(a, phase ? x1 : x0, phase ? x0 : x1, flags, bx, by, bz,
+                                                               partials, t);
         if (flags & FL_CHECK) __syncthreads();  // the block reduction's shared array is reused by the next item
     }
 }
 
-// FP64 anchor residual of the defect-correction form (same tiles as the sweeps)
-template <bool TR, int TY, int KZ>
+// FP64 anchor residual of the defect-correction form (same tiles as the sweeps;
+// a tile with CPL x-cells per lane is walked as CPL chunks of 64 cells)
+template <bool TR, int TY, int KZ, int CPL>
 __global__ __launch_bounds__(64 * TY) void batch_dc_residual_kernel(const DcArgs *__restrict__ dtab,
                                                                     const IterArgs<float> *__restrict__ tab32,
                                                                     const uint2 *__restrict__ items,
@@ -103,15 +117,19 @@ __global__ __launch_bounds__(64 * TY) void batch_dc_residual_kernel(const DcArgs
         const DcArgs &a = dtab[it.x];
         int bx, by, bz;
         tile_of(it.y, tab32[it.x].nbx, tab32[it.x].nby, bx, by, bz);
-        const int i = bx * 64 + (int)(threadIdx.x & 63u), j = by * TY + (int)(threadIdx.x >> 6);
+        const int j = by * TY + (int)(threadIdx.x >> 6);
         const int k0 = bz * KZ, k1 = min(k0 + KZ, a.nz);
         double s_hi = 0.0, s_lo = 0.0, mx = 0.0;
-        if (i < a.nx && j < a.ny)
-            for (int k = k0; k < k1; ++k) {
-                const double af = fabs(dc_residual_cell<TR>(a, i, j, k));
-                dd_add(s_hi, s_lo, af);
-                mx = (af > mx || af != af) ? af : mx;
-            }
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) {
+            const int i = (bx * CPL + q) * 64 + (int)(threadIdx.x & 63u);
+            if (i < a.nx && j < a.ny)
+                for (int k = k0; k < k1; ++k) {
+                    const double af = fabs(dc_residual_cell<TR>(a, i, j, k));
+                    dd_add(s_hi, s_lo, af);
+                    mx = (af > mx || af != af) ? af : mx;
+                }
+        }
         const double h4[4] = {s_hi, mx, 0.0, 0.0}, l4[4] = {s_lo, 0.0, 0.0, 0.0};
         block_reduce4(h4, l4, 0x2u, partials, t);
         __syncthreads();
@@ -119,7 +137,7 @@ __global__ __launch_bounds__(64 * TY) void batch_dc_residual_kernel(const DcArgs
 }
 
 // x += e (FP64 += FP32) and e = 0 on the interior; `phase` selects the e buffer
-template <int TY, int KZ>
+template <int TY, int KZ, int CPL>
 __global__ __launch_bounds__(64 * TY) void batch_dc_update_kernel(const DcArgs *__restrict__ dtab,
                                                                   const IterArgs<float> *__restrict__ tab32,
                                                                   const uint2 *__restrict__ items,
@@ -135,14 +153,19 @@ __global__ __launch_bounds__(64 * TY) void batch_dc_update_kernel(const DcArgs *
         float *e = phase ? a.Xout : const_cast<float *>(a.Xin);
         int bx, by, bz;
         tile_of(it.y, a.nbx, a.nby, bx, by, bz);
-        const int i = bx * 64 + (int)(threadIdx.x & 63u), j = by * TY + (int)(threadIdx.x >> 6);
+        const int j = by * TY + (int)(threadIdx.x >> 6);
         const int k0 = bz * KZ, k1 = min(k0 + KZ, a.nz);
-        if (i < a.nx && j < a.ny)
-            for (int k = k0; k < k1; ++k) {
-                const size_t idx = (size_t)(i + 1) + (size_t)(j + 1) * (a.nx + 2) + (size_t)(k + 1) * (a.nx + 2) * (a.ny + 2);
-                X[idx] += (double)e[idx];
-                e[idx] = 0.0f;
-            }
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) {
+            const int i = (bx * CPL + q) * 64 + (int)(threadIdx.x & 63u);
+            if (i < a.nx && j < a.ny)
+                for (int k = k0; k < k1; ++k) {
+                    const size_t idx =
+                        (size_t)(i + 1) + (size_t)(j + 1) * (a.nx + 2) + (size_t)(k + 1) * (a.nx + 2) * (a.ny + 2);
+                    X[idx] += (double)e[idx];
+                    e[idx] = 0.0f;
+                }
+        }
     }
 }
 
@@ -333,15 +356,25 @@ template <typename T, bool TR>
 static void launch_iter_b(int tile, const BLaunch &L, int phase, unsigned flags) {
     const IterArgs<T> *tab = (const IterArgs<T> *)L.tab;
     hipStream_t s = ctx().stream;
-#define BI_(TY, KZ) hipLaunchKernelGGL((batch_iter_kernel<T, TR, TY, KZ>), dim3(L.grid), dim3(64 * TY), 0, s, tab, \
-                                       L.items, L.n_items, L.st, L.partials, phase, flags)
+#define BI_(TY, KZ, CPL) hipLaunchKernelGGL((batch_iter_kernel<T, TR, TY, KZ, CPL>), dim3(L.grid), dim3(64 * TY), 0, s, \
+                                            tab, L.items, L.n_items, L.st, L.partials, phase, flags)
     switch (tile) {
-    case 0: BI_(2, 8); break;
-    case 1: BI_(4, 4); break;
-    case 2: BI_(4, 8); break;
-    case 3: BI_(8, 4); break;
-    case 4: BI_(4, 2); break;
-    default: BI_(1, 8); break;
+    case 0: BI_(2, 8, 1); break;
+    case 1: BI_(4, 4, 1); break;
+    case 2: BI_(4, 8, 1); break;
+    case 3: BI_(8, 4, 1); break;
+    case 4: BI_(4, 2, 1); break;
+    case 5: BI_(1, 8, 1); break;
+    case 6: BI_(4, 4, 2); break;
+    case 7: BI_(4, 8, 2); break;
+    case 8: BI_(2, 8, 2); break;
+    default:   // 4 cells per lane: FP32 only (btile_ok)
+        if constexpr (sizeof(T) == 4) {
+            if (tile == 9) BI_(4, 4, 4);
+            else if (tile == 10) BI_(4, 8, 4);
+            else BI_(2, 8, 4);
+        }
+        break;
     }
 #undef BI_
 }
@@ -350,15 +383,21 @@ template <bool TR>
 static void launch_res_b(int tile, const BLaunch &L) {
     const IterArgs<float> *tab = (const IterArgs<float> *)L.tab;
     hipStream_t s = ctx().stream;
-#define BR_(TY, KZ) hipLaunchKernelGGL((batch_dc_residual_kernel<TR, TY, KZ>), dim3(L.grid), dim3(64 * TY), 0, s, \
-                                       L.dtab, tab, L.items, L.n_items, L.st, L.partials)
+#define BR_(TY, KZ, CPL) hipLaunchKernelGGL((batch_dc_residual_kernel<TR, TY, KZ, CPL>), dim3(L.grid), dim3(64 * TY), \
+                                            0, s, L.dtab, tab, L.items, L.n_items, L.st, L.partials)
     switch (tile) {
-    case 0: BR_(2, 8); break;
-    case 1: BR_(4, 4); break;
-    case 2: BR_(4, 8); break;
-    case 3: BR_(8, 4); break;
-    case 4: BR_(4, 2); break;
-    default: BR_(1, 8); break;
+    case 0: BR_(2, 8, 1); break;
+    case 1: BR_(4, 4, 1); break;
+    case 2: BR_(4, 8, 1); break;
+    case 3: BR_(8, 4, 1); break;
+    case 4: BR_(4, 2, 1); break;
+    case 5: BR_(1, 8, 1); break;
+    case 6: BR_(4, 4, 2); break;
+    case 7: BR_(4, 8, 2); break;
+    case 8: BR_(2, 8, 2); break;
+    case 9: BR_(4, 4, 4); break;
+    case 10: BR_(4, 8, 4); break;
+    default: BR_(2, 8, 4); break;
     }
 #undef BR_
 }
@@ -366,15 +405,21 @@ static void launch_res_b(int tile, const BLaunch &L) {
 static void launch_upd_b(int tile, const BLaunch &L, int phase) {
     const IterArgs<float> *tab = (const IterArgs<float> *)L.tab;
     hipStream_t s = ctx().stream;
-#define BU_(TY, KZ) hipLaunchKernelGGL((batch_dc_update_kernel<TY, KZ>), dim3(L.grid), dim3(64 * TY), 0, s, L.dtab, \
-                                       tab, L.items, L.n_items, L.st, phase)
+#define BU_(TY, KZ, CPL) hipLaunchKernelGGL((batch_dc_update_kernel<TY, KZ, CPL>), dim3(L.grid), dim3(64 * TY), 0, s, \
+                                            L.dtab, tab, L.items, L.n_items, L.st, phase)
     switch (tile) {
-    case 0: BU_(2, 8); break;
-    case 1: BU_(4, 4); break;
-    case 2: BU_(4, 8); break;
-    case 3: BU_(8, 4); break;
-    case 4: BU_(4, 2); break;
-    default: BU_(1, 8); break;
+    case 0: BU_(2, 8, 1); break;
+    case 1: BU_(4, 4, 1); break;
+    case 2: BU_(4, 8, 1); break;
+    case 3: BU_(8, 4, 1); break;
+    case 4: BU_(4, 2, 1); break;
+    case 5: BU_(1, 8, 1); break;
+    case 6: BU_(4, 4, 2); break;
+    case 7: BU_(4, 8, 2); break;
+    case 8: BU_(2, 8, 2); break;
+    case 9: BU_(4, 4, 4); break;
+    case 10: BU_(4, 8, 4); break;
+    default: BU_(2, 8, 4); break;
     }
 #undef BU_
 }
@@ -468,7 +513,7 @@ static size_t build_items(ht3d_batch *b, int tile, const std::vector<char> &act,
     size_t n = 0;
     for (int q = 0; q < b->nprob; ++q) {
         const ht3d_hrl *m = b->m[q];
-        const unsigned cnt = div_up(m->nx, 64) * div_up(m->ny, kBTiles[tile].ty) * div_up(m->nz, kBTiles[tile].kz);
+        const unsigned cnt = btile_count(m->nx, m->ny, m->nz, tile);
         range[q] = make_uint2((unsigned)n, act[q] ? cnt : 0u);
         if (!act[q]) continue;
         for (unsigned l = 0; l < cnt; ++l) items[n++] = make_uint2((unsigned)q, l);
@@ -510,7 +555,8 @@ static void fill_tables(ht3d_batch *b, int kind, const std::vector<ht3d_dyrel_fi
         a.R = d.R; a.kx = d.kx; a.ky = d.ky; a.kz = d.kz; a.Dst = nullptr;
         a.F0 = b->f0 + b->oc[q]; a.Fst = b->fst + b->oc[q]; a.partials = nullptr;
         a.nx = nx; a.ny = ny; a.nz = nz;
-        a.nbx = (int)div_up(nx, 64); a.nby = (int)div_up(ny, kBTiles[tile64].ty); a.nbz = (int)div_up(nz, kBTiles[tile64].kz);
+        a.nbx = (int)div_up(nx, 64 * kBTiles[tile64].cpl); a.nby = (int)div_up(ny, kBTiles[tile64].ty);
+        a.nbz = (int)div_up(nz, kBTiles[tile64].kz);
         a._dx = 1.0 / d.dx; a._dy = 1.0 / d.dy; a._dz = 1.0 / d.dz; a._dt = 1.0 / d.dt;
         a.dx = d.dx; a.dy = d.dy; a.dz = d.dz; a.rc_tr_dt = rc_tr_dt;
         set_inv_h2(a, d.dx, d.dy, d.dz);
@@ -524,7 +570,8 @@ static void fill_tables(ht3d_batch *b, int kind, const std::vector<ht3d_dyrel_fi
         g.kx = b->k32 + b->of[q]; g.ky = g.kx + nfx; g.kz = g.ky + nfy; g.Dst = nullptr;
         g.F0 = b->f032 + b->oc[q]; g.Fst = b->fst32 + b->oc[q]; g.partials = nullptr;
         g.nx = nx; g.ny = ny; g.nz = nz;
-        g.nbx = (int)div_up(nx, 64); g.nby = (int)div_up(ny, kBTiles[tile32].ty); g.nbz = (int)div_up(nz, kBTiles[tile32].kz);
+        g.nbx = (int)div_up(nx, 64 * kBTiles[tile32].cpl); g.nby = (int)div_up(ny, kBTiles[tile32].ty);
+        g.nbz = (int)div_up(nz, kBTiles[tile32].kz);
         g._dx = (float)a._dx; g._dy = (float)a._dy; g._dz = (float)a._dz; g._dt = (float)a._dt;
         g.dx = (float)d.dx; g.dy = (float)d.dy; g.dz = (float)d.dz; g.rc_tr_dt = (float)rc_tr_dt;
         set_inv_h2(g, d.dx, d.dy, d.dz);
@@ -566,7 +613,7 @@ static BLaunch make_launch(ht3d_batch *b, int kind, bool fp32, int tile) {
     size_t n = 0;
     for (int q = 0; q < b->nprob; ++q) {
         const ht3d_hrl *m = b->m[q];
-        n += (size_t)div_up(m->nx, 64) * div_up(m->ny, kBTiles[tile].ty) * div_up(m->nz, kBTiles[tile].kz);
+        n += btile_count(m->nx, m->ny, m->nz, tile);
     }
     L.grid = persistent_grid(b, n, tile);
     L.threads = 64 * kBTiles[tile].ty;
@@ -654,14 +701,35 @@ static int run_block(ht3d_batch *b, const std::vector<BOp> &ops, int kind, bool 
 
 static inline bool lam_it(int it, int nitout) { return it <= 2 || (nitout > 0 && it % nitout == 0); }
 
+// a tile with CPL x-cells per lane needs every problem's ncx to be a multiple
+// of CPL; 4 cells per lane only in FP32 (in FP64 they exhaust the registers)
+static bool btile_ok(const ht3d_batch *b, int tile, bool fp32) {
+    const int cpl = kBTiles[tile].cpl;
+    if (cpl > (fp32 ? 4 : 2)) return false;
+    for (const ht3d_hrl *m : b->m)
+        if (m->nx % cpl) return false;
+    return true;
+}
+
+static int g_force_btile = -1;   // ht3d_batch_tune_force
+
 // time `reps` plain batched sweeps of every tile and keep the fastest
 static int autotune_batch(ht3d_batch *b, int kind, bool transient, bool fp32, const std::vector<ht3d_dyrel_field> &f,
                           int *tile_out) {
-    if (g_force_tile >= 0 || !g_autotune) {
-        *tile_out = (g_force_tile >= 0 && g_force_tile < kNumBTiles) ? g_force_tile : kBDefaultTile;
+    if (g_force_btile >= 0 || !g_autotune) {   // a pinned tile the batch cannot run falls back to the default
+        *tile_out = (g_force_btile >= 0 && btile_ok(b, g_force_btile, fp32)) ? g_force_btile : kBDefaultTile;
         return HT3D_OK;
     }
     hipStream_t s = ctx().stream;
+    if (fp32) {   // the timed anchor below adds e (the phase-0 buffer, which the
+                  // phase-0 sweeps only read) to the problems' unknowns: it must
+                  // hold zeros, and the sweeps should run on defined data
+        int rc;
+        if ((rc = zero_dev(b->e0, b->se * sizeof(float))) || (rc = zero_dev(b->e1, b->se * sizeof(float))) ||
+            (rc = zero_dev(b->r32, b->sc * sizeof(float))) || (rc = zero_dev(b->f032, b->sc * sizeof(float))) ||
+            (rc = zero_dev(b->fst32, b->sc * sizeof(float))))
+            return rc;
+    }
     hipLaunchKernelGGL(batch_activate_kernel, dim3(div_up(b->nprob, 256)), dim3(256), 0, s, b->d_state, b->nprob);
     HIP_TRY(hipGetLastError());
     std::vector<char> all(b->nprob, 1);
@@ -669,6 +737,7 @@ static int autotune_batch(ht3d_batch *b, int kind, bool transient, bool fp32, co
     int best_tile = kBDefaultTile;
     ht3d_dyrel_params p{};
     for (int t = 0; t < kNumBTiles; ++t) {
+        if (!btile_ok(b, t, fp32)) continue;
         fill_tables(b, kind, f, t, t);
         int rc;
         if ((rc = upload_tables(b, kind))) return rc;
@@ -962,7 +1031,7 @@ int ht3d_batch_create(ht3d_hrl *const *models, int nprob, ht3d_batch **out) {
         size_t n = 0;
         for (int q = 0; q < nprob; ++q) {
             const ht3d_hrl *m = models[q];
-            n += (size_t)div_up(m->nx, 64) * div_up(m->ny, kBTiles[t].ty) * div_up(m->nz, kBTiles[t].kz);
+            n += btile_count(m->nx, m->ny, m->nz, t);
         }
         if (n > maxit) maxit = n;
     }
@@ -1123,6 +1192,8 @@ int ht3d_batch_step(ht3d_batch *b, ht3d_step_report *reps) {
 int ht3d_batch_stats_get(const ht3d_batch *b, ht3d_batch_stats *s) {
     if (!b || !s) return arg_fail("ht3d_batch_stats_get: bad argument");
     *s = b->stats;
+    const int tiles[4] = {b->tile64[0], b->tile64[1], b->tile32[0], b->tile32[1]};
+    for (int q = 0; q < 4; ++q) s->tile_cpl[q] = tiles[q] >= 0 ? kBTiles[tiles[q]].cpl : 0;
     s->tile_rows[0] = b->tile64[0] >= 0 ? kBTiles[b->tile64[0]].ty : 0;
     s->tile_rows[1] = b->tile64[1] >= 0 ? kBTiles[b->tile64[1]].ty : 0;
     s->tile_rows[2] = b->tile32[0] >= 0 ? kBTiles[b->tile32[0]].ty : 0;
@@ -1133,6 +1204,14 @@ int ht3d_batch_stats_get(const ht3d_batch *b, ht3d_batch_stats *s) {
     s->tile_z[3] = b->tile32[1] >= 0 ? kBTiles[b->tile32[1]].kz : 0;
     return HT3D_OK;
 }
+
+int ht3d_batch_tune_force(int tile) {
+    if (tile >= kNumBTiles) return arg_fail("ht3d_batch_tune_force: tile index past the end of the table");
+    g_force_btile = tile < 0 ? -1 : tile;
+    return HT3D_OK;
+}
+
+int ht3d_batch_tile_count(void) { return kNumBTiles; }
 
 int ht3d_batch_stats_reset(ht3d_batch *b) {
     if (!b) return arg_fail("ht3d_batch_stats_reset: null batch");

@@ -525,6 +525,7 @@ typedef struct {
     double ms_step;             /* whole batched steps (host wall clock)                  */
     long long steps, blocks, graph_launches, graphs, repacks, iterations;
     int tile_rows[4], tile_z[4];/* chosen tiles: thermal FP64, Darcy FP64, thermal FP32, Darcy FP32 */
+    int tile_cpl[4];            /* their x cells per lane                                 */
 } ht3d_batch_stats;
 
 int ht3d_batch_create(ht3d_hrl *const *models, int nprob, ht3d_batch **out);
@@ -535,6 +536,15 @@ int ht3d_batch_set_params(ht3d_batch *b, const ht3d_dyrel_params *pT, const ht3d
 int ht3d_batch_step(ht3d_batch *b, ht3d_step_report *reps);
 int ht3d_batch_stats_get(const ht3d_batch *b, ht3d_batch_stats *s);
 int ht3d_batch_stats_reset(ht3d_batch *b);
+/* The batch's own tile table: (rows per block, z cells per thread, x cells per
+ * lane), the 2- and 4-cell tiles offered only when every problem's ncx is a
+ * multiple of the count (4 only for the FP32 sweeps).  By default each solve
+ * kind times every tile the batch can run on its first step;
+ * ht3d_batch_tune_force(t) pins entry t for batches created or re-tuned
+ * afterwards (a tile a batch cannot run falls back to the default), -1
+ * restores timing.  No reference counterpart (the reference has no batch).   */
+int ht3d_batch_tune_force(int tile);
+int ht3d_batch_tile_count(void);
 
 /* Lognormal permeability on vertices (spectral randomisation) plus an optional
  * planar fault; K (3*M) and phi (M) in host memory.  Set-up only.            */

@@ -20,11 +20,11 @@ SIZES = [(16, 8, 16), (24, 12, 20), (20, 20, 20), (13, 9, 11), (32, 16, 24), (12
 TOL = 1e-10
 
 
-def _workloads():
+def _workloads(sizes=SIZES):
     from hydrothermal3d_amd import inputs
-    ws = inputs.batch_workloads(len(SIZES), seed=13000, steps=3)
+    ws = inputs.batch_workloads(len(sizes), seed=13000, steps=3)
     return [inputs.replace(w, ncx=a, ncy=b, ncz=c, Lx=1.0, Ly=b / a, Lz=c / a, modes=64)
-            for w, (a, b, c) in zip(ws, SIZES)]
+            for w, (a, b, c) in zip(ws, sizes)]
 
 
 def _models(ht, ws, p, precision):
@@ -50,8 +50,8 @@ def _fields(m):
             "Vz": m["Vz"].get()}
 
 
-def _run(ht, precision, graphs, nsteps=3):
-    ws = _workloads()
+def _run(ht, precision, graphs, nsteps=3, sizes=SIZES):
+    ws = _workloads(sizes)
     p = ht.DyrelParams(nitmax=20000, nitout=100, ncheck=25, norm=1, tol=1e-9)
     ms = _models(ht, ws, p, precision)
     b = ht.HydroThermal3DBatch(ms, p, p, precision=precision, graphs=graphs)
@@ -123,3 +123,44 @@ def test_batch_nan_raises(ht):
     b.close()
     for m in ms:
         m.close()
+
+
+# every ncx a multiple of 4 (the 2- and 4-cell batch tiles run), 136 leaving a
+# partial x tile for both
+EVEN = [(16, 8, 16), (24, 12, 20), (36, 20, 12), (136, 8, 6), (12, 12, 12)]
+BATCH_TILE_CPL = (1, 1, 1, 1, 1, 1, 2, 2, 2, 4, 4, 4)
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32dc"])
+def test_batch_tiles_bitwise(ht, precision):
+    """Each batch tile pinned in turn (ht3d_batch_tune_force).  The tiles with 2
+    or 4 x-cells per lane compute every cell with the one-cell operations in
+    the same order, so steps, reports and fields equal the default tile's bit
+    for bit.  4 cells per lane is not offered to FP64 sweeps: pinned there, the
+    batch falls back to the default tile."""
+    n = ht._lib.lib().ht3d_batch_tile_count()
+    assert n == len(BATCH_TILE_CPL)
+    ref = None
+    try:
+        for t in [1] + [t for t in range(n) if t != 1]:
+            ht._lib.call("ht3d_batch_tune_force", t)
+            reps, out, stats = _run(ht, precision, graphs=True, nsteps=2, sizes=EVEN)
+            cpl = BATCH_TILE_CPL[t] if (precision == 1 or BATCH_TILE_CPL[t] <= 2) else 1
+            used = stats["tile_cpl"][2:] if precision == 1 else stats["tile_cpl"][:2]
+            assert used == [cpl, cpl], (t, stats["tile_cpl"])
+            if ref is None:
+                ref = (reps, out)
+                continue
+            for stepr, stepo in zip(reps, ref[0]):
+                for a, c in zip(stepr, stepo):
+                    for kind in ("thermal", "darcy"):
+                        for key in ("iters", "status", "n_lambda", "n_checks", "lmin", "nF_abs"):
+                            assert a[kind][key] == c[kind][key], (t, kind, key)
+                    assert a["Nu"] == c["Nu"] and a["dt_next"] == c["dt_next"], t
+            for a, c in zip(out, ref[1]):
+                for k in a:
+                    assert np.array_equal(a[k], c[k]), (t, k)
+    finally:
+        ht._lib.call("ht3d_batch_tune_force", -1)
+    with pytest.raises(ht._lib.HT3DError):
+        ht._lib.call("ht3d_batch_tune_force", n)
