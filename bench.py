@@ -414,6 +414,7 @@ def main(argv=None) -> int:
     ap.add_argument("--forward-only", action="store_true",
                     help="sweep every launch forwards (default: alternate the direction between launches)")
     ap.add_argument("--v18-only", action="store_true", help="run only the v18 leg and print its JSON")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched leg and print its JSON")
     ap.add_argument("--v18-iters", type=int, default=400,
                     help="pseudo-iterations per fused v18 solve in the v18 leg (0: skip it)")
     ap.add_argument("--probe", action="store_true",
@@ -438,6 +439,11 @@ def main(argv=None) -> int:
         return 0
     ncheck = -1 if args.check == "auto" else int(args.check)
     prm = ht.production_params(tol=args.tol, ncheck=ncheck)
+    if args.batch_only:
+        log(f"batched leg: {max(args.batch, 1)} problems")
+        print(json.dumps({"batch": run_batch(max(args.batch, 1), args.batch_steps, args.batch_warmup,
+                                             ht.production_params(tol=args.batch_tol, ncheck=ncheck))}), flush=True)
+        return 0
     precision = _lib.FP32_DC if args.precision == "dc" else _lib.FP64
     log(f"device: {_lib.device_name()}; workload {w.name}; precision {args.precision}")
     wall, st, reports, kfv, fields = run_gpu(w, args.steps, args.warmup, precision, prm)
