@@ -306,10 +306,15 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     float *f0buf[2] = {(float *)f0, (float *)f0b};
     int cur = 0, fc = 0;
     bool rev = false;   // sweep direction of the next launch
+    // The anchor cost feeds the check-interval tuner: it is measured (the second
+    // anchor, on a drained stream) on the first solve of a (grid, operator) and
+    // whenever that tuner is in use.
+    const bool timed = te && (te->anchor_us <= 0.0 || p->ncheck < 0);
     auto plain = [&](int it) { return !lam_iter(it, p->nitout) && !(it < p->nitmax && lam_iter(it + 1, p->nitout)); };
     for (outer = 0;; ++outer) {
         // anchor: FP64 residual, FP32 right-hand side, FP64 norm
-        if (outer == 1 && te) HIP_TRY(hipStreamSynchronize(cx.stream));   // time the second anchor alone
+        const bool time_anchor = outer == 1 && timed;
+        if (time_anchor) HIP_TRY(hipStreamSynchronize(cx.stream));   // time the second anchor alone
         const auto ta = std::chrono::steady_clock::now();
         if (res2) {
             if (f->transient) hipLaunchKernelGGL(dc_residual_kernel2<true>, rgrid, dim3(256), 0, cx.stream, d);
@@ -328,7 +333,7 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
         // them) go in this way, and not while the anchor is being timed.
         const int run = (p->nitmax - inner_total) < anchor ? (p->nitmax - inner_total) : anchor;
         int spec = 0;   // sweeps already queued for this run
-        if (run > 0 && !(outer == 1 && te) && plain(inner_total + 1)) {
+        if (run > 0 && !time_anchor && plain(inner_total + 1)) {
             const int it = inner_total + 1;
             timer.before_launch();
             if (pcfg.tile >= 0 && run > 1 && it + 1 < p->nitmax && plain(it + 1)) {
@@ -356,7 +361,7 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
         }
         double r4[4];
         if ((rc = wait_result4(r4))) return rc;
-        if (outer == 1 && te) {
+        if (time_anchor) {
             const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - ta).count();
             te->anchor_us = te->anchor_us > 0 && te->anchor_us < us ? te->anchor_us : us;
         }
