@@ -137,3 +137,30 @@ def test_dc_anchor_intervals(ht, transient, ncheck):
     if not transient:
         a, b = a - a.mean(), b - b.mean()
     assert rel_err(a, b) <= 1e-6
+
+
+@pytest.mark.parametrize("pairs", [0, 2], ids=["single", "pairs"])
+def test_dc_status_nitmax_and_nan(ht, pairs):
+    """The defect-correction solve stops at nitmax with status 10 and exactly
+    nitmax pseudo-iterations, also when nitmax is not a multiple of the anchor
+    interval (a last run shorter than the interval, possibly queued whole
+    before the anchor norm is read), and NaN data raises like the FP64 solve
+    (HydroThermal3D_v18.jl:464)."""
+    grid = (16, 9, 12)
+    X, R, k = _problem(grid, 0, 5)
+    h = dict(dx=0.1, dy=0.1, dz=0.1, dt=1.0)
+    ht._lib.call("ht3d_tune_pairs", pairs)
+    try:
+        for nitmax, ncheck in ((3, 50), (7, 3), (50, 7), (41, 4)):
+            D = [ht.DeviceArray.from_host(a) for a in (X, R, *k)]
+            prm = ht.DyrelParams(nitmax=nitmax, nitout=1000, ncheck=ncheck, tol=0.0)
+            rep = ht.dyrel_solve(*D, transient=0, rhoC=0.0, params=prm, precision=1, **h)
+            assert rep["status"] == 10 and rep["iters"] == nitmax, (nitmax, ncheck, rep)
+        Rn = np.asfortranarray(R.copy())
+        Rn[3, 3, 3] = np.nan
+        D = [ht.DeviceArray.from_host(a) for a in (X, Rn, *k)]
+        with pytest.raises(ht.SolveError, match="Nan"):
+            ht.dyrel_solve(*D, transient=0, rhoC=0.0, params=ht.DyrelParams(nitmax=50, nitout=1000, ncheck=7),
+                           precision=1, **h)
+    finally:
+        ht._lib.call("ht3d_tune_pairs", 1)
