@@ -1084,4 +1084,81 @@ __device__ __forceinline__ double dc_residual_cell(const DcArgs &a, int i, int j
     return f;
 }
 
+// The FP64 defect-correction residual r(x) of two x-neighbouring cells (i,
+// i+1; nx even) marched over the planes [k0, k1): X, the face coefficients, R
+// and r32 as 2-element vector loads and stores, X(k-1), X(k), X(k+1) and
+// kz(k), kz(k+1) kept in registers, the cell formula and order of
+// dc_residual_cell.  Adds |r| to the double-double sum (s_hi, s_lo) and to the
+// max mx.  The march is unrolled by UNROLL.
+template <bool TR, int UNROLL>
+__device__ __forceinline__ void dc_residual_march2(const DcArgs &a, int i, int j, int k0, int k1, double &s_hi,
+                                                   double &s_lo, double &mx) {
+    typedef CellVec<double, 2>::type V;
+    typedef CellVec<float, 2>::type F2;
+    const uint32_t nx = a.nx, ny = a.ny;
+    const uint32_t sE1 = nx + 2, sE2 = sE1 * (ny + 2), sC2 = nx * ny, sX2 = (nx + 1) * ny, sY2 = nx * (ny + 1);
+    uint32_t e = (uint32_t)(i + 1) + (uint32_t)(j + 1) * sE1 + (uint32_t)(k0 + 1) * sE2;
+    uint32_t c = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k0 * sC2;
+    uint32_t fx = (uint32_t)i + (uint32_t)j * (nx + 1) + (uint32_t)k0 * sX2;
+    uint32_t fy = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k0 * sY2;
+    V xc = ldv<double, 2>(a.X + e), xB = ldv<double, 2>(a.X + e - sE2), kB = ldv<double, 2>(a.kz + c);
+    if (k0 == 0) {
+        xB.x = ghost(a.bct[4], a.bcv[4], (double)xc.x);
+        xB.y = ghost(a.bct[4], a.bcv[4], (double)xc.y);
+    }
+    constexpr int kUnroll = UNROLL;
+#pragma unroll kUnroll
+    for (int k = k0; k < k1; ++k) {
+        V xF = ldv<double, 2>(a.X + e + sE2);
+        if (k == a.nz - 1) {
+            xF.x = ghost(a.bct[5], a.bcv[5], (double)xc.x);
+            xF.y = ghost(a.bct[5], a.bcv[5], (double)xc.y);
+        }
+        double xW = a.X[e - 1], xE = a.X[e + 2];
+        V xS = ldv<double, 2>(a.X + e - sE1), xN = ldv<double, 2>(a.X + e + sE1);
+        if (i == 0) xW = ghost(a.bct[0], a.bcv[0], (double)xc.x);
+        if ((uint32_t)i + 2 == nx) xE = ghost(a.bct[1], a.bcv[1], (double)xc.y);
+        if (j == 0) {
+            xS.x = ghost(a.bct[2], a.bcv[2], (double)xc.x);
+            xS.y = ghost(a.bct[2], a.bcv[2], (double)xc.y);
+        }
+        if ((uint32_t)j == ny - 1) {
+            xN.x = ghost(a.bct[3], a.bcv[3], (double)xc.x);
+            xN.y = ghost(a.bct[3], a.bcv[3], (double)xc.y);
+        }
+        const double kW = a.kx[fx];
+        const V kE = ldv<double, 2>(a.kx + fx + 1);
+        const V kS = ldv<double, 2>(a.ky + fy), kN = ldv<double, 2>(a.ky + fy + nx), kF = ldv<double, 2>(a.kz + c + sC2);
+        const V R = ldv<double, 2>(a.R + c);
+        V f;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const double x0 = xc[q], w = q == 0 ? xW : (double)xc[0], ea = q == 1 ? xE : (double)xc[1];
+            const double kw = q == 0 ? kW : (double)kE[0];
+            const double qW = -kw * (a._dx * (x0 - w)), qE = -kE[q] * (a._dx * (ea - x0));
+            const double qS = -kS[q] * (a._dy * (x0 - xS[q])), qN = -kN[q] * (a._dy * (xN[q] - x0));
+            const double qB = -kB[q] * (a._dz * (x0 - xB[q])), qF = -kF[q] * (a._dz * (xF[q] - x0));
+            if (TR) f[q] = R[q] + a._dt * x0 + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
+            else f[q] = R[q] + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
+        }
+        F2 r;
+        r.x = (float)f.x;
+        r.y = (float)f.y;
+        stv<float, 2>(r, a.r32 + c);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const double af = fabs((double)f[q]);
+            dd_add(s_hi, s_lo, af);
+            mx = (af > mx || af != af) ? af : mx;
+        }
+        xB = xc;
+        xc = xF;
+        kB = kF;
+        e += sE2;
+        c += sC2;
+        fx += sX2;
+        fy += sY2;
+    }
+}
+
 }  // namespace ht3d
