@@ -335,11 +335,16 @@ int ht3d_DYRELUpdate(double *F0, double *X, const double *F, double h1, double h
     });
 }
 
+// SwapDYREL!  kernels_HT3D_dimensional.jl:122-126: Xc = inn(X0c_ex), Yc = Y0c
 int ht3d_SwapDYREL(double *Xc, double *Yc, const double *X0c_ex, const double *Y0c, int nx, int ny, int nz) {
-    (void)Xc; (void)X0c_ex;
     CHECK_GRID(nx, ny, nz);
-    CHECK_PTR(Yc); CHECK_PTR(Y0c);
-    return ht3d_ArrayEqualArray(Yc, Y0c, nx, ny, nz);
+    CHECK_PTR(Xc); CHECK_PTR(Yc); CHECK_PTR(X0c_ex); CHECK_PTR(Y0c);
+    Grid g{nx, ny, nz};
+    return launch3d("SwapDYREL", nx, ny, nz, [=] __device__(int i, int j, int k) {
+        const size_t c = g.c(i, j, k);
+        Xc[c] = X0c_ex[g.e(i + 1, j + 1, k + 1)];
+        Yc[c] = Y0c[c];
+    });
 }
 
 // the Gershgorin row sum of notebooks/Gershgorin_TH3D.ipynb (cell 2 output)

@@ -257,10 +257,13 @@ EXPORT void o_DYRELUpdate(double *F0, double *X, const double *F, double h1, dou
     }
 }
 
-/* SwapDYREL!  kernels_HT3D_dimensional.jl:122-126.  Only the residual copy is
- * performed; the first argument is left untouched exactly as in the reference. */
+/* SwapDYREL!  kernels_HT3D_dimensional.jl:122-126: @all(Xc) = @inn(X0c_ex),
+ * @all(Yc) = @all(Y0c).  The solve loops below pass Xc = NULL: the drivers
+ * never read Xcit (HydroThermal3D_v18.jl:322,451,509), so only Fcit is kept. */
 EXPORT void o_SwapDYREL(double *Xc, double *Yc, const double *X0c_ex, const double *Y0c, int nx, int ny, int nz) {
-    (void)Xc; (void)X0c_ex;
+    if (Xc) {
+        PFOR_K(nz) FOR_JI(nx, ny) Cc(Xc, i, j, k) = Ee(X0c_ex, i + 1, j + 1, k + 1);
+    }
     memcpy(Yc, Y0c, (size_t)nx * ny * nz * sizeof(double));
 }
 

@@ -246,20 +246,6 @@ def test_gershgorin_uniform(orc, rhoC, transient):
         assert np.all(G == 2.0)
 
 
-def test_dyrel_scalars(orc):
-    """v18:433-439: lambda_min = lambda_max/500, dtau = 2/sqrt(lambda_max)*CFL,
-    c = 2 sqrt(lambda_min) cfact, h1 = (2 - c dtau)/(2 + c dtau), h2 = 2 dtau/(2 + c dtau)."""
-    X, R, k = _problem((5, 5, 5), 0, 1, sigma=0.0)
-    rep, _ = orc.dyrel_solve(X, R, *k, dx=0.1, dy=0.1, dz=0.1, dt=1.0, transient=0, rhoC=0.0,
-                             p=orc.params(nitmax=1, tol=0.0))
-    lmax = 2.0
-    lmin = lmax / 500
-    dtau = 2.0 / math.sqrt(lmax) * 0.99
-    c = 2.0 * math.sqrt(lmin) * 0.5
-    assert rep["lmax"] == lmax and rep["dtau"] == dtau
-    assert rep["h1"] == (2 - c * dtau) / (2 + c * dtau) and rep["h2"] == 2 * dtau / (2 + c * dtau)
-
-
 def test_rayleigh_quotient_on_eigenvector(orc):
     """RayleighQuotient{Numerator,Denominator}! (kernels_HT3D_dimensional.jl:349-363)
     return the eigenvalue when the update direction is an eigenvector of the
@@ -449,16 +435,6 @@ def test_mean_pressure_preserved(orc):
     orc.dyrel_solve(X, R, *k, dx=0.1, dy=0.1, dz=0.1, dt=1.0, transient=0, rhoC=0.0,
                     p=orc.params(nitmax=300, tol=0.0))
     assert abs(X[1:-1, 1:-1, 1:-1].mean() - m0) < 1e-12
-
-
-def test_swapdyrel_leaves_first_argument(orc):
-    """SwapDYREL! (kernels_HT3D_dimensional.jl:122-126) only copies the residual."""
-    rng = np.random.default_rng(1)
-    sh = shapes(4, 4, 4)
-    Xc, Yc, X0, Y0 = rand(sh["c"], rng), rand(sh["c"], rng), rand(sh["E"], rng), rand(sh["c"], rng)
-    Xc0 = Xc.copy()
-    orc.SwapDYREL(Xc, Yc, X0, Y0)
-    assert np.array_equal(Xc, Xc0) and np.array_equal(Yc, Y0)
 
 
 # ---------------------------------------------------------------- failure handling (CPU)
