@@ -17,7 +17,7 @@ HEADER = ROOT / "include" / "ht3d.h"
 LIBPATH = Path(os.environ.get("HT3D_LIB", PKG / "libht3d.so"))
 
 # status codes (include/ht3d.h)
-OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_STATE = 0, 1, 2, 3, 4
+OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_INTERNAL = 0, 1, 2, 3, 4, 5
 NOT_CONVERGED, NAN, DIVERGED = 10, 11, 12
 SUM, SUM_ABS, MAX, MIN, MAX_ABS = 0, 1, 2, 3, 4
 FP64, FP32_DC = 0, 1

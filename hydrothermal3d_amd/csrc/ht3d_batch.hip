@@ -1017,6 +1017,7 @@ static int batch_solve(ht3d_batch *b, int kind, const ht3d_dyrel_params &p_in, c
 extern "C" {
 
 int ht3d_batch_create(ht3d_hrl *const *models, int nprob, ht3d_batch **out) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     if (!models || nprob < 1 || !out) return arg_fail("ht3d_batch_create: bad argument");
     for (int q = 0; q < nprob; ++q)
@@ -1105,9 +1106,11 @@ int ht3d_batch_create(ht3d_hrl *const *models, int nprob, ht3d_batch **out) {
     if (e != hipSuccess) { ht3d_batch_destroy(b); return hip_fail(e, "ht3d_batch_create"); }
     *out = b;
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_batch_destroy(ht3d_batch *b) {
+    HT3D_API_BEGIN
     if (!b) return HT3D_OK;
     if (ctx().initialised) (void)hipStreamSynchronize(ctx().stream);
     clear_graphs(b);
@@ -1125,10 +1128,12 @@ int ht3d_batch_destroy(ht3d_batch *b) {
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     delete b;
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_batch_set_params(ht3d_batch *b, const ht3d_dyrel_params *pT, const ht3d_dyrel_params *pP, int precision,
                           int use_graphs) {
+    HT3D_API_BEGIN
     if (!b) return arg_fail("ht3d_batch_set_params: null batch");
     if (precision != HT3D_FP64 && precision != HT3D_FP32_DC) return arg_fail("ht3d_batch_set_params: bad precision");
     for (const ht3d_dyrel_params *p : {pT, pP})
@@ -1140,9 +1145,11 @@ int ht3d_batch_set_params(ht3d_batch *b, const ht3d_dyrel_params *pT, const ht3d
     b->graphs = use_graphs != 0;
     clear_graphs(b);  // the graphs carry the parameters as launch arguments
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_batch_step(ht3d_batch *b, ht3d_step_report *reps) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     if (!b) return arg_fail("ht3d_batch_step: null batch");
     const auto t0 = std::chrono::steady_clock::now();
@@ -1203,9 +1210,11 @@ int ht3d_batch_step(ht3d_batch *b, ht3d_step_report *reps) {
     b->stats.ms_step += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     ++b->stats.steps;
     return worst;
+    HT3D_API_END
 }
 
 int ht3d_batch_stats_get(const ht3d_batch *b, ht3d_batch_stats *s) {
+    HT3D_API_BEGIN
     if (!b || !s) return arg_fail("ht3d_batch_stats_get: bad argument");
     *s = b->stats;
     const int tiles[4] = {b->tile64[0], b->tile64[1], b->tile32[0], b->tile32[1]};
@@ -1219,20 +1228,25 @@ int ht3d_batch_stats_get(const ht3d_batch *b, ht3d_batch_stats *s) {
     s->tile_z[2] = b->tile32[0] >= 0 ? kBTiles[b->tile32[0]].kz : 0;
     s->tile_z[3] = b->tile32[1] >= 0 ? kBTiles[b->tile32[1]].kz : 0;
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_batch_tune_force(int tile) {
+    HT3D_API_BEGIN
     if (tile >= kNumBTiles) return arg_fail("ht3d_batch_tune_force: tile index past the end of the table");
     g_force_btile = tile < 0 ? -1 : tile;
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_batch_tile_count(void) { return kNumBTiles; }
 
 int ht3d_batch_stats_reset(ht3d_batch *b) {
+    HT3D_API_BEGIN
     if (!b) return arg_fail("ht3d_batch_stats_reset: null batch");
     std::memset(&b->stats, 0, sizeof b->stats);
     return HT3D_OK;
+    HT3D_API_END
 }
 
 }  // extern "C"

@@ -386,9 +386,11 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
 
 extern "C" int ht3d_dyrel_solve_dc(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep,
                                    int *outer_iters, int *inner_iters) {
+    HT3D_API_BEGIN
     int o = 0, i = 0;
     int rc = ht3d::dc_solve_impl(f, p, rep, nullptr, &o, &i);
     if (outer_iters) *outer_iters = o;
     if (inner_iters) *inner_iters = i;
     return rc;
+    HT3D_API_END
 }

@@ -44,6 +44,7 @@ extern "C" {
 
 int ht3d_ComputeThermalConductivity(double *ktv, const double *Tce, const double *phv, double phi0, double scale_kt,
                                     double scale_T, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(ktv); CHECK_PTR(Tce); CHECK_PTR(phv);
     Grid g{nx, ny, nz};
@@ -53,12 +54,14 @@ int ht3d_ComputeThermalConductivity(double *ktv, const double *Tce, const double
         const double ph = phv[g.v(i, j, k)];
         ktv[g.v(i, j, k)] = ((1.0 - phi_of(ph, phi0)) * kTs(Tv) + phi_of(ph, phi0) * kTf(Tv)) / scale_kt;
     });
+    HT3D_API_END
 }
 
 int ht3d_ComputeHydroConductivity(double *kfv, const double *Tce, const double *Pce, const double *phv, double ymin,
                                   double dy, double k_fact, double delta, double scale_sigma, double scale_T,
                                   double scale_L, double scale_t, double scale_eta, int mode, int nx, int ny,
                                   int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(kfv); CHECK_PTR(Tce); CHECK_PTR(Pce); CHECK_PTR(phv);
     Grid g{nx, ny, nz};
@@ -80,10 +83,12 @@ int ht3d_ComputeHydroConductivity(double *kfv, const double *Tce, const double *
         if (phv[n] == 2.0) out *= k_fact;
         kfv[n] = out;
     });
+    HT3D_API_END
 }
 
 int ht3d_ComputeFluidDensity(double *rhofv, const double *Tce, const double *Pce, const double *phv,
                              double scale_sigma, double scale_T, double scale_rho, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(rhofv); CHECK_PTR(Tce); CHECK_PTR(Pce); CHECK_PTR(phv);
     Grid g{nx, ny, nz};
@@ -95,29 +100,35 @@ int ht3d_ComputeFluidDensity(double *rhofv, const double *Tce, const double *Pce
         const size_t n = g.v(i, j, k);
         rhofv[n] = phv[n] == 1.0 ? 0.0 / scale_rho : rhof_C(Tv - 273.15, Pv) / scale_rho;
     });
+    HT3D_API_END
 }
 
 int ht3d_SmoothConductivityV2C(double *ktc, const double *ktv, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(ktc); CHECK_PTR(ktv);
     Grid g{nx, ny, nz};
     return launch3d("SmoothConductivityV2C", nx, ny, nz, [=] __device__(int i, int j, int k) {
         ktc[g.c(i, j, k)] = cube_mean(ktv, [=](int a, int b, int c) { return g.v(a, b, c); }, i, j, k);
     });
+    HT3D_API_END
 }
 
 int ht3d_SmoothConductivityC2V(double *ktv, const double *ktc, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(ktc); CHECK_PTR(ktv);
     Grid g{nx, ny, nz};
     return launch3d("SmoothConductivityC2V", nx - 1, ny - 1, nz - 1, [=] __device__(int i, int j, int k) {
         ktv[g.v(i + 1, j + 1, k + 1)] = cube_mean(ktc, [=](int a, int b, int c) { return g.c(a, b, c); }, i, j, k);
     });
+    HT3D_API_END
 }
 
 int ht3d_ComputeRhoCeffective(double *rhoC_eff, const double *Tc_ex, const double *Pc_ex, const double *phc_ex,
                               double phi_i, double sca_T, double sca_sigma, double sca_C, double sca_rho, int nx,
                               int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(rhoC_eff); CHECK_PTR(Tc_ex); CHECK_PTR(Pc_ex); CHECK_PTR(phc_ex);
     Grid g{nx, ny, nz};
@@ -132,6 +143,7 @@ int ht3d_ComputeRhoCeffective(double *rhoC_eff, const double *Tc_ex, const doubl
         const double Ceff = phi * Cf(Tsca) + (1.0 - phi) * Cs(Tsca);
         rhoC_eff[g.c(i, j, k)] = (rho_eff * Ceff) / sca_rho / sca_C;
     });
+    HT3D_API_END
 }
 
 // ϕρf, ϕρf_old and ϕdρdP!: cells whose phase is 1 are left unchanged.
@@ -154,24 +166,31 @@ static int phirhof(const char *name, double *out, const double *T, const double 
 
 int ht3d_PhiRhof(double *phirhofc, const double *Tc_ex, const double *Pc_ex, const double *phc_ex, double phi_i,
                  double scale_rho, double scale_T, double scale_sigma, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     return phirhof("PhiRhof", phirhofc, Tc_ex, Pc_ex, true, phc_ex, phi_i, scale_rho, scale_T, scale_sigma, false,
                    nx, ny, nz);
+    HT3D_API_END
 }
 
 int ht3d_PhiRhof_old(double *phirhofc, const double *Tc0, const double *Pc0, const double *phc_ex, double phi_i,
                      double scale_rho, double scale_T, double scale_sigma, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     return phirhof("PhiRhof_old", phirhofc, Tc0, Pc0, false, phc_ex, phi_i, scale_rho, scale_T, scale_sigma, false,
                    nx, ny, nz);
+    HT3D_API_END
 }
 
 int ht3d_PhiDrhoDP(double *phidrhodP, const double *Tc_ex, const double *Pc_ex, const double *phc_ex, double phi_i,
                    double scale_rho, double scale_T, double scale_sigma, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     return phirhof("PhiDrhoDP", phidrhodP, Tc_ex, Pc_ex, true, phc_ex, phi_i, scale_rho, scale_T, scale_sigma, true,
                    nx, ny, nz);
+    HT3D_API_END
 }
 
 int ht3d_dim_Init_vel(double *Vx, double *Vy, double *Vz, const double *qx, const double *qy, const double *qz,
                       const double *rhofv, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Vx); CHECK_PTR(Vy); CHECK_PTR(Vz); CHECK_PTR(qx); CHECK_PTR(qy); CHECK_PTR(qz); CHECK_PTR(rhofv);
     Grid g{nx, ny, nz};
@@ -192,28 +211,33 @@ int ht3d_dim_Init_vel(double *Vx, double *Vy, double *Vz, const double *qx, cons
                           rhofv[g.v(i + 1, j + 1, k)]) * 0.25;
         Vz[g.fz(i, j, k)] = qz[g.fz(i, j, k)] / r;
     });
+    HT3D_API_END
 }
 
 int ht3d_dim_SetTemperatureBCs(double *Tc_ex, const double *ky, const double *wN, const double *phc_ex, double qyS,
                                double _dy, double kS, double TS, double TN, double y_plateau, double a2, double b2,
                                double dTdy, double dx, double dy, double sticky_air, double sca_T, double sca_kT,
                                double phi_i, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     (void)kS; (void)TS; (void)y_plateau; (void)a2; (void)b2; (void)dTdy; (void)dx; (void)dy; (void)sticky_air;
     (void)sca_T; (void)sca_kT; (void)phi_i;
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Tc_ex); CHECK_PTR(ky); CHECK_PTR(wN); CHECK_PTR(phc_ex);
     BcArgs a{Tc_ex, ky, wN, phc_ex, TN, qyS / _dy, Grid{nx, ny, nz}, 1};
     return bc_launch("dim_SetTemperatureBCs", a, Tc_ex);
+    HT3D_API_END
 }
 
 int ht3d_dim_SetPressureBCs(double *Pc_ex, const double *wN, const double *phc_ex, double PS, double PN,
                             double y_plateau, double a2, double b2, double rhof, double g_, double dx, double dy,
                             double sticky_air, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     (void)PS; (void)y_plateau; (void)a2; (void)b2; (void)rhof; (void)g_; (void)dx; (void)dy; (void)sticky_air;
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Pc_ex); CHECK_PTR(wN); CHECK_PTR(phc_ex);
     BcArgs a{Pc_ex, nullptr, wN, phc_ex, PN, 0.0, Grid{nx, ny, nz}, 0};
     return bc_launch("dim_SetPressureBCs", a, Pc_ex);
+    HT3D_API_END
 }
 
 // ---------------------------------------------------------------- output fields
@@ -221,6 +245,7 @@ int ht3d_dim_SetPressureBCs(double *Pc_ex, const double *wN, const double *phc_e
 // NaN in air unless noted.
 
 int ht3d_Phase(double *X, const double *phc_ex, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(X); CHECK_PTR(phc_ex);
     Grid g{nx, ny, nz};
@@ -228,6 +253,7 @@ int ht3d_Phase(double *X, const double *phc_ex, int nx, int ny, int nz) {
         const double ph = phc_ex[g.e(i + 1, j + 1, k + 1)];
         X[g.c(i, j, k)] = ph != 1.0 ? ph : NAN;
     });
+    HT3D_API_END
 }
 
 static int scaled_field(const char *name, double *X, const double *F, const double *phc_ex, double scale, int nx,
@@ -242,15 +268,20 @@ static int scaled_field(const char *name, double *X, const double *F, const doub
 }
 
 int ht3d_Pressure(double *X, const double *Pc_ex, const double *phc_ex, double scale_sigma, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     return scaled_field("Pressure", X, Pc_ex, phc_ex, scale_sigma, nx, ny, nz);
+    HT3D_API_END
 }
 
 int ht3d_Temperature(double *X, const double *Tc_ex, const double *phc_ex, double scale_T, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     return scaled_field("Temperature", X, Tc_ex, phc_ex, scale_T, nx, ny, nz);
+    HT3D_API_END
 }
 
 int ht3d_Velocity(double *X, double *Y, double *Z, const double *Vx, const double *Vy, const double *Vz,
                   const double *phc_ex, double scale_V, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(X); CHECK_PTR(Y); CHECK_PTR(Z); CHECK_PTR(Vx); CHECK_PTR(Vy); CHECK_PTR(Vz); CHECK_PTR(phc_ex);
     Grid g{nx, ny, nz};
@@ -264,10 +295,12 @@ int ht3d_Velocity(double *X, double *Y, double *Z, const double *Vx, const doubl
             X[c] = NAN;   // Y and Z are left as they are, as in the reference
         }
     });
+    HT3D_API_END
 }
 
 int ht3d_EffectiveThermalConductivity(double *X, const double *Tc_ex, double phi0, const double *phc_ex,
                                       double scale_T, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(X); CHECK_PTR(Tc_ex); CHECK_PTR(phc_ex);
     Grid g{nx, ny, nz};
@@ -281,11 +314,13 @@ int ht3d_EffectiveThermalConductivity(double *X, const double *Tc_ex, double phi
             X[g.c(i, j, k)] = NAN;
         }
     });
+    HT3D_API_END
 }
 
 int ht3d_EffectiveHeatCapacity(double *Ceff, const double *Tc_ex, const double *Pc_ex, const double *phc_ex,
                                double phi_i, double sca_T, double sca_sigma, double sca_C, double sca_rho, int nx,
                                int ny, int nz) {
+    HT3D_API_BEGIN
     (void)phi_i; (void)sca_sigma; (void)sca_C; (void)sca_rho;
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Ceff); CHECK_PTR(Tc_ex); CHECK_PTR(Pc_ex); CHECK_PTR(phc_ex);
@@ -294,10 +329,12 @@ int ht3d_EffectiveHeatCapacity(double *Ceff, const double *Tc_ex, const double *
         const size_t e = g.e(i + 1, j + 1, k + 1);
         if (phc_ex[e] != 1.0) Ceff[g.c(i, j, k)] = Cs(Tc_ex[e] * sca_T);
     });
+    HT3D_API_END
 }
 
 int ht3d_Permeability(double *X, double k_fact, double ymin, double dy, const double *phc_ex, double delta,
                       double scale_L, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(X); CHECK_PTR(phc_ex);
     Grid g{nx, ny, nz};
@@ -312,10 +349,12 @@ int ht3d_Permeability(double *X, double k_fact, double ymin, double dy, const do
             X[g.c(i, j, k)] = NAN;
         }
     });
+    HT3D_API_END
 }
 
 int ht3d_FluidDensity(double *X, const double *Tc_ex, const double *Pc_ex, const double *phc_ex, double scale_T,
                       double scale_sigma, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(X); CHECK_PTR(Tc_ex); CHECK_PTR(Pc_ex); CHECK_PTR(phc_ex);
     Grid g{nx, ny, nz};
@@ -323,9 +362,11 @@ int ht3d_FluidDensity(double *X, const double *Tc_ex, const double *Pc_ex, const
         const size_t e = g.e(i + 1, j + 1, k + 1);
         X[g.c(i, j, k)] = phc_ex[e] != 1.0 ? rhof_C(Tc_ex[e] * scale_T - 273.15, Pc_ex[e] * scale_sigma) : NAN;
     });
+    HT3D_API_END
 }
 
 int ht3d_Viscosity(double *X, const double *Tc_ex, const double *phc_ex, double scale_T, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(X); CHECK_PTR(Tc_ex); CHECK_PTR(phc_ex);
     Grid g{nx, ny, nz};
@@ -333,11 +374,13 @@ int ht3d_Viscosity(double *X, const double *Tc_ex, const double *phc_ex, double 
         const size_t e = g.e(i + 1, j + 1, k + 1);
         X[g.c(i, j, k)] = phc_ex[e] != 1.0 ? muf(Tc_ex[e] * scale_T) : NAN;
     });
+    HT3D_API_END
 }
 
 int ht3d_Peclet(double *X, const double *Tc_ex, const double *Pc_ex, const double *Vx, const double *Vy,
                 const double *Vz, const double *phc_ex, double phi0, double scale_T, double scale_sigma,
                 double scale_V, double scale_L, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(X); CHECK_PTR(Tc_ex); CHECK_PTR(Pc_ex); CHECK_PTR(Vx); CHECK_PTR(Vy); CHECK_PTR(Vz); CHECK_PTR(phc_ex);
     Grid g{nx, ny, nz};
@@ -359,6 +402,7 @@ int ht3d_Peclet(double *X, const double *Tc_ex, const double *Pc_ex, const doubl
             X[g.c(i, j, k)] = NAN;
         }
     });
+    HT3D_API_END
 }
 
 }  // extern "C"

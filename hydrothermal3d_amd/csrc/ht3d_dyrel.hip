@@ -357,76 +357,98 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
 extern "C" {
 
 int ht3d_dyrel_solve(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep) {
+    HT3D_API_BEGIN
     return ht3d::dyrel_solve_impl(f, p, rep, nullptr, 1.0);
+    HT3D_API_END
 }
 
 int ht3d_tune_set(int autotune, int force_tile, int force_variant) {
+    HT3D_API_BEGIN
     if (force_tile >= ht3d::kNumTiles || force_variant < 0 || force_variant > 7)
         return ht3d::arg_fail("ht3d_tune_set: no such tile configuration or variant");
     ht3d::g_autotune = autotune != 0;
     ht3d::g_force_tile = force_tile;
     ht3d::g_force_variant = force_variant;
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_tune_clear(void) {
+    HT3D_API_BEGIN
     ht3d::g_tune.clear();
     ht3d::v18::clear_tiles();
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_tune_last(int *rows_per_block, int *z_per_thread, int *variant) {
+    HT3D_API_BEGIN
     const ht3d::IterCfg &c = ht3d::last_cfg();
     if (rows_per_block) *rows_per_block = ht3d::kTiles[c.tile].ty;
     if (z_per_thread) *z_per_thread = ht3d::kTiles[c.tile].kz;
     if (variant) *variant = c.variant;
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_tune_last_tile(int *rows_per_block, int *z_per_thread, int *x_per_thread, int *variant) {
+    HT3D_API_BEGIN
     const ht3d::IterCfg &c = ht3d::last_cfg();
     if (x_per_thread) *x_per_thread = ht3d::kTiles[c.tile].cpl;
     return ht3d_tune_last(rows_per_block, z_per_thread, variant);
+    HT3D_API_END
 }
 
 int ht3d_tile_count(void) { return ht3d::kNumTiles; }
 
 int ht3d_tune_check_interval(double iters, double t_iter_us, double t_check_us, int nitout) {
+    HT3D_API_BEGIN
     if (nitout < 1) return 0;
     return ht3d::tuned_check_interval(iters, t_iter_us, t_check_us, nitout);
+    HT3D_API_END
 }
 
 int ht3d_tune_sweep(int alternate) {
+    HT3D_API_BEGIN
     ht3d::g_alternate = alternate ? 1 : 0;
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_tune_pairs(int mode) {
+    HT3D_API_BEGIN
     if (mode < 0 || mode > 2) return ht3d::arg_fail("ht3d_tune_pairs: mode must be 0, 1 or 2");
     ht3d::g_pairs = mode;
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_tune_last_pair(int *rows_per_block, int *z_per_block, int *variant) {
+    HT3D_API_BEGIN
     const ht3d::PairCfg &c = ht3d::last_pair();
     if (rows_per_block) *rows_per_block = c.tile >= 0 ? ht3d::kPairTiles[c.tile].ty : 0;
     if (z_per_block) *z_per_block = c.tile >= 0 ? ht3d::kPairTiles[c.tile].kz : 0;
     if (variant) *variant = c.tile >= 0 ? c.variant : -1;
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_tune_last_pair_tile(int *rows_per_block, int *z_per_block, int *x_per_thread, int *variant) {
+    HT3D_API_BEGIN
     const ht3d::PairCfg &c = ht3d::last_pair();
     if (x_per_thread) *x_per_thread = c.tile >= 0 ? ht3d::kPairTiles[c.tile].cpl : 0;
     return ht3d_tune_last_pair(rows_per_block, z_per_block, variant);
+    HT3D_API_END
 }
 
 int ht3d_tune_force_pair(int tile, int variant) {
+    HT3D_API_BEGIN
     if (tile >= ht3d::kNumPairTiles || variant < 0 || variant > 3)
         return ht3d::arg_fail("ht3d_tune_force_pair: no such pair tile or variant");
     ht3d::g_force_pair_tile = tile;
     ht3d::g_force_pair_variant = variant;
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_pair_tile_count(void) { return ht3d::kNumPairTiles; }

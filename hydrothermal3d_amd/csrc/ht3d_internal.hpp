@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <new>
 #include <string>
 
 #include "../../include/ht3d.h"
@@ -35,9 +36,30 @@ struct Context {
 };
 
 Context &ctx();
-void set_error(const std::string &msg);
-int hip_fail(hipError_t e, const char *what);
-int arg_fail(const char *what);
+void set_error(const char *msg) noexcept;
+int hip_fail(hipError_t e, const char *what) noexcept;
+int arg_fail(const char *what) noexcept;
+
+// Every extern "C" entry runs its body through api_guard: a C++ exception
+// (std::bad_alloc from a host container, or anything else) becomes a status
+// code and the error text, instead of unwinding into the caller (ctypes,
+// Julia's ccall).  maybe_inject_failure throws on the next guarded calls
+// after ht3d_debug_inject_host_failure(n), so tests can force that path.
+void maybe_inject_failure();
+int guard_failure(const char *entry, bool nomem) noexcept;
+template <typename F>
+inline int api_guard(const char *entry, F &&body) noexcept {
+    try {
+        maybe_inject_failure();
+        return body();
+    } catch (const std::bad_alloc &) {
+        return guard_failure(entry, true);
+    } catch (...) {
+        return guard_failure(entry, false);
+    }
+}
+#define HT3D_API_BEGIN return ::ht3d::api_guard(__func__, [&]() -> int {
+#define HT3D_API_END });
 
 // Brackets runs of back-to-back iteration launches with events (when timing is on).
 struct SegmentTimer {

@@ -47,6 +47,7 @@ extern "C" {
 
 // ------------------------------------------------------------------ pointwise
 int ht3d_ResetA(double *A, double *B, int n1, int n2, int n3) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     CHECK_PTR(A);
     CHECK_PTR(B);
@@ -55,27 +56,33 @@ int ht3d_ResetA(double *A, double *B, int n1, int n2, int n3) {
         A[n] = 0.0;
         B[n] = 0.0;
     });
+    HT3D_API_END
 }
 
 int ht3d_Cpy_inn_to_all(double *A, const double *B, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(A);
     CHECK_PTR(B);
     Grid g{nx, ny, nz};
     return launch3d("Cpy_inn_to_all", nx, ny, nz,
                     [=] __device__(int i, int j, int k) { A[g.c(i, j, k)] = B[g.e(i + 1, j + 1, k + 1)]; });
+    HT3D_API_END
 }
 
 int ht3d_Cpy_all_to_inn(double *A, const double *B, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(A);
     CHECK_PTR(B);
     Grid g{nx, ny, nz};
     return launch3d("Cpy_all_to_inn", nx, ny, nz,
                     [=] __device__(int i, int j, int k) { A[g.e(i + 1, j + 1, k + 1)] = B[g.c(i, j, k)]; });
+    HT3D_API_END
 }
 
 int ht3d_ArrayEqualArray(double *F1, const double *F2, int n1, int n2, int n3) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     CHECK_PTR(F1);
     CHECK_PTR(F2);
@@ -83,9 +90,11 @@ int ht3d_ArrayEqualArray(double *F1, const double *F2, int n1, int n2, int n3) {
         size_t n = ((size_t)k * n2 + j) * n1 + i;
         F1[n] = F2[n];
     });
+    HT3D_API_END
 }
 
 int ht3d_Multiply(double *a, const double *b, const double *c, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(a);
     CHECK_PTR(b);
@@ -95,6 +104,7 @@ int ht3d_Multiply(double *a, const double *b, const double *c, int nx, int ny, i
         size_t n = g.c(i, j, k);
         a[n] = b[n] * c[n];
     });
+    HT3D_API_END
 }
 
 // ------------------------------------------------------------------ coefficients
@@ -116,6 +126,7 @@ static int face_averages(const char *name, double *kx, double *ky, double *kz, c
 
 int ht3d_InitThermal(double *Rt, double *kx, double *ky, double *kz, const double *Tc_ex, const double *ktv,
                      double _dt, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Rt); CHECK_PTR(kx); CHECK_PTR(ky); CHECK_PTR(kz); CHECK_PTR(Tc_ex); CHECK_PTR(ktv);
     Grid g{nx, ny, nz};
@@ -123,16 +134,20 @@ int ht3d_InitThermal(double *Rt, double *kx, double *ky, double *kz, const doubl
                       [=] __device__(int i, int j, int k) { Rt[g.c(i, j, k)] = (-_dt) * Tc_ex[g.e(i + 1, j + 1, k + 1)]; });
     if (rc) return rc;
     return face_averages("InitThermal", kx, ky, kz, ktv, nx, ny, nz);
+    HT3D_API_END
 }
 
 int ht3d_InitConductivity(double *kx, double *ky, double *kz, const double *ktv, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(kx); CHECK_PTR(ky); CHECK_PTR(kz); CHECK_PTR(ktv);
     return face_averages("InitConductivity", kx, ky, kz, ktv, nx, ny, nz);
+    HT3D_API_END
 }
 
 int ht3d_InitDarcy(double *Ty, double *kx, double *ky, double *kz, const double *Tc_ex, const double *kfv,
                    double _dt, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     (void)_dt;
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Ty); CHECK_PTR(kx); CHECK_PTR(ky); CHECK_PTR(kz); CHECK_PTR(Tc_ex); CHECK_PTR(kfv);
@@ -142,24 +157,29 @@ int ht3d_InitDarcy(double *Ty, double *kx, double *ky, double *kz, const double 
     });
     if (rc) return rc;
     return face_averages("InitDarcy", kx, ky, kz, kfv, nx, ny, nz);
+    HT3D_API_END
 }
 
 int ht3d_KillNorthSouthFluxes(double *Ty, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Ty);
     Grid g{nx, ny, nz};
     return launch3d("KillNorthSouthFluxes", nx, 2, nz, [=] __device__(int i, int j, int k) {
         Ty[g.fy(i, j == 0 ? 0 : ny, k)] = 0.0;
     });
+    HT3D_API_END
 }
 
 int ht3d_Compute_Rp(double *Rp, const double *ky, const double *Ty, double Ra, double _dy, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Rp); CHECK_PTR(ky); CHECK_PTR(Ty);
     Grid g{nx, ny, nz};
     return launch3d("Compute_Rp", nx, ny, nz, [=] __device__(int i, int j, int k) {
         Rp[g.c(i, j, k)] = Ra * _dy * ((ky[g.fy(i, j + 1, k)] * Ty[g.fy(i, j + 1, k)]) - (ky[g.fy(i, j, k)] * Ty[g.fy(i, j, k)]));
     });
+    HT3D_API_END
 }
 
 // ------------------------------------------------------------------ ghost fills
@@ -202,33 +222,40 @@ int bc_fill_table(double *A, int nx, int ny, int nz, const int type[6], const do
 extern "C" {
 
 int ht3d_SetTemperatureBCs(double *Tc_ex, double Tbot, double Ttop, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Tc_ex);
     const int t[6] = {0, 0, 1, 1, 0, 0};
     const double v[6] = {0.0, 0.0, Tbot, Ttop, 0.0, 0.0};
     return bc_fill6("SetTemperatureBCs", Tc_ex, nx, ny, nz, t, v);
+    HT3D_API_END
 }
 
 int ht3d_SetPressureBCs(double *Pc_ex, double Pbot, double Ptop, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     (void)Pbot; (void)Ptop;
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Pc_ex);
     const int t[6] = {0, 0, 0, 0, 0, 0};
     const double v[6] = {0, 0, 0, 0, 0, 0};
     return bc_fill6("SetPressureBCs", Pc_ex, nx, ny, nz, t, v);
+    HT3D_API_END
 }
 
 int ht3d_SetPressureBCs_v7(double *Pc_ex, double Pbot, double Ptop, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Pc_ex);
     const int t[6] = {0, 0, 1, 1, 0, 0};
     const double v[6] = {0.0, 0.0, Pbot, Ptop, 0.0, 0.0};
     return bc_fill6("SetPressureBCs_v7", Pc_ex, nx, ny, nz, t, v);
+    HT3D_API_END
 }
 
 // ------------------------------------------------------------------ fluxes / residuals / updates
 int ht3d_ComputeFlux(double *qx, double *qy, double *qz, const double *kx, const double *ky, const double *kz,
                      const double *A, double _dx, double _dy, double _dz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(qx); CHECK_PTR(qy); CHECK_PTR(qz); CHECK_PTR(kx); CHECK_PTR(ky); CHECK_PTR(kz); CHECK_PTR(A);
     Grid g{nx, ny, nz};
@@ -243,6 +270,7 @@ int ht3d_ComputeFlux(double *qx, double *qy, double *qz, const double *kx, const
     return launch3d("ComputeFlux", nx, ny, nz + 1, [=] __device__(int i, int j, int k) {
         qz[g.fz(i, j, k)] = -kz[g.fz(i, j, k)] * (_dz * (A[g.e(i + 1, j + 1, k + 1)] - A[g.e(i + 1, j + 1, k)]));
     });
+    HT3D_API_END
 }
 
 #define DQX (qx[g.fx(i + 1, j, k)] - qx[g.fx(i, j, k)])
@@ -251,6 +279,7 @@ int ht3d_ComputeFlux(double *qx, double *qy, double *qz, const double *kx, const
 
 int ht3d_UpdateT(double *F, double *T, const double *R, const double *qx, const double *qy, const double *qz,
                  double _dt, double dtau, double _dx, double _dy, double _dz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(F); CHECK_PTR(T); CHECK_PTR(R); CHECK_PTR(qx); CHECK_PTR(qy); CHECK_PTR(qz);
     Grid g{nx, ny, nz};
@@ -260,31 +289,37 @@ int ht3d_UpdateT(double *F, double *T, const double *R, const double *qx, const 
         F[c] = f;
         T[e] = T[e] - dtau * f;
     });
+    HT3D_API_END
 }
 
 int ht3d_ResidualDiffusion(double *F, const double *R, const double *qx, const double *qy, const double *qz,
                            double _dx, double _dy, double _dz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(F); CHECK_PTR(R); CHECK_PTR(qx); CHECK_PTR(qy); CHECK_PTR(qz);
     Grid g{nx, ny, nz};
     return launch3d("ResidualDiffusion", nx, ny, nz, [=] __device__(int i, int j, int k) {
         F[g.c(i, j, k)] = R[g.c(i, j, k)] + _dx * DQX + _dy * DQY + _dz * DQZ;
     });
+    HT3D_API_END
 }
 
 int ht3d_ResidualDiffusionTransient(double *F, const double *T, const double *R, const double *qx,
                                     const double *qy, const double *qz, double _dt, double _dx, double _dy,
                                     double _dz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(F); CHECK_PTR(T); CHECK_PTR(R); CHECK_PTR(qx); CHECK_PTR(qy); CHECK_PTR(qz);
     Grid g{nx, ny, nz};
     return launch3d("ResidualDiffusionTransient", nx, ny, nz, [=] __device__(int i, int j, int k) {
         F[g.c(i, j, k)] = R[g.c(i, j, k)] + _dt * T[g.e(i + 1, j + 1, k + 1)] + _dx * DQX + _dy * DQY + _dz * DQZ;
     });
+    HT3D_API_END
 }
 
 int ht3d_ResidualDiffusionPC(double *F, const double *R, const double *qx, const double *qy, const double *qz,
                              const double *PC, double _dx, double _dy, double _dz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(F); CHECK_PTR(R); CHECK_PTR(qx); CHECK_PTR(qy); CHECK_PTR(qz); CHECK_PTR(PC);
     Grid g{nx, ny, nz};
@@ -292,11 +327,13 @@ int ht3d_ResidualDiffusionPC(double *F, const double *R, const double *qx, const
         double f = R[g.c(i, j, k)] + _dx * DQX + _dy * DQY + _dz * DQZ;
         F[g.c(i, j, k)] = f / PC[g.c(i, j, k)];
     });
+    HT3D_API_END
 }
 
 int ht3d_ResidualDiffusionTransientPC(double *F, const double *T, const double *R, const double *qx,
                                       const double *qy, const double *qz, const double *PC, double _dt,
                                       double _dx, double _dy, double _dz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(F); CHECK_PTR(T); CHECK_PTR(R); CHECK_PTR(qx); CHECK_PTR(qy); CHECK_PTR(qz); CHECK_PTR(PC);
     Grid g{nx, ny, nz};
@@ -304,9 +341,11 @@ int ht3d_ResidualDiffusionTransientPC(double *F, const double *T, const double *
         double f = R[g.c(i, j, k)] + _dt * T[g.e(i + 1, j + 1, k + 1)] + _dx * DQX + _dy * DQY + _dz * DQZ;
         F[g.c(i, j, k)] = f / PC[g.c(i, j, k)];
     });
+    HT3D_API_END
 }
 
 int ht3d_UpdateP(double *F0, double *P, const double *F, double dampx, double dtau, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(F0); CHECK_PTR(P); CHECK_PTR(F);
     Grid g{nx, ny, nz};
@@ -316,14 +355,18 @@ int ht3d_UpdateP(double *F0, double *P, const double *F, double dampx, double dt
         F0[c] = f0;
         P[e] = P[e] - dtau * f0;
     });
+    HT3D_API_END
 }
 
 int ht3d_DampedUpdate(double *F0, double *X, const double *F, double dampx, double dtau, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     return ht3d_UpdateP(F0, X, F, dampx, dtau, nx, ny, nz);
+    HT3D_API_END
 }
 
 int ht3d_DYRELUpdate(double *F0, double *X, const double *F, double h1, double h2, double dtau, int nx, int ny,
                      int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(F0); CHECK_PTR(X); CHECK_PTR(F);
     Grid g{nx, ny, nz};
@@ -333,10 +376,12 @@ int ht3d_DYRELUpdate(double *F0, double *X, const double *F, double h1, double h
         F0[c] = f0;
         X[e] = X[e] - dtau * f0;
     });
+    HT3D_API_END
 }
 
 // SwapDYREL!  kernels_HT3D_dimensional.jl:122-126: Xc = inn(X0c_ex), Yc = Y0c
 int ht3d_SwapDYREL(double *Xc, double *Yc, const double *X0c_ex, const double *Y0c, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Xc); CHECK_PTR(Yc); CHECK_PTR(X0c_ex); CHECK_PTR(Y0c);
     Grid g{nx, ny, nz};
@@ -345,6 +390,7 @@ int ht3d_SwapDYREL(double *Xc, double *Yc, const double *X0c_ex, const double *Y
         Xc[c] = X0c_ex[g.e(i + 1, j + 1, k + 1)];
         Yc[c] = Y0c[c];
     });
+    HT3D_API_END
 }
 
 // the Gershgorin row sum of notebooks/Gershgorin_TH3D.ipynb (cell 2 output)
@@ -361,6 +407,7 @@ __device__ __forceinline__ double gersh_offsum(double kW, double kE, double kS, 
 int ht3d_GershgorinPoisson(double *G, double *D, const double *rhoC, const double *kx, const double *ky,
                            const double *kz, double transient, double dt, double dx, double dy, double dz, int nx,
                            int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(G); CHECK_PTR(D); CHECK_PTR(rhoC); CHECK_PTR(kx); CHECK_PTR(ky); CHECK_PTR(kz);
     Grid g{nx, ny, nz};
@@ -373,10 +420,12 @@ int ht3d_GershgorinPoisson(double *G, double *D, const double *rhoC, const doubl
         D[g.c(i, j, k)] = diag;
         G[g.c(i, j, k)] = gg / diag;
     });
+    HT3D_API_END
 }
 
 int ht3d_GershgorinHydro(double *G, const double *phidrhodP, const double *kx, const double *ky, const double *kz,
                          double transient, double dt, double dx, double dy, double dz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(G); CHECK_PTR(phidrhodP); CHECK_PTR(kx); CHECK_PTR(ky); CHECK_PTR(kz);
     Grid g{nx, ny, nz};
@@ -387,10 +436,12 @@ int ht3d_GershgorinHydro(double *G, const double *phidrhodP, const double *kx, c
         double diag = gersh_diag(kW, kE, kS, kN, kB, kF, phidrhodP[g.c(i, j, k)] * transient / dt, dx, dy, dz);
         G[g.c(i, j, k)] = gersh_offsum(kW, kE, kS, kN, kB, kF, dx, dy, dz) + fabs(diag);
     });
+    HT3D_API_END
 }
 
 int ht3d_RayleighQuotientNumerator(double *G, const double *Fc0, const double *Fcit, const double *Fc, double dtau,
                                    int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(G); CHECK_PTR(Fc0); CHECK_PTR(Fcit); CHECK_PTR(Fc);
     Grid g{nx, ny, nz};
@@ -399,9 +450,11 @@ int ht3d_RayleighQuotientNumerator(double *G, const double *Fc0, const double *F
         double d = dtau * Fc0[c];
         G[c] = (-d) * (Fc[c] - Fcit[c]);
     });
+    HT3D_API_END
 }
 
 int ht3d_RayleighQuotientDenominator(double *G, const double *Fc0, double dtau, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(G); CHECK_PTR(Fc0);
     Grid g{nx, ny, nz};
@@ -410,12 +463,14 @@ int ht3d_RayleighQuotientDenominator(double *G, const double *Fc0, double dtau, 
         double d = dtau * Fc0[c];
         G[c] = d * d;
     });
+    HT3D_API_END
 }
 
 int ht3d_ResidualTemperatureLinearised(double *F, const double *Tc_ex, const double *Tc_old, const double *rhoC_eff,
                                        const double *phc_ex, const double *PC, const double *qx, const double *qy,
                                        const double *qz, double Qt, double transient, double _dt, double _dx,
                                        double _dy, double _dz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(F); CHECK_PTR(Tc_ex); CHECK_PTR(Tc_old); CHECK_PTR(rhoC_eff); CHECK_PTR(phc_ex); CHECK_PTR(PC);
     CHECK_PTR(qx); CHECK_PTR(qy); CHECK_PTR(qz);
@@ -437,12 +492,14 @@ int ht3d_ResidualTemperatureLinearised(double *F, const double *Tc_ex, const dou
             F[c] = 0.0;
         }
     });
+    HT3D_API_END
 }
 
 int ht3d_ResidualFluidPressure(double *F, const double *phc_ex, const double *phirhof, const double *phirhof0,
                                const double *PC, const double *qx, const double *qy, const double *qz,
                                double transient, double _dt, double _dx, double _dy, double _dz, int nx, int ny,
                                int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(F); CHECK_PTR(phc_ex); CHECK_PTR(phirhof); CHECK_PTR(phirhof0); CHECK_PTR(PC);
     CHECK_PTR(qx); CHECK_PTR(qy); CHECK_PTR(qz);
@@ -461,11 +518,13 @@ int ht3d_ResidualFluidPressure(double *F, const double *phc_ex, const double *ph
             F[c] = 0.0;
         }
     });
+    HT3D_API_END
 }
 
 int ht3d_ComputeDarcyFlux(double *qx, double *qy, double *qz, const double *rhofv, const double *kx,
                           const double *ky, const double *kz, const double *Pf, double gr, double _dx, double _dy,
                           double _dz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(qx); CHECK_PTR(qy); CHECK_PTR(qz); CHECK_PTR(rhofv); CHECK_PTR(kx); CHECK_PTR(ky); CHECK_PTR(kz);
     CHECK_PTR(Pf);
@@ -482,11 +541,13 @@ int ht3d_ComputeDarcyFlux(double *qx, double *qy, double *qz, const double *rhof
     return launch3d("ComputeDarcyFlux", nx, ny, nz + 1, [=] __device__(int i, int j, int k) {
         qz[g.fz(i, j, k)] = -kz[g.fz(i, j, k)] * (_dz * (Pf[g.e(i + 1, j + 1, k + 1)] - Pf[g.e(i + 1, j + 1, k)]));
     });
+    HT3D_API_END
 }
 
 int ht3d_Init_vel(double *Vx, double *Vy, double *Vz, const double *kx, const double *ky, const double *kz,
                   const double *Pc_ex, const double *Ty, double Ra, double _dx, double _dy, double _dz, int nx,
                   int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Vx); CHECK_PTR(Vy); CHECK_PTR(Vz); CHECK_PTR(kx); CHECK_PTR(ky); CHECK_PTR(kz); CHECK_PTR(Pc_ex);
     CHECK_PTR(Ty);
@@ -502,6 +563,7 @@ int ht3d_Init_vel(double *Vx, double *Vy, double *Vz, const double *kx, const do
     return launch3d("Init_vel", nx, ny, nz + 1, [=] __device__(int i, int j, int k) {
         Vz[g.fz(i, j, k)] = -kz[g.fz(i, j, k)] * _dz * (Pc_ex[g.e(i + 1, j + 1, k + 1)] - Pc_ex[g.e(i + 1, j + 1, k)]);
     });
+    HT3D_API_END
 }
 
 // Temperature-dependent viscosity mu = exp(-a T): face coefficients become the
@@ -512,6 +574,7 @@ int ht3d_Init_vel(double *Vx, double *Vy, double *Vz, const double *kx, const do
 // Dimensional/kernels_HT3D_dimensional.jl (ComputeHydroConductivity), applied
 // once per Darcy solve after InitDarcy!.
 int ht3d_ApplyViscosity(double *kx, double *ky, double *kz, const double *Tc_ex, double a, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(kx); CHECK_PTR(ky); CHECK_PTR(kz); CHECK_PTR(Tc_ex);
     Grid g{nx, ny, nz};
@@ -526,6 +589,7 @@ int ht3d_ApplyViscosity(double *kx, double *ky, double *kz, const double *Tc_ex,
     return launch3d("ApplyViscosity", nx, ny, nz + 1, [=] __device__(int i, int j, int k) {
         kz[g.fz(i, j, k)] = kz[g.fz(i, j, k)] * exp(a * ((Tc_ex[g.e(i + 1, j + 1, k)] + Tc_ex[g.e(i + 1, j + 1, k + 1)]) * 0.5));
     });
+    HT3D_API_END
 }
 
 // ------------------------------------------------------------------ WENO5 (v1 kernels)
@@ -545,23 +609,30 @@ static int plus_minus(const char *name, double *Vm, double *Vp, const double *V,
 }
 
 int ht3d_VxPlusMinus(double *Vxm, double *Vxp, const double *Vx, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Vxm); CHECK_PTR(Vxp); CHECK_PTR(Vx);
     return plus_minus("VxPlusMinus", Vxm, Vxp, Vx, 1, nx, ny, nz);
+    HT3D_API_END
 }
 int ht3d_VyPlusMinus(double *Vym, double *Vyp, const double *Vy, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Vym); CHECK_PTR(Vyp); CHECK_PTR(Vy);
     return plus_minus("VyPlusMinus", Vym, Vyp, Vy, 2, nx, ny, nz);
+    HT3D_API_END
 }
 int ht3d_VzPlusMinus(double *Vzm, double *Vzp, const double *Vz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Vzm); CHECK_PTR(Vzp); CHECK_PTR(Vz);
     return plus_minus("VzPlusMinus", Vzm, Vzp, Vz, 3, nx, ny, nz);
+    HT3D_API_END
 }
 
 int ht3d_dFdx_Weno5(double *dFdxi, const double *V1, const double *V2, const double *V3, const double *V4,
                     const double *V5, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(dFdxi); CHECK_PTR(V1); CHECK_PTR(V2); CHECK_PTR(V3); CHECK_PTR(V4); CHECK_PTR(V5);
     Grid g{nx, ny, nz};
@@ -569,10 +640,12 @@ int ht3d_dFdx_Weno5(double *dFdxi, const double *V1, const double *V2, const dou
         const size_t c = g.c(i, j, k);
         dFdxi[c] = weno5_js(V1[c], V2[c], V3[c], V4[c], V5[c]);
     });
+    HT3D_API_END
 }
 
 int ht3d_Gradients_Weno5(double *v1, double *v2, double *v3, double *v4, double *v5, const double *X,
                          int axis, int plus, double _dx, double _dy, double _dz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(v1); CHECK_PTR(v2); CHECK_PTR(v3); CHECK_PTR(v4); CHECK_PTR(v5); CHECK_PTR(X);
     if (axis < 1 || axis > 3) return arg_fail("Gradients_Weno5: axis must be 1..3"), HT3D_ERR_ARG;
@@ -596,10 +669,12 @@ int ht3d_Gradients_Weno5(double *v1, double *v2, double *v3, double *v4, double 
             v5[c] = h * (xs(-1) - xs(-2));
         }
     });
+    HT3D_API_END
 }
 
 int ht3d_Boundaries_Weno5(double *Fc_exxx, const double *Fc, int axis, int src_ext, int type_lo, double val_lo,
                           int type_hi, double val_hi, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Fc_exxx); CHECK_PTR(Fc);
     if (axis < 1 || axis > 3) return arg_fail("Boundaries_Weno5: axis must be 1..3"), HT3D_ERR_ARG;
@@ -630,10 +705,12 @@ int ht3d_Boundaries_Weno5(double *Fc_exxx, const double *Fc, int axis, int src_e
     };
     const int na = axis == 1 ? ny : nx, nb = axis == 3 ? ny : nz;
     return launch3d("Boundaries_Weno5", na, nb, 6, [=] __device__(int a, int b, int s) { ghost(s % 3 + 1, s / 3, a, b); });
+    HT3D_API_END
 }
 
 int ht3d_Advect(double *Fc, const double *Vxp, const double *dTdxm, const double *Vxm, const double *dTdxp,
                 double dt, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Fc); CHECK_PTR(Vxp); CHECK_PTR(dTdxm); CHECK_PTR(Vxm); CHECK_PTR(dTdxp);
     Grid g{nx, ny, nz};
@@ -641,9 +718,11 @@ int ht3d_Advect(double *Fc, const double *Vxp, const double *dTdxm, const double
         const size_t c = g.c(i, j, k);
         Fc[c] = Fc[c] - dt * (Vxp[c] * dTdxm[c] + Vxm[c] * dTdxp[c]);
     });
+    HT3D_API_END
 }
 
 int ht3d_TimeAveraging(double *Fc, const double *Fold, double order, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Fc); CHECK_PTR(Fold);
     Grid g{nx, ny, nz};
@@ -651,9 +730,11 @@ int ht3d_TimeAveraging(double *Fc, const double *Fold, double order, int nx, int
         const size_t c = g.c(i, j, k);
         Fc[c] = (1.0 / order) * Fc[c] + (1.0 - 1.0 / order) * Fold[c];
     });
+    HT3D_API_END
 }
 
 int ht3d_TimeAveraging_v2(double *Fc, const double *Fold, double order, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Fc); CHECK_PTR(Fold);
     Grid g{nx, ny, nz};
@@ -661,10 +742,12 @@ int ht3d_TimeAveraging_v2(double *Fc, const double *Fold, double order, int nx, 
         const size_t e = g.e(i + 1, j + 1, k + 1);
         Fc[e] = (1.0 / order) * Fc[e] + (1.0 - 1.0 / order) * Fold[g.c(i, j, k)];
     });
+    HT3D_API_END
 }
 
 int ht3d_Advect_v2(double *Fc, const double *X, const double *V, int axis, double dt, double _dx, double _dy,
                    double _dz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Fc); CHECK_PTR(X); CHECK_PTR(V);
     if (axis < 1 || axis > 3) return arg_fail("Advect_v2: axis must be 1..3"), HT3D_ERR_ARG;
@@ -685,6 +768,7 @@ int ht3d_Advect_v2(double *Fc, const double *X, const double *V, int axis, doubl
         const size_t e = g.e(i + 1, j + 1, k + 1);
         Fc[e] = Fc[e] - dt * (Vp * dm + Vm * dp);
     });
+    HT3D_API_END
 }
 
 
@@ -696,24 +780,29 @@ int ht3d_Advect_v2(double *Fc, const double *X, const double *V, int axis, doubl
 // (HydroThermal3D_v7.jl:13).  ht3d_to_f32 and ht3d_add_f32 move data between the
 // FP64 anchor and the FP32 correction.
 int ht3d_SetTemperatureBCs_f32(float *Tc_ex, float Tbot, float Ttop, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Tc_ex);
     const int t[6] = {0, 0, 1, 1, 0, 0};
     const double v[6] = {0.0, 0.0, (double)Tbot, (double)Ttop, 0.0, 0.0};
     return bc_fill6("SetTemperatureBCs_f32", Tc_ex, nx, ny, nz, t, v);
+    HT3D_API_END
 }
 
 int ht3d_SetPressureBCs_f32(float *Pc_ex, float Pbot, float Ptop, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     (void)Pbot; (void)Ptop;
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(Pc_ex);
     const int t[6] = {0, 0, 0, 0, 0, 0};
     const double v[6] = {0, 0, 0, 0, 0, 0};
     return bc_fill6("SetPressureBCs_f32", Pc_ex, nx, ny, nz, t, v);
+    HT3D_API_END
 }
 
 int ht3d_ComputeFlux_f32(float *qx, float *qy, float *qz, const float *kx, const float *ky, const float *kz,
                          const float *A, float _dx, float _dy, float _dz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(qx); CHECK_PTR(qy); CHECK_PTR(qz); CHECK_PTR(kx); CHECK_PTR(ky); CHECK_PTR(kz); CHECK_PTR(A);
     Grid g{nx, ny, nz};
@@ -728,31 +817,37 @@ int ht3d_ComputeFlux_f32(float *qx, float *qy, float *qz, const float *kx, const
     return launch3d("ComputeFlux_f32", nx, ny, nz + 1, [=] __device__(int i, int j, int k) {
         qz[g.fz(i, j, k)] = -kz[g.fz(i, j, k)] * (_dz * (A[g.e(i + 1, j + 1, k + 1)] - A[g.e(i + 1, j + 1, k)]));
     });
+    HT3D_API_END
 }
 
 int ht3d_ResidualDiffusion_f32(float *F, const float *R, const float *qx, const float *qy, const float *qz, float _dx,
                                float _dy, float _dz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(F); CHECK_PTR(R); CHECK_PTR(qx); CHECK_PTR(qy); CHECK_PTR(qz);
     Grid g{nx, ny, nz};
     return launch3d("ResidualDiffusion_f32", nx, ny, nz, [=] __device__(int i, int j, int k) {
         F[g.c(i, j, k)] = R[g.c(i, j, k)] + _dx * DQX + _dy * DQY + _dz * DQZ;
     });
+    HT3D_API_END
 }
 
 int ht3d_ResidualDiffusionTransient_f32(float *F, const float *T, const float *R, const float *qx, const float *qy,
                                         const float *qz, float _dt, float _dx, float _dy, float _dz, int nx, int ny,
                                         int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(F); CHECK_PTR(T); CHECK_PTR(R); CHECK_PTR(qx); CHECK_PTR(qy); CHECK_PTR(qz);
     Grid g{nx, ny, nz};
     return launch3d("ResidualDiffusionTransient_f32", nx, ny, nz, [=] __device__(int i, int j, int k) {
         F[g.c(i, j, k)] = R[g.c(i, j, k)] + _dt * T[g.e(i + 1, j + 1, k + 1)] + _dx * DQX + _dy * DQY + _dz * DQZ;
     });
+    HT3D_API_END
 }
 
 int ht3d_ResidualDiffusionPC_f32(float *F, const float *R, const float *qx, const float *qy, const float *qz,
                                  const float *PC, float _dx, float _dy, float _dz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(F); CHECK_PTR(R); CHECK_PTR(qx); CHECK_PTR(qy); CHECK_PTR(qz); CHECK_PTR(PC);
     Grid g{nx, ny, nz};
@@ -760,11 +855,13 @@ int ht3d_ResidualDiffusionPC_f32(float *F, const float *R, const float *qx, cons
         float f = R[g.c(i, j, k)] + _dx * DQX + _dy * DQY + _dz * DQZ;
         F[g.c(i, j, k)] = f / PC[g.c(i, j, k)];
     });
+    HT3D_API_END
 }
 
 int ht3d_ResidualDiffusionTransientPC_f32(float *F, const float *T, const float *R, const float *qx, const float *qy,
                                           const float *qz, const float *PC, float _dt, float _dx, float _dy,
                                           float _dz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(F); CHECK_PTR(T); CHECK_PTR(R); CHECK_PTR(qx); CHECK_PTR(qy); CHECK_PTR(qz); CHECK_PTR(PC);
     Grid g{nx, ny, nz};
@@ -772,10 +869,12 @@ int ht3d_ResidualDiffusionTransientPC_f32(float *F, const float *T, const float 
         float f = R[g.c(i, j, k)] + _dt * T[g.e(i + 1, j + 1, k + 1)] + _dx * DQX + _dy * DQY + _dz * DQZ;
         F[g.c(i, j, k)] = f / PC[g.c(i, j, k)];
     });
+    HT3D_API_END
 }
 
 int ht3d_DYRELUpdate_f32(float *F0, float *X, const float *F, float h1, float h2, float dtau, int nx, int ny,
                          int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(F0); CHECK_PTR(X); CHECK_PTR(F);
     Grid g{nx, ny, nz};
@@ -785,11 +884,13 @@ int ht3d_DYRELUpdate_f32(float *F0, float *X, const float *F, float h1, float h2
         F0[c] = f0;
         X[e] = X[e] - dtau * f0;
     });
+    HT3D_API_END
 }
 
 int ht3d_GershgorinPoisson_f32(float *G, float *D, const float *rhoC, const float *kx, const float *ky,
                                const float *kz, float transient, float dt, float dx, float dy, float dz, int nx,
                                int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(G); CHECK_PTR(D); CHECK_PTR(rhoC); CHECK_PTR(kx); CHECK_PTR(ky); CHECK_PTR(kz);
     Grid g{nx, ny, nz};
@@ -804,17 +905,21 @@ int ht3d_GershgorinPoisson_f32(float *G, float *D, const float *rhoC, const floa
         D[g.c(i, j, k)] = diag;
         G[g.c(i, j, k)] = (off + fabsf(diag)) / diag;
     });
+    HT3D_API_END
 }
 
 int ht3d_to_f32(float *out, const double *in, size_t n) {
+    HT3D_API_BEGIN
     CHECK_PTR(out); CHECK_PTR(in);
     if (!n) return HT3D_OK;
     int rc = to_float(out, in, n);
     if (rc) return rc;
     return after_launch("to_f32");
+    HT3D_API_END
 }
 
 int ht3d_add_f32(double *X, const float *e, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     CHECK_GRID(nx, ny, nz);
     CHECK_PTR(X); CHECK_PTR(e);
     Grid g{nx, ny, nz};
@@ -822,5 +927,6 @@ int ht3d_add_f32(double *X, const float *e, int nx, int ny, int nz) {
         const size_t c = g.e(i + 1, j + 1, k + 1);
         X[c] += (double)e[c];
     });
+    HT3D_API_END
 }
 }  // extern "C"

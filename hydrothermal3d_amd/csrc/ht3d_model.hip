@@ -312,6 +312,7 @@ extern "C" {
 
 int ht3d_weno5_advect(double *Tc_ex, const double *Vx, const double *Vy, const double *Vz, double dt, double _dx,
                       double _dy, double _dz, double Ttop, double Tbot, int version, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     if (!grid_ok(nx, ny, nz)) return arg_fail("ht3d_weno5_advect: extents must be >= 3");
     if (!Tc_ex || !Vx || !Vy || !Vz) return arg_fail("ht3d_weno5_advect: null pointer");
@@ -322,11 +323,13 @@ int ht3d_weno5_advect(double *Tc_ex, const double *Vx, const double *Vy, const d
     rc = weno_advect(Tc_ex, (double *)w, Vx, Vy, Vz, dt, _dx, _dy, _dz, Ttop, Tbot, version, nx, ny, nz);
     if (rc) return rc;
     return after_launch("ht3d_weno5_advect");
+    HT3D_API_END
 }
 
 int ht3d_lognormal_field(double *kfv, int nx, int ny, int nz, double xmin, double ymin, double zmin, double dx,
                          double dy, double dz, const double *K, const double *phi, int M, double sigma,
                          double fault_mult, double fault_half, const double *fault_n, const double *fault_x0) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     if (!kfv || !K || !phi || M < 1 || nx < 1 || ny < 1 || nz < 1) return arg_fail("ht3d_lognormal_field: bad argument");
     double *dK = nullptr, *dphi = nullptr;
@@ -343,12 +346,14 @@ int ht3d_lognormal_field(double *kfv, int nx, int ny, int nz, double xmin, doubl
     (void)hipFree(dK);
     (void)hipFree(dphi);
     return rc;
+    HT3D_API_END
 }
 
 int ht3d_velocity_dt_nu(double *Vx, double *Vy, double *Vz, const double *kx, const double *ky, const double *kz,
                         const double *Pc_ex, const double *Ty, double Ra, double dx, double dy, double dz, double cfl,
                         const double *Tc_ex, const double *kty, double Lx, double Lz, double *out, int nx, int ny,
                         int nz) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     if (!grid_ok(nx, ny, nz)) return arg_fail("ht3d_velocity_dt_nu: extents must be >= 3");
     if (!Vx || !Vy || !Vz || !kx || !ky || !kz || !Pc_ex || !Ty || !out || (Tc_ex && !kty))
@@ -362,9 +367,11 @@ int ht3d_velocity_dt_nu(double *Vx, double *Vy, double *Vz, const double *kx, co
     out[1] = cfl * fmin(fmin(dx, dy), dz) / vmax;                       // HydroThermal3D_v18.jl:554 rule
     out[2] = Tc_ex ? nusum / (Lx * Lz) / (double)(nx * nz) : 0.0;      // HydroThermal3D_Alban_v6.jl:346
     return after_launch("ht3d_velocity_dt_nu");
+    HT3D_API_END
 }
 
 int ht3d_hrl_create(int nx, int ny, int nz, double Lx, double Ly, double Lz, double Ra, double cfl, ht3d_hrl **out) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     if (!out || !grid_ok(nx, ny, nz) || !(Lx > 0) || !(Ly > 0) || !(Lz > 0) || !(cfl > 0))
         return arg_fail("ht3d_hrl_create: bad argument");
@@ -402,9 +409,11 @@ int ht3d_hrl_create(int nx, int ny, int nz, double Lx, double Ly, double Lz, dou
     }
     *out = m;
     return after_launch("ht3d_hrl_create", true);
+    HT3D_API_END
 }
 
 int ht3d_hrl_destroy(ht3d_hrl *m) {
+    HT3D_API_BEGIN
     if (!m) return HT3D_OK;
     if (ctx().initialised) (void)hipStreamSynchronize(ctx().stream);
     double *ptrs[] = {m->Tc_ex, m->Pc_ex, m->ktv, m->kfv, m->Ty, m->kx, m->ky, m->kz, m->ktx, m->kty,
@@ -413,9 +422,11 @@ int ht3d_hrl_destroy(ht3d_hrl *m) {
         if (p) (void)hipFree(p);
     delete m;
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_hrl_field(ht3d_hrl *m, const char *name, double **dptr, size_t *count) {
+    HT3D_API_BEGIN
     if (!m || !name || !dptr) return arg_fail("ht3d_hrl_field: bad argument");
     const int nx = m->nx, ny = m->ny, nz = m->nz;
     const size_t c = (size_t)nx * ny * nz, E = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
@@ -433,10 +444,12 @@ int ht3d_hrl_field(ht3d_hrl *m, const char *name, double **dptr, size_t *count) 
             return HT3D_OK;
         }
     return arg_fail("ht3d_hrl_field: unknown field name");
+    HT3D_API_END
 }
 
 int ht3d_hrl_set_params(ht3d_hrl *m, const ht3d_dyrel_params *pT, const ht3d_dyrel_params *pP, int weno_version,
                         int precision) {
+    HT3D_API_BEGIN
     if (!m) return arg_fail("ht3d_hrl_set_params: null model");
     if (weno_version != 1 && weno_version != 2) return arg_fail("ht3d_hrl_set_params: weno_version must be 1 or 2");
     if (precision != HT3D_FP64 && precision != HT3D_FP32_DC) return arg_fail("ht3d_hrl_set_params: bad precision");
@@ -445,27 +458,35 @@ int ht3d_hrl_set_params(ht3d_hrl *m, const ht3d_dyrel_params *pT, const ht3d_dyr
     m->weno_version = weno_version;
     m->precision = precision;
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_hrl_set_viscosity(ht3d_hrl *m, double a) {
+    HT3D_API_BEGIN
     if (!m || !(a == a)) return arg_fail("ht3d_hrl_set_viscosity: bad argument");
     m->mu_a = a;
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_hrl_set_dt_rule(ht3d_hrl *m, int rule) {
+    HT3D_API_BEGIN
     if (!m || rule < HT3D_DT_V18 || rule > HT3D_DT_V7) return arg_fail("ht3d_hrl_set_dt_rule: bad argument");
     m->dt_rule = rule;
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_hrl_set_dt(ht3d_hrl *m, double dt) {
+    HT3D_API_BEGIN
     if (!m || !(dt > 0)) return arg_fail("ht3d_hrl_set_dt: bad argument");
     m->dt = dt;
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_hrl_step(ht3d_hrl *m, ht3d_step_report *rep) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     if (!m) return arg_fail("ht3d_hrl_step: null model");
     SyncGuard guard;
@@ -503,6 +524,7 @@ int ht3d_hrl_step(ht3d_hrl *m, ht3d_step_report *rep) {
     rc = after_launch("ht3d_hrl_step", true);
     if (rc) return rc;
     return r.status;
+    HT3D_API_END
 }
 
 }  // extern "C"

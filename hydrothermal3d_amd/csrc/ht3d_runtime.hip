@@ -7,26 +7,43 @@
 #include <cstdio>
 #include <cstring>
 #include <cmath>
+#include <new>
 
 #include "ht3d_internal.hpp"
 
 namespace ht3d {
 
 static Context g_ctx;
-static thread_local std::string g_err;
+// the last error's text, in a fixed buffer: reporting an error allocates nothing
+static thread_local char g_err[512];
 
 Context &ctx() { return g_ctx; }
 
-void set_error(const std::string &msg) { g_err = msg; }
+void set_error(const char *msg) noexcept { std::snprintf(g_err, sizeof g_err, "%s", msg ? msg : ""); }
 
-int hip_fail(hipError_t e, const char *what) {
-    g_err = std::string(what) + ": " + hipGetErrorString(e);
+int hip_fail(hipError_t e, const char *what) noexcept {
+    std::snprintf(g_err, sizeof g_err, "%s: %s", what ? what : "", hipGetErrorString(e));
     return HT3D_ERR_HIP;
 }
 
-int arg_fail(const char *what) {
-    g_err = what;
+int arg_fail(const char *what) noexcept {
+    set_error(what);
     return HT3D_ERR_ARG;
+}
+
+static int g_inject_throw = 0;   // ht3d_debug_inject_host_failure
+
+void maybe_inject_failure() {
+    if (g_inject_throw > 0) {
+        --g_inject_throw;
+        throw std::bad_alloc();
+    }
+}
+
+int guard_failure(const char *entry, bool nomem) noexcept {
+    std::snprintf(g_err, sizeof g_err, "%s: %s", entry,
+                  nomem ? "host memory allocation failed" : "internal error (C++ exception)");
+    return nomem ? HT3D_ERR_NOMEM : HT3D_ERR_INTERNAL;
 }
 
 int workspace(int slot, size_t bytes, void **out) {
@@ -187,6 +204,7 @@ using namespace ht3d;
 extern "C" {
 
 int ht3d_init(int device) {
+    HT3D_API_BEGIN
     Context &c = ctx();
     if (c.initialised) {
         if (c.device == device) return HT3D_OK;
@@ -204,28 +222,36 @@ int ht3d_init(int device) {
     c.initialised = true;
     c.sync = true;
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_timing_enable(int on) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     ctx().timing = on != 0;
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_timing_get(ht3d_timing_stats *out) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     if (!out) return arg_fail("ht3d_timing_get: null pointer");
     *out = ctx().stats;
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_timing_reset(void) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     ctx().stats = ht3d_timing_stats{};
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_finalize(void) {
+    HT3D_API_BEGIN
     Context &c = ctx();
     if (!c.initialised) return HT3D_OK;
     (void)hipStreamSynchronize(c.stream);
@@ -243,6 +269,12 @@ int ht3d_finalize(void) {
     c.stream = nullptr;
     c.initialised = false;
     return HT3D_OK;
+    HT3D_API_END
+}
+
+int ht3d_debug_inject_host_failure(int count) {
+    g_inject_throw = count > 0 ? count : 0;
+    return HT3D_OK;
 }
 
 const char *ht3d_strerror(int code) {
@@ -252,6 +284,7 @@ const char *ht3d_strerror(int code) {
     case HT3D_ERR_HIP: return "HIP runtime error";
     case HT3D_ERR_NOMEM: return "out of memory";
     case HT3D_ERR_STATE: return "library not initialised";
+    case HT3D_ERR_INTERNAL: return "internal error";
     case HT3D_NOT_CONVERGED: return "pseudo-transient solve reached nitmax";
     case HT3D_NAN: return "NaN in residual norm";
     case HT3D_DIVERGED: return "pseudo-transient solve diverged";
@@ -259,21 +292,26 @@ const char *ht3d_strerror(int code) {
     }
 }
 
-const char *ht3d_last_error(void) { return g_err.c_str(); }
+const char *ht3d_last_error(void) { return g_err; }
 
 int ht3d_set_sync(int synchronous) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     ctx().sync = synchronous != 0;
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_sync(void) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     HIP_TRY(hipStreamSynchronize(ctx().stream));
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_device_name(char *buf, size_t len) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     hipDeviceProp_t p;
     HIP_TRY(hipGetDeviceProperties(&p, ctx().device));
@@ -283,9 +321,11 @@ int ht3d_device_name(char *buf, size_t len) {
                       p.multiProcessorCount, (double)p.totalGlobalMem / (1024.0 * 1024.0 * 1024.0));
     }
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_alloc(size_t bytes, void **dptr) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     if (!dptr) return arg_fail("ht3d_alloc: null out pointer");
     *dptr = nullptr;
@@ -293,51 +333,63 @@ int ht3d_alloc(size_t bytes, void **dptr) {
     if (e == hipErrorOutOfMemory) return hip_fail(e, "hipMalloc"), HT3D_ERR_NOMEM;
     if (e != hipSuccess) return hip_fail(e, "hipMalloc");
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_free(void *dptr) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     if (!dptr) return HT3D_OK;
     HIP_TRY(hipStreamSynchronize(ctx().stream));
     HIP_TRY(hipFree(dptr));
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_h2d(void *dst, const void *src, size_t bytes) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     if (!bytes) return HT3D_OK;
     if (!dst || !src) return arg_fail("ht3d_h2d: null pointer");
     HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx().stream));
     HIP_TRY(hipStreamSynchronize(ctx().stream));
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_d2h(void *dst, const void *src, size_t bytes) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     if (!bytes) return HT3D_OK;
     if (!dst || !src) return arg_fail("ht3d_d2h: null pointer");
     HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx().stream));
     HIP_TRY(hipStreamSynchronize(ctx().stream));
     return HT3D_OK;
+    HT3D_API_END
 }
 
 int ht3d_d2d(void *dst, const void *src, size_t bytes) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     if (!bytes) return HT3D_OK;
     if (!dst || !src) return arg_fail("ht3d_d2d: null pointer");
     HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx().stream));
     return after_launch("ht3d_d2d");
+    HT3D_API_END
 }
 
 int ht3d_memset(void *dptr, int value, size_t bytes) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     if (!bytes) return HT3D_OK;
     if (!dptr) return arg_fail("ht3d_memset: null pointer");
     HIP_TRY(hipMemsetAsync(dptr, value, bytes, ctx().stream));
     return after_launch("ht3d_memset");
+    HT3D_API_END
 }
 
 int ht3d_reduce(double *out_host, const double *a, size_t n, int op) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     if (!out_host || !a || op < HT3D_SUM || op > HT3D_MAX_ABS) return arg_fail("ht3d_reduce: bad argument");
     if (n == 0) {
@@ -361,6 +413,7 @@ int ht3d_reduce(double *out_host, const double *a, size_t n, int op) {
     if (rc) return rc;
     *out_host = (op == HT3D_MIN) ? -r4[0] : r4[0];
     return HT3D_OK;
+    HT3D_API_END
 }
 
 }  // extern "C"

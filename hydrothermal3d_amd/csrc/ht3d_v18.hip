@@ -621,21 +621,27 @@ static int solve(const ht3d_v18_field *f, const ht3d_dyrel_params *p, ht3d_dyrel
 extern "C" {
 
 int ht3d_v18_thermal_solve(const ht3d_v18_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep) {
+    HT3D_API_BEGIN
     return ht3d::v18::solve<false>(f, p, rep);
+    HT3D_API_END
 }
 
 int ht3d_v18_darcy_solve(const ht3d_v18_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep) {
+    HT3D_API_BEGIN
     return ht3d::v18::solve<true>(f, p, rep);
+    HT3D_API_END
 }
 
 int ht3d_v18_tile_count(void) { return ht3d::v18::kNumV18Tiles; }
 
 int ht3d_v18_tune(int force_tile, int *last_tile) {
+    HT3D_API_BEGIN
     HT3D_CHECK_INIT();
     if (force_tile >= ht3d::v18::kNumV18Tiles) return ht3d::arg_fail("ht3d_v18_tune: tile index"), HT3D_ERR_ARG;
     ht3d::v18::g_v18_force_tile = force_tile;
     if (last_tile) *last_tile = ht3d::v18::g_v18_last_tile;
     return HT3D_OK;
+    HT3D_API_END
 }
 
 }  // extern "C"

@@ -20,7 +20,8 @@
  *                 sweeps + FP64 defect correction) solve.
  *
  * Conventions: every function returns 0 (HT3D_OK) or an error code; nothing
- * throws across the ABI.  Buffers are owned by the caller (ht3d_alloc/ht3d_free);
+ * throws across the ABI (each entry catches C++ exceptions and returns
+ * HT3D_ERR_NOMEM for a failed host allocation, HT3D_ERR_INTERNAL otherwise).  Buffers are owned by the caller (ht3d_alloc/ht3d_free);
  * the library never frees caller memory.  One host thread per context.  Kernels
  * run on the library stream; the per-kernel entries synchronise before returning
  * (ParallelStencil launches are synchronous) unless ht3d_set_sync(0) was called.
@@ -41,6 +42,7 @@ enum {
     HT3D_ERR_HIP = 2,        /* HIP runtime error (see ht3d_last_error)         */
     HT3D_ERR_NOMEM = 3,
     HT3D_ERR_STATE = 4,      /* library not initialised                         */
+    HT3D_ERR_INTERNAL = 5,   /* a C++ exception caught at the ABI boundary      */
     HT3D_NOT_CONVERGED = 10, /* solve hit nitmax (silent in the reference)      */
     HT3D_NAN = 11,           /* v18:464 error("Nan ...")                        */
     HT3D_DIVERGED = 12       /* v18:465 error("Diverged ...")                   */
@@ -55,6 +57,10 @@ const char *ht3d_last_error(void);
 int ht3d_set_sync(int synchronous);
 int ht3d_sync(void);
 int ht3d_device_name(char *buf, size_t len);
+/* Test hook, no reference counterpart: the next `count` calls of guarded
+ * entries fail as if a host allocation had thrown (HT3D_ERR_NOMEM), so callers
+ * can exercise their error path without exhausting memory.  0 disarms it. */
+int ht3d_debug_inject_host_failure(int count);
 
 /* @zeros / Data.Array(host) / Array(device) (HydroThermal3D_v7.jl:87-121,153-157) */
 int ht3d_alloc(size_t bytes, void **dptr);

@@ -80,3 +80,50 @@ def test_check_interval_rule():
         best = math.sqrt(2 * it * tc / ti)
         divs = [c for c in (10, 20, 25, 50, 100, 200, 250, 500, 1000) if c <= nout and nout % c == 0]
         assert got == min(divs, key=lambda c: abs(math.log(c / best)))
+
+
+def test_exceptions_stop_at_the_abi():
+    """Every extern "C" entry runs its body through a guard that turns a C++
+    exception into a status code.  With the failure hook armed, entries return
+    HT3D_ERR_NOMEM (before any device access) and name themselves in the error
+    text; once disarmed they behave normally.  No GPU call is made."""
+    from hydrothermal3d_amd import _lib
+    L = _lib.lib()
+    try:
+        L.ht3d_debug_inject_host_failure(3)
+        assert L.ht3d_tune_check_interval(2000.0, 100.0, 70.0, 100) == _lib.ERR_NOMEM
+        assert b"ht3d_tune_check_interval" in L.ht3d_last_error()
+        assert L.ht3d_batch_tune_force(0) == _lib.ERR_NOMEM
+        assert L.ht3d_ComputeFlux(*([None] * 7), 1.0, 1.0, 1.0, 4, 4, 4) == _lib.ERR_NOMEM
+        assert b"host memory allocation failed" in L.ht3d_last_error()
+        assert L.ht3d_tune_check_interval(2000.0, 100.0, 70.0, 100) == 50     # the hook is spent
+    finally:
+        L.ht3d_debug_inject_host_failure(0)
+    assert L.ht3d_strerror(_lib.ERR_INTERNAL).decode() == "internal error"
+
+
+def test_every_entry_body_is_guarded():
+    """Source check: each extern "C" entry of csrc/ with a multi-line body opens
+    with HT3D_API_BEGIN and closes with HT3D_API_END; one-line entries only
+    return a constant or a field."""
+    unguarded = []
+    for src in sorted((ROOT / "hydrothermal3d_amd" / "csrc").glob("*.hip")):
+        lines = src.read_text().split("\n")
+        for n, ln in enumerate(lines):
+            m = re.match(r'^(?:extern "C" )?int (ht3d_\w+)\(', ln)
+            if not m or m.group(1) == "ht3d_debug_inject_host_failure":
+                continue
+            j = n
+            while not lines[j].rstrip().endswith(("{", ";", "}")):
+                j += 1
+            if lines[j].rstrip().endswith(";"):
+                continue
+            if lines[j].rstrip().endswith("}"):
+                assert re.search(r"\{ return [\w:]+; \}$", lines[j]), f"{src.name}: {m.group(1)}"
+                continue
+            k = j + 1
+            while lines[k] != "}":
+                k += 1
+            if lines[j + 1].strip() != "HT3D_API_BEGIN" or lines[k - 1].strip() != "HT3D_API_END":
+                unguarded.append(f"{src.name}: {m.group(1)}")
+    assert not unguarded, unguarded
