@@ -155,10 +155,12 @@ def _batch_state(ms, reps):
     return out
 
 
-def run_batch(n: int, steps: int, warmup: int, prm):
+def run_batch(n: int, steps: int, warmup: int, prm, graphs: bool = True,
+              modes=("plain_fp64", "batched_fp64", "batched_dc")):
     """Many-problem workload: plain FP64 (each model stepped alone), then the
-    batched FP64 and FP32+DC forms with graph-replayed iteration blocks, all on
-    the same inputs and the same number of steps."""
+    batched FP64 and FP32+DC forms with graph-replayed iteration blocks (kernel
+    by kernel when `graphs` is false), all on the same inputs and the same
+    number of steps; `modes` picks a subset of the three."""
     import numpy as np
     import hydrothermal3d_amd as ht
     from hydrothermal3d_amd import _lib, inputs
@@ -167,10 +169,11 @@ def run_batch(n: int, steps: int, warmup: int, prm):
     cells = sum(w.ncell for w in ws)
     res = {}
     states = {}
-    for mode in ("plain_fp64", "batched_fp64", "batched_dc"):
+    for mode in modes:
         precision = _lib.FP32_DC if mode == "batched_dc" else _lib.FP64
         ms = _batch_models(ws, prm, precision)
-        b = None if mode == "plain_fp64" else ht.HydroThermal3DBatch(ms, prm, prm, precision=precision, graphs=True)
+        b = None if mode == "plain_fp64" else ht.HydroThermal3DBatch(ms, prm, prm, precision=precision,
+                                                                     graphs=graphs)
         reps = None
         for s in range(warmup + steps):
             if s == warmup:
@@ -205,14 +208,35 @@ def run_batch(n: int, steps: int, warmup: int, prm):
                 "P": max(rel(x["P"], y["P"]) for x, y in zip(a, c)),
                 "Nu": max(abs(x["Nu"] - y["Nu"]) / abs(y["Nu"]) for x, y in zip(a, c))}
 
-    plain = res["plain_fp64"]["s_per_step"]
-    return {"workload": f"{n} problems, edge n in {{32,48,64,96,128}}, n^3 or n x n/2 x n, Ra log-uniform "
-                        f"[50,2000], sigma(log10 k) U[0,1], seeds 13000+p",
-            "problems": n, "cells": cells, "steps": steps, "warmup": warmup, **res,
-            "speedup_batched_fp64_vs_plain": round(plain / res["batched_fp64"]["s_per_step"], 3),
-            "speedup_batched_dc_vs_plain_fp64": round(plain / res["batched_dc"]["s_per_step"], 3),
-            "rel_err_batched_fp64_vs_plain": errs(states["batched_fp64"], states["plain_fp64"]),
-            "rel_err_batched_dc_vs_plain_fp64": errs(states["batched_dc"], states["plain_fp64"])}
+    out = {"workload": f"{n} problems, edge n in {{32,48,64,96,128}}, n^3 or n x n/2 x n, Ra log-uniform "
+                       f"[50,2000], sigma(log10 k) U[0,1], seeds 13000+p",
+           "problems": n, "cells": cells, "steps": steps, "warmup": warmup, "graphs": graphs, **res}
+    if "plain_fp64" in res:
+        plain = res["plain_fp64"]["s_per_step"]
+        for mode in ("batched_fp64", "batched_dc"):
+            if mode in res:
+                tag = "dc" if mode == "batched_dc" else "fp64"
+                out[f"speedup_batched_{tag}_vs_plain" + ("_fp64" if tag == "dc" else "")] = \
+                    round(plain / res[mode]["s_per_step"], 3)
+                out[f"rel_err_{mode}_vs_plain" + ("_fp64" if tag == "dc" else "")] = \
+                    errs(states[mode], states["plain_fp64"])
+    if "batched_fp64" in res and "batched_dc" in res:
+        out["speedup_batched_dc_vs_batched_fp64"] = round(res["batched_fp64"]["s_per_step"] /
+                                                          res["batched_dc"]["s_per_step"], 3)
+    return out
+
+
+def calibration_copies(copies: int) -> None:
+    """`copies` plain copies of a 256x128x256 cell array (ArrayEqualArray: N words
+    read, N written), whose counter readings calibrate FETCH_SIZE/WRITE_SIZE."""
+    import hydrothermal3d_amd as ht
+    from hydrothermal3d_amd import _lib
+    a = ht.DeviceArray.zeros((256, 128, 256))
+    b = ht.DeviceArray.zeros((256, 128, 256))
+    for _ in range(copies):
+        ht.kernels.ArrayEqualArray(a, b)
+    _lib.sync()
+    log(f"{copies} calibration copies of {8 * 256 * 128 * 256} bytes")
 
 
 def probe(w, iters: int, copies: int) -> int:
@@ -411,12 +435,20 @@ def main(argv=None) -> int:
     ap.add_argument("--batch-tol", type=float, default=1e-9,
                     help="DYREL tolerance of the batched leg (1e-9 keeps the solver error of both forms below 1e-6)")
     ap.add_argument("--batch-warmup", type=int, default=1)
+    ap.add_argument("--batch-graphs", type=int, choices=[0, 1], default=1,
+                    help="replay the batch's pseudo-iteration blocks as HIP graphs (1) or launch them kernel by "
+                         "kernel (0; the form to run under a profiler)")
+    ap.add_argument("--batch-modes", default="plain_fp64,batched_fp64,batched_dc",
+                    help="comma list of the batched leg's forms: plain_fp64, batched_fp64, batched_dc")
     ap.add_argument("--forward-only", action="store_true",
                     help="sweep every launch forwards (default: alternate the direction between launches)")
     ap.add_argument("--v18-only", action="store_true", help="run only the v18 leg and print its JSON")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched leg and print its JSON")
     ap.add_argument("--v18-iters", type=int, default=400,
                     help="pseudo-iterations per fused v18 solve in the v18 leg (0: skip it)")
+    ap.add_argument("--calib-copies", type=int, default=0,
+                    help="after the run, copy a 256x128x256 cell array this many times (ArrayEqualArray: 64 MiB read "
+                         "and 64 MiB written each), the yardstick of rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
     ap.add_argument("--probe", action="store_true",
                     help="counter probe only (fused iterations + calibration copies), no JSON line")
     args = ap.parse_args(argv)
@@ -442,7 +474,11 @@ def main(argv=None) -> int:
     if args.batch_only:
         log(f"batched leg: {max(args.batch, 1)} problems")
         print(json.dumps({"batch": run_batch(max(args.batch, 1), args.batch_steps, args.batch_warmup,
-                                             ht.production_params(tol=args.batch_tol, ncheck=ncheck))}), flush=True)
+                                             ht.production_params(tol=args.batch_tol, ncheck=ncheck),
+                                             bool(args.batch_graphs), tuple(args.batch_modes.split(",")))}),
+              flush=True)
+        if args.calib_copies > 0:
+            calibration_copies(args.calib_copies)
         return 0
     precision = _lib.FP32_DC if args.precision == "dc" else _lib.FP64
     log(f"device: {_lib.device_name()}; workload {w.name}; precision {args.precision}")
@@ -517,7 +553,8 @@ def main(argv=None) -> int:
     if args.batch > 0:
         log(f"batched leg: {args.batch} problems")
         out["batch"] = run_batch(args.batch, args.batch_steps, args.batch_warmup,
-                                 ht.production_params(tol=args.batch_tol, ncheck=ncheck))
+                                 ht.production_params(tol=args.batch_tol, ncheck=ncheck), bool(args.batch_graphs),
+                                 tuple(args.batch_modes.split(",")))
         out["batch"]["tol"] = args.batch_tol
     if args.v18_iters > 0:
         log("v18 leg: fused dimensional DYREL loops vs the unfused launch sequence")
@@ -527,6 +564,8 @@ def main(argv=None) -> int:
     if dc is not None:
         out["fp32_dc"] = dc
     print(json.dumps(out), flush=True)
+    if args.calib_copies > 0:
+        calibration_copies(args.calib_copies)
     return 0
 
 
