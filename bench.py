@@ -82,7 +82,8 @@ def run_gpu(w, steps: int, warmup: int, precision: int, prm):
         r = g.step()
         reports.append(r)
         log(f"step {s}: iters T {r['thermal']['iters']} P {r['darcy']['iters']} Nu {r['Nu']:.6f} "
-            f"dt {r['dt_used']:.3e} ({time.perf_counter() - t0:.2f} s)")
+            f"dt {r['dt_used']:.3e} norms T {r['thermal']['nF_abs']:.3e} P {r['darcy']['nF_abs']:.3e} "
+            f"({time.perf_counter() - t0:.2f} s)")
     _lib.sync()
     wall = time.perf_counter() - t0
     _lib.call("ht3d_timing_enable", 0)

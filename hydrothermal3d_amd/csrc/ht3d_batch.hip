@@ -104,9 +104,8 @@ __global__ __launch_bounds__(64 * TY) void batch_iter_kernel(const IterArgs<T> *
     }
 }
 
-// FP64 anchor residual of the defect-correction form (same tiles as the sweeps;
-// a two-cell tile marches its lanes' cell pairs in registers, a four-cell tile
-// is walked as 4 chunks of 64 cells)
+// FP64 anchor residual of the defect-correction form, on the same tiles as the
+// sweeps (dc_residual_tile: the single solve's residual body)
 template <bool TR, int TY, int KZ, int CPL>
 __global__ __launch_bounds__(64 * TY) void batch_dc_residual_kernel(const DcArgs *__restrict__ dtab,
                                                                     const IterArgs<float> *__restrict__ tab32,
@@ -119,27 +118,10 @@ __global__ __launch_bounds__(64 * TY) void batch_dc_residual_kernel(const DcArgs
         const unsigned t = xcd_tile(w, n);
         const uint2 it = items[t];
         if (!st[it.x].active) continue;
-        const DcArgs &a = dtab[it.x];
         int bx, by, bz;
         tile_of(it.y, tab32[it.x].nbx, tab32[it.x].nby, bx, by, bz);
-        const int j = by * TY + (int)(threadIdx.x >> 6);
-        const int k0 = bz * KZ, k1 = min(k0 + KZ, a.nz);
         double s_hi = 0.0, s_lo = 0.0, mx = 0.0;
-        if constexpr (CPL == 2) {   // the tile's lanes hold two x-neighbouring cells: the register march
-            const int i = bx * 128 + 2 * (int)(threadIdx.x & 63u);
-            if (i < a.nx && j < a.ny) dc_residual_march2<TR, KZ>(a, i, j, k0, k1, s_hi, s_lo, mx);
-        } else {
-#pragma unroll
-            for (int q = 0; q < CPL; ++q) {
-                const int i = (bx * CPL + q) * 64 + (int)(threadIdx.x & 63u);
-                if (i < a.nx && j < a.ny)
-                    for (int k = k0; k < k1; ++k) {
-                        const double af = fabs(dc_residual_cell<TR>(a, i, j, k));
-                        dd_add(s_hi, s_lo, af);
-                        mx = (af > mx || af != af) ? af : mx;
-                    }
-            }
-        }
+        dc_residual_tile<TR, TY, KZ, CPL>(dtab[it.x], bx, by, bz, s_hi, s_lo, mx);
         const double h4[4] = {s_hi, mx, 0.0, 0.0}, l4[4] = {s_lo, 0.0, 0.0, 0.0};
         block_reduce4(h4, l4, 0x2u, partials, t);
         __syncthreads();

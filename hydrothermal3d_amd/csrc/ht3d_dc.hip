@@ -24,41 +24,93 @@ int gershgorin_lmax(const double *kx, const double *ky, const double *kz, int nx
 int bc_fill_table(double *A, int nx, int ny, int nz, const int type[6], const double val[6]);
 int nu_sum(const double *T, const double *ky, int nx, int ny, int nz, double _dy, double *sum);
 
-// FP64 residual r(x) with on-the-fly ghosts; slots: 0 sum|r| (dd), 1 max|r|.
-// A thread marches kResZ planes (unrolled by 8: the loads of several planes in
-// flight), so a launch leaves few block partials for the finishing reduction.
-constexpr int kResZ = 16;
-template <bool TR>
-__global__ __launch_bounds__(256) void dc_residual_kernel(DcArgs a) {
-    const int i = blockIdx.x * 64 + (int)(threadIdx.x & 63u);
-    const int j = blockIdx.y * 4 + (int)(threadIdx.x >> 6);
-    const int k0 = blockIdx.z * kResZ, k1 = min(k0 + kResZ, a.nz);
+// FP64 residual r(x) with on-the-fly ghosts; partial slots: 0 sum|r| (dd),
+// 1 max|r|.  Tiles of dc_residual_tile in the XCD-contiguous order of the
+// iteration kernel, so the halo rows and planes a tile shares with its
+// neighbours are read while they are still in that XCD's L2.
+template <bool TR, int TY, int KZ, int CPL>
+__global__ __launch_bounds__(64 * TY) void dc_residual_tiled(DcArgs a, int nbx, int nby, int nbz) {
+    const unsigned t = xcd_tile(blockIdx.x, (unsigned)nbx * nby * nbz);
+    const int bx = t % nbx, by = (t / nbx) % nby, bz = t / (nbx * nby);
     double s_hi = 0.0, s_lo = 0.0, mx = 0.0;
-    if (i < a.nx && j < a.ny) {
-#pragma unroll 8
-        for (int k = k0; k < k1; ++k) {
-            const double af = fabs(dc_residual_cell<TR>(a, i, j, k));
-            dd_add(s_hi, s_lo, af);
-            mx = (af > mx || af != af) ? af : mx;
-        }
-    }
+    dc_residual_tile<TR, TY, KZ, CPL>(a, bx, by, bz, s_hi, s_lo, mx);
     const double h4[4] = {s_hi, mx, 0.0, 0.0}, l4[4] = {s_lo, 0.0, 0.0, 0.0};
-    const unsigned bl = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    block_reduce4(h4, l4, 0x2u, a.partials, bl);
+    block_reduce4(h4, l4, 0x2u, a.partials, blockIdx.x);
 }
 
-// dc_residual_kernel with two x-cells per thread (nx even), marching kResZ
-// planes with dc_residual_march2
-template <bool TR>
-__global__ __launch_bounds__(256) void dc_residual_kernel2(DcArgs a) {
-    const int i = blockIdx.x * 128 + 2 * (int)(threadIdx.x & 63u);
-    const int j = blockIdx.y * 4 + (int)(threadIdx.x >> 6);
-    const int k0 = blockIdx.z * kResZ, k1 = min(k0 + kResZ, a.nz);
-    double s_hi = 0.0, s_lo = 0.0, mx = 0.0;
-    if (i < a.nx && j < a.ny) dc_residual_march2<TR, 8>(a, i, j, k0, k1, s_hi, s_lo, mx);
-    const double h4[4] = {s_hi, mx, 0.0, 0.0}, l4[4] = {s_lo, 0.0, 0.0, 0.0};
-    const unsigned bl = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    block_reduce4(h4, l4, 0x2u, a.partials, bl);
+// residual tiles (rows, planes, x-cells per lane); the solve times the ones
+// the grid allows (nx % cells per lane == 0) on its first anchor of a grid
+struct ResTile {
+    int ty, kz, cpl;
+};
+static constexpr ResTile kResTiles[] = {{4, 8, 2}, {4, 16, 2}, {8, 8, 2}, {8, 16, 2}, {2, 16, 2},
+                                        {4, 32, 2}, {4, 16, 4}, {4, 16, 1}, {8, 16, 1}};
+static constexpr int kNumResTiles = sizeof(kResTiles) / sizeof(kResTiles[0]);
+
+static unsigned res_blocks(int tile, int nx, int ny, int nz) {
+    const ResTile &t = kResTiles[tile];
+    return div_up(nx, 64 * t.cpl) * div_up(ny, t.ty) * div_up(nz, t.kz);
+}
+
+static int launch_residual(const DcArgs &d, bool tr, int tile, unsigned *nblocks) {
+    const ResTile &t = kResTiles[tile];
+    const int nbx = (int)div_up(d.nx, 64 * t.cpl), nby = (int)div_up(d.ny, t.ty), nbz = (int)div_up(d.nz, t.kz);
+    const unsigned nb = (unsigned)nbx * nby * nbz;
+    hipStream_t s = ctx().stream;
+#define RT_(TY, KZ, CPL)                                                                                              \
+    do {                                                                                                              \
+        if (tr) hipLaunchKernelGGL((dc_residual_tiled<true, TY, KZ, CPL>), dim3(nb), dim3(64 * TY), 0, s, d, nbx, nby, nbz); \
+        else hipLaunchKernelGGL((dc_residual_tiled<false, TY, KZ, CPL>), dim3(nb), dim3(64 * TY), 0, s, d, nbx, nby, nbz); \
+    } while (0)
+    switch (tile) {
+    case 0: RT_(4, 8, 2); break;
+    case 1: RT_(4, 16, 2); break;
+    case 2: RT_(8, 8, 2); break;
+    case 3: RT_(8, 16, 2); break;
+    case 4: RT_(2, 16, 2); break;
+    case 5: RT_(4, 32, 2); break;
+    case 6: RT_(4, 16, 4); break;
+    case 7: RT_(4, 16, 1); break;
+    default: RT_(8, 16, 1); break;
+    }
+#undef RT_
+    HIP_TRY(hipGetLastError());
+    *nblocks = nb;
+    return HT3D_OK;
+}
+
+// The fastest residual tile for the grid: three interleaved rounds of 4 timed
+// launches per tile, best of each (the residual reads X and writes only the
+// FP32 right-hand side and the partials, both overwritten by the next anchor).
+static int tune_residual(const DcArgs &d, bool tr, int *tile_out) {
+    Context &cx = ctx();
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    double best[kNumResTiles];
+    for (int t = 0; t < kNumResTiles; ++t) best[t] = 1e300;
+    for (int round = 0; round < 3; ++round)
+        for (int t = 0; t < kNumResTiles; ++t) {
+            if (d.nx % kResTiles[t].cpl) continue;
+            unsigned nb = 0;
+            int rc = launch_residual(d, tr, t, &nb);
+            if (rc) return rc;
+            HIP_TRY(hipEventRecord(e0, cx.stream));
+            for (int r = 0; r < 4 && !rc; ++r) rc = launch_residual(d, tr, t, &nb);
+            if (rc) return rc;
+            HIP_TRY(hipEventRecord(e1, cx.stream));
+            HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            best[t] = ms < best[t] ? ms : best[t];
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    int bt = kResTiles[7].cpl == 1 ? 7 : 0;
+    for (int t = 0; t < kNumResTiles; ++t)
+        if (best[t] < best[bt]) bt = t;
+    *tile_out = bt;
+    return HT3D_OK;
 }
 
 // DYREL scalars of the FP32 sweeps, kept on the device: st = {lmin, h1, h2,
@@ -148,11 +200,10 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     void *f0b = nullptr, *d32 = nullptr;
     if ((rc = workspace(WS_F32_F0B, N * sizeof(float), &f0b))) return rc;
     if ((rc = workspace(WS_F32_DIAG, N * sizeof(float), &d32))) return rc;
-    const unsigned max_blocks = div_up(nx, 64) * div_up(ny, 1) * div_up(nz, 2);
-    const unsigned res_blocks = div_up(nx, 64) * div_up(ny, 4) * div_up(nz, kResZ);
-    if ((rc = workspace(WS_PARTIALS, (size_t)(max_blocks > res_blocks ? max_blocks : res_blocks) * 8 * sizeof(double),
-                        &part)))
-        return rc;
+    unsigned max_blocks = div_up(nx, 64) * div_up(ny, 1) * div_up(nz, 2);
+    for (int t = 0; t < kNumResTiles; ++t)
+        if (res_blocks(t, nx, ny, nz) > max_blocks) max_blocks = res_blocks(t, nx, ny, nz);
+    if ((rc = workspace(WS_PARTIALS, (size_t)max_blocks * 8 * sizeof(double), &part))) return rc;
     if ((rc = workspace(WS_RESULT, 64 * sizeof(double), &res))) return rc;
     float *kx32 = (float *)k32, *ky32 = kx32 + nfx, *kz32 = ky32 + nfy;
     if ((rc = to_float(kx32, f->kx, nfx)) || (rc = to_float(ky32, f->ky, nfy)) || (rc = to_float(kz32, f->kz, nfz)))
@@ -172,9 +223,6 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     d.partials = (double *)part; d.nx = nx; d.ny = ny; d.nz = nz;
     d._dx = 1.0 / f->dx; d._dy = 1.0 / f->dy; d._dz = 1.0 / f->dz; d._dt = 1.0 / f->dt;
     for (int q = 0; q < 6; ++q) { d.bct[q] = f->bc_type[q]; d.bcv[q] = f->bc_val[q]; }
-    // residual: kResZ planes per block, two x-cells per thread when nx is even
-    const bool res2 = nx % 2 == 0;
-    const dim3 rgrid(div_up(nx, res2 ? 128 : 64), div_up(ny, 4), div_up(nz, kResZ));
     const dim3 ugrid(div_up(NE, 512));                                     // update: two entries per thread
 
     IterArgs<float> a;
@@ -211,6 +259,15 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     // FP64 solve up to FP32 rounding, and re-anchoring keeps e small enough that
     // FP32 never has to resolve more than the residual drop between anchors.
     TuneEntry *te = tune_find(TuneKey{nx, ny, nz, f->transient != 0 ? 1 : 0, 1});
+    int res_tile = te && te->res_tile >= 0 ? te->res_tile : -1;
+    if (res_tile < 0) {
+        if (g_autotune && g_force_tile < 0) {
+            if ((rc = tune_residual(d, f->transient != 0, &res_tile))) return rc;
+        } else {
+            res_tile = nx % 2 == 0 ? 1 : 7;   // 4 x 16 planes, two x-cells per lane when nx is even
+        }
+        if (te) te->res_tile = res_tile;
+    }
     int anchor = p->ncheck > 0 ? p->ncheck : (p->nitout > 0 ? p->nitout : 50);
     if (p->ncheck < 0)   // the check-interval tuner, with the anchor as the check
         anchor = tuned_check_interval(te ? te->last_iters : -1.0,
@@ -248,15 +305,9 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
         const bool time_anchor = outer == 1 && timed;
         if (time_anchor) HIP_TRY(hipStreamSynchronize(cx.stream));   // time the second anchor alone
         const auto ta = std::chrono::steady_clock::now();
-        if (res2) {
-            if (f->transient) hipLaunchKernelGGL(dc_residual_kernel2<true>, rgrid, dim3(256), 0, cx.stream, d);
-            else hipLaunchKernelGGL(dc_residual_kernel2<false>, rgrid, dim3(256), 0, cx.stream, d);
-        } else {
-            if (f->transient) hipLaunchKernelGGL(dc_residual_kernel<true>, rgrid, dim3(256), 0, cx.stream, d);
-            else hipLaunchKernelGGL(dc_residual_kernel<false>, rgrid, dim3(256), 0, cx.stream, d);
-        }
-        HIP_TRY(hipGetLastError());
-        if ((rc = finish_partials((double *)part, rgrid.x * rgrid.y * rgrid.z, 0x2u, (double *)res))) return rc;
+        unsigned rnb = 0;
+        if ((rc = launch_residual(d, f->transient != 0, res_tile, &rnb))) return rc;
+        if ((rc = finish_partials((double *)part, rnb, 0x2u, (double *)res))) return rc;
         if ((rc = post_result4((double *)res))) return rc;
         // The next run's first sweep is queued before the anchor norm is waited
         // for, so the GPU works through the read-back.  Sweeps write only e and

@@ -813,6 +813,7 @@ struct TuneEntry {
     double check_us = -1.0;   // extra cost of one convergence check (check launch, finish, read-back)
     double last_iters = -1.0; // iterations of the last solve with this key
     double anchor_us = -1.0;  // defect-correction anchor: FP64 residual, finish, read-back
+    int res_tile = -1;        // defect-correction residual tile (ht3d_dc.hip)
 };
 extern std::vector<TuneEntry> g_tune;
 extern bool g_autotune;
@@ -1037,10 +1038,9 @@ inline int choose_cfg(IterArgs<T> &a, bool transient, bool can_store_d, int prec
 }
 
 // ------------------------------------------------------------ defect correction
-// FP64 residual of the dimensionless operators at one cell, with on-the-fly
-// ghost values: r = R [+ x/dt] + div q(x) (ResidualDiffusion(Transient)!,
-// kernels_HT3D.jl:131, HydroThermal3D_Alban_v6.jl:87); r is also stored in FP32
-// as the right-hand side of the correction equation.
+// Inputs of the FP64 residual of the dimensionless operators, with on-the-fly
+// ghost values: r = R [+ x/dt] + div q(x); r is also stored in FP32 as the
+// right-hand side of the correction equation.
 struct DcArgs {
     const double *__restrict__ X;   // E
     const double *__restrict__ R;   // c
@@ -1055,102 +1055,71 @@ struct DcArgs {
     double bcv[6];
 };
 
-template <bool TR>
-__device__ __forceinline__ double dc_residual_cell(const DcArgs &a, int i, int j, int k) {
+// The FP64 defect-correction residual of one (x, y) tile of 64*CPL x TY cells
+// marched over the planes [bz*KZ, +KZ): the load pattern of iter_tile_v
+// (X(k-1), X(k), X(k+1) and kz(k), kz(k+1) in registers, CPL-element vector
+// loads, a face between two cells of a lane loaded once); per cell r = R
+// [+ x/dt] + div q(x) in the order of ComputeFlux! and
+// ResidualDiffusion(Transient)! (kernels_HT3D.jl:112-114,131,
+// HydroThermal3D_Alban_v6.jl:87).  Writes r32, adds |r| to the double-double
+// sum (s_hi, s_lo) and to the max mx.  It reads five words and writes half a word per
+// cell, the fused iteration's stencil without the update; a full chunk of KZ
+// planes is unrolled so later planes' loads issue ahead.  Needs nx % CPL == 0.
+template <bool TR, int TY, int KZ, int CPL>
+__device__ __forceinline__ void dc_residual_tile(const DcArgs &a, int bx, int by, int bz, double &s_hi, double &s_lo,
+                                                 double &mx) {
+    typedef typename CellVec<double, CPL>::type V;
+    typedef typename CellVec<float, CPL>::type FV;
+    const int i = bx * (64 * CPL) + CPL * (int)(threadIdx.x & 63u);
+    const int j = by * TY + (int)(threadIdx.x >> 6);
+    const int k0 = bz * KZ, k1 = min(k0 + KZ, a.nz);
     const uint32_t nx = a.nx, ny = a.ny;
-    const uint32_t sE1 = nx + 2, sE2 = sE1 * (ny + 2);
-    const uint32_t e = (uint32_t)(i + 1) + (uint32_t)(j + 1) * sE1 + (uint32_t)(k + 1) * sE2;
-    const uint32_t c = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k * nx * ny;
-    const uint32_t fx = (uint32_t)i + (uint32_t)j * (nx + 1) + (uint32_t)k * (nx + 1) * ny;
-    const uint32_t fy = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k * nx * (ny + 1);
-    const double xc = a.X[e];
-    double xW = a.X[e - 1], xE = a.X[e + 1], xS = a.X[e - sE1], xN = a.X[e + sE1], xB = a.X[e - sE2],
-           xF = a.X[e + sE2];
-    if (i == 0) xW = ghost(a.bct[0], a.bcv[0], xc);
-    if ((uint32_t)i == nx - 1) xE = ghost(a.bct[1], a.bcv[1], xc);
-    if (j == 0) xS = ghost(a.bct[2], a.bcv[2], xc);
-    if ((uint32_t)j == ny - 1) xN = ghost(a.bct[3], a.bcv[3], xc);
-    if (k == 0) xB = ghost(a.bct[4], a.bcv[4], xc);
-    if (k == a.nz - 1) xF = ghost(a.bct[5], a.bcv[5], xc);
-    const double kW = a.kx[fx], kE = a.kx[fx + 1], kS = a.ky[fy], kN = a.ky[fy + nx], kB = a.kz[c],
-                 kF = a.kz[c + nx * ny];
-    const double qW = -kW * (a._dx * (xc - xW)), qE = -kE * (a._dx * (xE - xc));
-    const double qS = -kS * (a._dy * (xc - xS)), qN = -kN * (a._dy * (xN - xc));
-    const double qB = -kB * (a._dz * (xc - xB)), qF = -kF * (a._dz * (xF - xc));
-    double f;
-    if (TR) f = a.R[c] + a._dt * xc + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
-    else f = a.R[c] + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
-    a.r32[c] = (float)f;
-    return f;
-}
-
-// The FP64 defect-correction residual r(x) of two x-neighbouring cells (i,
-// i+1; nx even) marched over the planes [k0, k1): X, the face coefficients, R
-// and r32 as 2-element vector loads and stores, X(k-1), X(k), X(k+1) and
-// kz(k), kz(k+1) kept in registers, the cell formula and order of
-// dc_residual_cell.  Adds |r| to the double-double sum (s_hi, s_lo) and to the
-// max mx.  The march is unrolled by UNROLL.
-template <bool TR, int UNROLL>
-__device__ __forceinline__ void dc_residual_march2(const DcArgs &a, int i, int j, int k0, int k1, double &s_hi,
-                                                   double &s_lo, double &mx) {
-    typedef CellVec<double, 2>::type V;
-    typedef CellVec<float, 2>::type F2;
-    const uint32_t nx = a.nx, ny = a.ny;
+    if ((uint32_t)i >= nx || (uint32_t)j >= ny) return;
     const uint32_t sE1 = nx + 2, sE2 = sE1 * (ny + 2), sC2 = nx * ny, sX2 = (nx + 1) * ny, sY2 = nx * (ny + 1);
+    const bool wlo = i == 0, whi = (uint32_t)i + CPL == nx, slo = j == 0, shi = (uint32_t)j == ny - 1;
     uint32_t e = (uint32_t)(i + 1) + (uint32_t)(j + 1) * sE1 + (uint32_t)(k0 + 1) * sE2;
     uint32_t c = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k0 * sC2;
     uint32_t fx = (uint32_t)i + (uint32_t)j * (nx + 1) + (uint32_t)k0 * sX2;
     uint32_t fy = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k0 * sY2;
-    V xc = ldv<double, 2>(a.X + e), xB = ldv<double, 2>(a.X + e - sE2), kB = ldv<double, 2>(a.kz + c);
+    V xc = ldv<double, CPL>(a.X + e), xB = ldv<double, CPL>(a.X + e - sE2), kB = ldv<double, CPL>(a.kz + c);
     if (k0 == 0) {
-        xB.x = ghost(a.bct[4], a.bcv[4], (double)xc.x);
-        xB.y = ghost(a.bct[4], a.bcv[4], (double)xc.y);
-    }
-    constexpr int kUnroll = UNROLL;
-#pragma unroll kUnroll
-    for (int k = k0; k < k1; ++k) {
-        V xF = ldv<double, 2>(a.X + e + sE2);
-        if (k == a.nz - 1) {
-            xF.x = ghost(a.bct[5], a.bcv[5], (double)xc.x);
-            xF.y = ghost(a.bct[5], a.bcv[5], (double)xc.y);
-        }
-        double xW = a.X[e - 1], xE = a.X[e + 2];
-        V xS = ldv<double, 2>(a.X + e - sE1), xN = ldv<double, 2>(a.X + e + sE1);
-        if (i == 0) xW = ghost(a.bct[0], a.bcv[0], (double)xc.x);
-        if ((uint32_t)i + 2 == nx) xE = ghost(a.bct[1], a.bcv[1], (double)xc.y);
-        if (j == 0) {
-            xS.x = ghost(a.bct[2], a.bcv[2], (double)xc.x);
-            xS.y = ghost(a.bct[2], a.bcv[2], (double)xc.y);
-        }
-        if ((uint32_t)j == ny - 1) {
-            xN.x = ghost(a.bct[3], a.bcv[3], (double)xc.x);
-            xN.y = ghost(a.bct[3], a.bcv[3], (double)xc.y);
-        }
-        const double kW = a.kx[fx];
-        const V kE = ldv<double, 2>(a.kx + fx + 1);
-        const V kS = ldv<double, 2>(a.ky + fy), kN = ldv<double, 2>(a.ky + fy + nx), kF = ldv<double, 2>(a.kz + c + sC2);
-        const V R = ldv<double, 2>(a.R + c);
-        V f;
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const double x0 = xc[q], w = q == 0 ? xW : (double)xc[0], ea = q == 1 ? xE : (double)xc[1];
-            const double kw = q == 0 ? kW : (double)kE[0];
+        for (int q = 0; q < CPL; ++q) xB[q] = ghost(a.bct[4], a.bcv[4], (double)xc[q]);
+    }
+    auto plane = [&](int k) {
+        V xF = ldv<double, CPL>(a.X + e + sE2);
+        double xW = a.X[e - 1], xE = a.X[e + CPL];
+        V xS = ldv<double, CPL>(a.X + e - sE1), xN = ldv<double, CPL>(a.X + e + sE1);
+        const double kW = a.kx[fx];
+        const V kE = ldv<double, CPL>(a.kx + fx + 1);
+        const V kS = ldv<double, CPL>(a.ky + fy), kN = ldv<double, CPL>(a.ky + fy + nx);
+        const V kF = ldv<double, CPL>(a.kz + c + sC2);
+        const V R = ldv_nt<double, CPL>(a.R + c);
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) {
+            if (k + 1 == a.nz) xF[q] = ghost(a.bct[5], a.bcv[5], (double)xc[q]);
+            if (slo) xS[q] = ghost(a.bct[2], a.bcv[2], (double)xc[q]);
+            if (shi) xN[q] = ghost(a.bct[3], a.bcv[3], (double)xc[q]);
+        }
+        if (wlo) xW = ghost(a.bct[0], a.bcv[0], (double)xc[0]);
+        if (whi) xE = ghost(a.bct[1], a.bcv[1], (double)xc[CPL - 1]);
+        FV r;
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) {
+            const double x0 = xc[q], w = q == 0 ? xW : (double)xc[q - 1], ea = q == CPL - 1 ? xE : (double)xc[q + 1];
+            const double kw = q == 0 ? kW : (double)kE[q - 1];
             const double qW = -kw * (a._dx * (x0 - w)), qE = -kE[q] * (a._dx * (ea - x0));
             const double qS = -kS[q] * (a._dy * (x0 - xS[q])), qN = -kN[q] * (a._dy * (xN[q] - x0));
             const double qB = -kB[q] * (a._dz * (x0 - xB[q])), qF = -kF[q] * (a._dz * (xF[q] - x0));
-            if (TR) f[q] = R[q] + a._dt * x0 + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
-            else f[q] = R[q] + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
-        }
-        F2 r;
-        r.x = (float)f.x;
-        r.y = (float)f.y;
-        stv<float, 2>(r, a.r32 + c);
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const double af = fabs((double)f[q]);
+            double f;
+            if (TR) f = R[q] + a._dt * x0 + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
+            else f = R[q] + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
+            r[q] = (float)f;
+            const double af = fabs(f);
             dd_add(s_hi, s_lo, af);
             mx = (af > mx || af != af) ? af : mx;
         }
+        stv_nt<float, CPL>(r, a.r32 + c);
         xB = xc;
         xc = xF;
         kB = kF;
@@ -1158,6 +1127,12 @@ __device__ __forceinline__ void dc_residual_march2(const DcArgs &a, int i, int j
         c += sC2;
         fx += sX2;
         fy += sY2;
+    };
+    if (k1 - k0 == KZ) {
+#pragma unroll
+        for (int q = 0; q < KZ; ++q) plane(k0 + q);
+    } else {
+        for (int k = k0; k < k1; ++k) plane(k);
     }
 }
 

@@ -708,6 +708,84 @@ int ht3d_Boundaries_Weno5(double *Fc_exxx, const double *Fc, int axis, int src_e
     HT3D_API_END
 }
 
+// Per-name entries of tools/Weno5_Routines.jl:70-255 with the reference's
+// argument orders: the six one-sided difference kernels and the three ghost
+// fills of a cell array (Fc is c); the _v2 fills take the extended array (Fc
+// is E, tools/Weno5_Routines_v2.jl:221-347).  Each forwards to the
+// parameterised kernel above.
+int ht3d_Gradients_minus_x_Weno5(double *v1, double *v2, double *v3, double *v4, double *v5, const double *Fc_exxx,
+                              double _dx, double _dy, double _dz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
+    return ht3d_Gradients_Weno5(v1, v2, v3, v4, v5, Fc_exxx, 1, 0, _dx, _dy, _dz, nx, ny, nz);
+    HT3D_API_END
+}
+int ht3d_Gradients_plus_x_Weno5(double *v1, double *v2, double *v3, double *v4, double *v5, const double *Fc_exxx,
+                              double _dx, double _dy, double _dz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
+    return ht3d_Gradients_Weno5(v1, v2, v3, v4, v5, Fc_exxx, 1, 1, _dx, _dy, _dz, nx, ny, nz);
+    HT3D_API_END
+}
+int ht3d_Gradients_minus_y_Weno5(double *v1, double *v2, double *v3, double *v4, double *v5, const double *Fc_exxx,
+                              double _dx, double _dy, double _dz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
+    return ht3d_Gradients_Weno5(v1, v2, v3, v4, v5, Fc_exxx, 2, 0, _dx, _dy, _dz, nx, ny, nz);
+    HT3D_API_END
+}
+int ht3d_Gradients_plus_y_Weno5(double *v1, double *v2, double *v3, double *v4, double *v5, const double *Fc_exxx,
+                              double _dx, double _dy, double _dz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
+    return ht3d_Gradients_Weno5(v1, v2, v3, v4, v5, Fc_exxx, 2, 1, _dx, _dy, _dz, nx, ny, nz);
+    HT3D_API_END
+}
+int ht3d_Gradients_minus_z_Weno5(double *v1, double *v2, double *v3, double *v4, double *v5, const double *Fc_exxx,
+                              double _dx, double _dy, double _dz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
+    return ht3d_Gradients_Weno5(v1, v2, v3, v4, v5, Fc_exxx, 3, 0, _dx, _dy, _dz, nx, ny, nz);
+    HT3D_API_END
+}
+int ht3d_Gradients_plus_z_Weno5(double *v1, double *v2, double *v3, double *v4, double *v5, const double *Fc_exxx,
+                              double _dx, double _dy, double _dz, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
+    return ht3d_Gradients_Weno5(v1, v2, v3, v4, v5, Fc_exxx, 3, 1, _dx, _dy, _dz, nx, ny, nz);
+    HT3D_API_END
+}
+int ht3d_Boundaries_x_Weno5(double *Fc_exxx, const double *Fc, int type_W, double val_W, int type_E,
+                              double val_E, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
+    return ht3d_Boundaries_Weno5(Fc_exxx, Fc, 1, 0, type_W, val_W, type_E, val_E, nx, ny, nz);
+    HT3D_API_END
+}
+int ht3d_Boundaries_x_Weno5_v2(double *Fc_exxx, const double *Fc, int type_W, double val_W, int type_E,
+                              double val_E, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
+    return ht3d_Boundaries_Weno5(Fc_exxx, Fc, 1, 1, type_W, val_W, type_E, val_E, nx, ny, nz);
+    HT3D_API_END
+}
+int ht3d_Boundaries_y_Weno5(double *Fc_exxx, const double *Fc, int type_S, double val_S, int type_N,
+                              double val_N, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
+    return ht3d_Boundaries_Weno5(Fc_exxx, Fc, 2, 0, type_S, val_S, type_N, val_N, nx, ny, nz);
+    HT3D_API_END
+}
+int ht3d_Boundaries_y_Weno5_v2(double *Fc_exxx, const double *Fc, int type_S, double val_S, int type_N,
+                              double val_N, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
+    return ht3d_Boundaries_Weno5(Fc_exxx, Fc, 2, 1, type_S, val_S, type_N, val_N, nx, ny, nz);
+    HT3D_API_END
+}
+int ht3d_Boundaries_z_Weno5(double *Fc_exxx, const double *Fc, int type_B, double val_B, int type_F,
+                              double val_F, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
+    return ht3d_Boundaries_Weno5(Fc_exxx, Fc, 3, 0, type_B, val_B, type_F, val_F, nx, ny, nz);
+    HT3D_API_END
+}
+int ht3d_Boundaries_z_Weno5_v2(double *Fc_exxx, const double *Fc, int type_B, double val_B, int type_F,
+                              double val_F, int nx, int ny, int nz) {
+    HT3D_API_BEGIN
+    return ht3d_Boundaries_Weno5(Fc_exxx, Fc, 3, 1, type_B, val_B, type_F, val_F, nx, ny, nz);
+    HT3D_API_END
+}
+
 int ht3d_Advect(double *Fc, const double *Vxp, const double *dTdxm, const double *Vxm, const double *dTdxp,
                 double dt, int nx, int ny, int nz) {
     HT3D_API_BEGIN

@@ -267,48 +267,42 @@ VzPlusMinus = _make("VzPlusMinus", "ht3d_VzPlusMinus", [("Vzm", "c"), ("Vzp", "c
 dFdx_Weno5 = _make("dFdx_Weno5", "ht3d_dFdx_Weno5",
                    [("dFdxi", "c"), ("V1", "c"), ("V2", "c"), ("V3", "c"), ("V4", "c"), ("V5", "c")],
                    "tools/Weno5_Routines.jl:44-68")
-_Gradients = _make("Gradients_Weno5", "ht3d_Gradients_Weno5",
-                   [("v1", "c"), ("v2", "c"), ("v3", "c"), ("v4", "c"), ("v5", "c"), ("Fc_exxx", "X6"), ("axis", "i"),
-                    ("plus", "i"), ("_dx", "d"), ("_dy", "d"), ("_dz", "d")], "tools/Weno5_Routines.jl:70-128")
+_GRAD = [("v1", "c"), ("v2", "c"), ("v3", "c"), ("v4", "c"), ("v5", "c"), ("Fc_exxx", "X6"), ("_dx", "d"), ("_dy", "d"),
+         ("_dz", "d")]
+Gradients_minus_x_Weno5 = _make("Gradients_minus_x_Weno5", "ht3d_Gradients_minus_x_Weno5", _GRAD,
+                                "tools/Weno5_Routines.jl:70-78")
+Gradients_plus_x_Weno5 = _make("Gradients_plus_x_Weno5", "ht3d_Gradients_plus_x_Weno5", _GRAD,
+                               "tools/Weno5_Routines.jl:80-88")
+Gradients_minus_y_Weno5 = _make("Gradients_minus_y_Weno5", "ht3d_Gradients_minus_y_Weno5", _GRAD,
+                                "tools/Weno5_Routines.jl:90-98")
+Gradients_plus_y_Weno5 = _make("Gradients_plus_y_Weno5", "ht3d_Gradients_plus_y_Weno5", _GRAD,
+                               "tools/Weno5_Routines.jl:100-108")
+Gradients_minus_z_Weno5 = _make("Gradients_minus_z_Weno5", "ht3d_Gradients_minus_z_Weno5", _GRAD,
+                                "tools/Weno5_Routines.jl:110-118")
+Gradients_plus_z_Weno5 = _make("Gradients_plus_z_Weno5", "ht3d_Gradients_plus_z_Weno5", _GRAD,
+                               "tools/Weno5_Routines.jl:120-128")
 
 
-def _gradients(axis: int, plus: int, name: str, ref: str):
-    def fn(v1, v2, v3, v4, v5, Fc_exxx, _dx, _dy, _dz):
-        _Gradients(v1, v2, v3, v4, v5, Fc_exxx, axis, plus, _dx, _dy, _dz)
-
-    fn.__name__ = name
-    fn.__doc__ = f"{name}!(v1, v2, v3, v4, v5, Fc_exxx, _dx, _dy, _dz) -- {ref}"
-    return fn
-
-
-Gradients_minus_x_Weno5 = _gradients(1, 0, "Gradients_minus_x_Weno5", "tools/Weno5_Routines.jl:70-78")
-Gradients_plus_x_Weno5 = _gradients(1, 1, "Gradients_plus_x_Weno5", "tools/Weno5_Routines.jl:80-88")
-Gradients_minus_y_Weno5 = _gradients(2, 0, "Gradients_minus_y_Weno5", "tools/Weno5_Routines.jl:90-98")
-Gradients_plus_y_Weno5 = _gradients(2, 1, "Gradients_plus_y_Weno5", "tools/Weno5_Routines.jl:100-108")
-Gradients_minus_z_Weno5 = _gradients(3, 0, "Gradients_minus_z_Weno5", "tools/Weno5_Routines.jl:110-118")
-Gradients_plus_z_Weno5 = _gradients(3, 1, "Gradients_plus_z_Weno5", "tools/Weno5_Routines.jl:120-128")
-
-
-def _boundaries(axis: int, name: str, ref: str):
+def _boundaries(axis: str, name: str, ref: str):
     def fn(Fc_exxx: DeviceArray, Fc: DeviceArray, type_lo: int, val_lo: float, type_hi: int, val_hi: float):
         grid = _grid_from(Fc_exxx, "X6")
         _check(Fc_exxx, "X6", grid, name, "Fc_exxx")
         if Fc.shape == tuple(g + 2 for g in grid):
-            src_ext = 1      # tools/Weno5_Routines_v2.jl (Fc is the extended array)
+            entry = f"ht3d_Boundaries_{axis}_Weno5_v2"    # tools/Weno5_Routines_v2.jl (Fc is the extended array)
         else:
             _check(Fc, "c", grid, name, "Fc")
-            src_ext = 0      # tools/Weno5_Routines.jl (Fc is the cell array)
-        _lib.call("ht3d_Boundaries_Weno5", C.c_void_p(Fc_exxx.ptr), C.c_void_p(Fc.ptr), axis, src_ext, int(type_lo),
-                  C.c_double(val_lo), int(type_hi), C.c_double(val_hi), *grid)
+            entry = f"ht3d_Boundaries_{axis}_Weno5"       # tools/Weno5_Routines.jl (Fc is the cell array)
+        _lib.call(entry, C.c_void_p(Fc_exxx.ptr), C.c_void_p(Fc.ptr), int(type_lo), C.c_double(val_lo),
+                  int(type_hi), C.c_double(val_hi), *grid)
 
     fn.__name__ = name
     fn.__doc__ = f"{name}!(Fc_exxx, Fc, type_lo, val_lo, type_hi, val_hi) -- {ref}"
     return fn
 
 
-Boundaries_x_Weno5 = _boundaries(1, "Boundaries_x_Weno5", "tools/Weno5_Routines.jl:130-170, _v2.jl:221-263")
-Boundaries_y_Weno5 = _boundaries(2, "Boundaries_y_Weno5", "tools/Weno5_Routines.jl:172-212, _v2.jl:265-305")
-Boundaries_z_Weno5 = _boundaries(3, "Boundaries_z_Weno5", "tools/Weno5_Routines.jl:214-255, _v2.jl:307-347")
+Boundaries_x_Weno5 = _boundaries("x", "Boundaries_x_Weno5", "tools/Weno5_Routines.jl:130-170, _v2.jl:221-263")
+Boundaries_y_Weno5 = _boundaries("y", "Boundaries_y_Weno5", "tools/Weno5_Routines.jl:172-212, _v2.jl:265-305")
+Boundaries_z_Weno5 = _boundaries("z", "Boundaries_z_Weno5", "tools/Weno5_Routines.jl:214-255, _v2.jl:307-347")
 
 _Advect_v1 = _make("Advect", "ht3d_Advect",
                    [("Fc", "c"), ("Vxp", "c"), ("dTdxm", "c"), ("Vxm", "c"), ("dTdxp", "c"), ("dt", "d")],

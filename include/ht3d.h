@@ -338,6 +338,40 @@ int ht3d_Gradients_Weno5(double *v1, double *v2, double *v3, double *v4, double 
  * Neumann / Dirichlet / periodic                                             */
 int ht3d_Boundaries_Weno5(double *Fc_exxx, const double *Fc, int axis, int src_ext, int type_lo, double val_lo,
                           int type_hi, double val_hi, int ncx, int ncy, int ncz);
+/* The same kernels under the reference's names and argument orders:
+ * Gradients_{minus,plus}_{x,y,z}_Weno5!  tools/Weno5_Routines.jl:70-128, and
+ * Boundaries_{x,y,z}_Weno5!  tools/Weno5_Routines.jl:130-255 (Fc a cell array;
+ * the _v2 entries take Fc extended, tools/Weno5_Routines_v2.jl:221-347).     */
+int ht3d_Gradients_minus_x_Weno5(double *v1, double *v2, double *v3, double *v4, double *v5,
+                              const double *Fc_exxx, double _dx, double _dy, double _dz, int ncx, int ncy,
+                              int ncz);
+int ht3d_Gradients_plus_x_Weno5(double *v1, double *v2, double *v3, double *v4, double *v5,
+                              const double *Fc_exxx, double _dx, double _dy, double _dz, int ncx, int ncy,
+                              int ncz);
+int ht3d_Gradients_minus_y_Weno5(double *v1, double *v2, double *v3, double *v4, double *v5,
+                              const double *Fc_exxx, double _dx, double _dy, double _dz, int ncx, int ncy,
+                              int ncz);
+int ht3d_Gradients_plus_y_Weno5(double *v1, double *v2, double *v3, double *v4, double *v5,
+                              const double *Fc_exxx, double _dx, double _dy, double _dz, int ncx, int ncy,
+                              int ncz);
+int ht3d_Gradients_minus_z_Weno5(double *v1, double *v2, double *v3, double *v4, double *v5,
+                              const double *Fc_exxx, double _dx, double _dy, double _dz, int ncx, int ncy,
+                              int ncz);
+int ht3d_Gradients_plus_z_Weno5(double *v1, double *v2, double *v3, double *v4, double *v5,
+                              const double *Fc_exxx, double _dx, double _dy, double _dz, int ncx, int ncy,
+                              int ncz);
+int ht3d_Boundaries_x_Weno5(double *Fc_exxx, const double *Fc, int type_W, double val_W,
+                              int type_E, double val_E, int ncx, int ncy, int ncz);
+int ht3d_Boundaries_x_Weno5_v2(double *Fc_exxx, const double *Fc, int type_W, double val_W,
+                              int type_E, double val_E, int ncx, int ncy, int ncz);
+int ht3d_Boundaries_y_Weno5(double *Fc_exxx, const double *Fc, int type_S, double val_S,
+                              int type_N, double val_N, int ncx, int ncy, int ncz);
+int ht3d_Boundaries_y_Weno5_v2(double *Fc_exxx, const double *Fc, int type_S, double val_S,
+                              int type_N, double val_N, int ncx, int ncy, int ncz);
+int ht3d_Boundaries_z_Weno5(double *Fc_exxx, const double *Fc, int type_B, double val_B,
+                              int type_F, double val_F, int ncx, int ncy, int ncz);
+int ht3d_Boundaries_z_Weno5_v2(double *Fc_exxx, const double *Fc, int type_B, double val_B,
+                              int type_F, double val_F, int ncx, int ncy, int ncz);
 /* tools/Weno5_Routines.jl:257-269 */
 int ht3d_Advect(double *Fc, const double *Vxp, const double *dTdxm, const double *Vxm, const double *dTdxp,
                 double dt, int ncx, int ncy, int ncz);
