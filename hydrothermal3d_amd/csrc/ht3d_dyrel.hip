@@ -189,9 +189,12 @@ int copy_ghosts(double *dst, const double *src, int nx, int ny, int nz) {
 }
 
 // The DYREL solve.  If nu_out is given (heat solve), the Nusselt number of the
-// last pseudo-iteration is returned through it (sum / nu_div).
+// last pseudo-iteration is returned through it (sum / nu_div).  With `given`
+// the device buffers come from the caller and nothing is allocated: no tuning
+// pass (the cached tile of the grid, else the default tile) and F0, Fst
+// cleared by a fill kernel.
 int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep,
-                     double *nu_out, double nu_div) {
+                     double *nu_out, double nu_div, const SolveBuffers *given) {
     HT3D_CHECK_INIT();
     if (!f || !p) return arg_fail("ht3d_dyrel_solve: null argument");
     const int nx = f->ncx, ny = f->ncy, nz = f->ncz;
@@ -203,25 +206,37 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
 
     void *xb = nullptr, *f0 = nullptr, *f0b = nullptr, *fst = nullptr, *part = nullptr, *res = nullptr, *dst = nullptr;
     int rc;
-    if ((rc = workspace(WS_XB, NE * sizeof(double), &xb))) return rc;
-    if ((rc = workspace(WS_F0, N * sizeof(double), &f0))) return rc;
-    if ((rc = workspace(WS_F0B, N * sizeof(double), &f0b))) return rc;
-    if ((rc = workspace(WS_FCIT, N * sizeof(double), &fst))) return rc;
-    if ((rc = workspace(WS_DIAG, N * sizeof(double), &dst))) return rc;
-
     IterArgs<double> a;
     std::memset(&a, 0, sizeof a);
     a.nx = nx; a.ny = ny; a.nz = nz;
-    // most blocks of any tile configuration (1 row, or 2 planes, per block at the least)
-    const unsigned max_blocks = div_up(nx, 64) * div_up(ny, 1) * div_up(nz, 2);
-    if ((rc = workspace(WS_PARTIALS, (size_t)max_blocks * 8 * sizeof(double), &part))) return rc;
-    if ((rc = workspace(WS_RESULT, 64 * sizeof(double), &res))) return rc;
-
-    // set-up (v18:432-439); the diagonal D is kept for the DSTORED variants
     const double rc_tr_dt = f->rhoC * (double)f->transient / f->dt;
     double lmax = 0.0;
-    if ((rc = gershgorin_lmax(f->kx, f->ky, f->kz, nx, ny, nz, f->dx, f->dy, f->dz, rc_tr_dt, &lmax, (double *)dst)))
-        return rc;
+    if (given) {
+        xb = given->xb; f0 = given->f0; f0b = given->f0b; fst = given->fst; dst = given->dst;
+        part = given->partials; res = given->result;
+        // set-up (v18:432-439) on the given partials; the diagonal D is kept for the DSTORED variants
+        const unsigned nb = gershgorin_blocks(N);
+        if ((rc = launch_gershgorin(f->kx, f->ky, f->kz, nx, ny, nz, f->dx, f->dy, f->dz, rc_tr_dt, (double *)part, nb,
+                                    (double *)dst)) ||
+            (rc = finish_partials((double *)part, nb, 1u, (double *)res)))
+            return rc;
+        double r4[4];
+        if ((rc = read_result4((double *)res, r4))) return rc;
+        lmax = r4[0];
+    } else {
+        if ((rc = workspace(WS_XB, NE * sizeof(double), &xb))) return rc;
+        if ((rc = workspace(WS_F0, N * sizeof(double), &f0))) return rc;
+        if ((rc = workspace(WS_F0B, N * sizeof(double), &f0b))) return rc;
+        if ((rc = workspace(WS_FCIT, N * sizeof(double), &fst))) return rc;
+        if ((rc = workspace(WS_DIAG, N * sizeof(double), &dst))) return rc;
+        // most blocks of any tile configuration (1 row, or 2 planes, per block at the least)
+        const unsigned max_blocks = div_up(nx, 64) * div_up(ny, 1) * div_up(nz, 2);
+        if ((rc = workspace(WS_PARTIALS, (size_t)max_blocks * 8 * sizeof(double), &part))) return rc;
+        if ((rc = workspace(WS_RESULT, 64 * sizeof(double), &res))) return rc;
+        // set-up (v18:432-439); the diagonal D is kept for the DSTORED variants
+        if ((rc = gershgorin_lmax(f->kx, f->ky, f->kz, nx, ny, nz, f->dx, f->dy, f->dz, rc_tr_dt, &lmax, (double *)dst)))
+            return rc;
+    }
     double lmin = lmax / p->lmin_div;
     const double dtau = 2.0 / sqrt(lmax) * p->CFL;
     double c = 2.0 * sqrt(lmin) * p->cfact;
@@ -238,9 +253,26 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
     a.Xout = (double *)xb;
     IterCfg cfg;
     PairCfg pcfg;
-    if ((rc = choose_cfg(a, f->transient != 0, true, 0, (double *)f0b, &cfg, &pcfg))) return rc;
-    const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
     TuneEntry *te = tune_find(TuneKey{nx, ny, nz, f->transient != 0 ? 1 : 0, 0});
+    if (given) {   // no tuning pass: the grid's cached choice, else the default tile
+        if (te) {
+            cfg = te->cfg;
+            if (te->pair.tile >= 0 && (g_pairs == 2 || (g_pairs == 1 && te->pair_us < te->us))) pcfg = te->pair;
+        } else {
+            cfg = IterCfg{tile_for_grid(kDefaultTile, nx), 6};
+        }
+        set_tiling(a, cfg.tile);
+        if (given->max_blocks < (size_t)a.nbx * a.nby * a.nbz)
+            return arg_fail("ht3d_dyrel_solve_noalloc: the grid needs more block partials than the scratch holds");
+        if (pcfg.tile >= 0) {
+            IterArgs<double> t = a;
+            set_pair_tiling(t, pcfg.tile);
+            if (given->max_blocks < (size_t)t.nbx * t.nby * t.nbz) pcfg.tile = -1;
+        }
+    } else if ((rc = choose_cfg(a, f->transient != 0, true, 0, (double *)f0b, &cfg, &pcfg))) {
+        return rc;
+    }
+    const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
     int ncheck = p->ncheck;
     if (ncheck < 0)   // the check-interval tuner
         ncheck = tuned_check_interval(te ? te->last_iters : -1.0,
@@ -249,8 +281,12 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
     IterArgs<double> ap = a;   // the same arguments with the pair tiling
     if (pcfg.tile >= 0) set_pair_tiling(ap, pcfg.tile);
 
-    HIP_TRY(hipMemsetAsync(f0, 0, N * sizeof(double), cx.stream));   // ResetA!(Fc, Fc0)
-    HIP_TRY(hipMemsetAsync(fst, 0, N * sizeof(double), cx.stream));
+    if (given) {   // ResetA!(Fc, Fc0) with a fill kernel
+        if ((rc = fill_zero(f0, N * sizeof(double))) || (rc = fill_zero(fst, N * sizeof(double)))) return rc;
+    } else {
+        HIP_TRY(hipMemsetAsync(f0, 0, N * sizeof(double), cx.stream));   // ResetA!(Fc, Fc0)
+        HIP_TRY(hipMemsetAsync(fst, 0, N * sizeof(double), cx.stream));
+    }
 
     double *buf[2] = {f->X, (double *)xb};
     double *f0buf[2] = {(double *)f0, (double *)f0b};

@@ -197,6 +197,21 @@ __device__ inline void block_reduce4(const double hi_in[4], const double lo_in[4
 
 // Reduce `nblocks` x 8 partials into result[4] (single block launch).
 int finish_partials(const double *partials, unsigned nblocks, unsigned ops, double *result_dev);
+// Zero `bytes` (a multiple of 8) at p with a fill kernel on the library stream.
+int fill_zero(void *p, size_t bytes);
+
+// Device buffers of a DYREL solve handed in by the caller instead of taken
+// from the workspaces (the allocation-free entry, ht3d_noalloc.hip).
+struct SolveBuffers {
+    double *xb, *f0, *f0b, *fst, *dst;   // E, then four cell arrays
+    double *partials;                    // max_blocks x 8 doubles
+    double *result;                      // 64 doubles
+    size_t max_blocks;
+};
+int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep, double *nu_out,
+                     double nu_div, const SolveBuffers *given = nullptr);
+// per-device address of the allocation-free entry's scratch array (ht3d_init)
+int noalloc_setup(int device);
 // Read back 4 doubles from the device result buffer (synchronises the stream).
 int read_result4(const double *result_dev, double out[4]);
 // read_result4 in two halves: queue the copy, and later wait for it (work

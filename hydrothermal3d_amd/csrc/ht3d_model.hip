@@ -20,8 +20,6 @@
 
 namespace ht3d {
 
-int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep,
-                     double *nu_out, double nu_div);
 int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep,
                   double *nu_out, int *outer, int *inner);
 

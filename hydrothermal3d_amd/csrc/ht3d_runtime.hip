@@ -148,6 +148,21 @@ __global__ __launch_bounds__(1024) void finish_kernel(const double *__restrict__
     }
 }
 
+__global__ __launch_bounds__(256) void fill_zero_kernel(double *__restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+int fill_zero(void *p, size_t bytes) {
+    const size_t n = bytes / sizeof(double);
+    if (!n) return HT3D_OK;
+    unsigned nb = div_up(n, 256);
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(nb), dim3(256), 0, g_ctx.stream, (double *)p, n);
+    HIP_TRY(hipGetLastError());
+    return HT3D_OK;
+}
+
 int finish_partials(const double *partials, unsigned nblocks, unsigned ops, double *result_dev) {
     hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(1024), 0, g_ctx.stream, partials, nblocks, ops, result_dev);
     hipError_t e = hipGetLastError();
@@ -218,6 +233,7 @@ int ht3d_init(int device) {
     HIP_TRY(hipHostMalloc((void **)&c.pinned, 64 * sizeof(double), hipHostMallocDefault));
     HIP_TRY(hipEventCreate(&c.ev0));
     HIP_TRY(hipEventCreate(&c.ev1));
+    if (int rc = noalloc_setup(device)) return rc;
     c.device = device;
     c.initialised = true;
     c.sync = true;

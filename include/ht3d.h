@@ -422,6 +422,17 @@ typedef struct {
 
 int ht3d_dyrel_solve(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep);
 
+/* ht3d_dyrel_solve without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (serving loops,
+ * harnesses that forbid hipMalloc).  Every device buffer of the solve comes
+ * from a static device array of the library (about 105 MB per GPU, held until
+ * the process ends), so the grid is limited to 2^21 cells and 2^22 entries of
+ * its extended array (HT3D_ERR_ARG beyond); no tuning pass runs (the tile the
+ * tuner chose for this grid earlier in the process, else the default tile:
+ * the results are the same bit for bit).  Calls that overlap on one GPU race
+ * on that array; each call clears what must start at zero. */
+int ht3d_dyrel_solve_noalloc(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep);
+
 /* ---- Fused DYREL solves of the dimensional (v18) model: the heat loop of
  * Dimensional/HydroThermal3D_v18.jl:432-478 (SetTemperatureBCs!, ComputeFlux!,
  * ResidualTemperatureLinearised!, DYRELUpdate!) and the Darcy loop of :489-537
