@@ -4,8 +4,9 @@
 // permeability and steps them in lockstep: thermal solves of all problems,
 // then Darcy solves, then velocity, time step and advection per problem.
 //
-// Solves.  Every problem's domain is cut into the tiles of the fused
-// pseudo-iteration (64 x TY cells, KZ planes); the (problem, tile) pairs of the
+// Solves.  Every problem's domain is cut into the items of the fused
+// pseudo-iteration (a patch of 64 x TY lanes over the row-major (x, y) plane,
+// CPL x-cells per lane, KZ planes deep); the (problem, item) pairs of the
 // problems still iterating form a packed work list.  One persistent launch per
 // pseudo-iteration walks that list (block b takes items b, b + G, ...), each
 // item reading its problem's operator, pointers and DYREL scalars from a
@@ -63,15 +64,22 @@ static constexpr TileCfg kBTiles[] = {{2, 8},    {4, 4},    {4, 8},    {8, 4},  
 static constexpr int kNumBTiles = sizeof(kBTiles) / sizeof(kBTiles[0]);
 static constexpr int kBDefaultTile = 1;
 
-// tiles of one problem under batch tile `tile`
+// A batch item is a patch of 64*TY consecutive lane-groups (CPL x-cells each)
+// of a problem's row-major (x, y) plane, marched over KZ planes (patch_cell):
+// the small grids of a batch (32 to 128 cells wide) keep every lane busy,
+// where a tile of whole 64*CPL-cell rows would leave up to 3/4 of them idle.
+static inline unsigned bpatch_count(int nx, int ny, int tile) {
+    return div_up((size_t)(nx / kBTiles[tile].cpl) * ny, 64 * kBTiles[tile].ty);
+}
+// items of one problem under batch tile `tile`
 static inline unsigned btile_count(int nx, int ny, int nz, int tile) {
-    return div_up(nx, 64 * kBTiles[tile].cpl) * div_up(ny, kBTiles[tile].ty) * div_up(nz, kBTiles[tile].kz);
+    return bpatch_count(nx, ny, tile) * div_up(nz, kBTiles[tile].kz);
 }
 
-__device__ __forceinline__ void tile_of(unsigned l, int nbx, int nby, int &bx, int &by, int &bz) {
-    bx = (int)(l % (unsigned)nbx);
-    by = (int)((l / (unsigned)nbx) % (unsigned)nby);
-    bz = (int)(l / ((unsigned)nbx * (unsigned)nby));
+// item l of a problem -> its patch and its chunk of planes
+__device__ __forceinline__ void item_of(unsigned l, int npatch, int &p, int &bz) {
+    p = (int)(l % (unsigned)npatch);
+    bz = (int)(l / (unsigned)npatch);
 }
 
 // ---------------------------------------------------------------- kernels
@@ -88,17 +96,18 @@ __global__ __launch_bounds__(64 * TY) void batch_iter_kernel(const IterArgs<T> *
         const uint2 it = items[t];
         if (!st[it.x].active) continue;
         const IterArgs<T> &a = tab[it.x];
-        int bx, by, bz;
-        tile_of(it.y, a.nbx, a.nby, bx, by, bz);
+        int p, bz, i, j;
+        item_of(it.y, a.nbx, p, bz);
+        patch_cell<TY, CPL>(p, a.nx / CPL, i, j);
         T *x0 = const_cast<T *>(a.Xin), *x1 = a.Xout;
         // FP64 sweeps: non-temporal store of the unknown, cached face coefficients
         // (the single solve's variant 6; 9 % faster per batched FP64 step); the
         // FP32 sweeps measured no difference and keep the plain policy
         constexpr bool V6 = sizeof(T) == 8;
         if constexpr (CPL == 1)
-            iter_tile<T, TR, false, V6, V6, TY, KZ>(a, phase ? x1 : x0, phase ? x0 : x1, flags, bx, by, bz, partials, t);
+            iter_tile<T, TR, false, V6, V6, TY, KZ>(a, phase ? x1 : x0, phase ? x0 : x1, flags, i, j, bz, partials, t);
         else
-            iter_tile_v<T, TR, false, V6, V6, TY, KZ, CPL>(a, phase ? x1 : x0, phase ? x0 : x1, flags, bx, by, bz,
+            iter_tile_v<T, TR, false, V6, V6, TY, KZ, CPL>(a, phase ? x1 : x0, phase ? x0 : x1, flags, i, j, bz,
                                                          partials, t);
         if (flags & FL_CHECK) __syncthreads();  // the block reduction's shared array is reused by the next item
     }
@@ -118,10 +127,11 @@ __global__ __launch_bounds__(64 * TY) void batch_dc_residual_kernel(const DcArgs
         const unsigned t = xcd_tile(w, n);
         const uint2 it = items[t];
         if (!st[it.x].active) continue;
-        int bx, by, bz;
-        tile_of(it.y, tab32[it.x].nbx, tab32[it.x].nby, bx, by, bz);
+        int p, bz, i, j;
+        item_of(it.y, tab32[it.x].nbx, p, bz);
+        patch_cell<TY, CPL>(p, dtab[it.x].nx / CPL, i, j);
         double s_hi = 0.0, s_lo = 0.0, mx = 0.0;
-        dc_residual_tile<TR, TY, KZ, CPL>(dtab[it.x], bx, by, bz, s_hi, s_lo, mx);
+        dc_residual_tile<TR, TY, KZ, CPL>(dtab[it.x], i, j, bz, s_hi, s_lo, mx);
         const double h4[4] = {s_hi, mx, 0.0, 0.0}, l4[4] = {s_lo, 0.0, 0.0, 0.0};
         block_reduce4(h4, l4, 0x2u, partials, t);
         __syncthreads();
@@ -143,21 +153,20 @@ __global__ __launch_bounds__(64 * TY) void batch_dc_update_kernel(const DcArgs *
         const IterArgs<float> &a = tab32[it.x];
         double *X = const_cast<double *>(dtab[it.x].X);
         float *e = phase ? a.Xout : const_cast<float *>(a.Xin);
-        int bx, by, bz;
-        tile_of(it.y, a.nbx, a.nby, bx, by, bz);
-        const int j = by * TY + (int)(threadIdx.x >> 6);
+        int p, bz, i, j;
+        item_of(it.y, a.nbx, p, bz);
+        patch_cell<TY, CPL>(p, a.nx / CPL, i, j);
         const int k0 = bz * KZ, k1 = min(k0 + KZ, a.nz);
+        if (j < a.ny)
+            for (int k = k0; k < k1; ++k) {
+                const size_t idx =
+                    (size_t)(i + 1) + (size_t)(j + 1) * (a.nx + 2) + (size_t)(k + 1) * (a.nx + 2) * (a.ny + 2);
 #pragma unroll
-        for (int q = 0; q < CPL; ++q) {
-            const int i = (bx * CPL + q) * 64 + (int)(threadIdx.x & 63u);
-            if (i < a.nx && j < a.ny)
-                for (int k = k0; k < k1; ++k) {
-                    const size_t idx =
-                        (size_t)(i + 1) + (size_t)(j + 1) * (a.nx + 2) + (size_t)(k + 1) * (a.nx + 2) * (a.ny + 2);
-                    X[idx] += (double)e[idx];
-                    e[idx] = 0.0f;
+                for (int q = 0; q < CPL; ++q) {
+                    X[idx + q] += (double)e[idx + q];
+                    e[idx + q] = 0.0f;
                 }
-        }
+            }
     }
 }
 
@@ -547,7 +556,7 @@ static void fill_tables(ht3d_batch *b, int kind, const std::vector<ht3d_dyrel_fi
         a.R = d.R; a.kx = d.kx; a.ky = d.ky; a.kz = d.kz; a.Dst = nullptr;
         a.F0 = b->f0 + b->oc[q]; a.Fst = b->fst + b->oc[q]; a.partials = nullptr;
         a.nx = nx; a.ny = ny; a.nz = nz;
-        a.nbx = (int)div_up(nx, 64 * kBTiles[tile64].cpl); a.nby = (int)div_up(ny, kBTiles[tile64].ty);
+        a.nbx = (int)bpatch_count(nx, ny, tile64); a.nby = 1;   // patches of the (x, y) plane
         a.nbz = (int)div_up(nz, kBTiles[tile64].kz);
         a._dx = 1.0 / d.dx; a._dy = 1.0 / d.dy; a._dz = 1.0 / d.dz; a._dt = 1.0 / d.dt;
         a.dx = d.dx; a.dy = d.dy; a.dz = d.dz; a.rc_tr_dt = rc_tr_dt;
@@ -562,7 +571,7 @@ static void fill_tables(ht3d_batch *b, int kind, const std::vector<ht3d_dyrel_fi
         g.kx = b->k32 + b->of[q]; g.ky = g.kx + nfx; g.kz = g.ky + nfy; g.Dst = nullptr;
         g.F0 = b->f032 + b->oc[q]; g.Fst = b->fst32 + b->oc[q]; g.partials = nullptr;
         g.nx = nx; g.ny = ny; g.nz = nz;
-        g.nbx = (int)div_up(nx, 64 * kBTiles[tile32].cpl); g.nby = (int)div_up(ny, kBTiles[tile32].ty);
+        g.nbx = (int)bpatch_count(nx, ny, tile32); g.nby = 1;
         g.nbz = (int)div_up(nz, kBTiles[tile32].kz);
         g._dx = (float)a._dx; g._dy = (float)a._dy; g._dz = (float)a._dz; g._dt = (float)a._dt;
         g.dx = (float)d.dx; g.dy = (float)d.dy; g.dz = (float)d.dz; g.rc_tr_dt = (float)rc_tr_dt;

@@ -32,8 +32,10 @@ template <bool TR, int TY, int KZ, int CPL>
 __global__ __launch_bounds__(64 * TY) void dc_residual_tiled(DcArgs a, int nbx, int nby, int nbz) {
     const unsigned t = xcd_tile(blockIdx.x, (unsigned)nbx * nby * nbz);
     const int bx = t % nbx, by = (t / nbx) % nby, bz = t / (nbx * nby);
+    int i, j;
+    tile_cell<TY, CPL>(bx, by, i, j);
     double s_hi = 0.0, s_lo = 0.0, mx = 0.0;
-    dc_residual_tile<TR, TY, KZ, CPL>(a, bx, by, bz, s_hi, s_lo, mx);
+    dc_residual_tile<TR, TY, KZ, CPL>(a, i, j, bz, s_hi, s_lo, mx);
     const double h4[4] = {s_hi, mx, 0.0, 0.0}, l4[4] = {s_lo, 0.0, 0.0, 0.0};
     block_reduce4(h4, l4, 0x2u, a.partials, blockIdx.x);
 }

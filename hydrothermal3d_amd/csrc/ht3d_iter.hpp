@@ -46,6 +46,24 @@ struct IterArgs {
     unsigned flags;
 };
 
+// The first cell (i, j) of this lane's CPL x-cells: in tile (bx, by) of
+// 64*CPL x TY cells (one row of the tile per wave) ...
+template <int TY, int CPL>
+__device__ __forceinline__ void tile_cell(int bx, int by, int &i, int &j) {
+    i = bx * (64 * CPL) + CPL * (int)(threadIdx.x & 63u);
+    j = by * TY + (int)(threadIdx.x >> 6);
+}
+// ... or in patch p of the (x, y) plane seen as a row-major sequence of
+// groups of CPL x-cells, gx = nx / CPL groups per row: a block covers the
+// 64*TY consecutive groups from p*64*TY, so a wave spans several rows of a
+// narrow grid and keeps all its lanes busy (j >= ny past the last row)
+template <int TY, int CPL>
+__device__ __forceinline__ void patch_cell(int p, int gx, int &i, int &j) {
+    const unsigned g = (unsigned)p * (64u * TY) + threadIdx.x;
+    j = (int)(g / (unsigned)gx);
+    i = (int)(g - (unsigned)j * (unsigned)gx) * CPL;
+}
+
 // block id -> logical tile id, making the blocks of one XCD (id mod 8) contiguous;
 // with `rev` each XCD walks its slab from the far end
 __device__ __forceinline__ unsigned xcd_tile(unsigned bid, unsigned nblocks, bool rev = false) {
@@ -106,16 +124,16 @@ __device__ __forceinline__ T ld_k(const T *p) {
 // and the boundary condition is applied by selecting the ghost value computed
 // from the centre cell; offsets are 32-bit (E arrays stay below 2^32 words).
 //
-// iter_tile is one (x,y) tile of 64 x TY cells marched over KZ planes from
-// plane bz*KZ; `slot` is where a check iteration writes its block partials.
+// iter_tile marches the lane's cell (i, j) over the KZ planes from plane
+// bz*KZ (tile_cell: a tile of 64 x TY cells; patch_cell: a patch of 64*TY
+// consecutive cells of the row-major (x, y) plane); `slot` is where a check
+// iteration writes its block partials.
 // Xin/Xout and flags come separately so the batched kernel can take them from
 // its launch while the rest of `a` lives in a per-problem table.
 template <typename T, bool TRANSIENT, bool DSTORED, bool NTX, bool KT, int TY, int KZ>
 __device__ __forceinline__ void iter_tile(const IterArgs<T> &a, const T *__restrict__ Xin, T *__restrict__ Xout,
-                                          unsigned flags, int bx, int by, int bz, double *__restrict__ partials,
+                                          unsigned flags, int i, int j, int bz, double *__restrict__ partials,
                                           unsigned slot) {
-    const int i = bx * 64 + (int)(threadIdx.x & 63u);
-    const int j = by * TY + (int)(threadIdx.x >> 6);
     const int k0 = bz * KZ;
     const int k1 = min(k0 + KZ, a.nz);
     const uint32_t nx = a.nx, ny = a.ny;
@@ -229,8 +247,8 @@ __device__ __forceinline__ void stv_nt(typename CellVec<T, CPL>::type v, T *p) {
     __builtin_nontemporal_store(v, reinterpret_cast<typename CellVec<T, CPL>::type *>(p));
 }
 
-// iter_tile with CPL (2 or 4) x-neighbouring cells per lane: a block covers
-// 64*CPL x TY cells.  The centre, lateral and z rows of X, the face
+// iter_tile with CPL (2 or 4) x-neighbouring cells per lane, from (i, j) (a
+// tile_cell block covers 64*CPL x TY cells).  The centre, lateral and z rows of X, the face
 // coefficients, R, F0 (and D, F) arrive as CPL-element vector loads, which
 // divides the load instructions per cell by about CPL; the west neighbour of
 // the first cell and the east neighbour of the last are single loads, and a
@@ -240,11 +258,9 @@ __device__ __forceinline__ void stv_nt(typename CellVec<T, CPL>::type v, T *p) {
 // Needs nx % CPL == 0 (a lane's cells never cross the end of a row).
 template <typename T, bool TRANSIENT, bool DSTORED, bool NTX, bool KT, int TY, int KZ, int CPL>
 __device__ __forceinline__ void iter_tile_v(const IterArgs<T> &a, const T *__restrict__ Xin, T *__restrict__ Xout,
-                                            unsigned flags, int bx, int by, int bz, double *__restrict__ partials,
+                                            unsigned flags, int i, int j, int bz, double *__restrict__ partials,
                                             unsigned slot) {
     typedef typename CellVec<T, CPL>::type V;
-    const int i = bx * (64 * CPL) + CPL * (int)(threadIdx.x & 63u);
-    const int j = by * TY + (int)(threadIdx.x >> 6);
     const int k0 = bz * KZ;
     const int k1 = min(k0 + KZ, a.nz);
     const uint32_t nx = a.nx, ny = a.ny;
@@ -350,12 +366,13 @@ __global__ __launch_bounds__(64 * TY) void dyrel_iter_kernel(IterArgs<T> a) {
     const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
     const unsigned t = xcd_tile(blockIdx.x, nblocks, a.flags & FL_REVERSE);
     const int bx = t % a.nbx, by = (t / a.nbx) % a.nby, bz = t / (a.nbx * a.nby);
+    int i, j;
+    tile_cell<TY, CPL>(bx, by, i, j);
     if constexpr (CPL > 1)
-        iter_tile_v<T, TRANSIENT, DSTORED, NTX, KT, TY, KZ, CPL>(a, a.Xin, a.Xout, a.flags, bx, by, bz, a.partials,
+        iter_tile_v<T, TRANSIENT, DSTORED, NTX, KT, TY, KZ, CPL>(a, a.Xin, a.Xout, a.flags, i, j, bz, a.partials,
                                                                  blockIdx.x);
     else
-        iter_tile<T, TRANSIENT, DSTORED, NTX, KT, TY, KZ>(a, a.Xin, a.Xout, a.flags, bx, by, bz, a.partials,
-                                                          blockIdx.x);
+        iter_tile<T, TRANSIENT, DSTORED, NTX, KT, TY, KZ>(a, a.Xin, a.Xout, a.flags, i, j, bz, a.partials, blockIdx.x);
 }
 
 // Two pseudo-iterations per launch, for runs of plain iterations (no check, no
@@ -1055,8 +1072,9 @@ struct DcArgs {
     double bcv[6];
 };
 
-// The FP64 defect-correction residual of one (x, y) tile of 64*CPL x TY cells
-// marched over the planes [bz*KZ, +KZ): the load pattern of iter_tile_v
+// The FP64 defect-correction residual of the lane's CPL cells from (i, j)
+// (tile_cell or patch_cell) marched over the planes [bz*KZ, +KZ): the load
+// pattern of iter_tile_v
 // (X(k-1), X(k), X(k+1) and kz(k), kz(k+1) in registers, CPL-element vector
 // loads, a face between two cells of a lane loaded once); per cell r = R
 // [+ x/dt] + div q(x) in the order of ComputeFlux! and
@@ -1066,12 +1084,10 @@ struct DcArgs {
 // cell, the fused iteration's stencil without the update; a full chunk of KZ
 // planes is unrolled so later planes' loads issue ahead.  Needs nx % CPL == 0.
 template <bool TR, int TY, int KZ, int CPL>
-__device__ __forceinline__ void dc_residual_tile(const DcArgs &a, int bx, int by, int bz, double &s_hi, double &s_lo,
+__device__ __forceinline__ void dc_residual_tile(const DcArgs &a, int i, int j, int bz, double &s_hi, double &s_lo,
                                                  double &mx) {
     typedef typename CellVec<double, CPL>::type V;
     typedef typename CellVec<float, CPL>::type FV;
-    const int i = bx * (64 * CPL) + CPL * (int)(threadIdx.x & 63u);
-    const int j = by * TY + (int)(threadIdx.x >> 6);
     const int k0 = bz * KZ, k1 = min(k0 + KZ, a.nz);
     const uint32_t nx = a.nx, ny = a.ny;
     if ((uint32_t)i >= nx || (uint32_t)j >= ny) return;
