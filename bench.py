@@ -198,7 +198,8 @@ def run_batch(n: int, steps: int, warmup: int, prm, graphs: bool = True,
                               "graph_launches": st["graph_launches"], "graphs": st["graphs"],
                               "repacks": st["repacks"],
                               "tiles": [f"{r}x{z}" + (f" x{c}" if c > 1 else "")
-                                        for r, z, c in zip(st["tile_rows"], st["tile_z"], st["tile_cpl"])]})
+                                        for r, z, c in zip(st["tile_rows"], st["tile_z"], st["tile_cpl"])],
+                              "grid": ["per-item" if g else "persistent" for g in st["grid_mode"]]})
             b.close()
         states[mode] = _batch_state(ms, reps)
         for m in ms:

@@ -70,7 +70,8 @@ class BatchStats(C.Structure):
     _fields_ = [("ms_solve", C.c_double), ("bytes_solve", C.c_double), ("ms_step", C.c_double),
                 ("steps", C.c_longlong), ("blocks", C.c_longlong), ("graph_launches", C.c_longlong),
                 ("graphs", C.c_longlong), ("repacks", C.c_longlong), ("iterations", C.c_longlong),
-                ("tile_rows", C.c_int * 4), ("tile_z", C.c_int * 4), ("tile_cpl", C.c_int * 4)]
+                ("tile_rows", C.c_int * 4), ("tile_z", C.c_int * 4), ("tile_cpl", C.c_int * 4),
+                ("grid_mode", C.c_int * 4)]
 
     def as_dict(self) -> dict:
         d = {}

@@ -55,12 +55,18 @@ struct BState {
     int active, outer, inner, pad;
 };
 
-// tiles of the batched launches (short tiles suit the small grids of a batch);
-// the last six hold 2 or 4 x-cells per lane (vector loads, bit-identical
-// cells), offered when every problem's ncx is a multiple of that count, and 4
-// only to the FP32 sweeps
-static constexpr TileCfg kBTiles[] = {{2, 8},    {4, 4},    {4, 8},    {8, 4},    {4, 2},    {1, 8},
-                                      {4, 4, 2}, {4, 8, 2}, {2, 8, 2}, {4, 4, 4}, {4, 8, 4}, {2, 8, 4}};
+// tiles of the batched launches (rows of lanes per block, planes per item,
+// x-cells per lane); the 2- and 4-cell tiles (vector loads, bit-identical
+// cells) are offered when every problem's ncx is a multiple of that count, and
+// 4 only to the FP32 sweeps.  The list drives every batch kernel's dispatch.
+#define HT3D_BATCH_TILES(X) \
+    X(0, 2, 8, 1) X(1, 4, 4, 1) X(2, 4, 8, 1) X(3, 8, 4, 1) X(4, 4, 2, 1) X(5, 1, 8, 1) X(6, 4, 4, 2) X(7, 4, 8, 2) \
+    X(8, 2, 8, 2) X(9, 4, 4, 4) X(10, 4, 8, 4) X(11, 2, 8, 4) X(12, 4, 16, 2) X(13, 8, 8, 2) X(14, 2, 16, 2) \
+    X(15, 4, 16, 4) X(16, 4, 2, 2) X(17, 8, 2, 2) X(18, 2, 4, 2) X(19, 8, 4, 2) X(20, 4, 2, 4) X(21, 8, 2, 4) \
+    X(22, 2, 4, 4)
+#define HT3D_TILE_CFG(N, TY, KZ, CPL) {TY, KZ, CPL},
+static constexpr TileCfg kBTiles[] = {HT3D_BATCH_TILES(HT3D_TILE_CFG)};
+#undef HT3D_TILE_CFG
 static constexpr int kNumBTiles = sizeof(kBTiles) / sizeof(kBTiles[0]);
 static constexpr int kBDefaultTile = 1;
 
@@ -357,26 +363,14 @@ template <typename T, bool TR>
 static void launch_iter_b(int tile, const BLaunch &L, int phase, unsigned flags) {
     const IterArgs<T> *tab = (const IterArgs<T> *)L.tab;
     hipStream_t s = ctx().stream;
-#define BI_(TY, KZ, CPL) hipLaunchKernelGGL((batch_iter_kernel<T, TR, TY, KZ, CPL>), dim3(L.grid), dim3(64 * TY), 0, s, \
-                                            tab, L.items, L.n_items, L.st, L.partials, phase, flags)
-    switch (tile) {
-    case 0: BI_(2, 8, 1); break;
-    case 1: BI_(4, 4, 1); break;
-    case 2: BI_(4, 8, 1); break;
-    case 3: BI_(8, 4, 1); break;
-    case 4: BI_(4, 2, 1); break;
-    case 5: BI_(1, 8, 1); break;
-    case 6: BI_(4, 4, 2); break;
-    case 7: BI_(4, 8, 2); break;
-    case 8: BI_(2, 8, 2); break;
-    default:   // 4 cells per lane: FP32 only (btile_ok)
-        if constexpr (sizeof(T) == 4) {
-            if (tile == 9) BI_(4, 4, 4);
-            else if (tile == 10) BI_(4, 8, 4);
-            else BI_(2, 8, 4);
-        }
+    // 4 cells per lane: FP32 only (btile_ok)
+#define BI_(N, TY, KZ, CPL)                                                                                          \
+    case N:                                                                                                          \
+        if constexpr (CPL < 4 || sizeof(T) == 4)                                                                     \
+            hipLaunchKernelGGL((batch_iter_kernel<T, TR, TY, KZ, CPL>), dim3(L.grid), dim3(64 * TY), 0, s, tab,       \
+                               L.items, L.n_items, L.st, L.partials, phase, flags);                                 \
         break;
-    }
+    switch (tile) { HT3D_BATCH_TILES(BI_) default: break; }
 #undef BI_
 }
 
@@ -384,44 +378,24 @@ template <bool TR>
 static void launch_res_b(int tile, const BLaunch &L) {
     const IterArgs<float> *tab = (const IterArgs<float> *)L.tab;
     hipStream_t s = ctx().stream;
-#define BR_(TY, KZ, CPL) hipLaunchKernelGGL((batch_dc_residual_kernel<TR, TY, KZ, CPL>), dim3(L.grid), dim3(64 * TY), \
-                                            0, s, L.dtab, tab, L.items, L.n_items, L.st, L.partials)
-    switch (tile) {
-    case 0: BR_(2, 8, 1); break;
-    case 1: BR_(4, 4, 1); break;
-    case 2: BR_(4, 8, 1); break;
-    case 3: BR_(8, 4, 1); break;
-    case 4: BR_(4, 2, 1); break;
-    case 5: BR_(1, 8, 1); break;
-    case 6: BR_(4, 4, 2); break;
-    case 7: BR_(4, 8, 2); break;
-    case 8: BR_(2, 8, 2); break;
-    case 9: BR_(4, 4, 4); break;
-    case 10: BR_(4, 8, 4); break;
-    default: BR_(2, 8, 4); break;
-    }
+#define BR_(N, TY, KZ, CPL)                                                                                          \
+    case N:                                                                                                          \
+        hipLaunchKernelGGL((batch_dc_residual_kernel<TR, TY, KZ, CPL>), dim3(L.grid), dim3(64 * TY), 0, s, L.dtab,   \
+                           tab, L.items, L.n_items, L.st, L.partials);                                              \
+        break;
+    switch (tile) { HT3D_BATCH_TILES(BR_) default: break; }
 #undef BR_
 }
 
 static void launch_upd_b(int tile, const BLaunch &L, int phase) {
     const IterArgs<float> *tab = (const IterArgs<float> *)L.tab;
     hipStream_t s = ctx().stream;
-#define BU_(TY, KZ, CPL) hipLaunchKernelGGL((batch_dc_update_kernel<TY, KZ, CPL>), dim3(L.grid), dim3(64 * TY), 0, s, \
-                                            L.dtab, tab, L.items, L.n_items, L.st, phase)
-    switch (tile) {
-    case 0: BU_(2, 8, 1); break;
-    case 1: BU_(4, 4, 1); break;
-    case 2: BU_(4, 8, 1); break;
-    case 3: BU_(8, 4, 1); break;
-    case 4: BU_(4, 2, 1); break;
-    case 5: BU_(1, 8, 1); break;
-    case 6: BU_(4, 4, 2); break;
-    case 7: BU_(4, 8, 2); break;
-    case 8: BU_(2, 8, 2); break;
-    case 9: BU_(4, 4, 4); break;
-    case 10: BU_(4, 8, 4); break;
-    default: BU_(2, 8, 4); break;
-    }
+#define BU_(N, TY, KZ, CPL)                                                                                          \
+    case N:                                                                                                          \
+        hipLaunchKernelGGL((batch_dc_update_kernel<TY, KZ, CPL>), dim3(L.grid), dim3(64 * TY), 0, s, L.dtab, tab,   \
+                           L.items, L.n_items, L.st, phase);                                                         \
+        break;
+    switch (tile) { HT3D_BATCH_TILES(BU_) default: break; }
 #undef BU_
 }
 
@@ -459,6 +433,11 @@ struct ht3d_batch {
     std::vector<IterArgs<float>> h_it32[2];
     std::vector<DcArgs> h_dc[2];
     int tile64[2] = {-1, -1}, tile32[2] = {-1, -1};
+    // launch grid per [kind][fp32]: 0 persistent (a few blocks per CU, each
+    // walking items b, b + G, ...), 1 one block per item; gmode_cur is the one
+    // the launches being enqueued use
+    int gmode[2][2] = {{0, 0}, {0, 0}};
+    int gmode_cur = 0;
     // check-interval tuner inputs per [kind][fp32]: sweep and check costs (us), last iterations
     double t_it[2][2] = {{-1, -1}, {-1, -1}}, t_chk[2][2] = {{-1, -1}, {-1, -1}};
     double last_iters[2][2] = {{-1, -1}, {-1, -1}};
@@ -616,7 +595,7 @@ static BLaunch make_launch(ht3d_batch *b, int kind, bool fp32, int tile) {
         const ht3d_hrl *m = b->m[q];
         n += btile_count(m->nx, m->ny, m->nz, tile);
     }
-    L.grid = persistent_grid(b, n, tile);
+    L.grid = b->gmode_cur ? (unsigned)(n ? n : 1) : persistent_grid(b, n, tile);
     L.threads = 64 * kBTiles[tile].ty;
     return L;
 }
@@ -676,7 +655,7 @@ static int run_block(ht3d_batch *b, const std::vector<BOp> &ops, int kind, bool 
     if (!b->graphs) return enqueue_ops(b, ops, kind, transient, tile, p);
     std::string key;
     key.reserve(ops.size() * 16 + 16);
-    const int hdr[4] = {kind, transient ? 1 : 0, tile, b->precision};
+    const int hdr[5] = {kind, transient ? 1 : 0, tile, b->precision, b->gmode_cur};
     key.append((const char *)hdr, sizeof hdr);
     for (const BOp &o : ops) key.append((const char *)&o, sizeof o);
     auto it = b->cache.find(key);
@@ -713,10 +692,12 @@ static bool btile_ok(const ht3d_batch *b, int tile, bool fp32) {
 }
 
 static int g_force_btile = -1;   // ht3d_batch_tune_force
+static int g_force_bgrid = -1;   // ht3d_batch_tune_force_grid
 
 // time `reps` plain batched sweeps of every tile and keep the fastest
 static int autotune_batch(ht3d_batch *b, int kind, bool transient, bool fp32, const std::vector<ht3d_dyrel_field> &f,
-                          int *tile_out) {
+                          int *tile_out, int *gmode_out) {
+    *gmode_out = g_force_bgrid > 0 ? 1 : 0;
     if (g_force_btile >= 0 || !g_autotune) {   // a pinned tile the batch cannot run falls back to the default
         *tile_out = (g_force_btile >= 0 && btile_ok(b, g_force_btile, fp32)) ? g_force_btile : kBDefaultTile;
         return HT3D_OK;
@@ -735,14 +716,16 @@ static int autotune_batch(ht3d_batch *b, int kind, bool transient, bool fp32, co
     HIP_TRY(hipGetLastError());
     std::vector<char> all(b->nprob, 1);
     double best = 1e300;
-    int best_tile = kBDefaultTile;
+    int best_tile = kBDefaultTile, best_gm = 0;
     ht3d_dyrel_params p{};
-    // every tile is timed in three interleaved rounds, keeping its best: one
-    // pass ranked tiles a few percent apart by noise
-    std::vector<double> tbest(kNumBTiles, 1e300);
+    // every (tile, grid mode) is timed in three interleaved rounds, keeping its
+    // best: one pass ranked tiles a few percent apart by noise
+    std::vector<double> tbest(2 * kNumBTiles, 1e300);
     for (int round = 0; round < 3; ++round)
-        for (int t = 0; t < kNumBTiles; ++t) {
-            if (!btile_ok(b, t, fp32)) continue;
+        for (int tg = 0; tg < 2 * kNumBTiles; ++tg) {
+            const int t = tg >> 1;
+            if (!btile_ok(b, t, fp32) || (g_force_bgrid >= 0 && (tg & 1) != g_force_bgrid)) continue;
+            b->gmode_cur = tg & 1;
             fill_tables(b, kind, f, t, t);
             int rc;
             if ((rc = upload_tables(b, kind))) return rc;
@@ -758,14 +741,17 @@ static int autotune_batch(ht3d_batch *b, int kind, bool transient, bool fp32, co
             HIP_TRY(hipEventSynchronize(b->ev1));
             float ms = 0.f;
             HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-            if (ms < tbest[t]) tbest[t] = ms;
+            if (ms < tbest[tg]) tbest[tg] = ms;
         }
-    for (int t = 0; t < kNumBTiles; ++t)
-        if (tbest[t] < best) {
-            best = tbest[t];
-            best_tile = t;
+    for (int tg = 0; tg < 2 * kNumBTiles; ++tg)
+        if (tbest[tg] < best) {
+            best = tbest[tg];
+            best_tile = tg >> 1;
+            best_gm = tg & 1;
         }
     *tile_out = best_tile;
+    *gmode_out = best_gm;
+    b->gmode_cur = best_gm;
     // costs for the check-interval tuner: one sweep, and what a check (FP64) or
     // an anchor with its update (defect correction) adds, on the chosen tile
     {
@@ -866,7 +852,8 @@ static int batch_solve(ht3d_batch *b, int kind, const ht3d_dyrel_params &p_in, c
                 return rc;
         }
     int &tile = fp32 ? b->tile32[kind] : b->tile64[kind];
-    if (tile < 0 && (rc = autotune_batch(b, kind, transient, fp32, f, &tile))) return rc;
+    if (tile < 0 && (rc = autotune_batch(b, kind, transient, fp32, f, &tile, &b->gmode[kind][fp32 ? 1 : 0]))) return rc;
+    b->gmode_cur = b->gmode[kind][fp32 ? 1 : 0];
     ht3d_dyrel_params p = p_in;
     const int pk = fp32 ? 1 : 0;
     if (p.ncheck < 0)   // the check-interval tuner (the anchor interval in the defect-correction form)
@@ -1218,6 +1205,7 @@ int ht3d_batch_stats_get(const ht3d_batch *b, ht3d_batch_stats *s) {
     s->tile_z[1] = b->tile64[1] >= 0 ? kBTiles[b->tile64[1]].kz : 0;
     s->tile_z[2] = b->tile32[0] >= 0 ? kBTiles[b->tile32[0]].kz : 0;
     s->tile_z[3] = b->tile32[1] >= 0 ? kBTiles[b->tile32[1]].kz : 0;
+    for (int q = 0; q < 4; ++q) s->grid_mode[q] = b->gmode[q & 1][q >> 1];
     return HT3D_OK;
     HT3D_API_END
 }
@@ -1231,6 +1219,14 @@ int ht3d_batch_tune_force(int tile) {
 }
 
 int ht3d_batch_tile_count(void) { return kNumBTiles; }
+
+int ht3d_batch_tune_force_grid(int mode) {
+    HT3D_API_BEGIN
+    if (mode > 1) return arg_fail("ht3d_batch_tune_force_grid: mode must be -1, 0 or 1");
+    g_force_bgrid = mode < 0 ? -1 : mode;
+    return HT3D_OK;
+    HT3D_API_END
+}
 
 int ht3d_batch_stats_reset(ht3d_batch *b) {
     HT3D_API_BEGIN

@@ -577,6 +577,7 @@ typedef struct {
     long long steps, blocks, graph_launches, graphs, repacks, iterations;
     int tile_rows[4], tile_z[4];/* chosen tiles: thermal FP64, Darcy FP64, thermal FP32, Darcy FP32 */
     int tile_cpl[4];            /* their x cells per lane                                 */
+    int grid_mode[4];           /* their launch grids: 0 persistent, 1 one block per item */
 } ht3d_batch_stats;
 
 int ht3d_batch_create(ht3d_hrl *const *models, int nprob, ht3d_batch **out);
@@ -596,6 +597,11 @@ int ht3d_batch_stats_reset(ht3d_batch *b);
  * restores timing.  No reference counterpart (the reference has no batch).   */
 int ht3d_batch_tune_force(int tile);
 int ht3d_batch_tile_count(void);
+/* The batch's launch grid, also chosen by timing (ht3d_batch_stats.grid_mode):
+ * 0 persistent (a few blocks per CU walk the work list), 1 one block per work
+ * item.  Pins it for batches tuned afterwards (-1: time both).  Results are
+ * the same bit for bit.  No reference counterpart.                           */
+int ht3d_batch_tune_force_grid(int mode);
 
 /* Lognormal permeability on vertices (spectral randomisation) plus an optional
  * planar fault; K (3*M) and phi (M) in host memory.  Set-up only.            */

@@ -128,7 +128,7 @@ def test_batch_nan_raises(ht):
 # every ncx a multiple of 4 (the 2- and 4-cell batch tiles run), 136 leaving a
 # partial x tile for both
 EVEN = [(16, 8, 16), (24, 12, 20), (36, 20, 12), (136, 8, 6), (12, 12, 12)]
-BATCH_TILE_CPL = (1, 1, 1, 1, 1, 1, 2, 2, 2, 4, 4, 4)
+BATCH_TILE_CPL = (1, 1, 1, 1, 1, 1, 2, 2, 2, 4, 4, 4, 2, 2, 2, 4, 2, 2, 2, 2, 4, 4, 4)
 
 
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32dc"])
@@ -164,3 +164,27 @@ def test_batch_tiles_bitwise(ht, precision):
         ht._lib.call("ht3d_batch_tune_force", -1)
     with pytest.raises(ht._lib.HT3DError):
         ht._lib.call("ht3d_batch_tune_force", n)
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32dc"])
+def test_batch_grid_modes_bitwise(ht, precision):
+    """The persistent grid and the one-block-per-item grid walk the same items
+    with the same operations: identical reports and fields."""
+    res = []
+    try:
+        for mode in (0, 1):
+            ht._lib.call("ht3d_batch_tune_force", 7 if precision == 0 else 10)
+            ht._lib.call("ht3d_batch_tune_force_grid", mode)
+            reps, out, stats = _run(ht, precision, graphs=True, nsteps=2, sizes=EVEN)
+            assert stats["grid_mode"][2 * precision:2 * precision + 2] == [mode, mode]
+            res.append((reps, out))
+    finally:
+        ht._lib.call("ht3d_batch_tune_force", -1)
+        ht._lib.call("ht3d_batch_tune_force_grid", -1)
+    (ra, fa), (rb, fb) = res
+    for sa, sb in zip(ra, rb):
+        for a, c in zip(sa, sb):
+            assert a["thermal"] == c["thermal"] and a["darcy"] == c["darcy"] and a["Nu"] == c["Nu"]
+    for a, c in zip(fa, fb):
+        for k in a:
+            assert np.array_equal(a[k], c[k]), k
