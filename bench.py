@@ -427,6 +427,9 @@ def main(argv=None) -> int:
                          "on the same inputs and report the DC speed-up and its error against FP64")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--check", default="50", help="check interval: an integer, or 'auto' for the check-interval tuner")
+    ap.add_argument("--ref-tol", type=float, default=0.0,
+                    help="also run FP64 at this tighter tolerance on the same inputs and steps, and report the T, P "
+                         "and Nu errors of both forms against it (0: skip)")
     ap.add_argument("--cpu-iters", type=int, default=3000,
                     help="pseudo-iterations of the CPU-baseline sample (about 10 s on 16 host threads)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -502,6 +505,23 @@ def main(argv=None) -> int:
               if st_dc["ms_fp32"] > 0 else None,
               "inner_iters_per_step": [(r["inner_thermal"], r["inner_darcy"]) for r in rep_dc],
               "outer_iters_per_step": [(r["dc_outer_thermal"], r["dc_outer_darcy"]) for r in rep_dc]}
+    ref = None
+    if args.ref_tol > 0:
+        import numpy as np
+        log(f"FP64 reference at tol {args.ref_tol:g} on the same inputs")
+        _, _, rep_ref, _, f_ref = run_gpu(w, args.steps, args.warmup, _lib.FP64,
+                                          ht.production_params(tol=args.ref_tol, ncheck=ncheck))
+        Pr = f_ref["P"] - f_ref["P"].mean()
+        nu_ref = np.array([r["Nu"] for r in rep_ref])
+
+        def vs_ref(f, reps):
+            nu = np.array([r["Nu"] for r in reps])
+            return {"T": rel(f["T"], f_ref["T"]), "P": rel(f["P"] - f["P"].mean(), Pr),
+                    "Nu": float(np.max(np.abs(nu - nu_ref) / np.abs(nu_ref)))}
+        ref = {"tol": args.ref_tol, "iters_per_step": [(r["thermal"]["iters"], r["darcy"]["iters"]) for r in rep_ref],
+               "rel_err_fp64": vs_ref(fields, reports)}
+        if args.precision == "both":
+            ref["rel_err_dc"] = vs_ref(f_dc, rep_dc)
     teff = st["bytes"] / (st["ms"] / 1e3) / 1e9 if st["ms"] > 0 else 0.0
     teff32 = st["bytes_fp32"] / (st["ms_fp32"] / 1e3) / 1e9 if st["ms_fp32"] > 0 else None
     avg_us = st["ms"] / max(st["launches"], 1) * 1e3
@@ -565,6 +585,8 @@ def main(argv=None) -> int:
         out["t_eff_fp32_inner"] = round(teff32, 2)
     if dc is not None:
         out["fp32_dc"] = dc
+    if ref is not None:
+        out["vs_converged_fp64"] = ref
     print(json.dumps(out), flush=True)
     if args.calib_copies > 0:
         calibration_copies(args.calib_copies)

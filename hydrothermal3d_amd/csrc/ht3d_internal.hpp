@@ -87,6 +87,23 @@ int workspace(int slot, size_t bytes, void **out);
         if (e_ != hipSuccess) return ::ht3d::hip_fail(e_, #expr);             \
     } while (0)
 
+// Two timing events owned by a tuner, destroyed on every return path.
+struct TimerEvents {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int create() {
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        return 0;
+    }
+    TimerEvents() = default;
+    TimerEvents(const TimerEvents &) = delete;
+    TimerEvents &operator=(const TimerEvents &) = delete;
+    ~TimerEvents() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
 // after a launch: check the launch, then synchronise if requested
 int after_launch(const char *name, bool force_sync = false);
 

@@ -884,9 +884,9 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
     te.key = key;
     te.us = 1e300;
     te.pair_us = 1e300;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    TimerEvents ev;
+    if (int rc_ = ev.create()) return rc_;
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     // The timed launches ping-pong the iterate between two buffers, as the
     // solve does (what the MALL holds at the next launch depends on it): a
     // scratch copy of the input and the solve's second buffer, so the solve's
@@ -1013,8 +1013,6 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
             }
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     g_tune.push_back(te);
     *out = te;
     return HT3D_OK;

@@ -381,9 +381,9 @@ static int choose_tile(const Args &base, double *scratch_x, double *scratch_f0, 
     hipStream_t st = ctx().stream;
     const size_t N = (size_t)base.nx * base.ny * base.nz;
     HIP_TRY(hipMemsetAsync(scratch_f0, 0, N * sizeof(double), st));
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    TimerEvents ev;
+    if (int rc_ = ev.create()) return rc_;
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     // the timed launches ping-pong the iterate between a scratch copy of the
     // input and scratch_x, as the solve does (the MALL contents at each launch
     // depend on it); each tile keeps its best of three interleaved rounds
@@ -421,8 +421,6 @@ static int choose_tile(const Args &base, double *scratch_x, double *scratch_f0, 
     for (int tile = 1; tile < kNumV18Tiles; ++tile)
         if (tbest[tile] < tbest[best]) best = tile;
     HIP_TRY(hipGetLastError());
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     tile_cache()[key] = best;
     *tile_out = best;
     return HT3D_OK;

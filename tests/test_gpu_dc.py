@@ -103,6 +103,44 @@ def test_fault_copy_oracle_and_dc(ht, orc, name):
     assert np.max(np.abs(N1 - N0) / np.abs(N0)) <= 1e-6
 
 
+def _fault_steps(ht, w, p, prec, nsteps):
+    from hydrothermal3d_amd import inputs
+    m = ht.HydroThermal3D(w.ncx, w.ncy, w.ncz, w.Lx, w.Ly, w.Lz, w.Ra, w.cfl)
+    m["Tc_ex"].set(inputs.initial_temperature(w))
+    inputs.permeability_device(w, m["kfv"])
+    m.set_viscosity(w.mu_a)
+    m.set_params(p, p, 1, prec)
+    reps = [m.step() for _ in range(nsteps)]
+    T, P = m["Tc_ex"].get()[1:-1, 1:-1, 1:-1], m["Pc_ex"].get()[1:-1, 1:-1, 1:-1]
+    m.close()
+    return T, P - P.mean(), np.array([r["Nu"] for r in reps]), reps
+
+
+def test_768_fault_copy_dc_at_tol_1e8(ht):
+    """The 768^3-class faulted configuration (Ra 2000, lognormal k x 100 in a
+    60-degree slab, mu = exp(-ln10 T)) on a 128x64x128 copy at its tolerance
+    1e-8, two steps.  P and Nu of the FP32+DC steps agree with the FP64 steps
+    within 1e-6.  T is compared with both forms' distance to the converged
+    solution (FP64 at tol 1e-11): T at the few cells where the fault meets the
+    thermal boundary layer moves with P errors far below the 1e-8 stopping
+    test, so at this tolerance FP64 itself sits up to about 1e-5 from the
+    converged T (measured at full size); the bar is that DC is no further from
+    it than 1e-6 or twice the FP64 distance, whichever is larger."""
+    from hydrothermal3d_amd import inputs
+    w = inputs.WORKLOADS["hrl768_fault"].scaled(128, 64, 128)
+    T0, P0, N0, r0 = _fault_steps(ht, w, ht.production_params(tol=1e-8), 0, 2)
+    T1, P1, N1, r1 = _fault_steps(ht, w, ht.production_params(tol=1e-8), 1, 2)
+    Tr, Pr, Nr, _ = _fault_steps(ht, w, ht.production_params(tol=1e-11), 0, 2)
+    for a, c in zip(r0, r1):
+        assert a["thermal"]["status"] == c["thermal"]["status"] == 0
+        assert a["darcy"]["status"] == c["darcy"]["status"] == 0
+    assert rel_err(P1, P0) <= 1e-6
+    assert np.max(np.abs(N1 - N0) / np.abs(N0)) <= 1e-6
+    e64, edc = rel_err(T0, Tr), rel_err(T1, Tr)
+    assert edc <= max(1e-6, 2.0 * e64), (edc, e64, rel_err(T1, T0))
+    assert rel_err(P1, Pr) <= max(1e-6, 2.0 * rel_err(P0, Pr))
+
+
 @pytest.mark.parametrize("ncheck", [1, 2, 3, 7])
 @pytest.mark.parametrize("transient", [0, 1], ids=["darcy", "heat"])
 def test_dc_anchor_intervals(ht, transient, ncheck):
