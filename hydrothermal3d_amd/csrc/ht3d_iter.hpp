@@ -84,11 +84,44 @@ __device__ __forceinline__ T ghost(int type, T val, T inner) {
 template <typename T, bool TRANSIENT>
 __device__ __forceinline__ T cell_residual(const IterArgs<T> &a, T xc, T xW, T xE, T xS, T xN, T xB, T xF, T kW,
                                            T kE, T kS, T kN, T kB, T kF, T r) {
+    if constexpr (sizeof(T) == 4) {
+        // FP32 sweeps: _dx (qE - qW) = _dx^2 (kW (xc - xW) - kE (xE - xc)), fused
+        const T tx = __builtin_fmaf(kW, xc - xW, -(kE * (xE - xc)));
+        const T ty = __builtin_fmaf(kS, xc - xS, -(kN * (xN - xc)));
+        const T tz = __builtin_fmaf(kB, xc - xB, -(kF * (xF - xc)));
+        const T s = TRANSIENT ? __builtin_fmaf(a._dt, xc, r) : r;
+        return __builtin_fmaf(a.idx2, tx, __builtin_fmaf(a.idy2, ty, __builtin_fmaf(a.idz2, tz, s)));
+    }
     const T qW = -kW * (a._dx * (xc - xW)), qE = -kE * (a._dx * (xE - xc));
     const T qS = -kS * (a._dy * (xc - xS)), qN = -kN * (a._dy * (xN - xc));
     const T qB = -kB * (a._dz * (xc - xB)), qF = -kF * (a._dz * (xF - xc));
     if (TRANSIENT) return r + a._dt * xc + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
     return r + a._dx * (qE - qW) + a._dy * (qN - qS) + a._dz * (qF - qB);
+}
+
+// The preconditioned residual F/D, the DYREL momentum h1 F0 + h2 F
+// (DYRELUpdate!, kernels_HT3D_dimensional.jl:323-324) and the update
+// x - dtau F0.  FP64 evaluates them as written, one rounding per operation and
+// an IEEE division: bit parity with the oracle.  The FP32 sweeps of the defect
+// correction have no reference counterpart and, at large grids, are limited by
+// their instruction count rather than by HBM: they multiply by the hardware
+// reciprocal (1 ulp) and fuse the multiply-adds.  Every FP32 kernel (single,
+// pair, vector and batched) goes through these same helpers, so they stay bit
+// for bit identical to each other.
+template <typename T>
+__device__ __forceinline__ T precond(T f, T D) {
+    if constexpr (sizeof(T) == 4) return f * __builtin_amdgcn_rcpf(D);
+    else return f / D;
+}
+template <typename T>
+__device__ __forceinline__ T momentum(T h1, T f0o, T h2, T f) {
+    if constexpr (sizeof(T) == 4) return __builtin_fmaf(h1, f0o, h2 * f);
+    else return h1 * f0o + h2 * f;
+}
+template <typename T>
+__device__ __forceinline__ T step_x(T xc, T dtau, T f0) {
+    if constexpr (sizeof(T) == 4) return __builtin_fmaf(-dtau, f0, xc);
+    else return xc - dtau * f0;
 }
 
 // GershgorinPoisson! diagonal D of one cell (kernels_HT3D_dimensional.jl:207-221).
@@ -99,7 +132,7 @@ __device__ __forceinline__ T cell_residual(const IterArgs<T> &a, T xc, T xW, T x
 template <typename T>
 __device__ __forceinline__ T cell_diag(const IterArgs<T> &a, T kW, T kE, T kS, T kN, T kB, T kF) {
     if constexpr (sizeof(T) == 4)
-        return (kB + kF) * a.idz2 + (kN + kS) * a.idy2 + (kE + kW) * a.idx2 + a.rc_tr_dt;
+        return __builtin_fmaf(kB + kF, a.idz2, __builtin_fmaf(kN + kS, a.idy2, __builtin_fmaf(kE + kW, a.idx2, a.rc_tr_dt)));
     return (kB / a.dz + kF / a.dz) / a.dz + (kN / a.dy + kS / a.dy) / a.dy + (kE / a.dx + kW / a.dx) / a.dx +
            a.rc_tr_dt;
 }
@@ -176,12 +209,12 @@ __device__ __forceinline__ void iter_tile(const IterArgs<T> &a, const T *__restr
             // diagonal D (stored once per solve, or recomputed from the face
             // coefficients already in registers) and "F /= PC"
             const T D = DSTORED ? a.Dst[c] : cell_diag(a, kW, kE, kS, kN, kB, kF);
-            f = f / D;
+            f = precond<T>(f, D);
             // DYRELUpdate! (kernels_HT3D_dimensional.jl:323-324)
-            const T f0 = h1 * f0o + h2 * f;
+            const T f0 = momentum<T>(h1, f0o, h2, f);
             __builtin_nontemporal_store(f0, a.F0 + c);
-            if (NTX) __builtin_nontemporal_store(xc - a.dtau * f0, Xout + e);
-            else Xout[e] = xc - a.dtau * f0;
+            if (NTX) __builtin_nontemporal_store(step_x<T>(xc, a.dtau, f0), Xout + e);
+            else Xout[e] = step_x<T>(xc, a.dtau, f0);
             if (check) {
                 const double fd = fabs((double)(f * D));
                 dd_add(s_hi, s_lo, fd);
@@ -316,9 +349,9 @@ __device__ __forceinline__ void iter_tile_v(const IterArgs<T> &a, const T *__res
             if (DSTORED) D = ldv<T, CPL>(a.Dst + c);
 #pragma unroll
             for (int q = 0; q < CPL; ++q) {
-                f[q] = f[q] / D[q];
-                g[q] = h1 * f0o[q] + h2 * f[q];   // DYRELUpdate! (kernels_HT3D_dimensional.jl:323-324)
-                xo[q] = xc[q] - a.dtau * g[q];
+                f[q] = precond<T>(f[q], D[q]);
+                g[q] = momentum<T>(h1, f0o[q], h2, f[q]);   // DYRELUpdate! (kernels_HT3D_dimensional.jl:323-324)
+                xo[q] = step_x<T>(xc[q], a.dtau, g[q]);
             }
             stv_nt<T, CPL>(g, a.F0 + c);
             if (NTX) stv_nt<T, CPL>(xo, Xout + e);
@@ -436,9 +469,9 @@ __global__ __launch_bounds__(64 * TY) void dyrel_pair_kernel(IterArgs<T> a, cons
             if (shi) xN = ghost(a.bct[3], a.bcv[3], x0c);
             T f = cell_residual<T, TRANSIENT>(a, x0c, xW, xE, xS, xN, x0m, xp, kW, kE, kS, kN, kB, kF, r);
             const T D = DSTORED ? __builtin_nontemporal_load(a.Dst + c) : cell_diag(a, kW, kE, kS, kN, kB, kF);
-            f = f / D;
-            const T f0 = h1 * f0o + h2 * f;
-            x1p = x0c - a.dtau * f0;
+            f = precond<T>(f, D);
+            const T f0 = momentum<T>(h1, f0o, h2, f);
+            x1p = step_x<T>(x0c, a.dtau, f0);
             L[p % 3][ly][lx] = x1p;
             x0m = x0c;
             x0c = xp;
@@ -454,10 +487,10 @@ __global__ __launch_bounds__(64 * TY) void dyrel_pair_kernel(IterArgs<T> a, cons
                 if (slo) yS = ghost(a.bct[2], a.bcv[2], x1c);
                 if (shi) yN = ghost(a.bct[3], a.bcv[3], x1c);
                 T g = cell_residual<T, TRANSIENT>(a, x1c, yW, yE, yS, yN, yB, x1p, pkW, pkE, pkS, pkN, pkB, pkF, pr);
-                g = g / pD;
-                const T g0 = h1 * pf0 + h2 * g;
+                g = precond<T>(g, pD);
+                const T g0 = momentum<T>(h1, pf0, h2, g);
                 __builtin_nontemporal_store(g0, a.F0 + pc);
-                a.Xout[pe_idx] = x1c - a.dtau * g0;
+                a.Xout[pe_idx] = step_x<T>(x1c, a.dtau, g0);
             }
             x1m = x1c;
             x1c = x1p;
@@ -481,10 +514,10 @@ __global__ __launch_bounds__(64 * TY) void dyrel_pair_kernel(IterArgs<T> a, cons
                 if (slo) yS = ghost(a.bct[2], a.bcv[2], x1c);
                 if (shi) yN = ghost(a.bct[3], a.bcv[3], x1c);
                 T g = cell_residual<T, TRANSIENT>(a, x1c, yW, yE, yS, yN, yB, yF, pkW, pkE, pkS, pkN, pkB, pkF, pr);
-                g = g / pD;
-                const T g0 = h1 * pf0 + h2 * g;
+                g = precond<T>(g, pD);
+                const T g0 = momentum<T>(h1, pf0, h2, g);
                 __builtin_nontemporal_store(g0, a.F0 + pc);
-                a.Xout[pe_idx] = x1c - a.dtau * g0;
+                a.Xout[pe_idx] = step_x<T>(x1c, a.dtau, g0);
             }
         }
     }
@@ -555,9 +588,9 @@ __global__ __launch_bounds__(64 * TY) void dyrel_pair_kernel_v(IterArgs<T> a, co
             const T kw = q == 0 ? pkW : (T)pkE[q - 1];
             T g = cell_residual<T, TRANSIENT>(a, x1c[q], w, ea, yS[q], yN[q], yB[q], yF[q], kw, pkE[q], pkS[q], pkN[q],
                                               pkB[q], pkF[q], pr[q]);
-            g = g / pD[q];
-            g0[q] = h1 * pf0[q] + h2 * g;
-            xo[q] = x1c[q] - a.dtau * g0[q];
+            g = precond<T>(g, pD[q]);
+            g0[q] = momentum<T>(h1, pf0[q], h2, g);
+            xo[q] = step_x<T>(x1c[q], a.dtau, g0[q]);
         }
         stv_nt<T, CPL>(g0, a.F0 + pc);
         stv<T, CPL>(xo, a.Xout + pe_idx);
@@ -591,9 +624,9 @@ __global__ __launch_bounds__(64 * TY) void dyrel_pair_kernel_v(IterArgs<T> a, co
                 T f = cell_residual<T, TRANSIENT>(a, x0c[q], w, ea, xS[q], xN[q], x0m[q], xp[q], kw, kE[q], kS[q], kN[q],
                                                   kB[q], kF[q], r[q]);
                 if (!DSTORED) D[q] = cell_diag(a, kw, (T)kE[q], (T)kS[q], (T)kN[q], (T)kB[q], (T)kF[q]);
-                f = f / D[q];
-                f0[q] = h1 * f0o[q] + h2 * f;
-                x1p[q] = x0c[q] - a.dtau * f0[q];
+                f = precond<T>(f, D[q]);
+                f0[q] = momentum<T>(h1, f0o[q], h2, f);
+                x1p[q] = step_x<T>(x0c[q], a.dtau, f0[q]);
             }
             *reinterpret_cast<V *>(&L[p % 3][ly][CPL * lx]) = x1p;
             x0m = x0c;
