@@ -430,6 +430,10 @@ def main(argv=None) -> int:
     ap.add_argument("--ref-tol", type=float, default=0.0,
                     help="also run FP64 at this tighter tolerance on the same inputs and steps, and report the T, P "
                          "and Nu errors of both forms against it (0: skip)")
+    ap.add_argument("--ref-schedule-steps", type=int, default=2,
+                    help="also time this many FP64 steps (after one warm-up) on the reference's solver schedule: "
+                         "lambda_min refresh and mean-norm convergence test at iter<=2 or every 1000 iterations, "
+                         "nitmax 10000 (HydroThermal3D_v18.jl:309-310,450-476); 0: skip")
     ap.add_argument("--cpu-iters", type=int, default=3000,
                     help="pseudo-iterations of the CPU-baseline sample (about 10 s on 16 host threads)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -522,6 +526,16 @@ def main(argv=None) -> int:
                "rel_err_fp64": vs_ref(fields, reports)}
         if args.precision == "both":
             ref["rel_err_dc"] = vs_ref(f_dc, rep_dc)
+    refsched = None
+    if args.ref_schedule_steps > 0:
+        log("FP64 on the reference's solver schedule (checks and lambda refresh every 1000 iterations)")
+        wall_r, st_r, rep_r, _, _ = run_gpu(w, args.ref_schedule_steps, 1, _lib.FP64, ht.reference_params(tol=args.tol))
+        refsched = {"ms_per_step": round(wall_r / args.ref_schedule_steps * 1e3, 3),
+                    "iters_per_step": [(r["thermal"]["iters"], r["darcy"]["iters"]) for r in rep_r],
+                    "status_per_step": [(r["thermal"]["status"], r["darcy"]["status"]) for r in rep_r],
+                    "t_eff": round(st_r["bytes"] / (st_r["ms"] / 1e3) / 1e9, 2) if st_r["ms"] > 0 else None,
+                    "schedule": "nitout 1000, convergence test at iter<=2 or iter%1000==0, mean|F*D|/sqrt(N), "
+                                "nitmax 10000"}
     teff = st["bytes"] / (st["ms"] / 1e3) / 1e9 if st["ms"] > 0 else 0.0
     teff32 = st["bytes_fp32"] / (st["ms_fp32"] / 1e3) / 1e9 if st["ms_fp32"] > 0 else None
     avg_us = st["ms"] / max(st["launches"], 1) * 1e3
@@ -587,6 +601,8 @@ def main(argv=None) -> int:
         out["fp32_dc"] = dc
     if ref is not None:
         out["vs_converged_fp64"] = ref
+    if refsched is not None:
+        out["reference_schedule_fp64"] = refsched
     print(json.dumps(out), flush=True)
     if args.calib_copies > 0:
         calibration_copies(args.calib_copies)
