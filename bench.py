@@ -199,7 +199,8 @@ def run_batch(n: int, steps: int, warmup: int, prm, graphs: bool = True,
                               "repacks": st["repacks"],
                               "tiles": [f"{r}x{z}" + (f" x{c}" if c > 1 else "")
                                         for r, z, c in zip(st["tile_rows"], st["tile_z"], st["tile_cpl"])],
-                              "grid": ["per-item" if g else "persistent" for g in st["grid_mode"]]})
+                              "grid": ["per-item" if g else "persistent" for g in st["grid_mode"]],
+                              "pairs": [f"{r}x{z}" if r else None for r, z in zip(st["pair_rows"], st["pair_z"])]})
             b.close()
         states[mode] = _batch_state(ms, reps)
         for m in ms:
@@ -447,6 +448,8 @@ def main(argv=None) -> int:
     ap.add_argument("--batch-graphs", type=int, choices=[0, 1], default=1,
                     help="replay the batch's pseudo-iteration blocks as HIP graphs (1) or launch them kernel by "
                          "kernel (0; the form to run under a profiler)")
+    ap.add_argument("--batch-pairs", type=int, choices=[-1, 0, 1], default=-1,
+                    help="two pseudo-iterations per launch in the batch: -1 when timed faster, 0 never, 1 always")
     ap.add_argument("--batch-modes", default="plain_fp64,batched_fp64,batched_dc",
                     help="comma list of the batched leg's forms: plain_fp64, batched_fp64, batched_dc")
     ap.add_argument("--forward-only", action="store_true",
@@ -471,6 +474,7 @@ def main(argv=None) -> int:
     from hydrothermal3d_amd import _lib, inputs
 
     w = inputs.WORKLOADS[args.workload]
+    _lib.call("ht3d_batch_tune_pairs", args.batch_pairs)
     if args.forward_only:
         _lib.call("ht3d_tune_sweep", 0)
     if args.probe:

@@ -347,6 +347,147 @@ __global__ __launch_bounds__(256) void batch_zero_kernel(uint4 *__restrict__ p, 
         p[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
+// Pair tiles of the batch (rows of lanes per block, planes per item), one x-cell per lane.
+#define HT3D_BATCH_PAIR_TILES(X) X(0, 8, 16) X(1, 8, 32) X(2, 16, 8) X(3, 16, 16) X(4, 16, 32)
+#define HT3D_PAIR_CFG(N, TY, KZ) {TY, KZ, 1},
+static constexpr TileCfg kBPairTiles[] = {HT3D_BATCH_PAIR_TILES(HT3D_PAIR_CFG)};
+#undef HT3D_PAIR_CFG
+static constexpr int kNumBPairTiles = sizeof(kBPairTiles) / sizeof(kBPairTiles[0]);
+
+// rows of a band (iteration n+1) and inner rows (iteration n+2) of pair tile
+// `pt` on a grid nx cells wide; the tile fits when every band has an inner row
+static inline int pair_band_rows(int pt, int nx) { return 64 * kBPairTiles[pt].ty / nx; }
+static inline bool pair_tile_fits(int pt, int nx) { return pair_band_rows(pt, nx) >= 3; }
+static inline unsigned pair_items(int nx, int ny, int nz, int pt) {
+    return div_up(ny, pair_band_rows(pt, nx) - 2) * div_up(nz, kBPairTiles[pt].kz);
+}
+
+// Two plain pseudo-iterations per launch, on bands of whole rows.  An item is
+// a band of RB = 64*TY/nx rows of one problem (the batch's grids are 32 to 128
+// cells wide), marched over KZ planes.  Iteration n+1 runs on the band's rows
+// [bb*RI - 1, bb*RI + RI + 1), iteration n+2 on the RI = RB - 2 inner rows, so
+// neighbouring bands overlap by one row on each side.  A lane holds cell
+// (tid mod nx, row tid / nx) of the band: its x neighbours are lanes tid -+ 1
+// and its y neighbours lanes tid -+ nx, reached through a triple-buffered LDS
+// plane; the z neighbours go through a register queue, n+1 running one plane
+// ahead of n+2 (the scheme of dyrel_pair_kernel).  Lanes past the last whole
+// row, and rows outside the grid, compute on clamped (in-bounds) cells and
+// write nothing.  F0 is read from one of the two F0 buffers (fpar) and
+// written to the other, since an overlapping band still needs the old F0; X
+// goes from the `phase` buffer to the other.  Every cell is computed with the
+// single iteration's operations in the same order: two single launches and one
+// pair launch give bit-identical X and F0.
+template <typename T, bool TR, int TY, int KZ>
+__global__ __launch_bounds__(64 * TY) void batch_pair_kernel(const IterArgs<T> *__restrict__ tab,
+                                                             const uint2 *__restrict__ items,
+                                                             const unsigned *__restrict__ n_items,
+                                                             const BState *__restrict__ st, int phase, int fpar,
+                                                             long long dF0, unsigned flags) {
+    constexpr int NT = 64 * TY;
+    __shared__ T L[3][NT];
+    const unsigned n = *n_items;
+    const int tid = (int)threadIdx.x;
+    for (unsigned w = blockIdx.x; w < n; w += gridDim.x) {
+        const unsigned t = xcd_tile(w, n, flags & FL_REVERSE);
+        const uint2 it = items[t];
+        if (!st[it.x].active) continue;   // the whole block skips the item
+        const IterArgs<T> &a = tab[it.x];
+        const int nx = a.nx, ny = a.ny, nz = a.nz;
+        const int RB = NT / nx, RI = RB - 2;
+        const int bb = (int)(it.y % (unsigned)a.nband), bz = (int)(it.y / (unsigned)a.nband);
+        const int r = tid / nx, i = tid - r * nx;
+        const int j = bb * RI - 1 + r;
+        const bool in_dom = r < RB && j >= 0 && j < ny;
+        const bool inner = in_dom && r >= 1 && r <= RB - 2;   // computes iteration n+2
+        const int jc = min(max(j, 0), ny - 1);                // every load in bounds
+        // Pointers read from the table are generic to the compiler, whose flat
+        // accesses count against the LDS wait counter as well: typed as global,
+        // the loads and stores of the march do not hold up the LDS waits.
+        typedef __attribute__((address_space(1))) T GT;
+        typedef __attribute__((address_space(1))) const T GCT;
+        GCT *__restrict__ Xin = (GCT *)(phase ? a.Xout : a.Xin);
+        GT *__restrict__ Xout = (GT *)(phase ? const_cast<T *>(a.Xin) : a.Xout);
+        GCT *__restrict__ F0in = (GCT *)(fpar ? a.F0 + dF0 : a.F0);
+        GT *__restrict__ F0out = (GT *)(fpar ? a.F0 : a.F0 + dF0);
+        GCT *__restrict__ kxg = (GCT *)a.kx, *__restrict__ kyg = (GCT *)a.ky, *__restrict__ kzg = (GCT *)a.kz;
+        GCT *__restrict__ Rg = (GCT *)a.R;
+        const int k0 = bz * KZ, k1 = min(k0 + KZ, nz);
+        const uint32_t sE1 = nx + 2, sE2 = sE1 * (ny + 2);
+        const uint32_t sC2 = (uint32_t)nx * ny, sX1 = nx + 1, sX2 = sX1 * ny, sY2 = (uint32_t)nx * (ny + 1);
+        const bool wlo = i == 0, whi = i == nx - 1, slo = jc == 0, shi = jc == ny - 1;
+        const int ps = max(k0 - 1, 0), pe = min(k1, nz - 1);   // planes of iteration n+1
+        const T h1 = a.hv ? a.hv[0] : a.h1, h2 = a.hv ? a.hv[1] : a.h2;
+        uint32_t e = (uint32_t)(i + 1) + (uint32_t)(jc + 1) * sE1 + (uint32_t)(ps + 1) * sE2;
+        uint32_t c = (uint32_t)i + (uint32_t)jc * nx + (uint32_t)ps * sC2;
+        uint32_t fx = (uint32_t)i + (uint32_t)jc * sX1 + (uint32_t)ps * sX2;
+        uint32_t fy = (uint32_t)i + (uint32_t)jc * nx + (uint32_t)ps * sY2;
+        T x0c = Xin[e];
+        T x0m = Xin[e - sE2];
+        if (ps == 0) x0m = ghost(a.bct[4], a.bcv[4], x0c);
+        T kzB = kzg[c];
+        T x1m = 0, x1c = 0, pkW = 0, pkE = 0, pkS = 0, pkN = 0, pkB = 0, pkF = 0, pr = 0, pD = 1, pf0 = 0;
+        uint32_t pe_idx = 0, pc = 0;
+        // iteration n+2 on plane p-1, from the n+1 plane in L[(p-1) % 3]
+        auto second = [&](int p, T yF) {
+            const int q = (p - 1) % 3;
+            T yW = L[q][tid - 1], yE = L[q][tid + 1], yS = L[q][tid - nx], yN = L[q][tid + nx];
+            T yB = x1m;
+            if (p - 1 == 0) yB = ghost(a.bct[4], a.bcv[4], x1c);
+            if (wlo) yW = ghost(a.bct[0], a.bcv[0], x1c);
+            if (whi) yE = ghost(a.bct[1], a.bcv[1], x1c);
+            if (slo) yS = ghost(a.bct[2], a.bcv[2], x1c);
+            if (shi) yN = ghost(a.bct[3], a.bcv[3], x1c);
+            T g = cell_residual<T, TR>(a, x1c, yW, yE, yS, yN, yB, yF, pkW, pkE, pkS, pkN, pkB, pkF, pr);
+            g = precond<T>(g, pD);
+            const T g0 = momentum<T>(h1, pf0, h2, g);
+            __builtin_nontemporal_store(g0, F0out + pc);
+            Xout[pe_idx] = step_x<T>(x1c, a.dtau, g0);
+        };
+        for (int p = ps; p <= pe + 1; ++p) {
+            if (p <= pe) {   // iteration n+1 on plane p
+                T xp = Xin[e + sE2];
+                T xW = Xin[e - 1], xE = Xin[e + 1];
+                T xS = Xin[e - sE1], xN = Xin[e + sE1];
+                const T kW = kxg[fx], kE = kxg[fx + 1];
+                const T kS = kyg[fy], kN = kyg[fy + nx];
+                const T kF = kzg[c + sC2];
+                const T kB = kzB;
+                const T rr = __builtin_nontemporal_load(Rg + c);
+                const T f0o = __builtin_nontemporal_load(F0in + c);
+                if (p + 1 == nz) xp = ghost(a.bct[5], a.bcv[5], x0c);
+                if (wlo) xW = ghost(a.bct[0], a.bcv[0], x0c);
+                if (whi) xE = ghost(a.bct[1], a.bcv[1], x0c);
+                if (slo) xS = ghost(a.bct[2], a.bcv[2], x0c);
+                if (shi) xN = ghost(a.bct[3], a.bcv[3], x0c);
+                T f = cell_residual<T, TR>(a, x0c, xW, xE, xS, xN, x0m, xp, kW, kE, kS, kN, kB, kF, rr);
+                const T D = cell_diag(a, kW, kE, kS, kN, kB, kF);
+                f = precond<T>(f, D);
+                const T f0 = momentum<T>(h1, f0o, h2, f);
+                const T x1p = step_x<T>(x0c, a.dtau, f0);
+                L[p % 3][tid] = x1p;
+                x0m = x0c;
+                x0c = xp;
+                kzB = kF;
+                __syncthreads();
+                if (p - 1 >= k0 && inner) second(p, x1p);
+                x1m = x1c;
+                x1c = x1p;
+                pkW = kW; pkE = kE; pkS = kS; pkN = kN; pkB = kB; pkF = kF; pr = rr; pD = D; pf0 = f0;
+                pe_idx = e;
+                pc = c;
+                e += sE2;
+                c += sC2;
+                fx += sX2;
+                fy += sY2;
+            } else {         // past the top plane: iteration n+2 on plane nz-1 of the top chunk
+                __syncthreads();
+                if (p - 1 >= k0 && p - 1 < k1 && inner) second(p, ghost(a.bct[5], a.bcv[5], x1c));
+            }
+        }
+        __syncthreads();   // L is rewritten by the block's next item
+    }
+}
+
 // ---------------------------------------------------------------- launchers
 struct BLaunch {  // device pointers shared by every launch of a solve phase
     const void *tab;       // IterArgs<T> table of the phase
@@ -357,6 +498,9 @@ struct BLaunch {  // device pointers shared by every launch of a solve phase
     double *partials;
     unsigned grid;
     unsigned threads;
+    const uint2 *pitems;       // work list of the pair launches
+    const unsigned *n_pitems;
+    unsigned pgrid;
 };
 
 template <typename T, bool TR>
@@ -372,6 +516,19 @@ static void launch_iter_b(int tile, const BLaunch &L, int phase, unsigned flags)
         break;
     switch (tile) { HT3D_BATCH_TILES(BI_) default: break; }
 #undef BI_
+}
+
+template <typename T, bool TR>
+static void launch_pair_b(int pt, const BLaunch &L, int phase, int fpar, long long dF0, unsigned flags) {
+    const IterArgs<T> *tab = (const IterArgs<T> *)L.tab;
+    hipStream_t s = ctx().stream;
+#define BP_(N, TY, KZ)                                                                                               \
+    case N:                                                                                                          \
+        hipLaunchKernelGGL((batch_pair_kernel<T, TR, TY, KZ>), dim3(L.pgrid), dim3(64 * TY), 0, s, tab, L.pitems,    \
+                           L.n_pitems, L.st, phase, fpar, dF0, flags);                                              \
+        break;
+    switch (pt) { HT3D_BATCH_PAIR_TILES(BP_) default: break; }
+#undef BP_
 }
 
 template <bool TR>
@@ -401,11 +558,11 @@ static void launch_upd_b(int tile, const BLaunch &L, int phase) {
 
 // one operation of a block of pseudo-iterations
 enum BOpKind : int { OP_ITER64, OP_CHECK64, OP_FINAL64, OP_ITER32, OP_LAM32, OP_ANCHOR_RES, OP_ANCHOR, OP_UPDATE,
-                     OP_ADVANCE };
+                     OP_ADVANCE, OP_PAIR64, OP_PAIR32 };
 struct BOp {
     int kind;
     unsigned flags;
-    int a, b;  // phase / offset / run / L, lambda / rayleigh
+    int a, b;  // phase / offset / run / L, lambda / rayleigh / F0 buffer of a pair
 };
 
 }  // namespace ht3d
@@ -423,8 +580,9 @@ struct ht3d_batch {
     // per-problem solver buffers, offsets into contiguous regions
     std::vector<size_t> oc, oe, of;        // cell, E and face offsets (elements)
     size_t sc = 0, se = 0, sf = 0;         // totals
-    double *xb = nullptr, *f0 = nullptr, *fst = nullptr;
-    float *e0 = nullptr, *e1 = nullptr, *f032 = nullptr, *fst32 = nullptr, *r32 = nullptr, *k32 = nullptr;
+    double *xb = nullptr, *f0 = nullptr, *fst = nullptr, *f0b = nullptr;
+    float *e0 = nullptr, *e1 = nullptr, *f032 = nullptr, *fst32 = nullptr, *r32 = nullptr, *k32 = nullptr,
+          *f032b = nullptr;  // f0b, f032b: the second F0 buffers of the pair launches
     // tables, per solve kind (0 thermal, 1 Darcy)
     IterArgs<double> *d_it64[2] = {nullptr, nullptr};
     IterArgs<float> *d_it32[2] = {nullptr, nullptr};
@@ -438,19 +596,23 @@ struct ht3d_batch {
     // the launches being enqueued use
     int gmode[2][2] = {{0, 0}, {0, 0}};
     int gmode_cur = 0;
+    // pair tile (-1: single iterations only) and its launch grid per [kind][fp32];
+    // ptile_cur / pgmode_cur are those of the solve being enqueued
+    int ptile[2][2] = {{-1, -1}, {-1, -1}}, pgmode[2][2] = {{0, 0}, {0, 0}};
+    int ptile_cur = -1, pgmode_cur = 0;
     // check-interval tuner inputs per [kind][fp32]: sweep and check costs (us), last iterations
     double t_it[2][2] = {{-1, -1}, {-1, -1}}, t_chk[2][2] = {{-1, -1}, {-1, -1}};
     double last_iters[2][2] = {{-1, -1}, {-1, -1}};
     // work list, partials, states
-    uint2 *d_items = nullptr, *d_range = nullptr, *d_grange = nullptr;
-    unsigned *d_nitems = nullptr;  // [0] item count, [1] iteration base
-    size_t max_items = 0;
+    uint2 *d_items = nullptr, *d_range = nullptr, *d_grange = nullptr, *d_pitems = nullptr;
+    unsigned *d_nitems = nullptr;  // [0] item count, [1] iteration base, [2] pair item count
+    size_t max_items = 0, max_pitems = 0;
     double *d_part = nullptr, *d_gpart = nullptr, *d_nupart = nullptr, *d_nures = nullptr;
     std::vector<unsigned> g_off, g_nb, nu_off, nu_nb;
     BState *d_state = nullptr;
     // pinned host staging
     BState *h_state[2] = {nullptr, nullptr};
-    uint2 *h_items[2] = {nullptr, nullptr}, *h_range[2] = {nullptr, nullptr};
+    uint2 *h_items[2] = {nullptr, nullptr}, *h_range[2] = {nullptr, nullptr}, *h_pitems[2] = {nullptr, nullptr};
     unsigned *h_nitems[2] = {nullptr, nullptr};
     void *h_tabs = nullptr;
     size_t tabs_bytes = 0;
@@ -501,12 +663,27 @@ static size_t build_items(ht3d_batch *b, int tile, const std::vector<char> &act,
     return n;
 }
 
-static unsigned persistent_grid(const ht3d_batch *b, size_t items, int tile) {
-    const size_t threads = 64 * (size_t)kBTiles[tile].ty;
+// pair items (bands x plane chunks) of the problems with `act[q]` set
+static size_t build_pitems(ht3d_batch *b, int pt, const std::vector<char> &act, uint2 *items) {
+    size_t n = 0;
+    for (int q = 0; q < b->nprob; ++q) {
+        if (!act[q]) continue;
+        const ht3d_hrl *m = b->m[q];
+        const unsigned cnt = pair_items(m->nx, m->ny, m->nz, pt);
+        for (unsigned l = 0; l < cnt; ++l) items[n++] = make_uint2((unsigned)q, l);
+    }
+    return n;
+}
+
+static unsigned persistent_grid_threads(const ht3d_batch *b, size_t items, size_t threads) {
     size_t g = (size_t)b->ncu * (2048 / threads);
     if (g > items) g = items;
     if (g >= 8) g &= ~(size_t)7;  // a multiple of 8 keeps item w on XCD w mod 8
     return (unsigned)(g ? g : 1);
+}
+
+static unsigned persistent_grid(const ht3d_batch *b, size_t items, int tile) {
+    return persistent_grid_threads(b, items, 64 * (size_t)kBTiles[tile].ty);
 }
 
 // upload the work list of the active problems (pinned staging, stream ordered)
@@ -514,17 +691,23 @@ static int upload_items(ht3d_batch *b, int tile, const std::vector<char> &act, s
     const int s = b->stage;
     b->stage ^= 1;
     const size_t n = build_items(b, tile, act, b->h_items[s], b->h_range[s]);
+    const size_t np = b->ptile_cur >= 0 ? build_pitems(b, b->ptile_cur, act, b->h_pitems[s]) : 0;
     b->h_nitems[s][0] = (unsigned)n;
+    b->h_nitems[s][2] = (unsigned)np;
     hipStream_t st = ctx().stream;  // the staging half used here was last read before the previous poll's event
     if (n) HIP_TRY(hipMemcpyAsync(b->d_items, b->h_items[s], n * sizeof(uint2), hipMemcpyHostToDevice, st));
+    if (np) HIP_TRY(hipMemcpyAsync(b->d_pitems, b->h_pitems[s], np * sizeof(uint2), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(b->d_range, b->h_range[s], b->nprob * sizeof(uint2), hipMemcpyHostToDevice, st));
+    // the item count ([0]) and the pair item count ([2]), not the iteration base ([1]) between them
     HIP_TRY(hipMemcpyAsync(b->d_nitems, b->h_nitems[s], sizeof(unsigned), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(b->d_nitems + 2, b->h_nitems[s] + 2, sizeof(unsigned), hipMemcpyHostToDevice, st));
     if (n_out) *n_out = n;
     return HT3D_OK;
 }
 
 // fill the host tables of a solve kind from the problems' field descriptions
-static void fill_tables(ht3d_batch *b, int kind, const std::vector<ht3d_dyrel_field> &f, int tile64, int tile32) {
+static void fill_tables(ht3d_batch *b, int kind, const std::vector<ht3d_dyrel_field> &f, int tile64, int tile32,
+                        int pt64 = -1, int pt32 = -1) {
     for (int q = 0; q < b->nprob; ++q) {
         const ht3d_dyrel_field &d = f[q];
         const int nx = d.ncx, ny = d.ncy, nz = d.ncz;
@@ -537,6 +720,7 @@ static void fill_tables(ht3d_batch *b, int kind, const std::vector<ht3d_dyrel_fi
         a.nx = nx; a.ny = ny; a.nz = nz;
         a.nbx = (int)bpatch_count(nx, ny, tile64); a.nby = 1;   // patches of the (x, y) plane
         a.nbz = (int)div_up(nz, kBTiles[tile64].kz);
+        a.nband = pt64 >= 0 ? (int)div_up(ny, pair_band_rows(pt64, nx) - 2) : 0;
         a._dx = 1.0 / d.dx; a._dy = 1.0 / d.dy; a._dz = 1.0 / d.dz; a._dt = 1.0 / d.dt;
         a.dx = d.dx; a.dy = d.dy; a.dz = d.dz; a.rc_tr_dt = rc_tr_dt;
         set_inv_h2(a, d.dx, d.dy, d.dz);
@@ -552,6 +736,7 @@ static void fill_tables(ht3d_batch *b, int kind, const std::vector<ht3d_dyrel_fi
         g.nx = nx; g.ny = ny; g.nz = nz;
         g.nbx = (int)bpatch_count(nx, ny, tile32); g.nby = 1;
         g.nbz = (int)div_up(nz, kBTiles[tile32].kz);
+        g.nband = pt32 >= 0 ? (int)div_up(ny, pair_band_rows(pt32, nx) - 2) : 0;
         g._dx = (float)a._dx; g._dy = (float)a._dy; g._dz = (float)a._dz; g._dt = (float)a._dt;
         g.dx = (float)d.dx; g.dy = (float)d.dy; g.dz = (float)d.dz; g.rc_tr_dt = (float)rc_tr_dt;
         set_inv_h2(g, d.dx, d.dy, d.dz);
@@ -597,6 +782,15 @@ static BLaunch make_launch(ht3d_batch *b, int kind, bool fp32, int tile) {
     }
     L.grid = b->gmode_cur ? (unsigned)(n ? n : 1) : persistent_grid(b, n, tile);
     L.threads = 64 * kBTiles[tile].ty;
+    L.pitems = b->d_pitems;
+    L.n_pitems = b->d_nitems + 2;
+    L.pgrid = 1;
+    if (b->ptile_cur >= 0) {
+        size_t np = 0;
+        for (int q = 0; q < b->nprob; ++q) np += pair_items(b->m[q]->nx, b->m[q]->ny, b->m[q]->nz, b->ptile_cur);
+        L.pgrid = b->pgmode_cur ? (unsigned)(np ? np : 1)
+                                : persistent_grid_threads(b, np, 64 * (size_t)kBPairTiles[b->ptile_cur].ty);
+    }
     return L;
 }
 
@@ -616,6 +810,16 @@ static int enqueue_ops(ht3d_batch *b, const std::vector<BOp> &ops, int kind, boo
         case OP_ITER32:
             if (transient) launch_iter_b<float, true>(tile, L, o.a, o.flags);
             else launch_iter_b<float, false>(tile, L, o.a, o.flags);
+            break;
+        case OP_PAIR64:
+            if (b->ptile_cur < 0) return arg_fail("batch: pair launch without a pair tile");
+            if (transient) launch_pair_b<double, true>(b->ptile_cur, L, o.a, o.b, (long long)(b->f0b - b->f0), o.flags);
+            else launch_pair_b<double, false>(b->ptile_cur, L, o.a, o.b, (long long)(b->f0b - b->f0), o.flags);
+            break;
+        case OP_PAIR32:
+            if (b->ptile_cur < 0) return arg_fail("batch: pair launch without a pair tile");
+            if (transient) launch_pair_b<float, true>(b->ptile_cur, L, o.a, o.b, (long long)(b->f032b - b->f032), o.flags);
+            else launch_pair_b<float, false>(b->ptile_cur, L, o.a, o.b, (long long)(b->f032b - b->f032), o.flags);
             break;
         case OP_CHECK64:
             hipLaunchKernelGGL(batch_check_kernel, dim3(nq), dim3(1024), 0, s, b->d_state, b->d_it64[kind], b->d_range,
@@ -655,7 +859,7 @@ static int run_block(ht3d_batch *b, const std::vector<BOp> &ops, int kind, bool 
     if (!b->graphs) return enqueue_ops(b, ops, kind, transient, tile, p);
     std::string key;
     key.reserve(ops.size() * 16 + 16);
-    const int hdr[5] = {kind, transient ? 1 : 0, tile, b->precision, b->gmode_cur};
+    const int hdr[7] = {kind, transient ? 1 : 0, tile, b->precision, b->gmode_cur, b->ptile_cur, b->pgmode_cur};
     key.append((const char *)hdr, sizeof hdr);
     for (const BOp &o : ops) key.append((const char *)&o, sizeof o);
     auto it = b->cache.find(key);
@@ -693,16 +897,85 @@ static bool btile_ok(const ht3d_batch *b, int tile, bool fp32) {
 
 static int g_force_btile = -1;   // ht3d_batch_tune_force
 static int g_force_bgrid = -1;   // ht3d_batch_tune_force_grid
+static int g_bpairs = -1;        // ht3d_batch_tune_pairs
+static int g_force_bptile = -1;  // ht3d_batch_tune_force_pair
+
+// The pair tile of a solve kind (-1: none).  Every pair tile that fits all
+// problems is timed with both launch grids on the whole work list, like the
+// single tiles; the fastest is kept when its time per iteration is at least 5 %
+// below `single_us` (the single-iteration launch), or whenever pairs are
+// forced.  The margin: in the solves, where runs of pairs stop at every check
+// and the work list shrinks as problems converge, FP64 pairs timed 2 % faster
+// than single launches on the whole list made the step 3-4 % slower; the FP32
+// pairs, timed far faster, make it 12-24 % faster.
+// Without timing (a pinned single tile, or the autotuner off) pairs are used
+// only when forced: the pinned pair tile, else the first that fits.
+static int autotune_pairs(ht3d_batch *b, int kind, bool transient, bool fp32, const std::vector<ht3d_dyrel_field> &f,
+                          int tile, double single_us, bool timed, int *pt_out, int *pg_out) {
+    *pt_out = -1;
+    *pg_out = g_force_bgrid > 0 ? 1 : 0;
+    if (g_bpairs == 0) return HT3D_OK;
+    std::vector<int> cand;
+    for (int t = 0; t < kNumBPairTiles; ++t) {
+        if (g_force_bptile >= 0 && t != g_force_bptile) continue;
+        bool fits = true;
+        for (const ht3d_hrl *m : b->m) fits = fits && pair_tile_fits(t, m->nx);
+        if (fits) cand.push_back(t);
+    }
+    if (cand.empty()) return HT3D_OK;
+    if (!timed) {
+        if (g_bpairs == 1) *pt_out = cand[0];
+        return HT3D_OK;
+    }
+    hipStream_t s = ctx().stream;
+    std::vector<char> all(b->nprob, 1);
+    std::vector<double> tbest(2 * cand.size(), 1e300);
+    for (int round = 0; round < 3; ++round)
+        for (size_t cg = 0; cg < 2 * cand.size(); ++cg) {
+            const int gm = (int)(cg & 1);
+            if (g_force_bgrid >= 0 && gm != g_force_bgrid) continue;
+            b->ptile_cur = cand[cg >> 1];
+            b->pgmode_cur = gm;
+            fill_tables(b, kind, f, tile, tile, b->ptile_cur, b->ptile_cur);
+            int rc;
+            if ((rc = upload_tables(b, kind)) || (rc = upload_items(b, tile, all, nullptr))) return rc;
+            HIP_TRY(hipStreamSynchronize(s));  // the staging is reused by the next candidate
+            const std::vector<BOp> ops(1, BOp{fp32 ? OP_PAIR32 : OP_PAIR64, 0u, 0, 0});
+            if ((rc = enqueue_ops(b, ops, kind, transient, tile, ht3d_dyrel_params{}))) return rc;
+            HIP_TRY(hipEventRecord(b->ev0, s));
+            for (int r = 0; r < 4; ++r)
+                if ((rc = enqueue_ops(b, ops, kind, transient, tile, ht3d_dyrel_params{}))) return rc;
+            HIP_TRY(hipEventRecord(b->ev1, s));
+            HIP_TRY(hipEventSynchronize(b->ev1));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            if (ms < tbest[cg]) tbest[cg] = ms;
+        }
+    size_t bc = 0;
+    for (size_t cg = 1; cg < tbest.size(); ++cg)
+        if (tbest[cg] < tbest[bc]) bc = cg;
+    b->ptile_cur = -1;
+    const double pair_us = 1e3 * tbest[bc] / 8;   // per iteration
+    if (g_bpairs == 1 || pair_us < 0.95 * single_us) {
+        *pt_out = cand[bc >> 1];
+        *pg_out = (int)(bc & 1);
+        b->t_it[kind][fp32 ? 1 : 0] = pair_us;
+    }
+    return HT3D_OK;
+}
 
 // time `reps` plain batched sweeps of every tile and keep the fastest
 static int autotune_batch(ht3d_batch *b, int kind, bool transient, bool fp32, const std::vector<ht3d_dyrel_field> &f,
                           int *tile_out, int *gmode_out) {
     *gmode_out = g_force_bgrid > 0 ? 1 : 0;
+    const int pk = fp32 ? 1 : 0;
     if (g_force_btile >= 0 || !g_autotune) {   // a pinned tile the batch cannot run falls back to the default
         *tile_out = (g_force_btile >= 0 && btile_ok(b, g_force_btile, fp32)) ? g_force_btile : kBDefaultTile;
-        return HT3D_OK;
+        return autotune_pairs(b, kind, transient, fp32, f, *tile_out, 0.0, false, &b->ptile[kind][pk],
+                              &b->pgmode[kind][pk]);
     }
     hipStream_t s = ctx().stream;
+    b->ptile_cur = -1;   // the single tiles are timed alone
     if (fp32) {   // the timed anchor below adds e (the phase-0 buffer, which the
                   // phase-0 sweeps only read) to the problems' unknowns: it must
                   // hold zeros, and the sweeps should run on defined data
@@ -786,7 +1059,8 @@ static int autotune_batch(ht3d_batch *b, int kind, bool transient, bool fp32, co
         b->t_it[kind][fp32 ? 1 : 0] = t_sweep;
         b->t_chk[kind][fp32 ? 1 : 0] = fp32 ? fmax(chk - (plain - t_sweep), 1.0) : fmax(chk - plain, 1.0);
     }
-    return HT3D_OK;
+    return autotune_pairs(b, kind, transient, fp32, f, best_tile, 1e3 * best / 4, true, &b->ptile[kind][pk],
+                          &b->pgmode[kind][pk]);
 }
 
 // Poll loop shared by both precisions: `next` appends the next block's
@@ -854,11 +1128,13 @@ static int batch_solve(ht3d_batch *b, int kind, const ht3d_dyrel_params &p_in, c
     int &tile = fp32 ? b->tile32[kind] : b->tile64[kind];
     if (tile < 0 && (rc = autotune_batch(b, kind, transient, fp32, f, &tile, &b->gmode[kind][fp32 ? 1 : 0]))) return rc;
     b->gmode_cur = b->gmode[kind][fp32 ? 1 : 0];
+    b->ptile_cur = b->ptile[kind][fp32 ? 1 : 0];
+    b->pgmode_cur = b->pgmode[kind][fp32 ? 1 : 0];
     ht3d_dyrel_params p = p_in;
     const int pk = fp32 ? 1 : 0;
     if (p.ncheck < 0)   // the check-interval tuner (the anchor interval in the defect-correction form)
         p.ncheck = tuned_check_interval(b->last_iters[kind][pk], b->t_it[kind][pk], b->t_chk[kind][pk], p.nitout);
-    fill_tables(b, kind, f, tile, tile);
+    fill_tables(b, kind, f, tile, tile, b->ptile_cur, b->ptile_cur);
     if ((rc = upload_tables(b, kind))) return rc;
     std::vector<char> all(b->nprob, 1);
     if ((rc = upload_items(b, tile, all, nullptr))) return rc;
@@ -888,27 +1164,57 @@ static int batch_solve(ht3d_batch *b, int kind, const ht3d_dyrel_params &p_in, c
     hipLaunchKernelGGL(batch_set_kernel, dim3(1), dim3(64), 0, s, b->d_nitems + 1, 0u);  // iteration base
     HIP_TRY(hipGetLastError());
 
+    // A run of `m` plain iterations from `first` (no check, lambda update or
+    // stored F): pairs when the solve has a pair tile, always an even number
+    // of them so that X and F0 are back in the buffers a single launch would
+    // have left them in, then single iterations.  `xph` is the buffer holding
+    // the current iterate.
+    auto rev = [&](int it) { return (g_alternate && (it & 1) == 0) ? FL_REVERSE : 0u; };
+    auto emit_run = [&](std::vector<BOp> &ops, int first, int m, int &xph) {
+        int npair = b->ptile_cur >= 0 ? m / 2 : 0;
+        if (npair & 1) --npair;
+        int it = first, fpar = 0;
+        for (int k = 0; k < npair; ++k, it += 2) {
+            ops.push_back(BOp{fp32 ? OP_PAIR32 : OP_PAIR64, rev(it), xph, fpar});
+            xph ^= 1;
+            fpar ^= 1;
+        }
+        for (; it < first + m; ++it) {
+            ops.push_back(BOp{fp32 ? OP_ITER32 : OP_ITER64, rev(it), xph, 0});
+            xph ^= 1;
+        }
+    };
     if (!fp32) {
         int it0 = 0;
         rc = poll_loop(b, kind, transient, tile, p, [&](std::vector<BOp> &ops) {
             if (it0 >= p.nitmax) return false;
-            int L = 0, it = it0;
-            bool chk = false;
-            while (true) {
-                ++it;
+            auto lam = [&](int it) { return lam_it(it, p.nitout); };
+            auto chk = [&](int it) { return lam(it) || (p.ncheck > 0 && it % p.ncheck == 0); };
+            auto store = [&](int it) { return it < p.nitmax && lam_it(it + 1, p.nitout); };
+            auto plain = [&](int it) { return !chk(it) && !store(it) && it < p.nitmax; };
+            int end = it0;   // the block runs to the next check, or to nitmax
+            do ++end; while (!chk(end) && end < p.nitmax);
+            int xph = it0 & 1, L = 0;
+            for (int it = it0 + 1; it <= end;) {
+                if (plain(it)) {
+                    int m = 1;
+                    while (it + m <= end && plain(it + m)) ++m;
+                    emit_run(ops, it, m, xph);
+                    it += m;
+                    L += m;
+                    continue;
+                }
                 ++L;
-                const bool lam = lam_it(it, p.nitout);
-                chk = lam || (p.ncheck > 0 && it % p.ncheck == 0);
-                const bool store = it < p.nitmax && lam_it(it + 1, p.nitout);
-                const unsigned fl = (chk ? FL_CHECK : 0u) | (lam ? FL_LAMBDA : 0u) | (store ? FL_STORE_F : 0u) |
-                                    ((g_alternate && (it & 1) == 0) ? FL_REVERSE : 0u);
-                ops.push_back(BOp{OP_ITER64, fl, (it - 1) & 1, 0});
-                if (chk) ops.push_back(BOp{OP_CHECK64, 0u, L, lam ? 1 : 0});
-                if (chk || it == p.nitmax) break;
+                const unsigned fl = (chk(it) ? FL_CHECK : 0u) | (lam(it) ? FL_LAMBDA : 0u) |
+                                    (store(it) ? FL_STORE_F : 0u) | rev(it);
+                ops.push_back(BOp{OP_ITER64, fl, xph, 0});
+                xph ^= 1;
+                if (chk(it)) ops.push_back(BOp{OP_CHECK64, 0u, L, lam(it) ? 1 : 0});
+                ++it;
             }
-            if (it == p.nitmax) ops.push_back(BOp{OP_FINAL64, 0u, 0, 0});
+            if (end == p.nitmax) ops.push_back(BOp{OP_FINAL64, 0u, 0, 0});
             ops.push_back(BOp{OP_ADVANCE, 0u, L, 0});
-            it0 = it;
+            it0 = end;
             b->stats.iterations += L;
             return true;
         });
@@ -921,14 +1227,23 @@ static int batch_solve(ht3d_batch *b, int kind, const ht3d_dyrel_params &p_in, c
             const int run = (p.nitmax - i0) < anchor ? (p.nitmax - i0) : anchor;
             ops.push_back(BOp{OP_ANCHOR_RES, 0u, 0, 0});
             ops.push_back(BOp{OP_ANCHOR, 0u, run, 0});
-            for (int k = 0; k < run; ++k) {
-                const int it = i0 + k + 1;
-                const bool lam = lam_it(it, p.nitout);
-                const bool store = it < p.nitmax && lam_it(it + 1, p.nitout);
-                const unsigned fl = (lam ? (FL_CHECK | FL_LAMBDA) : 0u) | (store ? FL_STORE_F : 0u) |
-                                    ((g_alternate && (it & 1) == 0) ? FL_REVERSE : 0u);
-                ops.push_back(BOp{OP_ITER32, fl, (i0 + k) & 1, 0});
-                if (lam) ops.push_back(BOp{OP_LAM32, 0u, 0, it > 1 ? 1 : 0});
+            auto lam = [&](int it) { return lam_it(it, p.nitout); };
+            auto store = [&](int it) { return it < p.nitmax && lam_it(it + 1, p.nitout); };
+            auto plain = [&](int it) { return !lam(it) && !store(it) && it < p.nitmax; };
+            int xph = i0 & 1;
+            for (int it = i0 + 1; it <= i0 + run;) {
+                if (plain(it)) {
+                    int m = 1;
+                    while (it + m <= i0 + run && plain(it + m)) ++m;
+                    emit_run(ops, it, m, xph);
+                    it += m;
+                    continue;
+                }
+                const unsigned fl = (lam(it) ? (FL_CHECK | FL_LAMBDA) : 0u) | (store(it) ? FL_STORE_F : 0u) | rev(it);
+                ops.push_back(BOp{OP_ITER32, fl, xph, 0});
+                xph ^= 1;
+                if (lam(it)) ops.push_back(BOp{OP_LAM32, 0u, 0, it > 1 ? 1 : 0});
+                ++it;
             }
             if (run > 0) ops.push_back(BOp{OP_UPDATE, 0u, (i0 + run) & 1, 0});
             else last = true;
@@ -1031,10 +1346,24 @@ int ht3d_batch_create(ht3d_hrl *const *models, int nprob, ht3d_batch **out) {
         if (n > maxit) maxit = n;
     }
     b->max_items = maxit;
+    size_t maxp = 0;   // the largest pair work list among the pair tiles that fit every problem
+    for (int t = 0; t < kNumBPairTiles; ++t) {
+        size_t n = 0;
+        bool fits = true;
+        for (int q = 0; q < nprob && fits; ++q) {
+            const ht3d_hrl *m = models[q];
+            fits = pair_tile_fits(t, m->nx);
+            if (fits) n += pair_items(m->nx, m->ny, m->nz, t);
+        }
+        if (fits && n > maxp) maxp = n;
+    }
+    b->max_pitems = maxp;
     int rc = HT3D_OK;
     auto A = [&](void **p, size_t bytes) { if (!rc) rc = b_alloc(b, p, bytes); };
     A((void **)&b->xb, b->se * sizeof(double));
     A((void **)&b->f0, b->sc * sizeof(double));
+    A((void **)&b->f0b, b->sc * sizeof(double));
+    A((void **)&b->f032b, b->sc * sizeof(float));
     A((void **)&b->fst, b->sc * sizeof(double));
     A((void **)&b->e0, b->se * sizeof(float));
     A((void **)&b->e1, b->se * sizeof(float));
@@ -1051,6 +1380,7 @@ int ht3d_batch_create(ht3d_hrl *const *models, int nprob, ht3d_batch **out) {
         b->h_dc[k].resize(nprob);
     }
     A((void **)&b->d_items, maxit * sizeof(uint2));
+    A((void **)&b->d_pitems, (maxp ? maxp : 1) * sizeof(uint2));
     A((void **)&b->d_range, nprob * sizeof(uint2));
     A((void **)&b->d_grange, nprob * sizeof(uint2));
     A((void **)&b->d_nitems, 64);
@@ -1065,6 +1395,7 @@ int ht3d_batch_create(ht3d_hrl *const *models, int nprob, ht3d_batch **out) {
     for (int s = 0; s < 2 && e == hipSuccess; ++s) {
         e = hipHostMalloc((void **)&b->h_state[s], nprob * sizeof(BState), hipHostMallocDefault);
         if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_items[s], (maxit ? maxit : 1) * sizeof(uint2), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_pitems[s], (maxp ? maxp : 1) * sizeof(uint2), hipHostMallocDefault);
         if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_range[s], nprob * sizeof(uint2), hipHostMallocDefault);
         if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_nitems[s], 64, hipHostMallocDefault);
         if (e == hipSuccess) std::memset(b->h_nitems[s], 0, 64);
@@ -1096,6 +1427,7 @@ int ht3d_batch_destroy(ht3d_batch *b) {
     for (int s = 0; s < 2; ++s) {
         if (b->h_state[s]) (void)hipHostFree(b->h_state[s]);
         if (b->h_items[s]) (void)hipHostFree(b->h_items[s]);
+        if (b->h_pitems[s]) (void)hipHostFree(b->h_pitems[s]);
         if (b->h_range[s]) (void)hipHostFree(b->h_range[s]);
         if (b->h_nitems[s]) (void)hipHostFree(b->h_nitems[s]);
         if (b->ev[s]) (void)hipEventDestroy(b->ev[s]);
@@ -1205,7 +1537,12 @@ int ht3d_batch_stats_get(const ht3d_batch *b, ht3d_batch_stats *s) {
     s->tile_z[1] = b->tile64[1] >= 0 ? kBTiles[b->tile64[1]].kz : 0;
     s->tile_z[2] = b->tile32[0] >= 0 ? kBTiles[b->tile32[0]].kz : 0;
     s->tile_z[3] = b->tile32[1] >= 0 ? kBTiles[b->tile32[1]].kz : 0;
-    for (int q = 0; q < 4; ++q) s->grid_mode[q] = b->gmode[q & 1][q >> 1];
+    for (int q = 0; q < 4; ++q) {
+        s->grid_mode[q] = b->gmode[q & 1][q >> 1];
+        const int pt = b->ptile[q & 1][q >> 1];
+        s->pair_rows[q] = pt >= 0 ? kBPairTiles[pt].ty : 0;
+        s->pair_z[q] = pt >= 0 ? kBPairTiles[pt].kz : 0;
+    }
     return HT3D_OK;
     HT3D_API_END
 }
@@ -1227,6 +1564,24 @@ int ht3d_batch_tune_force_grid(int mode) {
     return HT3D_OK;
     HT3D_API_END
 }
+
+int ht3d_batch_tune_pairs(int mode) {
+    HT3D_API_BEGIN
+    if (mode < -1 || mode > 1) return arg_fail("ht3d_batch_tune_pairs: mode must be -1, 0 or 1");
+    g_bpairs = mode;
+    return HT3D_OK;
+    HT3D_API_END
+}
+
+int ht3d_batch_tune_force_pair(int tile) {
+    HT3D_API_BEGIN
+    if (tile >= kNumBPairTiles) return arg_fail("ht3d_batch_tune_force_pair: tile index past the end of the table");
+    g_force_bptile = tile < 0 ? -1 : tile;
+    return HT3D_OK;
+    HT3D_API_END
+}
+
+int ht3d_batch_pair_tile_count(void) { return kNumBPairTiles; }
 
 int ht3d_batch_stats_reset(ht3d_batch *b) {
     HT3D_API_BEGIN

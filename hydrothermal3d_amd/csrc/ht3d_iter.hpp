@@ -36,6 +36,7 @@ struct IterArgs {
     double *__restrict__ partials;
     int nx, ny, nz;
     int nbx, nby, nbz;           // tile grid
+    int nband;                   // batch: bands of rows per plane chunk of the pair launches
     T _dx, _dy, _dz, _dt;
     T dx, dy, dz, rc_tr_dt;
     T idx2, idy2, idz2;          // 1/dx^2, 1/dy^2, 1/dz^2 (the FP32 diagonal)

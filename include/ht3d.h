@@ -578,6 +578,8 @@ typedef struct {
     int tile_rows[4], tile_z[4];/* chosen tiles: thermal FP64, Darcy FP64, thermal FP32, Darcy FP32 */
     int tile_cpl[4];            /* their x cells per lane                                 */
     int grid_mode[4];           /* their launch grids: 0 persistent, 1 one block per item */
+    int pair_rows[4], pair_z[4];/* pair tiles (two iterations per launch) in the same order;
+                                   0: the solve kind runs single iterations only          */
 } ht3d_batch_stats;
 
 int ht3d_batch_create(ht3d_hrl *const *models, int nprob, ht3d_batch **out);
@@ -602,6 +604,18 @@ int ht3d_batch_tile_count(void);
  * item.  Pins it for batches tuned afterwards (-1: time both).  Results are
  * the same bit for bit.  No reference counterpart.                           */
 int ht3d_batch_tune_force_grid(int mode);
+/* Two pseudo-iterations per launch for the runs of plain iterations (no check,
+ * lambda update or stored F): bands of whole rows of a problem, 64*rows/ncx
+ * rows each, overlapping by one row, with the intermediate iterate in LDS.  A
+ * pair tile fits a batch when every problem has at least three rows per band.
+ * mode -1 (default): used when the timing of the fastest pair tile beats the
+ * single-iteration launches; 0: never; 1: always, when a tile fits.
+ * ht3d_batch_tune_force_pair(t) pins pair tile t (-1: time them); results are
+ * the same bit for bit either way.  Both act on batches tuned afterwards.  No
+ * reference counterpart.                                                     */
+int ht3d_batch_tune_pairs(int mode);
+int ht3d_batch_tune_force_pair(int tile);
+int ht3d_batch_pair_tile_count(void);
 
 /* Lognormal permeability on vertices (spectral randomisation) plus an optional
  * planar fault; K (3*M) and phi (M) in host memory.  Set-up only.            */

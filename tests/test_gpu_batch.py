@@ -188,3 +188,43 @@ def test_batch_grid_modes_bitwise(ht, precision):
     for a, c in zip(fa, fb):
         for k in a:
             assert np.array_equal(a[k], c[k]), k
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32dc"])
+def test_batch_pairs_bitwise(ht, precision):
+    """Two pseudo-iterations per launch on bands of whole rows (each pair tile
+    pinned in turn, ht3d_batch_tune_force_pair) against single launches only
+    (ht3d_batch_tune_pairs(0)): the pair kernel computes every cell with the
+    single iteration's operations, and runs of pairs come in even numbers, so
+    steps, reports and fields are identical bit for bit."""
+    lib = ht._lib.lib()
+    n = lib.ht3d_batch_pair_tile_count()
+    try:
+        ht._lib.call("ht3d_batch_tune_pairs", 0)
+        ref_reps, ref_out, ref_stats = _run(ht, precision, graphs=True, nsteps=2, sizes=EVEN)
+        assert ref_stats["pair_rows"] == [0, 0, 0, 0]
+        ht._lib.call("ht3d_batch_tune_pairs", 1)
+        seen = set()
+        for t in range(n):
+            ht._lib.call("ht3d_batch_tune_force_pair", t)
+            for graphs in (True, False):
+                reps, out, stats = _run(ht, precision, graphs=graphs, nsteps=2, sizes=EVEN)
+                used = stats["pair_rows"][2 * precision:2 * precision + 2]
+                assert min(used) > 0, (t, stats["pair_rows"])
+                seen.add((tuple(used), tuple(stats["pair_z"][2 * precision:2 * precision + 2])))
+                for stepr, stepo in zip(reps, ref_reps):
+                    for a, c in zip(stepr, stepo):
+                        assert a["thermal"] == c["thermal"] and a["darcy"] == c["darcy"], (t, graphs)
+                        assert a["Nu"] == c["Nu"] and a["dt_next"] == c["dt_next"], (t, graphs)
+                for a, c in zip(out, ref_out):
+                    for k in a:
+                        assert np.array_equal(a[k], c[k]), (t, graphs, k)
+        assert len(seen) == n
+    finally:
+        ht._lib.call("ht3d_batch_tune_pairs", -1)
+        ht._lib.call("ht3d_batch_tune_force_pair", -1)
+    for bad in (n,):
+        with pytest.raises(ht._lib.HT3DError):
+            ht._lib.call("ht3d_batch_tune_force_pair", bad)
+    with pytest.raises(ht._lib.HT3DError):
+        ht._lib.call("ht3d_batch_tune_pairs", 2)
