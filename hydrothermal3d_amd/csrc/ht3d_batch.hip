@@ -283,7 +283,10 @@ __global__ void batch_final_kernel(BState *st, int n, int nitmax) {
 }
 
 // Anchor of the defect-correction form: FP64 residual norm and the control of
-// the FP64 solve; `run` FP32 sweeps follow for the problems that continue.
+// the FP64 solve; `run` FP32 sweeps follow for the problems that continue.  A
+// problem that stops here gets one more update after the solve
+// (dc_final_kernel), as the FP64 check iteration applies its own: it counts
+// as an iteration.
 __global__ __launch_bounds__(1024) void batch_anchor_kernel(BState *__restrict__ st,
                                                             const IterArgs<float> *__restrict__ tab32,
                                                             const uint2 *__restrict__ range,
@@ -303,10 +306,16 @@ __global__ __launch_bounds__(1024) void batch_anchor_kernel(BState *__restrict__
     if (s.outer == 0) s.r.nF_ini = nF;
     s.r.nF_abs = nF;
     s.r.nF_rel = nF / s.r.nF_ini;
-    if (nF != nF) { s.r.status = HT3D_NAN; s.active = 0; return; }
-    if (s.r.nF_rel > 100) { s.r.status = HT3D_DIVERGED; s.active = 0; return; }
-    if (nF < p.tol || s.r.nF_rel < p.tol) { s.r.status = HT3D_OK; s.active = 0; return; }
-    if (s.inner >= p.nitmax) { s.active = 0; return; }
+    bool stop = true;
+    if (nF != nF) s.r.status = HT3D_NAN;
+    else if (s.r.nF_rel > 100) s.r.status = HT3D_DIVERGED;
+    else if (nF < p.tol || s.r.nF_rel < p.tol) s.r.status = HT3D_OK;
+    else stop = s.inner >= p.nitmax - 1;
+    if (stop) {
+        s.r.iters = s.inner + 1;
+        s.active = 0;
+        return;
+    }
     s.outer++;
     s.inner += run;
     s.r.iters = s.inner;
@@ -1229,7 +1238,11 @@ static int batch_solve(ht3d_batch *b, int kind, const ht3d_dyrel_params &p_in, c
         bool last = false;
         rc = poll_loop(b, kind, transient, tile, p, [&](std::vector<BOp> &ops) {
             if (last) return false;
-            const int run = (p.nitmax - i0) < anchor ? (p.nitmax - i0) : anchor;
+            // the first run is one sweep short: the anchors test the iterates
+            // x_{k*anchor - 1}, as the FP64 check iterations do, the last one
+            // at nitmax - 1 at the latest (the single solve's schedule)
+            const int want = (i0 == 0 && anchor > 1) ? anchor - 1 : anchor;
+            const int run = std::max(0, std::min(want, p.nitmax - 1 - i0));
             ops.push_back(BOp{OP_ANCHOR_RES, 0u, 0, 0});
             ops.push_back(BOp{OP_ANCHOR, 0u, run, 0});
             auto lam = [&](int it) { return lam_it(it, p.nitout); };
@@ -1289,6 +1302,8 @@ static int batch_solve(ht3d_batch *b, int kind, const ht3d_dyrel_params &p_in, c
                 HIP_TRY(hipMemcpyAsync(d.X, res, (size_t)(d.ncx + 2) * (d.ncy + 2) * (d.ncz + 2) * sizeof(double),
                                        hipMemcpyDeviceToDevice, s));
         } else {
+            // ghosts and Nu from the tested iterate, then the interior one
+            // update further (the single defect-correction solve's result)
             if ((rc = bc_fill_table(d.X, d.ncx, d.ncy, d.ncz, d.bc_type, d.bc_val))) return rc;
             if (nusum) {
                 if ((rc = launch_nu_partials(nu_src, d.ky, d.ncx, d.ncy, d.ncz, 1.0 / d.dy,
@@ -1296,6 +1311,12 @@ static int batch_solve(ht3d_batch *b, int kind, const ht3d_dyrel_params &p_in, c
                     (rc = finish_partials(b->d_nupart + (size_t)8 * b->nu_off[q], b->nu_nb[q], 0u, b->d_nures + 4 * q)))
                     return rc;
             }
+            // {h1, h2, dtau} of the problem, as the solve left them in its device table
+            const float *hsc = &b->d_it32[kind][q].h1;
+            const size_t N = (size_t)d.ncx * d.ncy * d.ncz;
+            const unsigned nb = (unsigned)std::min<size_t>(div_up(N, 256), 2048);
+            hipLaunchKernelGGL(dc_final_kernel<float>, dim3(nb), dim3(256), 0, s, b->h_it32[kind][q], hsc, d.X);
+            HIP_TRY(hipGetLastError());
         }
         // algorithmic bytes: 8 N words per pseudo-iteration, 6 N FP64 words per anchor
         const double N = (double)d.ncx * d.ncy * d.ncz;

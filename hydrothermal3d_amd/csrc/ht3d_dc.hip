@@ -138,14 +138,17 @@ static int tune_residual(const DcArgs &d, bool tr, const Span *touch, int ntouch
 }
 
 // DYREL scalars of the FP32 sweeps, kept on the device: st = {lmin, h1, h2,
-// lambda updates so far}, hv = {h1, h2} in FP32 for the sweep kernels.
-__global__ void dc_lambda_set(double *__restrict__ st, float *__restrict__ hv, double lmin, double h1, double h2) {
+// lambda updates so far}, hv = {h1, h2, dtau} in FP32 for the sweep kernels
+// (which take dtau from their arguments) and the final update.
+__global__ void dc_lambda_set(double *__restrict__ st, float *__restrict__ hv, double lmin, double h1, double h2,
+                              float dtau) {
     st[0] = lmin;
     st[1] = h1;
     st[2] = h2;
     st[3] = 0.0;
     hv[0] = (float)h1;
     hv[1] = (float)h2;
+    hv[2] = dtau;
 }
 
 // The Rayleigh-quotient update of lambda_min, c, h1, h2
@@ -314,7 +317,7 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     // sweep kernels read them through hv
     double *st = (double *)res + 16;
     float *hv = (float *)((double *)res + 24);
-    hipLaunchKernelGGL(dc_lambda_set, dim3(1), dim3(1), 0, cx.stream, st, hv, lmin, h1, h2);
+    hipLaunchKernelGGL(dc_lambda_set, dim3(1), dim3(1), 0, cx.stream, st, hv, lmin, h1, h2, a.dtau);
     HIP_TRY(hipGetLastError());
     a.hv = hv;
     ap.hv = hv;
@@ -327,6 +330,17 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     // whenever that tuner is in use.
     const bool timed = te && (te->anchor_us <= 0.0 || p->ncheck < 0);
     auto plain = [&](int it) { return !lam_iter(it, p->nitout) && !(it < p->nitmax && lam_iter(it + 1, p->nitout)); };
+    // Sweeps between anchors.  The first run is one sweep short, so the anchors
+    // test the iterates x_{k*anchor - 1}, the ones the FP64 solve's check
+    // iterations k*ncheck evaluate; the last anchor comes at nitmax - 1 at the
+    // latest.  After the last anchor one more update is applied (below), as
+    // the FP64 check iteration applies its own.
+    auto run_len = [&](int done) {
+        const int want = (done == 0 && anchor > 1) ? anchor - 1 : anchor;
+        const int left = p->nitmax - 1 - done;
+        return left < want ? (left > 0 ? left : 0) : want;
+    };
+    int spec = 0;   // sweeps queued after the current anchor, before its norm was read
     for (outer = 0;; ++outer) {
         // anchor: FP64 residual, FP32 right-hand side, FP64 norm
         const bool time_anchor = outer == 1 && timed;
@@ -341,8 +355,8 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
         // F0 (FP32), never X: when the anchor shows convergence the sweep is
         // dropped.  Only plain sweeps (a pair when the loop below would pair
         // them) go in this way, and not while the anchor is being timed.
-        const int run = (p->nitmax - inner_total) < anchor ? (p->nitmax - inner_total) : anchor;
-        int spec = 0;   // sweeps already queued for this run
+        const int run = run_len(inner_total);
+        spec = 0;
         if (run > 0 && !time_anchor && plain(inner_total + 1)) {
             const int it = inner_total + 1;
             timer.before_launch();
@@ -383,7 +397,7 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
         if (std::isnan(nF_abs)) { status = HT3D_NAN; break; }
         if (nF_rel > 100) { status = HT3D_DIVERGED; break; }
         if (nF_abs < p->tol || nF_rel < p->tol) { status = HT3D_OK; break; }
-        if (inner_total >= p->nitmax) break;
+        if (inner_total >= p->nitmax - 1) break;
         // `anchor` FP32 sweeps of the same DYREL iteration (the first may be running)
         inner_total += spec;
         for (int s = spec; s < run; ++s) {
@@ -437,11 +451,25 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
         h2 = stv[2];
         n_lambda = (int)stv[3];
     }
-    // ghosts of the result from its own interior; Nu from the converged state
+    // The result, as the FP64 solve leaves it: ghosts (and Nu) from the tested
+    // iterate x, then the interior one DYREL update further (dc_final_kernel).
+    // A single sweep queued after the last anchor is that update already; a
+    // queued pair read e = 0 and the old F0 without writing them, and is
+    // dropped.
     if ((rc = bc_fill_table(f->X, nx, ny, nz, f->bc_type, f->bc_val))) return rc;
     if (nu_out) {
         if ((rc = nu_sum(f->X, f->ky, nx, ny, nz, 1.0 / f->dy, nu_out))) return rc;
     }
+    if (spec == 1) {
+        hipLaunchKernelGGL(dc_update_kernel, ugrid, dim3(256), 0, cx.stream, f->X, buf[cur], (size_t)NE);
+    } else {
+        if (spec == 2) fc ^= 1;
+        a.F0 = f0buf[fc];
+        const unsigned nb = (unsigned)std::min<size_t>(div_up(N, 256), 2048);
+        hipLaunchKernelGGL(dc_final_kernel<float>, dim3(nb), dim3(256), 0, cx.stream, a, (const float *)hv, f->X);
+    }
+    HIP_TRY(hipGetLastError());
+    ++inner_total;
     if (te) te->last_iters = inner_total;
     if (rep) {
         rep->ncheck = anchor;

@@ -848,6 +848,35 @@ __global__ __launch_bounds__(256) void diag_kernel(IterArgs<T> a, T *__restrict_
     }
 }
 
+// The last update of a defect-correction solve, after the anchor that stopped
+// it.  The FP64 solve stops at a check iteration: its stopping test evaluates
+// the residual of the iterate that iteration starts from, and the iteration
+// still applies its DYREL update, so the solve returns the tested iterate plus
+// one update, with ghosts from the tested iterate (in the reference the BC
+// kernel runs before the update, HydroThermal3D_v18.jl:448-456).  The anchor
+// has just set x = x + e and e = 0, so the FP32 sweep that would follow it
+// reduces to a pointwise step from the anchor's right-hand side R:
+// F = R / D, F0 = h1 F0 + h2 F, x += -dtau F0 -- the sweep kernels' operations
+// on e = 0, bit for bit.  hsc = {h1, h2, dtau} on the device (the solve's
+// current DYREL scalars).  Only interior cells of X are written: its ghosts
+// stay as the caller filled them from the tested iterate.
+template <typename T>
+__global__ __launch_bounds__(256) void dc_final_kernel(IterArgs<T> a, const T *__restrict__ hsc,
+                                                       double *__restrict__ X) {
+    const uint32_t nx = a.nx, ny = a.ny;
+    const size_t N = (size_t)nx * ny * a.nz;
+    const T h1 = hsc[0], h2 = hsc[1], dtau = hsc[2];
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < N; c += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t i = (uint32_t)(c % nx), j = (uint32_t)((c / nx) % ny), k = (uint32_t)(c / ((size_t)nx * ny));
+        const size_t fx = i + (size_t)j * (nx + 1) + (size_t)k * (nx + 1) * ny;
+        const size_t fy = i + (size_t)j * nx + (size_t)k * nx * (ny + 1);
+        const T D = cell_diag(a, a.kx[fx], a.kx[fx + 1], a.ky[fy], a.ky[fy + nx], a.kz[c], a.kz[c + (size_t)nx * ny]);
+        const T f0 = momentum<T>(h1, a.F0[c], h2, precond<T>(a.R[c], D));
+        a.F0[c] = f0;
+        X[(i + 1) + (size_t)(j + 1) * (nx + 2) + (size_t)(k + 1) * (nx + 2) * (ny + 2)] += (double)step_x<T>((T)0, dtau, f0);
+    }
+}
+
 // ---------------------------------------------------------------- autotuner
 struct TuneKey {
     int nx, ny, nz, transient, prec;
