@@ -200,7 +200,8 @@ def run_batch(n: int, steps: int, warmup: int, prm, graphs: bool = True,
                               "tiles": [f"{r}x{z}" + (f" x{c}" if c > 1 else "")
                                         for r, z, c in zip(st["tile_rows"], st["tile_z"], st["tile_cpl"])],
                               "grid": ["per-item" if g else "persistent" for g in st["grid_mode"]],
-                              "pairs": [f"{r}x{z}" if r else None for r, z in zip(st["pair_rows"], st["pair_z"])]})
+                              "pairs": [f"{r}x{z}" + (f" x{c}" if c > 1 else "") if r else None
+                                        for r, z, c in zip(st["pair_rows"], st["pair_z"], st["pair_cpl"])]})
             b.close()
         states[mode] = _batch_state(ms, reps)
         for m in ms:
@@ -442,8 +443,8 @@ def main(argv=None) -> int:
                     help="physical steps of the 32^3 configuration run in full on CPU and GPU (0: skip)")
     ap.add_argument("--batch", type=int, default=96, help="problems of the batched leg (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=2)
-    ap.add_argument("--batch-tol", type=float, default=1e-9,
-                    help="DYREL tolerance of the batched leg (1e-9 keeps the solver error of both forms below 1e-6)")
+    ap.add_argument("--batch-tol", type=float, default=1e-8,
+                    help="DYREL tolerance of the batched leg (the configurations' 1e-8 by default)")
     ap.add_argument("--batch-warmup", type=int, default=1)
     ap.add_argument("--batch-graphs", type=int, choices=[0, 1], default=1,
                     help="replay the batch's pseudo-iteration blocks as HIP graphs (1) or launch them kernel by "

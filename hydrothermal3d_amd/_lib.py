@@ -71,7 +71,8 @@ class BatchStats(C.Structure):
                 ("steps", C.c_longlong), ("blocks", C.c_longlong), ("graph_launches", C.c_longlong),
                 ("graphs", C.c_longlong), ("repacks", C.c_longlong), ("iterations", C.c_longlong),
                 ("tile_rows", C.c_int * 4), ("tile_z", C.c_int * 4), ("tile_cpl", C.c_int * 4),
-                ("grid_mode", C.c_int * 4), ("pair_rows", C.c_int * 4), ("pair_z", C.c_int * 4)]
+                ("grid_mode", C.c_int * 4), ("pair_rows", C.c_int * 4), ("pair_z", C.c_int * 4),
+                ("pair_cpl", C.c_int * 4)]
 
     def as_dict(self) -> dict:
         d = {}

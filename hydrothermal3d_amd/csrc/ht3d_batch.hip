@@ -27,6 +27,7 @@
 // No reference counterpart: the reference steps one problem per process.
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <new>
@@ -356,17 +357,23 @@ __global__ __launch_bounds__(256) void batch_zero_kernel(uint4 *__restrict__ p, 
         p[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
-// Pair tiles of the batch (rows of lanes per block, planes per item), one x-cell per lane.
-#define HT3D_BATCH_PAIR_TILES(X) X(0, 8, 16) X(1, 8, 32) X(2, 16, 8) X(3, 16, 16) X(4, 16, 32)
-#define HT3D_PAIR_CFG(N, TY, KZ) {TY, KZ, 1},
+// Pair tiles of the batch (rows of lanes per block, planes per item, x-cells
+// per lane); the 2-cell tiles need every problem's ncx even.
+#define HT3D_BATCH_PAIR_TILES(X)                                                                                     \
+    X(0, 8, 16, 1) X(1, 8, 32, 1) X(2, 16, 8, 1) X(3, 16, 16, 1) X(4, 16, 32, 1) X(5, 8, 16, 2) X(6, 8, 32, 2)       \
+    X(7, 16, 8, 2) X(8, 16, 16, 2) X(9, 16, 32, 2)
+#define HT3D_PAIR_CFG(N, TY, KZ, CPL) {TY, KZ, CPL},
 static constexpr TileCfg kBPairTiles[] = {HT3D_BATCH_PAIR_TILES(HT3D_PAIR_CFG)};
 #undef HT3D_PAIR_CFG
 static constexpr int kNumBPairTiles = sizeof(kBPairTiles) / sizeof(kBPairTiles[0]);
 
-// rows of a band (iteration n+1) and inner rows (iteration n+2) of pair tile
-// `pt` on a grid nx cells wide; the tile fits when every band has an inner row
-static inline int pair_band_rows(int pt, int nx) { return 64 * kBPairTiles[pt].ty / nx; }
-static inline bool pair_tile_fits(int pt, int nx) { return pair_band_rows(pt, nx) >= 3; }
+// rows of a band (iteration n+1; the inner ones run iteration n+2) of pair
+// tile `pt` on a grid nx cells wide: 64*TY lanes of CPL cells.  The tile fits
+// when a lane's cells stay in one row and every band has an inner row.
+static inline int pair_band_rows(int pt, int nx) { return 64 * kBPairTiles[pt].ty * kBPairTiles[pt].cpl / nx; }
+static inline bool pair_tile_fits(int pt, int nx) {
+    return nx % kBPairTiles[pt].cpl == 0 && pair_band_rows(pt, nx) >= 3;
+}
 static inline unsigned pair_items(int nx, int ny, int nz, int pt) {
     return div_up(ny, pair_band_rows(pt, nx) - 2) * div_up(nz, kBPairTiles[pt].kz);
 }
@@ -497,6 +504,166 @@ __global__ __launch_bounds__(64 * TY) void batch_pair_kernel(const IterArgs<T> *
     }
 }
 
+// batch_pair_kernel with CPL (2 or 4) x-cells per lane.  A band row holds
+// gx = nx / CPL lanes, so a band of 64*TY lanes is RB = 64*TY / gx rows: CPL
+// times as many as with one cell per lane, which divides the share of overlap
+// rows that iteration n+1 recomputes (on a 128-wide grid with 1024 lanes, 16
+// rows per band and 14 inner against 8 and 6).  X, the face coefficients, R and
+// F0 arrive as CPL-element vector loads (the pattern of iter_tile_v), a lane's
+// cells go through the LDS plane as one vector, and x neighbours inside a lane
+// stay in registers; a lane's first cell is at x = CPL * (lane mod gx), so a
+// row's first and last lanes meet the walls.  Same operations in the same
+// order as the single iteration: bit-identical X and F0.  Needs nx % CPL == 0.
+template <typename T, bool TR, int TY, int KZ, int CPL>
+__global__ __launch_bounds__(64 * TY) void batch_pair_kernel_v(const IterArgs<T> *__restrict__ tab,
+                                                               const uint2 *__restrict__ items,
+                                                               const unsigned *__restrict__ n_items,
+                                                               const BState *__restrict__ st, int phase, int fpar,
+                                                               long long dF0, unsigned flags) {
+    typedef typename CellVec<T, CPL>::type V;
+    constexpr int NT = 64 * TY;
+    __shared__ T L[3][NT * CPL];
+    const unsigned n = *n_items;
+    const int tid = (int)threadIdx.x;
+    const int s0 = tid * CPL;   // this lane's slot in an LDS plane
+    for (unsigned w = blockIdx.x; w < n; w += gridDim.x) {
+        const unsigned t = xcd_tile(w, n, flags & FL_REVERSE);
+        const uint2 it = items[t];
+        if (!st[it.x].active) continue;   // the whole block skips the item
+        const IterArgs<T> &a = tab[it.x];
+        const int nx = a.nx, ny = a.ny, nz = a.nz;
+        const int gx = nx / CPL, RB = NT / gx, RI = RB - 2;
+        const int bb = (int)(it.y % (unsigned)a.nband), bz = (int)(it.y / (unsigned)a.nband);
+        const int r = tid / gx, i = (tid - r * gx) * CPL;
+        const int j = bb * RI - 1 + r;
+        const bool in_dom = r < RB && j >= 0 && j < ny;
+        const bool inner = in_dom && r >= 1 && r <= RB - 2;   // computes iteration n+2
+        const int jc = min(max(j, 0), ny - 1);                // every load in bounds
+        // global-typed pointers and vectors (see batch_pair_kernel)
+        typedef __attribute__((address_space(1))) T GT;
+        typedef __attribute__((address_space(1))) const T GCT;
+        typedef __attribute__((address_space(1))) V GV;
+        typedef __attribute__((address_space(1))) const V GCV;
+        GCT *__restrict__ Xin = (GCT *)(phase ? a.Xout : a.Xin);
+        GT *__restrict__ Xout = (GT *)(phase ? const_cast<T *>(a.Xin) : a.Xout);
+        GCT *__restrict__ F0in = (GCT *)(fpar ? a.F0 + dF0 : a.F0);
+        GT *__restrict__ F0out = (GT *)(fpar ? a.F0 : a.F0 + dF0);
+        GCT *__restrict__ kxg = (GCT *)a.kx, *__restrict__ kyg = (GCT *)a.ky, *__restrict__ kzg = (GCT *)a.kz;
+        GCT *__restrict__ Rg = (GCT *)a.R;
+        auto ld = [](GCT *p) { return *reinterpret_cast<GCV *>(p); };
+        auto ld_nt = [](GCT *p) { return __builtin_nontemporal_load(reinterpret_cast<GCV *>(p)); };
+        const int k0 = bz * KZ, k1 = min(k0 + KZ, nz);
+        const uint32_t sE1 = nx + 2, sE2 = sE1 * (ny + 2);
+        const uint32_t sC2 = (uint32_t)nx * ny, sX1 = nx + 1, sX2 = sX1 * ny, sY2 = (uint32_t)nx * (ny + 1);
+        const bool wlo = i == 0, whi = i + CPL == nx, slo = jc == 0, shi = jc == ny - 1;
+        const int ps = max(k0 - 1, 0), pe = min(k1, nz - 1);   // planes of iteration n+1
+        const T h1 = a.hv ? a.hv[0] : a.h1, h2 = a.hv ? a.hv[1] : a.h2;
+        uint32_t e = (uint32_t)(i + 1) + (uint32_t)(jc + 1) * sE1 + (uint32_t)(ps + 1) * sE2;
+        uint32_t c = (uint32_t)i + (uint32_t)jc * nx + (uint32_t)ps * sC2;
+        uint32_t fx = (uint32_t)i + (uint32_t)jc * sX1 + (uint32_t)ps * sX2;
+        uint32_t fy = (uint32_t)i + (uint32_t)jc * nx + (uint32_t)ps * sY2;
+        V x0c = ld(Xin + e);
+        V x0m = ld(Xin + e - sE2);
+        if (ps == 0) {
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) x0m[q] = ghost(a.bct[4], a.bcv[4], (T)x0c[q]);
+        }
+        V kzB = ld(kzg + c);
+        // plane p-1 as iteration n+1 left it: the iterate, its neighbours in z,
+        // and the coefficients, R, D and F0 the n+2 update needs
+        V x1m = {}, x1c = {}, pkE = {}, pkS = {}, pkN = {}, pkB = {}, pkF = {}, pr = {}, pD = {}, pf0 = {};
+        T pkW = 0;
+        uint32_t pe_idx = 0, pc = 0;
+        // iteration n+2 on plane p-1 of this lane (yF: the n+1 iterate of plane p)
+        auto second = [&](int p, const V &yF) {
+            const int qb = (p - 1) % 3;
+            T yW = L[qb][s0 - 1], yE = L[qb][s0 + CPL];
+            V yS = *reinterpret_cast<const V *>(&L[qb][s0 - gx * CPL]);
+            V yN = *reinterpret_cast<const V *>(&L[qb][s0 + gx * CPL]);
+            V yB = x1m;
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                if (p - 1 == 0) yB[q] = ghost(a.bct[4], a.bcv[4], (T)x1c[q]);
+                if (slo) yS[q] = ghost(a.bct[2], a.bcv[2], (T)x1c[q]);
+                if (shi) yN[q] = ghost(a.bct[3], a.bcv[3], (T)x1c[q]);
+            }
+            if (wlo) yW = ghost(a.bct[0], a.bcv[0], (T)x1c[0]);
+            if (whi) yE = ghost(a.bct[1], a.bcv[1], (T)x1c[CPL - 1]);
+            V g0, xo;
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                const T wv = q == 0 ? yW : (T)x1c[q - 1], ev = q == CPL - 1 ? yE : (T)x1c[q + 1];
+                const T kw = q == 0 ? pkW : (T)pkE[q - 1];
+                T g = cell_residual<T, TR>(a, x1c[q], wv, ev, yS[q], yN[q], yB[q], yF[q], kw, pkE[q], pkS[q], pkN[q],
+                                           pkB[q], pkF[q], pr[q]);
+                g = precond<T>(g, pD[q]);
+                g0[q] = momentum<T>(h1, pf0[q], h2, g);
+                xo[q] = step_x<T>(x1c[q], a.dtau, g0[q]);
+            }
+            __builtin_nontemporal_store(g0, reinterpret_cast<GV *>(F0out + pc));
+            *reinterpret_cast<GV *>(Xout + pe_idx) = xo;
+        };
+        for (int p = ps; p <= pe + 1; ++p) {
+            if (p <= pe) {   // iteration n+1 on plane p
+                V xp = ld(Xin + e + sE2);
+                T xW = Xin[e - 1], xE = Xin[e + CPL];
+                V xS = ld(Xin + e - sE1), xN = ld(Xin + e + sE1);
+                const T kW = kxg[fx];
+                const V kE = ld(kxg + fx + 1);
+                const V kS = ld(kyg + fy), kN = ld(kyg + fy + nx);
+                const V kF = ld(kzg + c + sC2);
+                const V kB = kzB;
+                const V rr = ld_nt(Rg + c);
+                const V f0o = ld_nt(F0in + c);
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) {
+                    if (p + 1 == nz) xp[q] = ghost(a.bct[5], a.bcv[5], (T)x0c[q]);
+                    if (slo) xS[q] = ghost(a.bct[2], a.bcv[2], (T)x0c[q]);
+                    if (shi) xN[q] = ghost(a.bct[3], a.bcv[3], (T)x0c[q]);
+                }
+                if (wlo) xW = ghost(a.bct[0], a.bcv[0], (T)x0c[0]);
+                if (whi) xE = ghost(a.bct[1], a.bcv[1], (T)x0c[CPL - 1]);
+                V D, f0, x1p;
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) {
+                    const T wv = q == 0 ? xW : (T)x0c[q - 1], ev = q == CPL - 1 ? xE : (T)x0c[q + 1];
+                    const T kw = q == 0 ? kW : (T)kE[q - 1];
+                    T f = cell_residual<T, TR>(a, x0c[q], wv, ev, xS[q], xN[q], x0m[q], xp[q], kw, kE[q], kS[q], kN[q],
+                                               kB[q], kF[q], rr[q]);
+                    D[q] = cell_diag(a, kw, (T)kE[q], (T)kS[q], (T)kN[q], (T)kB[q], (T)kF[q]);
+                    f = precond<T>(f, D[q]);
+                    f0[q] = momentum<T>(h1, f0o[q], h2, f);
+                    x1p[q] = step_x<T>(x0c[q], a.dtau, f0[q]);
+                }
+                *reinterpret_cast<V *>(&L[p % 3][s0]) = x1p;
+                x0m = x0c;
+                x0c = xp;
+                kzB = kF;
+                __syncthreads();
+                if (p - 1 >= k0 && inner) second(p, x1p);
+                x1m = x1c;
+                x1c = x1p;
+                pkW = kW; pkE = kE; pkS = kS; pkN = kN; pkB = kB; pkF = kF; pr = rr; pD = D; pf0 = f0;
+                pe_idx = e;
+                pc = c;
+                e += sE2;
+                c += sC2;
+                fx += sX2;
+                fy += sY2;
+            } else {         // past the top plane: iteration n+2 on plane nz-1 of the top chunk
+                __syncthreads();
+                if (p - 1 >= k0 && p - 1 < k1 && inner) {
+                    V yF;
+#pragma unroll
+                    for (int q = 0; q < CPL; ++q) yF[q] = ghost(a.bct[5], a.bcv[5], (T)x1c[q]);
+                    second(p, yF);
+                }
+            }
+        }
+        __syncthreads();   // L is rewritten by the block's next item
+    }
+}
+
 // ---------------------------------------------------------------- launchers
 struct BLaunch {  // device pointers shared by every launch of a solve phase
     const void *tab;       // IterArgs<T> table of the phase
@@ -531,10 +698,14 @@ template <typename T, bool TR>
 static void launch_pair_b(int pt, const BLaunch &L, int phase, int fpar, long long dF0, unsigned flags) {
     const IterArgs<T> *tab = (const IterArgs<T> *)L.tab;
     hipStream_t s = ctx().stream;
-#define BP_(N, TY, KZ)                                                                                               \
+#define BP_(N, TY, KZ, CPL)                                                                                          \
     case N:                                                                                                          \
-        hipLaunchKernelGGL((batch_pair_kernel<T, TR, TY, KZ>), dim3(L.pgrid), dim3(64 * TY), 0, s, tab, L.pitems,    \
-                           L.n_pitems, L.st, phase, fpar, dF0, flags);                                              \
+        if constexpr (CPL == 1)                                                                                      \
+            hipLaunchKernelGGL((batch_pair_kernel<T, TR, TY, KZ>), dim3(L.pgrid), dim3(64 * TY), 0, s, tab, L.pitems,\
+                               L.n_pitems, L.st, phase, fpar, dF0, flags);                                          \
+        else                                                                                                         \
+            hipLaunchKernelGGL((batch_pair_kernel_v<T, TR, TY, KZ, CPL>), dim3(L.pgrid), dim3(64 * TY), 0, s, tab,   \
+                               L.pitems, L.n_pitems, L.st, phase, fpar, dF0, flags);                                \
         break;
     switch (pt) { HT3D_BATCH_PAIR_TILES(BP_) default: break; }
 #undef BP_
@@ -1335,6 +1506,19 @@ static int batch_solve(ht3d_batch *b, int kind, const ht3d_dyrel_params &p_in, c
 
 extern "C" {
 
+// A kernel trace under rocprofv3 ended in a host-side segmentation fault inside
+// the runtime's frames below hipGraphLaunch of a newly captured block graph,
+// while the same graphs run cleanly without the profiler (and the eager form
+// runs cleanly under it).  So when a rocprofiler tool library is configured
+// (rocprofv3 sets ROCP_TOOL_LIBRARIES), the batch launches its blocks kernel
+// by kernel: the same kernels on the same work lists, bit-identical results.
+// HT3D_BATCH_GRAPHS_UNDER_PROFILER=1 keeps the graphs anyway.
+static bool profiler_tool_configured() {
+    const char *t = std::getenv("ROCP_TOOL_LIBRARIES");
+    const char *force = std::getenv("HT3D_BATCH_GRAPHS_UNDER_PROFILER");
+    return t && t[0] && !(force && force[0] == '1');
+}
+
 int ht3d_batch_create(ht3d_hrl *const *models, int nprob, ht3d_batch **out) {
     HT3D_API_BEGIN
     HT3D_CHECK_INIT();
@@ -1348,6 +1532,7 @@ int ht3d_batch_create(ht3d_hrl *const *models, int nprob, ht3d_batch **out) {
     b->pT = models[0]->pT;
     b->pP = models[0]->pP;
     b->precision = models[0]->precision;
+    b->graphs = !profiler_tool_configured();
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx().device) == hipSuccess && ncu > 0)
         b->ncu = ncu;
@@ -1478,7 +1663,7 @@ int ht3d_batch_set_params(ht3d_batch *b, const ht3d_dyrel_params *pT, const ht3d
     if (pT) b->pT = *pT;
     if (pP) b->pP = *pP;
     b->precision = precision;
-    b->graphs = use_graphs != 0;
+    b->graphs = use_graphs != 0 && !profiler_tool_configured();
     clear_graphs(b);  // the graphs carry the parameters as launch arguments
     return HT3D_OK;
     HT3D_API_END
@@ -1568,6 +1753,7 @@ int ht3d_batch_stats_get(const ht3d_batch *b, ht3d_batch_stats *s) {
         const int pt = b->ptile[q & 1][q >> 1];
         s->pair_rows[q] = pt >= 0 ? kBPairTiles[pt].ty : 0;
         s->pair_z[q] = pt >= 0 ? kBPairTiles[pt].kz : 0;
+        s->pair_cpl[q] = pt >= 0 ? kBPairTiles[pt].cpl : 0;
     }
     return HT3D_OK;
     HT3D_API_END

@@ -565,7 +565,10 @@ int ht3d_hrl_step(ht3d_hrl *m, ht3d_step_report *rep);
  * solves of all problems run as one persistent launch per pseudo-iteration
  * over a packed (problem, tile) work list; the DYREL control runs on the
  * device, so each run of pseudo-iterations between two checks is replayed as
- * one HIP graph (use_graphs = 1) or launched kernel by kernel (0).  The DYREL
+ * one HIP graph (use_graphs = 1) or launched kernel by kernel (0; also
+ * whenever a rocprofiler tool is configured, ROCP_TOOL_LIBRARIES set, unless
+ * HT3D_BATCH_GRAPHS_UNDER_PROFILER=1: a kernel trace of the graph form ended in
+ * a host-side fault inside the runtime's graph launch).  The DYREL
  * parameters and the precision are the batch's (all problems share the check
  * schedule); results equal those of stepping each model alone.              */
 typedef struct ht3d_batch ht3d_batch;
@@ -580,6 +583,7 @@ typedef struct {
     int grid_mode[4];           /* their launch grids: 0 persistent, 1 one block per item */
     int pair_rows[4], pair_z[4];/* pair tiles (two iterations per launch) in the same order;
                                    0: the solve kind runs single iterations only          */
+    int pair_cpl[4];            /* their x cells per lane                                 */
 } ht3d_batch_stats;
 
 int ht3d_batch_create(ht3d_hrl *const *models, int nprob, ht3d_batch **out);

@@ -192,8 +192,9 @@ def test_batch_grid_modes_bitwise(ht, precision):
 
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32dc"])
 def test_batch_pairs_bitwise(ht, precision):
-    """Two pseudo-iterations per launch on bands of whole rows (each pair tile
-    pinned in turn, ht3d_batch_tune_force_pair) against single launches only
+    """Two pseudo-iterations per launch on bands of whole rows, one or two
+    x-cells per lane (each pair tile pinned in turn,
+    ht3d_batch_tune_force_pair) against single launches only
     (ht3d_batch_tune_pairs(0)): the pair kernel computes every cell with the
     single iteration's operations, and runs of pairs come in even numbers, so
     steps, reports and fields are identical bit for bit."""
@@ -211,7 +212,8 @@ def test_batch_pairs_bitwise(ht, precision):
                 reps, out, stats = _run(ht, precision, graphs=graphs, nsteps=2, sizes=EVEN)
                 used = stats["pair_rows"][2 * precision:2 * precision + 2]
                 assert min(used) > 0, (t, stats["pair_rows"])
-                seen.add((tuple(used), tuple(stats["pair_z"][2 * precision:2 * precision + 2])))
+                seen.add((tuple(used), tuple(stats["pair_z"][2 * precision:2 * precision + 2]),
+                          tuple(stats["pair_cpl"][2 * precision:2 * precision + 2])))
                 for stepr, stepo in zip(reps, ref_reps):
                     for a, c in zip(stepr, stepo):
                         assert a["thermal"] == c["thermal"] and a["darcy"] == c["darcy"], (t, graphs)

@@ -25,18 +25,18 @@ SAMPLE = (5, 14, 2, 11, 0, 18, 8, 21, 49, 22)
 FP64_TOL, DC_TOL = 1e-10, 1e-6
 
 
-def _params(ht):
-    return ht.DyrelParams(nitmax=20000, nitout=100, ncheck=25, norm=1, tol=1e-9)
+def _params(ht, tol=1e-9):
+    return ht.DyrelParams(nitmax=20000, nitout=100, ncheck=25, norm=1, tol=tol)
 
 
-def _models(ht, ws, precision):
+def _models(ht, ws, precision, tol=1e-9):
     from hydrothermal3d_amd import inputs
     ms = []
     for w in ws:
         m = ht.HydroThermal3D(w.ncx, w.ncy, w.ncz, w.Lx, w.Ly, w.Lz, w.Ra, w.cfl)
         m["Tc_ex"].set(inputs.initial_temperature(w))
         inputs.permeability_device(w, m["kfv"])
-        m.set_params(_params(ht), _params(ht), 1, precision)
+        m.set_params(_params(ht, tol), _params(ht, tol), 1, precision)
         ms.append(m)
     return ms
 
@@ -103,19 +103,23 @@ def test_batch_sample_matches_oracle(ht, sample, precision):
 
 
 def test_batch_96_problem_workload(ht):
-    """One step of the whole workload, three ways on the same inputs."""
+    """One step of the whole workload, three ways on the same inputs, at the
+    configurations' tolerance 1e-8: each model stepped alone in FP64, the FP64
+    batch and the defect-correction batch (same iteration counts within one
+    check interval)."""
     from hydrothermal3d_amd import inputs
+    tol = 1e-8
     ws = inputs.batch_workloads(N_PROBLEMS, seed=13000, steps=1)
     assert sum(w.ncell for w in ws) == 36784128
-    single = _models(ht, ws, 0)
+    single = _models(ht, ws, 0, tol)
     rs = [m.step() for m in single]
     fs = [_state(m) for m in single]
     for m in single:
         m.close()
     res = {}
     for precision in (0, 1):
-        ms = _models(ht, ws, precision)
-        b = ht.HydroThermal3DBatch(ms, _params(ht), _params(ht), precision=precision, graphs=True)
+        ms = _models(ht, ws, precision, tol)
+        b = ht.HydroThermal3DBatch(ms, _params(ht, tol), _params(ht, tol), precision=precision, graphs=True)
         res[precision] = (b.step(), [_state(m) for m in ms], b.stats())
         b.close()
         for m in ms:
@@ -132,6 +136,8 @@ def test_batch_96_problem_workload(ht):
     rd, fd, _ = res[1]
     for q in range(N_PROBLEMS):
         assert rd[q]["thermal"]["status"] == rd[q]["darcy"]["status"] == 0, q
+        for kind in ("thermal", "darcy"):
+            assert abs(rd[q][kind]["iters"] - rb[q][kind]["iters"]) <= _params(ht).ncheck, (q, kind)
         assert abs(rd[q]["Nu"] - rb[q]["Nu"]) <= DC_TOL * abs(rb[q]["Nu"]), q
         for k in ("T", "P"):
             assert rel_err(fd[q][k], fb[q][k]) <= DC_TOL, (q, k, rel_err(fd[q][k], fb[q][k]))

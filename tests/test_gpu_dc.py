@@ -119,13 +119,14 @@ def _fault_steps(ht, w, p, prec, nsteps):
 def test_768_fault_copy_dc_at_tol_1e8(ht):
     """The 768^3-class faulted configuration (Ra 2000, lognormal k x 100 in a
     60-degree slab, mu = exp(-ln10 T)) on a 128x64x128 copy at its tolerance
-    1e-8, two steps.  P and Nu of the FP32+DC steps agree with the FP64 steps
-    within 1e-6.  T is compared with both forms' distance to the converged
-    solution (FP64 at tol 1e-11): T at the few cells where the fault meets the
-    thermal boundary layer moves with P errors far below the 1e-8 stopping
-    test, so at this tolerance FP64 itself sits up to about 1e-5 from the
-    converged T (measured at full size); the bar is that DC is no further from
-    it than 1e-6 or twice the FP64 distance, whichever is larger."""
+    1e-8, two steps: the FP32+DC steps against the FP64 steps, 1e-6 in T, P
+    and Nu, with the same iteration counts.  Both solves end on the same kind
+    of state (the tested iterate plus one update, ghosts from the tested
+    iterate); T at the cells where the fault meets the bottom wall moves by up
+    to 1e-5 when the ghosts come from the final iterate instead, since the
+    closed wall then sees a pressure gradient across it.  Both forms are also
+    compared with the converged solution (FP64 at tol 1e-11): DC is no further
+    from it than FP64 is, within 1e-6."""
     from hydrothermal3d_amd import inputs
     w = inputs.WORKLOADS["hrl768_fault"].scaled(128, 64, 128)
     T0, P0, N0, r0 = _fault_steps(ht, w, ht.production_params(tol=1e-8), 0, 2)
@@ -134,11 +135,13 @@ def test_768_fault_copy_dc_at_tol_1e8(ht):
     for a, c in zip(r0, r1):
         assert a["thermal"]["status"] == c["thermal"]["status"] == 0
         assert a["darcy"]["status"] == c["darcy"]["status"] == 0
+        assert a["thermal"]["iters"] == c["thermal"]["iters"]
+        assert a["darcy"]["iters"] == c["darcy"]["iters"]
+    assert rel_err(T1, T0) <= 1e-6
     assert rel_err(P1, P0) <= 1e-6
     assert np.max(np.abs(N1 - N0) / np.abs(N0)) <= 1e-6
-    e64, edc = rel_err(T0, Tr), rel_err(T1, Tr)
-    assert edc <= max(1e-6, 2.0 * e64), (edc, e64, rel_err(T1, T0))
-    assert rel_err(P1, Pr) <= max(1e-6, 2.0 * rel_err(P0, Pr))
+    assert rel_err(T1, Tr) <= rel_err(T0, Tr) + 1e-6
+    assert rel_err(P1, Pr) <= rel_err(P0, Pr) + 1e-6
 
 
 @pytest.mark.parametrize("ncheck", [1, 2, 3, 7])
