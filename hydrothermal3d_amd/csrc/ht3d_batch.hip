@@ -1083,11 +1083,12 @@ static int g_force_bptile = -1;  // ht3d_batch_tune_force_pair
 // The pair tile of a solve kind (-1: none).  Every pair tile that fits all
 // problems is timed with both launch grids on the whole work list, like the
 // single tiles; the fastest is kept when its time per iteration is at least
-// 10 % below `single_us` (the single-iteration launch), or whenever pairs are
+// 7 % below `single_us` (the single-iteration launch), or whenever pairs are
 // forced.  The margin: in the solves, where runs of pairs stop at every check
 // and the work list shrinks as problems converge, FP64 pairs timed 2-5 % faster
-// than single launches on the whole list made the step 2-4 % slower; the FP32
-// pairs, timed 20-30 % faster, make it 12-24 % faster.
+// than single launches on the whole list made the step 2-4 % slower, FP64
+// pairs with two x-cells per lane timed 10-13 % faster made it 2 % faster, and
+// the FP32 pairs, timed 20-30 % faster, make it 12-24 % faster.
 // Without timing (a pinned single tile, or the autotuner off) pairs are used
 // only when forced: the pinned pair tile, else the first that fits.
 static int autotune_pairs(ht3d_batch *b, int kind, bool transient, bool fp32, const std::vector<ht3d_dyrel_field> &f,
@@ -1141,7 +1142,7 @@ static int autotune_pairs(ht3d_batch *b, int kind, bool transient, bool fp32, co
         if (tbest[cg] < tbest[bc]) bc = cg;
     b->ptile_cur = -1;
     const double pair_us = 1e3 * tbest[bc] / 8;   // per iteration
-    if (g_bpairs == 1 || pair_us < 0.9 * single_us) {
+    if (g_bpairs == 1 || pair_us < 0.93 * single_us) {
         *pt_out = cand[bc >> 1];
         *pg_out = (int)(bc & 1);
         b->t_it[kind][fp32 ? 1 : 0] = pair_us;
