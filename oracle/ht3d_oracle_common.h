@@ -11,9 +11,27 @@
 
 #define EXPORT __attribute__((visibility("default")))
 
-/* 1-based, column-major element access for an array of leading extents (n1,n2). */
+/* 1-based, column-major element access for an array of leading extents (n1,n2).
+ * The checked build (-DHT3D_ORACLE_BOUNDS, `make sanitize`) aborts on an index
+ * outside 1..n1, 1..n2 or below 1 in the last dimension; AddressSanitizer in the
+ * same build catches accesses past the end of an allocation. */
+#ifdef HT3D_ORACLE_BOUNDS
+#include <stdio.h>
+#include <stdlib.h>
+static inline size_t o_index_checked(size_t n1, size_t n2, long i, long j, long k, const char *file, int line) {
+    if (i < 1 || i > (long)n1 || j < 1 || j > (long)n2 || k < 1) {
+        fprintf(stderr, "%s:%d: index (%ld, %ld, %ld) outside the extents (%zu, %zu, .)\n", file, line, i, j, k, n1,
+                n2);
+        abort();
+    }
+    return ((size_t)(k - 1) * n2 + (size_t)(j - 1)) * n1 + (size_t)(i - 1);
+}
+#define AT(a, n1, n2, i, j, k) \
+    ((a)[o_index_checked((size_t)(n1), (size_t)(n2), (long)(i), (long)(j), (long)(k), __FILE__, __LINE__)])
+#else
 #define AT(a, n1, n2, i, j, k) \
     ((a)[(((size_t)(k) - 1) * (size_t)(n2) + ((size_t)(j) - 1)) * (size_t)(n1) + ((size_t)(i) - 1)])
+#endif
 
 #define Cc(a, i, j, k)  AT(a, nx, ny, i, j, k)
 #define Ee(a, i, j, k)  AT(a, nx + 2, ny + 2, i, j, k)

@@ -303,7 +303,7 @@ static double bc_value(const bc_ctx *c, int i, int j, int k, int depth) {
         case R_ZLO: return bc_value(c, i, j, 2, depth + 1);
         case R_ZHI: return bc_value(c, i, j, c->nz + 1, depth + 1);
         case R_IMM: {
-            const double w = AT(c->wN, nx + 2, 1, i, k, 1);
+            const double w = AT(c->wN, nx + 2, c->nz + 2, i, k, 1);   /* wN: (ncx+2) x (ncz+2) */
             return (c->top - (1.0 - w) * bc_value(c, i, j - 1, k, depth + 1)) / w;
         }
         case R_XLO: return bc_value(c, 2, j, k, depth + 1);

@@ -11,13 +11,16 @@ each forwards to the C restatement in ht3d_oracle.c.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB = HERE / "libht3d_oracle.so"
+# HT3D_ORACLE_LIB selects another build of the same sources (the sanitized
+# one, `make -C oracle sanitize`); default: the optimised build.
+LIB = Path(os.environ["HT3D_ORACLE_LIB"]) if os.environ.get("HT3D_ORACLE_LIB") else HERE / "libht3d_oracle.so"
 
 _OFF = {"c": (0, 0, 0), "E": (2, 2, 2), "X6": (6, 6, 6), "V": (1, 1, 1), "Fx": (1, 0, 0), "Fy": (0, 1, 0),
         "Fz": (0, 0, 1)}
