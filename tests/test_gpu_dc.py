@@ -29,12 +29,17 @@ def test_dc_solve_matches_fp64(ht, grid, transient):
     h = dict(dx=1.0 / grid[0], dy=0.5 / grid[1], dz=1.0 / grid[2], dt=1e-3)
     bc = dict(bc_type=(0, 0, 1, 1, 0, 0), bc_val=(0.0, 0.0, 1.0, 0.0, 0.0, 0.0)) if transient else {}
     prm = ht.production_params(tol=1e-11, nitout=100, ncheck=25)
-    out = {}
+    out, reps = {}, {}
     for prec in (0, 1):
         D = [ht.DeviceArray.from_host(a) for a in (X, R, *k)]
         rep = ht.dyrel_solve(*D, transient=transient, rhoC=float(transient), params=prm, precision=prec, **h, **bc)
         assert rep["status"] == 0, rep
         out[prec] = D[0].get()[1:-1, 1:-1, 1:-1]
+        reps[prec] = rep
+    # the anchors test the iterates the FP64 check iterations test, and the
+    # counts include the final update: equal up to a decision one check apart
+    assert abs(reps[1]["iters"] - reps[0]["iters"]) <= prm.ncheck, (reps[0]["iters"], reps[1]["iters"])
+    assert reps[1]["iters"] % prm.ncheck == 0, reps[1]["iters"]
     a, b = out[1], out[0]
     if not transient:
         a, b = a - a.mean(), b - b.mean()
