@@ -125,6 +125,7 @@ class HydroThermal3D:
             _lib.call("ht3d_hrl_field", self._h, name.encode(), C.byref(ptr), C.byref(cnt))
             shape = tuple(g + d for g, d in zip(self.grid, off[kind]))
             self._fields[name] = DeviceArray(shape, ptr=ptr.value, owner=self)
+        self.history: list[tuple[float, float]] = []   # (time, Nu) after each step
         self.set_params()
 
     def __getitem__(self, name: str) -> DeviceArray:
@@ -157,7 +158,20 @@ class HydroThermal3D:
             _lib.check(st, "ht3d_hrl_step")
         _status_check(rep.thermal.status, "T")
         _status_check(rep.darcy.status, "P")
+        self.history.append((rep.time, rep.Nu))
         return _step_dict(rep)
+
+    def save_nu_history(self, name, nt: int | None = None):
+        """HT3DOutput%05d.h5 (Dimensionless/HydroThermal3D_Alban_v6.jl:500-505):
+        the vectors time_vec and Nu_vec of the reference, entry 1 the initial
+        state (0, 0), entry it+1 the time and Nu after step it (:466-469), zero
+        past the steps taken when the run's step count `nt` is given."""
+        from .h5io import write_nu_history
+        n = max(len(self.history) + 1, (nt + 1) if nt is not None else 0)
+        t, nu = np.zeros(n), np.zeros(n)
+        for q, (a, b) in enumerate(self.history):
+            t[q + 1], nu[q + 1] = a, b
+        return write_nu_history(name, t, nu)
 
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:
@@ -204,9 +218,10 @@ class HydroThermal3DBatch:
         st = _lib.lib().ht3d_batch_step(self._h, reps)
         if st not in (_lib.OK, _lib.NOT_CONVERGED, _lib.NAN, _lib.DIVERGED):
             _lib.check(st, "ht3d_batch_step")
-        for r in reps:
+        for m, r in zip(self.models, reps):
             _status_check(r.thermal.status, "T")
             _status_check(r.darcy.status, "P")
+            m.history.append((r.time, r.Nu))
         return [_step_dict(r) for r in reps]
 
     def stats(self) -> dict:

@@ -8,10 +8,12 @@ data are appended raw (little-endian FP64, each block preceded by its byte
 count as UInt64), a layout every VTK reader accepts.  `read_vtr` reads such a
 file back.
 
-Breakpoints (`Breakpoint%05d`, v18:579-588, restart :384-395) hold Tc_ex,
-Pc_ex, time and dt.  The reference writes HDF5; no HDF5 library is available
-here, so `save_breakpoint` writes the same four datasets to a NumPy `.npz`
-(uncompressed, no pickled objects) and `load_breakpoint` restores them.
+Breakpoints (`Breakpoint%05d.h5`, v18:579-588, restart :384-395) hold Tc_ex,
+Pc_ex, time and dt.  `save_breakpoint` writes them as HDF5 in the layout
+HDF5.jl gives them (hydrothermal3d_amd.h5io), so the reference's restart code
+reads them and a breakpoint of the reference restarts this library's model;
+a name ending in `.npz` writes the same four datasets to a NumPy archive
+instead (uncompressed, no pickled objects).  `load_breakpoint` reads either.
 """
 from __future__ import annotations
 
@@ -98,15 +100,25 @@ def read_vtr(path: str | Path) -> tuple[tuple[np.ndarray, np.ndarray, np.ndarray
 
 
 def save_breakpoint(name: str | Path, Tc_ex: np.ndarray, Pc_ex: np.ndarray, time: float, dt: float) -> Path:
-    """Breakpoint%05d (v18:579-588): Tc_ex, Pc_ex, time, dt."""
-    path = Path(str(name) + ".npz") if not str(name).endswith(".npz") else Path(name)
-    np.savez(path, Tc_ex=np.asfortranarray(Tc_ex), Pc_ex=np.asfortranarray(Pc_ex), time=np.float64(time),
-             dt=np.float64(dt))
-    return path
+    """Breakpoint%05d (v18:579-588): Tc_ex, Pc_ex, time, dt, as `name`.h5
+    (HDF5, the reference's format) or, for a name ending in .npz, a NumPy
+    archive."""
+    if str(name).endswith(".npz"):
+        path = Path(name)
+        np.savez(path, Tc_ex=np.asfortranarray(Tc_ex), Pc_ex=np.asfortranarray(Pc_ex), time=np.float64(time),
+                 dt=np.float64(dt))
+        return path
+    from .h5io import write_datasets
+    path = Path(name) if str(name).endswith(".h5") else Path(str(name) + ".h5")
+    return write_datasets(path, {"Tc_ex": Tc_ex, "Pc_ex": Pc_ex, "time": float(time), "dt": float(dt)})
 
 
 def load_breakpoint(path: str | Path) -> dict:
-    """The datasets of a breakpoint (restart, v18:384-395)."""
-    with np.load(path, allow_pickle=False) as f:
-        return {"Tc_ex": np.asfortranarray(f["Tc_ex"]), "Pc_ex": np.asfortranarray(f["Pc_ex"]),
-                "time": float(f["time"]), "dt": float(f["dt"])}
+    """The datasets of a breakpoint (restart, v18:384-395): an HDF5 file as the
+    reference writes it, or a NumPy archive (.npz)."""
+    if str(path).endswith(".npz"):
+        with np.load(path, allow_pickle=False) as f:
+            return {"Tc_ex": np.asfortranarray(f["Tc_ex"]), "Pc_ex": np.asfortranarray(f["Pc_ex"]),
+                    "time": float(f["time"]), "dt": float(f["dt"])}
+    from .h5io import read_datasets
+    return read_datasets(path, ("Tc_ex", "Pc_ex", "time", "dt"))

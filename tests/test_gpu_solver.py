@@ -256,3 +256,36 @@ def test_precision_workload_128_copy(ht, orc, precision):
     finally:
         g.close()
         o.close()
+
+
+def test_nu_history_file(ht, tmp_path):
+    """HT3DOutput%05d.h5 of Dimensionless/HydroThermal3D_Alban_v6.jl:500-505:
+    time_vec and Nu_vec with entry it+1 from step it (:466-469), zeros past the
+    steps taken; the batch records each model's history too."""
+    pytest.importorskip("hydrothermal3d_amd.h5io")
+    from hydrothermal3d_amd import h5io, inputs
+    try:
+        h5io.lib()
+    except RuntimeError:
+        pytest.skip("no HDF5 C library in this image")
+    w = inputs.WORKLOADS["hrl32"].scaled(16, 12, 10)
+    prm = ht.production_params(tol=1e-9)
+    models = []
+    for _ in range(2):
+        m = ht.HydroThermal3D(w.ncx, w.ncy, w.ncz, w.Lx, w.Ly, w.Lz, w.Ra, w.cfl)
+        m["Tc_ex"].set(inputs.initial_temperature(w))
+        m.set_params(prm, prm)
+        models.append(m)
+    reps = [models[0].step() for _ in range(3)]
+    b = ht.HydroThermal3DBatch([models[1]], prm, prm)
+    breps = [b.step()[0] for _ in range(3)]
+    b.close()
+    for m, rs in zip(models, (reps, breps)):
+        p = m.save_nu_history(tmp_path / f"HT3DOutput{id(m) % 97:05d}.h5", nt=5)
+        d = h5io.read_datasets(p, ("time", "Nu"))
+        assert d["time"].shape == d["Nu"].shape == (6,)
+        assert d["time"][0] == 0.0 and d["Nu"][0] == 0.0
+        assert d["time"][1:4].tolist() == [r["time"] for r in rs]
+        assert d["Nu"][1:4].tolist() == [r["Nu"] for r in rs]
+        assert d["time"][4:].tolist() == [0.0, 0.0] and d["Nu"][4:].tolist() == [0.0, 0.0]
+        m.close()
