@@ -551,10 +551,17 @@ def main(argv=None) -> int:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         log(f"CPU baseline: oracle Darcy DYREL, {args.cpu_iters} iterations at full size, {threads} threads")
         cpu_teff, cpu_s = cpu_baseline(w, args.cpu_iters, kfv)
+        # per physical step: the sample's time per pseudo-iteration times the
+        # pseudo-iterations the timed GPU steps took (same algorithm, same counts)
+        mean_iters = sum(a + b for a, b in iters) / max(len(iters), 1)
         cpu = {"value": round(cpu_teff, 3), "unit": "GB/s", "cores": threads, "kind": "port", "cpu_model": _cpu_model(),
                "sample": f"{w.name}: CPU oracle (unfused reference kernel sequence, FP64) Darcy DYREL solve, "
                          f"{args.cpu_iters} pseudo-iterations at full size incl. one Gershgorin set-up pass, "
-                         f"{cpu_s:.2f} s"}
+                         f"{cpu_s:.2f} s",
+               "est_s_per_step": round(cpu_s / args.cpu_iters * mean_iters, 3),
+               "est_s_per_step_basis": f"sample time per pseudo-iteration x {mean_iters:.0f} pseudo-iterations "
+                                       f"(thermal + Darcy, mean of the timed GPU steps); advection not included",
+               "gpu_fp64_speedup_est": round(cpu_s / args.cpu_iters * mean_iters / (ms_step / 1e3), 1)}
 
     out = {
         "metric": METRIC,
