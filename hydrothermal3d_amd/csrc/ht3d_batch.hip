@@ -358,10 +358,12 @@ __global__ __launch_bounds__(256) void batch_zero_kernel(uint4 *__restrict__ p, 
 }
 
 // Pair tiles of the batch (rows of lanes per block, planes per item, x-cells
-// per lane); the 2-cell tiles need every problem's ncx even.
+// per lane); the 2- and 4-cell tiles need every problem's ncx a multiple of
+// the count, and 4 cells per lane are for the FP32 sweeps only (in FP64 they
+// exhaust the registers).
 #define HT3D_BATCH_PAIR_TILES(X)                                                                                     \
     X(0, 8, 16, 1) X(1, 8, 32, 1) X(2, 16, 8, 1) X(3, 16, 16, 1) X(4, 16, 32, 1) X(5, 8, 16, 2) X(6, 8, 32, 2)       \
-    X(7, 16, 8, 2) X(8, 16, 16, 2) X(9, 16, 32, 2)
+    X(7, 16, 8, 2) X(8, 16, 16, 2) X(9, 16, 32, 2) X(10, 8, 16, 4) X(11, 16, 8, 4) X(12, 16, 16, 4)
 #define HT3D_PAIR_CFG(N, TY, KZ, CPL) {TY, KZ, CPL},
 static constexpr TileCfg kBPairTiles[] = {HT3D_BATCH_PAIR_TILES(HT3D_PAIR_CFG)};
 #undef HT3D_PAIR_CFG
@@ -378,291 +380,8 @@ static inline unsigned pair_items(int nx, int ny, int nz, int pt) {
     return div_up(ny, pair_band_rows(pt, nx) - 2) * div_up(nz, kBPairTiles[pt].kz);
 }
 
-// Two plain pseudo-iterations per launch, on bands of whole rows.  An item is
-// a band of RB = 64*TY/nx rows of one problem (the batch's grids are 32 to 128
-// cells wide), marched over KZ planes.  Iteration n+1 runs on the band's rows
-// [bb*RI - 1, bb*RI + RI + 1), iteration n+2 on the RI = RB - 2 inner rows, so
-// neighbouring bands overlap by one row on each side.  A lane holds cell
-// (tid mod nx, row tid / nx) of the band: its x neighbours are lanes tid -+ 1
-// and its y neighbours lanes tid -+ nx, reached through a triple-buffered LDS
-// plane; the z neighbours go through a register queue, n+1 running one plane
-// ahead of n+2 (the scheme of dyrel_pair_kernel).  Lanes past the last whole
-// row, and rows outside the grid, compute on clamped (in-bounds) cells and
-// write nothing.  F0 is read from one of the two F0 buffers (fpar) and
-// written to the other, since an overlapping band still needs the old F0; X
-// goes from the `phase` buffer to the other.  Every cell is computed with the
-// single iteration's operations in the same order: two single launches and one
-// pair launch give bit-identical X and F0.
-template <typename T, bool TR, int TY, int KZ>
-__global__ __launch_bounds__(64 * TY) void batch_pair_kernel(const IterArgs<T> *__restrict__ tab,
-                                                             const uint2 *__restrict__ items,
-                                                             const unsigned *__restrict__ n_items,
-                                                             const BState *__restrict__ st, int phase, int fpar,
-                                                             long long dF0, unsigned flags) {
-    constexpr int NT = 64 * TY;
-    __shared__ T L[3][NT];
-    const unsigned n = *n_items;
-    const int tid = (int)threadIdx.x;
-    for (unsigned w = blockIdx.x; w < n; w += gridDim.x) {
-        const unsigned t = xcd_tile(w, n, flags & FL_REVERSE);
-        const uint2 it = items[t];
-        if (!st[it.x].active) continue;   // the whole block skips the item
-        const IterArgs<T> &a = tab[it.x];
-        const int nx = a.nx, ny = a.ny, nz = a.nz;
-        const int RB = NT / nx, RI = RB - 2;
-        const int bb = (int)(it.y % (unsigned)a.nband), bz = (int)(it.y / (unsigned)a.nband);
-        const int r = tid / nx, i = tid - r * nx;
-        const int j = bb * RI - 1 + r;
-        const bool in_dom = r < RB && j >= 0 && j < ny;
-        const bool inner = in_dom && r >= 1 && r <= RB - 2;   // computes iteration n+2
-        const int jc = min(max(j, 0), ny - 1);                // every load in bounds
-        // Pointers read from the table are generic to the compiler, whose flat
-        // accesses count against the LDS wait counter as well: typed as global,
-        // the loads and stores of the march do not hold up the LDS waits.
-        typedef __attribute__((address_space(1))) T GT;
-        typedef __attribute__((address_space(1))) const T GCT;
-        GCT *__restrict__ Xin = (GCT *)(phase ? a.Xout : a.Xin);
-        GT *__restrict__ Xout = (GT *)(phase ? const_cast<T *>(a.Xin) : a.Xout);
-        GCT *__restrict__ F0in = (GCT *)(fpar ? a.F0 + dF0 : a.F0);
-        GT *__restrict__ F0out = (GT *)(fpar ? a.F0 : a.F0 + dF0);
-        GCT *__restrict__ kxg = (GCT *)a.kx, *__restrict__ kyg = (GCT *)a.ky, *__restrict__ kzg = (GCT *)a.kz;
-        GCT *__restrict__ Rg = (GCT *)a.R;
-        const int k0 = bz * KZ, k1 = min(k0 + KZ, nz);
-        const uint32_t sE1 = nx + 2, sE2 = sE1 * (ny + 2);
-        const uint32_t sC2 = (uint32_t)nx * ny, sX1 = nx + 1, sX2 = sX1 * ny, sY2 = (uint32_t)nx * (ny + 1);
-        const bool wlo = i == 0, whi = i == nx - 1, slo = jc == 0, shi = jc == ny - 1;
-        const int ps = max(k0 - 1, 0), pe = min(k1, nz - 1);   // planes of iteration n+1
-        const T h1 = a.hv ? a.hv[0] : a.h1, h2 = a.hv ? a.hv[1] : a.h2;
-        uint32_t e = (uint32_t)(i + 1) + (uint32_t)(jc + 1) * sE1 + (uint32_t)(ps + 1) * sE2;
-        uint32_t c = (uint32_t)i + (uint32_t)jc * nx + (uint32_t)ps * sC2;
-        uint32_t fx = (uint32_t)i + (uint32_t)jc * sX1 + (uint32_t)ps * sX2;
-        uint32_t fy = (uint32_t)i + (uint32_t)jc * nx + (uint32_t)ps * sY2;
-        T x0c = Xin[e];
-        T x0m = Xin[e - sE2];
-        if (ps == 0) x0m = ghost(a.bct[4], a.bcv[4], x0c);
-        T kzB = kzg[c];
-        T x1m = 0, x1c = 0, pkW = 0, pkE = 0, pkS = 0, pkN = 0, pkB = 0, pkF = 0, pr = 0, pD = 1, pf0 = 0;
-        uint32_t pe_idx = 0, pc = 0;
-        // iteration n+2 on plane p-1, from the n+1 plane in L[(p-1) % 3]
-        auto second = [&](int p, T yF) {
-            const int q = (p - 1) % 3;
-            T yW = L[q][tid - 1], yE = L[q][tid + 1], yS = L[q][tid - nx], yN = L[q][tid + nx];
-            T yB = x1m;
-            if (p - 1 == 0) yB = ghost(a.bct[4], a.bcv[4], x1c);
-            if (wlo) yW = ghost(a.bct[0], a.bcv[0], x1c);
-            if (whi) yE = ghost(a.bct[1], a.bcv[1], x1c);
-            if (slo) yS = ghost(a.bct[2], a.bcv[2], x1c);
-            if (shi) yN = ghost(a.bct[3], a.bcv[3], x1c);
-            T g = cell_residual<T, TR>(a, x1c, yW, yE, yS, yN, yB, yF, pkW, pkE, pkS, pkN, pkB, pkF, pr);
-            g = precond<T>(g, pD);
-            const T g0 = momentum<T>(h1, pf0, h2, g);
-            __builtin_nontemporal_store(g0, F0out + pc);
-            Xout[pe_idx] = step_x<T>(x1c, a.dtau, g0);
-        };
-        for (int p = ps; p <= pe + 1; ++p) {
-            if (p <= pe) {   // iteration n+1 on plane p
-                T xp = Xin[e + sE2];
-                T xW = Xin[e - 1], xE = Xin[e + 1];
-                T xS = Xin[e - sE1], xN = Xin[e + sE1];
-                const T kW = kxg[fx], kE = kxg[fx + 1];
-                const T kS = kyg[fy], kN = kyg[fy + nx];
-                const T kF = kzg[c + sC2];
-                const T kB = kzB;
-                const T rr = __builtin_nontemporal_load(Rg + c);
-                const T f0o = __builtin_nontemporal_load(F0in + c);
-                if (p + 1 == nz) xp = ghost(a.bct[5], a.bcv[5], x0c);
-                if (wlo) xW = ghost(a.bct[0], a.bcv[0], x0c);
-                if (whi) xE = ghost(a.bct[1], a.bcv[1], x0c);
-                if (slo) xS = ghost(a.bct[2], a.bcv[2], x0c);
-                if (shi) xN = ghost(a.bct[3], a.bcv[3], x0c);
-                T f = cell_residual<T, TR>(a, x0c, xW, xE, xS, xN, x0m, xp, kW, kE, kS, kN, kB, kF, rr);
-                const T D = cell_diag(a, kW, kE, kS, kN, kB, kF);
-                f = precond<T>(f, D);
-                const T f0 = momentum<T>(h1, f0o, h2, f);
-                const T x1p = step_x<T>(x0c, a.dtau, f0);
-                L[p % 3][tid] = x1p;
-                x0m = x0c;
-                x0c = xp;
-                kzB = kF;
-                __syncthreads();
-                if (p - 1 >= k0 && inner) second(p, x1p);
-                x1m = x1c;
-                x1c = x1p;
-                pkW = kW; pkE = kE; pkS = kS; pkN = kN; pkB = kB; pkF = kF; pr = rr; pD = D; pf0 = f0;
-                pe_idx = e;
-                pc = c;
-                e += sE2;
-                c += sC2;
-                fx += sX2;
-                fy += sY2;
-            } else {         // past the top plane: iteration n+2 on plane nz-1 of the top chunk
-                __syncthreads();
-                if (p - 1 >= k0 && p - 1 < k1 && inner) second(p, ghost(a.bct[5], a.bcv[5], x1c));
-            }
-        }
-        __syncthreads();   // L is rewritten by the block's next item
-    }
-}
-
-// batch_pair_kernel with CPL (2 or 4) x-cells per lane.  A band row holds
-// gx = nx / CPL lanes, so a band of 64*TY lanes is RB = 64*TY / gx rows: CPL
-// times as many as with one cell per lane, which divides the share of overlap
-// rows that iteration n+1 recomputes (on a 128-wide grid with 1024 lanes, 16
-// rows per band and 14 inner against 8 and 6).  X, the face coefficients, R and
-// F0 arrive as CPL-element vector loads (the pattern of iter_tile_v), a lane's
-// cells go through the LDS plane as one vector, and x neighbours inside a lane
-// stay in registers; a lane's first cell is at x = CPL * (lane mod gx), so a
-// row's first and last lanes meet the walls.  Same operations in the same
-// order as the single iteration: bit-identical X and F0.  Needs nx % CPL == 0.
-template <typename T, bool TR, int TY, int KZ, int CPL>
-__global__ __launch_bounds__(64 * TY) void batch_pair_kernel_v(const IterArgs<T> *__restrict__ tab,
-                                                               const uint2 *__restrict__ items,
-                                                               const unsigned *__restrict__ n_items,
-                                                               const BState *__restrict__ st, int phase, int fpar,
-                                                               long long dF0, unsigned flags) {
-    typedef typename CellVec<T, CPL>::type V;
-    constexpr int NT = 64 * TY;
-    __shared__ T L[3][NT * CPL];
-    const unsigned n = *n_items;
-    const int tid = (int)threadIdx.x;
-    const int s0 = tid * CPL;   // this lane's slot in an LDS plane
-    for (unsigned w = blockIdx.x; w < n; w += gridDim.x) {
-        const unsigned t = xcd_tile(w, n, flags & FL_REVERSE);
-        const uint2 it = items[t];
-        if (!st[it.x].active) continue;   // the whole block skips the item
-        const IterArgs<T> &a = tab[it.x];
-        const int nx = a.nx, ny = a.ny, nz = a.nz;
-        const int gx = nx / CPL, RB = NT / gx, RI = RB - 2;
-        const int bb = (int)(it.y % (unsigned)a.nband), bz = (int)(it.y / (unsigned)a.nband);
-        const int r = tid / gx, i = (tid - r * gx) * CPL;
-        const int j = bb * RI - 1 + r;
-        const bool in_dom = r < RB && j >= 0 && j < ny;
-        const bool inner = in_dom && r >= 1 && r <= RB - 2;   // computes iteration n+2
-        const int jc = min(max(j, 0), ny - 1);                // every load in bounds
-        // global-typed pointers and vectors (see batch_pair_kernel)
-        typedef __attribute__((address_space(1))) T GT;
-        typedef __attribute__((address_space(1))) const T GCT;
-        typedef __attribute__((address_space(1))) V GV;
-        typedef __attribute__((address_space(1))) const V GCV;
-        GCT *__restrict__ Xin = (GCT *)(phase ? a.Xout : a.Xin);
-        GT *__restrict__ Xout = (GT *)(phase ? const_cast<T *>(a.Xin) : a.Xout);
-        GCT *__restrict__ F0in = (GCT *)(fpar ? a.F0 + dF0 : a.F0);
-        GT *__restrict__ F0out = (GT *)(fpar ? a.F0 : a.F0 + dF0);
-        GCT *__restrict__ kxg = (GCT *)a.kx, *__restrict__ kyg = (GCT *)a.ky, *__restrict__ kzg = (GCT *)a.kz;
-        GCT *__restrict__ Rg = (GCT *)a.R;
-        auto ld = [](GCT *p) { return *reinterpret_cast<GCV *>(p); };
-        auto ld_nt = [](GCT *p) { return __builtin_nontemporal_load(reinterpret_cast<GCV *>(p)); };
-        const int k0 = bz * KZ, k1 = min(k0 + KZ, nz);
-        const uint32_t sE1 = nx + 2, sE2 = sE1 * (ny + 2);
-        const uint32_t sC2 = (uint32_t)nx * ny, sX1 = nx + 1, sX2 = sX1 * ny, sY2 = (uint32_t)nx * (ny + 1);
-        const bool wlo = i == 0, whi = i + CPL == nx, slo = jc == 0, shi = jc == ny - 1;
-        const int ps = max(k0 - 1, 0), pe = min(k1, nz - 1);   // planes of iteration n+1
-        const T h1 = a.hv ? a.hv[0] : a.h1, h2 = a.hv ? a.hv[1] : a.h2;
-        uint32_t e = (uint32_t)(i + 1) + (uint32_t)(jc + 1) * sE1 + (uint32_t)(ps + 1) * sE2;
-        uint32_t c = (uint32_t)i + (uint32_t)jc * nx + (uint32_t)ps * sC2;
-        uint32_t fx = (uint32_t)i + (uint32_t)jc * sX1 + (uint32_t)ps * sX2;
-        uint32_t fy = (uint32_t)i + (uint32_t)jc * nx + (uint32_t)ps * sY2;
-        V x0c = ld(Xin + e);
-        V x0m = ld(Xin + e - sE2);
-        if (ps == 0) {
-#pragma unroll
-            for (int q = 0; q < CPL; ++q) x0m[q] = ghost(a.bct[4], a.bcv[4], (T)x0c[q]);
-        }
-        V kzB = ld(kzg + c);
-        // plane p-1 as iteration n+1 left it: the iterate, its neighbours in z,
-        // and the coefficients, R, D and F0 the n+2 update needs
-        V x1m = {}, x1c = {}, pkE = {}, pkS = {}, pkN = {}, pkB = {}, pkF = {}, pr = {}, pD = {}, pf0 = {};
-        T pkW = 0;
-        uint32_t pe_idx = 0, pc = 0;
-        // iteration n+2 on plane p-1 of this lane (yF: the n+1 iterate of plane p)
-        auto second = [&](int p, const V &yF) {
-            const int qb = (p - 1) % 3;
-            T yW = L[qb][s0 - 1], yE = L[qb][s0 + CPL];
-            V yS = *reinterpret_cast<const V *>(&L[qb][s0 - gx * CPL]);
-            V yN = *reinterpret_cast<const V *>(&L[qb][s0 + gx * CPL]);
-            V yB = x1m;
-#pragma unroll
-            for (int q = 0; q < CPL; ++q) {
-                if (p - 1 == 0) yB[q] = ghost(a.bct[4], a.bcv[4], (T)x1c[q]);
-                if (slo) yS[q] = ghost(a.bct[2], a.bcv[2], (T)x1c[q]);
-                if (shi) yN[q] = ghost(a.bct[3], a.bcv[3], (T)x1c[q]);
-            }
-            if (wlo) yW = ghost(a.bct[0], a.bcv[0], (T)x1c[0]);
-            if (whi) yE = ghost(a.bct[1], a.bcv[1], (T)x1c[CPL - 1]);
-            V g0, xo;
-#pragma unroll
-            for (int q = 0; q < CPL; ++q) {
-                const T wv = q == 0 ? yW : (T)x1c[q - 1], ev = q == CPL - 1 ? yE : (T)x1c[q + 1];
-                const T kw = q == 0 ? pkW : (T)pkE[q - 1];
-                T g = cell_residual<T, TR>(a, x1c[q], wv, ev, yS[q], yN[q], yB[q], yF[q], kw, pkE[q], pkS[q], pkN[q],
-                                           pkB[q], pkF[q], pr[q]);
-                g = precond<T>(g, pD[q]);
-                g0[q] = momentum<T>(h1, pf0[q], h2, g);
-                xo[q] = step_x<T>(x1c[q], a.dtau, g0[q]);
-            }
-            __builtin_nontemporal_store(g0, reinterpret_cast<GV *>(F0out + pc));
-            *reinterpret_cast<GV *>(Xout + pe_idx) = xo;
-        };
-        for (int p = ps; p <= pe + 1; ++p) {
-            if (p <= pe) {   // iteration n+1 on plane p
-                V xp = ld(Xin + e + sE2);
-                T xW = Xin[e - 1], xE = Xin[e + CPL];
-                V xS = ld(Xin + e - sE1), xN = ld(Xin + e + sE1);
-                const T kW = kxg[fx];
-                const V kE = ld(kxg + fx + 1);
-                const V kS = ld(kyg + fy), kN = ld(kyg + fy + nx);
-                const V kF = ld(kzg + c + sC2);
-                const V kB = kzB;
-                const V rr = ld_nt(Rg + c);
-                const V f0o = ld_nt(F0in + c);
-#pragma unroll
-                for (int q = 0; q < CPL; ++q) {
-                    if (p + 1 == nz) xp[q] = ghost(a.bct[5], a.bcv[5], (T)x0c[q]);
-                    if (slo) xS[q] = ghost(a.bct[2], a.bcv[2], (T)x0c[q]);
-                    if (shi) xN[q] = ghost(a.bct[3], a.bcv[3], (T)x0c[q]);
-                }
-                if (wlo) xW = ghost(a.bct[0], a.bcv[0], (T)x0c[0]);
-                if (whi) xE = ghost(a.bct[1], a.bcv[1], (T)x0c[CPL - 1]);
-                V D, f0, x1p;
-#pragma unroll
-                for (int q = 0; q < CPL; ++q) {
-                    const T wv = q == 0 ? xW : (T)x0c[q - 1], ev = q == CPL - 1 ? xE : (T)x0c[q + 1];
-                    const T kw = q == 0 ? kW : (T)kE[q - 1];
-                    T f = cell_residual<T, TR>(a, x0c[q], wv, ev, xS[q], xN[q], x0m[q], xp[q], kw, kE[q], kS[q], kN[q],
-                                               kB[q], kF[q], rr[q]);
-                    D[q] = cell_diag(a, kw, (T)kE[q], (T)kS[q], (T)kN[q], (T)kB[q], (T)kF[q]);
-                    f = precond<T>(f, D[q]);
-                    f0[q] = momentum<T>(h1, f0o[q], h2, f);
-                    x1p[q] = step_x<T>(x0c[q], a.dtau, f0[q]);
-                }
-                *reinterpret_cast<V *>(&L[p % 3][s0]) = x1p;
-                x0m = x0c;
-                x0c = xp;
-                kzB = kF;
-                __syncthreads();
-                if (p - 1 >= k0 && inner) second(p, x1p);
-                x1m = x1c;
-                x1c = x1p;
-                pkW = kW; pkE = kE; pkS = kS; pkN = kN; pkB = kB; pkF = kF; pr = rr; pD = D; pf0 = f0;
-                pe_idx = e;
-                pc = c;
-                e += sE2;
-                c += sC2;
-                fx += sX2;
-                fy += sY2;
-            } else {         // past the top plane: iteration n+2 on plane nz-1 of the top chunk
-                __syncthreads();
-                if (p - 1 >= k0 && p - 1 < k1 && inner) {
-                    V yF;
-#pragma unroll
-                    for (int q = 0; q < CPL; ++q) yF[q] = ghost(a.bct[5], a.bcv[5], (T)x1c[q]);
-                    second(p, yF);
-                }
-            }
-        }
-        __syncthreads();   // L is rewritten by the block's next item
-    }
-}
+// The band-pair kernels (batch_pair_kernel, batch_pair_kernel_v) live in
+// ht3d_iter.hpp: the single-problem defect-correction solve runs them too.
 
 // ---------------------------------------------------------------- launchers
 struct BLaunch {  // device pointers shared by every launch of a solve phase
@@ -701,11 +420,11 @@ static void launch_pair_b(int pt, const BLaunch &L, int phase, int fpar, long lo
 #define BP_(N, TY, KZ, CPL)                                                                                          \
     case N:                                                                                                          \
         if constexpr (CPL == 1)                                                                                      \
-            hipLaunchKernelGGL((batch_pair_kernel<T, TR, TY, KZ>), dim3(L.pgrid), dim3(64 * TY), 0, s, tab, L.pitems,\
-                               L.n_pitems, L.st, phase, fpar, dF0, flags);                                          \
-        else                                                                                                         \
-            hipLaunchKernelGGL((batch_pair_kernel_v<T, TR, TY, KZ, CPL>), dim3(L.pgrid), dim3(64 * TY), 0, s, tab,   \
+            hipLaunchKernelGGL((batch_pair_kernel<T, TR, TY, KZ, BState>), dim3(L.pgrid), dim3(64 * TY), 0, s, tab,  \
                                L.pitems, L.n_pitems, L.st, phase, fpar, dF0, flags);                                \
+        else if constexpr (CPL < 4 || sizeof(T) == 4)                                                                \
+            hipLaunchKernelGGL((batch_pair_kernel_v<T, TR, TY, KZ, CPL, BState>), dim3(L.pgrid), dim3(64 * TY), 0, s, \
+                               tab, L.pitems, L.n_pitems, L.st, phase, fpar, dF0, flags);                           \
         break;
     switch (pt) { HT3D_BATCH_PAIR_TILES(BP_) default: break; }
 #undef BP_
@@ -1099,6 +818,7 @@ static int autotune_pairs(ht3d_batch *b, int kind, bool transient, bool fp32, co
     std::vector<int> cand;
     for (int t = 0; t < kNumBPairTiles; ++t) {
         if (g_force_bptile >= 0 && t != g_force_bptile) continue;
+        if (kBPairTiles[t].cpl == 4 && !fp32) continue;   // 4 cells per lane: FP32 only
         bool fits = true;
         for (const ht3d_hrl *m : b->m) fits = fits && pair_tile_fits(t, m->nx);
         if (fits) cand.push_back(t);
@@ -1795,6 +1515,16 @@ int ht3d_batch_tune_force_pair(int tile) {
 }
 
 int ht3d_batch_pair_tile_count(void) { return kNumBPairTiles; }
+
+int ht3d_batch_pair_tile_info(int tile, int *rows, int *planes, int *cells_per_lane) {
+    HT3D_API_BEGIN
+    if (tile < 0 || tile >= kNumBPairTiles) return arg_fail("ht3d_batch_pair_tile_info: tile index out of range");
+    if (rows) *rows = kBPairTiles[tile].ty;
+    if (planes) *planes = kBPairTiles[tile].kz;
+    if (cells_per_lane) *cells_per_lane = kBPairTiles[tile].cpl;
+    return HT3D_OK;
+    HT3D_API_END
+}
 
 int ht3d_batch_stats_reset(ht3d_batch *b) {
     HT3D_API_BEGIN

@@ -220,12 +220,14 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
          *res = nullptr;
     if ((rc = workspace(WS_F32_X, NE * sizeof(float), &e0))) return rc;
     if ((rc = workspace(WS_F32_XB, NE * sizeof(float), &e1))) return rc;
-    if ((rc = workspace(WS_F32_F0, N * sizeof(float), &f0))) return rc;
+    // F0 and its twin for the pair launches in one allocation (the band pairs
+    // address the twin as F0 + dF0)
+    const size_t Npad = (N + 63) / 64 * 64;
+    if ((rc = workspace(WS_F32_F0, 2 * Npad * sizeof(float), &f0))) return rc;
     if ((rc = workspace(WS_F32_R, N * sizeof(float), &r32))) return rc;
     if ((rc = workspace(WS_F32_K, (nfx + nfy + nfz) * sizeof(float), &k32))) return rc;
     if ((rc = workspace(WS_DC_RES, N * sizeof(float), &fst))) return rc;
-    void *f0b = nullptr, *d32 = nullptr;
-    if ((rc = workspace(WS_F32_F0B, N * sizeof(float), &f0b))) return rc;
+    void *f0b = (float *)f0 + Npad, *d32 = nullptr;
     if ((rc = workspace(WS_F32_DIAG, N * sizeof(float), &d32))) return rc;
     unsigned max_blocks = div_up(nx, 64) * div_up(ny, 1) * div_up(nz, 2);
     for (int t = 0; t < kNumResTiles; ++t)
@@ -323,6 +325,22 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     ap.hv = hv;
     float *buf[2] = {(float *)e0, (float *)e1};
     float *f0buf[2] = {(float *)f0, (float *)f0b};
+    // pair launches: overlapped 2-D tiles (launch_pair) or bands of whole rows
+    // (launch_band_pair, from a one-entry device table of this solve)
+    BandCtx<float> band;
+    const bool banded = pcfg.tile >= 0 && kPairTiles[pcfg.tile].band;
+    if (banded) {
+        IterArgs<float> bt = a;
+        bt.Xin = buf[0];
+        bt.Xout = buf[1];
+        bt.F0 = f0buf[0];
+        set_pair_tiling(bt, pcfg.tile);
+        if ((rc = band_setup(bt, (long long)Npad, WS_BAND, &band))) return rc;
+    }
+    auto pair_launch = [&](const float *F0in) {
+        return banded ? launch_band_pair(band, pcfg.tile, f->transient != 0, ap.Xin, F0in, ap.flags)
+                      : launch_pair(ap, F0in, f->transient != 0, pcfg);
+    };
     int cur = 0, fc = 0;
     bool rev = false;   // sweep direction of the next launch
     // The anchor cost feeds the check-interval tuner: it is measured (the second
@@ -366,7 +384,7 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
                 ap.F0 = f0buf[fc ^ 1];
                 ap.flags = rev ? FL_REVERSE : 0u;
                 rev = g_alternate && !rev;
-                if ((rc = launch_pair(ap, (const float *)f0buf[fc], f->transient != 0, pcfg))) return rc;
+                if ((rc = pair_launch(f0buf[fc]))) return rc;
                 timer.count += 2;
                 fc ^= 1;
                 spec = 2;
@@ -409,7 +427,7 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
                 ap.flags = rev ? FL_REVERSE : 0u;
                 rev = g_alternate && !rev;
                 timer.before_launch();
-                if ((rc = launch_pair(ap, (const float *)f0buf[fc], f->transient != 0, pcfg))) return rc;
+                if ((rc = pair_launch(f0buf[fc]))) return rc;
                 timer.count += 2;
                 cur ^= 1;
                 fc ^= 1;

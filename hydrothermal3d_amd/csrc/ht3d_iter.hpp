@@ -8,6 +8,9 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "ht3d_internal.hpp"
@@ -661,7 +664,8 @@ __global__ __launch_bounds__(64 * TY) void dyrel_pair_kernel_v(IterArgs<T> a, co
 // autotuner picks one per (grid, operator, precision, D mode).
 struct TileCfg {
     int ty, kz;
-    int cpl = 1;   // cells per lane along x
+    int cpl = 1;    // cells per lane along x
+    int band = 0;   // pair tiles: 1 = bands of whole rows (batch_pair_kernel), 0 = overlapped 2-D tiles
 };
 static constexpr TileCfg kTiles[] = {{2, 8}, {2, 16}, {4, 4}, {4, 8}, {4, 16}, {4, 32}, {8, 4}, {8, 8},
                                       {8, 16}, {4, 2}, {8, 2}, {16, 2}, {16, 4}, {1, 8},
@@ -759,11 +763,15 @@ inline int launch_iter(const IterArgs<T> &a, bool transient, const IterCfg &cfg)
 }
 
 // --------------------------------------------------------- two-iteration launches
-// Pair tiles (rows of the overlapped block, planes per block); variant bits:
-// 1 = stored diagonal, 2 = cached face-coefficient loads.  tile < 0: no pairs.
+// Pair tiles (rows of the overlapped block, planes per block, x-cells per
+// lane); variant bits: 1 = stored diagonal, 2 = cached face-coefficient loads.
+// The last five are bands of whole rows (64*rows lanes of CPL cells, the
+// band-pair kernels of the batch with a one-entry table), FP32 sweeps only,
+// without variants.  tile < 0: no pairs.
 static constexpr TileCfg kPairTiles[] = {{8, 16}, {8, 32}, {16, 16}, {16, 32}, {16, 64},
                                           {8, 16, 2}, {8, 32, 2}, {16, 32, 2},
-                                          {8, 16, 4}, {8, 32, 4}, {16, 32, 4}};
+                                          {8, 16, 4}, {8, 32, 4}, {16, 32, 4},
+                                          {8, 16, 4, 1}, {16, 16, 4, 1}, {16, 8, 4, 1}, {16, 16, 2, 1}, {16, 16, 1, 1}};
 static constexpr int kNumPairTiles = sizeof(kPairTiles) / sizeof(kPairTiles[0]);
 
 struct PairCfg {
@@ -777,12 +785,21 @@ struct PairCfg {
 template <typename T>
 inline bool pair_tile_ok(int tile, int nx) {
     const int cpl = kPairTiles[tile].cpl;
+    if (kPairTiles[tile].band)   // whole rows in a band of at least 3 (one inner row), FP32 only
+        return sizeof(T) == 4 && nx % cpl == 0 && 64 * kPairTiles[tile].ty * cpl / nx >= 3;
     if (sizeof(T) == 8 && (cpl == 4 || (cpl == 2 && kPairTiles[tile].ty > 8))) return false;
     return nx % cpl == 0;
 }
 
 template <typename T>
 inline void set_pair_tiling(IterArgs<T> &a, int tile) {
+    if (kPairTiles[tile].band) {   // bands of rb rows overlapping by one row, chunks of kz planes
+        const int rb = 64 * kPairTiles[tile].ty * kPairTiles[tile].cpl / a.nx;
+        a.nband = (int)div_up(a.ny, rb - 2);
+        a.nbx = a.nby = 1;
+        a.nbz = (int)div_up(a.nz, kPairTiles[tile].kz);
+        return;
+    }
     a.nbx = (int)div_up(a.nx, 62 * kPairTiles[tile].cpl);
     a.nby = (int)div_up(a.ny, kPairTiles[tile].ty - 2);
     a.nbz = (int)div_up(a.nz, kPairTiles[tile].kz);
@@ -808,9 +825,9 @@ inline void launch_pair_tile(int tile, const IterArgs<T> &a, const T *F0in, unsi
         if constexpr (sizeof(T) == 4) {   // pair_tile_ok keeps FP64 off these
             if (tile == 8) PV_(8, 16, 4);
             else if (tile == 9) PV_(8, 32, 4);
-            else PV_(16, 32, 4);
+            else if (tile == 10) PV_(16, 32, 4);
         }
-        break;
+        break;   // band tiles go through launch_band_pair
     }
 #undef PL_
 #undef PV_
@@ -819,6 +836,8 @@ inline void launch_pair_tile(int tile, const IterArgs<T> &a, const T *F0in, unsi
 // a: pair tiling set, a.F0 = the F0 buffer written, F0in = the one read
 template <typename T>
 inline int launch_pair(const IterArgs<T> &a, const T *F0in, bool transient, const PairCfg &pc) {
+    if (pc.tile < 0 || pc.tile >= kNumPairTiles || kPairTiles[pc.tile].band)
+        return arg_fail("launch_pair: not an overlapped 2-D pair tile (band tiles go through launch_band_pair)");
     const unsigned nb = (unsigned)a.nbx * a.nby * a.nbz;
     hipStream_t st = ctx().stream;
     const bool ds = pc.variant & 1, kt = pc.variant & 2;
@@ -877,7 +896,397 @@ __global__ __launch_bounds__(256) void dc_final_kernel(IterArgs<T> a, const T *_
     }
 }
 
+// ------------------------------------------------ band pairs (batch and DC)
+// Per-problem state seen by the band-pair kernels: they skip the items of a
+// problem whose `active` is 0 (the batch's BState, or OneActive for a single
+// problem).
+struct OneActive {
+    int active;
+};
+
+// Two plain pseudo-iterations per launch, on bands of whole rows.  An item
+// (problem p, band-chunk index) of the work list is a band of RB = 64*TY/nx
+// rows of problem p (the batch's grids are 32 to 128 cells wide; the
+// single-problem solve uses a one-entry table), marched over KZ planes.  Iteration n+1 runs on the band's rows
+// [bb*RI - 1, bb*RI + RI + 1), iteration n+2 on the RI = RB - 2 inner rows, so
+// neighbouring bands overlap by one row on each side.  A lane holds cell
+// (tid mod nx, row tid / nx) of the band: its x neighbours are lanes tid -+ 1
+// and its y neighbours lanes tid -+ nx, reached through a triple-buffered LDS
+// plane; the z neighbours go through a register queue, n+1 running one plane
+// ahead of n+2 (the scheme of dyrel_pair_kernel).  Lanes past the last whole
+// row, and rows outside the grid, compute on clamped (in-bounds) cells and
+// write nothing.  F0 is read from one of the two F0 buffers (fpar) and
+// written to the other, since an overlapping band still needs the old F0; X
+// goes from the `phase` buffer to the other.  Every cell is computed with the
+// single iteration's operations in the same order: two single launches and one
+// pair launch give bit-identical X and F0.
+template <typename T, bool TR, int TY, int KZ, typename S>
+__global__ __launch_bounds__(64 * TY) void batch_pair_kernel(const IterArgs<T> *__restrict__ tab,
+                                                             const uint2 *__restrict__ items,
+                                                             const unsigned *__restrict__ n_items,
+                                                             const S *__restrict__ st, int phase, int fpar,
+                                                             long long dF0, unsigned flags) {
+    constexpr int NT = 64 * TY;
+    __shared__ T L[3][NT];
+    const unsigned n = *n_items;
+    const int tid = (int)threadIdx.x;
+    for (unsigned w = blockIdx.x; w < n; w += gridDim.x) {
+        const unsigned t = xcd_tile(w, n, flags & FL_REVERSE);
+        const uint2 it = items[t];
+        if (!st[it.x].active) continue;   // the whole block skips the item
+        const IterArgs<T> &a = tab[it.x];
+        const int nx = a.nx, ny = a.ny, nz = a.nz;
+        const int RB = NT / nx, RI = RB - 2;
+        const int bb = (int)(it.y % (unsigned)a.nband), bz = (int)(it.y / (unsigned)a.nband);
+        const int r = tid / nx, i = tid - r * nx;
+        const int j = bb * RI - 1 + r;
+        const bool in_dom = r < RB && j >= 0 && j < ny;
+        const bool inner = in_dom && r >= 1 && r <= RB - 2;   // computes iteration n+2
+        const int jc = min(max(j, 0), ny - 1);                // every load in bounds
+        // Pointers read from the table are generic to the compiler, whose flat
+        // accesses count against the LDS wait counter as well: typed as global,
+        // the loads and stores of the march do not hold up the LDS waits.
+        typedef __attribute__((address_space(1))) T GT;
+        typedef __attribute__((address_space(1))) const T GCT;
+        GCT *__restrict__ Xin = (GCT *)(phase ? a.Xout : a.Xin);
+        GT *__restrict__ Xout = (GT *)(phase ? const_cast<T *>(a.Xin) : a.Xout);
+        GCT *__restrict__ F0in = (GCT *)(fpar ? a.F0 + dF0 : a.F0);
+        GT *__restrict__ F0out = (GT *)(fpar ? a.F0 : a.F0 + dF0);
+        GCT *__restrict__ kxg = (GCT *)a.kx, *__restrict__ kyg = (GCT *)a.ky, *__restrict__ kzg = (GCT *)a.kz;
+        GCT *__restrict__ Rg = (GCT *)a.R;
+        const int k0 = bz * KZ, k1 = min(k0 + KZ, nz);
+        const uint32_t sE1 = nx + 2, sE2 = sE1 * (ny + 2);
+        const uint32_t sC2 = (uint32_t)nx * ny, sX1 = nx + 1, sX2 = sX1 * ny, sY2 = (uint32_t)nx * (ny + 1);
+        const bool wlo = i == 0, whi = i == nx - 1, slo = jc == 0, shi = jc == ny - 1;
+        const int ps = max(k0 - 1, 0), pe = min(k1, nz - 1);   // planes of iteration n+1
+        const T h1 = a.hv ? a.hv[0] : a.h1, h2 = a.hv ? a.hv[1] : a.h2;
+        uint32_t e = (uint32_t)(i + 1) + (uint32_t)(jc + 1) * sE1 + (uint32_t)(ps + 1) * sE2;
+        uint32_t c = (uint32_t)i + (uint32_t)jc * nx + (uint32_t)ps * sC2;
+        uint32_t fx = (uint32_t)i + (uint32_t)jc * sX1 + (uint32_t)ps * sX2;
+        uint32_t fy = (uint32_t)i + (uint32_t)jc * nx + (uint32_t)ps * sY2;
+        T x0c = Xin[e];
+        T x0m = Xin[e - sE2];
+        if (ps == 0) x0m = ghost(a.bct[4], a.bcv[4], x0c);
+        T kzB = kzg[c];
+        T x1m = 0, x1c = 0, pkW = 0, pkE = 0, pkS = 0, pkN = 0, pkB = 0, pkF = 0, pr = 0, pD = 1, pf0 = 0;
+        uint32_t pe_idx = 0, pc = 0;
+        // iteration n+2 on plane p-1, from the n+1 plane in L[(p-1) % 3]
+        auto second = [&](int p, T yF) {
+            const int q = (p - 1) % 3;
+            T yW = L[q][tid - 1], yE = L[q][tid + 1], yS = L[q][tid - nx], yN = L[q][tid + nx];
+            T yB = x1m;
+            if (p - 1 == 0) yB = ghost(a.bct[4], a.bcv[4], x1c);
+            if (wlo) yW = ghost(a.bct[0], a.bcv[0], x1c);
+            if (whi) yE = ghost(a.bct[1], a.bcv[1], x1c);
+            if (slo) yS = ghost(a.bct[2], a.bcv[2], x1c);
+            if (shi) yN = ghost(a.bct[3], a.bcv[3], x1c);
+            T g = cell_residual<T, TR>(a, x1c, yW, yE, yS, yN, yB, yF, pkW, pkE, pkS, pkN, pkB, pkF, pr);
+            g = precond<T>(g, pD);
+            const T g0 = momentum<T>(h1, pf0, h2, g);
+            __builtin_nontemporal_store(g0, F0out + pc);
+            Xout[pe_idx] = step_x<T>(x1c, a.dtau, g0);
+        };
+        for (int p = ps; p <= pe + 1; ++p) {
+            if (p <= pe) {   // iteration n+1 on plane p
+                T xp = Xin[e + sE2];
+                T xW = Xin[e - 1], xE = Xin[e + 1];
+                T xS = Xin[e - sE1], xN = Xin[e + sE1];
+                const T kW = kxg[fx], kE = kxg[fx + 1];
+                const T kS = kyg[fy], kN = kyg[fy + nx];
+                const T kF = kzg[c + sC2];
+                const T kB = kzB;
+                const T rr = __builtin_nontemporal_load(Rg + c);
+                const T f0o = __builtin_nontemporal_load(F0in + c);
+                if (p + 1 == nz) xp = ghost(a.bct[5], a.bcv[5], x0c);
+                if (wlo) xW = ghost(a.bct[0], a.bcv[0], x0c);
+                if (whi) xE = ghost(a.bct[1], a.bcv[1], x0c);
+                if (slo) xS = ghost(a.bct[2], a.bcv[2], x0c);
+                if (shi) xN = ghost(a.bct[3], a.bcv[3], x0c);
+                T f = cell_residual<T, TR>(a, x0c, xW, xE, xS, xN, x0m, xp, kW, kE, kS, kN, kB, kF, rr);
+                const T D = cell_diag(a, kW, kE, kS, kN, kB, kF);
+                f = precond<T>(f, D);
+                const T f0 = momentum<T>(h1, f0o, h2, f);
+                const T x1p = step_x<T>(x0c, a.dtau, f0);
+                L[p % 3][tid] = x1p;
+                x0m = x0c;
+                x0c = xp;
+                kzB = kF;
+                __syncthreads();
+                if (p - 1 >= k0 && inner) second(p, x1p);
+                x1m = x1c;
+                x1c = x1p;
+                pkW = kW; pkE = kE; pkS = kS; pkN = kN; pkB = kB; pkF = kF; pr = rr; pD = D; pf0 = f0;
+                pe_idx = e;
+                pc = c;
+                e += sE2;
+                c += sC2;
+                fx += sX2;
+                fy += sY2;
+            } else {         // past the top plane: iteration n+2 on plane nz-1 of the top chunk
+                __syncthreads();
+                if (p - 1 >= k0 && p - 1 < k1 && inner) second(p, ghost(a.bct[5], a.bcv[5], x1c));
+            }
+        }
+        __syncthreads();   // L is rewritten by the block's next item
+    }
+}
+
+// batch_pair_kernel with CPL (2 or 4) x-cells per lane.  A band row holds
+// gx = nx / CPL lanes, so a band of 64*TY lanes is RB = 64*TY / gx rows: CPL
+// times as many as with one cell per lane, which divides the share of overlap
+// rows that iteration n+1 recomputes (on a 128-wide grid with 1024 lanes, 16
+// rows per band and 14 inner against 8 and 6).  X, the face coefficients, R and
+// F0 arrive as CPL-element vector loads (the pattern of iter_tile_v), a lane's
+// cells go through the LDS plane as one vector, and x neighbours inside a lane
+// stay in registers; a lane's first cell is at x = CPL * (lane mod gx), so a
+// row's first and last lanes meet the walls.  Same operations in the same
+// order as the single iteration: bit-identical X and F0.  Needs nx % CPL == 0.
+template <typename T, bool TR, int TY, int KZ, int CPL, typename S>
+__global__ __launch_bounds__(64 * TY) void batch_pair_kernel_v(const IterArgs<T> *__restrict__ tab,
+                                                               const uint2 *__restrict__ items,
+                                                               const unsigned *__restrict__ n_items,
+                                                               const S *__restrict__ st, int phase, int fpar,
+                                                               long long dF0, unsigned flags) {
+    typedef typename CellVec<T, CPL>::type V;
+    constexpr int NT = 64 * TY;
+    __shared__ T L[3][NT * CPL];
+    const unsigned n = *n_items;
+    const int tid = (int)threadIdx.x;
+    const int s0 = tid * CPL;   // this lane's slot in an LDS plane
+    for (unsigned w = blockIdx.x; w < n; w += gridDim.x) {
+        const unsigned t = xcd_tile(w, n, flags & FL_REVERSE);
+        const uint2 it = items[t];
+        if (!st[it.x].active) continue;   // the whole block skips the item
+        const IterArgs<T> &a = tab[it.x];
+        const int nx = a.nx, ny = a.ny, nz = a.nz;
+        const int gx = nx / CPL, RB = NT / gx, RI = RB - 2;
+        const int bb = (int)(it.y % (unsigned)a.nband), bz = (int)(it.y / (unsigned)a.nband);
+        const int r = tid / gx, i = (tid - r * gx) * CPL;
+        const int j = bb * RI - 1 + r;
+        const bool in_dom = r < RB && j >= 0 && j < ny;
+        const bool inner = in_dom && r >= 1 && r <= RB - 2;   // computes iteration n+2
+        const int jc = min(max(j, 0), ny - 1);                // every load in bounds
+        // global-typed pointers and vectors (see batch_pair_kernel)
+        typedef __attribute__((address_space(1))) T GT;
+        typedef __attribute__((address_space(1))) const T GCT;
+        typedef __attribute__((address_space(1))) V GV;
+        typedef __attribute__((address_space(1))) const V GCV;
+        GCT *__restrict__ Xin = (GCT *)(phase ? a.Xout : a.Xin);
+        GT *__restrict__ Xout = (GT *)(phase ? const_cast<T *>(a.Xin) : a.Xout);
+        GCT *__restrict__ F0in = (GCT *)(fpar ? a.F0 + dF0 : a.F0);
+        GT *__restrict__ F0out = (GT *)(fpar ? a.F0 : a.F0 + dF0);
+        GCT *__restrict__ kxg = (GCT *)a.kx, *__restrict__ kyg = (GCT *)a.ky, *__restrict__ kzg = (GCT *)a.kz;
+        GCT *__restrict__ Rg = (GCT *)a.R;
+        auto ld = [](GCT *p) { return *reinterpret_cast<GCV *>(p); };
+        auto ld_nt = [](GCT *p) { return __builtin_nontemporal_load(reinterpret_cast<GCV *>(p)); };
+        const int k0 = bz * KZ, k1 = min(k0 + KZ, nz);
+        const uint32_t sE1 = nx + 2, sE2 = sE1 * (ny + 2);
+        const uint32_t sC2 = (uint32_t)nx * ny, sX1 = nx + 1, sX2 = sX1 * ny, sY2 = (uint32_t)nx * (ny + 1);
+        const bool wlo = i == 0, whi = i + CPL == nx, slo = jc == 0, shi = jc == ny - 1;
+        const int ps = max(k0 - 1, 0), pe = min(k1, nz - 1);   // planes of iteration n+1
+        const T h1 = a.hv ? a.hv[0] : a.h1, h2 = a.hv ? a.hv[1] : a.h2;
+        uint32_t e = (uint32_t)(i + 1) + (uint32_t)(jc + 1) * sE1 + (uint32_t)(ps + 1) * sE2;
+        uint32_t c = (uint32_t)i + (uint32_t)jc * nx + (uint32_t)ps * sC2;
+        uint32_t fx = (uint32_t)i + (uint32_t)jc * sX1 + (uint32_t)ps * sX2;
+        uint32_t fy = (uint32_t)i + (uint32_t)jc * nx + (uint32_t)ps * sY2;
+        V x0c = ld(Xin + e);
+        V x0m = ld(Xin + e - sE2);
+        if (ps == 0) {
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) x0m[q] = ghost(a.bct[4], a.bcv[4], (T)x0c[q]);
+        }
+        V kzB = ld(kzg + c);
+        // plane p-1 as iteration n+1 left it: the iterate, its neighbours in z,
+        // and the coefficients, R, D and F0 the n+2 update needs
+        V x1m = {}, x1c = {}, pkE = {}, pkS = {}, pkN = {}, pkB = {}, pkF = {}, pr = {}, pD = {}, pf0 = {};
+        T pkW = 0;
+        uint32_t pe_idx = 0, pc = 0;
+        // iteration n+2 on plane p-1 of this lane (yF: the n+1 iterate of plane p)
+        auto second = [&](int p, const V &yF) {
+            const int qb = (p - 1) % 3;
+            T yW = L[qb][s0 - 1], yE = L[qb][s0 + CPL];
+            V yS = *reinterpret_cast<const V *>(&L[qb][s0 - gx * CPL]);
+            V yN = *reinterpret_cast<const V *>(&L[qb][s0 + gx * CPL]);
+            V yB = x1m;
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                if (p - 1 == 0) yB[q] = ghost(a.bct[4], a.bcv[4], (T)x1c[q]);
+                if (slo) yS[q] = ghost(a.bct[2], a.bcv[2], (T)x1c[q]);
+                if (shi) yN[q] = ghost(a.bct[3], a.bcv[3], (T)x1c[q]);
+            }
+            if (wlo) yW = ghost(a.bct[0], a.bcv[0], (T)x1c[0]);
+            if (whi) yE = ghost(a.bct[1], a.bcv[1], (T)x1c[CPL - 1]);
+            V g0, xo;
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                const T wv = q == 0 ? yW : (T)x1c[q - 1], ev = q == CPL - 1 ? yE : (T)x1c[q + 1];
+                const T kw = q == 0 ? pkW : (T)pkE[q - 1];
+                T g = cell_residual<T, TR>(a, x1c[q], wv, ev, yS[q], yN[q], yB[q], yF[q], kw, pkE[q], pkS[q], pkN[q],
+                                           pkB[q], pkF[q], pr[q]);
+                g = precond<T>(g, pD[q]);
+                g0[q] = momentum<T>(h1, pf0[q], h2, g);
+                xo[q] = step_x<T>(x1c[q], a.dtau, g0[q]);
+            }
+            __builtin_nontemporal_store(g0, reinterpret_cast<GV *>(F0out + pc));
+            *reinterpret_cast<GV *>(Xout + pe_idx) = xo;
+        };
+        for (int p = ps; p <= pe + 1; ++p) {
+            if (p <= pe) {   // iteration n+1 on plane p
+                V xp = ld(Xin + e + sE2);
+                T xW = Xin[e - 1], xE = Xin[e + CPL];
+                V xS = ld(Xin + e - sE1), xN = ld(Xin + e + sE1);
+                const T kW = kxg[fx];
+                const V kE = ld(kxg + fx + 1);
+                const V kS = ld(kyg + fy), kN = ld(kyg + fy + nx);
+                const V kF = ld(kzg + c + sC2);
+                const V kB = kzB;
+                const V rr = ld_nt(Rg + c);
+                const V f0o = ld_nt(F0in + c);
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) {
+                    if (p + 1 == nz) xp[q] = ghost(a.bct[5], a.bcv[5], (T)x0c[q]);
+                    if (slo) xS[q] = ghost(a.bct[2], a.bcv[2], (T)x0c[q]);
+                    if (shi) xN[q] = ghost(a.bct[3], a.bcv[3], (T)x0c[q]);
+                }
+                if (wlo) xW = ghost(a.bct[0], a.bcv[0], (T)x0c[0]);
+                if (whi) xE = ghost(a.bct[1], a.bcv[1], (T)x0c[CPL - 1]);
+                V D, f0, x1p;
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) {
+                    const T wv = q == 0 ? xW : (T)x0c[q - 1], ev = q == CPL - 1 ? xE : (T)x0c[q + 1];
+                    const T kw = q == 0 ? kW : (T)kE[q - 1];
+                    T f = cell_residual<T, TR>(a, x0c[q], wv, ev, xS[q], xN[q], x0m[q], xp[q], kw, kE[q], kS[q], kN[q],
+                                               kB[q], kF[q], rr[q]);
+                    D[q] = cell_diag(a, kw, (T)kE[q], (T)kS[q], (T)kN[q], (T)kB[q], (T)kF[q]);
+                    f = precond<T>(f, D[q]);
+                    f0[q] = momentum<T>(h1, f0o[q], h2, f);
+                    x1p[q] = step_x<T>(x0c[q], a.dtau, f0[q]);
+                }
+                *reinterpret_cast<V *>(&L[p % 3][s0]) = x1p;
+                x0m = x0c;
+                x0c = xp;
+                kzB = kF;
+                __syncthreads();
+                if (p - 1 >= k0 && inner) second(p, x1p);
+                x1m = x1c;
+                x1c = x1p;
+                pkW = kW; pkE = kE; pkS = kS; pkN = kN; pkB = kB; pkF = kF; pr = rr; pD = D; pf0 = f0;
+                pe_idx = e;
+                pc = c;
+                e += sE2;
+                c += sC2;
+                fx += sX2;
+                fy += sY2;
+            } else {         // past the top plane: iteration n+2 on plane nz-1 of the top chunk
+                __syncthreads();
+                if (p - 1 >= k0 && p - 1 < k1 && inner) {
+                    V yF;
+#pragma unroll
+                    for (int q = 0; q < CPL; ++q) yF[q] = ghost(a.bct[5], a.bcv[5], (T)x1c[q]);
+                    second(p, yF);
+                }
+            }
+        }
+        __syncthreads();   // L is rewritten by the block's next item
+    }
+}
+
+// One problem's device-side table, work list and state for the band-pair
+// kernels, in a workspace slot of the solve.  The table's Xin/Xout are the two
+// iterate buffers and its F0 the first of two F0 buffers dF0 elements apart
+// in one allocation; a launch picks the direction with `phase` and `fpar`.
+template <typename T>
+struct BandCtx {
+    const IterArgs<T> *tab = nullptr;
+    const uint2 *items = nullptr;
+    const unsigned *n_items = nullptr;
+    const OneActive *st = nullptr;
+    unsigned n = 0;
+    const T *x0 = nullptr, *f0 = nullptr;
+    long long dF0 = 0;
+};
+
+// Upload the one-entry table `a` (band tiling set by set_pair_tiling), the
+// work list (every band x plane-chunk of the problem) and an active state.
+// Stream-ordered, then waited for: launches still reading the slot's previous
+// contents finish first, and the host staging is free on return.
+template <typename T>
+inline int band_setup(const IterArgs<T> &a, long long dF0, int slot, BandCtx<T> *out) {
+    const unsigned n = (unsigned)a.nband * (unsigned)a.nbz;
+    const size_t o_st = (sizeof(IterArgs<T>) + 255) / 256 * 256, o_n = o_st + 64, o_items = o_st + 256;
+    const size_t bytes = o_items + (size_t)n * sizeof(uint2);
+    void *p = nullptr;
+    if (int rc = workspace(slot, bytes, &p)) return rc;
+    std::vector<char> h(bytes, 0);
+    std::memcpy(h.data(), &a, sizeof(IterArgs<T>));
+    const OneActive act{1};
+    std::memcpy(h.data() + o_st, &act, sizeof act);
+    std::memcpy(h.data() + o_n, &n, sizeof n);
+    uint2 *it = reinterpret_cast<uint2 *>(h.data() + o_items);
+    for (unsigned q = 0; q < n; ++q) it[q] = uint2{0u, q};
+    hipStream_t st = ctx().stream;
+    HIP_TRY(hipMemcpyAsync(p, h.data(), bytes, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    char *d = (char *)p;
+    out->tab = (const IterArgs<T> *)d;
+    out->st = (const OneActive *)(d + o_st);
+    out->n_items = (const unsigned *)(d + o_n);
+    out->items = (const uint2 *)(d + o_items);
+    out->n = n;
+    out->x0 = a.Xin;
+    out->f0 = a.F0;
+    out->dF0 = dF0;
+    return HT3D_OK;
+}
+
+// Two iterations on bands of whole rows (pair tiles 11-15): from Xin (one of
+// the table's two iterate buffers) and F0in (one of its two F0 buffers) into
+// the other two.
+template <typename T>
+inline int launch_band_pair(const BandCtx<T> &b, int tile, bool transient, const T *Xin, const T *F0in,
+                            unsigned flags) {
+    const int phase = Xin == b.x0 ? 0 : 1, fpar = F0in == b.f0 ? 0 : 1;
+    const unsigned fl = flags & FL_REVERSE;
+    hipStream_t st = ctx().stream;
+    if constexpr (sizeof(T) == 4) {   // pair_tile_ok keeps FP64 off the band tiles
+#define BK_(TR, TY, KZ, CPL)                                                                                         \
+    do {                                                                                                             \
+        if constexpr (CPL == 1)                                                                                      \
+            hipLaunchKernelGGL((batch_pair_kernel<T, TR, TY, KZ, OneActive>), dim3(b.n), dim3(64 * TY), 0, st,       \
+                               b.tab, b.items, b.n_items, b.st, phase, fpar, b.dF0, fl);                            \
+        else                                                                                                         \
+            hipLaunchKernelGGL((batch_pair_kernel_v<T, TR, TY, KZ, CPL, OneActive>), dim3(b.n), dim3(64 * TY), 0,    \
+                               st, b.tab, b.items, b.n_items, b.st, phase, fpar, b.dF0, fl);                        \
+    } while (0)
+#define BT_(TY, KZ, CPL) do { if (transient) BK_(true, TY, KZ, CPL); else BK_(false, TY, KZ, CPL); } while (0)
+        switch (tile) {
+        case 11: BT_(8, 16, 4); break;
+        case 12: BT_(16, 16, 4); break;
+        case 13: BT_(16, 8, 4); break;
+        case 14: BT_(16, 16, 2); break;
+        case 15: BT_(16, 16, 1); break;
+        default: return arg_fail("launch_band_pair: not a band tile");
+        }
+#undef BT_
+#undef BK_
+    } else {
+        return arg_fail("launch_band_pair: band tiles are for the FP32 sweeps");
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "batch_pair_kernel (band pairs)");
+    return HT3D_OK;
+}
+
 // ---------------------------------------------------------------- autotuner
+// HT3D_TUNE_VERBOSE=1: the tuner prints each candidate's time to stderr
+inline bool tune_verbose() {
+    static const bool v = [] {
+        const char *e = std::getenv("HT3D_TUNE_VERBOSE");
+        return e && e[0] == '1';
+    }();
+    return v;
+}
+
 struct TuneKey {
     int nx, ny, nz, transient, prec;
     bool operator==(const TuneKey &o) const {
@@ -1033,6 +1442,9 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
     reps = 8;
     for (size_t q = 0; q < nfin; ++q)
         if (fin[q] < te.us) { te.us = fin[q]; te.cfg = cands[q].cfg; }
+    if (tune_verbose())
+        fprintf(stderr, "[ht3d tune] %dx%dx%d %s prec %d: single tile %d variant %d %.2f us\n", a.nx, a.ny, a.nz,
+                transient ? "heat" : "darcy", prec, te.cfg.tile, te.cfg.variant, te.us);
     if (a.partials) {   // cost of a check: check launch + finish kernel + read-back, less a plain launch
         void *res = nullptr;
         int rc = workspace(WS_RESULT, 64 * sizeof(double), &res);
@@ -1058,7 +1470,7 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
             if ((var & 1) && !can_store_d) continue;
             if (small && var < 2) continue;
             for (int tile = 0; tile < kNumPairTiles; ++tile) {
-                if (!pair_tile_ok<T>(tile, a.nx)) continue;
+                if (kPairTiles[tile].band || !pair_tile_ok<T>(tile, a.nx)) continue;   // bands: timed below
                 if (small && tile != 0 && tile != 5) continue;
                 PairCfg pc{tile, var};
                 IterArgs<T> b = a;
@@ -1072,7 +1484,38 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
                 }, &us);
                 if (rc) return rc;
                 us *= 0.5;   // per iteration
+                if (tune_verbose())
+                    fprintf(stderr, "[ht3d tune] %dx%dx%d %s prec %d: pair tile %d variant %d %.2f us per iteration\n",
+                            a.nx, a.ny, a.nz, transient ? "heat" : "darcy", prec, tile, var, us);
                 if (us < te.pair_us) { te.pair_us = us; te.pair = pc; }
+            }
+        }
+        // band tiles (FP32): F0b must follow a.F0 in the same allocation
+        if (sizeof(T) == 4 && F0b > a.F0 && !small) {
+            for (int tile = 0; tile < kNumPairTiles; ++tile) {
+                if (!kPairTiles[tile].band || !pair_tile_ok<T>(tile, a.nx)) continue;
+                IterArgs<T> b = a;
+                b.Xin = xs[0];
+                b.Xout = xs[1];
+                set_pair_tiling(b, tile);
+                BandCtx<T> bc;
+                if (int rc = band_setup(b, (long long)(F0b - a.F0), WS_BAND, &bc)) return rc;
+                int fp = 0;
+                unsigned fl = 0;
+                double us = 0;
+                int rc = timed([&] {
+                    if (g_alternate) fl ^= FL_REVERSE;
+                    const T *xin = xs[pp], *f0in = fp ? F0b : a.F0;
+                    pp ^= 1;
+                    fp ^= 1;
+                    return launch_band_pair(bc, tile, transient, xin, f0in, fl);
+                }, &us);
+                if (rc) return rc;
+                us *= 0.5;
+                if (tune_verbose())
+                    fprintf(stderr, "[ht3d tune] %dx%dx%d %s prec %d: band pair tile %d %.2f us per iteration\n", a.nx,
+                            a.ny, a.nz, transient ? "heat" : "darcy", prec, tile, us);
+                if (us < te.pair_us) { te.pair_us = us; te.pair = PairCfg{tile, 0}; }
             }
         }
     }
@@ -1106,7 +1549,7 @@ inline int choose_cfg(IterArgs<T> &a, bool transient, bool can_store_d, int prec
     if (g_force_pair_tile >= 0) {   // pinned: that tile, or no pairs where it cannot run
         const bool ok = pair_tile_ok<T>(g_force_pair_tile, a.nx) && a.ny >= 3;
         pc.tile = ok ? g_force_pair_tile : -1;
-        pc.variant = g_force_pair_variant & (can_store_d ? 3 : 2);
+        pc.variant = kPairTiles[g_force_pair_tile].band ? 0 : g_force_pair_variant & (can_store_d ? 3 : 2);
     }
     if (!F0b || g_pairs == 0) pc.tile = -1;
     set_tiling(a, c.tile);

@@ -294,5 +294,7 @@ ht3d_batch_tune_force_pair(tile) =
     ccall((:ht3d_batch_tune_force_pair, libht3d), Cint, (Cint,), tile)
 ht3d_batch_pair_tile_count() =
     ccall((:ht3d_batch_pair_tile_count, libht3d), Cint, ())
+ht3d_batch_pair_tile_info(tile, rows, planes, cells_per_lane) =
+    GC.@preserve rows planes cells_per_lane ccall((:ht3d_batch_pair_tile_info, libht3d), Cint, (Cint, Ptr{Cint}, Ptr{Cint}, Ptr{Cint}), tile, rows, planes, cells_per_lane)
 ht3d_lognormal_field(kfv, ncx, ncy, ncz, xmin, ymin, zmin, dx, dy, dz, K, phi, M, sigma, fault_mult, fault_half, fault_n, fault_x0) =
     GC.@preserve kfv K phi fault_n fault_x0 ccall((:ht3d_lognormal_field, libht3d), Cint, (Ptr{Cdouble}, Cint, Cint, Cint, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble, Ptr{Cdouble}, Ptr{Cdouble}, Cint, Cdouble, Cdouble, Cdouble, Ptr{Cdouble}, Ptr{Cdouble}), devptr(Cdouble, kfv), ncx, ncy, ncz, xmin, ymin, zmin, dx, dy, dz, K, phi, M, sigma, fault_mult, fault_half, fault_n, fault_x0)

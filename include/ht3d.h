@@ -620,6 +620,10 @@ int ht3d_batch_tune_force_grid(int mode);
 int ht3d_batch_tune_pairs(int mode);
 int ht3d_batch_tune_force_pair(int tile);
 int ht3d_batch_pair_tile_count(void);
+/* Pair tile `tile` of the batch's table: rows of 64 lanes per block, planes per
+ * item and x cells per lane (4 cells: the FP32 sweeps only).  No reference
+ * counterpart.                                                               */
+int ht3d_batch_pair_tile_info(int tile, int *rows, int *planes, int *cells_per_lane);
 
 /* Lognormal permeability on vertices (spectral randomisation) plus an optional
  * planar fault; K (3*M) and phi (M) in host memory.  Set-up only.            */

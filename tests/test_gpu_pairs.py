@@ -72,13 +72,19 @@ def test_sweep_direction_does_not_change_results(ht, precision):
     assert np.array_equal(X0, X1)
 
 
-PAIR_TILES = {0: 1, 1: 1, 2: 1, 3: 1, 4: 1, 5: 2, 6: 2, 7: 2, 8: 4, 9: 4, 10: 4}   # pair tile: x cells per lane
+PAIR_TILES = {0: 1, 1: 1, 2: 1, 3: 1, 4: 1, 5: 2, 6: 2, 7: 2, 8: 4, 9: 4, 10: 4,   # pair tile: x cells per lane
+              11: 4, 12: 4, 13: 4, 14: 2, 15: 1}
+BAND_ROWS = {11: 8, 12: 16, 13: 16, 14: 16, 15: 16}   # band tiles: rows of 64 lanes
 
 
 def _pair_tile_runs(tile, nx, precision):
     """Whether pair tile `tile` runs on this grid: nx divisible by its cells per
-    lane; in FP64 no 4-cell tile and no 2-cell tile with 16 rows."""
+    lane; in FP64 no 4-cell tile and no 2-cell tile with 16 rows; the band
+    tiles (bands of whole rows) in FP32 only and when a band holds at least 3
+    rows."""
     cpl = PAIR_TILES[tile]
+    if tile in BAND_ROWS:
+        return precision == 1 and nx % cpl == 0 and 64 * BAND_ROWS[tile] * cpl // nx >= 3
     if precision == 0 and (cpl == 4 or tile == 7):
         return False
     return nx % cpl == 0
@@ -91,7 +97,8 @@ def _pair_tile_runs(tile, nx, precision):
 def test_every_pair_tile_bitwise(ht, grid, transient, precision):
     """Each pair tile pinned in turn (ht3d_tune_force_pair), cycling through the
     variants: bit-identical to pairs off, or not used when the grid does not
-    allow it (the report then shows no pair tile)."""
+    allow it (the report then shows no pair tile).  Tiles 11-15 are bands of
+    whole rows (the batch's band-pair kernels on a one-entry table)."""
     import ctypes as C
     r0, X0, rows0 = _solve(ht, grid, transient, precision, 0)
     assert rows0 == 0
@@ -103,7 +110,8 @@ def test_every_pair_tile_bitwise(ht, grid, transient, precision):
             ty, kz, xc, var = C.c_int(), C.c_int(), C.c_int(), C.c_int()
             ht._lib.call("ht3d_tune_last_pair_tile", C.byref(ty), C.byref(kz), C.byref(xc), C.byref(var))
             if _pair_tile_runs(tile, grid[0], precision):
-                assert (xc.value, var.value) == (cpl, variant), (tile, xc.value, var.value)
+                want_var = 0 if tile in BAND_ROWS else variant   # the band tiles have no variants
+                assert (xc.value, var.value) == (cpl, want_var), (tile, xc.value, var.value)
             else:
                 assert rows == 0 and xc.value == 0, (tile, rows, xc.value)
             for key in ("iters", "status", "n_checks", "n_lambda", "lmin", "h1", "h2", "nF_abs"):
