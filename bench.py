@@ -7,8 +7,12 @@ velocity, WENO5 advection, CFL time step), inputs resident in HBM.
 
 Prints one JSON line:
   value         T_eff (GB/s) of the fused DYREL pseudo-iterations over the timed
-                steps: algorithmic bytes (8 words per cell per iteration) / summed
-                kernel time from HIP events on the library stream;
+                steps: algorithmic bytes / summed kernel time from HIP events on
+                the library stream.  The algorithmic bytes are those of the
+                kernel that ran: 8 words per cell per iteration when it reads the
+                three face-coefficient arrays (X and F0 read and written, R, kx,
+                ky, kz read), 6 when it forms them from the vertex field (R and
+                kv read), 7 when it also reads the stored diagonal;
   ms_per_step   wall time per physical step (all kernels, reductions and host
                 convergence checks);
   roofline      the iteration kernel against the 8 TB/s HBM peak;
@@ -258,7 +262,8 @@ def probe(w, iters: int, copies: int) -> int:
     k.KillNorthSouthFluxes(g["Ty"])
     k.Compute_Rp(g["Rp"], g["ky"], g["Ty"], w.Ra, 1.0 / w.h[1])
     ht.dyrel_solve(g["Pc_ex"], g["Rp"], g["kx"], g["ky"], g["kz"], dx=w.h[0], dy=w.h[1], dz=w.h[2], dt=1.0,
-                   transient=0, rhoC=0.0, params=ht.DyrelParams(nitmax=iters, nitout=10**6, tol=0.0))
+                   transient=0, rhoC=0.0, params=ht.DyrelParams(nitmax=iters, nitout=10**6, tol=0.0),
+                   kv=g["kfv"] if w.mu_a == 0.0 else None)
     a = ht.DeviceArray.zeros((w.ncx, w.ncy, w.ncz))
     b = ht.DeviceArray.zeros((w.ncx, w.ncy, w.ncz))
     for _ in range(copies):
@@ -542,6 +547,10 @@ def main(argv=None) -> int:
                     "schedule": "nitout 1000, convergence test at iter<=2 or iter%1000==0, mean|F*D|/sqrt(N), "
                                 "nitmax 10000"}
     teff = st["bytes"] / (st["ms"] / 1e3) / 1e9 if st["ms"] > 0 else 0.0
+    # words per cell and iteration of the kernel that ran (variant bit 8: face
+    # coefficients formed from the vertex field, bit 1: stored diagonal read;
+    # pairs always read the faces)
+    words = (7 if st["tile"][2] & 1 else 6) if (st["tile"][2] & 8) and st["pair"] is None else 8
     teff32 = st["bytes_fp32"] / (st["ms_fp32"] / 1e3) / 1e9 if st["ms_fp32"] > 0 else None
     avg_us = st["ms"] / max(st["launches"], 1) * 1e3
     iters = [(r["thermal"]["iters"], r["darcy"]["iters"]) for r in reports]
@@ -584,9 +593,11 @@ def main(argv=None) -> int:
         "hbm_frac": round(teff / HBM_PEAK_GBS, 4),
         "iters_per_step": iters,
         "iteration_kernel_avg_us": round(avg_us, 2),
+        "t_eff_8_words": round(8 * 8 * w.ncell / (avg_us * 1e-6) / 1e9, 2) if avg_us > 0 else None,
         "roofline": {"bound": "hbm", "achieved": round(teff, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(teff / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.workload, st["tile"]),
-                     "algorithmic_bytes": 8 * 8 * w.ncell,
+                     "algorithmic_bytes": words * 8 * w.ncell,
+                     "words_per_cell": words,
                      "tile": f"{tile_label(st['tile'])} variant {st['tile'][2]}",
                      "pair_tile": None if st["pair"] is None else
                      f"{tile_label(st['pair'])} variant {st['pair'][2]}"},

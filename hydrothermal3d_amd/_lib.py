@@ -44,7 +44,7 @@ class DyrelField(C.Structure):
                 ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
                 ("X", C.c_void_p), ("R", C.c_void_p), ("kx", C.c_void_p), ("ky", C.c_void_p),
                 ("kz", C.c_void_p), ("transient", C.c_int), ("rhoC", C.c_double), ("dt", C.c_double),
-                ("bc_type", C.c_int * 6), ("bc_val", C.c_double * 6)]
+                ("bc_type", C.c_int * 6), ("bc_val", C.c_double * 6), ("kv", C.c_void_p)]
 
 
 class V18Field(C.Structure):

@@ -48,6 +48,7 @@ int launch_nu_partials(const double *T, const double *ky, int nx, int ny, int nz
 int copy_ghosts(double *dst, const double *src, int nx, int ny, int nz);
 int bc_fill_table(double *A, int nx, int ny, int nz, const int type[6], const double val[6]);
 int to_float(float *out, const double *in, size_t n);
+int faces_f32(float *k32, const ht3d_dyrel_field *f);
 
 // per-problem solver state, device resident
 struct BState {
@@ -1021,16 +1022,9 @@ static int batch_solve(ht3d_batch *b, int kind, const ht3d_dyrel_params &p_in, c
     const bool transient = f[0].transient != 0;
     const bool fp32 = b->precision == HT3D_FP32_DC;
     int rc;
-    if (fp32)  // FP32 face coefficients of the correction equation
-        for (int q = 0; q < b->nprob; ++q) {
-            const ht3d_dyrel_field &d = f[q];
-            const size_t nfx = (size_t)(d.ncx + 1) * d.ncy * d.ncz, nfy = (size_t)d.ncx * (d.ncy + 1) * d.ncz,
-                         nfz = (size_t)d.ncx * d.ncy * (d.ncz + 1);
-            float *k = b->k32 + b->of[q];
-            if ((rc = to_float(k, d.kx, nfx)) || (rc = to_float(k + nfx, d.ky, nfy)) ||
-                (rc = to_float(k + nfx + nfy, d.kz, nfz)))
-                return rc;
-        }
+    if (fp32)  // FP32 face coefficients of the correction equation, as the single-problem solve forms them
+        for (int q = 0; q < b->nprob; ++q)
+            if ((rc = faces_f32(b->k32 + b->of[q], &f[q]))) return rc;
     int &tile = fp32 ? b->tile32[kind] : b->tile64[kind];
     if (tile < 0 && (rc = autotune_batch(b, kind, transient, fp32, f, &tile, &b->gmode[kind][fp32 ? 1 : 0]))) return rc;
     b->gmode_cur = b->gmode[kind][fp32 ? 1 : 0];

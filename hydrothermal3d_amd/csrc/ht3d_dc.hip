@@ -28,25 +28,30 @@ int nu_sum(const double *T, const double *ky, int nx, int ny, int nz, double _dy
 // 1 max|r|.  Tiles of dc_residual_tile in the XCD-contiguous order of the
 // iteration kernel, so the halo rows and planes a tile shares with its
 // neighbours are read while they are still in that XCD's L2.
-template <bool TR, int TY, int KZ, int CPL>
+template <bool TR, int TY, int KZ, int CPL, bool KV>
 __global__ __launch_bounds__(64 * TY) void dc_residual_tiled(DcArgs a, int nbx, int nby, int nbz) {
     const unsigned t = xcd_tile(blockIdx.x, (unsigned)nbx * nby * nbz);
     const int bx = t % nbx, by = (t / nbx) % nby, bz = t / (nbx * nby);
     int i, j;
     tile_cell<TY, CPL>(bx, by, i, j);
     double s_hi = 0.0, s_lo = 0.0, mx = 0.0;
-    dc_residual_tile<TR, TY, KZ, CPL>(a, i, j, bz, s_hi, s_lo, mx);
+    dc_residual_tile<TR, TY, KZ, CPL, KV>(a, i, j, bz, s_hi, s_lo, mx);
     const double h4[4] = {s_hi, mx, 0.0, 0.0}, l4[4] = {s_lo, 0.0, 0.0, 0.0};
     block_reduce4(h4, l4, 0x2u, a.partials, blockIdx.x);
 }
 
-// residual tiles (rows, planes, x-cells per lane); the solve times the ones
-// the grid allows (nx % cells per lane == 0) on its first anchor of a grid
+// residual tiles (rows, planes, x-cells per lane, face coefficients from the
+// vertex field); the solve times the ones the grid allows (nx % cells per
+// lane == 0; kv tiles when the solve has the vertex field) on its first
+// anchor of a grid
 struct ResTile {
     int ty, kz, cpl;
+    bool kv = false;
 };
-static constexpr ResTile kResTiles[] = {{4, 8, 2}, {4, 16, 2}, {8, 8, 2}, {8, 16, 2}, {2, 16, 2},
-                                        {4, 32, 2}, {4, 16, 4}, {4, 16, 1}, {8, 16, 1}};
+static constexpr ResTile kResTiles[] = {{4, 8, 2},        {4, 16, 2},       {8, 8, 2},        {8, 16, 2},
+                                        {2, 16, 2},       {4, 32, 2},       {4, 16, 4},       {4, 16, 1},
+                                        {8, 16, 1},       {4, 8, 2, true},  {4, 16, 2, true}, {8, 8, 2, true},
+                                        {8, 16, 2, true}, {2, 16, 2, true}, {4, 32, 2, true}};
 static constexpr int kNumResTiles = sizeof(kResTiles) / sizeof(kResTiles[0]);
 
 static unsigned res_blocks(int tile, int nx, int ny, int nz) {
@@ -59,21 +64,28 @@ static int launch_residual(const DcArgs &d, bool tr, int tile, unsigned *nblocks
     const int nbx = (int)div_up(d.nx, 64 * t.cpl), nby = (int)div_up(d.ny, t.ty), nbz = (int)div_up(d.nz, t.kz);
     const unsigned nb = (unsigned)nbx * nby * nbz;
     hipStream_t s = ctx().stream;
-#define RT_(TY, KZ, CPL)                                                                                              \
+    if (t.kv && !d.kv) return arg_fail("dc residual: vertex-coefficient tile without the vertex field");
+#define RT_(TY, KZ, CPL, KV)                                                                                          \
     do {                                                                                                              \
-        if (tr) hipLaunchKernelGGL((dc_residual_tiled<true, TY, KZ, CPL>), dim3(nb), dim3(64 * TY), 0, s, d, nbx, nby, nbz); \
-        else hipLaunchKernelGGL((dc_residual_tiled<false, TY, KZ, CPL>), dim3(nb), dim3(64 * TY), 0, s, d, nbx, nby, nbz); \
+        if (tr) hipLaunchKernelGGL((dc_residual_tiled<true, TY, KZ, CPL, KV>), dim3(nb), dim3(64 * TY), 0, s, d, nbx, nby, nbz); \
+        else hipLaunchKernelGGL((dc_residual_tiled<false, TY, KZ, CPL, KV>), dim3(nb), dim3(64 * TY), 0, s, d, nbx, nby, nbz); \
     } while (0)
     switch (tile) {
-    case 0: RT_(4, 8, 2); break;
-    case 1: RT_(4, 16, 2); break;
-    case 2: RT_(8, 8, 2); break;
-    case 3: RT_(8, 16, 2); break;
-    case 4: RT_(2, 16, 2); break;
-    case 5: RT_(4, 32, 2); break;
-    case 6: RT_(4, 16, 4); break;
-    case 7: RT_(4, 16, 1); break;
-    default: RT_(8, 16, 1); break;
+    case 0: RT_(4, 8, 2, false); break;
+    case 1: RT_(4, 16, 2, false); break;
+    case 2: RT_(8, 8, 2, false); break;
+    case 3: RT_(8, 16, 2, false); break;
+    case 4: RT_(2, 16, 2, false); break;
+    case 5: RT_(4, 32, 2, false); break;
+    case 6: RT_(4, 16, 4, false); break;
+    case 7: RT_(4, 16, 1, false); break;
+    case 8: RT_(8, 16, 1, false); break;
+    case 9: RT_(4, 8, 2, true); break;
+    case 10: RT_(4, 16, 2, true); break;
+    case 11: RT_(8, 8, 2, true); break;
+    case 12: RT_(8, 16, 2, true); break;
+    case 13: RT_(2, 16, 2, true); break;
+    default: RT_(4, 32, 2, true); break;
     }
 #undef RT_
     HIP_TRY(hipGetLastError());
@@ -115,7 +127,7 @@ static int tune_residual(const DcArgs &d, bool tr, const Span *touch, int ntouch
     for (int t = 0; t < kNumResTiles; ++t) best[t] = 1e300;
     for (int round = 0; round < 3; ++round)
         for (int t = 0; t < kNumResTiles; ++t) {
-            if (d.nx % kResTiles[t].cpl) continue;
+            if (d.nx % kResTiles[t].cpl || (kResTiles[t].kv && !d.kv)) continue;
             for (int q = 0; q < ntouch; ++q) {
                 hipLaunchKernelGGL(touch_kernel, dim3(2048), dim3(256), 0, cx.stream, (const float4 *)touch[q].p,
                                    touch[q].bytes / 16, (float *)d.partials);
@@ -194,10 +206,58 @@ __global__ __launch_bounds__(256) void to_float_kernel(float *__restrict__ out, 
         out[t] = (float)in[t];
 }
 
+// FP32 face coefficients as the face averages of the FP32 vertex field
+// ((float) of each vertex value, then iter_tile_kv's additions, av4): the
+// face-array FP32 kernels and the FP32 vertex-coefficient kernels then see the
+// same coefficients bit for bit.
+__global__ __launch_bounds__(256) void faces_from_vertex_f32(float *__restrict__ kx, float *__restrict__ ky,
+                                                             float *__restrict__ kz, const double *__restrict__ kv,
+                                                             int nx, int ny, int nz) {
+    const Grid g{nx, ny, nz};
+    const size_t nfx = (size_t)(nx + 1) * ny * nz, nfy = (size_t)nx * (ny + 1) * nz, nfz = (size_t)nx * ny * (nz + 1);
+    auto V = [&](int i, int j, int k) { return (float)kv[g.v(i, j, k)]; };
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < nfx + nfy + nfz;
+         t += (size_t)gridDim.x * blockDim.x) {
+        if (t < nfx) {   // @av_yza
+            const int i = (int)(t % (nx + 1)), j = (int)(t / (nx + 1) % ny), k = (int)(t / ((size_t)(nx + 1) * ny));
+            kx[t] = av4(V(i, j, k), V(i, j + 1, k), V(i, j, k + 1), V(i, j + 1, k + 1));
+        } else if (t < nfx + nfy) {   // @av_xza
+            const size_t u = t - nfx;
+            const int i = (int)(u % nx), j = (int)(u / nx % (ny + 1)), k = (int)(u / ((size_t)nx * (ny + 1)));
+            ky[u] = av4(V(i, j, k), V(i + 1, j, k), V(i, j, k + 1), V(i + 1, j, k + 1));
+        } else {   // @av_xya
+            const size_t u = t - nfx - nfy;
+            const int i = (int)(u % nx), j = (int)(u / nx % ny), k = (int)(u / ((size_t)nx * ny));
+            kz[u] = av4(V(i, j, k), V(i + 1, j, k), V(i, j + 1, k), V(i + 1, j + 1, k));
+        }
+    }
+}
+
 int to_float(float *out, const double *in, size_t n) {
     unsigned nb = div_up(n, 256);
     if (nb > 8192) nb = 8192;
     hipLaunchKernelGGL(to_float_kernel, dim3(nb), dim3(256), 0, ctx().stream, out, in, n);
+    HIP_TRY(hipGetLastError());
+    return HT3D_OK;
+}
+
+// The FP32 face coefficients of the correction equation (kx, ky, kz one after
+// the other in k32): the face averages of the FP32 vertex field when the solve
+// has one, else the FP64 faces rounded to FP32.  The single-problem and the
+// batched defect-correction solves both take them from here.
+int faces_f32(float *k32, const ht3d_dyrel_field *f) {
+    const int nx = f->ncx, ny = f->ncy, nz = f->ncz;
+    const size_t nfx = (size_t)(nx + 1) * ny * nz, nfy = (size_t)nx * (ny + 1) * nz, nfz = (size_t)nx * ny * (nz + 1);
+    float *kx = k32, *ky = kx + nfx, *kz = ky + nfy;
+    if (!f->kv) {
+        int rc;
+        if ((rc = to_float(kx, f->kx, nfx)) || (rc = to_float(ky, f->ky, nfy)) || (rc = to_float(kz, f->kz, nfz)))
+            return rc;
+        return HT3D_OK;
+    }
+    unsigned nb = div_up(nfx + nfy + nfz, 256);
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(faces_from_vertex_f32, dim3(nb), dim3(256), 0, ctx().stream, kx, ky, kz, f->kv, nx, ny, nz);
     HIP_TRY(hipGetLastError());
     return HT3D_OK;
 }
@@ -234,9 +294,15 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
         if (res_blocks(t, nx, ny, nz) > max_blocks) max_blocks = res_blocks(t, nx, ny, nz);
     if ((rc = workspace(WS_PARTIALS, (size_t)max_blocks * 8 * sizeof(double), &part))) return rc;
     if ((rc = workspace(WS_RESULT, 64 * sizeof(double), &res))) return rc;
-    float *kx32 = (float *)k32, *ky32 = kx32 + nfx, *kz32 = ky32 + nfy;
-    if ((rc = to_float(kx32, f->kx, nfx)) || (rc = to_float(ky32, f->ky, nfy)) || (rc = to_float(kz32, f->kz, nfz)))
-        return rc;
+    float *kx32 = (float *)k32, *ky32 = kx32 + nfx, *kz32 = ky32 + nfy, *kv32 = nullptr;
+    if ((rc = faces_f32(kx32, f))) return rc;
+    if (f->kv) {   // the FP32 vertex field of the vertex-coefficient kernels
+        const size_t nv = (size_t)(nx + 1) * (ny + 1) * (nz + 1);
+        void *pv = nullptr;
+        if ((rc = workspace(WS_F32_KV, nv * sizeof(float), &pv))) return rc;
+        kv32 = (float *)pv;
+        if ((rc = to_float(kv32, f->kv, nv))) return rc;
+    }
 
     // lambda_max and the DYREL scalars (HydroThermal3D_v18.jl:432-439)
     const double rc_tr_dt = f->rhoC * (double)f->transient / f->dt;
@@ -248,7 +314,7 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
 
     DcArgs d;
     std::memset(&d, 0, sizeof d);
-    d.X = f->X; d.R = f->R; d.kx = f->kx; d.ky = f->ky; d.kz = f->kz; d.r32 = (float *)r32;
+    d.X = f->X; d.R = f->R; d.kx = f->kx; d.ky = f->ky; d.kz = f->kz; d.kv = f->kv; d.r32 = (float *)r32;
     d.partials = (double *)part; d.nx = nx; d.ny = ny; d.nz = nz;
     d._dx = 1.0 / f->dx; d._dy = 1.0 / f->dy; d._dz = 1.0 / f->dz; d._dt = 1.0 / f->dt;
     for (int q = 0; q < 6; ++q) { d.bct[q] = f->bc_type[q]; d.bcv[q] = f->bc_val[q]; }
@@ -257,7 +323,7 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     IterArgs<float> a;
     std::memset(&a, 0, sizeof a);
     a.nx = nx; a.ny = ny; a.nz = nz;
-    a.R = (const float *)r32; a.kx = kx32; a.ky = ky32; a.kz = kz32; a.Dst = nullptr;
+    a.R = (const float *)r32; a.kx = kx32; a.ky = ky32; a.kz = kz32; a.kv = kv32; a.Dst = nullptr;
     a.F0 = (float *)f0; a.Fst = (float *)fst; a.partials = (double *)part;
     a._dx = (float)d._dx; a._dy = (float)d._dy; a._dz = (float)d._dz; a._dt = (float)d._dt;
     a.dx = (float)f->dx; a.dy = (float)f->dy; a.dz = (float)f->dz; a.rc_tr_dt = (float)rc_tr_dt;
@@ -287,7 +353,7 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     // and the lambda schedule run on uninterrupted, so the iteration follows the
     // FP64 solve up to FP32 rounding, and re-anchoring keeps e small enough that
     // FP32 never has to resolve more than the residual drop between anchors.
-    TuneEntry *te = tune_find(TuneKey{nx, ny, nz, f->transient != 0 ? 1 : 0, 1});
+    TuneEntry *te = tune_find(TuneKey{nx, ny, nz, f->transient != 0 ? 1 : 0, 1, f->kv ? 1 : 0});
     int res_tile = te && te->res_tile >= 0 ? te->res_tile : -1;
     if (res_tile < 0) {
         if (g_autotune && g_force_tile < 0) {
@@ -308,7 +374,7 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     double nF_abs = 0.0, nF_rel = 0.0, nF_ini = 0.0;
     int status = HT3D_NOT_CONVERGED, outer = 0, inner_total = 0, n_checks = 0, n_lambda = 0;
     SegmentTimer timer;
-    const double bytes32 = 8.0 * (double)N * sizeof(float);
+    const double bytes32 = ((cfg.variant & 8) && pcfg.tile < 0 ? 6.0 : 8.0) * (double)N * sizeof(float);
     HIP_TRY(hipMemsetAsync(e0, 0, NE * sizeof(float), cx.stream));
     HIP_TRY(hipMemsetAsync(e1, 0, NE * sizeof(float), cx.stream));
     HIP_TRY(hipMemsetAsync(f0, 0, N * sizeof(float), cx.stream));

@@ -242,7 +242,7 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
     double c = 2.0 * sqrt(lmin) * p->cfact;
     double h1 = (2 - c * dtau) / (2 + c * dtau), h2 = 2 * dtau / (2 + c * dtau);
 
-    a.R = f->R; a.kx = f->kx; a.ky = f->ky; a.kz = f->kz; a.Dst = (const double *)dst;
+    a.R = f->R; a.kx = f->kx; a.ky = f->ky; a.kz = f->kz; a.kv = f->kv; a.Dst = (const double *)dst;
     a.F0 = (double *)f0; a.Fst = (double *)fst; a.partials = (double *)part;
     a._dx = 1.0 / f->dx; a._dy = 1.0 / f->dy; a._dz = 1.0 / f->dz; a._dt = 1.0 / f->dt;
     a.dx = f->dx; a.dy = f->dy; a.dz = f->dz; a.rc_tr_dt = rc_tr_dt;
@@ -253,10 +253,11 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
     a.Xout = (double *)xb;
     IterCfg cfg;
     PairCfg pcfg;
-    TuneEntry *te = tune_find(TuneKey{nx, ny, nz, f->transient != 0 ? 1 : 0, 0});
+    TuneEntry *te = tune_find(TuneKey{nx, ny, nz, f->transient != 0 ? 1 : 0, 0, f->kv ? 1 : 0});
     if (given) {   // no tuning pass: the grid's cached choice, else the default tile
         if (te) {
             cfg = te->cfg;
+            kv_normalize(a, cfg);
             if (te->pair.tile >= 0 && (g_pairs == 2 || (g_pairs == 1 && te->pair_us < te->us))) pcfg = te->pair;
         } else {
             cfg = IterCfg{tile_for_grid(kDefaultTile, nx), 6};
@@ -295,7 +296,12 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
     SegmentTimer timer;
     last_cfg() = cfg;
     last_pair() = pcfg;
-    const double bytes_per_iter = 8.0 * (double)N * sizeof(double);
+    // algorithmic bytes per iteration: X, F0 read and written, R and the face
+    // coefficients read (8 words per cell), or R and the vertex field (6 words,
+    // 7 with the stored diagonal) when the single launches form the
+    // coefficients from it and no pairs run
+    const double words = (cfg.variant & 8) && pcfg.tile < 0 ? (cfg.variant & 1 ? 7.0 : 6.0) : 8.0;
+    const double bytes_per_iter = words * (double)N * sizeof(double);
     double nF_abs = 0.0, nF_rel = 0.0, nF_ini = 0.0;
     int status = HT3D_NOT_CONVERGED, iter, n_checks = 0, n_lambda = 0, done_iters = 0;
     auto plain = [&](int it) {   // no check, no stored F, no lambda update
@@ -400,7 +406,7 @@ int ht3d_dyrel_solve(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
 
 int ht3d_tune_set(int autotune, int force_tile, int force_variant) {
     HT3D_API_BEGIN
-    if (force_tile >= ht3d::kNumTiles || force_variant < 0 || force_variant > 7)
+    if (force_tile >= ht3d::kNumTiles || force_variant < 0 || force_variant > 15)
         return ht3d::arg_fail("ht3d_tune_set: no such tile configuration or variant");
     ht3d::g_autotune = autotune != 0;
     ht3d::g_force_tile = force_tile;

@@ -17,7 +17,7 @@ enum WsSlot {
     WS_PARTIALS = 0, WS_RESULT = 1, WS_XB = 2, WS_F0 = 3, WS_FCIT = 4, WS_MISC = 5, WS_DIAG = 6,
     WS_F32_X = 7, WS_F32_XB = 8, WS_F32_F0 = 9, WS_F32_R = 10, WS_F32_K = 11, WS_F32_D = 12, WS_DC_RES = 13,
     WS_F0B = 14, WS_F32_F0B = 15, WS_F32_DIAG = 16, WS_V18_CLS = 17, WS_V18_G = 18, WS_V18_GY = 19,
-    WS_TUNE_X = 20, WS_BAND = 21, WS_COUNT = 22
+    WS_TUNE_X = 20, WS_BAND = 21, WS_F32_KV = 22, WS_COUNT = 23
 };
 
 struct Context {

@@ -33,6 +33,7 @@ struct IterArgs {
     const T *__restrict__ kx;
     const T *__restrict__ ky;
     const T *__restrict__ kz;
+    const T *__restrict__ kv;    // vertex field whose face averages kx, ky, kz are (variant bit 8), or null
     const T *__restrict__ Dst;   // stored Gershgorin diagonal (DSTORED variants)
     T *__restrict__ F0;
     T *__restrict__ Fst;         // F of the previous lambda iteration (read) / this one (write)
@@ -397,6 +398,172 @@ __device__ __forceinline__ void iter_tile_v(const IterArgs<T> &a, const T *__res
     }
 }
 
+// Face coefficients formed from a vertex field.  In the closed-box model the
+// three face arrays are the face averages of one vertex field kv
+// ((nx+1)(ny+1)(nz+1) values, x fastest): kx = @av_yza(kv), ky = @av_xza(kv),
+// kz = @av_xya(kv) (InitThermal! / InitDarcy!, kernels_HT3D.jl:82-107).  The
+// iteration then needs one read-only word per cell for its coefficients
+// instead of three: 6 words per cell in all instead of 8.  av4 adds the four
+// vertex values in the order the averaging kernels add them (left to right,
+// then times 0.25), so every face value, and every result, is the same bit for
+// bit as with the face arrays.
+template <typename T>
+__device__ __forceinline__ T av4(T p, T q, T r, T s) {
+    return (p + q + r + s) * (T)0.25;
+}
+
+// iter_tile_v with the face coefficients formed from the vertex field a.kv.
+// The lane keeps the vertex values of its plane k, rows j and j+1, x
+// positions i .. i+CPL (2(CPL+1) values), and loads those of plane k+1 as one
+// CPL-vector and one scalar per row; with a0/a1 (plane k) and b0/b1 (plane
+// k+1) on rows j/j+1, the faces of cell i+q are
+//   west  kx(i+q)   = av4(a0[q], a1[q], b0[q], b1[q])      (@av_yza)
+//   south ky(j)     = av4(a0[q], a0[q+1], b0[q], b0[q+1])  (@av_xza)
+//   north ky(j+1)   = av4(a1[q], a1[q+1], b1[q], b1[q+1])
+//   back  kz(k)     = av4(a0[q], a0[q+1], a1[q], a1[q+1])  (@av_xya; the front
+//   front kz(k+1)   = av4(b0[q], b0[q+1], b1[q], b1[q+1])   face of plane k-1)
+// Every vertex index stays inside the vertex array (i + CPL <= nx, j + 1 <= ny,
+// k + 1 <= nz).  DSTORED reads the Gershgorin diagonal stored once per solve
+// (one word more, but none of the nine FP64 divisions that recomputing it
+// costs).  ROLL keeps the z march a rolled loop: unrolled, the compiler hoists
+// later planes' loads and the FP64 two-cell kernel needs 151 VGPRs (3 waves
+// per SIMD) instead of 122 (4 waves).  Needs nx % CPL == 0, CPL >= 2.
+template <typename T, bool TRANSIENT, bool DSTORED, bool NTX, bool ROLL, int TY, int KZ, int CPL>
+__device__ __forceinline__ void iter_tile_kv(const IterArgs<T> &a, const T *__restrict__ Xin, T *__restrict__ Xout,
+                                             unsigned flags, int i, int j, int bz, double *__restrict__ partials,
+                                             unsigned slot) {
+    typedef typename CellVec<T, CPL>::type V;
+    const int k0 = bz * KZ;
+    const int k1 = min(k0 + KZ, a.nz);
+    const uint32_t nx = a.nx, ny = a.ny;
+    const uint32_t sE1 = nx + 2, sE2 = sE1 * (ny + 2);
+    const uint32_t sC2 = nx * ny, sV1 = nx + 1, sV2 = sV1 * (ny + 1);
+
+    double s_hi = 0.0, s_lo = 0.0, acc_max = 0.0, n_hi = 0.0, n_lo = 0.0, d_hi = 0.0, d_lo = 0.0;
+    const bool check = flags & FL_CHECK, lam = flags & FL_LAMBDA, store = flags & FL_STORE_F;
+    const T h1 = a.hv ? a.hv[0] : a.h1, h2 = a.hv ? a.hv[1] : a.h2;
+
+    if ((uint32_t)i < nx && (uint32_t)j < ny) {
+        const bool wlo = i == 0, whi = (uint32_t)i + CPL == nx, slo = j == 0, shi = (uint32_t)j == ny - 1;
+        uint32_t e = (uint32_t)(i + 1) + (uint32_t)(j + 1) * sE1 + (uint32_t)(k0 + 1) * sE2;
+        uint32_t c = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k0 * sC2;
+        uint32_t v = (uint32_t)i + (uint32_t)j * sV1 + (uint32_t)k0 * sV2;
+        const T *__restrict__ kv = a.kv;
+        T a0[CPL + 1], a1[CPL + 1];
+        auto vrow = [&](uint32_t o, T *out) {   // CPL + 1 vertex values from kv[o]
+            const V w = ldv<T, CPL>(kv + o);
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) out[q] = w[q];
+            out[CPL] = kv[o + CPL];
+        };
+        vrow(v, a0);
+        vrow(v + sV1, a1);
+        V xc = ldv<T, CPL>(Xin + e);
+        V xm = ldv<T, CPL>(Xin + e - sE2);
+        if (k0 == 0) {
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) xm[q] = ghost(a.bct[4], a.bcv[4], (T)xc[q]);
+        }
+        T kzB[CPL];
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) kzB[q] = av4(a0[q], a0[q + 1], a1[q], a1[q + 1]);
+        auto plane = [&](int k) {
+            V xp = ldv<T, CPL>(Xin + e + sE2);
+            T xW = Xin[e - 1], xE = Xin[e + CPL];
+            V xS = ldv<T, CPL>(Xin + e - sE1), xN = ldv<T, CPL>(Xin + e + sE1);
+            T b0[CPL + 1], b1[CPL + 1];
+            vrow(v + sV2, b0);
+            vrow(v + sV2 + sV1, b1);
+            const V r = ldv_nt<T, CPL>(a.R + c);
+            const V f0o = ldv_nt<T, CPL>(a.F0 + c);
+            T kX[CPL + 1], kS[CPL], kN[CPL], kF[CPL];
+#pragma unroll
+            for (int q = 0; q <= CPL; ++q) kX[q] = av4(a0[q], a1[q], b0[q], b1[q]);
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                kS[q] = av4(a0[q], a0[q + 1], b0[q], b0[q + 1]);
+                kN[q] = av4(a1[q], a1[q + 1], b1[q], b1[q + 1]);
+                kF[q] = av4(b0[q], b0[q + 1], b1[q], b1[q + 1]);
+            }
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                if (k + 1 == a.nz) xp[q] = ghost(a.bct[5], a.bcv[5], (T)xc[q]);
+                if (slo) xS[q] = ghost(a.bct[2], a.bcv[2], (T)xc[q]);
+                if (shi) xN[q] = ghost(a.bct[3], a.bcv[3], (T)xc[q]);
+            }
+            if (wlo) xW = ghost(a.bct[0], a.bcv[0], (T)xc[0]);
+            if (whi) xE = ghost(a.bct[1], a.bcv[1], (T)xc[CPL - 1]);
+            V f, D, g, xo;
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                const T w = q == 0 ? xW : (T)xc[q - 1], ea = q == CPL - 1 ? xE : (T)xc[q + 1];
+                f[q] = cell_residual<T, TRANSIENT>(a, xc[q], w, ea, xS[q], xN[q], xm[q], xp[q], kX[q], kX[q + 1], kS[q],
+                                                   kN[q], kzB[q], kF[q], r[q]);
+                if (!DSTORED) D[q] = cell_diag(a, kX[q], kX[q + 1], kS[q], kN[q], kzB[q], kF[q]);
+            }
+            if (DSTORED) D = ldv_nt<T, CPL>(a.Dst + c);
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                f[q] = precond<T>(f[q], D[q]);
+                g[q] = momentum<T>(h1, f0o[q], h2, f[q]);   // DYRELUpdate! (kernels_HT3D_dimensional.jl:323-324)
+                xo[q] = step_x<T>(xc[q], a.dtau, g[q]);
+            }
+            stv_nt<T, CPL>(g, a.F0 + c);
+            if (NTX) stv_nt<T, CPL>(xo, Xout + e);
+            else stv<T, CPL>(xo, Xout + e);
+            if (check) {
+                V fst = xo;   // only read on lambda iterations
+                if (lam) fst = ldv<T, CPL>(a.Fst + c);
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) {
+                    const double fd = fabs((double)(f[q] * D[q]));
+                    dd_add(s_hi, s_lo, fd);
+                    acc_max = (fd > acc_max || fd != fd) ? fd : acc_max;
+                    if (lam) {   // RayleighQuotientNumerator! / Denominator! terms
+                        const double d = (double)(a.dtau * g[q]);
+                        dd_add(n_hi, n_lo, (-d) * ((double)f[q] - (double)fst[q]));
+                        dd_add(d_hi, d_lo, d * d);
+                    }
+                }
+            }
+            if (store) stv<T, CPL>(f, a.Fst + c);
+            xm = xc;
+            xc = xp;
+#pragma unroll
+            for (int q = 0; q <= CPL; ++q) {
+                a0[q] = b0[q];
+                a1[q] = b1[q];
+            }
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) kzB[q] = kF[q];
+            e += sE2;
+            c += sC2;
+            v += sV2;
+        };
+        if (!ROLL && k1 - k0 == KZ) {
+#pragma unroll
+            for (int q = 0; q < KZ; ++q) plane(k0 + q);
+        } else {
+            for (int k = k0; k < k1; ++k) plane(k);
+        }
+    }
+    if (check) {
+        const double h4[4] = {s_hi, acc_max, n_hi, d_hi}, l4[4] = {s_lo, 0.0, n_lo, d_lo};
+        block_reduce4(h4, l4, 0x2u, partials, slot);
+    }
+}
+
+template <typename T, bool TRANSIENT, bool DSTORED, bool NTX, bool ROLL, int TY, int KZ, int CPL>
+__global__ __launch_bounds__(64 * TY) void dyrel_iter_kv_kernel(IterArgs<T> a) {
+    const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
+    const unsigned t = xcd_tile(blockIdx.x, nblocks, a.flags & FL_REVERSE);
+    const int bx = t % a.nbx, by = (t / a.nbx) % a.nby, bz = t / (a.nbx * a.nby);
+    int i, j;
+    tile_cell<TY, CPL>(bx, by, i, j);
+    iter_tile_kv<T, TRANSIENT, DSTORED, NTX, ROLL, TY, KZ, CPL>(a, a.Xin, a.Xout, a.flags, i, j, bz, a.partials,
+                                                                blockIdx.x);
+}
+
 // CPL: cells per lane along x (1: iter_tile; 2 or 4: iter_tile_v, nx % CPL == 0)
 template <typename T, bool TRANSIENT, bool DSTORED, bool NTX, bool KT, int TY, int KZ, int CPL>
 __global__ __launch_bounds__(64 * TY) void dyrel_iter_kernel(IterArgs<T> a) {
@@ -712,7 +879,10 @@ inline void launch_tile(int tile, const IterArgs<T> &a, unsigned nb, hipStream_t
 
 // variant bits: 1 = read the stored diagonal instead of recomputing it,
 //               2 = non-temporal store of the updated unknown,
-//               4 = cached (not streamed) face-coefficient loads
+//               4 = cached (not streamed) face-coefficient loads,
+//               8 = face coefficients from the vertex field (dyrel_iter_kv_kernel;
+//                   with it bit 1 applies in FP64 only, and bit 4 means a
+//                   rolled z march, in FP32 only)
 struct IterCfg {
     int tile = kDefaultTile;
     int variant = 0;
@@ -737,8 +907,75 @@ inline void set_tiling(IterArgs<T> &a, int tile) {
     a.nbz = (int)div_up(a.nz, kTiles[tile].kz);
 }
 
+// Vertex-coefficient kernels (variant bit 8, dyrel_iter_kv_kernel), when the
+// solve has the vertex field: the tiles with two cells per lane, and in FP32
+// also those with four.
+template <typename T>
+inline bool kv_tile_ok(int tile, int nx) {
+    const int cpl = kTiles[tile].cpl;
+    return (cpl == 2 || (cpl == 4 && sizeof(T) == 4)) && nx % cpl == 0;
+}
+template <typename T>
+inline bool kv_cfg_ok(const IterArgs<T> &a, const IterCfg &cfg) {
+    return !(cfg.variant & 8) || (a.kv && kv_tile_ok<T>(cfg.tile, a.nx));
+}
+// drops the vertex bit where it cannot run (no vertex field, or another
+// tile); with it, bit 1 (stored diagonal) applies in FP64 only and bit 4
+// (rolled z march) in FP32 only
+template <typename T>
+inline void kv_normalize(const IterArgs<T> &a, IterCfg &cfg) {
+    if ((cfg.variant & 8) && !(a.kv && kv_tile_ok<T>(cfg.tile, a.nx))) cfg.variant &= ~(8 | 4);
+    if (cfg.variant & 8) cfg.variant &= sizeof(T) == 8 ? ~4 : ~1;
+}
+
+template <typename T, bool TR, bool DS, bool NX, bool RL, int TY, int KZ, int CPL>
+inline void launch_kv(const IterArgs<T> &a, unsigned nb, hipStream_t st) {
+    hipLaunchKernelGGL((dyrel_iter_kv_kernel<T, TR, DS, NX, RL, TY, KZ, CPL>), dim3(nb), dim3(64 * TY), 0, st, a);
+}
+
+template <typename T, bool TR, bool DS, bool NX, bool RL>
+inline void launch_kv_tile(int tile, const IterArgs<T> &a, unsigned nb, hipStream_t st) {
+    switch (tile) {
+    case 14: launch_kv<T, TR, DS, NX, RL, 4, 4, 2>(a, nb, st); break;
+    case 15: launch_kv<T, TR, DS, NX, RL, 4, 8, 2>(a, nb, st); break;
+    case 16: launch_kv<T, TR, DS, NX, RL, 2, 8, 2>(a, nb, st); break;
+    case 17: launch_kv<T, TR, DS, NX, RL, 8, 4, 2>(a, nb, st); break;
+    case 18: launch_kv<T, TR, DS, NX, RL, 4, 16, 2>(a, nb, st); break;
+    case 19: launch_kv<T, TR, DS, NX, RL, 2, 16, 2>(a, nb, st); break;
+    default:
+        if constexpr (sizeof(T) == 4) {   // 20-24 (kv_tile_ok admits 14-24 in FP32, 14-19 in FP64)
+            switch (tile) {
+            case 20: launch_kv<T, TR, DS, NX, RL, 4, 4, 4>(a, nb, st); break;
+            case 21: launch_kv<T, TR, DS, NX, RL, 4, 8, 4>(a, nb, st); break;
+            case 22: launch_kv<T, TR, DS, NX, RL, 2, 8, 4>(a, nb, st); break;
+            case 23: launch_kv<T, TR, DS, NX, RL, 4, 16, 4>(a, nb, st); break;
+            default: launch_kv<T, TR, DS, NX, RL, 2, 16, 4>(a, nb, st); break;
+            }
+        }
+        break;
+    }
+}
+
 template <typename T, bool TR>
 inline void launch_variant(const IterArgs<T> &a, const IterCfg &cfg, unsigned nb, hipStream_t st) {
+    if (cfg.variant & 8) {   // FP64: bits 1 and 2; FP32: bits 2 and 4 (kv_normalize)
+        if constexpr (sizeof(T) == 8) {
+            switch (cfg.variant & 3) {
+            case 0: launch_kv_tile<T, TR, false, false, false>(cfg.tile, a, nb, st); break;
+            case 1: launch_kv_tile<T, TR, true, false, false>(cfg.tile, a, nb, st); break;
+            case 2: launch_kv_tile<T, TR, false, true, false>(cfg.tile, a, nb, st); break;
+            default: launch_kv_tile<T, TR, true, true, false>(cfg.tile, a, nb, st); break;
+            }
+        } else {
+            switch (cfg.variant & 6) {
+            case 0: launch_kv_tile<T, TR, false, false, false>(cfg.tile, a, nb, st); break;
+            case 2: launch_kv_tile<T, TR, false, true, false>(cfg.tile, a, nb, st); break;
+            case 4: launch_kv_tile<T, TR, false, false, true>(cfg.tile, a, nb, st); break;
+            default: launch_kv_tile<T, TR, false, true, true>(cfg.tile, a, nb, st); break;
+            }
+        }
+        return;
+    }
     switch (cfg.variant & 7) {
     case 0: launch_tile<T, TR, false, false, false>(cfg.tile, a, nb, st); break;
     case 1: launch_tile<T, TR, true, false, false>(cfg.tile, a, nb, st); break;
@@ -753,6 +990,7 @@ inline void launch_variant(const IterArgs<T> &a, const IterCfg &cfg, unsigned nb
 
 template <typename T>
 inline int launch_iter(const IterArgs<T> &a, bool transient, const IterCfg &cfg) {
+    if (!kv_cfg_ok(a, cfg)) return arg_fail("dyrel_iter_kernel: vertex-coefficient variant on a tile or solve without it");
     const unsigned nb = (unsigned)a.nbx * a.nby * a.nbz;
     hipStream_t st = ctx().stream;
     if (transient) launch_variant<T, true>(a, cfg, nb, st);
@@ -1289,8 +1527,9 @@ inline bool tune_verbose() {
 
 struct TuneKey {
     int nx, ny, nz, transient, prec;
+    int kv = 0;   // the solve has the vertex field (variant bit 8 can run)
     bool operator==(const TuneKey &o) const {
-        return nx == o.nx && ny == o.ny && nz == o.nz && transient == o.transient && prec == o.prec;
+        return nx == o.nx && ny == o.ny && nz == o.nz && transient == o.transient && prec == o.prec && kv == o.kv;
     }
 };
 struct TuneEntry {
@@ -1345,7 +1584,7 @@ inline int tuned_check_interval(double iters_prev, double t_it, double t_chk, in
 // hold the diagonal; F0b (when not null) is a second F0 buffer for the pairs.
 template <typename T>
 inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T *F0b, TuneEntry *out) {
-    const TuneKey key{a.nx, a.ny, a.nz, transient ? 1 : 0, prec};
+    const TuneKey key{a.nx, a.ny, a.nz, transient ? 1 : 0, prec, a.kv ? 1 : 0};
     if (const TuneEntry *t = tune_find(key)) {
         *out = *t;
         return HT3D_OK;
@@ -1429,6 +1668,16 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
             cands.push_back(cd);
         }
     }
+    if (a.kv && !small)   // the vertex-coefficient kernels (FP64: variants 8-11, FP32: 8, 10, 12, 14)
+        for (int var = 8; var <= 15; ++var) {
+            if (sizeof(T) == 8 ? (var & 4) || ((var & 1) && !can_store_d) : (var & 1)) continue;
+            for (int tile = 0; tile < kNumTiles; ++tile) {
+                if (!kv_tile_ok<T>(tile, a.nx)) continue;
+                Cand cd{IterCfg{tile, var}, 0.0};
+                if (int rc = time_cfg(cd.cfg, &cd.us)) return rc;
+                cands.push_back(cd);
+            }
+        }
     std::sort(cands.begin(), cands.end(), [](const Cand &x, const Cand &y) { return x.us < y.us; });
     const size_t nfin = cands.size() < 4 ? cands.size() : 4;
     std::vector<double> fin(nfin, 1e300);
@@ -1536,13 +1785,15 @@ inline int choose_cfg(IterArgs<T> &a, bool transient, bool can_store_d, int prec
     PairCfg pc;
     if (g_force_tile >= 0) {
         c.tile = tile_for_grid(g_force_tile, a.nx);
-        c.variant = g_force_variant & (can_store_d ? 7 : 6);
+        c.variant = g_force_variant & (can_store_d ? 15 : 14);
+        kv_normalize(a, c);
     } else if (g_autotune) {
         TuneEntry te;
         int rc = autotune(a, transient, can_store_d, prec, F0b, &te);
         if (rc) return rc;
         c = te.cfg;
         if (!can_store_d) c.variant &= ~1;
+        kv_normalize(a, c);
         if (te.pair.tile >= 0 && (g_pairs == 2 || (g_pairs == 1 && te.pair_us < te.us))) pc = te.pair;
         if (!can_store_d) pc.variant &= ~1;
     }
@@ -1568,6 +1819,7 @@ struct DcArgs {
     const double *__restrict__ kx;
     const double *__restrict__ ky;
     const double *__restrict__ kz;
+    const double *__restrict__ kv;  // V or null: the vertex field whose face averages kx, ky, kz are
     float *__restrict__ r32;        // c: FP32 right-hand side of the correction equation
     double *__restrict__ partials;
     int nx, ny, nz;
@@ -1586,8 +1838,11 @@ struct DcArgs {
 // HydroThermal3D_Alban_v6.jl:87).  Writes r32, adds |r| to the double-double
 // sum (s_hi, s_lo) and to the max mx.  It reads five words and writes half a word per
 // cell, the fused iteration's stencil without the update; a full chunk of KZ
-// planes is unrolled so later planes' loads issue ahead.  Needs nx % CPL == 0.
-template <bool TR, int TY, int KZ, int CPL>
+// planes is unrolled so later planes' loads issue ahead.  With KV the face
+// coefficients are formed from the vertex field a.kv as in iter_tile_kv (the
+// same values bit for bit): three words read per cell instead of five.
+// Needs nx % CPL == 0 (and CPL >= 2 with KV).
+template <bool TR, int TY, int KZ, int CPL, bool KV = false>
 __device__ __forceinline__ void dc_residual_tile(const DcArgs &a, int i, int j, int bz, double &s_hi, double &s_lo,
                                                  double &mx) {
     typedef typename CellVec<double, CPL>::type V;
@@ -1601,7 +1856,24 @@ __device__ __forceinline__ void dc_residual_tile(const DcArgs &a, int i, int j, 
     uint32_t c = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k0 * sC2;
     uint32_t fx = (uint32_t)i + (uint32_t)j * (nx + 1) + (uint32_t)k0 * sX2;
     uint32_t fy = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k0 * sY2;
-    V xc = ldv<double, CPL>(a.X + e), xB = ldv<double, CPL>(a.X + e - sE2), kB = ldv<double, CPL>(a.kz + c);
+    const uint32_t sV1 = nx + 1, sV2 = sV1 * (ny + 1);
+    uint32_t v = (uint32_t)i + (uint32_t)j * sV1 + (uint32_t)k0 * sV2;
+    double a0[CPL + 1], a1[CPL + 1];   // KV: vertex values of plane k, rows j and j + 1
+    auto vrow = [&](uint32_t o, double *out) {
+        const V w = ldv<double, CPL>(a.kv + o);
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) out[q] = w[q];
+        out[CPL] = a.kv[o + CPL];
+    };
+    V xc = ldv<double, CPL>(a.X + e), xB = ldv<double, CPL>(a.X + e - sE2), kB;
+    if constexpr (KV) {
+        vrow(v, a0);
+        vrow(v + sV1, a1);
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) kB[q] = av4(a0[q], a0[q + 1], a1[q], a1[q + 1]);
+    } else {
+        kB = ldv<double, CPL>(a.kz + c);
+    }
     if (k0 == 0) {
 #pragma unroll
         for (int q = 0; q < CPL; ++q) xB[q] = ghost(a.bct[4], a.bcv[4], (double)xc[q]);
@@ -1610,10 +1882,32 @@ __device__ __forceinline__ void dc_residual_tile(const DcArgs &a, int i, int j, 
         V xF = ldv<double, CPL>(a.X + e + sE2);
         double xW = a.X[e - 1], xE = a.X[e + CPL];
         V xS = ldv<double, CPL>(a.X + e - sE1), xN = ldv<double, CPL>(a.X + e + sE1);
-        const double kW = a.kx[fx];
-        const V kE = ldv<double, CPL>(a.kx + fx + 1);
-        const V kS = ldv<double, CPL>(a.ky + fy), kN = ldv<double, CPL>(a.ky + fy + nx);
-        const V kF = ldv<double, CPL>(a.kz + c + sC2);
+        double kW;
+        V kE, kS, kN, kF;
+        if constexpr (KV) {
+            double b0[CPL + 1], b1[CPL + 1];
+            vrow(v + sV2, b0);
+            vrow(v + sV2 + sV1, b1);
+            kW = av4(a0[0], a1[0], b0[0], b1[0]);
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                kE[q] = av4(a0[q + 1], a1[q + 1], b0[q + 1], b1[q + 1]);
+                kS[q] = av4(a0[q], a0[q + 1], b0[q], b0[q + 1]);
+                kN[q] = av4(a1[q], a1[q + 1], b1[q], b1[q + 1]);
+                kF[q] = av4(b0[q], b0[q + 1], b1[q], b1[q + 1]);
+            }
+#pragma unroll
+            for (int q = 0; q <= CPL; ++q) {
+                a0[q] = b0[q];
+                a1[q] = b1[q];
+            }
+        } else {
+            kW = a.kx[fx];
+            kE = ldv<double, CPL>(a.kx + fx + 1);
+            kS = ldv<double, CPL>(a.ky + fy);
+            kN = ldv<double, CPL>(a.ky + fy + nx);
+            kF = ldv<double, CPL>(a.kz + c + sC2);
+        }
         const V R = ldv_nt<double, CPL>(a.R + c);
 #pragma unroll
         for (int q = 0; q < CPL; ++q) {
@@ -1647,6 +1941,7 @@ __device__ __forceinline__ void dc_residual_tile(const DcArgs &a, int i, int j, 
         c += sC2;
         fx += sX2;
         fy += sY2;
+        v += sV2;
     };
     if (k1 - k0 == KZ) {
 #pragma unroll

@@ -245,6 +245,7 @@ int hrl_pre_thermal(ht3d_hrl *m, ht3d_dyrel_field *f) {
     std::memset(f, 0, sizeof *f);
     f->ncx = nx; f->ncy = ny; f->ncz = nz; f->dx = m->dx; f->dy = m->dy; f->dz = m->dz; f->dt = m->dt;
     f->X = m->Tc_ex; f->R = m->Rt; f->kx = m->ktx; f->ky = m->kty; f->kz = m->ktz;
+    f->kv = m->ktv;   // ktx, kty, ktz are its face averages
     f->transient = 1; f->rhoC = 1.0;
     const int tt[6] = {0, 0, 1, 1, 0, 0};
     const double tv[6] = {0.0, 0.0, m->Tbot, m->Ttop, 0.0, 0.0};
@@ -263,6 +264,7 @@ int hrl_pre_darcy(ht3d_hrl *m, ht3d_dyrel_field *f) {
     std::memset(f, 0, sizeof *f);
     f->ncx = nx; f->ncy = ny; f->ncz = nz; f->dx = m->dx; f->dy = m->dy; f->dz = m->dz; f->dt = m->dt;
     f->X = m->Pc_ex; f->R = m->Rp; f->kx = m->kx; f->ky = m->ky; f->kz = m->kz;
+    f->kv = m->mu_a == 0.0 ? m->kfv : nullptr;   // the face averages of kfv, unless divided by mu(T)
     f->transient = 0; f->rhoC = 0.0;
     return HT3D_OK;
 }

@@ -73,14 +73,21 @@ def _status_check(status: int, what: str, report: dict | None = None) -> None:
 def dyrel_solve(X: DeviceArray, R: DeviceArray, kx: DeviceArray, ky: DeviceArray, kz: DeviceArray, *,
                 dx: float, dy: float, dz: float, dt: float, transient: int, rhoC: float,
                 bc_type=(0, 0, 0, 0, 0, 0), bc_val=(0.0,) * 6, params: DyrelParams | None = None,
-                precision: int = _lib.FP64) -> dict:
-    """One DYREL solve on the device; returns the solve report as a dict."""
+                precision: int = _lib.FP64, kv: DeviceArray | None = None) -> dict:
+    """One DYREL solve on the device; returns the solve report as a dict.
+
+    kv: the vertex field whose face averages kx, ky, kz are (InitThermal! /
+    InitDarcy!, kernels_HT3D.jl:82-107); the FP64 solve may then read it
+    instead of the three face arrays, with the same results bit for bit."""
     grid = _grid_from(X, "E")
     for nm, a, k in (("X", X, "E"), ("R", R, "c"), ("kx", kx, "Fx"), ("ky", ky, "Fy"), ("kz", kz, "Fz")):
         _check(a, k, grid, "dyrel_solve", nm)
+    if kv is not None:
+        _check(kv, "V", grid, "dyrel_solve", "kv")
     p = (params or DyrelParams()).c()
     f = _lib.DyrelField(grid[0], grid[1], grid[2], dx, dy, dz, X.ptr, R.ptr, kx.ptr, ky.ptr, kz.ptr, int(transient),
-                        float(rhoC), float(dt), (C.c_int * 6)(*bc_type), (C.c_double * 6)(*bc_val))
+                        float(rhoC), float(dt), (C.c_int * 6)(*bc_type), (C.c_double * 6)(*bc_val),
+                        kv.ptr if kv is not None else None)
     rep = _lib.DyrelReport()
     _lib.ensure_init()
     if precision == _lib.FP64:

@@ -418,6 +418,14 @@ typedef struct {
     double dt;
     int bc_type[6];                     /* x-,x+,y-,y+,z-,z+: 0 Neumann, 1 Dirichlet  */
     double bc_val[6];
+    const double *kv;                   /* V, or NULL: the vertex field whose face
+                                           averages kx, ky, kz are (@av_yza/@av_xza/@av_xya,
+                                           as InitThermal!/InitDarcy! form them,
+                                           kernels_HT3D.jl:82-107).  When given, the FP64
+                                           solve may read it instead of the three face
+                                           arrays (6 instead of 8 words per cell and
+                                           iteration); the results are the same bit for bit
+                                           when kx, ky, kz are those averages.            */
 } ht3d_dyrel_field;
 
 int ht3d_dyrel_solve(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep);
