@@ -7,12 +7,13 @@ velocity, WENO5 advection, CFL time step), inputs resident in HBM.
 
 Prints one JSON line:
   value         T_eff (GB/s) of the fused DYREL pseudo-iterations over the timed
-                steps: algorithmic bytes / summed kernel time from HIP events on
-                the library stream.  The algorithmic bytes are those of the
-                kernel that ran: 8 words per cell per iteration when it reads the
-                three face-coefficient arrays (X and F0 read and written, R, kx,
-                ky, kz read), 6 when it forms them from the vertex field (R and
-                kv read), 7 when it also reads the stored diagonal;
+                steps: algorithmic bytes of the reference's iteration (8 words
+                per cell: X and F0 read and written, R, kx, ky, kz read; the
+                ParallelStencil A_eff) / summed kernel time from HIP events on
+                the library stream.  The kernel that runs may read fewer: with
+                the vertex field it reads kv instead of kx, ky, kz (6 words, 7
+                with the stored diagonal), reported as roofline.kernel_words and
+                roofline.kernel_frac;
   ms_per_step   wall time per physical step (all kernels, reductions and host
                 convergence checks);
   roofline      the iteration kernel against the 8 TB/s HBM peak;
@@ -205,7 +206,8 @@ def run_batch(n: int, steps: int, warmup: int, prm, graphs: bool = True,
                                         for r, z, c in zip(st["tile_rows"], st["tile_z"], st["tile_cpl"])],
                               "grid": ["per-item" if g else "persistent" for g in st["grid_mode"]],
                               "pairs": [f"{r}x{z}" + (f" x{c}" if c > 1 else "") if r else None
-                                        for r, z, c in zip(st["pair_rows"], st["pair_z"], st["pair_cpl"])]})
+                                        for r, z, c in zip(st["pair_rows"], st["pair_z"], st["pair_cpl"])],
+                              "vertex": list(st["vertex"])})
             b.close()
         states[mode] = _batch_state(ms, reps)
         for m in ms:
@@ -547,10 +549,12 @@ def main(argv=None) -> int:
                     "schedule": "nitout 1000, convergence test at iter<=2 or iter%1000==0, mean|F*D|/sqrt(N), "
                                 "nitmax 10000"}
     teff = st["bytes"] / (st["ms"] / 1e3) / 1e9 if st["ms"] > 0 else 0.0
-    # words per cell and iteration of the kernel that ran (variant bit 8: face
-    # coefficients formed from the vertex field, bit 1: stored diagonal read;
-    # pairs always read the faces)
-    words = (7 if st["tile"][2] & 1 else 6) if (st["tile"][2] & 8) and st["pair"] is None else 8
+    # words per cell and iteration that the kernel which ran must move (variant
+    # bit 8: face coefficients formed from the vertex field, bit 1: stored
+    # diagonal read; pairs read the faces, once per two iterations)
+    kwords = (7 if st["tile"][2] & 1 else 6) if st["tile"][2] & 8 else (9 if st["tile"][2] & 1 else 8)
+    if st["pair"] is not None:
+        kwords = 4
     teff32 = st["bytes_fp32"] / (st["ms_fp32"] / 1e3) / 1e9 if st["ms_fp32"] > 0 else None
     avg_us = st["ms"] / max(st["launches"], 1) * 1e3
     iters = [(r["thermal"]["iters"], r["darcy"]["iters"]) for r in reports]
@@ -593,11 +597,11 @@ def main(argv=None) -> int:
         "hbm_frac": round(teff / HBM_PEAK_GBS, 4),
         "iters_per_step": iters,
         "iteration_kernel_avg_us": round(avg_us, 2),
-        "t_eff_8_words": round(8 * 8 * w.ncell / (avg_us * 1e-6) / 1e9, 2) if avg_us > 0 else None,
         "roofline": {"bound": "hbm", "achieved": round(teff, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(teff / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.workload, st["tile"]),
-                     "algorithmic_bytes": words * 8 * w.ncell,
-                     "words_per_cell": words,
+                     "algorithmic_bytes": 8 * 8 * w.ncell,
+                     "kernel_words": kwords,
+                     "kernel_frac": round(teff * kwords / 8 / HBM_PEAK_GBS, 4),
                      "tile": f"{tile_label(st['tile'])} variant {st['tile'][2]}",
                      "pair_tile": None if st["pair"] is None else
                      f"{tile_label(st['pair'])} variant {st['pair'][2]}"},

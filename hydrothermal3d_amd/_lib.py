@@ -72,7 +72,7 @@ class BatchStats(C.Structure):
                 ("graphs", C.c_longlong), ("repacks", C.c_longlong), ("iterations", C.c_longlong),
                 ("tile_rows", C.c_int * 4), ("tile_z", C.c_int * 4), ("tile_cpl", C.c_int * 4),
                 ("grid_mode", C.c_int * 4), ("pair_rows", C.c_int * 4), ("pair_z", C.c_int * 4),
-                ("pair_cpl", C.c_int * 4)]
+                ("pair_cpl", C.c_int * 4), ("vertex", C.c_int * 4)]
 
     def as_dict(self) -> dict:
         d = {}

@@ -91,7 +91,10 @@ __device__ __forceinline__ void item_of(unsigned l, int npatch, int &p, int &bz)
 }
 
 // ---------------------------------------------------------------- kernels
-template <typename T, bool TR, int TY, int KZ, int CPL>
+// KV: a problem whose table has the vertex field (a.kv) runs iter_tile_kv
+// (face coefficients formed from it: 6 words per cell instead of 8, the same
+// results bit for bit); the others run iter_tile_v
+template <typename T, bool TR, int TY, int KZ, int CPL, bool KV>
 __global__ __launch_bounds__(64 * TY) void batch_iter_kernel(const IterArgs<T> *__restrict__ tab,
                                                              const uint2 *__restrict__ items,
                                                              const unsigned *__restrict__ n_items,
@@ -114,6 +117,9 @@ __global__ __launch_bounds__(64 * TY) void batch_iter_kernel(const IterArgs<T> *
         constexpr bool V6 = sizeof(T) == 8;
         if constexpr (CPL == 1)
             iter_tile<T, TR, false, V6, V6, TY, KZ>(a, phase ? x1 : x0, phase ? x0 : x1, flags, i, j, bz, partials, t);
+        else if (KV && a.kv)
+            iter_tile_kv<T, TR, false, V6, false, TY, KZ, CPL>(a, phase ? x1 : x0, phase ? x0 : x1, flags, i, j, bz,
+                                                              partials, t);
         else
             iter_tile_v<T, TR, false, V6, V6, TY, KZ, CPL>(a, phase ? x1 : x0, phase ? x0 : x1, flags, i, j, bz,
                                                          partials, t);
@@ -122,8 +128,9 @@ __global__ __launch_bounds__(64 * TY) void batch_iter_kernel(const IterArgs<T> *
 }
 
 // FP64 anchor residual of the defect-correction form, on the same tiles as the
-// sweeps (dc_residual_tile: the single solve's residual body)
-template <bool TR, int TY, int KZ, int CPL>
+// sweeps (dc_residual_tile: the single solve's residual body; with KV, from the
+// vertex field for the problems that have it)
+template <bool TR, int TY, int KZ, int CPL, bool KV>
 __global__ __launch_bounds__(64 * TY) void batch_dc_residual_kernel(const DcArgs *__restrict__ dtab,
                                                                     const IterArgs<float> *__restrict__ tab32,
                                                                     const uint2 *__restrict__ items,
@@ -139,7 +146,8 @@ __global__ __launch_bounds__(64 * TY) void batch_dc_residual_kernel(const DcArgs
         item_of(it.y, tab32[it.x].nbx, p, bz);
         patch_cell<TY, CPL>(p, dtab[it.x].nx / CPL, i, j);
         double s_hi = 0.0, s_lo = 0.0, mx = 0.0;
-        dc_residual_tile<TR, TY, KZ, CPL>(dtab[it.x], i, j, bz, s_hi, s_lo, mx);
+        if (KV && dtab[it.x].kv) dc_residual_tile<TR, TY, KZ, CPL, KV>(dtab[it.x], i, j, bz, s_hi, s_lo, mx);
+        else dc_residual_tile<TR, TY, KZ, CPL>(dtab[it.x], i, j, bz, s_hi, s_lo, mx);
         const double h4[4] = {s_hi, mx, 0.0, 0.0}, l4[4] = {s_lo, 0.0, 0.0, 0.0};
         block_reduce4(h4, l4, 0x2u, partials, t);
         __syncthreads();
@@ -399,16 +407,26 @@ struct BLaunch {  // device pointers shared by every launch of a solve phase
     unsigned pgrid;
 };
 
+// kv: the vertex-coefficient form (bkv_ok: tiles with 2 cells per lane, and
+// 4 in FP32)
 template <typename T, bool TR>
-static void launch_iter_b(int tile, const BLaunch &L, int phase, unsigned flags) {
+static void launch_iter_b(int tile, const BLaunch &L, int phase, unsigned flags, bool kv) {
     const IterArgs<T> *tab = (const IterArgs<T> *)L.tab;
     hipStream_t s = ctx().stream;
     // 4 cells per lane: FP32 only (btile_ok)
 #define BI_(N, TY, KZ, CPL)                                                                                          \
     case N:                                                                                                          \
-        if constexpr (CPL < 4 || sizeof(T) == 4)                                                                     \
-            hipLaunchKernelGGL((batch_iter_kernel<T, TR, TY, KZ, CPL>), dim3(L.grid), dim3(64 * TY), 0, s, tab,       \
-                               L.items, L.n_items, L.st, L.partials, phase, flags);                                 \
+        if constexpr (CPL > 1 && (CPL < 4 || sizeof(T) == 4)) {                                                      \
+            if (kv)                                                                                                  \
+                hipLaunchKernelGGL((batch_iter_kernel<T, TR, TY, KZ, CPL, true>), dim3(L.grid), dim3(64 * TY), 0, s,  \
+                                   tab, L.items, L.n_items, L.st, L.partials, phase, flags);                        \
+            else                                                                                                     \
+                hipLaunchKernelGGL((batch_iter_kernel<T, TR, TY, KZ, CPL, false>), dim3(L.grid), dim3(64 * TY), 0, s, \
+                                   tab, L.items, L.n_items, L.st, L.partials, phase, flags);                        \
+        } else if constexpr (CPL < 4 || sizeof(T) == 4) {                                                            \
+            hipLaunchKernelGGL((batch_iter_kernel<T, TR, TY, KZ, CPL, false>), dim3(L.grid), dim3(64 * TY), 0, s,     \
+                               tab, L.items, L.n_items, L.st, L.partials, phase, flags);                            \
+        }                                                                                                            \
         break;
     switch (tile) { HT3D_BATCH_TILES(BI_) default: break; }
 #undef BI_
@@ -432,13 +450,17 @@ static void launch_pair_b(int pt, const BLaunch &L, int phase, int fpar, long lo
 }
 
 template <bool TR>
-static void launch_res_b(int tile, const BLaunch &L) {
+static void launch_res_b(int tile, const BLaunch &L, bool kv) {
     const IterArgs<float> *tab = (const IterArgs<float> *)L.tab;
     hipStream_t s = ctx().stream;
 #define BR_(N, TY, KZ, CPL)                                                                                          \
     case N:                                                                                                          \
-        hipLaunchKernelGGL((batch_dc_residual_kernel<TR, TY, KZ, CPL>), dim3(L.grid), dim3(64 * TY), 0, s, L.dtab,   \
-                           tab, L.items, L.n_items, L.st, L.partials);                                              \
+        if (CPL > 1 && kv)                                                                                           \
+            hipLaunchKernelGGL((batch_dc_residual_kernel<TR, TY, KZ, CPL, CPL != 1>), dim3(L.grid), dim3(64 * TY), 0, \
+                               s, L.dtab, tab, L.items, L.n_items, L.st, L.partials);                               \
+        else                                                                                                         \
+            hipLaunchKernelGGL((batch_dc_residual_kernel<TR, TY, KZ, CPL, false>), dim3(L.grid), dim3(64 * TY), 0, s, \
+                               L.dtab, tab, L.items, L.n_items, L.st, L.partials);                                  \
         break;
     switch (tile) { HT3D_BATCH_TILES(BR_) default: break; }
 #undef BR_
@@ -478,11 +500,16 @@ struct ht3d_batch {
     int precision = HT3D_FP64;
     int graphs = 1;
     // per-problem solver buffers, offsets into contiguous regions
-    std::vector<size_t> oc, oe, of;        // cell, E and face offsets (elements)
-    size_t sc = 0, se = 0, sf = 0;         // totals
+    std::vector<size_t> oc, oe, of, ov;    // cell, E, face and vertex offsets (elements)
+    size_t sc = 0, se = 0, sf = 0, sv = 0; // totals
     double *xb = nullptr, *f0 = nullptr, *fst = nullptr, *f0b = nullptr;
     float *e0 = nullptr, *e1 = nullptr, *f032 = nullptr, *fst32 = nullptr, *r32 = nullptr, *k32 = nullptr,
           *f032b = nullptr;  // f0b, f032b: the second F0 buffers of the pair launches
+    float *kv32 = nullptr;   // FP32 vertex fields of the problems that solve with one
+    // vertex-coefficient form per [kind][fp32] (timed like the tiles), and the
+    // one of the launches being enqueued
+    int kvm[2][2] = {{0, 0}, {0, 0}};
+    int kv_cur = 0;
     // tables, per solve kind (0 thermal, 1 Darcy)
     IterArgs<double> *d_it64[2] = {nullptr, nullptr};
     IterArgs<float> *d_it32[2] = {nullptr, nullptr};
@@ -615,7 +642,7 @@ static void fill_tables(ht3d_batch *b, int kind, const std::vector<ht3d_dyrel_fi
         IterArgs<double> &a = b->h_it64[kind][q];
         std::memset(&a, 0, sizeof a);
         a.Xin = d.X; a.Xout = b->xb + b->oe[q];
-        a.R = d.R; a.kx = d.kx; a.ky = d.ky; a.kz = d.kz; a.Dst = nullptr;
+        a.R = d.R; a.kx = d.kx; a.ky = d.ky; a.kz = d.kz; a.kv = d.kv; a.Dst = nullptr;
         a.F0 = b->f0 + b->oc[q]; a.Fst = b->fst + b->oc[q]; a.partials = nullptr;
         a.nx = nx; a.ny = ny; a.nz = nz;
         a.nbx = (int)bpatch_count(nx, ny, tile64); a.nby = 1;   // patches of the (x, y) plane
@@ -632,6 +659,7 @@ static void fill_tables(ht3d_batch *b, int kind, const std::vector<ht3d_dyrel_fi
         g.Xin = b->e0 + b->oe[q]; g.Xout = b->e1 + b->oe[q];
         g.R = b->r32 + b->oc[q];
         g.kx = b->k32 + b->of[q]; g.ky = g.kx + nfx; g.kz = g.ky + nfy; g.Dst = nullptr;
+        g.kv = d.kv ? b->kv32 + b->ov[q] : nullptr;
         g.F0 = b->f032 + b->oc[q]; g.Fst = b->fst32 + b->oc[q]; g.partials = nullptr;
         g.nx = nx; g.ny = ny; g.nz = nz;
         g.nbx = (int)bpatch_count(nx, ny, tile32); g.nby = 1;
@@ -644,7 +672,8 @@ static void fill_tables(ht3d_batch *b, int kind, const std::vector<ht3d_dyrel_fi
 
         DcArgs &c = b->h_dc[kind][q];
         std::memset(&c, 0, sizeof c);
-        c.X = d.X; c.R = d.R; c.kx = d.kx; c.ky = d.ky; c.kz = d.kz; c.r32 = (float *)g.R; c.partials = nullptr;
+        c.X = d.X; c.R = d.R; c.kx = d.kx; c.ky = d.ky; c.kz = d.kz; c.kv = d.kv; c.r32 = (float *)g.R;
+        c.partials = nullptr;
         c.nx = nx; c.ny = ny; c.nz = nz;
         c._dx = a._dx; c._dy = a._dy; c._dz = a._dz; c._dt = a._dt;
         for (int k = 0; k < 6; ++k) { c.bct[k] = d.bc_type[k]; c.bcv[k] = d.bc_val[k]; }
@@ -704,12 +733,12 @@ static int enqueue_ops(ht3d_batch *b, const std::vector<BOp> &ops, int kind, boo
     for (const BOp &o : ops) {
         switch (o.kind) {
         case OP_ITER64:
-            if (transient) launch_iter_b<double, true>(tile, L, o.a, o.flags);
-            else launch_iter_b<double, false>(tile, L, o.a, o.flags);
+            if (transient) launch_iter_b<double, true>(tile, L, o.a, o.flags, b->kv_cur);
+            else launch_iter_b<double, false>(tile, L, o.a, o.flags, b->kv_cur);
             break;
         case OP_ITER32:
-            if (transient) launch_iter_b<float, true>(tile, L, o.a, o.flags);
-            else launch_iter_b<float, false>(tile, L, o.a, o.flags);
+            if (transient) launch_iter_b<float, true>(tile, L, o.a, o.flags, b->kv_cur);
+            else launch_iter_b<float, false>(tile, L, o.a, o.flags, b->kv_cur);
             break;
         case OP_PAIR64:
             if (b->ptile_cur < 0) return arg_fail("batch: pair launch without a pair tile");
@@ -730,8 +759,8 @@ static int enqueue_ops(ht3d_batch *b, const std::vector<BOp> &ops, int kind, boo
                                p.nitmax);
             break;
         case OP_ANCHOR_RES:
-            if (transient) launch_res_b<true>(tile, L);
-            else launch_res_b<false>(tile, L);
+            if (transient) launch_res_b<true>(tile, L, b->kv_cur);
+            else launch_res_b<false>(tile, L, b->kv_cur);
             break;
         case OP_ANCHOR:
             hipLaunchKernelGGL(batch_anchor_kernel, dim3(nq), dim3(1024), 0, s, b->d_state, b->d_it32[kind], b->d_range,
@@ -759,7 +788,8 @@ static int run_block(ht3d_batch *b, const std::vector<BOp> &ops, int kind, bool 
     if (!b->graphs) return enqueue_ops(b, ops, kind, transient, tile, p);
     std::string key;
     key.reserve(ops.size() * 16 + 16);
-    const int hdr[7] = {kind, transient ? 1 : 0, tile, b->precision, b->gmode_cur, b->ptile_cur, b->pgmode_cur};
+    const int hdr[8] = {kind, transient ? 1 : 0, tile, b->precision, b->gmode_cur, b->ptile_cur, b->pgmode_cur,
+                        b->kv_cur};
     key.append((const char *)hdr, sizeof hdr);
     for (const BOp &o : ops) key.append((const char *)&o, sizeof o);
     auto it = b->cache.find(key);
@@ -798,6 +828,17 @@ static bool btile_ok(const ht3d_batch *b, int tile, bool fp32) {
 static int g_force_btile = -1;   // ht3d_batch_tune_force
 static int g_force_bgrid = -1;   // ht3d_batch_tune_force_grid
 static int g_bpairs = -1;        // ht3d_batch_tune_pairs
+static int g_bkv = -1;           // ht3d_batch_tune_vertex
+
+// the vertex-coefficient form can run on batch tile t: 2 cells per lane (and
+// 4 in FP32), and some problem solves with a vertex field
+static bool bkv_ok(int t, bool fp32, const std::vector<ht3d_dyrel_field> &f) {
+    const int cpl = kBTiles[t].cpl;
+    if (!(cpl == 2 || (cpl == 4 && fp32))) return false;
+    for (const ht3d_dyrel_field &d : f)
+        if (d.kv) return true;
+    return false;
+}
 static int g_force_bptile = -1;  // ht3d_batch_tune_force_pair
 
 // The pair tile of a solve kind (-1: none).  Every pair tile that fits all
@@ -878,6 +919,7 @@ static int autotune_batch(ht3d_batch *b, int kind, bool transient, bool fp32, co
     const int pk = fp32 ? 1 : 0;
     if (g_force_btile >= 0 || !g_autotune) {   // a pinned tile the batch cannot run falls back to the default
         *tile_out = (g_force_btile >= 0 && btile_ok(b, g_force_btile, fp32)) ? g_force_btile : kBDefaultTile;
+        b->kvm[kind][pk] = g_bkv > 0 && bkv_ok(*tile_out, fp32, f);
         return autotune_pairs(b, kind, transient, fp32, f, *tile_out, 0.0, false, &b->ptile[kind][pk],
                               &b->pgmode[kind][pk]);
     }
@@ -896,16 +938,19 @@ static int autotune_batch(ht3d_batch *b, int kind, bool transient, bool fp32, co
     HIP_TRY(hipGetLastError());
     std::vector<char> all(b->nprob, 1);
     double best = 1e300;
-    int best_tile = kBDefaultTile, best_gm = 0;
+    int best_tile = kBDefaultTile, best_gm = 0, best_kv = 0;
     ht3d_dyrel_params p{};
-    // every (tile, grid mode) is timed in three interleaved rounds, keeping its
-    // best: one pass ranked tiles a few percent apart by noise
-    std::vector<double> tbest(2 * kNumBTiles, 1e300);
+    // every (tile, grid mode, vertex form) is timed in three interleaved
+    // rounds, keeping its best: one pass ranked tiles a few percent apart by
+    // noise.  tg = 4 tile + 2 kv + grid mode
+    std::vector<double> tbest(4 * kNumBTiles, 1e300);
     for (int round = 0; round < 3; ++round)
-        for (int tg = 0; tg < 2 * kNumBTiles; ++tg) {
-            const int t = tg >> 1;
+        for (int tg = 0; tg < 4 * kNumBTiles; ++tg) {
+            const int t = tg >> 2, kv = (tg >> 1) & 1;
             if (!btile_ok(b, t, fp32) || (g_force_bgrid >= 0 && (tg & 1) != g_force_bgrid)) continue;
+            if (kv ? (g_bkv == 0 || !bkv_ok(t, fp32, f)) : g_bkv == 1 && bkv_ok(t, fp32, f)) continue;
             b->gmode_cur = tg & 1;
+            b->kv_cur = kv;
             fill_tables(b, kind, f, t, t);
             int rc;
             if ((rc = upload_tables(b, kind))) return rc;
@@ -923,15 +968,18 @@ static int autotune_batch(ht3d_batch *b, int kind, bool transient, bool fp32, co
             HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
             if (ms < tbest[tg]) tbest[tg] = ms;
         }
-    for (int tg = 0; tg < 2 * kNumBTiles; ++tg)
+    for (int tg = 0; tg < 4 * kNumBTiles; ++tg)
         if (tbest[tg] < best) {
             best = tbest[tg];
-            best_tile = tg >> 1;
+            best_tile = tg >> 2;
+            best_kv = (tg >> 1) & 1;
             best_gm = tg & 1;
         }
     *tile_out = best_tile;
     *gmode_out = best_gm;
     b->gmode_cur = best_gm;
+    b->kvm[kind][pk] = best_kv;
+    b->kv_cur = best_kv;
     // costs for the check-interval tuner: one sweep, and what a check (FP64) or
     // an anchor with its update (defect correction) adds, on the chosen tile
     {
@@ -1022,12 +1070,17 @@ static int batch_solve(ht3d_batch *b, int kind, const ht3d_dyrel_params &p_in, c
     const bool transient = f[0].transient != 0;
     const bool fp32 = b->precision == HT3D_FP32_DC;
     int rc;
-    if (fp32)  // FP32 face coefficients of the correction equation, as the single-problem solve forms them
-        for (int q = 0; q < b->nprob; ++q)
-            if ((rc = faces_f32(b->k32 + b->of[q], &f[q]))) return rc;
+    if (fp32)  // FP32 face coefficients (and vertex fields) of the correction equation, as the single-problem solve forms them
+        for (int q = 0; q < b->nprob; ++q) {
+            const ht3d_dyrel_field &d = f[q];
+            if ((rc = faces_f32(b->k32 + b->of[q], &d))) return rc;
+            if (d.kv && (rc = to_float(b->kv32 + b->ov[q], d.kv, (size_t)(d.ncx + 1) * (d.ncy + 1) * (d.ncz + 1))))
+                return rc;
+        }
     int &tile = fp32 ? b->tile32[kind] : b->tile64[kind];
     if (tile < 0 && (rc = autotune_batch(b, kind, transient, fp32, f, &tile, &b->gmode[kind][fp32 ? 1 : 0]))) return rc;
     b->gmode_cur = b->gmode[kind][fp32 ? 1 : 0];
+    b->kv_cur = b->kvm[kind][fp32 ? 1 : 0];
     b->ptile_cur = b->ptile[kind][fp32 ? 1 : 0];
     b->pgmode_cur = b->pgmode[kind][fp32 ? 1 : 0];
     ht3d_dyrel_params p = p_in;
@@ -1258,8 +1311,9 @@ int ht3d_batch_create(ht3d_hrl *const *models, int nprob, ht3d_batch **out) {
         const size_t c = (size_t)m->nx * m->ny * m->nz, E = (size_t)(m->nx + 2) * (m->ny + 2) * (m->nz + 2);
         const size_t F = (size_t)(m->nx + 1) * m->ny * m->nz + (size_t)m->nx * (m->ny + 1) * m->nz +
                          (size_t)m->nx * m->ny * (m->nz + 1);
-        b->oc.push_back(b->sc); b->oe.push_back(b->se); b->of.push_back(b->sf);
-        b->sc += round16(c, 8); b->se += round16(E, 8); b->sf += round16(F, 8);
+        const size_t V = (size_t)(m->nx + 1) * (m->ny + 1) * (m->nz + 1);
+        b->oc.push_back(b->sc); b->oe.push_back(b->se); b->of.push_back(b->sf); b->ov.push_back(b->sv);
+        b->sc += round16(c, 8); b->se += round16(E, 8); b->sf += round16(F, 8); b->sv += round16(V, 8);
         b->g_off.push_back((unsigned)gsum); b->g_nb.push_back(gershgorin_blocks(c)); gsum += b->g_nb.back();
         b->nu_off.push_back((unsigned)nusum); b->nu_nb.push_back(nu_blocks(m->nx, m->nz)); nusum += b->nu_nb.back();
     }
@@ -1297,6 +1351,7 @@ int ht3d_batch_create(ht3d_hrl *const *models, int nprob, ht3d_batch **out) {
     A((void **)&b->fst32, b->sc * sizeof(float));
     A((void **)&b->r32, b->sc * sizeof(float));
     A((void **)&b->k32, b->sf * sizeof(float));
+    A((void **)&b->kv32, b->sv * sizeof(float));
     for (int k = 0; k < 2; ++k) {
         A((void **)&b->d_it64[k], nprob * sizeof(IterArgs<double>));
         A((void **)&b->d_it32[k], nprob * sizeof(IterArgs<float>));
@@ -1465,6 +1520,7 @@ int ht3d_batch_stats_get(const ht3d_batch *b, ht3d_batch_stats *s) {
     s->tile_z[3] = b->tile32[1] >= 0 ? kBTiles[b->tile32[1]].kz : 0;
     for (int q = 0; q < 4; ++q) {
         s->grid_mode[q] = b->gmode[q & 1][q >> 1];
+        s->vertex[q] = b->kvm[q & 1][q >> 1];
         const int pt = b->ptile[q & 1][q >> 1];
         s->pair_rows[q] = pt >= 0 ? kBPairTiles[pt].ty : 0;
         s->pair_z[q] = pt >= 0 ? kBPairTiles[pt].kz : 0;
@@ -1483,6 +1539,14 @@ int ht3d_batch_tune_force(int tile) {
 }
 
 int ht3d_batch_tile_count(void) { return kNumBTiles; }
+
+int ht3d_batch_tune_vertex(int mode) {
+    HT3D_API_BEGIN
+    if (mode < -1 || mode > 1) return arg_fail("ht3d_batch_tune_vertex: mode must be -1, 0 or 1");
+    g_bkv = mode;
+    return HT3D_OK;
+    HT3D_API_END
+}
 
 int ht3d_batch_tune_force_grid(int mode) {
     HT3D_API_BEGIN

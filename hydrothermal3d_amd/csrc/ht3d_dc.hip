@@ -374,7 +374,7 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
     double nF_abs = 0.0, nF_rel = 0.0, nF_ini = 0.0;
     int status = HT3D_NOT_CONVERGED, outer = 0, inner_total = 0, n_checks = 0, n_lambda = 0;
     SegmentTimer timer;
-    const double bytes32 = ((cfg.variant & 8) && pcfg.tile < 0 ? 6.0 : 8.0) * (double)N * sizeof(float);
+    const double bytes32 = 8.0 * (double)N * sizeof(float);   // the reference iteration's 8 words per cell
     HIP_TRY(hipMemsetAsync(e0, 0, NE * sizeof(float), cx.stream));
     HIP_TRY(hipMemsetAsync(e1, 0, NE * sizeof(float), cx.stream));
     HIP_TRY(hipMemsetAsync(f0, 0, N * sizeof(float), cx.stream));

@@ -296,12 +296,11 @@ int dyrel_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d
     SegmentTimer timer;
     last_cfg() = cfg;
     last_pair() = pcfg;
-    // algorithmic bytes per iteration: X, F0 read and written, R and the face
-    // coefficients read (8 words per cell), or R and the vertex field (6 words,
-    // 7 with the stored diagonal) when the single launches form the
-    // coefficients from it and no pairs run
-    const double words = (cfg.variant & 8) && pcfg.tile < 0 ? (cfg.variant & 1 ? 7.0 : 6.0) : 8.0;
-    const double bytes_per_iter = words * (double)N * sizeof(double);
+    // algorithmic bytes per iteration of the reference's DYREL iteration: X, F0
+    // read and written, R and the face coefficients kx, ky, kz read (8 words
+    // per cell), whatever the launch reads (pairs cross HBM once per two
+    // iterations; the vertex-coefficient kernels read kv instead of the faces)
+    const double bytes_per_iter = 8.0 * (double)N * sizeof(double);
     double nF_abs = 0.0, nF_rel = 0.0, nF_ini = 0.0;
     int status = HT3D_NOT_CONVERGED, iter, n_checks = 0, n_lambda = 0, done_iters = 0;
     auto plain = [&](int it) {   // no check, no stored F, no lambda update

@@ -288,6 +288,8 @@ ht3d_batch_tile_count() =
     ccall((:ht3d_batch_tile_count, libht3d), Cint, ())
 ht3d_batch_tune_force_grid(mode) =
     ccall((:ht3d_batch_tune_force_grid, libht3d), Cint, (Cint,), mode)
+ht3d_batch_tune_vertex(mode) =
+    ccall((:ht3d_batch_tune_vertex, libht3d), Cint, (Cint,), mode)
 ht3d_batch_tune_pairs(mode) =
     ccall((:ht3d_batch_tune_pairs, libht3d), Cint, (Cint,), mode)
 ht3d_batch_tune_force_pair(tile) =

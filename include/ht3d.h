@@ -592,6 +592,8 @@ typedef struct {
     int pair_rows[4], pair_z[4];/* pair tiles (two iterations per launch) in the same order;
                                    0: the solve kind runs single iterations only          */
     int pair_cpl[4];            /* their x cells per lane                                 */
+    int vertex[4];              /* 1: the single-iteration launches form the face
+                                   coefficients from the vertex field (ht3d_batch_tune_vertex) */
 } ht3d_batch_stats;
 
 int ht3d_batch_create(ht3d_hrl *const *models, int nprob, ht3d_batch **out);
@@ -616,6 +618,16 @@ int ht3d_batch_tile_count(void);
  * item.  Pins it for batches tuned afterwards (-1: time both).  Results are
  * the same bit for bit.  No reference counterpart.                           */
 int ht3d_batch_tune_force_grid(int mode);
+/* The vertex-coefficient form of the batched single-iteration launches: for
+ * the problems whose solve has a vertex field (ht3d_dyrel_field.kv, the model's
+ * ktv, and kfv without viscosity), the face coefficients are formed from it in
+ * registers (6 instead of 8 words per cell and iteration, the FP64 anchor
+ * residual of the defect correction 3 instead of 5 words read).  mode -1
+ * (default): timed against the face-array form per solve kind and precision;
+ * 0: never; 1: always, on tiles with 2 cells per lane (and 4 in FP32).
+ * Results are the same bit for bit.  Acts on batches tuned afterwards.  No
+ * reference counterpart.                                                     */
+int ht3d_batch_tune_vertex(int mode);
 /* Two pseudo-iterations per launch for the runs of plain iterations (no check,
  * lambda update or stored F): bands of whole rows of a problem, 64*rows/ncx
  * rows each, overlapping by one row, with the intermediate iterate in LDS.  A

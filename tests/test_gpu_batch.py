@@ -191,6 +191,35 @@ def test_batch_grid_modes_bitwise(ht, precision):
 
 
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32dc"])
+def test_batch_vertex_bitwise(ht, precision):
+    """The vertex-coefficient form of the batched launches (face coefficients
+    formed from each problem's vertex field; the problem with a viscosity keeps
+    its face arrays for the Darcy solve) against the face-array form, on a
+    two- and a four-cell tile: identical reports and fields."""
+    res = []
+    tile = 7 if precision == 0 else 10   # 4 x 8 planes, 2 (FP64) or 4 (FP32) cells per lane
+    try:
+        for mode in (0, 1):
+            ht._lib.call("ht3d_batch_tune_force", tile)
+            ht._lib.call("ht3d_batch_tune_vertex", mode)
+            reps, out, stats = _run(ht, precision, graphs=True, nsteps=2, sizes=EVEN)
+            assert stats["vertex"][2 * precision:2 * precision + 2] == [mode, mode], stats["vertex"]
+            res.append((reps, out))
+    finally:
+        ht._lib.call("ht3d_batch_tune_force", -1)
+        ht._lib.call("ht3d_batch_tune_vertex", -1)
+    (ra, fa), (rb, fb) = res
+    for sa, sb in zip(ra, rb):
+        for a, c in zip(sa, sb):
+            assert a["thermal"] == c["thermal"] and a["darcy"] == c["darcy"] and a["Nu"] == c["Nu"]
+    for a, c in zip(fa, fb):
+        for k in a:
+            assert np.array_equal(a[k], c[k]), k
+    with pytest.raises(ht._lib.HT3DError):
+        ht._lib.call("ht3d_batch_tune_vertex", 2)
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32dc"])
 def test_batch_pairs_bitwise(ht, precision):
     """Two pseudo-iterations per launch on bands of whole rows, one, two or
     (FP32) four x-cells per lane (each pair tile pinned in turn,
