@@ -449,7 +449,8 @@ def main(argv=None) -> int:
     ap.add_argument("--config0-steps", type=int, default=20,
                     help="physical steps of the 32^3 configuration run in full on CPU and GPU (0: skip)")
     ap.add_argument("--batch", type=int, default=96, help="problems of the batched leg (0: skip it)")
-    ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--batch-steps", type=int, default=20,
+                    help="timed physical steps of the batched leg (the 96-problem workload is defined over 20)")
     ap.add_argument("--batch-tol", type=float, default=1e-8,
                     help="DYREL tolerance of the batched leg (the configurations' 1e-8 by default)")
     ap.add_argument("--batch-warmup", type=int, default=1)

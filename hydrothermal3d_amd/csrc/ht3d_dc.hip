@@ -404,7 +404,8 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
         if ((rc = band_setup(bt, (long long)Npad, WS_BAND, &band))) return rc;
     }
     auto pair_launch = [&](const float *F0in) {
-        return banded ? launch_band_pair(band, pcfg.tile, f->transient != 0, ap.Xin, F0in, ap.flags)
+        return banded ? launch_band_pair(band, pcfg.tile, f->transient != 0, ap.Xin, F0in, ap.flags,
+                                         (pcfg.variant & 8) != 0)
                       : launch_pair(ap, F0in, f->transient != 0, pcfg);
     };
     int cur = 0, fc = 0;

@@ -484,7 +484,7 @@ int ht3d_tune_last_pair_tile(int *rows_per_block, int *z_per_block, int *x_per_t
 
 int ht3d_tune_force_pair(int tile, int variant) {
     HT3D_API_BEGIN
-    if (tile >= ht3d::kNumPairTiles || variant < 0 || variant > 3)
+    if (tile >= ht3d::kNumPairTiles || variant < 0 || variant > 15)
         return ht3d::arg_fail("ht3d_tune_force_pair: no such pair tile or variant");
     ht3d::g_force_pair_tile = tile;
     ht3d::g_force_pair_variant = variant;

@@ -1279,7 +1279,10 @@ __global__ __launch_bounds__(64 * TY) void batch_pair_kernel(const IterArgs<T> *
 // stay in registers; a lane's first cell is at x = CPL * (lane mod gx), so a
 // row's first and last lanes meet the walls.  Same operations in the same
 // order as the single iteration: bit-identical X and F0.  Needs nx % CPL == 0.
-template <typename T, bool TR, int TY, int KZ, int CPL, typename S>
+// KV: iteration n+1 forms the face coefficients from the vertex field a.kv,
+// as iter_tile_kv does (the same values bit for bit); every item of the
+// launch must have one.
+template <typename T, bool TR, int TY, int KZ, int CPL, typename S, bool KV = false>
 __global__ __launch_bounds__(64 * TY) void batch_pair_kernel_v(const IterArgs<T> *__restrict__ tab,
                                                                const uint2 *__restrict__ items,
                                                                const unsigned *__restrict__ n_items,
@@ -1333,7 +1336,26 @@ __global__ __launch_bounds__(64 * TY) void batch_pair_kernel_v(const IterArgs<T>
 #pragma unroll
             for (int q = 0; q < CPL; ++q) x0m[q] = ghost(a.bct[4], a.bcv[4], (T)x0c[q]);
         }
-        V kzB = ld(kzg + c);
+        // KV: vertex values of plane p on rows jc and jc + 1 (iter_tile_kv)
+        const uint32_t sV1 = nx + 1, sV2 = sV1 * (ny + 1);
+        uint32_t v = (uint32_t)i + (uint32_t)jc * sV1 + (uint32_t)ps * sV2;
+        GCT *__restrict__ kvg = (GCT *)a.kv;
+        T a0[CPL + 1], a1[CPL + 1];
+        auto vrow = [&](uint32_t o, T *out) {
+            const V w = ld(kvg + o);
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) out[q] = w[q];
+            out[CPL] = kvg[o + CPL];
+        };
+        V kzB;
+        if constexpr (KV) {
+            vrow(v, a0);
+            vrow(v + sV1, a1);
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) kzB[q] = av4(a0[q], a0[q + 1], a1[q], a1[q + 1]);
+        } else {
+            kzB = ld(kzg + c);
+        }
         // plane p-1 as iteration n+1 left it: the iterate, its neighbours in z,
         // and the coefficients, R, D and F0 the n+2 update needs
         V x1m = {}, x1c = {}, pkE = {}, pkS = {}, pkN = {}, pkB = {}, pkF = {}, pr = {}, pD = {}, pf0 = {};
@@ -1373,10 +1395,32 @@ __global__ __launch_bounds__(64 * TY) void batch_pair_kernel_v(const IterArgs<T>
                 V xp = ld(Xin + e + sE2);
                 T xW = Xin[e - 1], xE = Xin[e + CPL];
                 V xS = ld(Xin + e - sE1), xN = ld(Xin + e + sE1);
-                const T kW = kxg[fx];
-                const V kE = ld(kxg + fx + 1);
-                const V kS = ld(kyg + fy), kN = ld(kyg + fy + nx);
-                const V kF = ld(kzg + c + sC2);
+                T kW;
+                V kE, kS, kN, kF;
+                if constexpr (KV) {
+                    T b0[CPL + 1], b1[CPL + 1];
+                    vrow(v + sV2, b0);
+                    vrow(v + sV2 + sV1, b1);
+                    kW = av4(a0[0], a1[0], b0[0], b1[0]);
+#pragma unroll
+                    for (int q = 0; q < CPL; ++q) {
+                        kE[q] = av4(a0[q + 1], a1[q + 1], b0[q + 1], b1[q + 1]);
+                        kS[q] = av4(a0[q], a0[q + 1], b0[q], b0[q + 1]);
+                        kN[q] = av4(a1[q], a1[q + 1], b1[q], b1[q + 1]);
+                        kF[q] = av4(b0[q], b0[q + 1], b1[q], b1[q + 1]);
+                    }
+#pragma unroll
+                    for (int q = 0; q <= CPL; ++q) {
+                        a0[q] = b0[q];
+                        a1[q] = b1[q];
+                    }
+                } else {
+                    kW = kxg[fx];
+                    kE = ld(kxg + fx + 1);
+                    kS = ld(kyg + fy);
+                    kN = ld(kyg + fy + nx);
+                    kF = ld(kzg + c + sC2);
+                }
                 const V kB = kzB;
                 const V rr = ld_nt(Rg + c);
                 const V f0o = ld_nt(F0in + c);
@@ -1415,6 +1459,7 @@ __global__ __launch_bounds__(64 * TY) void batch_pair_kernel_v(const IterArgs<T>
                 c += sC2;
                 fx += sX2;
                 fy += sY2;
+                v += sV2;
             } else {         // past the top plane: iteration n+2 on plane nz-1 of the top chunk
                 __syncthreads();
                 if (p - 1 >= k0 && p - 1 < k1 && inner) {
@@ -1480,9 +1525,11 @@ inline int band_setup(const IterArgs<T> &a, long long dF0, int slot, BandCtx<T> 
 // Two iterations on bands of whole rows (pair tiles 11-15): from Xin (one of
 // the table's two iterate buffers) and F0in (one of its two F0 buffers) into
 // the other two.
+// kv: the kernels that form the coefficients from the table's vertex field
+// (tiles with 2 or 4 cells per lane)
 template <typename T>
 inline int launch_band_pair(const BandCtx<T> &b, int tile, bool transient, const T *Xin, const T *F0in,
-                            unsigned flags) {
+                            unsigned flags, bool kv = false) {
     const int phase = Xin == b.x0 ? 0 : 1, fpar = F0in == b.f0 ? 0 : 1;
     const unsigned fl = flags & FL_REVERSE;
     hipStream_t st = ctx().stream;
@@ -1492,6 +1539,9 @@ inline int launch_band_pair(const BandCtx<T> &b, int tile, bool transient, const
         if constexpr (CPL == 1)                                                                                      \
             hipLaunchKernelGGL((batch_pair_kernel<T, TR, TY, KZ, OneActive>), dim3(b.n), dim3(64 * TY), 0, st,       \
                                b.tab, b.items, b.n_items, b.st, phase, fpar, b.dF0, fl);                            \
+        else if (kv)                                                                                                 \
+            hipLaunchKernelGGL((batch_pair_kernel_v<T, TR, TY, KZ, CPL, OneActive, true>), dim3(b.n), dim3(64 * TY), \
+                               0, st, b.tab, b.items, b.n_items, b.st, phase, fpar, b.dF0, fl);                     \
         else                                                                                                         \
             hipLaunchKernelGGL((batch_pair_kernel_v<T, TR, TY, KZ, CPL, OneActive>), dim3(b.n), dim3(64 * TY), 0,    \
                                st, b.tab, b.items, b.n_items, b.st, phase, fpar, b.dF0, fl);                        \
@@ -1739,10 +1789,13 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
                 if (us < te.pair_us) { te.pair_us = us; te.pair = pc; }
             }
         }
-        // band tiles (FP32): F0b must follow a.F0 in the same allocation
+        // band tiles (FP32): F0b must follow a.F0 in the same allocation; with
+        // the vertex field also their vertex-coefficient form (variant 8)
         if (sizeof(T) == 4 && F0b > a.F0 && !small) {
-            for (int tile = 0; tile < kNumPairTiles; ++tile) {
+            for (int tk = 0; tk < 2 * kNumPairTiles; ++tk) {
+                const int tile = tk >> 1, kvv = tk & 1;
                 if (!kPairTiles[tile].band || !pair_tile_ok<T>(tile, a.nx)) continue;
+                if (kvv && (!a.kv || kPairTiles[tile].cpl < 2)) continue;
                 IterArgs<T> b = a;
                 b.Xin = xs[0];
                 b.Xout = xs[1];
@@ -1757,14 +1810,14 @@ inline int autotune(IterArgs<T> a, bool transient, bool can_store_d, int prec, T
                     const T *xin = xs[pp], *f0in = fp ? F0b : a.F0;
                     pp ^= 1;
                     fp ^= 1;
-                    return launch_band_pair(bc, tile, transient, xin, f0in, fl);
+                    return launch_band_pair(bc, tile, transient, xin, f0in, fl, kvv != 0);
                 }, &us);
                 if (rc) return rc;
                 us *= 0.5;
                 if (tune_verbose())
-                    fprintf(stderr, "[ht3d tune] %dx%dx%d %s prec %d: band pair tile %d %.2f us per iteration\n", a.nx,
-                            a.ny, a.nz, transient ? "heat" : "darcy", prec, tile, us);
-                if (us < te.pair_us) { te.pair_us = us; te.pair = PairCfg{tile, 0}; }
+                    fprintf(stderr, "[ht3d tune] %dx%dx%d %s prec %d: band pair tile %d%s %.2f us per iteration\n",
+                            a.nx, a.ny, a.nz, transient ? "heat" : "darcy", prec, tile, kvv ? " (vertex)" : "", us);
+                if (us < te.pair_us) { te.pair_us = us; te.pair = PairCfg{tile, kvv ? 8 : 0}; }
             }
         }
     }
@@ -1796,11 +1849,14 @@ inline int choose_cfg(IterArgs<T> &a, bool transient, bool can_store_d, int prec
         kv_normalize(a, c);
         if (te.pair.tile >= 0 && (g_pairs == 2 || (g_pairs == 1 && te.pair_us < te.us))) pc = te.pair;
         if (!can_store_d) pc.variant &= ~1;
+        if (!a.kv) pc.variant &= ~8;
     }
     if (g_force_pair_tile >= 0) {   // pinned: that tile, or no pairs where it cannot run
         const bool ok = pair_tile_ok<T>(g_force_pair_tile, a.nx) && a.ny >= 3;
         pc.tile = ok ? g_force_pair_tile : -1;
-        pc.variant = kPairTiles[g_force_pair_tile].band ? 0 : g_force_pair_variant & (can_store_d ? 3 : 2);
+        pc.variant = kPairTiles[g_force_pair_tile].band
+                         ? ((g_force_pair_variant & 8) && a.kv && kPairTiles[g_force_pair_tile].cpl >= 2 ? 8 : 0)
+                         : g_force_pair_variant & (can_store_d ? 3 : 2);
     }
     if (!F0b || g_pairs == 0) pc.tile = -1;
     set_tiling(a, c.tile);

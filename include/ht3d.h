@@ -121,10 +121,11 @@ int ht3d_tune_last_pair(int *rows_per_block, int *z_per_block, int *variant);
  * divisible by that number; FP32 only for 4).  ht3d_tune_last_pair_tile also
  * reports the cells per lane (0 when no pair tile was used).
  * ht3d_tune_force_pair pins pair tile `tile` (< 0: none) and variant (bit 0
- * stored diagonal, bit 1 cached coefficients) for the plain runs of every
- * solve, whatever the timings; a solve whose grid the tile does not fit runs
- * without pairs, and pairs mode 0 still turns pairs off.  No reference
- * counterpart.                                                               */
+ * stored diagonal, bit 1 cached coefficients; on the band tiles of the FP32
+ * sweeps bit 3: coefficients from the vertex field, when the solve has one)
+ * for the plain runs of every solve, whatever the timings; a solve whose grid
+ * the tile does not fit runs without pairs, and pairs mode 0 still turns pairs
+ * off.  No reference counterpart.                                            */
 int ht3d_tune_last_pair_tile(int *rows_per_block, int *z_per_block, int *x_per_thread, int *variant);
 int ht3d_tune_force_pair(int tile, int variant);
 int ht3d_pair_tile_count(void);

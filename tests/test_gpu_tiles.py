@@ -191,3 +191,47 @@ def test_vertex_coefficients_dc_bitwise(ht, grid, transient):
     m[1:-1, 1:-1, 1:-1] = True
     err = np.max(np.abs(X0[m] - X64[m])) / np.max(np.abs(X64[m]))
     assert err <= 1e-6, err
+
+
+BAND_TILES = {11: 4, 12: 4, 13: 4, 14: 2}   # band pair tiles with 2 or 4 cells per lane: cells per lane
+
+
+@pytest.mark.parametrize("transient", [0, 1], ids=["darcy", "heat"])
+@pytest.mark.parametrize("grid", [(128, 37, 45), (256, 20, 19)], ids=lambda g: "x".join(map(str, g)))
+def test_vertex_coefficients_band_pairs_bitwise(ht, grid, transient):
+    """The FP32 band pairs of the defect correction with the coefficients formed
+    from the vertex field (pair variant bit 8) against the same tiles on the
+    FP32 face arrays and against single launches only: identical solves."""
+    r0, X0, _ = _kv_solve_pairs(ht, grid, transient, None)
+    for tile, cpl in BAND_TILES.items():
+        for variant in (0, 8):
+            r, X, (xc, var) = _kv_solve_pairs(ht, grid, transient, (tile, variant))
+            assert (xc, var) == (cpl, variant), (tile, xc, var)
+            for key in ("iters", "status", "n_checks", "n_lambda", "dc_outer", "nF_abs", "nF_rel"):
+                assert r0[key] == r[key], (tile, variant, key, r0[key], r[key])
+            assert np.array_equal(X0, X), (tile, variant)
+
+
+def _kv_solve_pairs(ht, grid, transient, pair):
+    """DC solve with the vertex field: single launches only (pair None) or the
+    band pair tile/variant pinned; returns the report, X and the pair tile's
+    (cells per lane, variant)."""
+    X, R, k, kv = _kv_problem(grid, transient)
+    if pair is None:
+        ht._lib.call("ht3d_tune_pairs", 0)
+    else:
+        ht._lib.call("ht3d_tune_force_pair", *pair)
+    try:
+        D = [ht.DeviceArray.from_host(a) for a in (X, R, *k)]
+        dkv = ht.DeviceArray.from_host(kv)
+        h = dict(dx=1.0 / grid[0], dy=0.5 / grid[1], dz=1.3 / grid[2], dt=1e-3)
+        bc = dict(bc_type=(0, 0, 1, 1, 0, 0), bc_val=(0.0, 0.0, 1.0, 0.0, 0.0, 0.0)) if transient else {}
+        prm = ht.DyrelParams(nitmax=20000, nitout=100, ncheck=25, norm=1, tol=1e-10)
+        rep = ht.dyrel_solve(*D, transient=transient, rhoC=float(transient), params=prm, kv=dkv, precision=1,
+                             **h, **bc)
+        ty, kz, xc, var = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        ht._lib.call("ht3d_tune_last_pair_tile", C.byref(ty), C.byref(kz), C.byref(xc), C.byref(var))
+        return rep, D[0].get(), (xc.value, var.value)
+    finally:
+        ht._lib.call("ht3d_tune_pairs", 1)
+        ht._lib.call("ht3d_tune_force_pair", -1, 0)
