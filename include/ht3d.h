@@ -94,7 +94,11 @@ int ht3d_timing_reset(void);
  * variant: bit 0 reads the stored Gershgorin diagonal instead of recomputing it
  * from the face coefficients (bit-identical either way), bit 1 writes the
  * updated unknown with a non-temporal store, bit 2 lets the face-coefficient
- * loads be cached.  With autotuning on (default) the
+ * loads be cached; bit 3 forms the face coefficients from the solve's vertex
+ * field (ht3d_dyrel_field.kv; tiles with 2 cells per lane, and 4 in FP32;
+ * there bit 0 applies in FP64 only and bit 2 means a rolled z march, in FP32
+ * only); a vertex bit the solve cannot honour is dropped.  With autotuning on
+ * (default) the
  * first solve of each (grid, operator, precision) times every configuration on
  * its own inputs and caches the fastest; force_tile >= 0 pins a table entry and
  * variant (a 2-cell entry on an odd ncx runs as its 1-cell twin).
