@@ -56,7 +56,7 @@ class V18Field(C.Structure):
                 ("PC", C.c_void_p), ("G", C.c_void_p), ("top", C.c_double), ("norm_scale", C.c_double),
                 ("Tc_old", C.c_void_p), ("qyS", C.c_double), ("Qt", C.c_double), ("Tc_ex", C.c_void_p),
                 ("phirhof0", C.c_void_p), ("rhofv", C.c_void_p), ("g", C.c_double), ("phi_i", C.c_double),
-                ("scale_rho", C.c_double), ("scale_T", C.c_double), ("scale_sigma", C.c_double)]
+                ("scale_rho", C.c_double), ("scale_T", C.c_double), ("scale_sigma", C.c_double), ("kv", C.c_void_p)]
 
 
 class StepReport(C.Structure):

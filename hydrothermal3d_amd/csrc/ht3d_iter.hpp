@@ -425,9 +425,10 @@ __device__ __forceinline__ T av4(T p, T q, T r, T s) {
 // Every vertex index stays inside the vertex array (i + CPL <= nx, j + 1 <= ny,
 // k + 1 <= nz).  DSTORED reads the Gershgorin diagonal stored once per solve
 // (one word more, but none of the nine FP64 divisions that recomputing it
-// costs).  ROLL keeps the z march a rolled loop: unrolled, the compiler hoists
-// later planes' loads and the FP64 two-cell kernel needs 151 VGPRs (3 waves
-// per SIMD) instead of 122 (4 waves).  Needs nx % CPL == 0, CPL >= 2.
+// costs).  ROLL unrolls a chunk's z march by two planes instead of all KZ:
+// fully unrolled, the compiler hoists later planes' loads and the FP32
+// four-cell kernel needs 137 VGPRs (3 waves per SIMD), by two 127 (4 waves).
+// Needs nx % CPL == 0, CPL >= 2.
 template <typename T, bool TRANSIENT, bool DSTORED, bool NTX, bool ROLL, int TY, int KZ, int CPL>
 __device__ __forceinline__ void iter_tile_kv(const IterArgs<T> &a, const T *__restrict__ Xin, T *__restrict__ Xout,
                                              unsigned flags, int i, int j, int bz, double *__restrict__ partials,
@@ -542,6 +543,9 @@ __device__ __forceinline__ void iter_tile_kv(const IterArgs<T> &a, const T *__re
         };
         if (!ROLL && k1 - k0 == KZ) {
 #pragma unroll
+            for (int q = 0; q < KZ; ++q) plane(k0 + q);
+        } else if (k1 - k0 == KZ) {
+#pragma unroll 2
             for (int q = 0; q < KZ; ++q) plane(k0 + q);
         } else {
             for (int k = k0; k < k1; ++k) plane(k);
@@ -881,8 +885,8 @@ inline void launch_tile(int tile, const IterArgs<T> &a, unsigned nb, hipStream_t
 //               2 = non-temporal store of the updated unknown,
 //               4 = cached (not streamed) face-coefficient loads,
 //               8 = face coefficients from the vertex field (dyrel_iter_kv_kernel;
-//                   with it bit 1 applies in FP64 only, and bit 4 means a
-//                   rolled z march, in FP32 only)
+//                   with it bit 1 applies in FP64 only, and bit 4 means a z
+//                   march unrolled by two planes, in FP32 only)
 struct IterCfg {
     int tile = kDefaultTile;
     int variant = 0;

@@ -58,6 +58,7 @@ struct Args {
     const double *__restrict__ kx;
     const double *__restrict__ ky;
     const double *__restrict__ kz;
+    const double *__restrict__ kv;     // vertex field whose face averages kx, ky, kz are (KV kernels)
     const uint8_t *__restrict__ cls;
     double *__restrict__ F0;
     double *__restrict__ Fst;
@@ -194,8 +195,12 @@ __device__ __forceinline__ double v18_cell(const Args &a, uint8_t cl, int i, int
 // covers 64*CPL x TY cells marched over KZ planes.  With CPL = 2 every stream
 // comes in as 2-element vector loads (nx even); the loads of the neighbours,
 // coefficients and operator streams are skipped when all the lane's cells are
-// air or immersed (F = 0).
-template <bool DARCY, int TY, int KZ, int CPL>
+// air or immersed (F = 0).  KV (CPL = 2): the face coefficients are formed
+// from the vertex field a.kv (kx, ky, kz = InitConductivity! of it,
+// kernels_HT3D_dimensional.jl:133-138) with the averaging kernel's additions
+// (av4, as iter_tile_kv does), one vertex word per cell instead of three face
+// words; its rows are loaded on every plane, air or not, to carry the march.
+template <bool DARCY, int TY, int KZ, int CPL, bool KV = false>
 __global__ __launch_bounds__(64 * TY) void iter_kernel(Args a) {
     typedef typename CellVec<double, CPL>::type V;
     const unsigned nblocks = (unsigned)a.nbx * a.nby * a.nbz;
@@ -216,16 +221,42 @@ __global__ __launch_bounds__(64 * TY) void iter_kernel(Args a) {
         uint32_t c = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k0 * sC2;
         uint32_t fx = (uint32_t)i + (uint32_t)j * sX1 + (uint32_t)k0 * sX2;
         uint32_t fy = (uint32_t)i + (uint32_t)j * nx + (uint32_t)k0 * sY2;
+        const uint32_t sV1 = nx + 1, sV2 = sV1 * (ny + 1);
+        uint32_t v = (uint32_t)i + (uint32_t)j * sV1 + (uint32_t)k0 * sV2;
+        double a0[CPL + 1], a1[CPL + 1];   // KV: vertex values of plane k, rows j and j + 1
+        auto vrow = [&](uint32_t o, double *out) {
+            const V w = ldv<double, CPL>(a.kv + o);
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) out[q] = w[q];
+            out[CPL] = a.kv[o + CPL];
+        };
         V xc = ldv<double, CPL>(a.Xin + e);
         V xm = ldv<double, CPL>(a.Xin + e - sE2);
-        V kzB = ldv<double, CPL>(a.kz + c);
+        V kzB;
+        if constexpr (KV) {
+            vrow(v, a0);
+            vrow(v + sV1, a1);
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) kzB[q] = av4(a0[q], a0[q + 1], a1[q], a1[q + 1]);
+        } else {
+            kzB = ldv<double, CPL>(a.kz + c);
+        }
         // the Darcy loop unrolled by 4 (the next planes' loads issue ahead of this
         // plane's arithmetic: 1 % faster); the heat loop measured no gain
         constexpr int kUnrollZ = DARCY ? 4 : 1;
 #pragma unroll kUnrollZ
         for (int k = k0; k < k1; ++k) {
             const V xp = ldv<double, CPL>(a.Xin + e + sE2);
-            const V kF = ldv<double, CPL>(a.kz + c + sC2);
+            V kF;
+            double b0[CPL + 1], b1[CPL + 1];
+            if constexpr (KV) {
+                vrow(v + sV2, b0);
+                vrow(v + sV2 + sV1, b1);
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) kF[q] = av4(b0[q], b0[q + 1], b1[q], b1[q + 1]);
+            } else {
+                kF = ldv<double, CPL>(a.kz + c + sC2);
+            }
             uint8_t cl[CPL];
             if constexpr (CPL == 1) {
                 cl[0] = __builtin_nontemporal_load(a.cls + c);
@@ -244,9 +275,22 @@ __global__ __launch_bounds__(64 * TY) void iter_kernel(Args a) {
                 all &= cl[q];
             }
             if (!(all & C_ZERO)) {   // air and immersed cells: F = 0, nothing else to load
-                const double kW = a.kx[fx];
-                const V kE = ldv<double, CPL>(a.kx + fx + 1);
-                const V kS = ldv<double, CPL>(a.ky + fy), kN = ldv<double, CPL>(a.ky + fy + nx);
+                double kW;
+                V kE, kS, kN;
+                if constexpr (KV) {
+                    kW = av4(a0[0], a1[0], b0[0], b1[0]);
+#pragma unroll
+                    for (int q = 0; q < CPL; ++q) {
+                        kE[q] = av4(a0[q + 1], a1[q + 1], b0[q + 1], b1[q + 1]);
+                        kS[q] = av4(a0[q], a0[q + 1], b0[q], b0[q + 1]);
+                        kN[q] = av4(a1[q], a1[q + 1], b1[q], b1[q + 1]);
+                    }
+                } else {
+                    kW = a.kx[fx];
+                    kE = ldv<double, CPL>(a.kx + fx + 1);
+                    kS = ldv<double, CPL>(a.ky + fy);
+                    kN = ldv<double, CPL>(a.ky + fy + nx);
+                }
                 const double xW = a.Xin[e - 1], xE = a.Xin[e + CPL];
                 const V xS = ldv<double, CPL>(a.Xin + e - sE1), xN = ldv<double, CPL>(a.Xin + e + sE1);
                 V t1, t2, Dv = {}, gS = {}, gN = {};
@@ -299,10 +343,18 @@ __global__ __launch_bounds__(64 * TY) void iter_kernel(Args a) {
             xm = xc;
             xc = xp;
             kzB = kF;
+            if constexpr (KV) {
+#pragma unroll
+                for (int q = 0; q <= CPL; ++q) {
+                    a0[q] = b0[q];
+                    a1[q] = b1[q];
+                }
+            }
             e += sE2;
             c += sC2;
             fx += sX2;
             fy += sY2;
+            v += sV2;
         }
     }
     if (check) {
@@ -313,10 +365,13 @@ __global__ __launch_bounds__(64 * TY) void iter_kernel(Args a) {
 
 struct Tile {
     int ty, kz;
-    int cpl = 1;   // x-cells per lane (2: nx even only)
+    int cpl = 1;       // x-cells per lane (2: nx even only)
+    bool kv = false;   // face coefficients from the vertex field (2 cells per lane; the solve must have one)
 };
-static constexpr Tile kV18Tiles[] = {{4, 4}, {4, 8}, {8, 4}, {2, 8}, {8, 8}, {4, 16},
-                                     {4, 4, 2}, {4, 8, 2}, {2, 8, 2}, {4, 16, 2}};
+static constexpr Tile kV18Tiles[] = {{4, 4},          {4, 8},          {8, 4},          {2, 8},
+                                     {8, 8},          {4, 16},         {4, 4, 2},       {4, 8, 2},
+                                     {2, 8, 2},       {4, 16, 2},      {4, 4, 2, true}, {4, 8, 2, true},
+                                     {2, 8, 2, true}, {4, 16, 2, true}};
 static constexpr int kNumV18Tiles = sizeof(kV18Tiles) / sizeof(kV18Tiles[0]);
 
 template <bool DARCY>
@@ -337,15 +392,23 @@ static void launch(int tile, Args &a, hipStream_t st) {
     case 6: hipLaunchKernelGGL((iter_kernel<DARCY, 4, 4, 2>), dim3(nb), dim3(256), 0, st, a); break;
     case 7: hipLaunchKernelGGL((iter_kernel<DARCY, 4, 8, 2>), dim3(nb), dim3(256), 0, st, a); break;
     case 8: hipLaunchKernelGGL((iter_kernel<DARCY, 2, 8, 2>), dim3(nb), dim3(128), 0, st, a); break;
-    default: hipLaunchKernelGGL((iter_kernel<DARCY, 4, 16, 2>), dim3(nb), dim3(256), 0, st, a); break;
+    case 9: hipLaunchKernelGGL((iter_kernel<DARCY, 4, 16, 2>), dim3(nb), dim3(256), 0, st, a); break;
+    case 10: hipLaunchKernelGGL((iter_kernel<DARCY, 4, 4, 2, true>), dim3(nb), dim3(256), 0, st, a); break;
+    case 11: hipLaunchKernelGGL((iter_kernel<DARCY, 4, 8, 2, true>), dim3(nb), dim3(256), 0, st, a); break;
+    case 12: hipLaunchKernelGGL((iter_kernel<DARCY, 2, 8, 2, true>), dim3(nb), dim3(128), 0, st, a); break;
+    default: hipLaunchKernelGGL((iter_kernel<DARCY, 4, 16, 2, true>), dim3(nb), dim3(256), 0, st, a); break;
     }
 }
 
-// A 2-cell tile needs an even nx; otherwise its 1-cell twin is used.
-static int tile_for_grid(int tile, int nx) {
-    if (kV18Tiles[tile].cpl == 1 || nx % 2 == 0) return tile;
+// A 2-cell tile needs an even nx, and a vertex-field tile the vertex field;
+// otherwise its twin with the same rows and planes is used (the face-array
+// 2-cell tile, or the 1-cell tile).
+static int tile_for_grid(int tile, int nx, bool has_kv) {
+    const Tile &tl = kV18Tiles[tile];
+    if (tl.cpl == 1 || (nx % 2 == 0 && (!tl.kv || has_kv))) return tile;
+    const int cpl = nx % 2 == 0 ? 2 : 1;
     for (int t = 0; t < kNumV18Tiles; ++t)
-        if (kV18Tiles[t].cpl == 1 && kV18Tiles[t].ty == kV18Tiles[tile].ty && kV18Tiles[t].kz == kV18Tiles[tile].kz)
+        if (!kV18Tiles[t].kv && kV18Tiles[t].cpl == cpl && kV18Tiles[t].ty == tl.ty && kV18Tiles[t].kz == tl.kz)
             return t;
     return 0;
 }
@@ -371,11 +434,12 @@ void clear_tiles() { tile_cache().clear(); }
 
 template <bool DARCY>
 static int choose_tile(const Args &base, double *scratch_x, double *scratch_f0, int *tile_out) {
+    const bool has_kv = base.kv != nullptr;
     if (g_v18_force_tile >= 0 && g_v18_force_tile < kNumV18Tiles) {
-        *tile_out = tile_for_grid(g_v18_force_tile, base.nx);
+        *tile_out = tile_for_grid(g_v18_force_tile, base.nx, has_kv);
         return HT3D_OK;
     }
-    const auto key = std::make_tuple(base.nx, base.ny, base.nz, DARCY ? 1 : 0);
+    const auto key = std::make_tuple(base.nx, base.ny, base.nz, (DARCY ? 1 : 0) + (has_kv ? 2 : 0));
     auto it = tile_cache().find(key);
     if (it != tile_cache().end()) { *tile_out = it->second; return HT3D_OK; }
     hipStream_t st = ctx().stream;
@@ -397,7 +461,7 @@ static int choose_tile(const Args &base, double *scratch_x, double *scratch_f0, 
     for (int tile = 0; tile < kNumV18Tiles; ++tile) tbest[tile] = 1e30f;
     for (int round = 0; round < 3; ++round)
         for (int tile = 0; tile < kNumV18Tiles; ++tile) {
-            if (tile_for_grid(tile, base.nx) != tile) continue;
+            if (tile_for_grid(tile, base.nx, has_kv) != tile) continue;
             Args a = base;
             a.F0 = scratch_f0;
             a.flags = 0;
@@ -499,7 +563,7 @@ static int solve(const ht3d_v18_field *f, const ht3d_dyrel_params *p, ht3d_dyrel
     Args a;
     std::memset(&a, 0, sizeof a);
     a.nx = nx; a.ny = ny; a.nz = nz;
-    a.kx = f->kx; a.ky = f->ky; a.kz = f->kz;
+    a.kx = f->kx; a.ky = f->ky; a.kz = f->kz; a.kv = f->kv;
     a.cls = (const uint8_t *)cls;
     a.Fst = (double *)fst;
     a.partials = (double *)part;

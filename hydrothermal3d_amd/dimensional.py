@@ -91,14 +91,17 @@ def _v18_call(name: str, X: DeviceArray, shapes: dict, arrays: dict, scalars: di
 
 
 def v18_thermal_solve(Tc_ex, Tc0, rhoC, kx, ky, kz, phc_ex, wN, *, dx, dy, dz, dt, transient, qyS, TN, Qt=0.0,
-                      norm_scale=1.0, params=None, PC=None, G=None) -> dict:
+                      norm_scale=1.0, params=None, PC=None, G=None, kv=None) -> dict:
     """The heat DYREL loop of HydroThermal3D_v18.jl:432-478 on the device (one
     fused launch per pseudo-iteration): Tc_ex is updated in place, PC/G (if
-    given) receive the Gershgorin diagonal and bound.  Returns the report."""
+    given) receive the Gershgorin diagonal and bound.  kv: the vertex field
+    whose face averages kx, ky, kz are (InitConductivity!), which the tuner may
+    then read instead of the three face arrays (same results bit for bit).
+    Returns the report."""
     shapes = {"Tc0": (Tc0, "c"), "rhoC": (rhoC, "c"), "kx": (kx, "Fx"), "ky": (ky, "Fy"), "kz": (kz, "Fz"),
-              "phc_ex": (phc_ex, "E"), "wN": (wN, "W"), "PC": (PC, "c"), "G": (G, "c")}
+              "phc_ex": (phc_ex, "E"), "wN": (wN, "W"), "PC": (PC, "c"), "G": (G, "c"), "kv": (kv, "V")}
     arrays = {"Tc_old": Tc0, "rhoC": rhoC, "kx": kx, "ky": ky, "kz": kz, "phc_ex": phc_ex, "wN": wN, "PC": PC,
-              "G": G}
+              "G": G, "kv": kv}
     scalars = {"dx": dx, "dy": dy, "dz": dz, "dt": dt, "transient": transient, "qyS": qyS, "top": TN, "Qt": Qt,
                "norm_scale": norm_scale}
     return _v18_call("v18_thermal_solve", Tc_ex, shapes, arrays, scalars, params)
@@ -106,15 +109,16 @@ def v18_thermal_solve(Tc_ex, Tc0, rhoC, kx, ky, kz, phc_ex, wN, *, dx, dy, dz, d
 
 def v18_darcy_solve(Pc_ex, Tc_ex, phirho, phirhof0, rhofv, kx, ky, kz, phc_ex, wN, *, dx, dy, dz, dt, transient,
                     PN, g, phi_i, scale_rho, scale_T, scale_sigma, norm_scale=1.0, params=None, PC=None,
-                    G=None) -> dict:
+                    G=None, kv=None) -> dict:
     """The Darcy DYREL loop of HydroThermal3D_v18.jl:489-537 on the device:
     phirho holds phi*drho/dP on entry (the Gershgorin diagonal) and phi*rho_f of
-    the last iteration on exit, as the reference's shared buffer does."""
+    the last iteration on exit, as the reference's shared buffer does.  kv: as
+    for v18_thermal_solve (the hydraulic conductivity on vertices)."""
     shapes = {"Tc_ex": (Tc_ex, "E"), "phirho": (phirho, "c"), "phirhof0": (phirhof0, "c"), "rhofv": (rhofv, "V"),
               "kx": (kx, "Fx"), "ky": (ky, "Fy"), "kz": (kz, "Fz"), "phc_ex": (phc_ex, "E"), "wN": (wN, "W"),
-              "PC": (PC, "c"), "G": (G, "c")}
+              "PC": (PC, "c"), "G": (G, "c"), "kv": (kv, "V")}
     arrays = {"Tc_ex": Tc_ex, "rhoC": phirho, "phirhof0": phirhof0, "rhofv": rhofv, "kx": kx, "ky": ky, "kz": kz,
-              "phc_ex": phc_ex, "wN": wN, "PC": PC, "G": G}
+              "phc_ex": phc_ex, "wN": wN, "PC": PC, "G": G, "kv": kv}
     scalars = {"dx": dx, "dy": dy, "dz": dz, "dt": dt, "transient": transient, "top": PN, "g": g, "phi_i": phi_i,
                "scale_rho": scale_rho, "scale_T": scale_T, "scale_sigma": scale_sigma, "norm_scale": norm_scale}
     return _v18_call("v18_darcy_solve", Pc_ex, shapes, arrays, scalars, params)

@@ -139,6 +139,9 @@ class V18Model:
             "Tc_ex": DeviceArray.from_host(geo["T"]), "Pc_ex": DeviceArray.from_host(geo["P"]),
             "Tc0": Z(c), "Pc0": Z(c), "Xc0": Z(c), "PC": Z(c), "rhoC_phirho": Z(c), "dumc": Z(c),
             "k_rhof": Z(V), "kx": Z((nx + 1, ny, nz)), "ky": Z((nx, ny + 1, nz)), "kz": Z((nx, ny, nz + 1)),
+            # the smoothed hydraulic conductivity of the Darcy solve, kept for it:
+            # the reference's shared vertex buffer takes the fluid density next
+            "kf_v": Z(V),
             "Vx": Z((nx + 1, ny, nz)), "Vy": Z((nx, ny + 1, nz)), "Vz": Z((nx, ny, nz + 1)),
         }
         self.a["PC"].set(np.ones(c, order="F"))
@@ -175,12 +178,14 @@ class V18Model:
         return v18_thermal_solve(a["Tc_ex"], a["Tc0"], a["rhoC_phirho"], a["kx"], a["ky"], a["kz"], a["phc_ex"],
                                  a["wN"], dx=self.dx, dy=self.dy, dz=self.dz, dt=self.dt, transient=transient,
                                  qyS=ph.qyS / (sc.W / sc.L ** 2), TN=ph.Ttop / sc.T, Qt=ph.Qt / (sc.W / sc.L ** 3),
-                                 norm_scale=sc.rho * sc.C * sc.T / sc.t, params=self.pT, PC=a["PC"], G=a["Xc0"])
+                                 norm_scale=sc.rho * sc.C * sc.T / sc.t, params=self.pT, PC=a["PC"], G=a["Xc0"],
+                                 kv=a["k_rhof"])   # kx, ky, kz are its face averages
 
     def darcy(self, transient: float) -> dict:
         a, sc, ph = self.a, self.sc, self.phys
         self._hydro(1)
         self._conductivity_faces()
+        a["kf_v"].copy_from(a["k_rhof"])   # kx, ky, kz are its face averages
         K.PhiDrhoDP(a["rhoC_phirho"], a["Tc_ex"], a["Pc_ex"], a["phc_ex"], ph.phi_i, sc.rho, sc.T, sc.sigma)
         K.PhiRhof_old(a["Xc0"], a["Tc0"], a["Pc0"], a["phc_ex"], ph.phi_i, sc.rho, sc.T, sc.sigma)
         self._fluid_density()
@@ -188,7 +193,7 @@ class V18Model:
                                a["kz"], a["phc_ex"], a["wN"], dx=self.dx, dy=self.dy, dz=self.dz, dt=self.dt,
                                transient=transient, PN=ph.Ptop / sc.sigma, g=ph.g / (sc.L / sc.t ** 2),
                                phi_i=ph.phi_i, scale_rho=sc.rho, scale_T=sc.T, scale_sigma=sc.sigma,
-                               norm_scale=sc.rho / sc.t, params=self.pP, PC=a["PC"])
+                               norm_scale=sc.rho / sc.t, params=self.pP, PC=a["PC"], kv=a["kf_v"])
 
     def velocity(self) -> None:
         a, ph, sc = self.a, self.phys, self.sc

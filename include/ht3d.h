@@ -96,7 +96,7 @@ int ht3d_timing_reset(void);
  * updated unknown with a non-temporal store, bit 2 lets the face-coefficient
  * loads be cached; bit 3 forms the face coefficients from the solve's vertex
  * field (ht3d_dyrel_field.kv; tiles with 2 cells per lane, and 4 in FP32;
- * there bit 0 applies in FP64 only and bit 2 means a rolled z march, in FP32
+ * there bit 0 applies in FP64 only and bit 2 means a z march unrolled by two, in FP32
  * only); a vertex bit the solve cannot honour is dropped.  With autotuning on
  * (default) the
  * first solve of each (grid, operator, precision) times every configuration on
@@ -472,12 +472,17 @@ typedef struct {
     const double *phirhof0;       /* Darcy: c, phi*rho_f of the previous step (ϕρf_old, Xc0)         */
     const double *rhofv;          /* Darcy: V, fluid density (ComputeFluidDensity)                  */
     double g, phi_i, scale_rho, scale_T, scale_sigma;   /* Darcy: gravity, porosity, scales        */
+    const double *kv;             /* V, or NULL: the vertex field whose face averages kx, ky, kz are
+                                     (InitConductivity!); when given, the tiles that read it instead
+                                     of the three face arrays are offered (same results bit for bit) */
 } ht3d_v18_field;
 int ht3d_v18_thermal_solve(const ht3d_v18_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep);
 int ht3d_v18_darcy_solve(const ht3d_v18_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep);
 /* tile of the v18 iteration kernel: force_tile >= 0 forces one of the
  * ht3d_v18_tile_count() candidates (6 with one x-cell per lane, then 4 with
- * two, which need an even ncx and otherwise run as their one-cell twin), -1
+ * two, which need an even ncx and otherwise run as their one-cell twin, then
+ * the same 4 forming the face coefficients from the field's vertex field,
+ * which without one run as their face-array twin), -1
  * lets the autotuner choose; *last_tile (may be NULL) returns the tile of the
  * last solve.  Every tile gives bit-identical results.  Analogue of
  * cublocks/cuthreads.                                                        */

@@ -147,7 +147,7 @@ def v18_problem(nx, ny, nz, seed=0, immersed=False, Qt=None):
     darcy = dict(dx=dx, dy=dy, dz=dz, dt=dt, transient=1.0, PN=Ptop, g=-9.81 / (sc.L / sc.t ** 2), phi_i=phi_i,
                  scale_rho=sc.rho, scale_T=sc.T, scale_sigma=sc.sigma, norm_scale=sc.rho / sc.t)
     return dict(T=T, P=P, T0=T0, rhoC=rhoC, kT=kT, phirho=phirho, phirhof0=phirhof0, rhofv=rhofv, kP=kP, phc=phc,
-                wN=wN, heat=heat, darcy=darcy, sc=sc)
+                wN=wN, heat=heat, darcy=darcy, sc=sc, ktv=ktv, kfv=kfv)
 
 
 class V18OracleModel:

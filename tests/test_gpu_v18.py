@@ -22,7 +22,9 @@ def _problem(grid, immersed, dt_s, seed=3):
     return d
 
 
-def _run(ht, orc, d, kind, prm):
+def _run(ht, orc, d, kind, prm, kv=False):
+    """kv: also hand the solve the vertex field whose face averages the
+    coefficients are (ktv for heat, kfv for Darcy)."""
     from hydrothermal3d_amd import dimensional as dim
     from hydrothermal3d_amd.solver import SolveError
     dev = lambda a: ht.DeviceArray.from_host(np.asfortranarray(a))
@@ -43,7 +45,8 @@ def _run(ht, orc, d, kind, prm):
         rep_o, w = orc.v18_thermal_solve(Xh, d["T0"], d["rhoC"], *d["kT"], d["phc"], d["wN"], **d["heat"], p=p)
         Xd = dev(d["T"])
         rep_g = solve(dim.v18_thermal_solve, Xd, dev(d["T0"]), dev(d["rhoC"]), *map(dev, d["kT"]), dev(d["phc"]),
-                                      dev(d["wN"]), **d["heat"], params=gp, PC=PCd, G=Gd)
+                                      dev(d["wN"]), **d["heat"], params=gp, PC=PCd, G=Gd,
+                      kv=dev(d["ktv"]) if kv else None)
         extra = None
     else:
         Xh, ph = d["P"].copy(order="F"), d["phirho"].copy(order="F")
@@ -51,7 +54,8 @@ def _run(ht, orc, d, kind, prm):
                                        **d["darcy"], p=p)
         Xd, phd = dev(d["P"]), dev(d["phirho"])
         rep_g = solve(dim.v18_darcy_solve, Xd, dev(d["T"]), phd, dev(d["phirhof0"]), dev(d["rhofv"]), *map(dev, d["kP"]),
-                                    dev(d["phc"]), dev(d["wN"]), **d["darcy"], params=gp, PC=PCd, G=Gd)
+                                    dev(d["phc"]), dev(d["wN"]), **d["darcy"], params=gp, PC=PCd, G=Gd,
+                      kv=dev(d["kfv"]) if kv else None)
         extra = (phd.get(), ph)
     return rep_o, rep_g, (Xd.get(), Xh), (PCd.get(), w["PC"]), (Gd.get(), w["G"]), extra
 
@@ -101,27 +105,34 @@ def test_divergence_detected_alike(ht, orc):
     _compare(d, out)
 
 
-V18_TWIN = {6: 0, 7: 1, 8: 3, 9: 5}   # 2-cell tile -> its 1-cell twin (odd nx)
+V18_TWIN = {6: 0, 7: 1, 8: 3, 9: 5, 10: 0, 11: 1, 12: 3, 13: 5}   # 2-cell tile -> its 1-cell twin (odd nx)
+V18_FACE_TWIN = {10: 6, 11: 7, 12: 8, 13: 9}   # vertex-field tile -> its face-array twin (no vertex field)
 
 
 @pytest.mark.parametrize("kind", ["heat", "darcy"])
 @pytest.mark.parametrize("grid", [(70, 37, 9), (33, 20, 7)], ids=lambda g: "x".join(map(str, g)))
 def test_every_tile_bitwise(ht, orc, grid, kind):
-    """Every v18 tile pinned in turn (1 and 2 x-cells per lane), with immersed
-    cells: bit-identical to the oracle.  On an odd nx a 2-cell tile runs as its
-    1-cell twin, which the tuner report shows."""
+    """Every v18 tile pinned in turn (1 and 2 x-cells per lane; the 2-cell
+    tiles also forming the face coefficients from the vertex field), with
+    immersed cells and with the vertex field given: bit-identical to the
+    oracle.  On an odd nx a 2-cell tile runs as its 1-cell twin, and without
+    the vertex field a vertex tile as its face-array twin, which the tuner
+    report shows."""
     import ctypes as C
     d = _problem(grid, True, 1e14)
     d["wN"][:] = np.where(d["wN"] < 1.0, 0.9, 1.0)
     ref = None
     try:
-        for t in range(ht._lib.lib().ht3d_v18_tile_count()):
-            print(f"v18 tile {t} {kind} {grid}", flush=True)
+        n = ht._lib.lib().ht3d_v18_tile_count()
+        assert n == 14
+        for t, kv in [(t, True) for t in range(n)] + [(t, False) for t in V18_FACE_TWIN]:
+            print(f"v18 tile {t} {kind} {grid} vertex field {kv}", flush=True)
             ht._lib.call("ht3d_v18_tune", t, None)
-            out = _run(ht, orc, d, kind, dict(nitmax=120, nitout=100, ncheck=0, norm=0, tol=0.0))
+            out = _run(ht, orc, d, kind, dict(nitmax=120, nitout=100, ncheck=0, norm=0, tol=0.0), kv=kv)
             last = C.c_int()
             ht._lib.call("ht3d_v18_tune", t, C.byref(last))
-            assert last.value == (V18_TWIN.get(t, t) if grid[0] % 2 else t), (t, last.value)
+            want = V18_TWIN.get(t, t) if grid[0] % 2 else (t if kv else V18_FACE_TWIN.get(t, t))
+            assert last.value == want, (t, kv, last.value)
             _compare(d, out)
             ref = out[2][0] if ref is None else ref
             assert np.array_equal(out[2][0], ref)
