@@ -362,7 +362,9 @@ int dc_solve_impl(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dy
                                   {r32, N * sizeof(float)},     {f->X, NE * sizeof(double)}};
             if ((rc = tune_residual(d, f->transient != 0, touch, 6, &res_tile))) return rc;
         } else {
-            res_tile = nx % 2 == 0 ? 1 : 7;   // 4 x 16 planes, two x-cells per lane when nx is even
+            // 4 x 16 planes, two x-cells per lane when nx is even; from the
+            // vertex field when the sweeps are pinned to a vertex variant
+            res_tile = nx % 2 == 0 ? ((cfg.variant & 8) && f->kv ? 10 : 1) : 7;
         }
         if (te) te->res_tile = res_tile;
     }
