@@ -375,6 +375,7 @@ struct V18Field
     Tc_old::Ptr{Cvoid}; qyS::Float64; Qt::Float64
     Tc_ex::Ptr{Cvoid}; phirhof0::Ptr{Cvoid}; rhofv::Ptr{Cvoid}
     g::Float64; phi_i::Float64; scale_rho::Float64; scale_T::Float64; scale_sigma::Float64
+    kv::Ptr{Cvoid}   # vertex field of kx, ky, kz (C_NULL: the solve reads the face arrays)
 end
 # the structs are plain bits: Ref(x) hands the C side a pointer to their fields
 function v18_solve!(solve, name, f::V18Field, p::DyrelParams, keep...)
@@ -391,7 +392,7 @@ function v18_thermal_solve!(Tc_ex, Tc0, ρC_ϕρ, kx, ky, kz, phc_ex, wN, PC, Xc
                             Qt, norm_scale, params::DyrelParams)
     n = cells_E(Tc_ex)
     f = V18Field(n..., dx, dy, dz, dt, transient, vp(Tc_ex), vp(kx), vp(ky), vp(kz), vp(phc_ex), vp(wN), vp(ρC_ϕρ),
-                 vp(PC), vp(Xc0), Ttop, norm_scale, vp(Tc0), qyS, Qt, C_NULL, C_NULL, C_NULL, 0, 0, 0, 0, 0)
+                 vp(PC), vp(Xc0), Ttop, norm_scale, vp(Tc0), qyS, Qt, C_NULL, C_NULL, C_NULL, 0, 0, 0, 0, 0, C_NULL)
     return v18_solve!(ht3d_v18_thermal_solve, "ht3d_v18_thermal_solve", f, params,
                       Tc_ex, Tc0, ρC_ϕρ, kx, ky, kz, phc_ex, wN, PC, Xc0)
 end
@@ -401,7 +402,7 @@ function v18_darcy_solve!(Pc_ex, Tc_ex, ρC_ϕρ, Xc0, k_ρf, kx, ky, kz, phc_ex
     n = cells_E(Pc_ex)
     f = V18Field(n..., dx, dy, dz, dt, transient, vp(Pc_ex), vp(kx), vp(ky), vp(kz), vp(phc_ex), vp(wN), vp(ρC_ϕρ),
                  vp(PC), C_NULL, Ptop, norm_scale, C_NULL, 0, 0, vp(Tc_ex), vp(Xc0), vp(k_ρf), g, ϕi, scale_ρ,
-                 scale_T, scale_σ)
+                 scale_T, scale_σ, C_NULL)
     return v18_solve!(ht3d_v18_darcy_solve, "ht3d_v18_darcy_solve", f, params,
                       Pc_ex, Tc_ex, ρC_ϕρ, Xc0, k_ρf, kx, ky, kz, phc_ex, wN, PC)
 end

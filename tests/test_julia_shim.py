@@ -117,3 +117,54 @@ def test_shim_methods_call_header_entries_with_their_arity():
                 assert n == arity[entry], f"{name}: {entry} called with {n} arguments, the header has {arity[entry]}"
                 checked += 1
     assert checked > 80
+
+
+def _c_struct_fields(name: str) -> list[tuple[str, str]]:
+    """(member, kind) of `typedef struct { ... } name;` in include/ht3d.h, in
+    declaration order; kind is "ptr" for pointers, else the C scalar type
+    (comments stripped, one member per declarator)."""
+    text = re.sub(r"/\*.*?\*/", "", (ROOT / "include" / "ht3d.h").read_text(), flags=re.S)
+    m = re.search(r"typedef struct \{([^}]*)\}\s*" + name + r";", text)
+    assert m, name
+    out = []
+    for decl in m.group(1).split(";"):
+        decl = " ".join(decl.split())
+        if not decl:
+            continue
+        base = re.match(r"(const )?(int|double|float|size_t)", decl)
+        assert base, decl
+        for d in decl[base.end():].split(","):
+            d = d.strip()
+            ptr = d.startswith("*")
+            nm = d.lstrip("*").strip()
+            arr = re.match(r"(\w+)\[(\d+)\]", nm)
+            if arr:
+                out += [(arr.group(1), base.group(2))] * int(arr.group(2))
+            else:
+                out.append((nm, "ptr" if ptr else base.group(2)))
+    return out
+
+
+def _jl_struct_fields(name: str) -> list[tuple[str, str]]:
+    text = (JL / "HT3D_HIP.jl").read_text()
+    m = re.search(r"^struct " + name + r"\n(.*?)^end", text, flags=re.S | re.M)
+    assert m, name
+    kinds = {"Cint": "int", "Float64": "double", "Ptr{Cvoid}": "ptr"}
+    out = []
+    for line in m.group(1).splitlines():
+        line = line.split("#")[0]
+        for f in line.split(";"):
+            f = f.strip()
+            if f:
+                nm, ty = f.split("::")
+                out.append((nm.strip(), kinds[ty.strip()]))
+    return out
+
+
+def test_julia_structs_mirror_the_header():
+    """The plain-bits structs the shim hands to the C entries by pointer have
+    the header's members in the header's order with matching kinds, so no
+    member the library reads lies past the end of the Julia value."""
+    for jl, c in (("DyrelParams", "ht3d_dyrel_params"), ("DyrelReport", "ht3d_dyrel_report"),
+                  ("V18Field", "ht3d_v18_field")):
+        assert _jl_struct_fields(jl) == _c_struct_fields(c), jl
