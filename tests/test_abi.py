@@ -127,3 +127,61 @@ def test_every_entry_body_is_guarded():
             if lines[j + 1].strip() != "HT3D_API_BEGIN" or lines[k - 1].strip() != "HT3D_API_END":
                 unguarded.append(f"{src.name}: {m.group(1)}")
     assert not unguarded, unguarded
+
+
+def _header_struct(name: str) -> list[tuple[str, str]]:
+    """(member, kind) of `typedef struct { ... } name;` in include/ht3d.h in
+    declaration order: kind "ptr" for pointers, the C type otherwise (arrays
+    expanded element by element)."""
+    import re
+    text = re.sub(r"/\*.*?\*/", "", (ROOT / "include" / "ht3d.h").read_text(), flags=re.S)
+    m = re.search(r"typedef struct \{([^}]*)\}\s*" + name + r";", text)
+    assert m, name
+    out = []
+    for decl in m.group(1).split(";"):
+        decl = " ".join(decl.split())
+        if not decl:
+            continue
+        base = re.match(r"(const )?(long long|int|double|float|size_t|ht3d_\w+)", decl)
+        assert base, decl
+        for d in decl[base.end():].split(","):
+            d = d.strip()
+            nm = d.lstrip("*").strip()
+            arr = re.match(r"(\w+)\[(\d+)\]", nm)
+            if arr:
+                out += [(arr.group(1), base.group(2))] * int(arr.group(2))
+            else:
+                out.append((nm, "ptr" if d.startswith("*") else base.group(2)))
+    return out
+
+
+def _ctypes_struct(cls) -> list[tuple[str, str]]:
+    import ctypes as C
+    kinds = {C.c_int: "int", C.c_double: "double", C.c_float: "float", C.c_void_p: "ptr", C.c_longlong: "long long",
+             C.c_size_t: "size_t"}
+    names = {v: k for k, v in STRUCTS.items()}
+    out = []
+    for nm, ty in cls._fields_:
+        if hasattr(ty, "_length_"):
+            out += [(nm, kinds[ty._type_])] * ty._length_
+        elif ty in names:
+            out.append((nm, names[ty]))
+        else:
+            out.append((nm, kinds[ty]))
+    return out
+
+
+STRUCTS = {}
+
+
+def test_ctypes_structs_mirror_the_header():
+    """The Python layer's ctypes structures have the header's members in the
+    header's order with matching types, so every member the library reads or
+    writes is where the caller put it (a member appended to the C struct but
+    not to its mirror would be read past the caller's struct)."""
+    from hydrothermal3d_amd import _lib
+    STRUCTS.update({"ht3d_dyrel_params": _lib.DyrelParams, "ht3d_dyrel_report": _lib.DyrelReport,
+                    "ht3d_dyrel_field": _lib.DyrelField, "ht3d_v18_field": _lib.V18Field,
+                    "ht3d_step_report": _lib.StepReport, "ht3d_batch_stats": _lib.BatchStats})
+    for cname, cls in STRUCTS.items():
+        assert _ctypes_struct(cls) == _header_struct(cname), cname
