@@ -430,7 +430,12 @@ typedef struct {
                                            solve may read it instead of the three face
                                            arrays (6 instead of 8 words per cell and
                                            iteration); the results are the same bit for bit
-                                           when kx, ky, kz are those averages.            */
+                                           when kx, ky, kz are those averages.  The
+                                           defect-correction solve then forms its FP32
+                                           faces as FP32 averages of the FP32 vertex field
+                                           (not the FP64 faces rounded), so its FP32 sweeps
+                                           differ from a solve without kv at FP32 rounding;
+                                           its FP64 residuals, and so its tolerance, do not. */
 } ht3d_dyrel_field;
 
 int ht3d_dyrel_solve(const ht3d_dyrel_field *f, const ht3d_dyrel_params *p, ht3d_dyrel_report *rep);
