@@ -1482,9 +1482,16 @@ int ht3d_batch_step(ht3d_batch *b, ht3d_step_report *reps) {
             if ((rc = hrl_pre_darcy(b->m[q], &f[q]))) return rc;
         if ((rc = batch_solve(b, 1, b->pP, f, rep, outer, nullptr))) return rc;
         if (!solve_status(1)) break;
-        // velocity, time step and advection per problem
+        // velocity of every problem, then one read-back of all the max|V|
+        // (a read-back per problem would drain the stream once per problem),
+        // then time step and advection per problem
+        for (int q = 0; q < n; ++q)
+            if ((rc = hrl_velocity_dev(b->m[q], b->d_nures + 4 * q))) return rc;
+        hipStream_t s = ctx().stream;
+        HIP_TRY(hipMemcpyAsync(b->h_nures, b->d_nures, (size_t)4 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
         for (int q = 0; q < n; ++q) {
-            if ((rc = hrl_advect(b->m[q], &r[q]))) return rc;
+            if ((rc = hrl_advect_vmax(b->m[q], b->h_nures[4 * q], &r[q]))) return rc;
             r[q].Nu = b->m[q]->Nu;
             r[q].status = (r[q].thermal.status == HT3D_NOT_CONVERGED || r[q].darcy.status == HT3D_NOT_CONVERGED)
                               ? HT3D_NOT_CONVERGED : HT3D_OK;

@@ -31,6 +31,10 @@ int hrl_pre_darcy(ht3d_hrl *m, ht3d_dyrel_field *f);
 // Init_vel!, the advective time step and WENO5 advection (HydroThermal3D_v18.jl:548-570);
 // fills dt_next, vmax and n_adv of the report and advances m->time by the step's dt
 int hrl_advect(ht3d_hrl *m, ht3d_step_report *r);
+// hrl_advect in two parts: the velocity and max|V| into res_dev[0] (queued, no
+// read-back), then the time step from max|V| and the advection
+int hrl_velocity_dev(ht3d_hrl *m, double *res_dev);
+int hrl_advect_vmax(ht3d_hrl *m, double vmax, ht3d_step_report *r);
 // Nu from the thermal flux sum (Alban_v6:345-348)
 inline void hrl_set_nu(ht3d_hrl *m, double nusum) { m->Nu = nusum / (m->Lx * m->Lz) / (double)(m->nx * m->nz); }
 

@@ -207,20 +207,35 @@ __global__ __launch_bounds__(256) void velocity_vmax_kernel(
 }
 
 // V, max|V| and (T given) the Nu sum in one pass and one read-back
-int velocity_vmax(double *Vx, double *Vy, double *Vz, const double *kx, const double *ky, const double *kz,
-                  const double *P, const double *Ty, double Ra, double _dx, double _dy, double _dz, const double *T,
-                  const double *kt_y, int nx, int ny, int nz, double *vmax, double *nusum) {
+// the velocity kernel's block partials: at most this many blocks, so one
+// workspace of this size serves any grid (and launches queued back to back
+// for several grids never see it reallocated)
+static constexpr unsigned kVelBlocks = 4096;
+
+// velocity, max|V| (and the Nu sum) into res_dev[0..1] on the device, no read-back
+int velocity_vmax_dev(double *Vx, double *Vy, double *Vz, const double *kx, const double *ky, const double *kz,
+                      const double *P, const double *Ty, double Ra, double _dx, double _dy, double _dz, const double *T,
+                      const double *kt_y, int nx, int ny, int nz, double *res_dev) {
     const size_t n = (size_t)(nx + 1) * (ny + 1) * (nz + 1);
     unsigned nb = div_up(n, 256);
-    if (nb > 4096) nb = 4096;
-    void *part = nullptr, *res = nullptr;
+    if (nb > kVelBlocks) nb = kVelBlocks;
+    void *part = nullptr;
     int rc;
-    if ((rc = workspace(WS_MISC, (size_t)nb * 8 * sizeof(double), &part))) return rc;
-    if ((rc = workspace(WS_RESULT, 64 * sizeof(double), &res))) return rc;
+    if ((rc = workspace(WS_MISC, (size_t)kVelBlocks * 8 * sizeof(double), &part))) return rc;
     hipLaunchKernelGGL(velocity_vmax_kernel, dim3(nb), dim3(256), 0, ctx().stream, Vx, Vy, Vz, kx, ky, kz, P, Ty, Ra,
                        _dx, _dy, _dz, T, kt_y, nx, ny, nz, (double *)part);
     HIP_TRY(hipGetLastError());
-    if ((rc = finish_partials((double *)part, nb, 0x1u, (double *)res))) return rc;
+    return finish_partials((double *)part, nb, 0x1u, res_dev);
+}
+
+int velocity_vmax(double *Vx, double *Vy, double *Vz, const double *kx, const double *ky, const double *kz,
+                  const double *P, const double *Ty, double Ra, double _dx, double _dy, double _dz, const double *T,
+                  const double *kt_y, int nx, int ny, int nz, double *vmax, double *nusum) {
+    void *res = nullptr;
+    int rc;
+    if ((rc = workspace(WS_RESULT, 64 * sizeof(double), &res))) return rc;
+    if ((rc = velocity_vmax_dev(Vx, Vy, Vz, kx, ky, kz, P, Ty, Ra, _dx, _dy, _dz, T, kt_y, nx, ny, nz, (double *)res)))
+        return rc;
     double r4[4];
     if ((rc = read_result4((double *)res, r4))) return rc;
     *vmax = r4[0];
@@ -269,18 +284,28 @@ int hrl_pre_darcy(ht3d_hrl *m, ht3d_dyrel_field *f) {
     return HT3D_OK;
 }
 
+int hrl_velocity_dev(ht3d_hrl *m, double *res_dev) {
+    return velocity_vmax_dev(m->Vx, m->Vy, m->Vz, m->kx, m->ky, m->kz, m->Pc_ex, m->Ty, m->Ra, 1.0 / m->dx,
+                             1.0 / m->dy, 1.0 / m->dz, nullptr, nullptr, m->nx, m->ny, m->nz, res_dev);
+}
+
 int hrl_advect(ht3d_hrl *m, ht3d_step_report *r) {
+    // velocity, advective time step from this velocity, advection
+    // (HydroThermal3D_v18.jl:551-570, HydroThermal3D_v7.jl:215-221)
+    double vmax = 0.0;
+    int rc;
+    if ((rc = velocity_vmax(m->Vx, m->Vy, m->Vz, m->kx, m->ky, m->kz, m->Pc_ex, m->Ty, m->Ra, 1.0 / m->dx,
+                            1.0 / m->dy, 1.0 / m->dz, nullptr, nullptr, m->nx, m->ny, m->nz, &vmax, nullptr)))
+        return rc;
+    return hrl_advect_vmax(m, vmax, r);
+}
+
+int hrl_advect_vmax(ht3d_hrl *m, double vmax, ht3d_step_report *r) {
     const int nx = m->nx, ny = m->ny, nz = m->nz;
     const double _dx = 1.0 / m->dx, _dy = 1.0 / m->dy, _dz = 1.0 / m->dz;
     const double dt = m->dt;
     int rc;
     m->time += dt;
-    // velocity, advective time step from this velocity, advection
-    // (HydroThermal3D_v18.jl:551-570, HydroThermal3D_v7.jl:215-221)
-    double vmax = 0.0;
-    if ((rc = velocity_vmax(m->Vx, m->Vy, m->Vz, m->kx, m->ky, m->kz, m->Pc_ex, m->Ty, m->Ra, _dx, _dy, _dz, nullptr,
-                            nullptr, nx, ny, nz, &vmax, nullptr)))
-        return rc;
     const double h = fmin(fmin(m->dx, m->dy), m->dz);
     const double dt_adv = m->cfl * h / vmax;
     int n_adv = 1;
