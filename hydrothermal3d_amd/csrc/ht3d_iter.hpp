@@ -924,8 +924,8 @@ inline bool kv_cfg_ok(const IterArgs<T> &a, const IterCfg &cfg) {
     return !(cfg.variant & 8) || (a.kv && kv_tile_ok<T>(cfg.tile, a.nx));
 }
 // drops the vertex bit where it cannot run (no vertex field, or another
-// tile); with it, bit 1 (stored diagonal) applies in FP64 only and bit 4
-// (rolled z march) in FP32 only
+// tile), and bit 4 with it; with the vertex bit, bit 1 (stored diagonal)
+// applies in FP64 only and bit 4 (z march unrolled by two) in FP32 only
 template <typename T>
 inline void kv_normalize(const IterArgs<T> &a, IterCfg &cfg) {
     if ((cfg.variant & 8) && !(a.kv && kv_tile_ok<T>(cfg.tile, a.nx))) cfg.variant &= ~(8 | 4);
