@@ -544,6 +544,7 @@ struct ht3d_batch {
     void *h_tabs = nullptr;
     size_t tabs_bytes = 0;
     double *h_nures = nullptr;
+    WenoItem *d_weno = nullptr, *h_weno = nullptr;   // the batched advection's table
     int stage = 0;
     hipEvent_t ev[2] = {nullptr, nullptr}, ev0 = nullptr, ev1 = nullptr;
     std::map<std::string, hipGraphExec_t> cache;
@@ -1370,6 +1371,7 @@ int ht3d_batch_create(ht3d_hrl *const *models, int nprob, ht3d_batch **out) {
     A((void **)&b->d_nupart, nusum * 8 * sizeof(double));
     A((void **)&b->d_nures, (size_t)4 * nprob * sizeof(double));
     A((void **)&b->d_state, nprob * sizeof(BState));
+    A((void **)&b->d_weno, nprob * sizeof(WenoItem));
     if (rc) { ht3d_batch_destroy(b); return rc; }
     b->tabs_bytes = 2 * (size_t)nprob * (sizeof(IterArgs<double>) + sizeof(IterArgs<float>) + sizeof(DcArgs));
     hipError_t e = hipSuccess;
@@ -1383,6 +1385,7 @@ int ht3d_batch_create(ht3d_hrl *const *models, int nprob, ht3d_batch **out) {
     }
     if (e == hipSuccess) e = hipHostMalloc(&b->h_tabs, b->tabs_bytes, hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_nures, (size_t)4 * nprob * sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_weno, nprob * sizeof(WenoItem), hipHostMallocDefault);
     for (int s = 0; s < 2 && e == hipSuccess; ++s) e = hipEventCreate(&b->ev[s]);
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
@@ -1415,6 +1418,7 @@ int ht3d_batch_destroy(ht3d_batch *b) {
     }
     if (b->h_tabs) (void)hipHostFree(b->h_tabs);
     if (b->h_nures) (void)hipHostFree(b->h_nures);
+    if (b->h_weno) (void)hipHostFree(b->h_weno);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     delete b;
@@ -1484,14 +1488,33 @@ int ht3d_batch_step(ht3d_batch *b, ht3d_step_report *reps) {
         if (!solve_status(1)) break;
         // velocity of every problem, then one read-back of all the max|V|
         // (a read-back per problem would drain the stream once per problem),
-        // then time step and advection per problem
+        // then the time steps and the first advection sub-step of every
+        // problem in six launches; further sub-steps (the constant-dt rule)
+        // per problem.  The read-back's synchronisation also guarantees that
+        // the previous step's table upload has finished before h_weno is
+        // refilled.
         for (int q = 0; q < n; ++q)
             if ((rc = hrl_velocity_dev(b->m[q], b->d_nures + 4 * q))) return rc;
         hipStream_t s = ctx().stream;
         HIP_TRY(hipMemcpyAsync(b->h_nures, b->d_nures, (size_t)4 * n * sizeof(double), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
+        std::vector<int> n_adv(n);
+        std::vector<double> dt_a(n);
+        unsigned nblocks = 0;
         for (int q = 0; q < n; ++q) {
-            if ((rc = hrl_advect_vmax(b->m[q], b->h_nures[4 * q], &r[q]))) return rc;
+            const ht3d_hrl *m = b->m[q];
+            if ((rc = hrl_advect_plan(b->m[q], b->h_nures[4 * q], &r[q], &n_adv[q], &dt_a[q]))) return rc;
+            const unsigned nb = weno_blocks(m->nx, m->ny, m->nz);
+            if (nblocks > 0x7fffffffu - nb) return arg_fail("ht3d_batch_step: advection grid too large");
+            b->h_weno[q] = hrl_weno_item(m, dt_a[q], nblocks);
+            nblocks += nb;
+        }
+        HIP_TRY(hipMemcpyAsync(b->d_weno, b->h_weno, (size_t)n * sizeof(WenoItem), hipMemcpyHostToDevice, s));
+        if ((rc = weno_advect_batch(b->d_weno, n, nblocks))) return rc;
+        for (int q = 0; q < n; ++q)
+            for (int sub = 1; sub < n_adv[q]; ++sub)
+                if ((rc = hrl_weno(b->m[q], dt_a[q]))) return rc;
+        for (int q = 0; q < n; ++q) {
             r[q].Nu = b->m[q]->Nu;
             r[q].status = (r[q].thermal.status == HT3D_NOT_CONVERGED || r[q].darcy.status == HT3D_NOT_CONVERGED)
                               ? HT3D_NOT_CONVERGED : HT3D_OK;

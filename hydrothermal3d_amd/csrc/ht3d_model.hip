@@ -56,13 +56,11 @@ struct WenoArgs {
     double vlo, vhi;
 };
 
-// One WENO5 pass along AXIS (0,1,2).  V2 selects the v2 minus-side stencil quirk
-// and its >= 0 upwind test; STAGE2 fuses TimeAveraging!(T, Told, 2).
+// One WENO5 pass along AXIS (0,1,2) at cell (i, j, k).  V2 selects the v2
+// minus-side stencil quirk and its >= 0 upwind test; STAGE2 fuses
+// TimeAveraging!(T, Told, 2).
 template <int AXIS, bool V2, bool STAGE2>
-__global__ __launch_bounds__(256) void weno_pass_kernel(WenoArgs a) {
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int k = blockIdx.z;
+__device__ __forceinline__ void weno_cell(const WenoArgs &a, const int i, const int j, const int k) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     if (i >= nx || j >= ny || k >= nz) return;
     const size_t sE1 = (size_t)nx + 2, sE2 = sE1 * (ny + 2);
@@ -110,6 +108,47 @@ __global__ __launch_bounds__(256) void weno_pass_kernel(WenoArgs a) {
 }
 
 template <int AXIS, bool V2, bool STAGE2>
+__global__ __launch_bounds__(256) void weno_pass_kernel(WenoArgs a) {
+    weno_cell<AXIS, V2, STAGE2>(a, blockIdx.x * 64 + (threadIdx.x & 63), blockIdx.y * 4 + (threadIdx.x >> 6),
+                                blockIdx.z);
+}
+
+// The same pass over every model of a batch in one launch: a 1-D grid whose
+// blocks are dealt to the models in item order, each model's share laid out as
+// weno_pass_kernel's grid (64 x 4 cells per block).  Pass 1 writes W from T,
+// pass 2 writes T from W and the old T, as weno_axis does per model.
+template <int AXIS, bool STAGE2>
+__global__ __launch_bounds__(256) void weno_pass_batch_kernel(const WenoItem *__restrict__ items, int n) {
+    const unsigned b = blockIdx.x;
+    int lo = 0, hi = n - 1;   // the last item whose first block is <= b
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].b0 <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const WenoItem &it = items[lo];
+    const unsigned lb = b - it.b0, bx = ((unsigned)it.nx + 63) / 64, by = ((unsigned)it.ny + 3) / 4;
+    WenoArgs a;
+    a.Tin = STAGE2 ? it.W : it.T;
+    a.Tout = STAGE2 ? it.T : it.W;
+    a.Told = it.T;
+    a.V = it.V[AXIS];
+    a.nx = it.nx;
+    a.ny = it.ny;
+    a.nz = it.nz;
+    a.h = it.h[AXIS];
+    a.dt = it.dt;
+    a.tlo = a.thi = AXIS == 1 ? 1 : 0;   // y: Dirichlet Tbot / Ttop; x, z: Neumann
+    a.vlo = AXIS == 1 ? it.Tbot : 0.0;
+    a.vhi = AXIS == 1 ? it.Ttop : 0.0;
+    const int i = (int)(lb % bx) * 64 + (threadIdx.x & 63);
+    const int j = (int)((lb / bx) % by) * 4 + (threadIdx.x >> 6);
+    const int k = (int)(lb / (bx * by));
+    if (it.v2) weno_cell<AXIS, true, STAGE2>(a, i, j, k);
+    else weno_cell<AXIS, false, STAGE2>(a, i, j, k);
+}
+
+template <int AXIS, bool V2, bool STAGE2>
 static int launch_weno(const WenoArgs &a) {
     dim3 blk(256), grd(div_up(a.nx, 64), div_up(a.ny, 4), a.nz);
     hipLaunchKernelGGL((weno_pass_kernel<AXIS, V2, STAGE2>), grd, blk, 0, ctx().stream, a);
@@ -141,6 +180,26 @@ int weno_advect(double *Tc_ex, double *W, const double *Vx, const double *Vy, co
     if ((rc = weno_axis<0, false>(Tc_ex, W, Vx, nx, ny, nz, _dx, dt, 0, 0.0, 0, 0.0))) return rc;
     if ((rc = weno_axis<1, false>(Tc_ex, W, Vy, nx, ny, nz, _dy, dt, 1, Tbot, 1, Ttop))) return rc;
     return weno_axis<2, false>(Tc_ex, W, Vz, nx, ny, nz, _dz, dt, 0, 0.0, 0, 0.0);
+}
+
+unsigned weno_blocks(int nx, int ny, int nz) { return div_up(nx, 64) * div_up(ny, 4) * (unsigned)nz; }
+
+template <int AXIS, bool STAGE2>
+static int launch_weno_batch(const WenoItem *d_items, int n, unsigned nblocks) {
+    hipLaunchKernelGGL((weno_pass_batch_kernel<AXIS, STAGE2>), dim3(nblocks), dim3(256), 0, ctx().stream, d_items, n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "weno_pass_batch_kernel");
+    return HT3D_OK;
+}
+
+int weno_advect_batch(const WenoItem *d_items, int n, unsigned nblocks) {
+    if (n < 1 || nblocks == 0) return HT3D_OK;
+    int rc;
+    if ((rc = launch_weno_batch<0, false>(d_items, n, nblocks)) || (rc = launch_weno_batch<0, true>(d_items, n, nblocks)) ||
+        (rc = launch_weno_batch<1, false>(d_items, n, nblocks)) || (rc = launch_weno_batch<1, true>(d_items, n, nblocks)) ||
+        (rc = launch_weno_batch<2, false>(d_items, n, nblocks)))
+        return rc;
+    return launch_weno_batch<2, true>(d_items, n, nblocks);
 }
 
 // ------------------------------------------------------------------ random field
@@ -300,11 +359,8 @@ int hrl_advect(ht3d_hrl *m, ht3d_step_report *r) {
     return hrl_advect_vmax(m, vmax, r);
 }
 
-int hrl_advect_vmax(ht3d_hrl *m, double vmax, ht3d_step_report *r) {
-    const int nx = m->nx, ny = m->ny, nz = m->nz;
-    const double _dx = 1.0 / m->dx, _dy = 1.0 / m->dy, _dz = 1.0 / m->dz;
+int hrl_advect_plan(ht3d_hrl *m, double vmax, ht3d_step_report *r, int *n_adv_out, double *dt_a_out) {
     const double dt = m->dt;
-    int rc;
     m->time += dt;
     const double h = fmin(fmin(m->dx, m->dy), m->dz);
     const double dt_adv = m->cfl * h / vmax;
@@ -320,14 +376,47 @@ int hrl_advect_vmax(ht3d_hrl *m, double vmax, ht3d_step_report *r) {
         if (m->dt_rule == HT3D_DT_V7) dt_a = dt;
         m->dt = dt_adv;
     }
-    for (int s = 0; s < n_adv; ++s)
-        if ((rc = weno_advect(m->Tc_ex, m->W, m->Vx, m->Vy, m->Vz, dt_a, _dx, _dy, _dz, m->Ttop, m->Tbot,
-                              m->weno_version, nx, ny, nz)))
-            return rc;
     r->n_adv = n_adv;
     r->dt_next = m->dt;
     r->time = m->time;
     r->vmax = vmax;
+    *n_adv_out = n_adv;
+    *dt_a_out = dt_a;
+    return HT3D_OK;
+}
+
+int hrl_weno(ht3d_hrl *m, double dt_a) {
+    return weno_advect(m->Tc_ex, m->W, m->Vx, m->Vy, m->Vz, dt_a, 1.0 / m->dx, 1.0 / m->dy, 1.0 / m->dz, m->Ttop,
+                       m->Tbot, m->weno_version, m->nx, m->ny, m->nz);
+}
+
+WenoItem hrl_weno_item(const ht3d_hrl *m, double dt_a, unsigned b0) {
+    WenoItem it;
+    it.T = m->Tc_ex;
+    it.W = m->W;
+    it.V[0] = m->Vx;
+    it.V[1] = m->Vy;
+    it.V[2] = m->Vz;
+    it.h[0] = 1.0 / m->dx;
+    it.h[1] = 1.0 / m->dy;
+    it.h[2] = 1.0 / m->dz;
+    it.dt = dt_a;
+    it.Tbot = m->Tbot;
+    it.Ttop = m->Ttop;
+    it.nx = m->nx;
+    it.ny = m->ny;
+    it.nz = m->nz;
+    it.v2 = m->weno_version == 2;
+    it.b0 = b0;
+    return it;
+}
+
+int hrl_advect_vmax(ht3d_hrl *m, double vmax, ht3d_step_report *r) {
+    int n_adv, rc;
+    double dt_a;
+    if ((rc = hrl_advect_plan(m, vmax, r, &n_adv, &dt_a))) return rc;
+    for (int s = 0; s < n_adv; ++s)
+        if ((rc = hrl_weno(m, dt_a))) return rc;
     return HT3D_OK;
 }
 

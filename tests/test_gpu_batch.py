@@ -27,7 +27,7 @@ def _workloads(sizes=SIZES):
             for w, (a, b, c) in zip(ws, sizes)]
 
 
-def _models(ht, ws, p, precision):
+def _models(ht, ws, p, precision, setup=None):
     from hydrothermal3d_amd import inputs
     ms = []
     for q, w in enumerate(ws):
@@ -39,6 +39,8 @@ def _models(ht, ws, p, precision):
             m.set_viscosity(math.log(10.0))
         if q == 3:
             m.set_dt_rule(ht._lib.DT_V7)
+        if setup is not None:
+            setup(q, m)
         ms.append(m)
     return ms
 
@@ -50,10 +52,10 @@ def _fields(m):
             "Vz": m["Vz"].get()}
 
 
-def _run(ht, precision, graphs, nsteps=3, sizes=SIZES):
+def _run(ht, precision, graphs, nsteps=3, sizes=SIZES, setup=None):
     ws = _workloads(sizes)
     p = ht.DyrelParams(nitmax=20000, nitout=100, ncheck=25, norm=1, tol=1e-9)
-    ms = _models(ht, ws, p, precision)
+    ms = _models(ht, ws, p, precision, setup)
     b = ht.HydroThermal3DBatch(ms, p, p, precision=precision, graphs=graphs)
     reps = [b.step() for _ in range(nsteps)]
     stats = b.stats()
@@ -64,10 +66,10 @@ def _run(ht, precision, graphs, nsteps=3, sizes=SIZES):
     return reps, out, stats
 
 
-def _run_single(ht, precision, nsteps=3):
+def _run_single(ht, precision, nsteps=3, setup=None):
     ws = _workloads()
     p = ht.DyrelParams(nitmax=20000, nitout=100, ncheck=25, norm=1, tol=1e-9)
-    ms = _models(ht, ws, p, precision)
+    ms = _models(ht, ws, p, precision, setup)
     reps = [[m.step() for m in ms] for _ in range(nsteps)]
     out = [_fields(m) for m in ms]
     for m in ms:
@@ -75,10 +77,7 @@ def _run_single(ht, precision, nsteps=3):
     return reps, out
 
 
-@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32dc"])
-def test_batch_matches_single_models(ht, precision):
-    rb, fb, stats = _run(ht, precision, graphs=True)
-    rs, fs = _run_single(ht, precision)
+def _compare_with_single(rb, fb, rs, fs, precision):
     for s, (stepb, steps_) in enumerate(zip(rb, rs)):
         for q, (a, c) in enumerate(zip(stepb, steps_)):
             for kind in ("thermal", "darcy"):
@@ -93,6 +92,13 @@ def test_batch_matches_single_models(ht, precision):
     for q, (a, c) in enumerate(zip(fb, fs)):
         for k in a:
             assert rel_err(a[k], c[k]) <= TOL, f"problem {q} field {k}: {rel_err(a[k], c[k])}"
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32dc"])
+def test_batch_matches_single_models(ht, precision):
+    rb, fb, stats = _run(ht, precision, graphs=True)
+    rs, fs = _run_single(ht, precision)
+    _compare_with_single(rb, fb, rs, fs, precision)
     assert stats["graphs"] > 0 and stats["graph_launches"] >= stats["graphs"]
     assert stats["repacks"] > 0          # problems of different sizes stop at different checks
 
@@ -108,6 +114,26 @@ def test_batch_graphs_equal_eager(ht, precision):
     for stepg, stepe in zip(rg, re_):
         for a, c in zip(stepg, stepe):
             assert a["thermal"] == c["thermal"] and a["darcy"] == c["darcy"] and a["Nu"] == c["Nu"]
+
+
+def test_batch_advection_substeps(ht):
+    """The constant-dt rule on every model: the batch advects every model's
+    first sub-step in one launch per pass (WENO v1 and v2 in the same launch)
+    and the further sub-steps per model; the results are those of the models
+    stepped alone, with more than one sub-step in some steps."""
+    def setup(q, m):
+        m.set_dt_rule(ht._lib.DT_V18_CONSTANT)
+        m.set_dt(2e-4)
+    # Two steps: with dt fixed, max|V| of the third problem grows about 15x a
+    # step (7.3e3, 1.1e5), alone as in the batch, and a third step would need
+    # more than the 1e6 sub-steps the step accepts.
+    rb, fb, _ = _run(ht, 0, graphs=True, nsteps=2, setup=setup)
+    rs, fs = _run_single(ht, 0, nsteps=2, setup=setup)
+    _compare_with_single(rb, fb, rs, fs, 0)
+    assert max(r["n_adv"] for step in rb for r in step) > 1
+    for stepb, steps_ in zip(rb, rs):
+        for a, c in zip(stepb, steps_):
+            assert a["dt_next"] == c["dt_next"] == 2e-4
 
 
 def test_batch_nan_raises(ht):
